@@ -1,0 +1,140 @@
+/* xfdtd_core_amd.h — C-ABI of the MI355X FDTD time-stepping library.
+ *
+ * This is the device boundary behind Simulation::run(): the host performs
+ * the reference's FP64 precompute (grid, material averaging, the 18 Yee
+ * coefficient arrays, CPML profiles, TFSF incident tables, lumped-element
+ * patches) and hands the result to xfd_create(); xfd_step() then runs the
+ * per-step loop that Domain::run() runs in the reference
+ * (src/domain/domain.cpp:82-135): updateH -> correctH -> updateE -> correctE
+ * -> record.  Plain pointers and sizes only; every entry point returns 0 on
+ * success and a negative code on failure (message in xfd_last_error()).
+ *
+ * Array conventions follow the reference: row-major, k (z) fastest
+ * (include/xfdtd/common/type_define.h:16-31), staggered extents as in
+ * Simulation::generateEMF (src/simulation/simulation.cpp:514-528):
+ *   Ex nx x (ny+1) x (nz+1)   Ey (nx+1) x ny x (nz+1)   Ez (nx+1) x (ny+1) x nz
+ *   Hx (nx+1) x ny x nz       Hy nx x (ny+1) x nz       Hz nx x ny x (nz+1)
+ * Coefficient arrays have their field's extent.  Host buffers are borrowed
+ * only for the duration of the call; the library owns all device memory and
+ * its HIP stream.
+ */
+#ifndef XFDTD_CORE_AMD_H_
+#define XFDTD_CORE_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XFD_OK 0
+#define XFD_EINVAL -1   /* bad argument or inconsistent descriptor */
+#define XFD_EDEVICE -2  /* HIP runtime error */
+#define XFD_ENOMEM -3   /* device allocation failed */
+
+enum xfd_dtype { XFD_F64 = 0, XFD_F32 = 1 };
+enum xfd_field { XFD_EX = 0, XFD_EY, XFD_EZ, XFD_HX, XFD_HY, XFD_HZ };
+
+/* One CPML slab: PML::init / calRecursiveConvolutionCoeff / correctCoefficient*
+ * (src/boundary/pml.cpp:98-196, 286-633) and the PMLCorrector ranges
+ * (src/include/boundary/pml_corrector.h:178-382) for a single domain.
+ * Tangential axes of `axis` are a = (axis+1)%3, b = (axis+2)%3.  The four
+ * c_psi arrays have the reference's psi extents: the slab box
+ * (n along axis, grid size along a and b) extended by one along b for
+ * c_ea_psi_hb / c_hb_psi_ea and along a for c_eb_psi_ha / c_ha_psi_eb. */
+typedef struct {
+  int32_t axis;     /* 0 = x, 1 = y, 2 = z */
+  int32_t n;        /* layers */
+  int32_t e_start;  /* first corrected E plane (PML::nodeENodeStartIndexMainAxis) */
+  int32_t h_start;  /* first corrected H plane (PML::nodeHNodeStartIndexMainAxis) */
+  const double *a_e, *b_e, *a_h, *b_h;  /* n each */
+  const double *c_ea_psi_hb, *c_eb_psi_ha, *c_ha_psi_eb, *c_hb_psi_ea;
+} xfd_cpml_desc;
+
+/* TFSF3D plane-wave source: the TFSFCorrector constructor arguments
+ * (src/waveform_source/tfsf_3d.cpp:21-45) for a single domain. */
+typedef struct {
+  int32_t origin[3], size[3];            /* TFSF box (TFSF::globalBox) */
+  const double *px_int, *py_int, *pz_int;    /* size[a] + 1 each */
+  const double *px_half, *py_half, *pz_half; /* size[a] + 2 each */
+  int64_t nt;                            /* rows of the incident tables */
+  int64_t aux;                           /* e_inc columns; h_inc has aux-1 */
+  const double *e_inc, *h_inc;           /* nt x aux, nt x (aux-1) */
+  double transform_e[3], transform_h[3];
+  double ca[3], cb[3];                   /* TFSF::cax..caz, cbx..cbz */
+} xfd_tfsf_desc;
+
+/* Lumped voltage or current source: E += coff * w[t] over the element's
+ * edges (src/object/lumped_element/lumped_element_corrrector.cpp:8-38). */
+typedef struct {
+  int32_t component;        /* 0..2: Ex, Ey, Ez */
+  int32_t origin[3], size[3];  /* edge box in grid indices */
+  const double* coff;       /* size[0] x size[1] x size[2] */
+  int64_t nt;
+  const double* waveform;   /* nt samples, indexed by the time step */
+} xfd_lumped_desc;
+
+/* A recording: `count` copies of the box [origin, end) of one field, taken
+ * after correctE of steps first, first + interval, ... (MovieMonitor,
+ * FieldMonitor, FieldTimeMonitor sampling in src/monitor/). */
+typedef struct {
+  int32_t field;
+  int32_t origin[3], end[3];
+  int64_t first, interval, count;
+} xfd_gather_desc;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t dtype;               /* enum xfd_dtype: field and update precision */
+  /* cexe cexhy cexhz ceye ceyhz ceyhx ceze cezhx cezhy
+   * chxh chxey chxez chyh chyez chyex chzh chzex chzey */
+  const double* coeff[18];
+  int32_t n_cpml;
+  const xfd_cpml_desc* cpml;   /* in boundary add order */
+  int32_t n_lumped;
+  const xfd_lumped_desc* lumped;  /* in object add order */
+  int32_t n_tfsf;
+  const xfd_tfsf_desc* tfsf;
+  int32_t n_gather;
+  const xfd_gather_desc* gather;
+} xfd_desc;
+
+typedef struct xfd_ctx xfd_ctx;
+
+const char* xfd_last_error(void);
+const char* xfd_version(void);
+int xfd_device_count(int32_t* count);
+
+/* Replaces Simulation::generateDomain + the Domain objects
+ * (src/simulation/simulation.cpp:401-494). */
+int xfd_create(const xfd_desc* desc, xfd_ctx** out);
+void xfd_destroy(xfd_ctx* ctx);
+
+/* Runs steps t0 .. t0+n-1 (Domain::run, src/domain/domain.cpp:82-135) and
+ * waits for them.  Time step t indexes the TFSF tables, lumped waveforms and
+ * the recording schedule. */
+int xfd_step(xfd_ctx* ctx, int64_t t0, int64_t n);
+
+/* Copies a field out to / in from a host array of the reference extent,
+ * converting to / from double. */
+int xfd_read_field(xfd_ctx* ctx, int32_t field, double* dst);
+int xfd_write_field(xfd_ctx* ctx, int32_t field, const double* src);
+
+/* Copies recording `g` (count x box volume doubles) to the host. */
+int xfd_read_gather(xfd_ctx* ctx, int32_t g, double* dst);
+
+/* Kernel timing with HIP events on the library's stream.  When enabled,
+ * xfd_step records an event pair around every launch; xfd_kernel_stats then
+ * reports, for kernel class `k`, its name, launch count and summed time. */
+int xfd_set_profiling(xfd_ctx* ctx, int32_t on);
+int xfd_kernel_stats(xfd_ctx* ctx, int32_t k, const char** name, int64_t* launches,
+                     double* total_ms);
+
+/* Device bytes held by the context. */
+int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XFDTD_CORE_AMD_H_ */

@@ -1,0 +1,410 @@
+// xfdtd_core_amd.hpp — C++20 scenario API with the reference's class names.
+//
+// Code written against XFDTD_CORE's public headers (include/xfdtd/**) for the
+// 3D time-stepping path builds the same scenario with these classes:
+// Simulation, Object, Cube/Sphere/Cylinder, Material/ElectroMagneticProperty,
+// PML, TFSF3D, Waveform, CurrentSource/VoltageSource/Resistor and the
+// FieldMonitor/MovieMonitor/FieldTimeMonitor monitors keep their constructor
+// arguments and meaning.  Simulation::run(n) performs the reference's FP64
+// precompute on the host and runs the time loop on the GPU through the C-ABI
+// in xfdtd_core_amd.h.  Arrays are row-major with k fastest, as in the
+// reference; xtensor is not used, so array accessors return Array3D/Array1D
+// views of this library.
+#ifndef XFDTD_CORE_AMD_HPP_
+#define XFDTD_CORE_AMD_HPP_
+
+#include <array>
+#include <cstddef>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace xfdtd {
+
+using Real = double;
+using Index = std::size_t;
+
+// include/xfdtd/exception/exception.h
+class XFDTDException : public std::runtime_error {
+ public:
+  explicit XFDTDException(const std::string& m = "XFDTD Exception") : std::runtime_error(m) {}
+};
+class XFDTDSimulationException : public XFDTDException {
+ public:
+  using XFDTDException::XFDTDException;
+};
+
+template <class T>
+struct Array1D {
+  std::vector<T> data;
+  Array1D() = default;
+  explicit Array1D(std::size_t n, T v = T{}) : data(n, v) {}
+  T& operator()(std::size_t i) { return data[i]; }
+  const T& operator()(std::size_t i) const { return data[i]; }
+  std::size_t size() const { return data.size(); }
+};
+
+template <class T>
+struct Array3D {
+  std::array<std::size_t, 3> shape{0, 0, 0};
+  std::vector<T> data;
+  Array3D() = default;
+  Array3D(std::size_t a, std::size_t b, std::size_t c, T v = T{}) : shape{a, b, c}, data(a * b * c, v) {}
+  T& operator()(std::size_t i, std::size_t j, std::size_t k) { return data[(i * shape[1] + j) * shape[2] + k]; }
+  const T& operator()(std::size_t i, std::size_t j, std::size_t k) const {
+    return data[(i * shape[1] + j) * shape[2] + k];
+  }
+  std::size_t size() const { return data.size(); }
+};
+
+struct Axis {
+  enum class XYZ { X = 0, Y = 1, Z = 2 };
+  enum class Direction { XN = 0, XP, YN, YP, ZN, ZP };
+};
+
+class Vector {
+ public:
+  Vector(Real x = 0, Real y = 0, Real z = 0) : _d{x, y, z} {}
+  Real x() const { return _d[0]; }
+  Real y() const { return _d[1]; }
+  Real z() const { return _d[2]; }
+  Real operator[](int i) const { return _d[i]; }
+
+ private:
+  std::array<Real, 3> _d;
+};
+
+class Cube;
+class Shape {
+ public:
+  virtual ~Shape() = default;
+  virtual bool isInside(const Vector& v, Real eps) const = 0;
+  virtual std::unique_ptr<Cube> wrappedCube() const = 0;
+  virtual std::unique_ptr<Shape> clone() const = 0;
+};
+
+class Cube : public Shape {
+ public:
+  Cube(Vector origin, Vector size);
+  Vector origin() const { return _origin; }
+  Vector size() const { return _size; }
+  Vector end() const { return _end; }
+  bool isInside(const Vector& v, Real eps) const override;
+  std::unique_ptr<Cube> wrappedCube() const override;
+  std::unique_ptr<Shape> clone() const override;
+
+ private:
+  Vector _origin, _size, _end;
+};
+
+class Sphere : public Shape {
+ public:
+  Sphere(Vector center, Real radius);
+  bool isInside(const Vector& v, Real eps) const override;
+  std::unique_ptr<Cube> wrappedCube() const override;
+  std::unique_ptr<Shape> clone() const override;
+
+ private:
+  Vector _center;
+  Real _radius;
+};
+
+class Cylinder : public Shape {
+ public:
+  Cylinder(Vector center, Real radius, Real height, Axis::XYZ axis);
+  bool isInside(const Vector& v, Real eps) const override;
+  std::unique_ptr<Cube> wrappedCube() const override;
+  std::unique_ptr<Shape> clone() const override;
+
+ private:
+  Vector _center;
+  Real _radius, _height;
+  Axis::XYZ _axis;
+};
+
+class ElectroMagneticProperty {
+ public:
+  ElectroMagneticProperty(Real epsilon_r = 1, Real mu_r = 1, Real sigma_e = 0, Real sigma_m = 0);
+  static ElectroMagneticProperty air() { return {1, 1, 0, 0}; }
+  Real epsilonR() const { return _eps_r; }
+  Real muR() const { return _mu_r; }
+  Real sigmaE() const { return _sigma_e; }
+  Real sigmaM() const { return _sigma_m; }
+  Real epsilon() const { return _eps; }
+  Real mu() const { return _mu; }
+
+ private:
+  Real _eps_r, _mu_r, _sigma_e, _sigma_m, _eps, _mu;
+};
+
+class Material {
+ public:
+  Material(std::string name, ElectroMagneticProperty p = {}) : _name(std::move(name)), _p(p) {}
+  virtual ~Material() = default;
+  static std::unique_ptr<Material> createAir(std::string name = "air") {
+    return std::make_unique<Material>(std::move(name), ElectroMagneticProperty::air());
+  }
+  const ElectroMagneticProperty& emProperty() const { return _p; }
+  virtual bool dispersion() const { return false; }
+
+ private:
+  std::string _name;
+  ElectroMagneticProperty _p;
+};
+
+class Waveform {
+ public:
+  explicit Waveform(std::function<Real(Real)> func, Real amplitude = 1.0)
+      : _func(std::move(func)), _amplitude(amplitude) {}
+  static std::unique_ptr<Waveform> gaussian(Real tau, Real t_0, Real amplitude = 1.0);
+  static std::unique_ptr<Waveform> cosineModulatedGaussian(Real tau, Real t_0, Real frequency,
+                                                           Real amplitude = 1.0);
+  static std::unique_ptr<Waveform> sine(Real frequency, Real amplitude = 1.0);
+  static std::unique_ptr<Waveform> cosine(Real frequency, Real amplitude = 1.0);
+  static std::unique_ptr<Waveform> step(Real t_0, Real amplitude = 1.0);
+  Real amplitude() const { return _amplitude; }
+  Real operator()(Real t) const { return _func(t); }
+  void init(const std::vector<Real>& time);  // waveform.cpp:28-34
+  const std::vector<Real>& value() const { return _value; }
+
+ private:
+  std::function<Real(Real)> _func;
+  Real _amplitude;
+  std::vector<Real> _value;
+};
+
+class Simulation;
+
+class Object {
+ public:
+  Object(std::string name, std::unique_ptr<Shape> shape, std::unique_ptr<Material> material);
+  virtual ~Object() = default;
+  const std::string& name() const { return _name; }
+  const Shape& shape() const { return *_shape; }
+  const Material& material() const { return *_material; }
+
+ private:
+  std::string _name;
+  std::unique_ptr<Shape> _shape;
+  std::unique_ptr<Material> _material;
+};
+
+class LumpedElement : public Object {
+ public:
+  LumpedElement(std::string name, std::unique_ptr<Cube> cube, Axis::XYZ xyz,
+                std::unique_ptr<Material> material = Material::createAir());
+  Axis::XYZ xyz() const { return _xyz; }
+
+ private:
+  Axis::XYZ _xyz;
+};
+
+class Resistor : public LumpedElement {
+ public:
+  Resistor(std::string name, std::unique_ptr<Cube> cube, Axis::XYZ xyz, Real resistance,
+           std::unique_ptr<Material> material = Material::createAir());
+  Real resistance() const { return _resistance; }
+
+ private:
+  Real _resistance;
+};
+
+class VoltageSource : public LumpedElement {
+ public:
+  VoltageSource(std::string name, std::unique_ptr<Cube> cube, Axis::Direction direction, Real resistance,
+                std::unique_ptr<Waveform> waveform, std::unique_ptr<Material> material = Material::createAir());
+  Axis::Direction direction() const { return _direction; }
+  Real resistance() const { return _resistance; }
+  Waveform& waveform() const { return *_waveform; }
+
+ private:
+  Axis::Direction _direction;
+  Real _resistance;
+  std::unique_ptr<Waveform> _waveform;
+};
+
+class CurrentSource : public LumpedElement {
+ public:
+  CurrentSource(std::string name, std::unique_ptr<Cube> cube, Axis::Direction direction, Real resistance,
+                std::unique_ptr<Waveform> waveform, std::unique_ptr<Material> material = Material::createAir());
+  Axis::Direction direction() const { return _direction; }
+  Real resistance() const { return _resistance; }
+  Waveform& waveform() const { return *_waveform; }
+
+ private:
+  Axis::Direction _direction;
+  Real _resistance;
+  std::unique_ptr<Waveform> _waveform;
+};
+
+class Boundary {
+ public:
+  virtual ~Boundary() = default;
+};
+
+class PML : public Boundary {
+ public:
+  PML(int thickness, Axis::Direction direction, int order = 4, Real sigma_ratio = 1, Real alpha_min = 0,
+      Real alpha_max = 0.05, Real kappa_max = 10);
+  int thickness() const { return _thickness; }
+  Axis::Direction direction() const { return _direction; }
+  int order() const { return _order; }
+  Real sigmaRatio() const { return _sigma_ratio; }
+  Real alphaMin() const { return _alpha_min; }
+  Real alphaMax() const { return _alpha_max; }
+  Real kappaMax() const { return _kappa_max; }
+
+ private:
+  int _thickness;
+  Axis::Direction _direction;
+  int _order;
+  Real _sigma_ratio, _alpha_min, _alpha_max, _kappa_max;
+};
+
+class WaveformSource {
+ public:
+  explicit WaveformSource(std::unique_ptr<Waveform> w) : _waveform(std::move(w)) {}
+  virtual ~WaveformSource() = default;
+  Waveform& waveform() const { return *_waveform; }
+
+ private:
+  std::unique_ptr<Waveform> _waveform;
+};
+
+class TFSF3D : public WaveformSource {
+ public:
+  TFSF3D(std::size_t x, std::size_t y, std::size_t z, Real theta, Real phi, Real psi,
+         std::unique_ptr<Waveform> waveform);
+  std::size_t x() const { return _x; }
+  std::size_t y() const { return _y; }
+  std::size_t z() const { return _z; }
+  Real theta() const { return _theta; }
+  Real phi() const { return _phi; }
+  Real psi() const { return _psi; }
+
+ private:
+  std::size_t _x, _y, _z;
+  Real _theta, _phi, _psi;
+};
+
+struct EMF {
+  enum class Field { EX = 0, EY, EZ, EM, HX, HY, HZ, HM };
+  Array3D<Real> ex, ey, ez, hx, hy, hz;
+  Array3D<Real>& field(Field f);
+};
+
+class Monitor {
+ public:
+  Monitor(std::string name, std::string output_dir) : _name(std::move(name)), _output_dir(std::move(output_dir)) {}
+  virtual ~Monitor() = default;
+  const std::string& name() const { return _name; }
+  const std::string& outputDir() const { return _output_dir; }
+  void setName(std::string n) { _name = std::move(n); }
+  void setOutputDir(std::string d) { _output_dir = std::move(d); }
+
+ private:
+  std::string _name, _output_dir;
+};
+
+class FieldMonitor : public Monitor {
+ public:
+  FieldMonitor(std::unique_ptr<Shape> shape, EMF::Field field, std::string name, std::string output_dir);
+  const Shape& shape() const { return *_shape; }
+  EMF::Field field() const { return _field; }
+  // Last frame written (field_monitor.cpp:82-96).
+  Array3D<Real>& data() { return _data; }
+
+ private:
+  std::unique_ptr<Shape> _shape;
+  EMF::Field _field;
+  Array3D<Real> _data;
+};
+
+class MovieMonitor : public Monitor {
+ public:
+  MovieMonitor(std::unique_ptr<Monitor> frame, std::size_t frame_interval, std::string name,
+               std::string output_dir);
+  FieldMonitor& frame() const { return *_frame; }
+  std::size_t frameInterval() const { return _interval; }
+  // Frames in recording order, named "%05d" of the record count.
+  std::vector<std::pair<std::string, Array3D<Real>>>& frames() { return _frames; }
+
+ private:
+  std::unique_ptr<FieldMonitor> _frame;
+  std::size_t _interval;
+  std::vector<std::pair<std::string, Array3D<Real>>> _frames;
+};
+
+class FieldTimeMonitor : public Monitor {
+ public:
+  FieldTimeMonitor(std::string name, std::unique_ptr<Cube> cube, std::string output_dir, EMF::Field field);
+  const Cube& cube() const { return *_cube; }
+  EMF::Field field() const { return _field; }
+  std::vector<Real>& time() { return _time; }
+  std::vector<Real>& data() { return _data; }
+
+ private:
+  std::unique_ptr<Cube> _cube;
+  EMF::Field _field;
+  std::vector<Real> _time, _data;
+};
+
+struct ThreadConfig {  // accepted for API compatibility; the GPU path ignores it
+  int x = 1, y = 1, z = 1;
+};
+
+enum class Precision { FP64, FP32 };
+
+class Simulation {
+ public:
+  Simulation(Real dx, Real dy, Real dz, Real cfl, ThreadConfig thread_config = ThreadConfig{});
+  ~Simulation();
+  Simulation(const Simulation&) = delete;
+  Simulation& operator=(const Simulation&) = delete;
+
+  void addObject(std::shared_ptr<Object> object);
+  void addWaveformSource(std::shared_ptr<WaveformSource> source);
+  void addBoundary(std::shared_ptr<Boundary> boundary);
+  void addMonitor(std::shared_ptr<Monitor> monitor);
+
+  // Field and update precision of the GPU loop (default FP64).  The
+  // precompute is FP64 in both cases.
+  void setPrecision(Precision p);
+
+  void init(Index time_step);  // simulation.cpp:373-389
+  // The host half of init(): grid, materials, coefficients, CPML, TFSF and
+  // lumped precompute, without touching the GPU.
+  void precompute(Index time_step);
+  void run(Index time_step);   // simulation.cpp:284-289
+  void run();                  // the time loop only, after init()
+
+  Real dt() const;
+  std::array<Index, 3> gridSize() const;
+  // Final fields, copied back after run() (simulation.h:76).
+  const std::shared_ptr<EMF>& emf() const;
+  // Precomputed Yee coefficient by reference name ("cexe" ... "chzey").
+  const Array3D<Real>& coefficient(const std::string& name) const;
+  const std::vector<Real>& eNode(int axis) const;
+  const std::vector<Real>& hSize(int axis) const;
+  // Time of the stepping loop in seconds (HIP events around the loop).
+  double loopSeconds() const;
+  struct KernelStat {
+    std::string name;
+    long long launches;
+    double total_ms;
+  };
+  void setProfiling(bool on);
+  std::vector<KernelStat> kernelStats() const;
+  long long deviceBytes() const;
+
+  struct Impl;
+
+ private:
+  std::unique_ptr<Impl> _impl;
+};
+
+}  // namespace xfdtd
+
+#endif  // XFDTD_CORE_AMD_HPP_

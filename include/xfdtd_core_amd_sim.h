@@ -1,0 +1,56 @@
+/* xfdtd_core_amd_sim.h — scenario-level C-ABI over the C++ Simulation API.
+ *
+ * For callers without a C++ toolchain (ctypes, cgo, JNI).  A scenario is the
+ * line-oriented spec text documented in xfdtd_core_amd/spec.py; each line is
+ * the constructor call of one reference object (Simulation, Object, PML,
+ * TFSF3D, lumped elements, monitors), so a scenario reads like the
+ * reference's example programs (test/src/examples/*.cpp).  These calls wrap
+ * xfdtd::Simulation (include/xfdtd_core_amd.hpp), which replaces the
+ * reference's Simulation::init/run (src/simulation/simulation.cpp:284-389).
+ * Return codes follow xfdtd_core_amd.h; messages via xfd_sim_last_error().
+ */
+#ifndef XFDTD_CORE_AMD_SIM_H_
+#define XFDTD_CORE_AMD_SIM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* xfd_sim_last_error(void);
+/* Builds a Simulation from spec text; NULL on a parse error. */
+void* xfd_sim_create(const char* spec);
+void xfd_sim_destroy(void* sim);
+/* 0 = FP64 (default), 1 = FP32 fields and updates. */
+int xfd_sim_set_precision(void* sim, int32_t precision);
+int xfd_sim_set_profiling(void* sim, int32_t on);
+/* Host precompute only (no GPU): Simulation::init without the device. */
+int xfd_sim_precompute(void* sim, int64_t steps);
+/* Simulation::init(steps): precompute and upload to the GPU. */
+int xfd_sim_init(void* sim, int64_t steps);
+/* The time loop after xfd_sim_init, then monitor outputs and final fields. */
+int xfd_sim_run_loop(void* sim);
+/* Simulation::run(steps) = init + loop. */
+int xfd_sim_run(void* sim, int64_t steps);
+double xfd_sim_dt(void* sim);
+double xfd_sim_loop_seconds(void* sim);
+int xfd_sim_grid(void* sim, int64_t* n3);
+int64_t xfd_sim_device_bytes(void* sim);
+/* which = 0: e-node coordinates, otherwise h sizes; returns the length. */
+int64_t xfd_sim_grid_array(void* sim, int32_t which, int32_t axis, double* dst);
+/* Arrays of the reference extent; dst may be NULL to query the shape. */
+int xfd_sim_field(void* sim, int32_t field, int64_t* shape3, double* dst);
+int xfd_sim_coeff(void* sim, const char* name, int64_t* shape3, double* dst);
+int xfd_sim_monitor_count(void* sim);
+int xfd_sim_monitor_records(void* sim, int32_t monitor);
+int xfd_sim_monitor_record(void* sim, int32_t monitor, int32_t record, int64_t* shape3, double* dst);
+int xfd_sim_kernel_count(void* sim);
+int xfd_sim_kernel_stat(void* sim, int32_t k, char* name, int32_t name_len, int64_t* launches,
+                        double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XFDTD_CORE_AMD_SIM_H_ */

@@ -1,0 +1,1297 @@
+// refcpu — FP64 CPU restatement of the XFDTD_CORE 3D time-stepping path.
+// TEST INFRASTRUCTURE ONLY (oracle + "port" CPU baseline); see refcpu.hpp.
+// Citations are file:line in the reference repository.
+#include "refcpu.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <limits>
+#include <map>
+#include <sstream>
+#include <thread>
+
+namespace refcpu {
+
+// ------------------------------------------------------------- helpers ---
+Arr1 linspace(Real start, Real stop, Index n) {
+  // xbuilder.hpp linspace: step = (stop - start) / max(1, n - 1); the
+  // arange generator assigns by running sum and pins the last sample.
+  Arr1 out(n);
+  Real step = (stop - start) / std::fmax(Real(1), Real(n - 1));
+  Real v = start;
+  for (Index i = 0; i < n; ++i) {
+    out(i) = v;
+    v += step;
+  }
+  if (n > 1) out(n - 1) = stop;
+  return out;
+}
+
+bool intersected(const Range& d, const Range& r) {  // index_task.h:84-87
+  return !(r.e <= d.s || d.e <= r.s);
+}
+bool intersected(const Task& a, const Task& b) {
+  return intersected(a.x, b.x) && intersected(a.y, b.y) && intersected(a.z, b.z);
+}
+std::optional<Task> intersection(const Task& a, const Task& b) {  // :97-115
+  if (!intersected(a, b)) return std::nullopt;
+  auto f = [](Range p, Range q) {
+    return Range{std::max(p.s, q.s), std::min(p.e, q.e)};
+  };
+  return Task{f(a.x, b.x), f(a.y, b.y), f(a.z, b.z)};
+}
+
+static Range decomposeSize(int rank, int np, Index size) {  // decompose_task.h:9-33
+  if (np <= 0) throw Error("Number of processes is less than or equal to zero");
+  if (size < static_cast<Index>(np))
+    throw Error("Problem size is less than number of processes");
+  Index q = size / np, r = size % np;
+  if (static_cast<Index>(rank) < r) {
+    Index s = rank * (q + 1);
+    return {s, s + q + 1};
+  }
+  Index s = rank * q + r;
+  return {s, s + q};
+}
+
+std::vector<Task> decomposeTask(const Task& p, int nx, int ny, int nz) {
+  // decompose_task.h:69-93 (row major)
+  if (nx <= 0 || ny <= 0 || nz <= 0) return {p};
+  auto dr = [](Range r, int np) {
+    std::vector<Range> out;
+    if (r.e <= r.s) throw Error("Invalid range");
+    for (int i = 0; i < np; ++i) {
+      auto t = decomposeSize(i, np, r.e - r.s);
+      out.push_back({t.s + r.s, t.e + r.s});
+    }
+    return out;
+  };
+  auto xs = dr(p.x, nx), ys = dr(p.y, ny), zs = dr(p.z, nz);
+  std::vector<Task> res;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) res.push_back({xs[i], ys[j], zs[k]});
+  return res;
+}
+
+// ---------------------------------------------------------------- shapes --
+Cube::Cube(Vec3 o, Vec3 s) : origin(o), size(s), end(o + s) {
+  // cube.cpp:128-139: a NaN end (inf - inf) becomes +inf
+  if (std::isnan(end.x)) end.x = std::numeric_limits<Real>::infinity();
+  if (std::isnan(end.y)) end.y = std::numeric_limits<Real>::infinity();
+  if (std::isnan(end.z)) end.z = std::numeric_limits<Real>::infinity();
+}
+
+bool Cube::isInside(const Vec3& p, Real eps) const {
+  bool xi = lessEqual(origin.x, p.x, eps) && lessEqual(p.x, end.x, eps);
+  bool yi = lessEqual(origin.y, p.y, eps) && lessEqual(p.y, end.y, eps);
+  bool zi = lessEqual(origin.z, p.z, eps) && lessEqual(p.z, end.z, eps);
+  return xi && yi && zi;
+}
+
+bool Sphere::isInside(const Vec3& p, Real eps) const {
+  return lessEqual((p - center).norm(), radius, eps);
+}
+
+Cube Sphere::wrapped() const {
+  return Cube(center - Vec3{radius, radius, radius},
+              Vec3{radius * 2, radius * 2, radius * 2});
+}
+
+Cube Cylinder::wrapped() const {  // cylinder.cpp:73-98
+  Real s = radius * 2;
+  if (axis == X)
+    return Cube(Vec3{center.x - height / 2, center.y - radius, center.z - radius},
+                Vec3{height, s, s});
+  if (axis == Y)
+    return Cube(Vec3{center.x - radius, center.y - height / 2, center.z - radius},
+                Vec3{s, height, s});
+  return Cube(Vec3{center.x - radius, center.y - radius, center.z - height / 2},
+              Vec3{s, s, height});
+}
+
+bool Cylinder::isInside(const Vec3& p, Real eps) const {  // cylinder.cpp:41-66
+  if (!wrapped().isInside(p, eps)) return false;
+  Real d;
+  if (axis == X)
+    d = std::sqrt(std::pow(p.y - center.y, 2) + std::pow(p.z - center.z, 2));
+  else if (axis == Y)
+    d = std::sqrt(std::pow(p.x - center.x, 2) + std::pow(p.z - center.z, 2));
+  else
+    d = std::sqrt(std::pow(p.x - center.x, 2) + std::pow(p.y - center.y, 2));
+  return lessEqual(d, radius, 1e-8);  // default epsilon of floatCompare
+}
+
+// ------------------------------------------------------------- waveform ---
+void Waveform::init(const Arr1& time) {
+  value = Arr1(time.size());
+  for (Index i = 0; i < time.size(); ++i) value(i) = amplitude * f(time(i));
+}
+
+Material Material::make(Real er, Real mr, Real se, Real sm) {
+  Material m;
+  m.eps_r = er;
+  m.mu_r = mr;
+  m.sigma_e = se == 0 ? SIGMA_ZERO : se;
+  m.sigma_m = sm == 0 ? SIGMA_ZERO : sm;
+  m.eps = er * EPSILON_0;
+  m.mu = mr * MU_0;
+  return m;
+}
+
+// ----------------------------------------------------------------- grid ---
+Real GridSpace::eps() const {
+  return std::min({min_d[0], min_d[1], min_d[2]}) / 1000;
+}
+
+Index GridSpace::nearest(int axis, Real x) const {
+  // xt::argmin(xt::abs(e_node - x)): first index of the minimum
+  const Arr1& e = e_node[axis];
+  Index best = 0;
+  Real bv = std::abs(e(0) - x);
+  for (Index i = 1; i < e.size(); ++i) {
+    Real v = std::abs(e(i) - x);
+    if (v < bv) {
+      bv = v;
+      best = i;
+    }
+  }
+  return best;
+}
+
+std::array<Index, 3> GridSpace::box(const Shape& s, Index* end) const {
+  Cube c = s.wrapped();
+  std::array<Index, 3> o{nearest(0, c.origin.x), nearest(1, c.origin.y),
+                         nearest(2, c.origin.z)};
+  end[0] = nearest(0, c.end.x);
+  end[1] = nearest(1, c.end.y);
+  end[2] = nearest(2, c.end.z);
+  return o;
+}
+
+// ---------------------------------------------------------- coefficients --
+Arr3& Coeff::self(int h, int c) {
+  if (!h) return c == 0 ? cexe : c == 1 ? ceye : ceze;
+  return c == 0 ? chxh : c == 1 ? chyh : chzh;
+}
+// coeff<attr, xyz, dual, tangentialA(xyz)> (fdtd_update_coefficient.h)
+Arr3& Coeff::dualA(int h, int c) {
+  if (!h) return c == 0 ? cexhy : c == 1 ? ceyhz : cezhx;
+  return c == 0 ? chxey : c == 1 ? chyez : chzex;
+}
+Arr3& Coeff::dualB(int h, int c) {
+  if (!h) return c == 0 ? cexhz : c == 1 ? ceyhx : cezhy;
+  return c == 0 ? chxez : c == 1 ? chyex : chzey;
+}
+Arr3* Coeff::byName(const std::string& n) {
+  static const char* names[18] = {"cexe", "cexhy", "cexhz", "ceye", "ceyhz", "ceyhx",
+                                  "ceze", "cezhx", "cezhy", "chxh", "chxey", "chxez",
+                                  "chyh", "chyez", "chyex", "chzh", "chzex", "chzey"};
+  Arr3* arrs[18] = {&cexe, &cexhy, &cexhz, &ceye, &ceyhz, &ceyhx, &ceze, &cezhx, &cezhy,
+                    &chxh, &chxey, &chxez, &chyh, &chyez, &chyex, &chzh, &chzex, &chzey};
+  for (int i = 0; i < 18; ++i)
+    if (n == names[i]) return arrs[i];
+  return nullptr;
+}
+
+// -------------------------------------------------------------- objects ---
+void Object::init(Sim& s) {
+  gb_origin = s.g.box(*shape, gb_end.data());
+}
+
+void Object::correctMaterialSpace(Sim& s, Index index) {
+  // object.cpp:114-198: every cell whose centre (h-node) is inside the shape
+  // takes the object's properties; later objects overwrite earlier ones.
+  material_index = index;
+  const Real eps = s.g.eps();
+  const Index nx = s.g.nx(), ny = s.g.ny(), nz = s.g.nz();
+  auto& mp = s.mp;
+  for (Index i = 0; i < nx; ++i)
+    for (Index j = 0; j < ny; ++j)
+      for (Index k = 0; k < nz; ++k) {
+        Vec3 p{s.g.h_node[0](i), s.g.h_node[1](j), s.g.h_node[2](k)};
+        if (!shape->isInside(p, eps)) continue;
+        s.material_grid[(i * ny + j) * nz + k] = index;
+        for (int a = 0; a < 3; ++a) {
+          mp.eps[a](i, j, k) = material.eps;
+          mp.mu[a](i, j, k) = material.mu;
+          mp.se[a](i, j, k) = material.sigma_e;
+          mp.sm[a](i, j, k) = material.sigma_m;
+        }
+      }
+}
+
+Index LumpedElement::nodeCount(int a) const {
+  Index s = a == 0 ? is : a == 1 ? js : ks;
+  Index e = a == 0 ? ie : a == 1 ? je : ke;
+  return a == static_cast<int>(xyz) ? e - s : e - s + 1;
+}
+Index LumpedElement::globalCountMain() const {
+  return xyz == X ? ie - is : xyz == Y ? je - js : ke - ks;
+}
+Index LumpedElement::globalCountSubA() const {
+  return xyz == X ? je - js + 1 : xyz == Y ? ke - ks + 1 : ie - is + 1;
+}
+Index LumpedElement::globalCountSubB() const {
+  return xyz == X ? ke - ks + 1 : xyz == Y ? ie - is + 1 : je - js + 1;
+}
+Range LumpedElement::range(int a) const {
+  Index s = a == 0 ? is : a == 1 ? js : ks;
+  Index e = a == 0 ? ie : a == 1 ? je : ke;
+  return a == static_cast<int>(xyz) ? Range{s, e} : Range{s, e + 1};
+}
+Task LumpedElement::indexTask() const {  // lumped_element.cpp:154-158
+  return {{is, is + nodeCount(0)}, {js, js + nodeCount(1)}, {ks, ks + nodeCount(2)}};
+}
+void LumpedElement::init(Sim& s) {
+  Object::init(s);
+  is = gb_origin[0];
+  js = gb_origin[1];
+  ks = gb_origin[2];
+  ie = gb_end[0];
+  je = gb_end[1];
+  ke = gb_end[2];
+}
+
+void LumpedElement::sizes(Sim& s, Arr3& da, Arr3& db, Arr3& dc) const {
+  // xt::meshgrid(view(x), view(y), view(z)) over rangeX/Y/Z, with the main
+  // axis taking e_size and the others h_size (current_source.cpp:55-105).
+  Range r[3] = {range(0), range(1), range(2)};
+  const Arr1* sz[3];
+  for (int a = 0; a < 3; ++a) sz[a] = a == static_cast<int>(xyz) ? &s.g.e_size[a] : &s.g.h_size[a];
+  Arr3 m[3];
+  for (int a = 0; a < 3; ++a) m[a] = Arr3(r[0].size(), r[1].size(), r[2].size());
+  for (Index i = 0; i < r[0].size(); ++i)
+    for (Index j = 0; j < r[1].size(); ++j)
+      for (Index k = 0; k < r[2].size(); ++k) {
+        m[0](i, j, k) = (*sz[0])(r[0].s + i);
+        m[1](i, j, k) = (*sz[1])(r[1].s + j);
+        m[2](i, j, k) = (*sz[2])(r[2].s + k);
+      }
+  // (da, db, dc) = sizes along (tanA, tanB, main)
+  da = m[tanA(xyz)];
+  db = m[tanB(xyz)];
+  dc = m[xyz];
+}
+
+// Applies the lumped-element coefficient patch (resistor.cpp:86-98,
+// voltage_source.cpp:109-132, current_source.cpp:111-131) over the element's
+// view, and fills `coff` with -2dt/(den*scale)*factor when requested.
+static void lumpedPatch(Sim& s, const LumpedElement& le, const Arr3& beta, const Arr3& da,
+                        const Arr3& db, const Arr3* coff_scale, bool coff_is_alpha,
+                        Real factor, Arr3* coff) {
+  const Real dt = s.dt;
+  int c = le.xyz;
+  Arr3& cece = s.c.self(0, c);
+  Arr3& cecha = s.c.dualA(0, c);
+  Arr3& cechb = s.c.dualB(0, c);
+  Arr3& eps = s.mp.eps[c];
+  Arr3& sig = s.mp.se[c];
+  Range r[3] = {le.range(0), le.range(1), le.range(2)};
+  if (coff) *coff = Arr3(r[0].size(), r[1].size(), r[2].size());
+  for (Index i = 0; i < r[0].size(); ++i)
+    for (Index j = 0; j < r[1].size(); ++j)
+      for (Index k = 0; k < r[2].size(); ++k) {
+        Index gi = r[0].s + i, gj = r[1].s + j, gk = r[2].s + k;
+        Real e = eps(gi, gj, gk), sg = sig(gi, gj, gk), b = beta(i, j, k);
+        cece(gi, gj, gk) = (2 * e - dt * sg - b) / (2 * e + dt * sg + b);
+        cecha(gi, gj, gk) = -2 * dt / ((2 * e + dt * sg + b) * db(i, j, k));
+        cechb(gi, gj, gk) = 2 * dt / ((2 * e + dt * sg + b) * da(i, j, k));
+        if (coff) {
+          Real scale = coff_is_alpha ? (*coff_scale)(i, j, k)
+                                     : (db(i, j, k) * da(i, j, k));
+          (*coff)(i, j, k) = -2 * dt / ((2 * e + dt * sg + b) * scale) * factor;
+        }
+      }
+}
+
+void Resistor::init(Sim& s) {  // resistor.cpp:31-83
+  LumpedElement::init(s);
+  rf = resistance * globalCountSubA() * globalCountSubB() / globalCountMain();
+  sizes(s, da, db, dc);
+}
+
+void Resistor::correctUpdateCoefficient(Sim& s) {  // resistor.cpp:85-127
+  beta = Arr3(dc.n0, dc.n1, dc.n2);
+  for (Index n = 0; n < beta.size(); ++n)
+    beta.v[n] = s.dt * dc.v[n] / (rf * da.v[n] * db.v[n]);
+  lumpedPatch(s, *this, beta, da, db, nullptr, false, 0, nullptr);
+}
+
+void SourceElement::init(Sim& s) {
+  // voltage_source.cpp:47-107, current_source.cpp:41-109
+  LumpedElement::init(s);
+  rf = resistance * globalCountSubA() * globalCountSubB() / globalCountMain();
+  if (voltage)
+    amp_factor = waveform.amplitude / globalCountMain();
+  else
+    amp_factor = waveform.amplitude / (globalCountSubA() * globalCountSubB());
+  if (dirNegative(direction)) amp_factor *= -1;
+  sizes(s, da, db, dc);
+  if (!voltage) {  // current_source.cpp:107-108: beta fixed at init
+    beta = Arr3(dc.n0, dc.n1, dc.n2);
+    for (Index n = 0; n < beta.size(); ++n)
+      beta.v[n] = (s.dt * dc.v[n]) / (rf * da.v[n] * db.v[n]);
+  }
+}
+
+void SourceElement::correctUpdateCoefficient(Sim& s) {
+  if (voltage) {  // voltage_source.cpp:109-114
+    alpha = Arr3(dc.n0, dc.n1, dc.n2);
+    beta = Arr3(dc.n0, dc.n1, dc.n2);
+    for (Index n = 0; n < alpha.size(); ++n) {
+      alpha.v[n] = da.v[n] * db.v[n] * rf;
+      beta.v[n] = s.dt * dc.v[n] / alpha.v[n];
+    }
+    lumpedPatch(s, *this, beta, da, db, &alpha, true, amp_factor, &coff);
+  } else {
+    lumpedPatch(s, *this, beta, da, db, nullptr, false, amp_factor, &coff);
+  }
+}
+
+void SourceElement::initTimeDependentVariable(Sim& s) { waveform.init(s.hTime()); }
+
+namespace {
+// VoltageSourceCorrector / CurrentSourceCorrector::correctE
+// (lumped_element_corrrector.cpp:8-38): E += coff * w[t] over the element.
+struct SourceCorrector : Corrector {
+  Sim* s;
+  SourceElement* el;
+  Task task, local;
+  void correctE(Index t) override {
+    Arr3& e = s->emf.f(0, el->xyz);
+    Real w = el->waveform.value(t);
+    for (Index i = 0; i < task.x.size(); ++i)
+      for (Index j = 0; j < task.y.size(); ++j)
+        for (Index k = 0; k < task.z.size(); ++k)
+          e(task.x.s + i, task.y.s + j, task.z.s + k) +=
+              el->coff(local.x.s + i, local.y.s + j, local.z.s + k) * w;
+  }
+  void correctH(Index) override {}
+};
+}  // namespace
+
+std::unique_ptr<Corrector> SourceElement::generateCorrector(Sim& s, const Task& t) {
+  Task dom = indexTask();
+  if (!intersected(t, dom)) return nullptr;
+  auto in = intersection(t, dom);
+  auto c = std::make_unique<SourceCorrector>();
+  c->s = &s;
+  c->el = this;
+  c->task = *in;
+  c->local = {{in->x.s - dom.x.s, in->x.e - dom.x.s},
+              {in->y.s - dom.y.s, in->y.e - dom.y.s},
+              {in->z.s - dom.z.s, in->z.e - dom.z.s}};
+  return c;
+}
+
+// ------------------------------------------------------------------- PML ---
+void PML::init(Sim& s) {  // pml.cpp:98-196 (single process: node == global)
+  n = static_cast<Index>(std::abs(thickness));
+  main = dirAxis(dir);
+  Index N = s.g.n(main);
+  switch (dir) {
+    case XN: case YN: case ZN:
+      g_e_start = 1;
+      g_h_start = 0;
+      break;
+    default:
+      g_e_start = N - n;
+      g_h_start = N - n;
+  }
+  Index na = s.g.e_size[tanA(main)].size();
+  Index nb = s.g.e_size[tanB(main)].size();
+  auto x = fromABC(main, 0, 0, g_h_start);
+  auto xe = fromABC(main, na, nb, g_h_start + n);
+  Task slab{{x[0], xe[0]}, {x[1], xe[1]}, {x[2], xe[2]}};
+  Task whole{{0, s.g.nx()}, {0, s.g.ny()}, {0, s.g.nz()}};
+  auto t = intersection(whole, slab);
+  if (!t) {
+    node_task = Task{};
+    return;
+  }
+  node_task = *t;
+  Index cstart = node_task.r(main).s;
+  node_e_start = dirNegative(dir) ? cstart + 1 : cstart;
+  node_h_start = cstart;
+}
+
+void PML::recursiveConvolution(Sim& s) {  // pml.cpp:526-633
+  auto rho = [&](const Arr1& size, Real frac) {
+    Arr1 interval(n), d(n);
+    for (Index i = 0; i < n; ++i) interval(i) = size(i) - frac * size(i);
+    d(0) = interval(0);
+    Real sum = 0;
+    for (Index i = 1; i < n; ++i) {
+      sum += size(i - 1);
+      d(i) = sum + interval(i);
+    }
+    sum += size(n - 1);
+    if (dirNegative(dir)) std::reverse(d.v.begin(), d.v.end());
+    for (Index i = 0; i < n; ++i) d(i) = d(i) / sum;
+    return d;
+  };
+  auto sigma_max = [&](Real dl) {
+    return sigma_ratio * (order + 1) / (150 * PI * dl);
+  };
+  const Real dt = s.dt;
+  const Real ord = static_cast<Real>(order);
+
+  Real min_h = *std::min_element(g_h_size.v.begin(), g_h_size.v.end());
+  Real smax_e = sigma_max(min_h);
+  Arr1 rho_e = rho(g_h_size, 0.75);
+  kappa_e = Arr1(n);
+  a_e = Arr1(n);
+  b_e = Arr1(n);
+  for (Index i = 0; i < n; ++i) {
+    Real sg = smax_e * std::pow(rho_e(i), ord);
+    Real al = alpha_min + (alpha_max - alpha_min) * (1 - rho_e(i));
+    Real ka = 1 + (kappa_max - 1) * std::pow(rho_e(i), ord);
+    kappa_e(i) = ka;
+    b_e(i) = std::exp((-dt / EPSILON_0) * (sg / ka + al));
+    a_e(i) = (b_e(i) - 1) * sg / ((sg + ka * al) * ka * g_h_size(i));
+  }
+
+  Real e2m = MU_0 / EPSILON_0;
+  Real min_e = *std::min_element(g_e_size.v.begin(), g_e_size.v.end());
+  Real smax_m = sigma_max(min_e) * e2m;
+  Arr1 rho_m = rho(g_e_size, 0.25);
+  kappa_h = Arr1(n);
+  a_h = Arr1(n);
+  b_h = Arr1(n);
+  for (Index i = 0; i < n; ++i) {
+    Real sg = smax_m * std::pow(rho_m(i), ord);
+    Real al = (alpha_min + (alpha_max - alpha_min) * (1 - rho_m(i))) * e2m;
+    Real ka = 1 + (kappa_max - 1) * std::pow(rho_m(i), ord);
+    kappa_h(i) = ka;
+    b_h(i) = std::exp((-dt / MU_0) * (sg / ka + al));
+    a_h(i) = (b_h(i) - 1) * sg / ((sg + ka * al) * ka * g_e_size(i));
+  }
+}
+
+void PML::correctUpdateCoefficient(Sim& s) {  // pml.cpp:214-244
+  if (!node_task.valid()) return;
+  g_h_size = Arr1(n);
+  g_e_size = Arr1(n);
+  for (Index i = 0; i < n; ++i) {
+    g_h_size(i) = s.g.h_size[main](g_e_start + i);
+    g_e_size(i) = s.g.e_size[main](g_h_start + i);
+  }
+  recursiveConvolution(s);
+
+  // pml.cpp:286-524, written once for the three axes through the (a,b,c)
+  // rotation.  For slab axis c: E_a is corrected through the c-derivative of
+  // H_b (coefficient dualB of E_a... see below) and E_b through H_a.
+  const Axis c = main, a = tanA(c), b = tanB(c);
+  const Index nn[3] = {node_task.x.size(), node_task.y.size(), node_task.z.size()};
+  // psi arrays: E_a / H_b are extended (+1) along b; E_b / H_a along a.
+  auto ext = [&](Axis along) {
+    Index m[3] = {nn[0], nn[1], nn[2]};
+    m[along] += 1;
+    return Arr3(m[0], m[1], m[2]);
+  };
+  c_ea_psi_hb = ext(b);
+  ea_psi_hb = ext(b);
+  c_eb_psi_ha = ext(a);
+  eb_psi_ha = ext(a);
+  c_ha_psi_eb = ext(a);
+  ha_psi_eb = ext(a);
+  c_hb_psi_ea = ext(b);
+  hb_psi_ea = ext(b);
+
+  // The E_a coefficient multiplying (H_b(c) - H_b(c-1)) is the E_a
+  // coefficient for its tangential-B dual when E_a's own tangential axes are
+  // (b', c') ... resolved concretely: X slab -> ceyhz, cezhy; Y slab ->
+  // cezhx, cexhz; Z slab -> cexhy, ceyhx (pml.cpp:300-305, 377-381, 457-461).
+  Arr3* ceahb;
+  Arr3* cebha;
+  Arr3* chaeb;
+  Arr3* chbea;
+  if (c == X) {
+    ceahb = &s.c.ceyhz; cebha = &s.c.cezhy; chaeb = &s.c.chyez; chbea = &s.c.chzey;
+  } else if (c == Y) {
+    ceahb = &s.c.cezhx; cebha = &s.c.cexhz; chaeb = &s.c.chzex; chbea = &s.c.chxez;
+  } else {
+    ceahb = &s.c.cexhy; cebha = &s.c.ceyhx; chaeb = &s.c.chxey; chbea = &s.c.chyex;
+  }
+  auto patch = [&](Arr3& cpsi, Arr3& coef, Index start, const Arr1& dl, const Arr1& kappa,
+                   Index gstart) {
+    // loops i, j, k over the psi extent; the main-axis index is offset by the
+    // node start (pml.cpp:307-327 and the Y/Z analogues)
+    for (Index i = 0; i < cpsi.n0; ++i)
+      for (Index j = 0; j < cpsi.n1; ++j)
+        for (Index k = 0; k < cpsi.n2; ++k) {
+          Index idx[3] = {i, j, k};
+          Index layer = idx[c];
+          idx[c] = layer + start;
+          Index g = idx[c];  // global == node (single process)
+          Real& cf = coef(idx[0], idx[1], idx[2]);
+          cpsi(i, j, k) = cf * dl(g - gstart);
+          cf = cf / kappa(g - gstart);
+        }
+  };
+  patch(c_ea_psi_hb, *ceahb, node_e_start, g_h_size, kappa_e, g_e_start);
+  patch(c_eb_psi_ha, *cebha, node_e_start, g_h_size, kappa_e, g_e_start);
+  patch(c_ha_psi_eb, *chaeb, node_h_start, g_e_size, kappa_h, g_h_start);
+  patch(c_hb_psi_ea, *chbea, node_h_start, g_e_size, kappa_h, g_h_start);
+}
+
+namespace {
+// correctPML (pml_corrector.h:14-52)
+template <bool IsE>
+void correctPML(Arr3& field, Arr3& psi, const Arr1& ca, const Arr1& cb, const Arr3& dual,
+                const Arr3& cpsi, Axis xyz, Index is, Index ie, Index js, Index je, Index ks,
+                Index ke, Index gstart, Index nstart) {
+  for (Index i = is; i < ie; ++i)
+    for (Index j = js; j < je; ++j)
+      for (Index k = ks; k < ke; ++k) {
+        auto abc = toABC(xyz, i, j, k);
+        Index gc = abc[2];
+        Real av = ca(gc - gstart), bv = cb(gc - gstart);
+        Index layer = abc[2] - nstart;
+        auto l = fromABC(xyz, abc[0], abc[1], layer);
+        Real cp = cpsi(l[0], l[1], l[2]);
+        Real& pv = psi(l[0], l[1], l[2]);
+        Real dp = dual(i, j, k);
+        auto q = fromABC(xyz, abc[0], abc[1], IsE ? abc[2] - 1 : abc[2] + 1);
+        Real dq = dual(q[0], q[1], q[2]);
+        if (IsE)
+          pv = bv * pv + av * (dp - dq);
+        else
+          pv = bv * pv + av * (dq - dp);
+        field(i, j, k) += cp * pv;
+      }
+}
+
+struct PMLCorrector : Corrector {  // pml_corrector.h:54-382
+  Sim* s;
+  PML* p;
+  Task task;
+  void correctE(Index) override {
+    if (!task.valid()) return;
+    const Axis c = p->main, a = tanA(c), b = tanB(c);
+    Index st[3] = {task.x.s, task.y.s, task.z.s};
+    Index off = 0;
+    {
+      auto abc = toABC(c, st[0], st[1], st[2]);
+      if (abc[2] == 0) {
+        auto q = fromABC(c, abc[0], abc[1], abc[2] + 1);
+        off = 1;
+        st[0] = q[0];
+        st[1] = q[1];
+        st[2] = q[2];
+      }
+    }
+    auto nt = toABC(c, p->node_task.x.e, p->node_task.y.e, p->node_task.z.e);
+    auto te = toABC(c, task.x.e, task.y.e, task.z.e);
+    {  // E_a with dual H_b: [a_s,a_e) x [b_s, b_e(+1)) x [c_s, c_e + off)
+      auto e = (te[1] == nt[1]) ? fromABC(c, te[0], te[1] + 1, te[2] + off)
+                                : fromABC(c, te[0], te[1], te[2] + off);
+      correctPML<true>(s->emf.f(0, a), p->ea_psi_hb, p->a_e, p->b_e, s->emf.f(1, b),
+                       p->c_ea_psi_hb, c, st[0], e[0], st[1], e[1], st[2], e[2], p->g_e_start,
+                       p->node_e_start);
+    }
+    {  // E_b with dual H_a: [a_s, a_e(+1)) x [b_s, b_e) x [c_s, c_e + off)
+      auto e = (te[0] == nt[0]) ? fromABC(c, te[0] + 1, te[1], te[2] + off)
+                                : fromABC(c, te[0], te[1], te[2] + off);
+      correctPML<true>(s->emf.f(0, b), p->eb_psi_ha, p->a_e, p->b_e, s->emf.f(1, a),
+                       p->c_eb_psi_ha, c, st[0], e[0], st[1], e[1], st[2], e[2], p->g_e_start,
+                       p->node_e_start);
+    }
+  }
+  void correctH(Index) override {
+    if (!task.valid()) return;
+    const Axis c = p->main, a = tanA(c), b = tanB(c);
+    auto nt = toABC(c, p->node_task.x.e, p->node_task.y.e, p->node_task.z.e);
+    auto te = toABC(c, task.x.e, task.y.e, task.z.e);
+    {  // H_a with dual E_b: [a_s, a_e(+1)) x [b_s, b_e) x [c_s, c_e)
+      auto e = (te[0] == nt[0]) ? fromABC(c, te[0] + 1, te[1], te[2])
+                                : fromABC(c, te[0], te[1], te[2]);
+      correctPML<false>(s->emf.f(1, a), p->ha_psi_eb, p->a_h, p->b_h, s->emf.f(0, b),
+                        p->c_ha_psi_eb, c, task.x.s, e[0], task.y.s, e[1], task.z.s, e[2],
+                        p->g_h_start, p->node_h_start);
+    }
+    {  // H_b with dual E_a: [a_s, a_e) x [b_s, b_e(+1)) x [c_s, c_e)
+      auto e = (te[1] == nt[1]) ? fromABC(c, te[0], te[1] + 1, te[2])
+                                : fromABC(c, te[0], te[1], te[2]);
+      correctPML<false>(s->emf.f(1, b), p->hb_psi_ea, p->a_h, p->b_h, s->emf.f(0, a),
+                        p->c_hb_psi_ea, c, task.x.s, e[0], task.y.s, e[1], task.z.s, e[2],
+                        p->g_h_start, p->node_h_start);
+    }
+  }
+};
+}  // namespace
+
+std::unique_ptr<Corrector> PML::generateCorrector(Sim& s, const Task& t) {
+  // pml.cpp:246-284
+  if (!node_task.valid() || !intersected(t, node_task)) return nullptr;
+  auto in = intersection(t, node_task);
+  auto c = std::make_unique<PMLCorrector>();
+  c->s = &s;
+  c->p = this;
+  c->task = *in;
+  return c;
+}
+
+// ------------------------------------------------------------------ TFSF ---
+Task TFSF3D::globalTask() const {
+  return {{box_origin[0], box_origin[0] + box_size[0]},
+          {box_origin[1], box_origin[1] + box_size[1]},
+          {box_origin[2], box_origin[2] + box_size[2]}};
+}
+
+void TFSF3D::init(Sim& s) {
+  // constructor (tfsf.cpp:14-30) and initTransform (tfsf.cpp:231-254)
+  st = std::sin(theta);
+  ct = std::cos(theta);
+  sp = std::sin(phi);
+  cp = std::cos(phi);
+  ss = std::sin(psi);
+  cs = std::cos(psi);
+  k = Vec3{st * cp, st * sp, ct};
+  Vec3 va{-sp, ct * cp, st * cp};
+  Vec3 vb{cp, ct * sp, st * sp};
+  Vec3 vc{0, -st, ct};
+  Vec3 ke{ss, cs, 0};
+  transform_e = Vec3{va.dot(ke), vb.dot(ke), vc.dot(ke)};
+  transform_h = k.cross(transform_e);
+
+  // defaultInit (tfsf.cpp:118-175)
+  Index sz[3] = {s.g.nx(), s.g.ny(), s.g.nz()};
+  Index off[3] = {x, y, z};
+  for (int a = 0; a < 3; ++a) {
+    box_origin[a] = 0 + off[a];
+    box_size[a] = sz[a] - 2 * off[a];
+  }
+  ratio_delta = 1 / std::sqrt(std::pow(st, 4) * (std::pow(cp, 4) + std::pow(sp, 4)) +
+                              std::pow(ct, 4));
+  aux = static_cast<Index>(
+      std::ceil(ratio_delta * (std::sqrt(std::pow(static_cast<Real>(box_size[0]), 2) +
+                                         std::pow(static_cast<Real>(box_size[1]), 2) +
+                                         std::pow(static_cast<Real>(box_size[2]), 2)))) +
+      4 + 1);
+
+  // calculateProjection (tfsf.cpp:256-302); injection corner (:304-339)
+  std::array<Index, 3> inj;
+  for (int a = 0; a < 3; ++a)
+    inj[a] = (k[a] >= 0) ? box_origin[a] : box_origin[a] + box_size[a];
+  Real inc[3];
+  for (int a = 0; a < 3; ++a) inc[a] = static_cast<Real>(inj[a]) - (2 * k[a] / ratio_delta);
+  auto get_int = [&](Index nn, Real inc_p, Index org, Real kk) {
+    Arr1 arr(nn);
+    for (Index i = 0; i < nn; ++i) arr(i) = (i + org - inc_p) * kk * ratio_delta;
+    return arr;
+  };
+  auto get_half = [&](Index nn, Index org, Real inc_p, Real kk) {
+    Arr1 arr(nn);
+    for (Index i = 0; i < nn; ++i) arr(i) = (i + org - inc_p - 0.5) * kk * ratio_delta;
+    return arr;
+  };
+  px_int = get_int(box_size[0] + 1, inc[0], box_origin[0], k.x);
+  py_int = get_int(box_size[1] + 1, inc[1], box_origin[1], k.y);
+  pz_int = get_int(box_size[2] + 1, inc[2], box_origin[2], k.z);
+  px_half = get_half(box_size[0] + 2, box_origin[0], inc[0], k.x);
+  py_half = get_half(box_size[1] + 2, box_origin[1], inc[1], k.y);
+  pz_half = get_half(box_size[2] + 2, box_origin[2], inc[2], k.z);
+
+  const Real dt = s.dt;
+  scaled_dl = s.g.based[2] / ratio_delta;
+  ceie = 1;
+  chih = 1;
+  ceihi = -(dt / (EPSILON_0 * scaled_dl));
+  chiei = -(dt / (MU_0 * scaled_dl));
+  abc0 = (C_0 * dt - scaled_dl) / (C_0 * dt + scaled_dl);
+  abc1 = 2 * scaled_dl / (C_0 * dt + scaled_dl);
+  for (int a = 0; a < 3; ++a) {  // tfsf.cpp:177-205
+    ca[a] = dt / (EPSILON_0 * s.g.based[a]);
+    cb[a] = dt / (MU_0 * s.g.based[a]);
+  }
+}
+
+void TFSF3D::initTimeDependentVariable(Sim& s) {  // tfsf.cpp:36-78
+  Arr1 t = s.eTime();
+  for (Index i = 0; i < t.size(); ++i) t(i) = t(i) - s.dt;
+  waveform.init(t);
+  const Index nt = s.size;
+  e_inc = Arr2(nt, aux);
+  h_inc = Arr2(nt, aux - 1);
+  auto& e = e_inc;
+  auto& h = h_inc;
+  const Index nl = aux - 1;
+  const Arr1& src = waveform.value;
+  if (nt == 0) return;
+  e(0, 0) = src(0);
+  for (Index l = 1; l < nl; ++l) h(0, l) = chih * h(0, l) + chiei * (e(0, l + 1) - e(0, l));
+  for (Index tt = 1; tt < nt; ++tt) {
+    for (Index l = 1; l < nl; ++l) {
+      e(tt, 0) = src(tt);
+      e(tt, l) = ceie * e(tt - 1, l) + ceihi * (h(tt - 1, l) - h(tt - 1, l - 1));
+    }
+    Real e_l_prev_t_prev_prev = (tt < 2) ? 0.0 : e(tt - 2, nl - 1);
+    Real e_l_t_prev_prev = (tt < 2) ? 0.0 : e(tt - 2, nl);
+    e(tt, nl) = -e_l_prev_t_prev_prev + abc0 * (e(tt, nl - 1) + e_l_t_prev_prev) +
+                abc1 * (e(tt - 1, nl) + e(tt - 1, nl - 1));
+    for (Index l = 0; l < nl; ++l) h(tt, l) = chih * h(tt - 1, l) + chiei * (e(tt, l + 1) - e(tt, l));
+  }
+}
+
+namespace {
+struct TFSFCorrector : Corrector {  // tfsf_corrector.h:12-351, tfsf_corrector.cpp
+  Sim* s;
+  TFSF3D* f;
+  Task task, node, global;
+  bool flag[6];
+
+  // incident field component `comp` (0..2) of attribute (H if h) at global
+  // (i, j, k), time row t (tfsf_corrector.cpp:16-122)
+  Real inc(bool h, int comp, Index t, Index i, Index j, Index k) const {
+    Index ii = i - global.x.s, jj = j - global.y.s, kk = k - global.z.s;
+    Real p;
+    if (!h) {
+      if (comp == 0) p = f->px_half(ii + 1) + f->py_int(jj) + f->pz_int(kk);
+      else if (comp == 1) p = f->px_int(ii) + f->py_half(jj + 1) + f->pz_int(kk);
+      else p = f->px_int(ii) + f->py_int(jj) + f->pz_half(kk + 1);
+    } else {
+      if (comp == 0) p = f->px_int(ii) + f->py_half(jj + 1) + f->pz_half(kk + 1) - 0.5;
+      else if (comp == 1) p = f->px_half(ii + 1) + f->py_int(jj) + f->pz_half(kk + 1) - 0.5;
+      else p = f->px_half(ii + 1) + f->py_half(jj + 1) + f->pz_int(kk) - 0.5;
+    }
+    Index idx = static_cast<Index>(p);
+    Real w = p - idx;
+    const Arr2& tab = h ? f->h_inc : f->e_inc;
+    Real tr = h ? f->transform_h[comp] : f->transform_e[comp];
+    return tr * ((1 - w) * tab(t, idx) + w * tab(t, idx + 1));
+  }
+
+  Index nodeGlobalEnd(int axis) const { return node.r(axis).e; }
+
+  // correctTFSF<direction, attribute, field_xyz> (tfsf_corrector.h:89-229)
+  void correct(Dir d, bool h, Axis fxyz, Index t) {
+    if (!flag[d]) return;
+    const Axis xyz = dirAxis(d);
+    const Axis da = tanA(xyz), db = tanB(xyz);
+    const Axis dual_field = (fxyz == da) ? db : da;
+    auto s0 = toABC(xyz, task.x.s, task.y.s, task.z.s);
+    auto e0 = toABC(xyz, task.x.e, task.y.e, task.z.e);
+    Index as = s0[0], bs = s0[1], cs = s0[2], ae = e0[0], be = e0[1], ce = e0[2];
+    const int offset_a = ((!h && fxyz == db) || (h && fxyz == da)) ? 1 : 0;
+    const int offset_b = offset_a == 1 ? 0 : 1;
+    if (offset_a == 1 && ae == nodeGlobalEnd(da)) ae += 1;
+    if (offset_b == 1 && be == nodeGlobalEnd(db)) be += 1;
+    const bool neg = dirNegative(d);
+    cs = neg ? cs : ce;
+    ce = cs + 1;
+    const int flagc = 1 * (h ? -1 : 1) * ((fxyz == da) ? -1 : 1) * (neg ? -1 : 1);
+    auto lo = fromABC(xyz, as, bs, cs);
+    auto hi = fromABC(xyz, ae, be, ce);
+    const Real coef_base = h ? f->cb[xyz] : f->ca[xyz];
+    Arr3& field = s->emf.f(h ? 1 : 0, fxyz);
+    for (Index i = lo[0]; i < hi[0]; ++i)
+      for (Index j = lo[1]; j < hi[1]; ++j)
+        for (Index k = lo[2]; k < hi[2]; ++k) {
+          auto abc = toABC(xyz, i, j, k);
+          const Real coefficient = coef_base * flagc;
+          if (neg) {
+            if (!h) {
+              auto q = fromABC(xyz, abc[0], abc[1], abc[2] - 1);
+              Real v = inc(!h, dual_field, t, q[0], q[1], q[2]);
+              field(i, j, k) += v * coefficient;
+            } else {
+              Real v = inc(!h, dual_field, t, i, j, k);
+              auto q = fromABC(xyz, abc[0], abc[1], abc[2] - 1);
+              field(q[0], q[1], q[2]) += v * coefficient;
+            }
+            continue;
+          }
+          Real v = inc(!h, dual_field, t, i, j, k);
+          field(i, j, k) += v * coefficient;
+        }
+  }
+
+  void correctE(Index t) override {  // tfsf_corrector.cpp:177-203
+    correct(XN, false, Z, t);
+    correct(XP, false, Z, t);
+    correct(YN, false, Z, t);
+    correct(YP, false, Z, t);
+    correct(ZN, false, Y, t);
+    correct(ZP, false, Y, t);
+    correct(XN, false, Y, t);
+    correct(XP, false, Y, t);
+    correct(ZN, false, X, t);
+    correct(ZP, false, X, t);
+    correct(YN, false, X, t);
+    correct(YP, false, X, t);
+  }
+  void correctH(Index t) override {  // tfsf_corrector.cpp:205-232
+    correct(XN, true, Y, t);
+    correct(XP, true, Y, t);
+    correct(YN, true, X, t);
+    correct(YP, true, X, t);
+    correct(ZN, true, X, t);
+    correct(ZP, true, X, t);
+    correct(XN, true, Z, t);
+    correct(XP, true, Z, t);
+    correct(YN, true, Z, t);
+    correct(YP, true, Z, t);
+    correct(ZN, true, Y, t);
+    correct(ZP, true, Y, t);
+  }
+};
+}  // namespace
+
+std::unique_ptr<Corrector> TFSF3D::generateCorrector(Sim& s, const Task& t) {
+  // tfsf_3d.cpp:21-45 (node offset 0 in a single process)
+  Task g = globalTask();
+  auto in = intersection(t, g);
+  if (!in) return nullptr;
+  auto c = std::make_unique<TFSFCorrector>();
+  c->s = &s;
+  c->f = this;
+  c->task = *in;
+  c->global = g;
+  Task whole{{0, s.g.nx()}, {0, s.g.ny()}, {0, s.g.nz()}};
+  auto nt = intersection(whole, g);
+  c->node = nt ? *nt : Task{};
+  c->flag[XN] = in->x.s == g.x.s;
+  c->flag[XP] = in->x.e == g.x.e;
+  c->flag[YN] = in->y.s == g.y.s;
+  c->flag[YP] = in->y.e == g.y.e;
+  c->flag[ZN] = in->z.s == g.z.s;
+  c->flag[ZP] = in->z.e == g.z.e;
+  return c;
+}
+
+// -------------------------------------------------------------- monitors --
+static void monitorBox(Sim& s, const Shape& sh, Index o[3], Index e[3]) {
+  // Monitor::defaultInit (monitor.cpp:103-127) then the +1 shift of a box
+  // that starts at index 0 (field_monitor.cpp:24-67).
+  Index end[3];
+  auto org = s.g.box(sh, end);
+  for (int a = 0; a < 3; ++a) {
+    o[a] = org[a];
+    e[a] = end[a];
+    if (o[a] == 0) o[a] = 1;  // origin +1, size -1: end unchanged
+  }
+}
+
+void FieldMonitor::init(Sim& s) { monitorBox(s, *shape, o, e); }
+
+void FieldMonitor::output(Sim& s) {
+  bool valid = o[0] < e[0] && o[1] < e[1] && o[2] < e[2];  // monitor.cpp:155
+  if (!valid) return;
+  Arr3& f = s.emf.byId(field);
+  data = Arr3(e[0] - o[0], e[1] - o[1], e[2] - o[2]);
+  for (Index i = 0; i < data.n0; ++i)
+    for (Index j = 0; j < data.n1; ++j)
+      for (Index k = 0; k < data.n2; ++k) data(i, j, k) = f(o[0] + i, o[1] + j, o[2] + k);
+  if (s.write_files && !out_dir.empty()) {
+    std::filesystem::create_directories(out_dir);
+    writeNpy(out_dir + "/" + name + ".npy", {data.n0, data.n1, data.n2}, data.v.data());
+  }
+}
+
+void MovieMonitor::init(Sim& s) {
+  frame->out_dir = out_dir;
+  frame->init(s);
+}
+
+void MovieMonitor::update(Sim& s, Index) {
+  if (count % interval == 0) {
+    char buf[32];
+    std::snprintf(buf, sizeof(buf), "%05zu", count);
+    frame->name = buf;
+    frame->output(s);
+    frames.emplace_back(buf, frame->data);
+  }
+  ++count;
+}
+
+void FieldTimeMonitor::init(Sim& s) { monitorBox(s, *shape, o, e); }
+
+void FieldTimeMonitor::initTimeDependentVariable(Sim& s) {
+  time = (field < 3) ? s.eTime() : s.hTime();
+  data = Arr1(time.size());
+}
+
+void FieldTimeMonitor::update(Sim& s, Index t) {
+  Arr3& f = s.emf.byId(field);
+  for (Index i = o[0]; i < e[0]; ++i)
+    for (Index j = o[1]; j < e[1]; ++j)
+      for (Index k = o[2]; k < e[2]; ++k) data(t) = f(i, j, k);
+}
+
+void FieldTimeMonitor::output(Sim& s) {
+  bool valid = o[0] < e[0] && o[1] < e[1] && o[2] < e[2];
+  if (!valid || !s.write_files || out_dir.empty()) return;
+  std::vector<Real> st(2 * time.size());
+  for (Index i = 0; i < time.size(); ++i) {
+    st[i] = time(i);
+    st[time.size() + i] = data(i);
+  }
+  std::filesystem::create_directories(out_dir);
+  writeNpy(out_dir + "/" + name + ".npy", {2, time.size()}, st.data());
+}
+
+// ------------------------------------------------------------ simulation --
+Arr1 Sim::eTime() const {
+  return linspace((start_step + 1) * dt, (start_step + size) * dt, size);
+}
+Arr1 Sim::hTime() const {
+  return linspace((start_step + 0.5) * dt, (start_step + size + 0.5) * dt, size);
+}
+
+void Sim::generateGridSpace() {
+  // Shape::makeWrappedCube (shape.cpp:8-38)
+  Real mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = std::numeric_limits<Real>::max();
+    mx[a] = std::numeric_limits<Real>::lowest();
+  }
+  if (objects.empty()) throw Error("No object is added");
+  for (auto& o : objects) {
+    Cube c = o->shape->wrapped();
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = std::min(mn[a], c.origin[a]);
+      mx[a] = std::max(mx[a], c.end[a]);
+    }
+  }
+  Cube dom(Vec3{mn[0], mn[1], mn[2]}, Vec3{mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]});
+  // GridSpaceGenerator::extendDomain (grid_space_generator.cpp:36-99)
+  Real d[3] = {dx, dy, dz};
+  if (!pmls.empty()) {
+    Real lo[3] = {dom.origin.x, dom.origin.y, dom.origin.z};
+    Real hi[3] = {dom.end.x, dom.end.y, dom.end.z};
+    for (auto& p : pmls) {
+      if (p->thickness < 0) continue;
+      int a = dirAxis(p->dir);
+      if (dirNegative(p->dir))
+        lo[a] -= p->thickness * d[a];
+      else
+        hi[a] += p->thickness * d[a];
+    }
+    dom = Cube(Vec3{lo[0], lo[1], lo[2]}, Vec3{hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+  }
+  // generateGridSpace3D (grid_space_generator.cpp:183-217)
+  Real lo[3] = {dom.origin.x, dom.origin.y, dom.origin.z};
+  Real hi[3] = {dom.end.x, dom.end.y, dom.end.z};
+  Index nn[3];
+  for (int a = 0; a < 3; ++a) {
+    nn[a] = static_cast<Index>(std::round((hi[a] - lo[a]) / d[a]));
+    if (nn[a] <= 1) throw Error("GridSpace is too small");
+  }
+  for (int a = 0; a < 3; ++a) hi[a] = lo[a] + d[a] * nn[a];
+  Cube region(Vec3{lo[0], lo[1], lo[2]}, Vec3{hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+  for (int a = 0; a < 3; ++a) {
+    g.based[a] = d[a];
+    g.e_node[a] = linspace(region.origin[a], region.end[a], nn[a] + 1);
+  }
+  // GridSpace3D::correctGridSpace (grid_space_3d.cpp:111-137, grid_space.cpp:526-544)
+  for (int a = 0; a < 3; ++a) {
+    const Arr1& e = g.e_node[a];
+    Index m = e.size() - 1;
+    g.h_node[a] = Arr1(m);
+    g.e_size[a] = Arr1(m);
+    for (Index i = 0; i < m; ++i) {
+      g.h_node[a](i) = (e(i) + e(i + 1)) / 2;
+      g.e_size[a](i) = e(i + 1) - e(i);
+    }
+    std::vector<Real> tmp(m + 2);
+    tmp[0] = e(0) - d[a] / 2;
+    for (Index i = 0; i < m; ++i) tmp[i + 1] = g.h_node[a](i);
+    tmp[m + 1] = e(m) + d[a] / 2;
+    g.h_size[a] = Arr1(m + 1);
+    for (Index i = 0; i < m + 1; ++i) g.h_size[a](i) = tmp[i + 1] - tmp[i];
+    Real md = std::numeric_limits<Real>::max();
+    for (Index i = 0; i < m; ++i) md = std::min(md, g.e_size[a](i));
+    g.min_d[a] = md;
+  }
+}
+
+void Sim::generateEMF() {
+  const Index nx = g.nx(), ny = g.ny(), nz = g.nz();
+  emf.ex = Arr3(nx, ny + 1, nz + 1);
+  emf.ey = Arr3(nx + 1, ny, nz + 1);
+  emf.ez = Arr3(nx + 1, ny + 1, nz);
+  emf.hx = Arr3(nx + 1, ny, nz);
+  emf.hy = Arr3(nx, ny + 1, nz);
+  emf.hz = Arr3(nx, ny, nz + 1);
+}
+
+static void extentOf(int h, int c, Index nx, Index ny, Index nz, Index out[3]) {
+  // electric component c is nx+1 on the other two axes; magnetic on its own
+  Index n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) out[a] = h ? (a == c ? n[a] + 1 : n[a]) : (a == c ? n[a] : n[a] + 1);
+}
+
+void Sim::generateMaterialSpace() {  // simulation.cpp:593-636
+  const Index nx = g.nx(), ny = g.ny(), nz = g.nz();
+  material_grid.assign(nx * ny * nz, std::numeric_limits<Index>::max());
+  for (int c = 0; c < 3; ++c) {  // material_param.cpp:365-393
+    Index ee[3], hh[3];
+    extentOf(0, c, nx, ny, nz, ee);
+    extentOf(1, c, nx, ny, nz, hh);
+    mp.eps[c] = Arr3(ee[0], ee[1], ee[2], EPSILON_0);
+    mp.se[c] = Arr3(ee[0], ee[1], ee[2], SIGMA_ZERO);
+    mp.mu[c] = Arr3(hh[0], hh[1], hh[2], MU_0);
+    mp.sm[c] = Arr3(hh[0], hh[1], hh[2], SIGMA_ZERO);
+  }
+  Index m_index = 0;
+  for (auto& o : objects) {
+    if (o->isPecPlane()) continue;
+    o->correctMaterialSpace(*this, m_index);
+    material_array.push_back(o->material);
+    ++m_index;
+  }
+  averageMaterial();
+  for (auto& o : objects)
+    if (o->isPecPlane()) o->correctMaterialSpace(*this, Index(-1));
+}
+
+void Sim::averageMaterial() {  // calculation_param.cpp:43-168
+  const Index nx = g.nx(), ny = g.ny(), nz = g.nz();
+  auto arith = [&](Arr3& a, int c) {
+    // average of the four cells sharing the edge; evaluated from a copy
+    Arr3 o = a;
+    Index is = c == 0 ? 0 : 1, js = c == 1 ? 0 : 1, ks = c == 2 ? 0 : 1;
+    for (Index i = is; i < nx; ++i)
+      for (Index j = js; j < ny; ++j)
+        for (Index k = ks; k < nz; ++k) {
+          Real v;
+          if (c == 0)
+            v = o(i, j, k) + o(i, j - 1, k) + o(i, j, k - 1) + o(i, j - 1, k - 1);
+          else if (c == 1)
+            v = o(i, j, k) + o(i - 1, j, k) + o(i, j, k - 1) + o(i - 1, j, k - 1);
+          else
+            v = o(i, j, k) + o(i - 1, j, k) + o(i, j - 1, k) + o(i - 1, j - 1, k);
+          a(i, j, k) = 0.25 * v;
+        }
+  };
+  auto harm = [&](Arr3& a, int c) {
+    Arr3 o = a;
+    Index is = c == 0 ? 1 : 0, js = c == 1 ? 1 : 0, ks = c == 2 ? 1 : 0;
+    for (Index i = is; i < nx; ++i)
+      for (Index j = js; j < ny; ++j)
+        for (Index k = ks; k < nz; ++k) {
+          Real p = o(i, j, k);
+          Real q = c == 0 ? o(i - 1, j, k) : c == 1 ? o(i, j - 1, k) : o(i, j, k - 1);
+          a(i, j, k) = 2 * p * q / (p + q);
+        }
+  };
+  for (int c = 0; c < 3; ++c) arith(mp.eps[c], c);
+  for (int c = 0; c < 3; ++c) harm(mp.mu[c], c);
+  for (int c = 0; c < 3; ++c) arith(mp.se[c], c);
+  for (int c = 0; c < 3; ++c) harm(mp.sm[c], c);
+}
+
+void Sim::calculateCoefficient() {  // calculation_param.cpp:170-264
+  const Index nx = g.nx(), ny = g.ny(), nz = g.nz();
+  for (int h = 0; h < 2; ++h)
+    for (int c = 0; c < 3; ++c) {
+      Index ext[3];
+      extentOf(h, c, nx, ny, nz, ext);
+      Arr3& cs = this->c.self(h, c);
+      Arr3& ca = this->c.dualA(h, c);
+      Arr3& cb = this->c.dualB(h, c);
+      cs = Arr3(ext[0], ext[1], ext[2]);
+      ca = Arr3(ext[0], ext[1], ext[2]);
+      cb = Arr3(ext[0], ext[1], ext[2]);
+      // sizes per axis: E component c uses e_size on axis c and h_size on
+      // the others; H component c uses h_size on c and e_size elsewhere.
+      const Arr1* sz[3];
+      for (int a = 0; a < 3; ++a)
+        sz[a] = h ? (a == c ? &g.h_size[a] : &g.e_size[a]) : (a == c ? &g.e_size[a] : &g.h_size[a]);
+      const Axis A = tanA(static_cast<Axis>(c)), B = tanB(static_cast<Axis>(c));
+      Arr3& m = h ? mp.mu[c] : mp.eps[c];
+      Arr3& sg = h ? mp.sm[c] : mp.se[c];
+      for (Index i = 0; i < ext[0]; ++i)
+        for (Index j = 0; j < ext[1]; ++j)
+          for (Index k = 0; k < ext[2]; ++k) {
+            Index id[3] = {i, j, k};
+            Real da = (*sz[A])(id[A]);
+            Real db = (*sz[B])(id[B]);
+            Real e = m(i, j, k), s = sg(i, j, k);
+            if (!h) {
+              cs(i, j, k) = (2 * e - dt * s) / (2 * e + dt * s);
+              ca(i, j, k) = -(2 * dt / db) / (2 * e + s * dt);
+              cb(i, j, k) = (2 * dt / da) / (2 * e + s * dt);
+            } else {
+              cs(i, j, k) = (2 * e - dt * s) / (2 * e + dt * s);
+              ca(i, j, k) = (2 * dt / db) / (2 * e + s * dt);
+              cb(i, j, k) = -(2 * dt / da) / (2 * e + s * dt);
+            }
+          }
+    }
+}
+
+void Sim::init() {  // simulation.cpp:319-371
+  if (objects.empty()) throw Error("No object is added");
+  generateGridSpace();
+  generateEMF();
+  // TimeParam::calculateDt (time_param.cpp:10-13)
+  dt = cfl / (C_0 * std::sqrt(1.0 / (g.min_d[0] * g.min_d[0]) + 1.0 / (g.min_d[1] * g.min_d[1]) +
+                              1.0 / (g.min_d[2] * g.min_d[2])));
+  for (auto& o : objects) o->init(*this);
+  for (auto& p : pmls) p->init(*this);
+  for (auto& t : tfsfs) t->init(*this);
+  generateMaterialSpace();
+  calculateCoefficient();
+  for (auto& o : objects) o->correctUpdateCoefficient(*this);  // simulation.cpp:638-652
+  for (auto& p : pmls) p->correctUpdateCoefficient(*this);
+  for (auto& m : monitors) m->init(*this);
+}
+
+void Sim::initSteps(Index n) {  // simulation.cpp:373-389
+  init();
+  start_step = 0;
+  size = n;
+  current = 0;
+  for (auto& o : objects) o->initTimeDependentVariable(*this);
+  for (auto& t : tfsfs) t->initTimeDependentVariable(*this);
+  for (auto& m : monitors) m->initTimeDependentVariable(*this);
+}
+
+namespace {
+// Domain::run (domain.cpp:82-135) for one thread task.
+struct Domain {
+  Sim* s;
+  Task task;
+  bool master;
+  std::vector<std::unique_ptr<Corrector>> correctors;
+
+  void updateH() {  // basic_updator.cpp:23-41, update_scheme.h:64-108
+    auto& f = s->emf;
+    auto& c = s->c;
+    for (Index i = task.x.s; i < task.x.e; ++i)
+      for (Index j = task.y.s; j < task.y.e; ++j)
+        for (Index k = task.z.s; k < task.z.e; ++k)
+          f.hx(i, j, k) = c.chxh(i, j, k) * f.hx(i, j, k) +
+                          c.chxey(i, j, k) * (f.ey(i, j, k + 1) - f.ey(i, j, k)) +
+                          c.chxez(i, j, k) * (f.ez(i, j + 1, k) - f.ez(i, j, k));
+    for (Index i = task.x.s; i < task.x.e; ++i)
+      for (Index j = task.y.s; j < task.y.e; ++j)
+        for (Index k = task.z.s; k < task.z.e; ++k)
+          f.hy(i, j, k) = c.chyh(i, j, k) * f.hy(i, j, k) +
+                          c.chyez(i, j, k) * (f.ez(i + 1, j, k) - f.ez(i, j, k)) +
+                          c.chyex(i, j, k) * (f.ex(i, j, k + 1) - f.ex(i, j, k));
+    for (Index i = task.x.s; i < task.x.e; ++i)
+      for (Index j = task.y.s; j < task.y.e; ++j)
+        for (Index k = task.z.s; k < task.z.e; ++k)
+          f.hz(i, j, k) = c.chzh(i, j, k) * f.hz(i, j, k) +
+                          c.chzex(i, j, k) * (f.ex(i, j + 1, k) - f.ex(i, j, k)) +
+                          c.chzey(i, j, k) * (f.ey(i + 1, j, k) - f.ey(i, j, k));
+  }
+
+  void updateE() {  // basic_updator_3d.cpp:38-69
+    auto& f = s->emf;
+    auto& c = s->c;
+    const Index xs = task.x.s, ys = task.y.s, zs = task.z.s;
+    const Index xe = task.x.e, ye = task.y.e, ze = task.z.e;
+    const Index xs1 = xs == 0 ? 1 : xs, ys1 = ys == 0 ? 1 : ys, zs1 = zs == 0 ? 1 : zs;
+    for (Index i = xs; i < xe; ++i)
+      for (Index j = ys1; j < ye; ++j)
+        for (Index k = zs1; k < ze; ++k)
+          f.ex(i, j, k) = c.cexe(i, j, k) * f.ex(i, j, k) +
+                          c.cexhy(i, j, k) * (f.hy(i, j, k) - f.hy(i, j, k - 1)) +
+                          c.cexhz(i, j, k) * (f.hz(i, j, k) - f.hz(i, j - 1, k));
+    for (Index i = xs1; i < xe; ++i)
+      for (Index j = ys; j < ye; ++j)
+        for (Index k = zs1; k < ze; ++k)
+          f.ey(i, j, k) = c.ceye(i, j, k) * f.ey(i, j, k) +
+                          c.ceyhz(i, j, k) * (f.hz(i, j, k) - f.hz(i - 1, j, k)) +
+                          c.ceyhx(i, j, k) * (f.hx(i, j, k) - f.hx(i, j, k - 1));
+    for (Index i = xs1; i < xe; ++i)
+      for (Index j = ys1; j < ye; ++j)
+        for (Index k = zs; k < ze; ++k)
+          f.ez(i, j, k) = c.ceze(i, j, k) * f.ez(i, j, k) +
+                          c.cezhx(i, j, k) * (f.hx(i, j, k) - f.hx(i, j - 1, k)) +
+                          c.cezhy(i, j, k) * (f.hy(i, j, k) - f.hy(i - 1, j, k));
+  }
+};
+}  // namespace
+
+void Sim::run(Index n) {  // simulation.cpp:284-289
+  initSteps(n);
+  loop();
+}
+
+void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
+  Task problem{{0, g.nx()}, {0, g.ny()}, {0, g.nz()}};
+  auto tasks = decomposeTask(problem, tcfg[0], tcfg[1], tcfg[2]);
+  if (tasks.size() > std::max(1u, std::thread::hardware_concurrency()))
+    throw Error("The number of threads is too large");
+  std::vector<Domain> doms(tasks.size());
+  // simulation.cpp:437-489: object correctors (whole problem) on the master
+  // domain, then per-task TFSF and PML correctors, in add order.
+  for (Index id = 0; id < tasks.size(); ++id) {
+    auto& d = doms[id];
+    d.s = this;
+    d.task = tasks[id];
+    d.master = id == 0;
+    if (d.master)
+      for (auto& o : objects)
+        if (auto c = o->generateCorrector(*this, problem)) d.correctors.push_back(std::move(c));
+    for (auto& t : tfsfs)
+      if (auto c = t->generateCorrector(*this, tasks[id])) d.correctors.push_back(std::move(c));
+    for (auto& p : pmls)
+      if (auto c = p->generateCorrector(*this, tasks[id])) d.correctors.push_back(std::move(c));
+  }
+  std::barrier bar(static_cast<std::ptrdiff_t>(doms.size()));
+  auto body = [&](Domain& d) {
+    for (Index t = start_step; t < start_step + size; ++t) {
+      d.updateH();
+      bar.arrive_and_wait();
+      for (auto& c : d.correctors) c->correctH(t);
+      bar.arrive_and_wait();
+      bar.arrive_and_wait();
+      d.updateE();
+      bar.arrive_and_wait();
+      for (auto& c : d.correctors) c->correctE(t);
+      bar.arrive_and_wait();
+      if (d.master) {
+        current = t;
+        for (auto& m : monitors) m->update(*this, t);
+      }
+      bar.arrive_and_wait();
+      bar.arrive_and_wait();
+    }
+  };
+  auto t0 = std::chrono::steady_clock::now();
+  {
+    std::vector<std::thread> th;
+    for (Index id = 1; id < doms.size(); ++id) th.emplace_back(body, std::ref(doms[id]));
+    body(doms[0]);
+    for (auto& x : th) x.join();
+  }
+  loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  current = start_step + size;
+  for (auto& m : monitors) m->output(*this);
+}
+
+// ------------------------------------------------------------------- npy --
+void writeNpy(const std::string& path, const std::vector<Index>& shape, const Real* data) {
+  std::string sh = "(";
+  for (Index i = 0; i < shape.size(); ++i) {
+    sh += std::to_string(shape[i]);
+    if (shape.size() == 1 || i + 1 < shape.size()) sh += ", ";
+  }
+  sh += ")";
+  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': " + sh + ", }";
+  Index total = 10 + hdr.size() + 1;
+  Index pad = (64 - total % 64) % 64;
+  hdr += std::string(pad, ' ') + "\n";
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) throw Error("cannot write " + path);
+  const char magic[] = "\x93NUMPY\x01\x00";
+  std::fwrite(magic, 1, 8, fp);
+  uint16_t hl = static_cast<uint16_t>(hdr.size());
+  std::fwrite(&hl, 2, 1, fp);
+  std::fwrite(hdr.data(), 1, hdr.size(), fp);
+  Index n = 1;
+  for (auto v : shape) n *= v;
+  std::fwrite(data, sizeof(Real), n, fp);
+  std::fclose(fp);
+}
+
+}  // namespace refcpu

@@ -1,0 +1,155 @@
+"""ctypes binding of the refcpu oracle (oracle/build/librefcpu.so).
+
+Test infrastructure: the oracle is the CPU restatement of the reference path
+the GPU results are checked against.  Only tests, the smoke check and the
+benchmark's CPU-baseline leg load it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import List, Tuple
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+LIB = ROOT / "oracle" / "build" / "librefcpu.so"
+FIELD_IDS = {"EX": 0, "EY": 1, "EZ": 2, "HX": 3, "HY": 4, "HZ": 5}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    L = ctypes.CDLL(str(LIB))
+    vp, i64, dbl, cp = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_char_p
+    p64 = ctypes.POINTER(ctypes.c_int64)
+    sig = {
+        "ref_last_error": (cp, []),
+        "ref_create": (vp, [cp]),
+        "ref_destroy": (None, [vp]),
+        "ref_set_write_files": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ref_set_threads": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "ref_run": (ctypes.c_int, [vp, i64]),
+        "ref_init": (ctypes.c_int, [vp, i64]),
+        "ref_loop": (ctypes.c_int, [vp]),
+        "ref_write_field": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+        "ref_dt": (dbl, [vp]),
+        "ref_loop_seconds": (dbl, [vp]),
+        "ref_grid": (ctypes.c_int, [vp, p64]),
+        "ref_grid_array": (i64, [vp, ctypes.c_int, ctypes.c_int, vp]),
+        "ref_field": (ctypes.c_int, [vp, ctypes.c_int, p64, vp]),
+        "ref_coeff": (ctypes.c_int, [vp, cp, p64, vp]),
+        "ref_material": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
+        "ref_pml_profile": (i64, [vp, ctypes.c_int, ctypes.c_int, vp]),
+        "ref_tfsf_table": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
+        "ref_monitor_count": (ctypes.c_int, [vp]),
+        "ref_monitor_records": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ref_monitor_record": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class RefSim:
+    def __init__(self, spec: str, threads: Tuple[int, int, int] = None):
+        self.L = lib()
+        self.h = self.L.ref_create(spec.encode())
+        if not self.h:
+            raise ValueError(self.L.ref_last_error().decode())
+        if threads is not None:
+            self.L.ref_set_threads(self.h, *threads)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+    def run(self, steps: int):
+        self._check(self.L.ref_run(self.h, steps))
+
+    def init(self, steps: int):
+        self._check(self.L.ref_init(self.h, steps))
+
+    def loop(self):
+        self._check(self.L.ref_loop(self.h))
+
+    def write_field(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        shape = self.field(name).shape
+        assert a.shape == shape, (a.shape, shape)
+        self._check(self.L.ref_write_field(self.h, FIELD_IDS[name], a.ctypes.data))
+
+    @property
+    def dt(self):
+        return self.L.ref_dt(self.h)
+
+    @property
+    def loop_seconds(self):
+        return self.L.ref_loop_seconds(self.h)
+
+    def grid(self):
+        n = (ctypes.c_int64 * 3)()
+        self._check(self.L.ref_grid(self.h, n))
+        return tuple(n)
+
+    def grid_array(self, which: int, axis: int) -> np.ndarray:
+        n = self.L.ref_grid_array(self.h, which, axis, None)
+        a = np.empty(n)
+        self.L.ref_grid_array(self.h, which, axis, a.ctypes.data)
+        return a
+
+    def _arr3(self, fn, *key):
+        shape = (ctypes.c_int64 * 3)()
+        self._check(fn(self.h, *key, shape, None))
+        a = np.empty(tuple(shape))
+        self._check(fn(self.h, *key, shape, a.ctypes.data))
+        return a
+
+    def field(self, name: str) -> np.ndarray:
+        return self._arr3(self.L.ref_field, FIELD_IDS[name])
+
+    def coeff(self, name: str) -> np.ndarray:
+        return self._arr3(self.L.ref_coeff, name.encode())
+
+    def material(self, kind: int, comp: int) -> np.ndarray:
+        return self._arr3(self.L.ref_material, kind, comp)
+
+    def pml_profile(self, b: int, which: int) -> np.ndarray:
+        n = self.L.ref_pml_profile(self.h, b, which, None)
+        a = np.empty(n)
+        self.L.ref_pml_profile(self.h, b, which, a.ctypes.data)
+        return a
+
+    def tfsf_table(self, i: int, which: int) -> np.ndarray:
+        shape = (ctypes.c_int64 * 3)()
+        self._check(self.L.ref_tfsf_table(self.h, i, which, shape, None))
+        a = np.empty((shape[0], shape[1]))
+        self._check(self.L.ref_tfsf_table(self.h, i, which, shape, a.ctypes.data))
+        return a
+
+    def monitor_records(self, m: int) -> List[np.ndarray]:
+        out = []
+        for r in range(self.L.ref_monitor_records(self.h, m)):
+            shape = (ctypes.c_int64 * 3)()
+            self._check(self.L.ref_monitor_record(self.h, m, r, shape, None))
+            a = np.empty(tuple(shape))
+            self._check(self.L.ref_monitor_record(self.h, m, r, shape, a.ctypes.data))
+            out.append(a[:, :, 0] if (shape[0] == 2 and shape[2] == 1) else a)
+        return out
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    den = np.linalg.norm(b.ravel())
+    num = np.linalg.norm((a - b).ravel())
+    return float(num / den) if den > 0 else float(num)

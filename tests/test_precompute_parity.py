@@ -1,0 +1,81 @@
+"""The product's FP64 host precompute equals the oracle's, bit for bit.
+
+No GPU: ``xfd_sim_precompute`` runs the host half of Simulation::init.  The
+oracle follows the reference's structure (full-grid staircase, per-corrector
+loops); the product reorganises the same arithmetic (bounding-box staircase,
+parallel sweeps, device-ready CPML/TFSF tables), so equality here checks that
+the reorganisation changed no value.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import RefSim
+from xfdtd_core_amd import native, scenarios
+from xfdtd_core_amd import spec as S
+
+COEFFS = native.COEFF_NAMES
+
+
+def _both(sim_spec, steps):
+    ref = RefSim(sim_spec)
+    ref.init(steps)
+    prod = native.Simulation(sim_spec)
+    prod.precompute(steps)
+    return ref, prod
+
+
+def _lumped_box():
+    """Small box with spheres, a cylinder, a resistor, a voltage source and a
+    current source on different axes, plus PML on all faces."""
+    dl = 1e-3
+    s = S.Simulation(dl, dl, dl, 0.95)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-0.012, -0.010, -0.011), S.Vector(0.024, 0.020, 0.022)),
+                         S.Material.createAir()))
+    s.addObject(S.Object("ball", S.Sphere(S.Vector(0.002, -0.001, 0.0015), 0.005),
+                         S.Material("m", S.ElectroMagneticProperty(4.0, 1.5, 0.02, 0.0))))
+    s.addObject(S.Object("rod", S.Cylinder(S.Vector(-0.004, 0.003, 0.0), 0.002, 0.01, "Y"),
+                         S.Material("r", S.ElectroMagneticProperty(2.5, 1.0, 0.0, 0.0))))
+    s.addObject(S.Resistor("r1", S.Cube(S.Vector(0.005, 0.005, -0.002), S.Vector(0.0, 0.0, 0.004)), "Z", 50.0))
+    s.addObject(S.VoltageSource("v1", S.Cube(S.Vector(-0.006, -0.005, 0.004), S.Vector(0.003, 0.0, 0.0)), "XP",
+                                50.0, S.Waveform.gaussian(2e-11, 9e-11, 1.0)))
+    s.addObject(S.CurrentSource("i1", S.Cube(S.Vector(0.0, -0.006, -0.005), S.Vector(0.001, 0.002, 0.0)), "YN",
+                                100.0, S.Waveform.sine(5e9, 0.5)))
+    for d, n in (("XN", 6), ("XP", 5), ("YN", 4), ("YP", 6), ("ZN", 5), ("ZP", 4)):
+        s.addBoundary(S.PML(n, d))
+    s.addWaveformSource(S.TFSF3D(8, 7, 7, 0.4, 0.7, 0.3, S.Waveform.gaussian(1.5e-11, 7e-11)))
+    return s
+
+
+CASES = {
+    "dielectric_sphere": lambda: scenarios.dielectric_sphere().spec(),
+    "inhomogeneous_box_small": lambda: scenarios.inhomogeneous_box(n_total=40, pml=6, n_spheres=5,
+                                                                   r_cells=(2, 6), seed=7).spec(),
+    "lumped_box": lambda: _lumped_box().spec(),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_coefficients_bitwise(case):
+    sp = CASES[case]()
+    ref, prod = _both(sp, 30)
+    assert ref.grid() == prod.grid()
+    assert ref.dt == prod.dt
+    for axis in range(3):
+        np.testing.assert_array_equal(ref.grid_array(0, axis), prod.grid_array(0, axis))
+        np.testing.assert_array_equal(ref.grid_array(3, axis), prod.grid_array(3, axis))
+    for name in COEFFS:
+        a, b = ref.coeff(name), prod.coeff(name)
+        assert a.shape == b.shape, name
+        bad = np.argwhere(a != b)
+        assert bad.size == 0, f"{name}: {len(bad)} cells differ, first {bad[:3].tolist()}"
+
+
+def test_readme_grid_matches_survey_numbers():
+    """README example: 63^3 cells and dt = 1.299937e-11 s (CFL 0.9, dl 7.5 mm)."""
+    ref = RefSim(scenarios.dielectric_sphere().spec())
+    ref.init(1)
+    assert ref.grid() == (63, 63, 63)
+    c0 = 1 / np.sqrt(8.854187817e-12 * 4 * np.pi * 1e-7)
+    dl = 7.5e-3
+    assert abs(ref.dt - 0.9 / (c0 * np.sqrt(3 / dl ** 2))) < 1e-24

@@ -1,0 +1,713 @@
+// C-ABI of the device library: owns device memory, turns the descriptor into
+// device-friendly tables and runs the step loop on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/xfdtd_core_amd.h"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct XfdError : std::runtime_error {
+  int code;
+  XfdError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define XFD_HIP(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess)                                                                   \
+      throw XfdError(e_ == hipErrorOutOfMemory ? XFD_ENOMEM : XFD_EDEVICE,                  \
+                     std::string(#x) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+void require(bool ok, const std::string& msg) {
+  if (!ok) throw XfdError(XFD_EINVAL, msg);
+}
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return XFD_OK;
+  } catch (const XfdError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return XFD_EINVAL;
+  }
+}
+
+// Reference extent of field `id` (src/simulation/simulation.cpp:514-528).
+void extentOf(int id, int nx, int ny, int nz, int out[3]) {
+  const int n[3] = {nx, ny, nz};
+  const int c = id % 3;
+  const bool h = id >= 3;
+  for (int a = 0; a < 3; ++a) out[a] = h ? (a == c ? n[a] + 1 : n[a]) : (a == c ? n[a] : n[a] + 1);
+}
+
+// coefficient slot -> field id it multiplies (self) for extent purposes
+int coefField(int slot) { return slot / 3; }
+
+enum KernelClass { K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_COUNT };
+const char* kKernelNames[K_COUNT] = {"update_h", "update_e", "cpml_h", "cpml_e",
+                                     "records_h", "records_e", "gather"};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    bytes = b;
+    if (b == 0) return;
+    XFD_HIP(hipMalloc(&p, b));
+    XFD_HIP(hipMemset(p, 0, b));
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace
+
+struct xfd_ctx {
+  int dtype = XFD_F64;
+  xfd::Grid g{};
+  hipStream_t stream = nullptr;
+  size_t elem = 8;
+  int ichunk = 16;
+  DevBuf fields[6];
+  DevBuf coefs[18];
+
+  struct Slab {
+    int axis, n, e_start, h_start;
+    int lo_e[3], lo_h[3], ext[3] = {0, 0, 0}, ext_h[3] = {0, 0, 0};
+    int extA_e[2], extB_e[2], extA_h[2], extB_h[2];
+    DevBuf cpsi[4], psi[4];  // ea_hb, eb_ha, ha_eb, hb_ea (box layout)
+    DevBuf ae, be, ah, bh;
+  };
+  std::vector<std::unique_ptr<Slab>> slabs;
+
+  DevBuf pool;
+  int64_t pool_rows = 0;  // smallest table row count: steps must stay below it
+  DevBuf rec_e, seg_e, rec_h, seg_h;
+  int64_t nseg_e = 0, nseg_h = 0;
+
+  struct Gather {
+    xfd_gather_desc d;
+    xfd::GatherArgs a;
+    DevBuf out;
+  };
+  std::vector<std::unique_ptr<Gather>> gathers;
+
+  bool prof = false;
+  int64_t launches[K_COUNT] = {};
+  double total_ms[K_COUNT] = {};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  ~xfd_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& p : pending) {
+      event_pool.push_back(p.second.first);
+      event_pool.push_back(p.second.second);
+    }
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int64_t deviceBytes() const {
+    int64_t b = 0;
+    for (auto& f : fields) b += f.bytes;
+    for (auto& c : coefs) b += c.bytes;
+    for (auto& s : slabs) {
+      for (auto& x : s->cpsi) b += x.bytes;
+      for (auto& x : s->psi) b += x.bytes;
+      b += s->ae.bytes + s->be.bytes + s->ah.bytes + s->bh.bytes;
+    }
+    b += pool.bytes + rec_e.bytes + seg_e.bytes + rec_h.bytes + seg_h.bytes;
+    for (auto& gth : gathers) b += gth->out.bytes;
+    return b;
+  }
+
+  hipEvent_t takeEvent() {
+    if (!event_pool.empty()) {
+      auto e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    XFD_HIP(hipEventCreate(&e));
+    return e;
+  }
+
+  template <class F>
+  void timed(int cls, F&& launch) {
+    if (!prof) {
+      launch();
+      return;
+    }
+    hipEvent_t a = takeEvent(), b = takeEvent();
+    XFD_HIP(hipEventRecord(a, stream));
+    launch();
+    XFD_HIP(hipEventRecord(b, stream));
+    pending.push_back({cls, {a, b}});
+  }
+
+  void drainEvents() {
+    for (auto& p : pending) {
+      float ms = 0;
+      XFD_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+      launches[p.first] += 1;
+      total_ms[p.first] += ms;
+      event_pool.push_back(p.second.first);
+      event_pool.push_back(p.second.second);
+    }
+    pending.clear();
+  }
+};
+
+namespace {
+
+template <class T>
+xfd::FieldPtrs<T> fieldPtrs(xfd_ctx& c) {
+  xfd::FieldPtrs<T> F;
+  for (int i = 0; i < 6; ++i) F.f[i] = c.fields[i].as<T>();
+  return F;
+}
+
+template <class T>
+xfd::CoefPtrs<T> coefPtrs(xfd_ctx& c) {
+  xfd::CoefPtrs<T> C;
+  for (int i = 0; i < 18; ++i) C.c[i] = c.coefs[i].as<T>();
+  return C;
+}
+
+// Uploads a double array of the reference extent into a padded device array.
+template <class T>
+void uploadPadded(xfd_ctx& c, const double* host, const int ext[3], T* dst, DevBuf& staging) {
+  const size_t n = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  XFD_HIP(hipMemcpyAsync(staging.p, host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  xfd::Launch<T>::unpack(c.stream, c.g, staging.as<double>(), dst, ext[0], ext[1], ext[2]);
+  XFD_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <class T>
+void uploadVector(const std::vector<T>& v, DevBuf& b) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) XFD_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+int64_t padIndex(const xfd::Grid& g, int64_t i, int64_t j, int64_t k) { return (i * g.NY + j) * g.pz + k; }
+
+// Expands the TFSF corrector loops (tfsf_corrector.h:89-229, call order
+// tfsf_corrector.cpp:177-232) into per-cell records for one half-step.
+void tfsfRecords(const xfd::Grid& g, const xfd_tfsf_desc& d, int64_t e_off, int64_t h_off, bool is_h,
+                 std::vector<xfd::Record>& out) {
+  const int64_t G0[3] = {d.origin[0], d.origin[1], d.origin[2]};
+  const int64_t G1[3] = {d.origin[0] + d.size[0], d.origin[1] + d.size[1], d.origin[2] + d.size[2]};
+  const double* pint[3] = {d.px_int, d.py_int, d.pz_int};
+  const double* phalf[3] = {d.px_half, d.py_half, d.pz_half};
+  const int64_t nint[3] = {d.size[0] + 1, d.size[1] + 1, d.size[2] + 1};
+  const int64_t nhalf[3] = {d.size[0] + 2, d.size[1] + 2, d.size[2] + 2};
+
+  // incident component `comp` of E (inc_h = false) or H at global (i,j,k):
+  // projection sum, then truncating index (tfsf_corrector.cpp:16-122)
+  auto lookup = [&](bool inc_h, int comp, int64_t p[3], xfd::Record& r) {
+    double proj = 0;
+    for (int a = 0; a < 3; ++a) {
+      int64_t rel = p[a] - G0[a];
+      // E_comp uses the half projection on its own axis, H_comp on the two
+      // other axes; a half projection is indexed one past the integer one.
+      bool half = inc_h ? (a != comp) : (a == comp);
+      int64_t idx = half ? rel + 1 : rel;
+      require(idx >= 0 && idx < (half ? nhalf[a] : nint[a]), "TFSF projection index out of range");
+      double v = half ? phalf[a][idx] : pint[a][idx];
+      proj = a == 0 ? v : proj + v;
+    }
+    if (inc_h) proj = proj - 0.5;
+    require(proj >= 0, "TFSF projection is negative");
+    const uint64_t idx = static_cast<uint64_t>(proj);
+    const double w = proj - static_cast<double>(idx);
+    const int64_t cols = inc_h ? d.aux - 1 : d.aux;
+    require(static_cast<int64_t>(idx) + 1 < cols, "TFSF incident index exceeds the auxiliary line");
+    r.w = w;
+    r.omw = 1 - w;
+    r.tr = inc_h ? d.transform_h[comp] : d.transform_e[comp];
+    r.pool = (inc_h ? h_off : e_off) + static_cast<int64_t>(idx);
+    r.row = cols;
+    r.kind = 0;
+  };
+
+  struct Call {
+    int dir;  // 0..5 = XN XP YN YP ZN ZP
+    int comp;
+  };
+  static const Call e_calls[12] = {{0, 2}, {1, 2}, {2, 2}, {3, 2}, {4, 1}, {5, 1},
+                                   {0, 1}, {1, 1}, {4, 0}, {5, 0}, {2, 0}, {3, 0}};
+  static const Call h_calls[12] = {{0, 1}, {1, 1}, {2, 0}, {3, 0}, {4, 0}, {5, 0},
+                                   {0, 2}, {1, 2}, {2, 2}, {3, 2}, {4, 1}, {5, 1}};
+  const Call* calls = is_h ? h_calls : e_calls;
+  for (int ci = 0; ci < 12; ++ci) {
+    const int dir = calls[ci].dir, fcomp = calls[ci].comp;
+    const int xyz = dir / 2;
+    const bool neg = dir % 2 == 0;
+    const int da = (xyz + 1) % 3, db = (xyz + 2) % 3;
+    const int dual_comp = fcomp == da ? db : da;
+    // task in abc coordinates, extended on the node's far end
+    int64_t lo[3] = {G0[0], G0[1], G0[2]}, hi[3] = {G1[0], G1[1], G1[2]};
+    const int offset_a = ((!is_h && fcomp == db) || (is_h && fcomp == da)) ? 1 : 0;
+    if (offset_a == 1) hi[da] += 1;
+    else hi[db] += 1;
+    const int64_t cs = neg ? G0[xyz] : G1[xyz];
+    lo[xyz] = cs;
+    hi[xyz] = cs + 1;
+    const int flag = (is_h ? -1 : 1) * (fcomp == da ? -1 : 1) * (neg ? -1 : 1);
+    const double coef = (is_h ? d.cb[xyz] : d.ca[xyz]) * flag;
+    const int field = (is_h ? 3 : 0) + fcomp;
+    for (int64_t i = lo[0]; i < hi[0]; ++i)
+      for (int64_t j = lo[1]; j < hi[1]; ++j)
+        for (int64_t k = lo[2]; k < hi[2]; ++k) {
+          int64_t p[3] = {i, j, k};
+          int64_t tgt[3] = {i, j, k};
+          int64_t src[3] = {i, j, k};
+          if (neg) {
+            if (!is_h) src[xyz] -= 1;  // E: incident H one cell outside
+            else tgt[xyz] -= 1;        // H: the outside H is corrected
+          }
+          (void)p;
+          xfd::Record r{};
+          lookup(!is_h, dual_comp, src, r);
+          r.coef = coef;
+          r.field = field;
+          int ext[3];
+          extentOf(field, g.nx, g.ny, g.nz, ext);
+          require(tgt[0] >= 0 && tgt[1] >= 0 && tgt[2] >= 0 && tgt[0] < ext[0] && tgt[1] < ext[1] &&
+                      tgt[2] < ext[2],
+                  "TFSF box touches the grid edge");
+          r.target = padIndex(g, tgt[0], tgt[1], tgt[2]);
+          out.push_back(r);
+        }
+  }
+}
+
+// Orders records by target (stable, keeping the corrector order per cell)
+// and returns segment starts.
+void segment(std::vector<xfd::Record>& rec, std::vector<int64_t>& seg) {
+  std::stable_sort(rec.begin(), rec.end(), [](const xfd::Record& a, const xfd::Record& b) {
+    return a.field != b.field ? a.field < b.field : a.target < b.target;
+  });
+  seg.clear();
+  for (size_t r = 0; r < rec.size(); ++r)
+    if (r == 0 || rec[r].field != rec[r - 1].field || rec[r].target != rec[r - 1].target)
+      seg.push_back(static_cast<int64_t>(r));
+  seg.push_back(static_cast<int64_t>(rec.size()));
+}
+
+template <class T>
+void createTyped(xfd_ctx& c, const xfd_desc& d) {
+  const xfd::Grid& g = c.g;
+  for (auto& f : c.fields) f.alloc(g.total * sizeof(T));
+  // coefficients
+  size_t max_ext = 0;
+  for (int s = 0; s < 18; ++s) {
+    int ext[3];
+    extentOf(coefField(s), g.nx, g.ny, g.nz, ext);
+    max_ext = std::max(max_ext, static_cast<size_t>(ext[0]) * ext[1] * ext[2]);
+  }
+  DevBuf staging;
+  staging.alloc(max_ext * sizeof(double));
+  for (int s = 0; s < 18; ++s) {
+    require(d.coeff[s] != nullptr, "coefficient array missing");
+    int ext[3];
+    extentOf(coefField(s), g.nx, g.ny, g.nz, ext);
+    c.coefs[s].alloc(g.total * sizeof(T));
+    uploadPadded<T>(c, d.coeff[s], ext, c.coefs[s].as<T>(), staging);
+  }
+
+  // CPML slabs
+  const int N[3] = {g.nx, g.ny, g.nz};
+  for (int si = 0; si < d.n_cpml; ++si) {
+    const xfd_cpml_desc& p = d.cpml[si];
+    require(p.axis >= 0 && p.axis < 3 && p.n > 0, "bad CPML slab");
+    const int cax = p.axis, a = (cax + 1) % 3, b = (cax + 2) % 3;
+    require(p.e_start >= 0 && p.e_start + p.n <= N[cax] && p.h_start >= 0 && p.h_start + p.n <= N[cax],
+            "CPML slab outside the grid");
+    auto s = std::make_unique<xfd_ctx::Slab>();
+    s->axis = cax;
+    s->n = p.n;
+    s->e_start = p.e_start;
+    s->h_start = p.h_start;
+    for (int q = 0; q < 3; ++q) {
+      s->lo_e[q] = 0;
+      s->lo_h[q] = 0;
+      s->ext[q] = N[q] + 1;
+    }
+    s->lo_e[cax] = p.e_start;
+    s->lo_h[cax] = p.h_start;
+    s->ext[cax] = p.n;
+    // pml_corrector.h:219-243 / 257-282 / 316-339 / 353-376
+    s->extA_e[0] = N[a];
+    s->extA_e[1] = N[b] + 1;
+    s->extB_e[0] = N[a] + 1;
+    s->extB_e[1] = N[b];
+    s->extA_h[0] = N[a] + 1;
+    s->extA_h[1] = N[b];
+    s->extB_h[0] = N[a];
+    s->extB_h[1] = N[b] + 1;
+    const int64_t vol = static_cast<int64_t>(s->ext[0]) * s->ext[1] * s->ext[2];
+    const double* src[4] = {p.c_ea_psi_hb, p.c_eb_psi_ha, p.c_ha_psi_eb, p.c_hb_psi_ea};
+    // psi extents in (a, b) per array
+    const int pe[4][2] = {{N[a], N[b] + 1}, {N[a] + 1, N[b]}, {N[a] + 1, N[b]}, {N[a], N[b] + 1}};
+    for (int q = 0; q < 4; ++q) {
+      require(src[q] != nullptr, "CPML c_psi array missing");
+      std::vector<T> box(vol, T(0));
+      // reference psi arrays are xyz-ordered with the layer index on the
+      // main axis (pml.cpp:297-337)
+      int ex[3];
+      ex[cax] = p.n;
+      ex[a] = pe[q][0];
+      ex[b] = pe[q][1];
+      for (int64_t x = 0; x < ex[0]; ++x)
+        for (int64_t y = 0; y < ex[1]; ++y)
+          for (int64_t z = 0; z < ex[2]; ++z) {
+            const int64_t lin = (x * ex[1] + y) * ex[2] + z;
+            // box-local: the main axis starts at the slab start in both boxes
+            const int64_t t = (x * s->ext[1] + y) * s->ext[2] + z;
+            box[t] = static_cast<T>(src[q][lin]);
+          }
+      uploadVector(box, s->cpsi[q]);
+      s->psi[q].alloc(vol * sizeof(T));
+    }
+    auto up = [&](const double* v, DevBuf& b) {
+      require(v != nullptr, "CPML profile missing");
+      std::vector<T> t(p.n);
+      for (int q = 0; q < p.n; ++q) t[q] = static_cast<T>(v[q]);
+      uploadVector(t, b);
+    };
+    up(p.a_e, s->ae);
+    up(p.b_e, s->be);
+    up(p.a_h, s->ah);
+    up(p.b_h, s->bh);
+    c.slabs.push_back(std::move(s));
+  }
+
+  // table pool: lumped waveforms, then TFSF e_inc / h_inc
+  std::vector<double> pool;
+  std::vector<int64_t> lumped_off(d.n_lumped), e_off(d.n_tfsf), h_off(d.n_tfsf);
+  int64_t min_rows = INT64_MAX;
+  for (int li = 0; li < d.n_lumped; ++li) {
+    const auto& L = d.lumped[li];
+    require(L.waveform != nullptr && L.nt > 0, "lumped waveform missing");
+    lumped_off[li] = static_cast<int64_t>(pool.size());
+    pool.insert(pool.end(), L.waveform, L.waveform + L.nt);
+    min_rows = std::min(min_rows, L.nt);
+  }
+  for (int ti = 0; ti < d.n_tfsf; ++ti) {
+    const auto& Tf = d.tfsf[ti];
+    require(Tf.e_inc && Tf.h_inc && Tf.nt > 0 && Tf.aux > 1, "TFSF tables missing");
+    e_off[ti] = static_cast<int64_t>(pool.size());
+    pool.insert(pool.end(), Tf.e_inc, Tf.e_inc + Tf.nt * Tf.aux);
+    h_off[ti] = static_cast<int64_t>(pool.size());
+    pool.insert(pool.end(), Tf.h_inc, Tf.h_inc + Tf.nt * (Tf.aux - 1));
+    min_rows = std::min(min_rows, Tf.nt);
+  }
+  c.pool_rows = pool.empty() ? INT64_MAX : min_rows;
+  uploadVector(pool, c.pool);
+
+  std::vector<xfd::Record> re, rh;
+  for (int li = 0; li < d.n_lumped; ++li) {  // object correctors first
+    const auto& L = d.lumped[li];
+    require(L.component >= 0 && L.component < 3 && L.coff != nullptr, "bad lumped element");
+    int ext[3];
+    extentOf(L.component, g.nx, g.ny, g.nz, ext);
+    for (int q = 0; q < 3; ++q)
+      require(L.origin[q] >= 0 && L.size[q] >= 0 && L.origin[q] + L.size[q] <= ext[q],
+              "lumped element outside the field extent");
+    for (int64_t i = 0; i < L.size[0]; ++i)
+      for (int64_t j = 0; j < L.size[1]; ++j)
+        for (int64_t k = 0; k < L.size[2]; ++k) {
+          xfd::Record r{};
+          r.kind = 1;
+          r.field = L.component;
+          r.target = padIndex(g, L.origin[0] + i, L.origin[1] + j, L.origin[2] + k);
+          r.coef = L.coff[(i * L.size[1] + j) * L.size[2] + k];
+          r.pool = lumped_off[li];
+          r.row = 1;
+          re.push_back(r);
+        }
+  }
+  for (int ti = 0; ti < d.n_tfsf; ++ti) {
+    tfsfRecords(g, d.tfsf[ti], e_off[ti], h_off[ti], false, re);
+    tfsfRecords(g, d.tfsf[ti], e_off[ti], h_off[ti], true, rh);
+  }
+  std::vector<int64_t> se, sh;
+  segment(re, se);
+  segment(rh, sh);
+  uploadVector(re, c.rec_e);
+  uploadVector(se, c.seg_e);
+  uploadVector(rh, c.rec_h);
+  uploadVector(sh, c.seg_h);
+  c.nseg_e = re.empty() ? 0 : static_cast<int64_t>(se.size()) - 1;
+  c.nseg_h = rh.empty() ? 0 : static_cast<int64_t>(sh.size()) - 1;
+
+  for (int gi = 0; gi < d.n_gather; ++gi) {
+    const auto& G = d.gather[gi];
+    require(G.field >= 0 && G.field < 6 && G.interval > 0 && G.count >= 0, "bad recording");
+    int ext[3];
+    extentOf(G.field, g.nx, g.ny, g.nz, ext);
+    for (int q = 0; q < 3; ++q)
+      require(G.origin[q] >= 0 && G.origin[q] <= G.end[q] && G.end[q] <= ext[q], "recording box outside field");
+    auto gg = std::make_unique<xfd_ctx::Gather>();
+    gg->d = G;
+    gg->a.field = G.field;
+    for (int q = 0; q < 3; ++q) {
+      gg->a.o[q] = G.origin[q];
+      gg->a.e[q] = G.end[q];
+    }
+    gg->a.frame_elems = static_cast<int64_t>(G.end[0] - G.origin[0]) * (G.end[1] - G.origin[1]) *
+                        (G.end[2] - G.origin[2]);
+    gg->out.alloc(static_cast<size_t>(gg->a.frame_elems * G.count) * sizeof(double));
+    c.gathers.push_back(std::move(gg));
+  }
+  XFD_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <class T>
+xfd::SlabArgs<T> slabArgs(xfd_ctx::Slab& s, bool is_e) {
+  xfd::SlabArgs<T> a{};
+  const int c = s.axis, ta = (c + 1) % 3, tb = (c + 2) % 3;
+  a.axis = c;
+  a.n = s.n;
+  a.start = is_e ? s.e_start : s.h_start;
+  for (int q = 0; q < 3; ++q) {
+    a.lo[q] = is_e ? s.lo_e[q] : s.lo_h[q];
+    a.ext[q] = s.ext[q];
+  }
+  if (is_e) {
+    a.extA[0] = s.extA_e[0];
+    a.extA[1] = s.extA_e[1];
+    a.extB[0] = s.extB_e[0];
+    a.extB[1] = s.extB_e[1];
+    a.fieldA = ta;
+    a.fieldB = tb;
+    a.dualA = 3 + ta;
+    a.dualB = 3 + tb;
+    a.cpsiA = s.cpsi[0].as<T>();
+    a.cpsiB = s.cpsi[1].as<T>();
+    a.psiA = s.psi[0].as<T>();
+    a.psiB = s.psi[1].as<T>();
+    a.ca = s.ae.as<T>();
+    a.cb = s.be.as<T>();
+  } else {
+    a.extA[0] = s.extA_h[0];
+    a.extA[1] = s.extA_h[1];
+    a.extB[0] = s.extB_h[0];
+    a.extB[1] = s.extB_h[1];
+    a.fieldA = 3 + ta;
+    a.fieldB = 3 + tb;
+    a.dualA = ta;
+    a.dualB = tb;
+    a.cpsiA = s.cpsi[2].as<T>();
+    a.cpsiB = s.cpsi[3].as<T>();
+    a.psiA = s.psi[2].as<T>();
+    a.psiB = s.psi[3].as<T>();
+    a.ca = s.ah.as<T>();
+    a.cb = s.bh.as<T>();
+  }
+  return a;
+}
+
+template <class T>
+void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
+  require(t0 >= 0 && n >= 0, "bad step range");
+  require(t0 + n <= c.pool_rows, "step beyond the source tables (run length given at init)");
+  const auto F = fieldPtrs<T>(c);
+  const auto C = coefPtrs<T>(c);
+  std::vector<xfd::SlabArgs<T>> se, sh;
+  for (auto& s : c.slabs) {
+    se.push_back(slabArgs<T>(*s, true));
+    sh.push_back(slabArgs<T>(*s, false));
+  }
+  const int64_t stride[3] = {c.g.plane, c.g.pz, 1};
+  for (auto& a : se) a.stride_c = stride[a.axis];
+  for (auto& a : sh) a.stride_c = stride[a.axis];
+  hipStream_t st = c.stream;
+  for (int64_t t = t0; t < t0 + n; ++t) {
+    c.timed(K_UPDATE_H, [&] { xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk); });
+    if (c.nseg_h)
+      c.timed(K_REC_H, [&] {
+        xfd::Launch<T>::records(st, F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h,
+                                c.pool.as<double>(), t);
+      });
+    for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, false); });
+    c.timed(K_UPDATE_E, [&] { xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk); });
+    if (c.nseg_e)
+      c.timed(K_REC_E, [&] {
+        xfd::Launch<T>::records(st, F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e,
+                                c.pool.as<double>(), t);
+      });
+    for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
+    for (auto& gp : c.gathers) {
+      const auto& d = gp->d;
+      if (t < d.first || (t - d.first) % d.interval != 0) continue;
+      const int64_t frame = (t - d.first) / d.interval;
+      if (frame >= d.count) continue;
+      c.timed(K_GATHER, [&] {
+        xfd::Launch<T>::gather(st, c.g, F, gp->a, gp->out.as<double>() + frame * gp->a.frame_elems);
+      });
+    }
+    XFD_HIP(hipGetLastError());
+    if (c.prof && c.pending.size() > 4096) {
+      XFD_HIP(hipStreamSynchronize(st));
+      c.drainEvents();
+    }
+  }
+  XFD_HIP(hipStreamSynchronize(st));
+  if (c.prof) c.drainEvents();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xfd_last_error(void) { return g_err.c_str(); }
+
+const char* xfd_version(void) { return "xfdtd_core_amd 0.1 (gfx950)"; }
+
+int xfd_device_count(int32_t* count) {
+  return guard([&] {
+    int n = 0;
+    XFD_HIP(hipGetDeviceCount(&n));
+    *count = n;
+  });
+}
+
+int xfd_create(const xfd_desc* d, xfd_ctx** out) {
+  return guard([&] {
+    require(d != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    require(d->nx > 1 && d->ny > 1 && d->nz > 1, "grid too small");
+    require(d->dtype == XFD_F64 || d->dtype == XFD_F32, "bad dtype");
+    require(d->n_cpml >= 0 && d->n_lumped >= 0 && d->n_tfsf >= 0 && d->n_gather >= 0, "bad counts");
+    auto c = std::make_unique<xfd_ctx>();
+    c->dtype = d->dtype;
+    c->elem = d->dtype == XFD_F64 ? 8 : 4;
+    xfd::Grid& g = c->g;
+    g.nx = d->nx;
+    g.ny = d->ny;
+    g.nz = d->nz;
+    g.NY = d->ny + 1;
+    const int64_t line = 128 / static_cast<int64_t>(c->elem);
+    g.pz = ((d->nz + 1) + line - 1) / line * line;
+    g.plane = static_cast<int64_t>(g.NY) * g.pz;
+    g.total = static_cast<int64_t>(d->nx + 1) * g.plane;
+    const int64_t bxy = ((g.nz + xfd::kLanesK - 1) / xfd::kLanesK) * ((g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ);
+    c->ichunk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(16, g.nx * bxy / 2048)));
+    XFD_HIP(hipStreamCreate(&c->stream));
+    if (c->dtype == XFD_F64)
+      createTyped<double>(*c, *d);
+    else
+      createTyped<float>(*c, *d);
+    *out = c.release();
+  });
+}
+
+void xfd_destroy(xfd_ctx* c) { delete c; }
+
+int xfd_step(xfd_ctx* c, int64_t t0, int64_t n) {
+  return guard([&] {
+    require(c != nullptr, "null context");
+    if (c->dtype == XFD_F64)
+      stepTyped<double>(*c, t0, n);
+    else
+      stepTyped<float>(*c, t0, n);
+  });
+}
+
+int xfd_read_field(xfd_ctx* c, int32_t field, double* dst) {
+  return guard([&] {
+    require(c && dst && field >= 0 && field < 6, "bad argument");
+    int ext[3];
+    extentOf(field, c->g.nx, c->g.ny, c->g.nz, ext);
+    const size_t n = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+    DevBuf tmp;
+    tmp.alloc(n * sizeof(double));
+    if (c->dtype == XFD_F64)
+      xfd::Launch<double>::pack(c->stream, c->g, c->fields[field].as<double>(), tmp.as<double>(), ext[0], ext[1],
+                                ext[2]);
+    else
+      xfd::Launch<float>::pack(c->stream, c->g, c->fields[field].as<float>(), tmp.as<double>(), ext[0], ext[1],
+                               ext[2]);
+    XFD_HIP(hipMemcpyAsync(dst, tmp.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XFD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfd_write_field(xfd_ctx* c, int32_t field, const double* src) {
+  return guard([&] {
+    require(c && src && field >= 0 && field < 6, "bad argument");
+    int ext[3];
+    extentOf(field, c->g.nx, c->g.ny, c->g.nz, ext);
+    DevBuf tmp;
+    tmp.alloc(static_cast<size_t>(ext[0]) * ext[1] * ext[2] * sizeof(double));
+    if (c->dtype == XFD_F64)
+      uploadPadded<double>(*c, src, ext, c->fields[field].as<double>(), tmp);
+    else
+      uploadPadded<float>(*c, src, ext, c->fields[field].as<float>(), tmp);
+  });
+}
+
+int xfd_read_gather(xfd_ctx* c, int32_t gi, double* dst) {
+  return guard([&] {
+    require(c && dst && gi >= 0 && gi < static_cast<int>(c->gathers.size()), "bad argument");
+    auto& g = *c->gathers[gi];
+    if (g.out.bytes)
+      XFD_HIP(hipMemcpyAsync(dst, g.out.p, g.out.bytes, hipMemcpyDeviceToHost, c->stream));
+    XFD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfd_set_profiling(xfd_ctx* c, int32_t on) {
+  return guard([&] {
+    require(c != nullptr, "null context");
+    c->prof = on != 0;
+    for (int k = 0; k < K_COUNT; ++k) {
+      c->launches[k] = 0;
+      c->total_ms[k] = 0;
+    }
+  });
+}
+
+int xfd_kernel_stats(xfd_ctx* c, int32_t k, const char** name, int64_t* launches, double* total_ms) {
+  return guard([&] {
+    require(c != nullptr && k >= 0 && k < K_COUNT, "bad kernel class");
+    if (name) *name = kKernelNames[k];
+    if (launches) *launches = c->launches[k];
+    if (total_ms) *total_ms = c->total_ms[k];
+  });
+}
+
+int xfd_device_bytes(xfd_ctx* c, int64_t* bytes) {
+  return guard([&] {
+    require(c && bytes, "bad argument");
+    *bytes = c->deviceBytes();
+  });
+}
+
+}  // extern "C"

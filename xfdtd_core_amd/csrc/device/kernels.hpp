@@ -1,0 +1,210 @@
+// Hot-path kernels for gfx950, templated on the field precision T.
+//
+// Every kernel here is an HBM-bound stencil or stream (about 0.1 flop per
+// byte), so the design goal is the fewest bytes per cell, not MFMA: lanes run
+// along k (the contiguous axis) so a 64-wide wavefront issues whole 128-byte
+// lines, a workgroup is 64 (k) x 4 (j), and each thread marches a chunk of
+// i-planes carrying the i-1 / i+1 neighbour of the previous plane in
+// registers, so the only re-reads are the j +/- 1 and k +/- 1 neighbours,
+// which hit in L1/L2.
+//
+// Arithmetic is written in the reference's evaluation order
+// (src/include/updator/update_scheme.h:26-62: c1*f + c2*(p-q) + c3*(p-q),
+// left to right).  The FP64 translation unit is compiled with
+// -ffp-contract=off, which makes its results bitwise equal to the
+// reference's CPU path; the FP32 unit lets the compiler fuse.
+#pragma once
+
+#include "layout.hpp"
+
+namespace xfd {
+
+constexpr int kLanesK = 64;
+constexpr int kRowsJ = 4;
+
+// BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41): Hx, Hy, Hz
+// over [0,nx) x [0,ny) x [0,nz).
+template <class T>
+__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, CoefPtrs<T> C, int ichunk) {
+  const int k = blockIdx.x * kLanesK + threadIdx.x;
+  const int j = blockIdx.y * kRowsJ + threadIdx.y;
+  const int i0 = blockIdx.z * ichunk;
+  if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
+  const int i1 = min(i0 + ichunk, g.nx);
+  const int64_t pz = g.pz, pl = g.plane;
+  int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  T* __restrict__ ex = F.f[0];
+  T* __restrict__ ey = F.f[1];
+  T* __restrict__ ez = F.f[2];
+  T* __restrict__ hx = F.f[3];
+  T* __restrict__ hy = F.f[4];
+  T* __restrict__ hz = F.f[5];
+  T ey_i = ey[idx];
+  T ez_i = ez[idx];
+  for (int i = i0; i < i1; ++i, idx += pl) {
+    const T ex0 = ex[idx], ex_j = ex[idx + pz], ex_k = ex[idx + 1];
+    const T ey_k = ey[idx + 1], ey_n = ey[idx + pl];
+    const T ez_j = ez[idx + pz], ez_n = ez[idx + pl];
+    hx[idx] = C.c[9][idx] * hx[idx] + C.c[10][idx] * (ey_k - ey_i) + C.c[11][idx] * (ez_j - ez_i);
+    hy[idx] = C.c[12][idx] * hy[idx] + C.c[13][idx] * (ez_n - ez_i) + C.c[14][idx] * (ex_k - ex0);
+    hz[idx] = C.c[15][idx] * hz[idx] + C.c[16][idx] * (ex_j - ex0) + C.c[17][idx] * (ey_n - ey_i);
+    ey_i = ey_n;
+    ez_i = ez_n;
+  }
+}
+
+// BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
+// Ex over [0,nx) x [1,ny) x [1,nz), Ey over [1,nx) x [0,ny) x [1,nz),
+// Ez over [1,nx) x [1,ny) x [0,nz).
+template <class T>
+__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, CoefPtrs<T> C, int ichunk) {
+  const int k = blockIdx.x * kLanesK + threadIdx.x;
+  const int j = blockIdx.y * kRowsJ + threadIdx.y;
+  const int i0 = blockIdx.z * ichunk;
+  if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
+  const int i1 = min(i0 + ichunk, g.nx);
+  const int64_t pz = g.pz, pl = g.plane;
+  int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  T* __restrict__ ex = F.f[0];
+  T* __restrict__ ey = F.f[1];
+  T* __restrict__ ez = F.f[2];
+  const T* __restrict__ hx = F.f[3];
+  const T* __restrict__ hy = F.f[4];
+  const T* __restrict__ hz = F.f[5];
+  T hz_p = i0 > 0 ? hz[idx - pl] : T(0);
+  T hy_p = i0 > 0 ? hy[idx - pl] : T(0);
+  const bool jk = j >= 1 && k >= 1;
+  for (int i = i0; i < i1; ++i, idx += pl) {
+    const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
+    if (jk)
+      ex[idx] = C.c[0][idx] * ex[idx] + C.c[1][idx] * (hy0 - hy[idx - 1]) +
+                C.c[2][idx] * (hz0 - hz[idx - pz]);
+    if (i >= 1 && k >= 1)
+      ey[idx] = C.c[3][idx] * ey[idx] + C.c[4][idx] * (hz0 - hz_p) +
+                C.c[5][idx] * (hx0 - hx[idx - 1]);
+    if (i >= 1 && j >= 1)
+      ez[idx] = C.c[6][idx] * ez[idx] + C.c[7][idx] * (hx0 - hx[idx - pz]) +
+                C.c[8][idx] * (hy0 - hy_p);
+    hz_p = hz0;
+    hy_p = hy0;
+  }
+}
+
+// correctPML (src/include/boundary/pml_corrector.h:14-52) for both
+// components of one slab and one half-step.  IsE: psi = b psi + a (p - q)
+// with q one plane below along the main axis; H: psi = b psi + a (q - p)
+// with q one plane above.  field += c_psi * psi.
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T> s) {
+  const int64_t vol = static_cast<int64_t>(s.ext[0]) * s.ext[1] * s.ext[2];
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= vol) return;
+  const int w = static_cast<int>(t % s.ext[2]);
+  const int v = static_cast<int>((t / s.ext[2]) % s.ext[1]);
+  const int u = static_cast<int>(t / (static_cast<int64_t>(s.ext[2]) * s.ext[1]));
+  const int p[3] = {s.lo[0] + u, s.lo[1] + v, s.lo[2] + w};
+  const int c = s.axis, a = (c + 1) % 3, b = (c + 2) % 3;
+  const int layer = p[c] - s.start;
+  const T ca = s.ca[layer], cb = s.cb[layer];
+  const int64_t idx = (static_cast<int64_t>(p[0]) * g.NY + p[1]) * g.pz + p[2];
+  const int64_t q = IsE ? idx - s.stride_c : idx + s.stride_c;
+  if (p[a] < s.extA[0] && p[b] < s.extA[1]) {
+    const T* dual = F.f[s.dualB];
+    T psi = s.psiA[t];
+    psi = IsE ? cb * psi + ca * (dual[idx] - dual[q]) : cb * psi + ca * (dual[q] - dual[idx]);
+    s.psiA[t] = psi;
+    T* f = F.f[s.fieldA];
+    f[idx] += s.cpsiA[t] * psi;
+  }
+  if (p[a] < s.extB[0] && p[b] < s.extB[1]) {
+    const T* dual = F.f[s.dualA];
+    T psi = s.psiB[t];
+    psi = IsE ? cb * psi + ca * (dual[idx] - dual[q]) : cb * psi + ca * (dual[q] - dual[idx]);
+    s.psiB[t] = psi;
+    T* f = F.f[s.fieldB];
+    f[idx] += s.cpsiB[t] * psi;
+  }
+}
+
+// Lumped-source and TFSF corrections: one thread per corrected cell applies
+// that cell's records in the reference's corrector order.
+template <class T>
+__global__ __launch_bounds__(256) void k_records(FieldPtrs<T> F, const Record* __restrict__ rec,
+                                                 const int64_t* __restrict__ seg, int64_t nseg,
+                                                 const double* __restrict__ pool, int64_t t) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int64_t r0 = seg[s], r1 = seg[s + 1];
+  T* f = F.f[rec[r0].field];
+  const int64_t target = rec[r0].target;
+  double acc = static_cast<double>(f[target]);
+  for (int64_t r = r0; r < r1; ++r) {
+    const Record R = rec[r];
+    const double* row = pool + R.pool + t * R.row;
+    if (R.kind == 0) {
+      // TFSFCorrector::exInc..hzInc (tfsf_corrector.cpp:16-122) times the
+      // signed coefficient (tfsf_corrector.h:216-225)
+      const double inc = R.tr * (R.omw * row[0] + R.w * row[1]);
+      acc = acc + inc * R.coef;
+    } else {
+      // Voltage/CurrentSourceCorrector: E += coff * w[t]
+      acc = acc + R.coef * row[0];
+    }
+    if (sizeof(T) == sizeof(float)) acc = static_cast<double>(static_cast<T>(acc));
+  }
+  f[target] = static_cast<T>(acc);
+}
+
+// Recording: copy the box [o, e) of one field into a double frame.
+template <class T>
+__global__ __launch_bounds__(256) void k_gather(Grid g, FieldPtrs<T> F, GatherArgs a, double* out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= a.frame_elems) return;
+  const int n2 = a.e[2] - a.o[2], n1 = a.e[1] - a.o[1];
+  const int w = static_cast<int>(t % n2);
+  const int v = static_cast<int>((t / n2) % n1);
+  const int u = static_cast<int>(t / (static_cast<int64_t>(n2) * n1));
+  const int64_t idx = (static_cast<int64_t>(a.o[0] + u) * g.NY + (a.o[1] + v)) * g.pz + (a.o[2] + w);
+  out[t] = static_cast<double>(F.f[a.field][idx]);
+}
+
+// Dense copy between the reference extent (k fastest, n0 x n1 x n2 doubles)
+// and the padded layout, converting precision.
+template <class T>
+__global__ __launch_bounds__(256) void k_unpack(Grid g, const double* __restrict__ src, T* dst, int n0, int n1,
+                                                int n2) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t vol = static_cast<int64_t>(n0) * n1 * n2;
+  if (t >= vol) return;
+  const int w = static_cast<int>(t % n2);
+  const int v = static_cast<int>((t / n2) % n1);
+  const int u = static_cast<int>(t / (static_cast<int64_t>(n2) * n1));
+  dst[(static_cast<int64_t>(u) * g.NY + v) * g.pz + w] = static_cast<T>(src[t]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pack(Grid g, const T* __restrict__ src, double* dst, int n0, int n1,
+                                              int n2) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t vol = static_cast<int64_t>(n0) * n1 * n2;
+  if (t >= vol) return;
+  const int w = static_cast<int>(t % n2);
+  const int v = static_cast<int>((t / n2) % n1);
+  const int u = static_cast<int>(t / (static_cast<int64_t>(n2) * n1));
+  dst[t] = static_cast<double>(src[(static_cast<int64_t>(u) * g.NY + v) * g.pz + w]);
+}
+
+// Host-side launchers, instantiated once per precision.
+template <class T>
+struct Launch {
+  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
+  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
+  static void cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
+  static void records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
+                      const double* pool, int64_t t);
+  static void gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out);
+  static void unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2);
+  static void pack(hipStream_t s, const Grid& g, const T* src, double* dst, int n0, int n1, int n2);
+};
+
+}  // namespace xfd

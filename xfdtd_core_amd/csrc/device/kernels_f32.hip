@@ -1,0 +1,5 @@
+// FP32 instantiation (fields, psi and coefficients in float; the TFSF and
+// lumped corrections are evaluated in double and rounded once per record).
+#include "launch_impl.hpp"
+
+template struct xfd::Launch<float>;

@@ -1,0 +1,63 @@
+// Launcher definitions; included by exactly one translation unit per
+// precision (kernels_f64.hip, kernels_f32.hip), which differ only in their
+// floating-point contraction flags.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace xfd {
+
+inline unsigned blocks_for(int64_t n, int per) { return static_cast<unsigned>((n + per - 1) / per); }
+
+template <class T>
+void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
+  dim3 block(kLanesK, kRowsJ);
+  dim3 grid(blocks_for(g.nz, kLanesK), blocks_for(g.ny, kRowsJ), blocks_for(g.nx, ichunk));
+  hipLaunchKernelGGL(k_update_h<T>, grid, block, 0, s, g, F, C, ichunk);
+}
+
+template <class T>
+void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
+  dim3 block(kLanesK, kRowsJ);
+  dim3 grid(blocks_for(g.nz, kLanesK), blocks_for(g.ny, kRowsJ), blocks_for(g.nx, ichunk));
+  hipLaunchKernelGGL(k_update_e<T>, grid, block, 0, s, g, F, C, ichunk);
+}
+
+template <class T>
+void Launch<T>::cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e) {
+  const int64_t vol = static_cast<int64_t>(a.ext[0]) * a.ext[1] * a.ext[2];
+  if (vol <= 0) return;
+  if (is_e)
+    hipLaunchKernelGGL((k_cpml<T, true>), dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, F, a);
+  else
+    hipLaunchKernelGGL((k_cpml<T, false>), dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, F, a);
+}
+
+template <class T>
+void Launch<T>::records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
+                        const double* pool, int64_t t) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_records<T>, dim3(blocks_for(nseg, 256)), dim3(256), 0, s, F, rec, seg, nseg, pool, t);
+}
+
+template <class T>
+void Launch<T>::gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out) {
+  if (a.frame_elems <= 0) return;
+  hipLaunchKernelGGL(k_gather<T>, dim3(blocks_for(a.frame_elems, 256)), dim3(256), 0, s, g, F, a, out);
+}
+
+template <class T>
+void Launch<T>::unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2) {
+  const int64_t vol = static_cast<int64_t>(n0) * n1 * n2;
+  if (vol <= 0) return;
+  hipLaunchKernelGGL(k_unpack<T>, dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, src, dst, n0, n1, n2);
+}
+
+template <class T>
+void Launch<T>::pack(hipStream_t s, const Grid& g, const T* src, double* dst, int n0, int n1, int n2) {
+  const int64_t vol = static_cast<int64_t>(n0) * n1 * n2;
+  if (vol <= 0) return;
+  hipLaunchKernelGGL(k_pack<T>, dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, src, dst, n0, n1, n2);
+}
+
+}  // namespace xfd

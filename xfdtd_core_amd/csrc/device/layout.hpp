@@ -1,0 +1,76 @@
+// Device-side data layout shared by the kernels and the C-ABI.
+//
+// All six field arrays, the 18 coefficient arrays and the recordings' source
+// fields live in one padded 3D layout of (nx+1) x (ny+1) x pz elements with k
+// fastest: element (i, j, k) of every component is at (i*(ny+1) + j)*pz + k.
+// pz rounds nz+1 up to a whole number of 128-byte lines, so every k-row
+// starts on a line boundary and a 64-lane wavefront reading 64 consecutive k
+// touches the minimum number of lines.  A component's own extent (e.g. Ex is
+// nx x (ny+1) x (nz+1)) is a sub-box of the common one; the padding cells are
+// zero and never written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace xfd {
+
+struct Grid {
+  int nx, ny, nz;
+  int NY;          // ny + 1
+  int64_t pz;      // padded k pitch (elements)
+  int64_t plane;   // NY * pz
+  int64_t total;   // (nx + 1) * plane
+};
+
+template <class T>
+struct FieldPtrs {
+  T* f[6];  // Ex Ey Ez Hx Hy Hz
+};
+
+template <class T>
+struct CoefPtrs {
+  const T* c[18];  // cexe cexhy cexhz ceye ceyhz ceyhx ceze cezhx cezhy chxh ... chzey
+};
+
+// One CPML slab for one half-step.  The thread space is the union box of the
+// two corrected components; `ext_a` / `ext_b` give each component's extent
+// along the tangential axes a and b (the other component's is one larger).
+template <class T>
+struct SlabArgs {
+  int axis;          // main axis c
+  int n;             // layers
+  int start;         // first corrected plane along c
+  int lo[3], ext[3]; // union box in grid indices
+  int extA[2];       // component A extent along (a, b)
+  int extB[2];       // component B extent along (a, b)
+  int fieldA, fieldB, dualA, dualB;  // field ids (0..5)
+  int64_t stride_c;  // padded-layout stride of the main axis
+  const T* cpsiA;    // box-local layout, k fastest
+  const T* cpsiB;
+  T* psiA;
+  T* psiB;
+  const T* ca;       // per-layer recursive-convolution coefficients
+  const T* cb;
+};
+
+// Additive correction applied after a Yee half-step, in the reference's
+// corrector order (objects, then TFSF; src/simulation/simulation.cpp:437-487).
+struct Record {
+  int64_t target;     // padded linear index
+  int64_t pool;       // table offset of column j at t = 0
+  int64_t row;        // table row stride
+  double w, omw;      // interpolation weights (kind 0)
+  double tr, coef;    // transform and coefficient (kind 0); coef (kind 1)
+  int32_t field;      // 0..5
+  int32_t kind;       // 0: TFSF interpolated incident field, 1: waveform * coef
+};
+
+struct GatherArgs {
+  int field;
+  int o[3], e[3];
+  int64_t frame_elems;
+};
+
+}  // namespace xfd

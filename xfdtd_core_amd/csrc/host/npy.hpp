@@ -1,0 +1,35 @@
+// Minimal .npy (version 1.0, little-endian f8, C order) writer: the format
+// xt::dump_npy produces for the reference's monitor outputs
+// (src/monitor/monitor.cpp:42-54).
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace npy {
+
+inline void write(const std::string& path, const std::vector<std::size_t>& shape, const double* data) {
+  std::string sh = "(";
+  for (std::size_t i = 0; i < shape.size(); ++i) sh += std::to_string(shape[i]) + (shape.size() == 1 || i + 1 < shape.size() ? ", " : "");
+  sh += ")";
+  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': " + sh + ", }";
+  const std::size_t unpadded = 10 + hdr.size() + 1;
+  hdr += std::string((64 - unpadded % 64) % 64, ' ');
+  hdr += '\n';
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  std::fwrite(magic, 1, 8, f);
+  const uint16_t len = static_cast<uint16_t>(hdr.size());
+  std::fwrite(&len, sizeof(len), 1, f);
+  std::fwrite(hdr.data(), 1, hdr.size(), f);
+  std::size_t n = 1;
+  for (auto s : shape) n *= s;
+  if (n) std::fwrite(data, sizeof(double), n, f);
+  std::fclose(f);
+}
+
+}  // namespace npy

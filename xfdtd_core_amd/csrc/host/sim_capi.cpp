@@ -1,0 +1,345 @@
+// Scenario-level C-ABI over the C++ Simulation API, for callers without a
+// C++ toolchain (the Python package, ctypes).  A scenario arrives as the spec
+// text written by xfdtd_core_amd/spec.py; each directive is turned into the
+// C++ object the reference's own examples would construct.
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+#include "../../../include/xfdtd_core_amd.h"
+#include "../../../include/xfdtd_core_amd.hpp"
+
+using namespace xfdtd;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct SimHandle {
+  std::unique_ptr<Simulation> sim;
+  std::vector<std::shared_ptr<Monitor>> monitors;
+  Precision precision = Precision::FP64;
+};
+
+Axis::Direction parseDir(const std::string& s) {
+  static const char* n[6] = {"XN", "XP", "YN", "YP", "ZN", "ZP"};
+  for (int i = 0; i < 6; ++i)
+    if (s == n[i]) return static_cast<Axis::Direction>(i);
+  throw XFDTDException("bad direction " + s);
+}
+Axis::XYZ parseAxis(const std::string& s) {
+  if (s == "X") return Axis::XYZ::X;
+  if (s == "Y") return Axis::XYZ::Y;
+  if (s == "Z") return Axis::XYZ::Z;
+  throw XFDTDException("bad axis " + s);
+}
+EMF::Field parseField(const std::string& s) {
+  static const char* n[6] = {"EX", "EY", "EZ", "HX", "HY", "HZ"};
+  static const EMF::Field f[6] = {EMF::Field::EX, EMF::Field::EY, EMF::Field::EZ,
+                                  EMF::Field::HX, EMF::Field::HY, EMF::Field::HZ};
+  for (int i = 0; i < 6; ++i)
+    if (s == n[i]) return f[i];
+  throw XFDTDException("bad field " + s);
+}
+
+struct Tokens {
+  std::istringstream is;
+  explicit Tokens(const std::string& line) : is(line) {}
+  std::string word() {
+    std::string t;
+    if (!(is >> t)) throw XFDTDException("spec: missing token");
+    return t;
+  }
+  Real num() { return std::strtod(word().c_str(), nullptr); }
+  bool more() {
+    is >> std::ws;
+    return !is.eof();
+  }
+};
+
+std::unique_ptr<Waveform> waveform(Tokens& t) {
+  const std::string kind = t.word();
+  if (kind == "gaussian") {
+    Real tau = t.num(), t0 = t.num();
+    return Waveform::gaussian(tau, t0, t.num());
+  }
+  if (kind == "cosine_modulated_gaussian") {
+    Real tau = t.num(), t0 = t.num(), f = t.num();
+    return Waveform::cosineModulatedGaussian(tau, t0, f, t.num());
+  }
+  if (kind == "sine") {
+    Real f = t.num();
+    return Waveform::sine(f, t.num());
+  }
+  if (kind == "cosine") {
+    Real f = t.num();
+    return Waveform::cosine(f, t.num());
+  }
+  if (kind == "step") {
+    Real t0 = t.num();
+    return Waveform::step(t0, t.num());
+  }
+  if (kind == "gaussian_derivative") {
+    // a user functor (include/xfdtd/waveform/waveform.h:13)
+    Real tau = t.num(), t0 = t.num();
+    return std::make_unique<Waveform>(
+        [tau, t0](Real x) {
+          const Real u = (x - t0) / tau;
+          return -u * std::exp(-(u * u));
+        },
+        t.num());
+  }
+  throw XFDTDException("unknown waveform " + kind);
+}
+
+std::unique_ptr<Cube> cube(Tokens& t) {
+  Real o[3], s[3];
+  for (auto& v : o) v = t.num();
+  for (auto& v : s) v = t.num();
+  return std::make_unique<Cube>(Vector(o[0], o[1], o[2]), Vector(s[0], s[1], s[2]));
+}
+
+std::unique_ptr<Shape> shape(Tokens& t) {
+  const std::string kind = t.word();
+  if (kind == "cube") return cube(t);
+  if (kind == "sphere") {
+    Real c[3];
+    for (auto& v : c) v = t.num();
+    return std::make_unique<Sphere>(Vector(c[0], c[1], c[2]), t.num());
+  }
+  if (kind == "cylinder") {
+    Real c[3];
+    for (auto& v : c) v = t.num();
+    Real r = t.num(), h = t.num();
+    return std::make_unique<Cylinder>(Vector(c[0], c[1], c[2]), r, h, parseAxis(t.word()));
+  }
+  throw XFDTDException("unknown shape " + kind);
+}
+
+std::unique_ptr<Material> material(Tokens& t, const std::string& name) {
+  const std::string kind = t.word();
+  if (kind == "plain") {
+    Real er = t.num(), mr = t.num(), se = t.num(), sm = t.num();
+    return std::make_unique<Material>(name, ElectroMagneticProperty(er, mr, se, sm));
+  }
+  throw XFDTDException("unknown material kind " + kind);
+}
+
+std::unique_ptr<SimHandle> build(const std::string& spec) {
+  auto h = std::make_unique<SimHandle>();
+  std::istringstream all(spec);
+  std::string line;
+  while (std::getline(all, line)) {
+    auto c = line.find('#');
+    if (c != std::string::npos) line.resize(c);
+    Tokens t(line);
+    if (!t.more()) continue;
+    const std::string d = t.word();
+    if (d == "sim") {
+      Real dx = t.num(), dy = t.num(), dz = t.num(), cfl = t.num();
+      h->sim = std::make_unique<Simulation>(dx, dy, dz, cfl);
+      continue;
+    }
+    if (!h->sim) throw XFDTDException("spec must start with 'sim'");
+    Simulation& s = *h->sim;
+    if (d == "threads") {
+      t.num();
+      t.num();
+      t.num();  // ThreadConfig: accepted, the GPU path is one domain
+    } else if (d == "object") {
+      std::string name = t.word();
+      auto sh = shape(t);
+      s.addObject(std::make_shared<Object>(name, std::move(sh), material(t, name)));
+    } else if (d == "pml") {
+      int n = static_cast<int>(t.num());
+      auto dir = parseDir(t.word());
+      if (t.more()) {
+        int order = static_cast<int>(t.num());
+        Real sr = t.num(), amin = t.num(), amax = t.num(), kmax = t.num();
+        s.addBoundary(std::make_shared<PML>(n, dir, order, sr, amin, amax, kmax));
+      } else {
+        s.addBoundary(std::make_shared<PML>(n, dir));
+      }
+    } else if (d == "tfsf3d") {
+      std::size_t x = static_cast<std::size_t>(t.num()), y = static_cast<std::size_t>(t.num()),
+                  z = static_cast<std::size_t>(t.num());
+      Real th = t.num(), ph = t.num(), ps = t.num();
+      s.addWaveformSource(std::make_shared<TFSF3D>(x, y, z, th, ph, ps, waveform(t)));
+    } else if (d == "current_source" || d == "voltage_source") {
+      std::string name = t.word();
+      auto cb = cube(t);
+      auto dir = parseDir(t.word());
+      Real r = t.num();
+      if (d == "current_source")
+        s.addObject(std::make_shared<CurrentSource>(name, std::move(cb), dir, r, waveform(t)));
+      else
+        s.addObject(std::make_shared<VoltageSource>(name, std::move(cb), dir, r, waveform(t)));
+    } else if (d == "resistor") {
+      std::string name = t.word();
+      auto cb = cube(t);
+      auto a = parseAxis(t.word());
+      s.addObject(std::make_shared<Resistor>(name, std::move(cb), a, t.num()));
+    } else if (d == "movie") {
+      auto f = parseField(t.word());
+      auto cb = cube(t);
+      std::size_t interval = static_cast<std::size_t>(t.num());
+      std::string name = t.word(), dir = t.word();
+      if (dir == "-") dir.clear();
+      auto m = std::make_shared<MovieMonitor>(std::make_unique<FieldMonitor>(std::move(cb), f, "", ""), interval,
+                                              name, dir);
+      s.addMonitor(m);
+      h->monitors.push_back(m);
+    } else if (d == "field_time") {
+      std::string name = t.word();
+      auto f = parseField(t.word());
+      auto cb = cube(t);
+      std::string dir = t.word();
+      if (dir == "-") dir.clear();
+      auto m = std::make_shared<FieldTimeMonitor>(name, std::move(cb), dir, f);
+      s.addMonitor(m);
+      h->monitors.push_back(m);
+    } else {
+      throw XFDTDException("unknown directive " + d);
+    }
+  }
+  if (!h->sim) throw XFDTDException("spec has no 'sim' line");
+  return h;
+}
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return XFD_OK;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return XFD_EINVAL;
+  }
+}
+
+SimHandle* H(void* p) { return static_cast<SimHandle*>(p); }
+
+int copyArray(const Array3D<Real>& a, int64_t* shape, double* dst) {
+  for (int q = 0; q < 3; ++q) shape[q] = static_cast<int64_t>(a.shape[q]);
+  if (dst && a.size()) std::memcpy(dst, a.data.data(), a.size() * sizeof(double));
+  return XFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xfd_sim_last_error(void) { return g_err.c_str(); }
+
+void* xfd_sim_create(const char* spec) {
+  std::unique_ptr<SimHandle> h;
+  if (guard([&] { h = build(spec ? spec : ""); }) != XFD_OK) return nullptr;
+  return h.release();
+}
+
+void xfd_sim_destroy(void* p) { delete H(p); }
+
+// precision: 0 = FP64, 1 = FP32
+int xfd_sim_set_precision(void* p, int32_t precision) {
+  return guard([&] {
+    H(p)->precision = precision == 1 ? Precision::FP32 : Precision::FP64;
+    H(p)->sim->setPrecision(H(p)->precision);
+  });
+}
+
+int xfd_sim_set_profiling(void* p, int32_t on) {
+  return guard([&] { H(p)->sim->setProfiling(on != 0); });
+}
+
+int xfd_sim_init(void* p, int64_t steps) {
+  return guard([&] { H(p)->sim->init(static_cast<Index>(steps)); });
+}
+
+int xfd_sim_precompute(void* p, int64_t steps) {
+  return guard([&] { H(p)->sim->precompute(static_cast<Index>(steps)); });
+}
+
+int xfd_sim_run_loop(void* p) {
+  return guard([&] { H(p)->sim->run(); });
+}
+
+int xfd_sim_run(void* p, int64_t steps) {
+  return guard([&] { H(p)->sim->run(static_cast<Index>(steps)); });
+}
+
+double xfd_sim_dt(void* p) { return H(p)->sim->dt(); }
+double xfd_sim_loop_seconds(void* p) { return H(p)->sim->loopSeconds(); }
+
+int xfd_sim_grid(void* p, int64_t* n3) {
+  return guard([&] {
+    auto g = H(p)->sim->gridSize();
+    for (int q = 0; q < 3; ++q) n3[q] = static_cast<int64_t>(g[q]);
+  });
+}
+
+int64_t xfd_sim_device_bytes(void* p) { return H(p)->sim->deviceBytes(); }
+
+int xfd_sim_field(void* p, int32_t field, int64_t* shape, double* dst) {
+  return guard([&] {
+    if (field < 0 || field > 5) throw XFDTDException("bad field id");
+    auto& e = *H(p)->sim->emf();
+    const Array3D<Real>* a[6] = {&e.ex, &e.ey, &e.ez, &e.hx, &e.hy, &e.hz};
+    copyArray(*a[field], shape, dst);
+  });
+}
+
+int xfd_sim_coeff(void* p, const char* name, int64_t* shape, double* dst) {
+  return guard([&] { copyArray(H(p)->sim->coefficient(name), shape, dst); });
+}
+
+// which: 0 = e_node, 2 = e_size (h_size is 3); returns the length
+int64_t xfd_sim_grid_array(void* p, int32_t which, int32_t axis, double* dst) {
+  const std::vector<Real>& v = which == 0 ? H(p)->sim->eNode(axis) : H(p)->sim->hSize(axis);
+  if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  return static_cast<int64_t>(v.size());
+}
+
+int xfd_sim_monitor_count(void* p) { return static_cast<int>(H(p)->monitors.size()); }
+
+int xfd_sim_monitor_records(void* p, int32_t m) {
+  auto& mon = H(p)->monitors.at(m);
+  if (auto* mv = dynamic_cast<MovieMonitor*>(mon.get())) return static_cast<int>(mv->frames().size());
+  if (dynamic_cast<FieldTimeMonitor*>(mon.get())) return 1;
+  return 0;
+}
+
+int xfd_sim_monitor_record(void* p, int32_t m, int32_t r, int64_t* shape, double* dst) {
+  return guard([&] {
+    auto& mon = H(p)->monitors.at(m);
+    if (auto* mv = dynamic_cast<MovieMonitor*>(mon.get())) {
+      copyArray(mv->frames().at(r).second, shape, dst);
+      return;
+    }
+    if (auto* ft = dynamic_cast<FieldTimeMonitor*>(mon.get())) {
+      const std::size_t nt = ft->time().size();
+      shape[0] = 2;
+      shape[1] = static_cast<int64_t>(nt);
+      shape[2] = 1;
+      if (dst) {
+        std::memcpy(dst, ft->time().data(), nt * sizeof(double));
+        std::memcpy(dst + nt, ft->data().data(), nt * sizeof(double));
+      }
+      return;
+    }
+    throw XFDTDException("monitor has no records");
+  });
+}
+
+int xfd_sim_kernel_stat(void* p, int32_t k, char* name, int32_t name_len, int64_t* launches, double* total_ms) {
+  return guard([&] {
+    auto st = H(p)->sim->kernelStats();
+    if (k < 0 || k >= static_cast<int>(st.size())) throw XFDTDException("bad kernel index");
+    std::snprintf(name, static_cast<std::size_t>(name_len), "%s", st[k].name.c_str());
+    *launches = st[k].launches;
+    *total_ms = st[k].total_ms;
+  });
+}
+
+int xfd_sim_kernel_count(void* p) { return static_cast<int>(H(p)->sim->kernelStats().size()); }
+
+}  // extern "C"

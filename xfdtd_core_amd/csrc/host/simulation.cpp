@@ -1,0 +1,1035 @@
+// Host side of Simulation: the FP64 precompute of the reference's init()
+// (src/simulation/simulation.cpp:319-389) and the hand-off to the device
+// library.  The arithmetic of every precomputed value follows the reference
+// expression for expression (cited per stage) so the coefficient arrays are
+// bit-identical; the work itself is organised for large grids: each stage
+// is a parallel sweep over i-planes, the staircase visits only the cells in
+// a shape's bounding box, and the CPML/TFSF/lumped data are emitted directly
+// in the form the C-ABI consumes.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <filesystem>
+#include <limits>
+#include <map>
+#include <thread>
+
+#include "../../../include/xfdtd_core_amd.h"
+#include "../../../include/xfdtd_core_amd.hpp"
+#include "npy.hpp"
+
+namespace xfdtd {
+
+namespace {
+
+constexpr Real kPi = 3.141592653589793238462643383279502884;
+constexpr Real kEps0 = 8.854187817e-12;  // include/xfdtd/common/constant.h:12-13
+constexpr Real kMu0 = 4 * kPi * 1e-7;
+const Real kC0 = 1 / std::sqrt(kEps0 * kMu0);
+constexpr Real kSigmaZero = 1e-20;  // material.h:40-41
+
+// Runs fn(lo, hi) over [0, n) split across the host's cores.
+template <class F>
+void parallelFor(std::size_t n, F&& fn) {
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  std::size_t parts = std::min<std::size_t>({static_cast<std::size_t>(hw), n, 32});
+  if (parts <= 1) {
+    if (n) fn(std::size_t{0}, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::size_t step = (n + parts - 1) / parts;
+  for (std::size_t p = 0; p < parts; ++p) {
+    std::size_t lo = p * step, hi = std::min(n, lo + step);
+    if (lo >= hi) break;
+    th.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+  }
+  for (auto& t : th) t.join();
+}
+
+// xt::linspace into a container: running sum of the step, last element
+// pinned to stop (xtensor 0.25 arange_generator::assign_to).  Used for the
+// e-nodes (grid_space_generator.cpp:210-212) and eTime/hTime
+// (time_param.cpp:63-71).
+std::vector<Real> linspaceAssign(Real start, Real stop, std::size_t n) {
+  std::vector<Real> v(n);
+  const Real step = (stop - start) / std::fmax(Real(1), Real(n - 1));
+  Real acc = start;
+  for (std::size_t i = 0; i < n; ++i) {
+    v[i] = acc;
+    acc += step;
+  }
+  if (n > 1) v[n - 1] = stop;
+  return v;
+}
+
+// float_compare.h:24-42, LessEqual
+bool leq(Real a, Real b, Real eps) { return std::abs(a - b) <= eps ? true : a < b; }
+
+int ax(Axis::XYZ a) { return static_cast<int>(a); }
+
+}  // namespace
+
+// ------------------------------------------------------------- shapes ----
+Cube::Cube(Vector o, Vector s) : _origin(o), _size(s) {
+  Real e[3];
+  for (int a = 0; a < 3; ++a) {
+    e[a] = o[a] + s[a];  // cube.cpp:128-139
+    if (std::isnan(e[a])) e[a] = std::numeric_limits<Real>::infinity();
+  }
+  _end = Vector(e[0], e[1], e[2]);
+}
+bool Cube::isInside(const Vector& p, Real eps) const {  // cube.cpp:107-118
+  bool in[3];
+  for (int a = 0; a < 3; ++a) in[a] = leq(_origin[a], p[a], eps) && leq(p[a], _end[a], eps);
+  return in[0] && in[1] && in[2];
+}
+std::unique_ptr<Cube> Cube::wrappedCube() const { return std::make_unique<Cube>(*this); }
+std::unique_ptr<Shape> Cube::clone() const { return std::make_unique<Cube>(*this); }
+
+Sphere::Sphere(Vector c, Real r) : _center(c), _radius(r) {
+  if (r < 0) throw XFDTDException("Sphere radius must be positive");
+}
+bool Sphere::isInside(const Vector& p, Real eps) const {  // sphere.cpp:29-32
+  const Real dx = p.x() - _center.x(), dy = p.y() - _center.y(), dz = p.z() - _center.z();
+  return leq(std::sqrt((dx * dx + dy * dy) + dz * dz), _radius, eps);
+}
+std::unique_ptr<Cube> Sphere::wrappedCube() const {  // sphere.cpp:34-37
+  return std::make_unique<Cube>(
+      Vector(_center.x() - _radius, _center.y() - _radius, _center.z() - _radius),
+      Vector(_radius * 2, _radius * 2, _radius * 2));
+}
+std::unique_ptr<Shape> Sphere::clone() const { return std::make_unique<Sphere>(*this); }
+
+Cylinder::Cylinder(Vector c, Real r, Real h, Axis::XYZ a) : _center(c), _radius(r), _height(h), _axis(a) {
+  if (r < 0 || h < 0) throw XFDTDException("Cylinder radius and height must be positive");
+}
+std::unique_ptr<Cube> Cylinder::wrappedCube() const {  // cylinder.cpp:73-98
+  const Real s = _radius * 2;
+  const int m = ax(_axis);
+  Real o[3], z[3];
+  for (int a = 0; a < 3; ++a) {
+    o[a] = a == m ? _center[a] - _height / 2 : _center[a] - _radius;
+    z[a] = a == m ? _height : s;
+  }
+  return std::make_unique<Cube>(Vector(o[0], o[1], o[2]), Vector(z[0], z[1], z[2]));
+}
+bool Cylinder::isInside(const Vector& p, Real eps) const {  // cylinder.cpp:41-66
+  if (!wrappedCube()->isInside(p, eps)) return false;
+  const int m = ax(_axis);
+  const int u = m == 0 ? 1 : 0, w = m == 2 ? 1 : 2;
+  const Real d = std::sqrt(std::pow(p[u] - _center[u], 2) + std::pow(p[w] - _center[w], 2));
+  return leq(d, _radius, 1e-8);
+}
+std::unique_ptr<Shape> Cylinder::clone() const { return std::make_unique<Cylinder>(*this); }
+
+ElectroMagneticProperty::ElectroMagneticProperty(Real er, Real mr, Real se, Real sm)
+    : _eps_r(er), _mu_r(mr), _sigma_e(se), _sigma_m(sm), _eps(er * kEps0), _mu(mr * kMu0) {
+  if (_sigma_e == 0) _sigma_e = kSigmaZero;  // material.cpp:20-25
+  if (_sigma_m == 0) _sigma_m = kSigmaZero;
+}
+
+// ----------------------------------------------------------- waveforms ---
+void Waveform::init(const std::vector<Real>& time) {
+  _value.assign(time.size(), 0);
+  for (std::size_t i = 0; i < time.size(); ++i) _value[i] = _amplitude * _func(time[i]);
+}
+std::unique_ptr<Waveform> Waveform::gaussian(Real tau, Real t_0, Real amplitude) {  // waveform.cpp:80-85
+  return std::make_unique<Waveform>([tau, t_0](Real t) { return std::exp(-std::pow((t - t_0) / tau, 2)); },
+                                    amplitude);
+}
+std::unique_ptr<Waveform> Waveform::cosineModulatedGaussian(Real tau, Real t_0, Real f, Real amplitude) {
+  return std::make_unique<Waveform>(
+      [tau, t_0, f](Real t) {
+        return std::exp(-std::pow((t - t_0) / tau, 2)) * std::cos(2 * kPi * f * (t - t_0));
+      },
+      amplitude);
+}
+std::unique_ptr<Waveform> Waveform::sine(Real f, Real amplitude) {
+  return std::make_unique<Waveform>([f](Real t) { return std::sin(2 * kPi * f * t); }, amplitude);
+}
+std::unique_ptr<Waveform> Waveform::cosine(Real f, Real amplitude) {
+  return std::make_unique<Waveform>([f](Real t) { return std::cos(2 * kPi * f * t); }, amplitude);
+}
+std::unique_ptr<Waveform> Waveform::step(Real t_0, Real amplitude) {
+  return std::make_unique<Waveform>([t_0](Real t) { return static_cast<Real>(t >= t_0); }, amplitude);
+}
+
+// ------------------------------------------------------------- objects ---
+Object::Object(std::string n, std::unique_ptr<Shape> s, std::unique_ptr<Material> m)
+    : _name(std::move(n)), _shape(std::move(s)), _material(std::move(m)) {}
+LumpedElement::LumpedElement(std::string n, std::unique_ptr<Cube> c, Axis::XYZ xyz, std::unique_ptr<Material> m)
+    : Object(std::move(n), std::move(c), std::move(m)), _xyz(xyz) {}
+Resistor::Resistor(std::string n, std::unique_ptr<Cube> c, Axis::XYZ xyz, Real r, std::unique_ptr<Material> m)
+    : LumpedElement(std::move(n), std::move(c), xyz, std::move(m)), _resistance(r == 0 ? 1e-10 : r) {}
+static Axis::XYZ dirAxis(Axis::Direction d) { return static_cast<Axis::XYZ>(static_cast<int>(d) / 2); }
+static bool dirNeg(Axis::Direction d) { return static_cast<int>(d) % 2 == 0; }
+VoltageSource::VoltageSource(std::string n, std::unique_ptr<Cube> c, Axis::Direction d, Real r,
+                             std::unique_ptr<Waveform> w, std::unique_ptr<Material> m)
+    : LumpedElement(std::move(n), std::move(c), dirAxis(d), std::move(m)),
+      _direction(d),
+      _resistance(r == 0 ? 1e-20 : r),
+      _waveform(std::move(w)) {}
+CurrentSource::CurrentSource(std::string n, std::unique_ptr<Cube> c, Axis::Direction d, Real r,
+                             std::unique_ptr<Waveform> w, std::unique_ptr<Material> m)
+    : LumpedElement(std::move(n), std::move(c), dirAxis(d), std::move(m)),
+      _direction(d),
+      _resistance(r == 0 ? 1e-20 : r),
+      _waveform(std::move(w)) {}
+PML::PML(int t, Axis::Direction d, int o, Real sr, Real amin, Real amax, Real kmax)
+    : _thickness(t), _direction(d), _order(o), _sigma_ratio(sr), _alpha_min(amin), _alpha_max(amax),
+      _kappa_max(kmax) {}
+TFSF3D::TFSF3D(std::size_t x, std::size_t y, std::size_t z, Real th, Real ph, Real ps, std::unique_ptr<Waveform> w)
+    : WaveformSource(std::move(w)), _x(x), _y(y), _z(z), _theta(th), _phi(ph), _psi(ps) {}
+
+Array3D<Real>& EMF::field(Field f) {
+  switch (f) {
+    case Field::EX: return ex;
+    case Field::EY: return ey;
+    case Field::EZ: return ez;
+    case Field::HX: return hx;
+    case Field::HY: return hy;
+    case Field::HZ: return hz;
+    default: throw XFDTDException("EMF::field: unsupported field");
+  }
+}
+
+FieldMonitor::FieldMonitor(std::unique_ptr<Shape> s, EMF::Field f, std::string n, std::string d)
+    : Monitor(std::move(n), std::move(d)), _shape(std::move(s)), _field(f) {}
+MovieMonitor::MovieMonitor(std::unique_ptr<Monitor> frame, std::size_t interval, std::string n, std::string d)
+    : Monitor(std::move(n), std::move(d)), _interval(interval) {
+  auto* fm = dynamic_cast<FieldMonitor*>(frame.get());
+  if (!fm) throw XFDTDException("MovieMonitor frame must be a FieldMonitor");
+  frame.release();
+  _frame.reset(fm);
+  if (_interval == 0) throw XFDTDException("MovieMonitor interval must be positive");
+}
+FieldTimeMonitor::FieldTimeMonitor(std::string n, std::unique_ptr<Cube> c, std::string d, EMF::Field f)
+    : Monitor(std::move(n), std::move(d)), _cube(std::move(c)), _field(f) {}
+
+// ---------------------------------------------------------- simulation ---
+namespace {
+
+// One staggered array in the reference extent, k fastest.
+struct Field3 {
+  std::size_t n[3] = {0, 0, 0};
+  std::vector<Real> v;
+  void resize(const std::size_t e[3], Real fill) {
+    for (int a = 0; a < 3; ++a) n[a] = e[a];
+    v.assign(e[0] * e[1] * e[2], fill);
+  }
+  Real& at(std::size_t i, std::size_t j, std::size_t k) { return v[(i * n[1] + j) * n[2] + k]; }
+  Real at(std::size_t i, std::size_t j, std::size_t k) const { return v[(i * n[1] + j) * n[2] + k]; }
+};
+
+// Extent of component c of E (h = false) or H.
+void extent(bool h, int c, const std::size_t N[3], std::size_t out[3]) {
+  for (int a = 0; a < 3; ++a) out[a] = h ? (a == c ? N[a] + 1 : N[a]) : (a == c ? N[a] : N[a] + 1);
+}
+
+const char* kCoefNames[18] = {"cexe", "cexhy", "cexhz", "ceye", "ceyhz", "ceyhx", "ceze", "cezhx", "cezhy",
+                              "chxh", "chxey", "chxez", "chyh", "chyez", "chyex", "chzh", "chzex", "chzey"};
+
+}  // namespace
+
+struct Simulation::Impl {
+  Real d[3], cfl;
+  Precision precision = Precision::FP64;
+  std::vector<std::shared_ptr<Object>> objects;
+  std::vector<std::shared_ptr<WaveformSource>> sources;
+  std::vector<std::shared_ptr<Boundary>> boundaries;
+  std::vector<std::shared_ptr<Monitor>> monitors;
+
+  // grid (grid_space_generator.cpp, grid_space.cpp:526-544)
+  std::size_t N[3] = {0, 0, 0};
+  std::vector<Real> e_node[3], h_node[3], e_size[3], h_size[3];
+  Real min_d[3];
+  Real dt = 0;
+  Index steps = 0;
+  bool initialised = false;
+
+  // material (eps, mu, sigma_e, sigma_m per component) and coefficients
+  Field3 eps[3], mu[3], se[3], sm[3];
+  Field3 coef[18];
+  mutable std::map<std::string, Array3D<Real>> coef_view;
+
+  // device hand-off
+  struct SlabData {
+    xfd_cpml_desc d{};
+    std::vector<Real> ae, be, ah, bh;
+    std::vector<Real> cpsi[4];
+  };
+  std::vector<SlabData> slabs;
+  struct LumpedData {
+    xfd_lumped_desc d{};
+    std::vector<Real> coff;
+    std::vector<Real> wave;
+  };
+  std::vector<LumpedData> lumped;
+  struct TfsfData {
+    xfd_tfsf_desc d{};
+    std::vector<Real> proj[6];
+    std::vector<Real> e_inc, h_inc;
+  };
+  std::vector<TfsfData> tfsf;
+  struct GatherPlan {
+    xfd_gather_desc d{};
+    Monitor* monitor;
+    bool time_series;
+  };
+  std::vector<GatherPlan> gathers;
+
+  xfd_ctx* ctx = nullptr;
+  std::shared_ptr<EMF> emf = std::make_shared<EMF>();
+  double loop_seconds = 0;
+  bool profiling = false;
+  std::vector<KernelStat> stats;
+  long long dev_bytes = 0;
+
+  ~Impl() {
+    if (ctx) xfd_destroy(ctx);
+  }
+
+  // ---------------------------------------------------------------- grid --
+  Real tolerance() const { return std::min({min_d[0], min_d[1], min_d[2]}) / 1000; }  // grid_space.cpp:459-468
+
+  std::size_t nearest(int a, Real x) const {  // xt::argmin(abs(e_node - x)), first minimum
+    const auto& e = e_node[a];
+    std::size_t best = 0;
+    Real bv = std::abs(e[0] - x);
+    for (std::size_t i = 1; i < e.size(); ++i) {
+      const Real v = std::abs(e[i] - x);
+      if (v < bv) {
+        bv = v;
+        best = i;
+      }
+    }
+    return best;
+  }
+
+  void gridBox(const Shape& s, std::size_t o[3], std::size_t e[3]) const {  // grid_space.cpp:282-293
+    auto c = s.wrappedCube();
+    for (int a = 0; a < 3; ++a) {
+      o[a] = nearest(a, c->origin()[a]);
+      e[a] = nearest(a, c->end()[a]);
+    }
+  }
+
+  void buildGrid() {
+    // Shape::makeWrappedCube (shape.cpp:8-38) -> extendDomain
+    // (grid_space_generator.cpp:36-99) -> generateGridSpace3D (:183-217)
+    if (objects.empty()) throw XFDTDSimulationException("No object is added");
+    Real lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::numeric_limits<Real>::max();
+      hi[a] = std::numeric_limits<Real>::lowest();
+    }
+    for (auto& o : objects) {
+      auto c = o->shape().wrappedCube();
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c->origin()[a]);
+        hi[a] = std::max(hi[a], c->end()[a]);
+      }
+    }
+    Cube dom(Vector(lo[0], lo[1], lo[2]), Vector(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]));
+    bool any_pml = false;
+    Real plo[3] = {dom.origin()[0], dom.origin()[1], dom.origin()[2]};
+    Real phi[3] = {dom.end()[0], dom.end()[1], dom.end()[2]};
+    for (auto& b : boundaries) {
+      auto* p = dynamic_cast<PML*>(b.get());
+      if (!p) continue;
+      any_pml = true;
+      if (p->thickness() < 0) continue;
+      const int a = ax(dirAxis(p->direction()));
+      if (dirNeg(p->direction()))
+        plo[a] -= p->thickness() * d[a];
+      else
+        phi[a] += p->thickness() * d[a];
+    }
+    if (any_pml)
+      dom = Cube(Vector(plo[0], plo[1], plo[2]), Vector(phi[0] - plo[0], phi[1] - plo[1], phi[2] - plo[2]));
+    Real mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = dom.origin()[a];
+      mx[a] = dom.end()[a];
+      N[a] = static_cast<std::size_t>(std::round((mx[a] - mn[a]) / d[a]));
+      if (N[a] <= 1) throw XFDTDException("GridSpace is too small");
+    }
+    for (int a = 0; a < 3; ++a) mx[a] = mn[a] + d[a] * N[a];
+    Cube region(Vector(mn[0], mn[1], mn[2]), Vector(mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]));
+    for (int a = 0; a < 3; ++a) {
+      e_node[a] = linspaceAssign(region.origin()[a], region.end()[a], N[a] + 1);
+      const auto& e = e_node[a];
+      const std::size_t m = N[a];
+      h_node[a].resize(m);
+      e_size[a].resize(m);
+      h_size[a].resize(m + 1);
+      for (std::size_t i = 0; i < m; ++i) {
+        h_node[a][i] = (e[i] + e[i + 1]) / 2;
+        e_size[a][i] = e[i + 1] - e[i];
+      }
+      // h sizes include the half cells beyond the end nodes
+      Real prev = e[0] - d[a] / 2;
+      for (std::size_t i = 0; i < m; ++i) {
+        h_size[a][i] = h_node[a][i] - prev;
+        prev = h_node[a][i];
+      }
+      h_size[a][m] = (e[m] + d[a] / 2) - prev;
+      min_d[a] = *std::min_element(e_size[a].begin(), e_size[a].end());
+    }
+    dt = cfl / (kC0 * std::sqrt(1.0 / (min_d[0] * min_d[0]) + 1.0 / (min_d[1] * min_d[1]) +
+                                1.0 / (min_d[2] * min_d[2])));  // time_param.cpp:10-13
+  }
+
+  // ------------------------------------------------------------ material --
+  void buildMaterial() {
+    for (int c = 0; c < 3; ++c) {  // material_param.cpp:365-393
+      std::size_t ee[3], hh[3];
+      extent(false, c, N, ee);
+      extent(true, c, N, hh);
+      eps[c].resize(ee, kEps0);
+      se[c].resize(ee, kSigmaZero);
+      mu[c].resize(hh, kMu0);
+      sm[c].resize(hh, kSigmaZero);
+    }
+    // Staircase (object.cpp:114-198): a cell takes the object's properties
+    // when its centre is inside the shape; objects in add order, the last
+    // wins.  Only cells whose centre lies inside the shape's bounding box
+    // widened by the tolerance can pass the test, so only those are visited.
+    const Real tol = tolerance();
+    for (auto& o : objects) {
+      const auto& p = o->material().emProperty();
+      auto box = o->shape().wrappedCube();
+      std::size_t lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        const auto& h = h_node[a];
+        const Real l = box->origin()[a] - tol * 2, u = box->end()[a] + tol * 2;
+        lo[a] = static_cast<std::size_t>(std::lower_bound(h.begin(), h.end(), l) - h.begin());
+        hi[a] = static_cast<std::size_t>(std::upper_bound(h.begin(), h.end(), u) - h.begin());
+        if (std::isinf(box->origin()[a])) lo[a] = 0;
+        if (std::isinf(box->end()[a])) hi[a] = N[a];
+      }
+      if (lo[0] >= hi[0] || lo[1] >= hi[1] || lo[2] >= hi[2]) continue;
+      const Shape& shape = o->shape();
+      parallelFor(hi[0] - lo[0], [&](std::size_t b, std::size_t e) {
+        for (std::size_t i = lo[0] + b; i < lo[0] + e; ++i)
+          for (std::size_t j = lo[1]; j < hi[1]; ++j)
+            for (std::size_t k = lo[2]; k < hi[2]; ++k) {
+              if (!shape.isInside(Vector(h_node[0][i], h_node[1][j], h_node[2][k]), tol)) continue;
+              for (int c = 0; c < 3; ++c) {
+                eps[c].at(i, j, k) = p.epsilon();
+                mu[c].at(i, j, k) = p.mu();
+                se[c].at(i, j, k) = p.sigmaE();
+                sm[c].at(i, j, k) = p.sigmaM();
+              }
+            }
+      });
+    }
+    // Edge averaging (calculation_param.cpp:43-168): eps/sigma_e the mean of
+    // the four cells around an edge, mu/sigma_m the harmonic mean of the two
+    // cells across a face; evaluated from the pre-averaging values.
+    for (int c = 0; c < 3; ++c) {
+      averageEdges(eps[c], c);
+      averageEdges(se[c], c);
+      averageFaces(mu[c], c);
+      averageFaces(sm[c], c);
+    }
+  }
+
+  void averageEdges(Field3& f, int c) {
+    const Field3 src = f;
+    const std::size_t s[3] = {c == 0 ? 0u : 1u, c == 1 ? 0u : 1u, c == 2 ? 0u : 1u};
+    const int u = (c + 1) % 3, w = (c + 2) % 3;  // the two axes averaged over
+    const std::size_t stride[3] = {src.n[1] * src.n[2], src.n[2], 1};
+    const std::size_t su = stride[u], sw = stride[w];
+    // operand order of the reference: p, p-(first of y/x), p-(z or y), p-both
+    // with (first, second) = (y, z) for x, (x, z) for y, (x, y) for z.
+    const std::size_t s1 = c == 0 ? stride[1] : stride[0];
+    const std::size_t s2 = c == 2 ? stride[1] : stride[2];
+    (void)su;
+    (void)sw;
+    parallelFor(N[0] - s[0], [&](std::size_t b, std::size_t e) {
+      for (std::size_t i = s[0] + b; i < s[0] + e; ++i)
+        for (std::size_t j = s[1]; j < N[1]; ++j)
+          for (std::size_t k = s[2]; k < N[2]; ++k) {
+            const std::size_t p = (i * src.n[1] + j) * src.n[2] + k;
+            f.v[p] = 0.25 * (src.v[p] + src.v[p - s1] + src.v[p - s2] + src.v[p - s1 - s2]);
+          }
+    });
+  }
+
+  void averageFaces(Field3& f, int c) {
+    const Field3 src = f;
+    const std::size_t s[3] = {c == 0 ? 1u : 0u, c == 1 ? 1u : 0u, c == 2 ? 1u : 0u};
+    const std::size_t stride[3] = {src.n[1] * src.n[2], src.n[2], 1};
+    const std::size_t sc = stride[c];
+    parallelFor(N[0] - s[0], [&](std::size_t b, std::size_t e) {
+      for (std::size_t i = s[0] + b; i < s[0] + e; ++i)
+        for (std::size_t j = s[1]; j < N[1]; ++j)
+          for (std::size_t k = s[2]; k < N[2]; ++k) {
+            const std::size_t p = (i * src.n[1] + j) * src.n[2] + k;
+            const Real a = src.v[p], q = src.v[p - sc];
+            f.v[p] = 2 * a * q / (a + q);
+          }
+    });
+  }
+
+  // -------------------------------------------------------- coefficients --
+  void buildCoefficients() {  // calculation_param.cpp:170-264
+    for (int h = 0; h < 2; ++h)
+      for (int c = 0; c < 3; ++c) {
+        std::size_t ext[3];
+        extent(h, c, N, ext);
+        Field3& cs = coef[h * 9 + c * 3];
+        Field3& ca = coef[h * 9 + c * 3 + 1];
+        Field3& cb = coef[h * 9 + c * 3 + 2];
+        cs.resize(ext, 0);
+        ca.resize(ext, 0);
+        cb.resize(ext, 0);
+        // E_c: e_size on c, h_size elsewhere; H_c the opposite
+        const std::vector<Real>* sz[3];
+        for (int a = 0; a < 3; ++a) sz[a] = (h != 0) == (a == c) ? &h_size[a] : &e_size[a];
+        const int A = (c + 1) % 3, B = (c + 2) % 3;
+        const Field3& m = h ? mu[c] : eps[c];
+        const Field3& s = h ? sm[c] : se[c];
+        const Real dtv = dt;
+        parallelFor(ext[0], [&](std::size_t lo, std::size_t hi) {
+          for (std::size_t i = lo; i < hi; ++i)
+            for (std::size_t j = 0; j < ext[1]; ++j)
+              for (std::size_t k = 0; k < ext[2]; ++k) {
+                const std::size_t id[3] = {i, j, k};
+                const Real da = (*sz[A])[id[A]], db = (*sz[B])[id[B]];
+                const std::size_t p = (i * ext[1] + j) * ext[2] + k;
+                const Real e = m.v[p], sg = s.v[p];
+                cs.v[p] = (2 * e - dtv * sg) / (2 * e + dtv * sg);
+                if (!h) {
+                  ca.v[p] = -(2 * dtv / db) / (2 * e + sg * dtv);
+                  cb.v[p] = (2 * dtv / da) / (2 * e + sg * dtv);
+                } else {
+                  ca.v[p] = (2 * dtv / db) / (2 * e + sg * dtv);
+                  cb.v[p] = -(2 * dtv / da) / (2 * e + sg * dtv);
+                }
+              }
+        });
+      }
+  }
+
+  // ------------------------------------------------------- lumped elements --
+  // LumpedElement ranges (lumped_element.h:239-264) and the meshgrid of cell
+  // sizes over them; the patch of resistor.cpp:85-127,
+  // voltage_source.cpp:109-161 and current_source.cpp:111-161.
+  void buildLumped(LumpedElement& le) {
+    std::size_t o[3], e[3];
+    gridBox(le.shape(), o, e);
+    const int m = ax(le.xyz()), A = (m + 1) % 3, B = (m + 2) % 3;
+    std::size_t lo[3], cnt[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = o[a];
+      cnt[a] = a == m ? e[a] - o[a] : e[a] - o[a] + 1;
+    }
+    const Real gmain = static_cast<Real>(e[m] - o[m]);
+    const std::size_t ga = e[A] - o[A] + 1, gb = e[B] - o[B] + 1;
+    const std::size_t gc = e[m] - o[m];
+    (void)gmain;
+    auto size_of = [&](int a, std::size_t idx) { return a == m ? e_size[a][idx] : h_size[a][idx]; };
+
+    auto* res = dynamic_cast<Resistor*>(&le);
+    auto* vs = dynamic_cast<VoltageSource*>(&le);
+    auto* cs = dynamic_cast<CurrentSource*>(&le);
+    const Real R = res ? res->resistance() : vs ? vs->resistance() : cs->resistance();
+    const Real rf = R * ga * gb / gc;
+    Real factor = 0;
+    if (vs) factor = vs->waveform().amplitude() / gc;
+    if (cs) factor = cs->waveform().amplitude() / (ga * gb);
+    if ((vs && dirNeg(vs->direction())) || (cs && dirNeg(cs->direction()))) factor *= -1;
+
+    Field3& cece = coef[m * 3];
+    Field3& cecha = coef[m * 3 + 1];
+    Field3& cechb = coef[m * 3 + 2];
+    const Field3& ep = eps[m];
+    const Field3& sg = se[m];
+    LumpedData ld;
+    ld.coff.resize(cnt[0] * cnt[1] * cnt[2]);
+    for (std::size_t i = 0; i < cnt[0]; ++i)
+      for (std::size_t j = 0; j < cnt[1]; ++j)
+        for (std::size_t k = 0; k < cnt[2]; ++k) {
+          const std::size_t g[3] = {lo[0] + i, lo[1] + j, lo[2] + k};
+          const Real da = size_of(A, g[A]), db = size_of(B, g[B]), dc = size_of(m, g[m]);
+          Real beta, alpha = 0;
+          if (vs) {
+            alpha = da * db * rf;
+            beta = dt * dc / alpha;
+          } else {
+            beta = dt * dc / (rf * da * db);
+          }
+          const Real e0 = ep.at(g[0], g[1], g[2]), s0 = sg.at(g[0], g[1], g[2]);
+          cece.at(g[0], g[1], g[2]) = (2 * e0 - dt * s0 - beta) / (2 * e0 + dt * s0 + beta);
+          cecha.at(g[0], g[1], g[2]) = -2 * dt / ((2 * e0 + dt * s0 + beta) * db);
+          cechb.at(g[0], g[1], g[2]) = 2 * dt / ((2 * e0 + dt * s0 + beta) * da);
+          const std::size_t q = (i * cnt[1] + j) * cnt[2] + k;
+          if (vs) ld.coff[q] = -2 * dt / ((2 * e0 + dt * s0 + beta) * alpha) * factor;
+          if (cs) ld.coff[q] = -2 * dt / ((2 * e0 + dt * s0 + beta) * (db * da)) * factor;
+        }
+    if (res) return;
+    ld.d.component = m;
+    for (int a = 0; a < 3; ++a) {
+      ld.d.origin[a] = static_cast<int32_t>(lo[a]);
+      ld.d.size[a] = static_cast<int32_t>(cnt[a]);
+    }
+    Waveform& w = vs ? vs->waveform() : cs->waveform();
+    w.init(hTime());  // voltage_source.cpp:163-167
+    ld.wave = w.value();
+    lumped.push_back(std::move(ld));
+  }
+
+  std::vector<Real> eTime() const { return linspaceAssign(1 * dt, steps * dt, steps); }
+  std::vector<Real> hTime() const { return linspaceAssign(0.5 * dt, (steps + 0.5) * dt, steps); }
+
+  // ------------------------------------------------------------------ CPML --
+  void buildPml(const PML& p) {
+    // PML::init (pml.cpp:98-196): E planes start one inside on the negative
+    // side; calRecursiveConvolutionCoeff (:526-633); correctCoefficient*
+    // (:286-524): c_psi = coefficient * size, then coefficient /= kappa.
+    const int c = ax(dirAxis(p.direction())), A = (c + 1) % 3, B = (c + 2) % 3;
+    const std::size_t n = static_cast<std::size_t>(std::abs(p.thickness()));
+    if (n == 0 || n > N[c]) throw XFDTDException("PML thickness does not fit the grid");
+    const bool neg = dirNeg(p.direction());
+    const std::size_t e0 = neg ? 1 : N[c] - n, h0 = neg ? 0 : N[c] - n;
+    if (e0 + n > N[c]) throw XFDTDException("PML slab leaves the grid");
+    SlabData s;
+    s.d.axis = c;
+    s.d.n = static_cast<int32_t>(n);
+    s.d.e_start = static_cast<int32_t>(e0);
+    s.d.h_start = static_cast<int32_t>(h0);
+    std::vector<Real> hs(h_size[c].begin() + e0, h_size[c].begin() + e0 + n);
+    std::vector<Real> es(e_size[c].begin() + h0, e_size[c].begin() + h0 + n);
+    const Real ord = static_cast<Real>(p.order());
+    auto depth = [&](const std::vector<Real>& sz, Real frac) {
+      std::vector<Real> r(n);
+      Real acc = 0;
+      for (std::size_t i = 0; i < n; ++i) {
+        const Real inner = sz[i] - frac * sz[i];
+        r[i] = i == 0 ? inner : acc + inner;
+        acc += sz[i];
+      }
+      if (neg) std::reverse(r.begin(), r.end());
+      for (auto& x : r) x = x / acc;
+      return r;
+    };
+    auto smax = [&](Real dl) { return p.sigmaRatio() * (p.order() + 1) / (150 * kPi * dl); };
+    const Real amin = p.alphaMin(), amax = p.alphaMax(), kmax = p.kappaMax();
+    std::vector<Real> ke(n), kh(n);
+    s.ae.resize(n);
+    s.be.resize(n);
+    s.ah.resize(n);
+    s.bh.resize(n);
+    {
+      const Real sm_e = smax(*std::min_element(hs.begin(), hs.end()));
+      const auto rho = depth(hs, 0.75);
+      for (std::size_t i = 0; i < n; ++i) {
+        const Real pw = std::pow(rho[i], ord);
+        const Real sg = sm_e * pw, al = amin + (amax - amin) * (1 - rho[i]), ka = 1 + (kmax - 1) * pw;
+        ke[i] = ka;
+        s.be[i] = std::exp((-dt / kEps0) * (sg / ka + al));
+        s.ae[i] = (s.be[i] - 1) * sg / ((sg + ka * al) * ka * hs[i]);
+      }
+    }
+    {
+      const Real e2m = kMu0 / kEps0;
+      const Real sm_m = smax(*std::min_element(es.begin(), es.end())) * e2m;
+      const auto rho = depth(es, 0.25);
+      for (std::size_t i = 0; i < n; ++i) {
+        const Real pw = std::pow(rho[i], ord);
+        const Real sg = sm_m * pw, al = (amin + (amax - amin) * (1 - rho[i])) * e2m,
+                   ka = 1 + (kmax - 1) * pw;
+        kh[i] = ka;
+        s.bh[i] = std::exp((-dt / kMu0) * (sg / ka + al));
+        s.ah[i] = (s.bh[i] - 1) * sg / ((sg + ka * al) * ka * es[i]);
+      }
+    }
+    // Patched coefficients and psi extents: E_a via its dual-B coefficient
+    // (extended along B), E_b via dual-A (extended along A); H_a extended
+    // along A, H_b along B.  Slot of the coefficient of component q for its
+    // tangential-A / tangential-B dual is 3q+1 / 3q+2.
+    struct P {
+      int slot;
+      int ext_axis;
+      std::size_t start;
+      const std::vector<Real>* size;
+      const std::vector<Real>* kappa;
+    };
+    // E_a's derivative along c (its tangential-B axis) acts on its
+    // tangential-A dual, whose coefficient is slot 3A+1 (x slab: ceyhz);
+    // E_b's acts on its tangential-B dual, slot 3B+2 (x slab: cezhy).
+    const P parts[4] = {{3 * A + 1, B, e0, &hs, &ke},
+                        {3 * B + 2, A, e0, &hs, &ke},
+                        {9 + 3 * A + 1, A, h0, &es, &kh},
+                        {9 + 3 * B + 2, B, h0, &es, &kh}};
+    for (int q = 0; q < 4; ++q) {
+      const P& pp = parts[q];
+      std::size_t ex[3];
+      ex[c] = n;
+      ex[A] = N[A];
+      ex[B] = N[B];
+      ex[pp.ext_axis] += 1;
+      auto& cpsi = s.cpsi[q];
+      cpsi.resize(ex[0] * ex[1] * ex[2]);
+      Field3& cf = coef[pp.slot];
+      parallelFor(ex[0], [&](std::size_t lo, std::size_t hi) {
+        for (std::size_t x = lo; x < hi; ++x)
+          for (std::size_t y = 0; y < ex[1]; ++y)
+            for (std::size_t z = 0; z < ex[2]; ++z) {
+              std::size_t g[3] = {x, y, z};
+              const std::size_t layer = g[c];
+              g[c] += pp.start;
+              Real& v = cf.at(g[0], g[1], g[2]);
+              cpsi[(x * ex[1] + y) * ex[2] + z] = v * (*pp.size)[layer];
+              v = v / (*pp.kappa)[layer];
+            }
+      });
+    }
+    slabs.push_back(std::move(s));
+  }
+
+  // ------------------------------------------------------------------ TFSF --
+  void buildTfsf(TFSF3D& t) {
+    // tfsf.cpp:14-30 (angles), :231-254 (transforms), :118-175 (box, aux
+    // line, 1D coefficients), :256-339 (projections), :36-78 (incident
+    // tables from the 1D auxiliary FDTD on eTime - dt).
+    TfsfData td;
+    const Real st = std::sin(t.theta()), ct = std::cos(t.theta());
+    const Real sp = std::sin(t.phi()), cp = std::cos(t.phi());
+    const Real ss = std::sin(t.psi()), cs = std::cos(t.psi());
+    const Real k[3] = {st * cp, st * sp, ct};
+    const Real va[3] = {-sp, ct * cp, st * cp}, vb[3] = {cp, ct * sp, st * sp}, vc[3] = {0, -st, ct};
+    const Real ke[3] = {ss, cs, 0};
+    auto dot = [](const Real* a, const Real* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const Real te[3] = {dot(va, ke), dot(vb, ke), dot(vc, ke)};
+    const Real th[3] = {k[1] * te[2] - k[2] * te[1], k[2] * te[0] - k[0] * te[2], k[0] * te[1] - k[1] * te[0]};
+    const std::size_t off[3] = {t.x(), t.y(), t.z()};
+    std::size_t org[3], sz[3];
+    for (int a = 0; a < 3; ++a) {
+      if (2 * off[a] >= N[a]) throw XFDTDException("TFSF box is empty");
+      org[a] = off[a];
+      sz[a] = N[a] - 2 * off[a];
+    }
+    const Real rd = 1 / std::sqrt(std::pow(st, 4) * (std::pow(cp, 4) + std::pow(sp, 4)) + std::pow(ct, 4));
+    const std::size_t aux = static_cast<std::size_t>(
+        std::ceil(rd * (std::sqrt(std::pow(static_cast<Real>(sz[0]), 2) + std::pow(static_cast<Real>(sz[1]), 2) +
+                                  std::pow(static_cast<Real>(sz[2]), 2)))) +
+        4 + 1);
+    for (int a = 0; a < 3; ++a) {
+      const std::size_t inj = k[a] >= 0 ? org[a] : org[a] + sz[a];
+      const Real inc = static_cast<Real>(inj) - (2 * k[a] / rd);
+      auto& pi = td.proj[a];
+      auto& ph = td.proj[3 + a];
+      pi.resize(sz[a] + 1);
+      ph.resize(sz[a] + 2);
+      for (std::size_t i = 0; i < pi.size(); ++i) pi[i] = (i + org[a] - inc) * k[a] * rd;
+      for (std::size_t i = 0; i < ph.size(); ++i) ph[i] = (i + org[a] - inc - 0.5) * k[a] * rd;
+    }
+    const Real sdl = d[2] / rd;
+    const Real ceie = 1, chih = 1;
+    const Real ceih = -(dt / (kEps0 * sdl)), chie = -(dt / (kMu0 * sdl));
+    const Real abc0 = (kC0 * dt - sdl) / (kC0 * dt + sdl), abc1 = 2 * sdl / (kC0 * dt + sdl);
+
+    std::vector<Real> tm = eTime();
+    for (auto& x : tm) x = x - dt;
+    t.waveform().init(tm);
+    const auto& src = t.waveform().value();
+    const std::size_t nt = steps, nl = aux - 1;
+    td.e_inc.assign(nt * aux, 0);
+    td.h_inc.assign(nt * nl, 0);
+    auto E = [&](std::size_t r, std::size_t l) -> Real& { return td.e_inc[r * aux + l]; };
+    auto H = [&](std::size_t r, std::size_t l) -> Real& { return td.h_inc[r * nl + l]; };
+    if (nt > 0) {
+      E(0, 0) = src[0];
+      for (std::size_t l = 1; l < nl; ++l) H(0, l) = chih * H(0, l) + chie * (E(0, l + 1) - E(0, l));
+      for (std::size_t r = 1; r < nt; ++r) {
+        if (nl > 1) E(r, 0) = src[r];
+        for (std::size_t l = 1; l < nl; ++l) E(r, l) = ceie * E(r - 1, l) + ceih * (H(r - 1, l) - H(r - 1, l - 1));
+        const Real a1 = r < 2 ? 0.0 : E(r - 2, nl - 1);
+        const Real a2 = r < 2 ? 0.0 : E(r - 2, nl);
+        E(r, nl) = -a1 + abc0 * (E(r, nl - 1) + a2) + abc1 * (E(r - 1, nl) + E(r - 1, nl - 1));
+        for (std::size_t l = 0; l < nl; ++l) H(r, l) = chih * H(r - 1, l) + chie * (E(r, l + 1) - E(r, l));
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      td.d.origin[a] = static_cast<int32_t>(org[a]);
+      td.d.size[a] = static_cast<int32_t>(sz[a]);
+      td.d.transform_e[a] = te[a];
+      td.d.transform_h[a] = th[a];
+      td.d.ca[a] = dt / (kEps0 * d[a]);  // tfsf.cpp:177-205
+      td.d.cb[a] = dt / (kMu0 * d[a]);
+    }
+    td.d.nt = static_cast<int64_t>(nt);
+    td.d.aux = static_cast<int64_t>(aux);
+    tfsf.push_back(std::move(td));
+  }
+
+  // -------------------------------------------------------------- monitors --
+  void monitorBox(const Shape& s, int32_t o[3], int32_t e[3]) const {
+    // Monitor::defaultInit (monitor.cpp:103-127) and the one-cell shift of a
+    // box starting at index 0 (field_monitor.cpp:24-67)
+    std::size_t so[3], se_[3];
+    gridBox(s, so, se_);
+    for (int a = 0; a < 3; ++a) {
+      o[a] = static_cast<int32_t>(so[a] == 0 ? 1 : so[a]);
+      e[a] = static_cast<int32_t>(se_[a]);
+    }
+  }
+
+  void buildMonitors() {
+    for (auto& m : monitors) {
+      GatherPlan gp;
+      gp.monitor = m.get();
+      if (auto* mv = dynamic_cast<MovieMonitor*>(m.get())) {
+        FieldMonitor& f = mv->frame();
+        gp.d.field = fieldId(f.field());
+        monitorBox(f.shape(), gp.d.origin, gp.d.end);
+        gp.d.first = 0;
+        gp.d.interval = static_cast<int64_t>(mv->frameInterval());
+        gp.d.count = steps == 0 ? 0 : static_cast<int64_t>((steps - 1) / mv->frameInterval() + 1);
+        gp.time_series = false;
+        mv->frames().clear();
+      } else if (auto* ft = dynamic_cast<FieldTimeMonitor*>(m.get())) {
+        gp.d.field = fieldId(ft->field());
+        int32_t o[3], e[3];
+        monitorBox(ft->cube(), o, e);
+        // field_time_monitor.cpp:81-95 keeps the last cell of the box
+        for (int a = 0; a < 3; ++a) {
+          gp.d.origin[a] = e[a] > o[a] ? e[a] - 1 : o[a];
+          gp.d.end[a] = e[a] > o[a] ? e[a] : o[a];
+        }
+        gp.d.first = 0;
+        gp.d.interval = 1;
+        gp.d.count = static_cast<int64_t>(steps);
+        gp.time_series = true;
+        ft->time() = ft->field() <= EMF::Field::EM ? eTime() : hTime();
+        ft->data().assign(steps, 0);
+      } else {
+        throw XFDTDException("unsupported monitor type");
+      }
+      gathers.push_back(gp);
+    }
+  }
+
+  static int fieldId(EMF::Field f) {
+    switch (f) {
+      case EMF::Field::EX: return 0;
+      case EMF::Field::EY: return 1;
+      case EMF::Field::EZ: return 2;
+      case EMF::Field::HX: return 3;
+      case EMF::Field::HY: return 4;
+      case EMF::Field::HZ: return 5;
+      default: throw XFDTDException("monitor field must be one component");
+    }
+  }
+
+  // ---------------------------------------------------------------- init --
+  void init(Index n) {
+    precompute(n);
+    createDevice();
+    initialised = true;
+  }
+
+  void precompute(Index n) {
+    initialised = false;
+    if (ctx) {
+      xfd_destroy(ctx);
+      ctx = nullptr;
+    }
+    slabs.clear();
+    lumped.clear();
+    tfsf.clear();
+    gathers.clear();
+    steps = n;
+    buildGrid();
+    buildMaterial();
+    buildCoefficients();
+    // Simulation::correctUpdateCoefficient (simulation.cpp:638-652):
+    // objects (lumped patches) in add order, then boundaries.
+    for (auto& o : objects)
+      if (auto* le = dynamic_cast<LumpedElement*>(o.get())) buildLumped(*le);
+    for (auto& b : boundaries)
+      if (auto* p = dynamic_cast<PML*>(b.get())) buildPml(*p);
+    for (auto& s : sources) {
+      auto* t = dynamic_cast<TFSF3D*>(s.get());
+      if (!t) throw XFDTDException("unsupported waveform source");
+      buildTfsf(*t);
+    }
+    buildMonitors();
+    coef_view.clear();
+  }
+
+  void createDevice() {
+    xfd_desc dd{};
+    dd.nx = static_cast<int32_t>(N[0]);
+    dd.ny = static_cast<int32_t>(N[1]);
+    dd.nz = static_cast<int32_t>(N[2]);
+    dd.dtype = precision == Precision::FP64 ? XFD_F64 : XFD_F32;
+    for (int s = 0; s < 18; ++s) dd.coeff[s] = coef[s].v.data();
+    std::vector<xfd_cpml_desc> cp;
+    for (auto& s : slabs) {
+      s.d.a_e = s.ae.data();
+      s.d.b_e = s.be.data();
+      s.d.a_h = s.ah.data();
+      s.d.b_h = s.bh.data();
+      s.d.c_ea_psi_hb = s.cpsi[0].data();
+      s.d.c_eb_psi_ha = s.cpsi[1].data();
+      s.d.c_ha_psi_eb = s.cpsi[2].data();
+      s.d.c_hb_psi_ea = s.cpsi[3].data();
+      cp.push_back(s.d);
+    }
+    std::vector<xfd_lumped_desc> lp;
+    for (auto& l : lumped) {
+      l.d.coff = l.coff.data();
+      l.d.nt = static_cast<int64_t>(l.wave.size());
+      l.d.waveform = l.wave.data();
+      lp.push_back(l.d);
+    }
+    std::vector<xfd_tfsf_desc> tp;
+    for (auto& t : tfsf) {
+      t.d.px_int = t.proj[0].data();
+      t.d.py_int = t.proj[1].data();
+      t.d.pz_int = t.proj[2].data();
+      t.d.px_half = t.proj[3].data();
+      t.d.py_half = t.proj[4].data();
+      t.d.pz_half = t.proj[5].data();
+      t.d.e_inc = t.e_inc.data();
+      t.d.h_inc = t.h_inc.data();
+      tp.push_back(t.d);
+    }
+    std::vector<xfd_gather_desc> gp;
+    for (auto& g : gathers) gp.push_back(g.d);
+    dd.n_cpml = static_cast<int32_t>(cp.size());
+    dd.cpml = cp.data();
+    dd.n_lumped = static_cast<int32_t>(lp.size());
+    dd.lumped = lp.data();
+    dd.n_tfsf = static_cast<int32_t>(tp.size());
+    dd.tfsf = tp.data();
+    dd.n_gather = static_cast<int32_t>(gp.size());
+    dd.gather = gp.data();
+    if (xfd_create(&dd, &ctx) != XFD_OK) throw XFDTDSimulationException(std::string("xfd_create: ") + xfd_last_error());
+    int64_t b = 0;
+    xfd_device_bytes(ctx, &b);
+    dev_bytes = b;
+  }
+
+  void runLoop() {
+    if (!initialised || !ctx) throw XFDTDSimulationException("Simulation is not initialised");
+    xfd_set_profiling(ctx, profiling ? 1 : 0);
+    hipless_timer_start();
+    if (xfd_step(ctx, 0, static_cast<int64_t>(steps)) != XFD_OK)
+      throw XFDTDSimulationException(std::string("xfd_step: ") + xfd_last_error());
+    loop_seconds = hipless_timer_stop();
+    stats.clear();
+    for (int k = 0;; ++k) {
+      const char* name = nullptr;
+      int64_t l = 0;
+      double ms = 0;
+      if (xfd_kernel_stats(ctx, k, &name, &l, &ms) != XFD_OK) break;
+      stats.push_back({name, static_cast<long long>(l), ms});
+    }
+    collectOutputs();
+  }
+
+  std::chrono::steady_clock::time_point t_start;
+  void hipless_timer_start() { t_start = std::chrono::steady_clock::now(); }
+  double hipless_timer_stop() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+
+  void collectOutputs() {
+    for (std::size_t gi = 0; gi < gathers.size(); ++gi) {
+      auto& g = gathers[gi];
+      const std::size_t n0 = g.d.end[0] - g.d.origin[0], n1 = g.d.end[1] - g.d.origin[1],
+                        n2 = g.d.end[2] - g.d.origin[2];
+      const std::size_t fe = n0 * n1 * n2;
+      std::vector<Real> buf(fe * static_cast<std::size_t>(g.d.count));
+      if (!buf.empty() && xfd_read_gather(ctx, static_cast<int32_t>(gi), buf.data()) != XFD_OK)
+        throw XFDTDSimulationException(std::string("xfd_read_gather: ") + xfd_last_error());
+      if (g.time_series) {
+        auto* ft = static_cast<FieldTimeMonitor*>(g.monitor);
+        if (fe == 1) ft->data() = buf;
+        if (!ft->outputDir().empty()) {  // time_monitor.cpp:11-21: stack(time, data)
+          std::vector<Real> st(ft->time());
+          st.insert(st.end(), ft->data().begin(), ft->data().end());
+          std::filesystem::create_directories(ft->outputDir());
+          npy::write(ft->outputDir() + "/" + ft->name() + ".npy", {2, ft->time().size()}, st.data());
+        }
+      } else {
+        auto* mv = static_cast<MovieMonitor*>(g.monitor);
+        auto& frames = mv->frames();
+        frames.clear();
+        for (int64_t f = 0; f < g.d.count; ++f) {
+          Array3D<Real> a(n0, n1, n2);
+          std::memcpy(a.data.data(), buf.data() + f * fe, fe * sizeof(Real));
+          char name[32];
+          std::snprintf(name, sizeof(name), "%05lld", static_cast<long long>(f * g.d.interval));
+          if (!mv->outputDir().empty() && fe > 0) {  // movie_monitor.cpp:26-34, monitor.cpp:42-54
+            std::filesystem::create_directories(mv->outputDir());
+            npy::write(mv->outputDir() + "/" + name + ".npy", {n0, n1, n2}, a.data.data());
+          }
+          if (f + 1 == g.d.count) mv->frame().data() = a;
+          frames.emplace_back(name, std::move(a));
+        }
+      }
+    }
+    // final fields (Simulation::emf)
+    Array3D<Real>* dst[6] = {&emf->ex, &emf->ey, &emf->ez, &emf->hx, &emf->hy, &emf->hz};
+    for (int f = 0; f < 6; ++f) {
+      std::size_t ext[3];
+      extent(f >= 3, f % 3, N, ext);
+      *dst[f] = Array3D<Real>(ext[0], ext[1], ext[2]);
+      if (xfd_read_field(ctx, f, dst[f]->data.data()) != XFD_OK)
+        throw XFDTDSimulationException(std::string("xfd_read_field: ") + xfd_last_error());
+    }
+  }
+};
+
+Simulation::Simulation(Real dx, Real dy, Real dz, Real cfl, ThreadConfig) : _impl(std::make_unique<Impl>()) {
+  _impl->d[0] = dx;
+  _impl->d[1] = dy;
+  _impl->d[2] = dz;
+  _impl->cfl = cfl;
+}
+Simulation::~Simulation() = default;
+void Simulation::addObject(std::shared_ptr<Object> o) { _impl->objects.push_back(std::move(o)); }
+void Simulation::addWaveformSource(std::shared_ptr<WaveformSource> s) { _impl->sources.push_back(std::move(s)); }
+void Simulation::addBoundary(std::shared_ptr<Boundary> b) { _impl->boundaries.push_back(std::move(b)); }
+void Simulation::addMonitor(std::shared_ptr<Monitor> m) { _impl->monitors.push_back(std::move(m)); }
+void Simulation::setPrecision(Precision p) { _impl->precision = p; }
+void Simulation::init(Index n) { _impl->init(n); }
+void Simulation::precompute(Index n) { _impl->precompute(n); }
+void Simulation::run() { _impl->runLoop(); }
+void Simulation::run(Index n) {
+  init(n);
+  run();
+}
+Real Simulation::dt() const { return _impl->dt; }
+std::array<Index, 3> Simulation::gridSize() const { return {_impl->N[0], _impl->N[1], _impl->N[2]}; }
+const std::shared_ptr<EMF>& Simulation::emf() const { return _impl->emf; }
+const Array3D<Real>& Simulation::coefficient(const std::string& name) const {
+  for (int s = 0; s < 18; ++s) {
+    if (name != kCoefNames[s]) continue;
+    auto it = _impl->coef_view.find(name);
+    if (it == _impl->coef_view.end()) {
+      const auto& f = _impl->coef[s];
+      Array3D<Real> a(f.n[0], f.n[1], f.n[2]);
+      a.data = f.v;
+      it = _impl->coef_view.emplace(name, std::move(a)).first;
+    }
+    return it->second;
+  }
+  throw XFDTDException("unknown coefficient " + name);
+}
+const std::vector<Real>& Simulation::eNode(int a) const { return _impl->e_node[a]; }
+const std::vector<Real>& Simulation::hSize(int a) const { return _impl->h_size[a]; }
+double Simulation::loopSeconds() const { return _impl->loop_seconds; }
+void Simulation::setProfiling(bool on) { _impl->profiling = on; }
+std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
+long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
+
+}  // namespace xfdtd

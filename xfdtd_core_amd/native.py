@@ -1,0 +1,181 @@
+"""ctypes binding of libxfdtd_core_amd.so.
+
+The library is built in-tree (``xfdtd_core_amd/build.py``) and loaded from
+``xfdtd_core_amd/lib``.  Nothing here computes fields: every step runs in the
+HIP kernels behind the C-ABI, and a missing library raises instead of falling
+back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+_LIB_PATH = _PKG / "lib" / "libxfdtd_core_amd.so"
+_lib: Optional[ctypes.CDLL] = None
+
+FIELD_IDS = {"EX": 0, "EY": 1, "EZ": 2, "HX": 3, "HY": 4, "HZ": 5}
+COEFF_NAMES = ("cexe", "cexhy", "cexhz", "ceye", "ceyhz", "ceyhx", "ceze", "cezhx", "cezhy",
+               "chxh", "chxey", "chxez", "chyh", "chyez", "chyex", "chzh", "chzex", "chzey")
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load_library() -> ctypes.CDLL:
+    """Loads the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise RuntimeError(f"{_LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(str(_LIB_PATH))
+    vp, i32, i64, dbl, cp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_char_p
+    p64 = ctypes.POINTER(ctypes.c_int64)
+    sig = {
+        "xfd_sim_last_error": (cp, []),
+        "xfd_sim_create": (vp, [cp]),
+        "xfd_sim_destroy": (None, [vp]),
+        "xfd_sim_set_precision": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_set_profiling": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_precompute": (ctypes.c_int, [vp, i64]),
+        "xfd_sim_init": (ctypes.c_int, [vp, i64]),
+        "xfd_sim_run_loop": (ctypes.c_int, [vp]),
+        "xfd_sim_run": (ctypes.c_int, [vp, i64]),
+        "xfd_sim_dt": (dbl, [vp]),
+        "xfd_sim_loop_seconds": (dbl, [vp]),
+        "xfd_sim_grid": (ctypes.c_int, [vp, p64]),
+        "xfd_sim_device_bytes": (i64, [vp]),
+        "xfd_sim_grid_array": (i64, [vp, i32, i32, vp]),
+        "xfd_sim_field": (ctypes.c_int, [vp, i32, p64, vp]),
+        "xfd_sim_coeff": (ctypes.c_int, [vp, cp, p64, vp]),
+        "xfd_sim_monitor_count": (ctypes.c_int, [vp]),
+        "xfd_sim_monitor_records": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_monitor_record": (ctypes.c_int, [vp, i32, i32, p64, vp]),
+        "xfd_sim_kernel_count": (ctypes.c_int, [vp]),
+        "xfd_sim_kernel_stat": (ctypes.c_int, [vp, i32, cp, i32, p64, ctypes.POINTER(ctypes.c_double)]),
+        "xfd_last_error": (cp, []),
+        "xfd_version": (cp, []),
+        "xfd_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
+        "xfd_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+        "xfd_destroy": (None, [vp]),
+        "xfd_step": (ctypes.c_int, [vp, i64, i64]),
+        "xfd_read_field": (ctypes.c_int, [vp, i32, vp]),
+        "xfd_write_field": (ctypes.c_int, [vp, i32, vp]),
+        "xfd_read_gather": (ctypes.c_int, [vp, i32, vp]),
+        "xfd_set_profiling": (ctypes.c_int, [vp, i32]),
+        "xfd_kernel_stats": (ctypes.c_int, [vp, i32, ctypes.POINTER(cp), p64, ctypes.POINTER(ctypes.c_double)]),
+        "xfd_device_bytes": (ctypes.c_int, [vp, p64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class Simulation:
+    """A scenario (spec text) run by the native library."""
+
+    def __init__(self, spec: str, precision: str = "f64"):
+        self._L = load_library()
+        self._h = self._L.xfd_sim_create(spec.encode())
+        if not self._h:
+            raise ValueError(self._L.xfd_sim_last_error().decode())
+        self.set_precision(precision)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise RuntimeError(self._L.xfd_sim_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.xfd_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_precision(self, precision: str):
+        if precision not in ("f64", "f32"):
+            raise ValueError("precision must be 'f64' or 'f32'")
+        self._check(self._L.xfd_sim_set_precision(self._h, 1 if precision == "f32" else 0))
+
+    def set_profiling(self, on: bool = True):
+        self._check(self._L.xfd_sim_set_profiling(self._h, int(on)))
+
+    def precompute(self, steps: int):
+        self._check(self._L.xfd_sim_precompute(self._h, steps))
+
+    def init(self, steps: int):
+        self._check(self._L.xfd_sim_init(self._h, steps))
+
+    def run_loop(self):
+        self._check(self._L.xfd_sim_run_loop(self._h))
+
+    def run(self, steps: int):
+        self._check(self._L.xfd_sim_run(self._h, steps))
+
+    @property
+    def dt(self) -> float:
+        return self._L.xfd_sim_dt(self._h)
+
+    @property
+    def loop_seconds(self) -> float:
+        return self._L.xfd_sim_loop_seconds(self._h)
+
+    def grid(self) -> Tuple[int, int, int]:
+        n = (ctypes.c_int64 * 3)()
+        self._check(self._L.xfd_sim_grid(self._h, n))
+        return tuple(n)
+
+    def device_bytes(self) -> int:
+        return int(self._L.xfd_sim_device_bytes(self._h))
+
+    def grid_array(self, which: int, axis: int) -> np.ndarray:
+        n = self._L.xfd_sim_grid_array(self._h, which, axis, None)
+        a = np.empty(n)
+        self._L.xfd_sim_grid_array(self._h, which, axis, a.ctypes.data)
+        return a
+
+    def _arr(self, fn, key) -> np.ndarray:
+        shape = (ctypes.c_int64 * 3)()
+        self._check(fn(self._h, key, shape, None))
+        a = np.empty(tuple(shape))
+        self._check(fn(self._h, key, shape, a.ctypes.data))
+        return a
+
+    def field(self, name: str) -> np.ndarray:
+        return self._arr(self._L.xfd_sim_field, FIELD_IDS[name])
+
+    def coeff(self, name: str) -> np.ndarray:
+        return self._arr(self._L.xfd_sim_coeff, name.encode())
+
+    def monitor_records(self, m: int) -> List[np.ndarray]:
+        out = []
+        for r in range(self._L.xfd_sim_monitor_records(self._h, m)):
+            shape = (ctypes.c_int64 * 3)()
+            self._check(self._L.xfd_sim_monitor_record(self._h, m, r, shape, None))
+            a = np.empty(tuple(shape))
+            self._check(self._L.xfd_sim_monitor_record(self._h, m, r, shape, a.ctypes.data))
+            out.append(a if shape[2] != 1 or shape[0] != 2 else a[:, :, 0])
+        return out
+
+    def monitor_count(self) -> int:
+        return self._L.xfd_sim_monitor_count(self._h)
+
+    def kernel_stats(self):
+        out = {}
+        for k in range(self._L.xfd_sim_kernel_count(self._h)):
+            name = ctypes.create_string_buffer(64)
+            launches = ctypes.c_int64()
+            ms = ctypes.c_double()
+            self._check(self._L.xfd_sim_kernel_stat(self._h, k, name, 64, ctypes.byref(launches), ctypes.byref(ms)))
+            out[name.value.decode()] = (launches.value, ms.value)
+        return out

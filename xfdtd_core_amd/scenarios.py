@@ -1,0 +1,96 @@
+"""The workloads the hot path is measured and checked on, built with the
+reference-shaped scenario API.
+
+``dielectric_sphere`` is the reference README example (README.md:73-122):
+63^3 cells including an 8-cell CPML, TFSF plane wave, Ex XZ movie.
+``inhomogeneous_box`` is the 384^3 box with 64 random dielectric spheres and a
+centred Ez dipole (a one-edge CurrentSource), in full size or scaled down for
+parity runs.  Random draws use numpy's PCG64 seeded per workload; the spec text
+carries the drawn numbers, so every consumer sees identical inputs.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import spec as S
+
+C0 = 1.0 / math.sqrt(8.854187817e-12 * 4 * math.pi * 1e-7)
+
+
+def dielectric_sphere(movie_dir: str = "", threads=(1, 1, 1), movie_interval: int = 10):
+    """README.md:73-122: dl=7.5 mm, 0.35 m air cube, sphere r=0.1 m (eps_r=3,
+    mu_r=2), TFSF at index 15, 6 x PML(8), CFL 0.9, Ex XZ movie every 10 steps."""
+    dl = 7.5e-3
+    s = S.Simulation(dl, dl, dl, 0.9, threads)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-0.175, -0.175, -0.175), S.Vector(0.35, 0.35, 0.35)),
+                         S.Material.createAir()))
+    s.addObject(S.Object("dielectric_sphere", S.Sphere(S.Vector(0, 0, 0), 0.1),
+                         S.Material("a", S.ElectroMagneticProperty(3, 2, 0, 0))))
+    l_min = dl * 20
+    tau = l_min / 6e8
+    t_0 = 4.5 * tau
+    s.addWaveformSource(S.TFSF3D(15, 15, 15, 0.0, 0.0, 0.0, S.Waveform.gaussian(tau, t_0)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(8, d))
+    s.addMonitor(S.MovieMonitor(
+        S.FieldMonitor(S.Cube(S.Vector(-0.175, 0, -0.175), S.Vector(0.35, dl, 0.35)), "EX", "", ""),
+        movie_interval, "movie", movie_dir))
+    return s
+
+
+def inhomogeneous_box(n_total: int = 384, pml: int = 12, n_spheres: int = 64,
+                      r_cells=(10, 40), dl: float = 1e-3, seed: int = 2028,
+                      snapshot_interval: int = 500, probes=(4, 4, 2),
+                      threads=(1, 1, 1), monitors: bool = True):
+    """Inhomogeneous box: n_total^3 cells including a `pml`-cell CPML, random
+    non-overlapping dielectric spheres (eps_r ~ U[1,6], sigma ~ U[0,0.05] S/m,
+    at least one cell clear of the CPML), a centred one-edge Ez current source
+    driven by a derivative-of-Gaussian, CFL 0.95.  Monitors: all six fields
+    every `snapshot_interval` steps and a lattice of Ez probes."""
+    rng = np.random.default_rng(seed)
+    n_in = n_total - 2 * pml
+    half = n_in * dl / 2
+    s = S.Simulation(dl, dl, dl, 0.95, threads)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    placed = []
+    tries = 0
+    while len(placed) < n_spheres and tries < 200000:
+        tries += 1
+        r = int(rng.integers(r_cells[0], r_cells[1] + 1))
+        lo, hi = -half + (r + 1) * dl, half - (r + 1) * dl
+        if hi <= lo:
+            continue
+        c = rng.uniform(lo, hi, size=3)
+        if any(np.linalg.norm(c - p) < (r + q) * dl for p, q in placed):
+            continue
+        placed.append((c, r))
+    for idx, (c, r) in enumerate(placed):
+        eps_r = float(rng.uniform(1.0, 6.0))
+        sigma = float(rng.uniform(0.0, 0.05))
+        s.addObject(S.Object(f"sphere{idx}", S.Sphere(S.Vector(*map(float, c)), r * dl),
+                             S.Material(f"m{idx}", S.ElectroMagneticProperty(eps_r, 1.0, sigma, 0.0))))
+    tau = 20 * dl / (2 * C0)
+    t0 = 4.5 * tau
+    s.addObject(S.CurrentSource("dipole", S.Cube(S.Vector(0.0, 0.0, 0.0), S.Vector(0.0, 0.0, dl)), "ZP",
+                                1e20, S.Waveform.gaussianDerivative(tau, t0)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(pml, d))
+    if monitors:
+        whole = S.Cube(S.Vector(-half - pml * dl, -half - pml * dl, -half - pml * dl),
+                       S.Vector(n_total * dl, n_total * dl, n_total * dl))
+        for f in S.FIELDS:
+            s.addMonitor(S.MovieMonitor(S.FieldMonitor(whole, f), snapshot_interval, f"snap_{f}", ""))
+        px, py, pz = probes
+        for a in range(px):
+            for b in range(py):
+                for c in range(pz):
+                    x = -half + (a + 1) * 2 * half / (px + 1)
+                    y = -half + (b + 1) * 2 * half / (py + 1)
+                    z = -half + (c + 1) * 2 * half / (pz + 1)
+                    s.addMonitor(S.FieldTimeMonitor(f"probe_{a}{b}{c}", S.Cube(S.Vector(x, y, z), S.Vector(dl, dl, dl)),
+                                                    "", "EZ"))
+    return s

@@ -1,0 +1,314 @@
+"""Scenario description mirroring the XFDTD_CORE C++ API, serialised as spec text.
+
+The classes carry the reference's names and argument meaning
+(``include/xfdtd/simulation/simulation.h:37-155``, ``object/object.h``,
+``shape/*.h``, ``material/material.h``, ``boundary/pml.h:19-21``,
+``waveform_source/tfsf_3d.h``, ``waveform/waveform.h``,
+``object/lumped_element/*.h``, ``monitor/*.h``).  ``Simulation.spec()``
+renders the scenario as line-oriented text, one directive per line:
+
+    sim DX DY DZ CFL
+    threads TX TY TZ
+    object NAME SHAPE... MATERIAL...
+        SHAPE    = cube OX OY OZ SX SY SZ | sphere CX CY CZ R
+                 | cylinder CX CY CZ R H AXIS
+        MATERIAL = plain EPS_R MU_R SIGMA_E SIGMA_M
+    pml THICKNESS DIR [ORDER SIGMA_RATIO ALPHA_MIN ALPHA_MAX KAPPA_MAX]
+    tfsf3d X Y Z THETA PHI PSI WAVEFORM...
+    current_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
+    voltage_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
+    resistor NAME OX OY OZ SX SY SZ AXIS R
+    movie FIELD OX OY OZ SX SY SZ INTERVAL NAME OUT_DIR
+    field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
+    WAVEFORM = gaussian TAU T0 AMP | cosine_modulated_gaussian TAU T0 F AMP
+             | sine F AMP | cosine F AMP | step T0 AMP
+             | gaussian_derivative TAU T0 AMP
+
+Numbers are written with ``repr`` so they round-trip exactly through
+``strtod``.  ``OUT_DIR`` of ``-`` means "keep in memory, write no file".
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+
+def _n(x: float) -> str:
+    return repr(float(x))
+
+
+@dataclass
+class Vector:
+    x: float
+    y: float
+    z: float
+
+    def tokens(self):
+        return [_n(self.x), _n(self.y), _n(self.z)]
+
+
+class Shape:
+    def tokens(self):  # pragma: no cover - interface
+        raise NotImplementedError
+
+
+@dataclass
+class Cube(Shape):
+    """``xfdtd::Cube(origin, size)`` (shape/cube.h)."""
+
+    origin: Vector
+    size: Vector
+
+    def tokens(self):
+        return ["cube"] + self.origin.tokens() + self.size.tokens()
+
+    def box_tokens(self):
+        return self.origin.tokens() + self.size.tokens()
+
+
+@dataclass
+class Sphere(Shape):
+    """``xfdtd::Sphere(center, radius)`` (shape/sphere.h)."""
+
+    center: Vector
+    radius: float
+
+    def tokens(self):
+        return ["sphere"] + self.center.tokens() + [_n(self.radius)]
+
+
+@dataclass
+class Cylinder(Shape):
+    """``xfdtd::Cylinder(center, radius, height, axis)`` (shape/cylinder.h)."""
+
+    center: Vector
+    radius: float
+    height: float
+    axis: str = "Z"
+
+    def tokens(self):
+        return ["cylinder"] + self.center.tokens() + [_n(self.radius), _n(self.height), self.axis]
+
+
+@dataclass
+class ElectroMagneticProperty:
+    """``ElectroMagneticProperty(epsilon_r, mu_r, sigma_e, sigma_m)`` (material.cpp:11-26)."""
+
+    epsilon_r: float = 1.0
+    mu_r: float = 1.0
+    sigma_e: float = 0.0
+    sigma_m: float = 0.0
+
+
+@dataclass
+class Material:
+    name: str
+    em_property: ElectroMagneticProperty = field(default_factory=ElectroMagneticProperty)
+
+    @staticmethod
+    def createAir(name: str = "air") -> "Material":
+        return Material(name, ElectroMagneticProperty(1, 1, 0, 0))
+
+    def tokens(self):
+        p = self.em_property
+        return ["plain", _n(p.epsilon_r), _n(p.mu_r), _n(p.sigma_e), _n(p.sigma_m)]
+
+
+@dataclass
+class Waveform:
+    """Source time function (waveform.cpp:80-101); sampled by the host."""
+
+    kind: str
+    params: List[float]
+    amplitude: float = 1.0
+
+    @staticmethod
+    def gaussian(tau, t_0, amplitude=1.0):
+        return Waveform("gaussian", [tau, t_0], amplitude)
+
+    @staticmethod
+    def cosineModulatedGaussian(tau, t_0, frequency, amplitude=1.0):
+        return Waveform("cosine_modulated_gaussian", [tau, t_0, frequency], amplitude)
+
+    @staticmethod
+    def sine(frequency, amplitude=1.0):
+        return Waveform("sine", [frequency], amplitude)
+
+    @staticmethod
+    def cosine(frequency, amplitude=1.0):
+        return Waveform("cosine", [frequency], amplitude)
+
+    @staticmethod
+    def step(t_0, amplitude=1.0):
+        return Waveform("step", [t_0], amplitude)
+
+    @staticmethod
+    def gaussianDerivative(tau, t_0, amplitude=1.0):
+        """-(t-t0)/tau * exp(-((t-t0)/tau)^2): a user functor (waveform.h:13)."""
+        return Waveform("gaussian_derivative", [tau, t_0], amplitude)
+
+    def tokens(self):
+        return [self.kind] + [_n(p) for p in self.params] + [_n(self.amplitude)]
+
+
+@dataclass
+class Object:
+    """``xfdtd::Object(name, shape, material)`` (object/object.h)."""
+
+    name: str
+    shape: Shape
+    material: Material
+
+    def line(self):
+        return " ".join(["object", self.name] + self.shape.tokens() + self.material.tokens())
+
+
+@dataclass
+class PML:
+    """``xfdtd::PML(thickness, direction, order=4, sigma_ratio=1, alpha_min=0,
+    alpha_max=0.05, kappa_max=10)`` (boundary/pml.h:19-21)."""
+
+    thickness: int
+    direction: str
+    order: int = 4
+    sigma_ratio: float = 1.0
+    alpha_min: float = 0.0
+    alpha_max: float = 0.05
+    kappa_max: float = 10.0
+
+    def line(self):
+        return " ".join(["pml", str(int(self.thickness)), self.direction, str(int(self.order)),
+                         _n(self.sigma_ratio), _n(self.alpha_min), _n(self.alpha_max),
+                         _n(self.kappa_max)])
+
+
+@dataclass
+class TFSF3D:
+    """``xfdtd::TFSF3D(x, y, z, theta, phi, psi, waveform)`` (waveform_source/tfsf_3d.h)."""
+
+    x: int
+    y: int
+    z: int
+    theta: float
+    phi: float
+    psi: float
+    waveform: Waveform
+
+    def line(self):
+        return " ".join(["tfsf3d", str(int(self.x)), str(int(self.y)), str(int(self.z)),
+                         _n(self.theta), _n(self.phi), _n(self.psi)] + self.waveform.tokens())
+
+
+@dataclass
+class CurrentSource:
+    """``CurrentSource(name, cube, direction, resistance, waveform)`` (current_source.h:11-14)."""
+
+    name: str
+    cube: Cube
+    direction: str
+    resistance: float
+    waveform: Waveform
+
+    def line(self):
+        return " ".join(["current_source", self.name] + self.cube.box_tokens() +
+                        [self.direction, _n(self.resistance)] + self.waveform.tokens())
+
+
+@dataclass
+class VoltageSource:
+    """``VoltageSource(name, cube, direction, resistance, waveform)`` (voltage_source.h)."""
+
+    name: str
+    cube: Cube
+    direction: str
+    resistance: float
+    waveform: Waveform
+
+    def line(self):
+        return " ".join(["voltage_source", self.name] + self.cube.box_tokens() +
+                        [self.direction, _n(self.resistance)] + self.waveform.tokens())
+
+
+@dataclass
+class Resistor:
+    """``Resistor(name, cube, xyz, resistance)`` (resistor.h)."""
+
+    name: str
+    cube: Cube
+    xyz: str
+    resistance: float
+
+    def line(self):
+        return " ".join(["resistor", self.name] + self.cube.box_tokens() +
+                        [self.xyz, _n(self.resistance)])
+
+
+@dataclass
+class FieldMonitor:
+    """``FieldMonitor(shape, field, name, output_dir)`` (monitor/field_monitor.h)."""
+
+    shape: Cube
+    field: str
+    name: str = ""
+    output_dir: str = ""
+
+
+@dataclass
+class MovieMonitor:
+    """``MovieMonitor(frame, frame_interval, name, output_dir)`` (monitor/movie_monitor.h)."""
+
+    frame: FieldMonitor
+    frame_interval: int
+    name: str = "movie"
+    output_dir: str = ""
+
+    def line(self):
+        return " ".join(["movie", self.frame.field] + self.frame.shape.box_tokens() +
+                        [str(int(self.frame_interval)), self.name or "movie",
+                         self.output_dir or "-"])
+
+
+@dataclass
+class FieldTimeMonitor:
+    """``FieldTimeMonitor(name, cube, output_dir, field)`` (monitor/field_time_monitor.h)."""
+
+    name: str
+    cube: Cube
+    output_dir: str
+    field: str
+
+    def line(self):
+        return " ".join(["field_time", self.name, self.field] + self.cube.box_tokens() +
+                        [self.output_dir or "-"])
+
+
+class Simulation:
+    """``xfdtd::Simulation(dx, dy, dz, cfl, ThreadConfig)`` (simulation.h:39-40)."""
+
+    def __init__(self, dx, dy, dz, cfl, thread_config=(1, 1, 1)):
+        self.dx, self.dy, self.dz, self.cfl = dx, dy, dz, cfl
+        self.thread_config = tuple(thread_config)
+        self._lines: List[str] = []
+        self.monitors: List[object] = []
+
+    def addObject(self, obj):
+        self._lines.append(obj.line())
+
+    def addWaveformSource(self, src):
+        self._lines.append(src.line())
+
+    def addBoundary(self, b):
+        self._lines.append(b.line())
+
+    def addMonitor(self, m):
+        self._lines.append(m.line())
+        self.monitors.append(m)
+
+    def spec(self) -> str:
+        head = [f"sim {_n(self.dx)} {_n(self.dy)} {_n(self.dz)} {_n(self.cfl)}",
+                "threads " + " ".join(str(int(t)) for t in self.thread_config)]
+        return "\n".join(head + self._lines) + "\n"
+
+
+FIELDS = ("EX", "EY", "EZ", "HX", "HY", "HZ")
