@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the FDTD time-stepping loop on one MI355X.
+
+Workload: the 384^3 inhomogeneous box (12-cell CPML on all faces, 64 random
+dielectric spheres, centred Ez current-source dipole, CFL 0.95, full-field
+snapshots every 500 steps and 32 Ez probes), synthetic inputs drawn from a
+fixed seed.  A "step" is one full time step of the hot path (Yee H, CPML H,
+Yee E, source and CPML E corrections, recordings).  Inputs are resident in
+HBM when the timed region starts; the host precompute and the upload are
+outside it, as Simulation::init is outside the reference's update timer.
+
+Prints ONE JSON line: `value` is FP32 Mcells/s (the headline form); the FP64
+form, measured in the same run on the same scenario, is reported under
+"fp64".  `roofline` is for the dominant kernel, from HIP-event launch
+durations measured on the library's stream over the timed region;
+`cpu_baseline` times the oracle (the FP64 CPU restatement of the reference
+path) on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import gc
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "Mcells/s + HBM GB/s, FP32 vs FP64 on 1 MI355X at stated field error"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int) -> float:
+    """Algorithmic HBM bytes of one launch of `kernel` on an n^3 grid.
+
+    update_h / update_e: per cell 3 own-field reads + 3 writes, 3 dual-field
+    reads (neighbours reused) and 9 coefficient reads = 18 * s bytes.
+    cpml_h / cpml_e (one slab): per slab cell and component psi read + write,
+    c_psi read, field read + write and one new dual-field read = 6 * s,
+    two components per slab."""
+    cells = n ** 3
+    if kernel in ("update_h", "update_e"):
+        return 18.0 * s * cells
+    if kernel in ("cpml_h", "cpml_e"):
+        return 2 * 6.0 * s * pml * (n + 1) * (n + 1)
+    return 0.0
+
+
+def step_alg_bytes(n: int, pml: int, s: int) -> float:
+    """Compulsory bytes of one whole step: fused Yee (6 field reads, 6 writes,
+    18 coefficient reads per cell = 30 s) plus CPML (12 s per slab cell per
+    slab: 4 components x psi read/write + c_psi read)."""
+    return 30.0 * s * n ** 3 + 12.0 * s * 6 * pml * n * n
+
+
+def run_device(sim, precision, warmup, steps, n, pml):
+    import torch
+
+    log(f"{precision}: init (precompute + upload)")
+    sim.set_precision(precision)
+    t0 = time.perf_counter()
+    sim.init(warmup + steps)
+    init_s = time.perf_counter() - t0
+    log(f"{precision}: init {init_s:.1f} s, device bytes {sim.device_bytes() / 2**30:.2f} GiB; warmup {warmup}")
+    sim.set_profiling(False)
+    sim.advance(0, warmup)
+    sim.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(warmup, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stats = sim.kernel_stats()
+    s = 4 if precision == "f32" else 8
+    cells = n ** 3
+    kernels = {}
+    for name, (launches, ms) in stats.items():
+        if launches == 0:
+            continue
+        avg_ms = ms / launches
+        b = alg_bytes_per_launch(name, n, pml, s)
+        kernels[name] = {"launches": launches, "avg_ms": round(avg_ms, 5), "total_ms": round(ms, 3),
+                         "alg_GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    ach = kernels[dom]["alg_GBps"]
+    res = {
+        "mcells_per_s": cells * steps / wall / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "init_s": round(init_s, 2),
+        "step_alg_GBps": round(step_alg_bytes(n, pml, s) * steps / wall / 1e9, 1),
+        "kernels": kernels,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4) if ach else None, "traffic": None},
+    }
+    log(f"{precision}: {res['mcells_per_s']:.0f} Mcells/s, {res['ms_per_step']:.3f} ms/step, dominant {dom} "
+        f"{ach} GB/s")
+    return res
+
+
+def cpu_baseline(spec, n, steps):
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle import RefSim  # CPU restatement of the reference path
+
+    cores = max(1, min(16, os.cpu_count() or 1))
+    log(f"cpu baseline: oracle init on {n}^3 (untimed), then {steps} steps on {cores} threads")
+    ref = RefSim(spec, threads=(1, 1, cores))
+    ref.init(steps)
+    ref.loop()
+    t = ref.loop_seconds
+    del ref
+    gc.collect()
+    return {"value": round(n ** 3 * steps / t / 1e6, 2), "unit": "Mcells/s", "cores": cores, "kind": "port",
+            "sample": f"same {n}^3 scenario, {steps} FP64 steps of the oracle (ThreadConfig{{1,1,{cores}}}), "
+                      f"loop time only ({t:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=384, help="cells per axis including the CPML")
+    ap.add_argument("--pml", type=int, default=12)
+    ap.add_argument("--precision", choices=["both", "f32", "f64"], default="both")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs a single domain on one MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    from xfdtd_core_amd import native, scenarios
+
+    spec = scenarios.inhomogeneous_box(n_total=args.n, pml=args.pml, snapshot_interval=500).spec()
+    sim = native.Simulation(spec)
+    out = {}
+    order = ["f64", "f32"] if args.precision == "both" else [args.precision]
+    for p in order:
+        out[p] = run_device(sim, p, args.warmup, args.steps, args.n, args.pml)
+    sim.close()
+    del sim
+    gc.collect()
+
+    head = out.get("f32") or out["f64"]
+    line = {
+        "metric": METRIC,
+        "value": round(head["mcells_per_s"], 1),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if "f32" in out else "f64",
+        "data": "synthetic (seeded random spheres; no dataset)",
+        "config": {"workload": f"inhomogeneous box {args.n}^3 incl. {args.pml}-cell CPML, 64 dielectric spheres, "
+                               f"Ez current-source dipole, CFL 0.95, snapshots every 500 steps + 32 Ez probes",
+                   "cells": args.n ** 3, "parallelism": "single domain, 1 GPU"},
+        "hbm_GBps_alg_step": head["step_alg_GBps"],
+        "roofline": head["roofline"],
+        "kernels": head["kernels"],
+    }
+    if "f64" in out and "f32" in out:
+        f64 = out["f64"]
+        line["fp64"] = {"value": round(f64["mcells_per_s"], 1), "unit": "Mcells/s",
+                        "ms_per_step": round(f64["ms_per_step"], 4), "hbm_GBps_alg_step": f64["step_alg_GBps"],
+                        "roofline": f64["roofline"], "kernels": f64["kernels"]}
+        line["fp32_over_fp64"] = round(out["f32"]["mcells_per_s"] / f64["mcells_per_s"], 3)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(spec, args.n, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

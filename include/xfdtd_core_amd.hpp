@@ -379,6 +379,11 @@ class Simulation {
   void precompute(Index time_step);
   void run(Index time_step);   // simulation.cpp:284-289
   void run();                  // the time loop only, after init()
+  // The loop in pieces: advance(t0, n) runs steps t0..t0+n-1 on the GPU and
+  // returns when they are done; finish() then copies monitor recordings and
+  // final fields back.  run() == advance(0, steps) + finish().
+  void advance(Index t0, Index n);
+  void finish();
 
   Real dt() const;
   std::array<Index, 3> gridSize() const;

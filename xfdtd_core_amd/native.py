@@ -47,6 +47,8 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_init": (ctypes.c_int, [vp, i64]),
         "xfd_sim_run_loop": (ctypes.c_int, [vp]),
         "xfd_sim_run": (ctypes.c_int, [vp, i64]),
+        "xfd_sim_advance": (ctypes.c_int, [vp, i64, i64]),
+        "xfd_sim_finish": (ctypes.c_int, [vp]),
         "xfd_sim_dt": (dbl, [vp]),
         "xfd_sim_loop_seconds": (dbl, [vp]),
         "xfd_sim_grid": (ctypes.c_int, [vp, p64]),
@@ -78,6 +80,112 @@ def load_library() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = L
     return L
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class CpmlDesc(ctypes.Structure):
+    """xfd_cpml_desc (include/xfdtd_core_amd.h)."""
+
+    _fields_ = [("axis", ctypes.c_int32), ("n", ctypes.c_int32), ("e_start", ctypes.c_int32),
+                ("h_start", ctypes.c_int32), ("a_e", _dp), ("b_e", _dp), ("a_h", _dp), ("b_h", _dp),
+                ("c_ea_psi_hb", _dp), ("c_eb_psi_ha", _dp), ("c_ha_psi_eb", _dp), ("c_hb_psi_ea", _dp)]
+
+
+class TfsfDesc(ctypes.Structure):
+    """xfd_tfsf_desc."""
+
+    _fields_ = [("origin", ctypes.c_int32 * 3), ("size", ctypes.c_int32 * 3), ("px_int", _dp), ("py_int", _dp),
+                ("pz_int", _dp), ("px_half", _dp), ("py_half", _dp), ("pz_half", _dp), ("nt", ctypes.c_int64),
+                ("aux", ctypes.c_int64), ("e_inc", _dp), ("h_inc", _dp), ("transform_e", ctypes.c_double * 3),
+                ("transform_h", ctypes.c_double * 3), ("ca", ctypes.c_double * 3), ("cb", ctypes.c_double * 3)]
+
+
+class LumpedDesc(ctypes.Structure):
+    """xfd_lumped_desc."""
+
+    _fields_ = [("component", ctypes.c_int32), ("origin", ctypes.c_int32 * 3), ("size", ctypes.c_int32 * 3),
+                ("coff", _dp), ("nt", ctypes.c_int64), ("waveform", _dp)]
+
+
+class GatherDesc(ctypes.Structure):
+    """xfd_gather_desc."""
+
+    _fields_ = [("field", ctypes.c_int32), ("origin", ctypes.c_int32 * 3), ("end", ctypes.c_int32 * 3),
+                ("first", ctypes.c_int64), ("interval", ctypes.c_int64), ("count", ctypes.c_int64)]
+
+
+class Desc(ctypes.Structure):
+    """xfd_desc."""
+
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("coeff", _dp * 18), ("n_cpml", ctypes.c_int32), ("cpml", ctypes.POINTER(CpmlDesc)),
+                ("n_lumped", ctypes.c_int32), ("lumped", ctypes.POINTER(LumpedDesc)),
+                ("n_tfsf", ctypes.c_int32), ("tfsf", ctypes.POINTER(TfsfDesc)),
+                ("n_gather", ctypes.c_int32), ("gather", ctypes.POINTER(GatherDesc))]
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """The hot-path C-ABI (xfd_create / xfd_step / xfd_read_field) driven
+    with coefficient arrays supplied by the caller (no CPML, TFSF or lumped
+    elements): the Yee loop of BasicUpdator3D alone."""
+
+    def __init__(self, coeffs, dtype: str = "f64", gathers=()):
+        self._L = load_library()
+        self._keep = [np.ascontiguousarray(coeffs[n], dtype=np.float64) for n in COEFF_NAMES]
+        d = Desc()
+        shape_ex = self._keep[0].shape
+        d.nx, d.ny, d.nz = shape_ex[0], shape_ex[1] - 1, shape_ex[2] - 1
+        d.dtype = 0 if dtype == "f64" else 1
+        for i, a in enumerate(self._keep):
+            d.coeff[i] = _ptr(a)
+        gl = (GatherDesc * max(1, len(gathers)))()
+        for i, g in enumerate(gathers):
+            gl[i] = g
+        d.n_gather = len(gathers)
+        d.gather = ctypes.cast(gl, ctypes.POINTER(GatherDesc))
+        self._gl = gl
+        self.shape = (d.nx, d.ny, d.nz)
+        h = ctypes.c_void_p()
+        if self._L.xfd_create(ctypes.byref(d), ctypes.byref(h)) != 0:
+            raise RuntimeError(self._L.xfd_last_error().decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._L.xfd_last_error().decode())
+
+    def extent(self, field: str):
+        n = list(self.shape)
+        c = FIELD_IDS[field] % 3
+        h = FIELD_IDS[field] >= 3
+        return tuple((n[a] + 1 if a == c else n[a]) if h else (n[a] if a == c else n[a] + 1) for a in range(3))
+
+    def write(self, field: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.extent(field)
+        self._check(self._L.xfd_write_field(self._h, FIELD_IDS[field], a.ctypes.data))
+
+    def read(self, field: str) -> np.ndarray:
+        a = np.empty(self.extent(field))
+        self._check(self._L.xfd_read_field(self._h, FIELD_IDS[field], a.ctypes.data))
+        return a
+
+    def step(self, t0: int, n: int):
+        self._check(self._L.xfd_step(self._h, t0, n))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.xfd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Simulation:
@@ -121,6 +229,14 @@ class Simulation:
 
     def run(self, steps: int):
         self._check(self._L.xfd_sim_run(self._h, steps))
+
+    def advance(self, t0: int, n: int):
+        """Runs steps t0 .. t0+n-1 on the GPU; returns when they are done."""
+        self._check(self._L.xfd_sim_advance(self._h, t0, n))
+
+    def finish(self):
+        """Copies monitor recordings and final fields back."""
+        self._check(self._L.xfd_sim_finish(self._h))
 
     @property
     def dt(self) -> float:
