@@ -62,7 +62,7 @@ def step_alg_bytes(n: int, pml: int, s: int) -> float:
 
 
 def run_device(sim, precision, warmup, steps, n, pml):
-    import torch
+    from xfdtd_core_amd import native
 
     log(f"{precision}: init (precompute + upload)")
     sim.set_precision(precision)
@@ -73,10 +73,12 @@ def run_device(sim, precision, warmup, steps, n, pml):
     sim.set_profiling(False)
     sim.advance(0, warmup)
     sim.set_profiling(True)
-    torch.cuda.synchronize()
+    # the library's step call returns after its stream drains; the device
+    # barrier on both sides brackets the timed region
+    native.device_synchronize()
     t0 = time.perf_counter()
     sim.advance(warmup, steps)
-    torch.cuda.synchronize()
+    native.device_synchronize()
     wall = time.perf_counter() - t0
     stats = sim.kernel_stats()
     s = 4 if precision == "f32" else 8
@@ -142,6 +144,8 @@ def main():
     from xfdtd_core_amd import native, scenarios
 
     spec = scenarios.inhomogeneous_box(n_total=args.n, pml=args.pml, snapshot_interval=500).spec()
+    dev_name, dev_bytes = native.device_info()
+    log(f"device: {dev_name}, {dev_bytes / 2**30:.0f} GiB")
     sim = native.Simulation(spec)
     out = {}
     order = ["f64", "f32"] if args.precision == "both" else [args.precision]
@@ -167,7 +171,7 @@ def main():
         "data": "synthetic (seeded random spheres; no dataset)",
         "config": {"workload": f"inhomogeneous box {args.n}^3 incl. {args.pml}-cell CPML, 64 dielectric spheres, "
                                f"Ez current-source dipole, CFL 0.95, snapshots every 500 steps + 32 Ez probes",
-                   "cells": args.n ** 3, "parallelism": "single domain, 1 GPU"},
+                   "cells": args.n ** 3, "parallelism": "single domain, 1 GPU", "device": dev_name},
         "hbm_GBps_alg_step": head["step_alg_GBps"],
         "roofline": head["roofline"],
         "kernels": head["kernels"],

@@ -104,6 +104,10 @@ typedef struct xfd_ctx xfd_ctx;
 const char* xfd_last_error(void);
 const char* xfd_version(void);
 int xfd_device_count(int32_t* count);
+/* Waits for all work on the current device (hipDeviceSynchronize). */
+int xfd_device_synchronize(void);
+/* Name of the current device and its HBM capacity in bytes. */
+int xfd_device_info(char* name, int32_t name_len, int64_t* total_bytes);
 
 /* Replaces Simulation::generateDomain + the Domain objects
  * (src/simulation/simulation.cpp:401-494). */

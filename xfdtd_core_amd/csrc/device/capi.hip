@@ -600,6 +600,21 @@ int xfd_device_count(int32_t* count) {
   });
 }
 
+int xfd_device_synchronize(void) {
+  return guard([&] { XFD_HIP(hipDeviceSynchronize()); });
+}
+
+int xfd_device_info(char* name, int32_t name_len, int64_t* total_bytes) {
+  return guard([&] {
+    int dev = 0;
+    XFD_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    XFD_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) std::snprintf(name, static_cast<size_t>(name_len), "%s (%s)", p.name, p.gcnArchName);
+    if (total_bytes) *total_bytes = static_cast<int64_t>(p.totalGlobalMem);
+  });
+}
+
 int xfd_create(const xfd_desc* d, xfd_ctx** out) {
   return guard([&] {
     require(d != nullptr && out != nullptr, "null argument");

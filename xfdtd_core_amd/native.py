@@ -64,6 +64,8 @@ def load_library() -> ctypes.CDLL:
         "xfd_last_error": (cp, []),
         "xfd_version": (cp, []),
         "xfd_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
+        "xfd_device_synchronize": (ctypes.c_int, []),
+        "xfd_device_info": (ctypes.c_int, [cp, i32, p64]),
         "xfd_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
         "xfd_destroy": (None, [vp]),
         "xfd_step": (ctypes.c_int, [vp, i64, i64]),
@@ -80,6 +82,29 @@ def load_library() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = L
     return L
+
+
+def device_count() -> int:
+    n = ctypes.c_int32(0)
+    if load_library().xfd_device_count(ctypes.byref(n)) != 0:
+        return 0
+    return int(n.value)
+
+
+def device_synchronize():
+    L = load_library()
+    if L.xfd_device_synchronize() != 0:
+        raise RuntimeError(L.xfd_last_error().decode())
+
+
+def device_info():
+    """(name, HBM bytes) of the current device."""
+    L = load_library()
+    name = ctypes.create_string_buffer(128)
+    total = ctypes.c_int64(0)
+    if L.xfd_device_info(name, 128, ctypes.byref(total)) != 0:
+        raise RuntimeError(L.xfd_last_error().decode())
+    return name.value.decode(), int(total.value)
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
