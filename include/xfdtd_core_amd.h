@@ -137,6 +137,12 @@ int xfd_kernel_stats(xfd_ctx* ctx, int32_t k, const char** name, int64_t* launch
 /* Device bytes held by the context. */
 int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
 
+/* Coefficient storage chosen by xfd_create: *classed = 1 when every
+ * component's distinct (self, dual-A, dual-B) triples fit a 16-bit class id
+ * (classes[q] = table size of Ex..Hz), 0 for the 18 dense arrays.  The
+ * environment variable XFD_COEF_MODE=dense forces the dense layout. */
+int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
+
 #ifdef __cplusplus
 }
 #endif

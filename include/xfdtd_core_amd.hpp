@@ -403,6 +403,9 @@ class Simulation {
   void setProfiling(bool on);
   std::vector<KernelStat> kernelStats() const;
   long long deviceBytes() const;
+  // True when the device holds per-cell coefficient class ids instead of the
+  // 18 dense coefficient arrays (same values; chosen at init).
+  bool classedCoefficients() const;
 
   struct Impl;
 

@@ -40,6 +40,8 @@ double xfd_sim_dt(void* sim);
 double xfd_sim_loop_seconds(void* sim);
 int xfd_sim_grid(void* sim, int64_t* n3);
 int64_t xfd_sim_device_bytes(void* sim);
+/* 1 when the device holds coefficient class ids (xfd_coef_layout), else 0. */
+int32_t xfd_sim_coef_classed(void* sim);
 /* which = 0: e-node coordinates, otherwise h sizes; returns the length. */
 int64_t xfd_sim_grid_array(void* sim, int32_t which, int32_t axis, double* dst);
 /* Arrays of the reference extent; dst may be NULL to query the shape. */

@@ -204,23 +204,40 @@ void Object::init(Sim& s) {
 void Object::correctMaterialSpace(Sim& s, Index index) {
   // object.cpp:114-198: every cell whose centre (h-node) is inside the shape
   // takes the object's properties; later objects overwrite earlier ones.
+  // Each cell is tested and written independently, so the i planes are
+  // split across host threads (the reference's XFDTD_CORE_PSTL_ENABLE
+  // branch is the same per-cell lambda under std::execution::par_unseq).
   material_index = index;
   const Real eps = s.g.eps();
   const Index nx = s.g.nx(), ny = s.g.ny(), nz = s.g.nz();
   auto& mp = s.mp;
-  for (Index i = 0; i < nx; ++i)
-    for (Index j = 0; j < ny; ++j)
-      for (Index k = 0; k < nz; ++k) {
-        Vec3 p{s.g.h_node[0](i), s.g.h_node[1](j), s.g.h_node[2](k)};
-        if (!shape->isInside(p, eps)) continue;
-        s.material_grid[(i * ny + j) * nz + k] = index;
-        for (int a = 0; a < 3; ++a) {
-          mp.eps[a](i, j, k) = material.eps;
-          mp.mu[a](i, j, k) = material.mu;
-          mp.se[a](i, j, k) = material.sigma_e;
-          mp.sm[a](i, j, k) = material.sigma_m;
+  auto planes = [&](Index i0, Index i1) {
+    for (Index i = i0; i < i1; ++i)
+      for (Index j = 0; j < ny; ++j)
+        for (Index k = 0; k < nz; ++k) {
+          Vec3 p{s.g.h_node[0](i), s.g.h_node[1](j), s.g.h_node[2](k)};
+          if (!shape->isInside(p, eps)) continue;
+          s.material_grid[(i * ny + j) * nz + k] = index;
+          for (int a = 0; a < 3; ++a) {
+            mp.eps[a](i, j, k) = material.eps;
+            mp.mu[a](i, j, k) = material.mu;
+            mp.se[a](i, j, k) = material.sigma_e;
+            mp.sm[a](i, j, k) = material.sigma_m;
+          }
         }
-      }
+  };
+  const Index parts = std::min<Index>(nx, std::max(1u, std::min(32u, std::thread::hardware_concurrency())));
+  if (parts <= 1 || nx * ny * nz < (Index(1) << 18)) {
+    planes(0, nx);
+    return;
+  }
+  std::vector<std::thread> th;
+  const Index step = (nx + parts - 1) / parts;
+  for (Index p = 0; p < parts; ++p) {
+    const Index i0 = p * step, i1 = std::min(nx, i0 + step);
+    if (i0 < i1) th.emplace_back(planes, i0, i1);
+  }
+  for (auto& t : th) t.join();
 }
 
 Index LumpedElement::nodeCount(int a) const {
@@ -318,6 +335,125 @@ void Resistor::correctUpdateCoefficient(Sim& s) {  // resistor.cpp:85-127
   for (Index n = 0; n < beta.size(); ++n)
     beta.v[n] = s.dt * dc.v[n] / (rf * da.v[n] * db.v[n]);
   lumpedPatch(s, *this, beta, da, db, nullptr, false, 0, nullptr);
+}
+
+void Capacitor::init(Sim& s) {  // capacitor.cpp:16-59
+  LumpedElement::init(s);
+  // cf(c, na, nb, nc) = c * nc / (nc * nb), the count product in integers
+  const Index na = globalCountSubA(), nb = globalCountSubB(), nc = globalCountMain();
+  (void)na;
+  cf = capacitance * nc / (nc * nb);
+  sizes(s, da, db, dc);
+  beta = Arr3(dc.n0, dc.n1, dc.n2);
+  for (Index n = 0; n < beta.size(); ++n) beta.v[n] = 2 * cf * dc.v[n] / (da.v[n] * db.v[n]);
+}
+
+void Capacitor::correctUpdateCoefficient(Sim& s) {  // capacitor.cpp:61-112
+  const Real dt = s.dt;
+  const int c = xyz;
+  Arr3& cece = s.c.self(0, c);
+  Arr3& cecha = s.c.dualA(0, c);
+  Arr3& cechb = s.c.dualB(0, c);
+  const Arr3& eps = s.mp.eps[c];
+  const Arr3& sig = s.mp.se[c];
+  Range r[3] = {range(0), range(1), range(2)};
+  for (Index i = 0; i < r[0].size(); ++i)
+    for (Index j = 0; j < r[1].size(); ++j)
+      for (Index k = 0; k < r[2].size(); ++k) {
+        const Index gi = r[0].s + i, gj = r[1].s + j, gk = r[2].s + k;
+        const Real e = eps(gi, gj, gk), sg = sig(gi, gj, gk), b = beta(i, j, k);
+        cece(gi, gj, gk) = (2 * e - dt * sg + b) / (2 * e + dt * sg + b);
+        cecha(gi, gj, gk) = -2 * dt / ((2 * e + dt * sg + b) * db(i, j, k));
+        cechb(gi, gj, gk) = 2 * dt / ((2 * e + dt * sg + b) * da(i, j, k));
+      }
+}
+
+void Inductor::init(Sim& s) {  // inductor.cpp:23-74
+  LumpedElement::init(s);
+  lf = inductance * globalCountSubA() * globalCountSubB() / globalCountMain();
+  sizes(s, da, db, dc);
+  cjcec = Arr3(dc.n0, dc.n1, dc.n2);
+  for (Index n = 0; n < cjcec.size(); ++n) cjcec.v[n] = s.dt * dc.v[n] / (lf * da.v[n] * db.v[n]);
+  j = Arr3(nodeCount(0), nodeCount(1), nodeCount(2));
+}
+
+void Inductor::correctUpdateCoefficient(Sim& s) {  // inductor.cpp:76-105
+  const Real dt = s.dt;
+  const int c = xyz;
+  const Arr3& eps = s.mp.eps[c];
+  const Arr3& sig = s.mp.se[c];
+  Range r[3] = {range(0), range(1), range(2)};
+  cecjc = Arr3(r[0].size(), r[1].size(), r[2].size());
+  for (Index i = 0; i < r[0].size(); ++i)
+    for (Index jj = 0; jj < r[1].size(); ++jj)
+      for (Index k = 0; k < r[2].size(); ++k) {
+        const Index gi = r[0].s + i, gj = r[1].s + jj, gk = r[2].s + k;
+        cecjc(i, jj, k) = -(2 * dt) / (2 * eps(gi, gj, gk) + dt * sig(gi, gj, gk));
+      }
+}
+
+namespace {
+// InductorCorrector::correctE (lumped_element_corrrector.cpp:40-99): per
+// edge in (i, j, k) order, E += cecjc * J, then J += cjcec * E.
+struct InductorCorrector : Corrector {
+  Sim* s;
+  Inductor* el;
+  Task task, local;
+  void correctE(Index) override {
+    Arr3& e = s->emf.f(0, el->xyz);
+    for (Index i = 0; i < task.x.size(); ++i)
+      for (Index j = 0; j < task.y.size(); ++j)
+        for (Index k = 0; k < task.z.size(); ++k) {
+          const Index li = local.x.s + i, lj = local.y.s + j, lk = local.z.s + k;
+          Real& ev = e(task.x.s + i, task.y.s + j, task.z.s + k);
+          ev += el->cecjc(li, lj, lk) * el->j(li, lj, lk);
+          el->j(li, lj, lk) += el->cjcec(li, lj, lk) * ev;
+        }
+  }
+  void correctH(Index) override {}
+};
+}  // namespace
+
+std::unique_ptr<Corrector> Inductor::generateCorrector(Sim& s, const Task& t) {  // inductor.cpp:111-135
+  Task dom = indexTask();
+  if (!intersected(t, dom)) return nullptr;
+  auto in = intersection(t, dom);
+  if (!in) return nullptr;
+  auto c = std::make_unique<InductorCorrector>();
+  c->s = &s;
+  c->el = this;
+  c->task = *in;
+  c->local = {{in->x.s - dom.x.s, in->x.e - dom.x.s},
+              {in->y.s - dom.y.s, in->y.e - dom.y.s},
+              {in->z.s - dom.z.s, in->z.e - dom.z.s}};
+  return c;
+}
+
+void PecPlane::correctMaterialSpace(Sim& s, Index) {  // pec_plane.cpp:12-67
+  const Real sg = material.sigma_e;
+  const Index is = gb_origin[0], js = gb_origin[1], ks = gb_origin[2];
+  const Index ie = gb_end[0], je = gb_end[1], ke = gb_end[2];
+  auto fill = [&](Arr3& a, Index i0, Index i1, Index j0, Index j1, Index k0, Index k1) {
+    for (Index i = i0; i < i1; ++i)
+      for (Index j = j0; j < j1; ++j)
+        for (Index k = k0; k < k1; ++k) a(i, j, k) = sg;
+  };
+  if (is == ie) {
+    fill(s.mp.se[1], is, is + 1, js, je, ks, ke + 1);
+    fill(s.mp.se[2], is, is + 1, js, je + 1, ks, ke);
+    return;
+  }
+  if (js == je) {
+    fill(s.mp.se[2], is, ie + 1, js, js + 1, ks, ke);
+    fill(s.mp.se[0], is, ie, js, js + 1, ks, ke + 1);
+    return;
+  }
+  if (ks == ke) {
+    fill(s.mp.se[0], is, ie, js, je + 1, ks, ks + 1);
+    fill(s.mp.se[1], is, ie + 1, js, je, ks, ks + 1);
+    return;
+  }
+  throw Error(name + " PecPlane is not a plane!");
 }
 
 void SourceElement::init(Sim& s) {
@@ -1238,6 +1374,7 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
       if (auto c = p->generateCorrector(*this, tasks[id])) d.correctors.push_back(std::move(c));
   }
   std::barrier bar(static_cast<std::ptrdiff_t>(doms.size()));
+  std::chrono::steady_clock::time_point t0;
   auto body = [&](Domain& d) {
     for (Index t = start_step; t < start_step + size; ++t) {
       d.updateH();
@@ -1255,9 +1392,13 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
       }
       bar.arrive_and_wait();
       bar.arrive_and_wait();
+      if (d.master)
+        step_end_seconds.push_back(
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     }
   };
-  auto t0 = std::chrono::steady_clock::now();
+  step_end_seconds.clear();
+  t0 = std::chrono::steady_clock::now();
   {
     std::vector<std::thread> th;
     for (Index id = 1; id < doms.size(); ++id) th.emplace_back(body, std::ref(doms[id]));

@@ -287,6 +287,35 @@ struct Resistor : LumpedElement {
   void correctUpdateCoefficient(Sim& s) override;
 };
 
+// Capacitor: coefficient-only patch with beta = 2 C' dc / (da db)
+// (capacitor.cpp:16-112).
+struct Capacitor : LumpedElement {
+  Real capacitance;
+  Real cf = 0;
+  Arr3 da, db, dc, beta;
+  void init(Sim& s) override;
+  void correctUpdateCoefficient(Sim& s) override;
+};
+
+// Inductor: a current state J per edge, E += cecjc*J; J += cjcec*E after
+// the E update (inductor.cpp:23-135, lumped_element_corrrector.cpp:40-99).
+struct Inductor : LumpedElement {
+  Real inductance;
+  Real lf = 0;
+  Arr3 da, db, dc, cecjc, cjcec, j;
+  void init(Sim& s) override;
+  void correctUpdateCoefficient(Sim& s) override;
+  std::unique_ptr<Corrector> generateCorrector(Sim& s, const Task& t) override;
+};
+
+// PecPlane: sigma_e of a PEC material on the plane's tangential edges,
+// written after material averaging (pec_plane.cpp:12-67,
+// simulation.cpp:629-635).
+struct PecPlane : Object {
+  bool isPecPlane() const override { return true; }
+  void correctMaterialSpace(Sim& s, Index idx) override;
+};
+
 struct SourceElement : LumpedElement {
   Dir direction;
   Real resistance;
@@ -397,6 +426,8 @@ struct Sim {
   Real dt = 0;
   Index start_step = 0, size = 0, current = 0;
   double loop_seconds = 0;
+  // seconds from the loop start to the end of each step (timing only)
+  std::vector<double> step_end_seconds;
 
   void generateGridSpace();             // simulation.cpp:496-512
   void generateEMF();                   // simulation.cpp:514-528

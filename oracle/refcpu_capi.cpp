@@ -192,6 +192,29 @@ void parseSpec(Sim& s, const std::string& spec) {
       if (e->resistance == 0) e->resistance = 1e-10;
       e->material = Material::make(1, 1, 0, 0);
       s.objects.push_back(std::move(e));
+    } else if (d == "capacitor") {
+      auto e = std::make_unique<Capacitor>();
+      e->name = word(is);
+      e->shape = parseCube(is);
+      e->xyz = parseAxis(word(is));
+      e->capacitance = num(is);
+      e->material = Material::make(1, 1, 0, 0);
+      s.objects.push_back(std::move(e));
+    } else if (d == "inductor") {
+      auto e = std::make_unique<Inductor>();
+      e->name = word(is);
+      e->shape = parseCube(is);
+      e->xyz = parseAxis(word(is));
+      e->inductance = num(is);
+      if (e->inductance == 0) e->inductance = 1e-10;  // inductor.cpp:15-17
+      e->material = Material::make(1, 1, 0, 0);
+      s.objects.push_back(std::move(e));
+    } else if (d == "pec_plane") {
+      auto e = std::make_unique<PecPlane>();
+      e->name = word(is);
+      e->shape = parseCube(is);
+      e->material = Material::make(1, 1, 1e10, 0);  // ElectroMagneticProperty::pec
+      s.objects.push_back(std::move(e));
     } else if (d == "movie") {
       auto m = std::make_unique<MovieMonitor>();
       auto f = std::make_unique<FieldMonitor>();
@@ -281,6 +304,13 @@ int ref_init(void* p, int64_t steps) {
 
 double ref_dt(void* p) { return static_cast<Handle*>(p)->sim.dt; }
 double ref_loop_seconds(void* p) { return static_cast<Handle*>(p)->sim.loop_seconds; }
+// Copies up to `cap` per-step end times (seconds since the loop start) of
+// the last loop; returns how many there are.
+int64_t ref_step_end_seconds(void* p, double* out, int64_t cap) {
+  const auto& v = static_cast<Handle*>(p)->sim.step_end_seconds;
+  for (int64_t i = 0; i < cap && i < static_cast<int64_t>(v.size()); ++i) out[i] = v[i];
+  return static_cast<int64_t>(v.size());
+}
 
 int ref_grid(void* p, int64_t* n3) {
   auto& g = static_cast<Handle*>(p)->sim.g;

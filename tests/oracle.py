@@ -38,6 +38,7 @@ def lib():
         "ref_write_field": (ctypes.c_int, [vp, ctypes.c_int, vp]),
         "ref_dt": (dbl, [vp]),
         "ref_loop_seconds": (dbl, [vp]),
+        "ref_step_end_seconds": (i64, [vp, ctypes.POINTER(dbl), i64]),
         "ref_grid": (ctypes.c_int, [vp, p64]),
         "ref_grid_array": (i64, [vp, ctypes.c_int, ctypes.c_int, vp]),
         "ref_field": (ctypes.c_int, [vp, ctypes.c_int, p64, vp]),
@@ -70,6 +71,10 @@ class RefSim:
         if rc != 0:
             raise RuntimeError(self.L.ref_last_error().decode())
 
+    def set_threads(self, tx: int, ty: int, tz: int):
+        """ThreadConfig of the next loop (Simulation's thread layout)."""
+        self._check(self.L.ref_set_threads(self.h, tx, ty, tz))
+
     def __del__(self):
         if getattr(self, "h", None):
             self.L.ref_destroy(self.h)
@@ -97,6 +102,13 @@ class RefSim:
     @property
     def loop_seconds(self):
         return self.L.ref_loop_seconds(self.h)
+
+    def step_end_seconds(self):
+        """Seconds from the start of the last loop to the end of each step."""
+        n = self.L.ref_step_end_seconds(self.h, None, 0)
+        out = (ctypes.c_double * max(1, n))()
+        self.L.ref_step_end_seconds(self.h, out, n)
+        return list(out)[:n]
 
     def grid(self):
         n = (ctypes.c_int64 * 3)()

@@ -50,13 +50,14 @@ def test_capi_cavity_eigenmode(field):
     np.testing.assert_allclose(gpu[field], g * e0, rtol=0, atol=2e-12 * np.abs(e0).max())
 
 
-def _compare_run(spec, steps, threads, label, precision="f64"):
+def _compare_run(spec, steps, threads, label, precision="f64", classed=True):
     say(f"[gpu] {label}: oracle {steps} steps")
     ref = RefSim(spec, threads=threads)
     ref.run(steps)
     say(f"[gpu] {label}: device {precision}")
     dev = native.Simulation(spec, precision)
     dev.run(steps)
+    assert dev.coef_classed() == classed, "coefficient layout"
     out = {}
     for f in FIELDS:
         a, b = dev.field(f), ref.field(f)
@@ -95,6 +96,50 @@ def test_inhomogeneous_box_fp64():
         assert e <= TOL_F64, (k, e)
 
 
+def test_inhomogeneous_box_dense_coefficients_fp64(monkeypatch):
+    """The same box with the 18 dense coefficient arrays on the device
+    (XFD_COEF_MODE=dense) instead of per-cell class ids."""
+    monkeypatch.setenv("XFD_COEF_MODE", "dense")
+    sp = scenarios.inhomogeneous_box(n_total=48, pml=6, n_spheres=6, r_cells=(3, 7), seed=11,
+                                     snapshot_interval=100).spec()
+    out = _compare_run(sp, 300, (1, 1, 1), "box48-dense", classed=False)
+    for k, (e, _) in out.items():
+        assert e <= TOL_F64, (k, e)
+
+
+def test_capi_coefficient_classes_match_dense(monkeypatch):
+    """Random per-cell coefficients drawn from a few values: the classed and
+    the dense device layouts give identical fields."""
+    say("[gpu] class ids vs dense arrays")
+    rng = np.random.default_rng(5)
+    n = (20, 18, 70)
+    ref = RefSim(em.cavity_spec(n, (1e-3, 1e-3, 1e-3)))
+    ref.init(10)
+    coeffs = {}
+    for c in native.COEFF_NAMES:
+        base = ref.coeff(c)
+        scale = rng.choice(np.array([1.0, 0.97, 1.01, 0.5]), size=base.shape)
+        coeffs[c] = base * scale
+    e0 = rng.standard_normal((n[0] + 1, n[1] + 1, n[2]))
+    out = {}
+    for mode in ("classed", "dense"):
+        if mode == "dense":
+            monkeypatch.setenv("XFD_COEF_MODE", "dense")
+        ctx = native.Context(coeffs, "f64")
+        classed, classes = ctx.coef_layout()
+        assert classed == (mode == "classed"), mode
+        if classed:
+            for q in range(6):
+                trip = np.stack([coeffs[native.COEFF_NAMES[3 * q + r]].ravel() for r in range(3)], -1)
+                assert classes[q] == np.unique(trip, axis=0).shape[0], (q, classes)
+        ctx.write("EZ", e0)
+        ctx.step(0, 40)
+        out[mode] = {f: ctx.read(f) for f in FIELDS}
+        ctx.close()
+    for f in FIELDS:
+        np.testing.assert_array_equal(out["classed"][f], out["dense"][f], err_msg=f)
+
+
 def test_lumped_oblique_tfsf_fp64():
     from test_precompute_parity import _lumped_box
 
@@ -105,11 +150,49 @@ def test_lumped_oblique_tfsf_fp64():
 
 
 def test_fp32_error_against_fp64_oracle():
-    """FP32 fields and updates: error of the final fields and snapshots
-    against the FP64 oracle (relative L2 per field), stated and bounded."""
+    """FP32 fields and updates against the FP64 oracle on the scaled box.
+
+    Stated figures: per snapshot frame the relative L2 of the E and H groups
+    with the frame's own norm and with the run's peak norm, and the relative
+    L2 of each probe series.  Bounds: while the pulse is in the box (step
+    100) the plain relative L2 is <= 1e-5; every frame and the final fields
+    are within 1e-5 of the run's peak norm; probe series within 1e-4.  The
+    late frames' own-norm figures (up to ~1e-3) are printed, not bounded:
+    the field has decayed about a thousandfold by then while the FP32
+    residue stays at ~1e-7 of the peak."""
+    from xfdtd_core_amd import accuracy
+
     sp = scenarios.inhomogeneous_box(n_total=48, pml=6, n_spheres=6, r_cells=(3, 7), seed=11,
                                      snapshot_interval=100).spec()
-    out = _compare_run(sp, 300, (1, 1, 1), "box48-f32", precision="f32")
-    worst = max(e for e, _ in out.values())
-    say(f"  FP32 worst relative L2 vs FP64 oracle: {worst:.3e}")
-    assert worst <= 1e-4
+    steps = 300
+    say("[gpu] box48-f32: oracle")
+    ref = RefSim(sp, threads=(1, 1, 1))
+    ref.run(steps)
+    say("[gpu] box48-f32: device f32")
+    dev = native.Simulation(sp, "f32")
+    dev.run(steps)
+    snaps_d = {f: dev.monitor_records(i) for i, f in enumerate(FIELDS)}
+    snaps_r = {f: ref.monitor_records(i) for i, f in enumerate(FIELDS)}
+    frames = accuracy.snapshot_errors(snaps_d, snaps_r, [100 * k for k in range(len(snaps_r["EX"]))])
+    for r in frames:
+        say(f"  step {r['step']}: E own {r['E']['own']:.2e} peak {r['E']['peak']:.2e} | "
+            f"H own {r['H']['own']:.2e} peak {r['H']['peak']:.2e}")
+    peaks = {g: max(np.sqrt(sum(float(np.sum(snaps_r[f][i] ** 2)) for f in fl))
+                    for i in range(len(snaps_r["EX"])))
+             for g, fl in accuracy.GROUPS.items()}
+    final = {}
+    for g, fl in accuracy.GROUPS.items():
+        d = np.sqrt(sum(float(np.sum((dev.field(f) - ref.field(f)) ** 2)) for f in fl))
+        n = np.sqrt(sum(float(np.sum(ref.field(f) ** 2)) for f in fl))
+        final[g] = (d / n, d / peaks[g])
+        say(f"  final {g}: own {final[g][0]:.2e} peak {final[g][1]:.2e}")
+    probes = [accuracy.series_error(dev.monitor_records(m)[0][1], ref.monitor_records(m)[0][1])
+              for m in range(len(FIELDS), dev.monitor_count())]
+    say(f"  probes: max relative L2 {max(probes):.2e}")
+    at100 = [r for r in frames if r["step"] == 100][0]
+    assert at100["E"]["own"] <= 1e-5 and at100["H"]["own"] <= 1e-5, at100
+    for r in frames:
+        assert r["E"]["peak"] <= 1e-5 and r["H"]["peak"] <= 1e-5, r
+    for g in final:
+        assert final[g][1] <= 1e-5, (g, final[g])
+    assert max(probes) <= 1e-4

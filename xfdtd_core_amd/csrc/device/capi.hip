@@ -3,10 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../../include/xfdtd_core_amd.h"
@@ -58,6 +62,108 @@ void extentOf(int id, int nx, int ny, int nz, int out[3]) {
 // coefficient slot -> field id it multiplies (self) for extent purposes
 int coefField(int slot) { return slot / 3; }
 
+// Distinct (self, dual-A, dual-B) coefficient triples of one component, by
+// bit pattern, and the per-cell class id in the padded layout.  A uniform
+// grid with a few materials has few classes (the 384^3 random-sphere box:
+// about 1.6k per E and 700 per H component), so 2 bytes per cell replace
+// three coefficients.  Returns false when a component has more than 65535
+// classes (the dense layout is used instead).
+struct Triple {
+  uint64_t v[3];
+  bool operator==(const Triple& o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+};
+struct TripleHash {
+  size_t operator()(const Triple& t) const {
+    uint64_t h = t.v[0] * 0x9E3779B97F4A7C15ull;
+    h ^= t.v[1] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h ^= t.v[2] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    return static_cast<size_t>(h);
+  }
+};
+
+bool classify(const double* A, const double* B, const double* C, const int ext[3], const xfd::Grid& g,
+              std::vector<uint16_t>& ids, std::vector<double>& table) {
+  const int64_t plane = static_cast<int64_t>(ext[1]) * ext[2];
+  const unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  const int parts = static_cast<int>(std::min<int64_t>(hw, ext[0]));
+  auto key = [&](int64_t p) {
+    Triple t;
+    std::memcpy(&t.v[0], A + p, 8);
+    std::memcpy(&t.v[1], B + p, 8);
+    std::memcpy(&t.v[2], C + p, 8);
+    return t;
+  };
+  // pass 1: distinct triples per thread, merged in thread order
+  std::vector<std::vector<Triple>> found(parts);
+  std::atomic<bool> overflow{false};
+  auto rangeOf = [&](int p, int& lo, int& hi) {
+    const int step = (ext[0] + parts - 1) / parts;
+    lo = p * step;
+    hi = std::min(ext[0], lo + step);
+  };
+  {
+    std::vector<std::thread> th;
+    for (int p = 0; p < parts; ++p)
+      th.emplace_back([&, p] {
+        int lo, hi;
+        rangeOf(p, lo, hi);
+        std::unordered_map<Triple, int, TripleHash> seen;
+        Triple last{};
+        bool have = false;
+        for (int64_t q = lo * plane; q < hi * plane; ++q) {
+          Triple t = key(q);
+          if (have && t == last) continue;
+          last = t;
+          have = true;
+          if (seen.emplace(t, 0).second) found[p].push_back(t);
+          if (seen.size() > 65535) {
+            overflow = true;
+            return;
+          }
+        }
+      });
+    for (auto& t : th) t.join();
+  }
+  if (overflow) return false;
+  std::unordered_map<Triple, uint16_t, TripleHash> id;
+  table.clear();
+  for (auto& f : found)
+    for (auto& t : f)
+      if (id.find(t) == id.end()) {
+        if (id.size() >= 65535) return false;
+        id.emplace(t, static_cast<uint16_t>(id.size()));
+        for (int q = 0; q < 3; ++q) {
+          double d;
+          std::memcpy(&d, &t.v[q], 8);
+          table.push_back(d);
+        }
+      }
+  // pass 2: ids in the padded layout (padding stays class 0, never read)
+  ids.assign(static_cast<size_t>(g.total), 0);
+  std::vector<std::thread> th;
+  for (int p = 0; p < parts; ++p)
+    th.emplace_back([&, p] {
+      int lo, hi;
+      rangeOf(p, lo, hi);
+      Triple last{};
+      uint16_t last_id = 0;
+      bool have = false;
+      for (int64_t i = lo; i < hi; ++i)
+        for (int64_t j = 0; j < ext[1]; ++j)
+          for (int64_t k = 0; k < ext[2]; ++k) {
+            const Triple t = key((i * ext[1] + j) * ext[2] + k);
+            if (!have || !(t == last)) {
+              last = t;
+              last_id = id.at(t);
+              have = true;
+            }
+            ids[(i * g.NY + j) * g.pz + k] = last_id;
+          }
+    });
+  for (auto& t : th) t.join();
+  return true;
+}
+
 enum KernelClass { K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_COUNT };
 const char* kKernelNames[K_COUNT] = {"update_h", "update_e", "cpml_h", "cpml_e",
                                      "records_h", "records_e", "gather"};
@@ -71,7 +177,13 @@ struct DevBuf {
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
   void alloc(size_t b) {
+    reset();
     bytes = b;
     if (b == 0) return;
     XFD_HIP(hipMalloc(&p, b));
@@ -92,7 +204,11 @@ struct xfd_ctx {
   size_t elem = 8;
   int ichunk = 16;
   DevBuf fields[6];
-  DevBuf coefs[18];
+  DevBuf coefs[18];   // dense coefficient layout
+  bool classed = false;
+  DevBuf ids[6];      // classed layout: uint16 class per cell and component
+  DevBuf tabs[6];     // and the triple table of each component
+  int64_t classes[6] = {0, 0, 0, 0, 0, 0};
 
   struct Slab {
     int axis, n, e_start, h_start;
@@ -112,8 +228,13 @@ struct xfd_ctx {
     xfd_gather_desc d;
     xfd::GatherArgs a;
     DevBuf out;
+    int probe = -1;  // row in the batched probe buffer, or -1
   };
   std::vector<std::unique_ptr<Gather>> gathers;
+  // single-cell, every-step recordings share one launch per step
+  DevBuf probe_list, probe_out;
+  int nprobe = 0;
+  int64_t probe_count = 0;
 
   bool prof = false;
   int64_t launches[K_COUNT] = {};
@@ -135,6 +256,8 @@ struct xfd_ctx {
     int64_t b = 0;
     for (auto& f : fields) b += f.bytes;
     for (auto& c : coefs) b += c.bytes;
+    for (int q = 0; q < 6; ++q) b += ids[q].bytes + tabs[q].bytes;
+    b += probe_list.bytes + probe_out.bytes;
     for (auto& s : slabs) {
       for (auto& x : s->cpsi) b += x.bytes;
       for (auto& x : s->psi) b += x.bytes;
@@ -330,14 +453,39 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     extentOf(coefField(s), g.nx, g.ny, g.nz, ext);
     max_ext = std::max(max_ext, static_cast<size_t>(ext[0]) * ext[1] * ext[2]);
   }
-  DevBuf staging;
-  staging.alloc(max_ext * sizeof(double));
-  for (int s = 0; s < 18; ++s) {
-    require(d.coeff[s] != nullptr, "coefficient array missing");
+  for (int s = 0; s < 18; ++s) require(d.coeff[s] != nullptr, "coefficient array missing");
+  // Classed layout unless a component has too many distinct triples, or
+  // XFD_COEF_MODE=dense asks for the 18-array layout.
+  const char* mode = std::getenv("XFD_COEF_MODE");
+  c.classed = !(mode && std::string(mode) == "dense");
+  for (int q = 0; q < 6 && c.classed; ++q) {
     int ext[3];
-    extentOf(coefField(s), g.nx, g.ny, g.nz, ext);
-    c.coefs[s].alloc(g.total * sizeof(T));
-    uploadPadded<T>(c, d.coeff[s], ext, c.coefs[s].as<T>(), staging);
+    extentOf(q, g.nx, g.ny, g.nz, ext);
+    std::vector<uint16_t> ids;
+    std::vector<double> tab;
+    if (!classify(d.coeff[3 * q], d.coeff[3 * q + 1], d.coeff[3 * q + 2], ext, g, ids, tab)) {
+      c.classed = false;
+      break;
+    }
+    uploadVector(ids, c.ids[q]);
+    std::vector<T> tt(tab.begin(), tab.end());
+    uploadVector(tt, c.tabs[q]);
+    c.classes[q] = static_cast<int64_t>(tab.size() / 3);
+  }
+  if (!c.classed) {
+    for (int q = 0; q < 6; ++q) {
+      c.ids[q].reset();
+      c.tabs[q].reset();
+      c.classes[q] = 0;
+    }
+    DevBuf staging;
+    staging.alloc(max_ext * sizeof(double));
+    for (int s = 0; s < 18; ++s) {
+      int ext[3];
+      extentOf(coefField(s), g.nx, g.ny, g.nz, ext);
+      c.coefs[s].alloc(g.total * sizeof(T));
+      uploadPadded<T>(c, d.coeff[s], ext, c.coefs[s].as<T>(), staging);
+    }
   }
 
   // CPML slabs
@@ -484,8 +632,23 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     }
     gg->a.frame_elems = static_cast<int64_t>(G.end[0] - G.origin[0]) * (G.end[1] - G.origin[1]) *
                         (G.end[2] - G.origin[2]);
-    gg->out.alloc(static_cast<size_t>(gg->a.frame_elems * G.count) * sizeof(double));
+    const bool probe = gg->a.frame_elems == 1 && G.first == 0 && G.interval == 1 && G.count > 0 &&
+                       (c.nprobe == 0 || G.count == c.probe_count);
+    if (probe) {
+      gg->probe = c.nprobe++;
+      c.probe_count = G.count;
+    } else {
+      gg->out.alloc(static_cast<size_t>(gg->a.frame_elems * G.count) * sizeof(double));
+    }
     c.gathers.push_back(std::move(gg));
+  }
+  if (c.nprobe) {
+    std::vector<xfd::Probe> pl(c.nprobe);
+    for (auto& gg : c.gathers)
+      if (gg->probe >= 0)
+        pl[gg->probe] = {padIndex(g, gg->a.o[0], gg->a.o[1], gg->a.o[2]), gg->a.field, 0};
+    uploadVector(pl, c.probe_list);
+    c.probe_out.alloc(static_cast<size_t>(c.nprobe * c.probe_count) * sizeof(double));
   }
   XFD_HIP(hipStreamSynchronize(c.stream));
 }
@@ -541,6 +704,11 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
   require(t0 + n <= c.pool_rows, "step beyond the source tables (run length given at init)");
   const auto F = fieldPtrs<T>(c);
   const auto C = coefPtrs<T>(c);
+  xfd::ClassCoef<T> K{};
+  for (int q = 0; q < 6; ++q) {
+    K.id[q] = c.ids[q].as<uint16_t>();
+    K.tab[q] = c.tabs[q].as<T>();
+  }
   std::vector<xfd::SlabArgs<T>> se, sh;
   for (auto& s : c.slabs) {
     se.push_back(slabArgs<T>(*s, true));
@@ -551,22 +719,38 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
   for (auto& a : sh) a.stride_c = stride[a.axis];
   hipStream_t st = c.stream;
   for (int64_t t = t0; t < t0 + n; ++t) {
-    c.timed(K_UPDATE_H, [&] { xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk); });
+    c.timed(K_UPDATE_H, [&] {
+      if (c.classed)
+        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk);
+      else
+        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk);
+    });
     if (c.nseg_h)
       c.timed(K_REC_H, [&] {
         xfd::Launch<T>::records(st, F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h,
                                 c.pool.as<double>(), t);
       });
     for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, false); });
-    c.timed(K_UPDATE_E, [&] { xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk); });
+    c.timed(K_UPDATE_E, [&] {
+      if (c.classed)
+        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk);
+      else
+        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk);
+    });
     if (c.nseg_e)
       c.timed(K_REC_E, [&] {
         xfd::Launch<T>::records(st, F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e,
                                 c.pool.as<double>(), t);
       });
     for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
+    if (c.nprobe && t < c.probe_count)
+      c.timed(K_GATHER, [&] {
+        xfd::Launch<T>::probes(st, F, c.probe_list.as<xfd::Probe>(), c.nprobe, c.probe_out.as<double>(),
+                               c.probe_count, t);
+      });
     for (auto& gp : c.gathers) {
       const auto& d = gp->d;
+      if (gp->probe >= 0) continue;
       if (t < d.first || (t - d.first) % d.interval != 0) continue;
       const int64_t frame = (t - d.first) / d.interval;
       if (frame >= d.count) continue;
@@ -694,7 +878,11 @@ int xfd_read_gather(xfd_ctx* c, int32_t gi, double* dst) {
   return guard([&] {
     require(c && dst && gi >= 0 && gi < static_cast<int>(c->gathers.size()), "bad argument");
     auto& g = *c->gathers[gi];
-    if (g.out.bytes)
+    if (g.probe >= 0)
+      XFD_HIP(hipMemcpyAsync(dst, c->probe_out.as<double>() + g.probe * c->probe_count,
+                             static_cast<size_t>(c->probe_count) * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    else if (g.out.bytes)
       XFD_HIP(hipMemcpyAsync(dst, g.out.p, g.out.bytes, hipMemcpyDeviceToHost, c->stream));
     XFD_HIP(hipStreamSynchronize(c->stream));
   });
@@ -717,6 +905,15 @@ int xfd_kernel_stats(xfd_ctx* c, int32_t k, const char** name, int64_t* launches
     if (name) *name = kKernelNames[k];
     if (launches) *launches = c->launches[k];
     if (total_ms) *total_ms = c->total_ms[k];
+  });
+}
+
+int xfd_coef_layout(xfd_ctx* c, int32_t* classed, int64_t* classes) {
+  return guard([&] {
+    require(c && classed, "bad argument");
+    *classed = c->classed ? 1 : 0;
+    if (classes)
+      for (int q = 0; q < 6; ++q) classes[q] = c->classes[q];
   });
 }
 

@@ -23,9 +23,9 @@ constexpr int kLanesK = 64;
 constexpr int kRowsJ = 4;
 
 // BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41): Hx, Hy, Hz
-// over [0,nx) x [0,ny) x [0,nz).
-template <class T>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, CoefPtrs<T> C, int ichunk) {
+// over [0,nx) x [0,ny) x [0,nz).  Coef is CoefPtrs (dense) or ClassCoef.
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, int ichunk) {
   const int k = blockIdx.x * kLanesK + threadIdx.x;
   const int j = blockIdx.y * kRowsJ + threadIdx.y;
   const int i0 = blockIdx.z * ichunk;
@@ -33,9 +33,9 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, CoefPt
   const int i1 = min(i0 + ichunk, g.nx);
   const int64_t pz = g.pz, pl = g.plane;
   int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
-  T* __restrict__ ex = F.f[0];
-  T* __restrict__ ey = F.f[1];
-  T* __restrict__ ez = F.f[2];
+  const T* __restrict__ ex = F.f[0];
+  const T* __restrict__ ey = F.f[1];
+  const T* __restrict__ ez = F.f[2];
   T* __restrict__ hx = F.f[3];
   T* __restrict__ hy = F.f[4];
   T* __restrict__ hz = F.f[5];
@@ -45,9 +45,13 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, CoefPt
     const T ex0 = ex[idx], ex_j = ex[idx + pz], ex_k = ex[idx + 1];
     const T ey_k = ey[idx + 1], ey_n = ey[idx + pl];
     const T ez_j = ez[idx + pz], ez_n = ez[idx + pl];
-    hx[idx] = C.c[9][idx] * hx[idx] + C.c[10][idx] * (ey_k - ey_i) + C.c[11][idx] * (ez_j - ez_i);
-    hy[idx] = C.c[12][idx] * hy[idx] + C.c[13][idx] * (ez_n - ez_i) + C.c[14][idx] * (ex_k - ex0);
-    hz[idx] = C.c[15][idx] * hz[idx] + C.c[16][idx] * (ex_j - ex0) + C.c[17][idx] * (ey_n - ey_i);
+    T s, a, b;
+    C.get(3, idx, s, a, b);
+    hx[idx] = s * hx[idx] + a * (ey_k - ey_i) + b * (ez_j - ez_i);
+    C.get(4, idx, s, a, b);
+    hy[idx] = s * hy[idx] + a * (ez_n - ez_i) + b * (ex_k - ex0);
+    C.get(5, idx, s, a, b);
+    hz[idx] = s * hz[idx] + a * (ex_j - ex0) + b * (ey_n - ey_i);
     ey_i = ey_n;
     ez_i = ez_n;
   }
@@ -56,8 +60,8 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, CoefPt
 // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
 // Ex over [0,nx) x [1,ny) x [1,nz), Ey over [1,nx) x [0,ny) x [1,nz),
 // Ez over [1,nx) x [1,ny) x [0,nz).
-template <class T>
-__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, CoefPtrs<T> C, int ichunk) {
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, int ichunk) {
   const int k = blockIdx.x * kLanesK + threadIdx.x;
   const int j = blockIdx.y * kRowsJ + threadIdx.y;
   const int i0 = blockIdx.z * ichunk;
@@ -76,18 +80,38 @@ __global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, CoefPt
   const bool jk = j >= 1 && k >= 1;
   for (int i = i0; i < i1; ++i, idx += pl) {
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
-    if (jk)
-      ex[idx] = C.c[0][idx] * ex[idx] + C.c[1][idx] * (hy0 - hy[idx - 1]) +
-                C.c[2][idx] * (hz0 - hz[idx - pz]);
-    if (i >= 1 && k >= 1)
-      ey[idx] = C.c[3][idx] * ey[idx] + C.c[4][idx] * (hz0 - hz_p) +
-                C.c[5][idx] * (hx0 - hx[idx - 1]);
-    if (i >= 1 && j >= 1)
-      ez[idx] = C.c[6][idx] * ez[idx] + C.c[7][idx] * (hx0 - hx[idx - pz]) +
-                C.c[8][idx] * (hy0 - hy_p);
+    T s, a, b;
+    if (jk) {
+      C.get(0, idx, s, a, b);
+      ex[idx] = s * ex[idx] + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
+    }
+    if (i >= 1 && k >= 1) {
+      C.get(1, idx, s, a, b);
+      ey[idx] = s * ey[idx] + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
+    }
+    if (i >= 1 && j >= 1) {
+      C.get(2, idx, s, a, b);
+      ez[idx] = s * ez[idx] + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
+    }
     hz_p = hz0;
     hy_p = hy0;
   }
+}
+
+// Single-cell time series (FieldTimeMonitor sampling): one launch per step
+// for all of them; probe p writes out[p * stride + t].
+struct Probe {
+  int64_t idx;
+  int32_t field;
+  int32_t pad;
+};
+
+template <class T>
+__global__ __launch_bounds__(64) void k_probes(FieldPtrs<T> F, const Probe* __restrict__ p, int n, double* out,
+                                               int64_t stride, int64_t t) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  out[q * stride + t] = static_cast<double>(F.f[p[q].field][p[q].idx]);
 }
 
 // correctPML (src/include/boundary/pml_corrector.h:14-52) for both
@@ -199,6 +223,10 @@ template <class T>
 struct Launch {
   static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
+  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk);
+  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk);
+  static void probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
+                     int64_t t);
   static void cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
   static void records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
                       const double* pool, int64_t t);

@@ -9,18 +9,38 @@ namespace xfd {
 
 inline unsigned blocks_for(int64_t n, int per) { return static_cast<unsigned>((n + per - 1) / per); }
 
-template <class T>
-void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
+template <class T, class Coef>
+void launchUpdate(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk) {
   dim3 block(kLanesK, kRowsJ);
   dim3 grid(blocks_for(g.nz, kLanesK), blocks_for(g.ny, kRowsJ), blocks_for(g.nx, ichunk));
-  hipLaunchKernelGGL(k_update_h<T>, grid, block, 0, s, g, F, C, ichunk);
+  if (is_h)
+    hipLaunchKernelGGL((k_update_h<T, Coef>), grid, block, 0, s, g, F, C, ichunk);
+  else
+    hipLaunchKernelGGL((k_update_e<T, Coef>), grid, block, 0, s, g, F, C, ichunk);
 }
 
 template <class T>
+void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
+  launchUpdate<T>(true, s, g, F, C, ichunk);
+}
+template <class T>
 void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
-  dim3 block(kLanesK, kRowsJ);
-  dim3 grid(blocks_for(g.nz, kLanesK), blocks_for(g.ny, kRowsJ), blocks_for(g.nx, ichunk));
-  hipLaunchKernelGGL(k_update_e<T>, grid, block, 0, s, g, F, C, ichunk);
+  launchUpdate<T>(false, s, g, F, C, ichunk);
+}
+template <class T>
+void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk) {
+  launchUpdate<T>(true, s, g, F, C, ichunk);
+}
+template <class T>
+void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk) {
+  launchUpdate<T>(false, s, g, F, C, ichunk);
+}
+
+template <class T>
+void Launch<T>::probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
+                       int64_t t) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_probes<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, F, p, n, out, stride, t);
 }
 
 template <class T>

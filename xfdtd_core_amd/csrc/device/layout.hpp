@@ -29,9 +29,31 @@ struct FieldPtrs {
   T* f[6];  // Ex Ey Ez Hx Hy Hz
 };
 
+// Coefficient access for component q (0..5 = Ex Ey Ez Hx Hy Hz): the
+// triple (self, tangential-A dual, tangential-B dual) of the reference's
+// per-cell arrays.  Dense: 18 padded arrays.  Classed: one uint16 class id
+// per cell and component into a table of distinct triples -- the same FP64
+// values, 2 bytes per cell instead of 3*sizeof(T).
 template <class T>
 struct CoefPtrs {
   const T* c[18];  // cexe cexhy cexhz ceye ceyhz ceyhx ceze cezhx cezhy chxh ... chzey
+  __device__ __forceinline__ void get(int q, int64_t idx, T& s, T& a, T& b) const {
+    s = c[3 * q][idx];
+    a = c[3 * q + 1][idx];
+    b = c[3 * q + 2][idx];
+  }
+};
+
+template <class T>
+struct ClassCoef {
+  const uint16_t* id[6];
+  const T* tab[6];  // 3 values per class
+  __device__ __forceinline__ void get(int q, int64_t idx, T& s, T& a, T& b) const {
+    const T* e = tab[q] + 3 * static_cast<int>(id[q][idx]);
+    s = e[0];
+    a = e[1];
+    b = e[2];
+  }
 };
 
 // One CPML slab for one half-step.  The thread space is the union box of the

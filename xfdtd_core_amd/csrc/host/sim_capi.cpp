@@ -286,6 +286,7 @@ int xfd_sim_grid(void* p, int64_t* n3) {
 }
 
 int64_t xfd_sim_device_bytes(void* p) { return H(p)->sim->deviceBytes(); }
+int32_t xfd_sim_coef_classed(void* p) { return H(p)->sim->classedCoefficients() ? 1 : 0; }
 
 int xfd_sim_field(void* p, int32_t field, int64_t* shape, double* dst) {
   return guard([&] {

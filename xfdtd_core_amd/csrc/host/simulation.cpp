@@ -316,6 +316,7 @@ struct Simulation::Impl {
   bool profiling = false;
   std::vector<KernelStat> stats;
   long long dev_bytes = 0;
+  bool coef_classed = false;
 
   ~Impl() {
     if (ctx) xfd_destroy(ctx);
@@ -1009,6 +1010,9 @@ struct Simulation::Impl {
     int64_t b = 0;
     xfd_device_bytes(ctx, &b);
     dev_bytes = b;
+    int32_t cl = 0;
+    xfd_coef_layout(ctx, &cl, nullptr);
+    coef_classed = cl != 0;
   }
 
   void runLoop() {
@@ -1144,5 +1148,6 @@ double Simulation::loopSeconds() const { return _impl->loop_seconds; }
 void Simulation::setProfiling(bool on) { _impl->profiling = on; }
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
+bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
 
 }  // namespace xfdtd

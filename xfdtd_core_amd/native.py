@@ -53,6 +53,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_loop_seconds": (dbl, [vp]),
         "xfd_sim_grid": (ctypes.c_int, [vp, p64]),
         "xfd_sim_device_bytes": (i64, [vp]),
+        "xfd_sim_coef_classed": (ctypes.c_int32, [vp]),
         "xfd_sim_grid_array": (i64, [vp, i32, i32, vp]),
         "xfd_sim_field": (ctypes.c_int, [vp, i32, p64, vp]),
         "xfd_sim_coeff": (ctypes.c_int, [vp, cp, p64, vp]),
@@ -75,6 +76,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_set_profiling": (ctypes.c_int, [vp, i32]),
         "xfd_kernel_stats": (ctypes.c_int, [vp, i32, ctypes.POINTER(cp), p64, ctypes.POINTER(ctypes.c_double)]),
         "xfd_device_bytes": (ctypes.c_int, [vp, p64]),
+        "xfd_coef_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), p64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -204,6 +206,13 @@ class Context:
     def step(self, t0: int, n: int):
         self._check(self._L.xfd_step(self._h, t0, n))
 
+    def coef_layout(self):
+        """(classed, [table size of Ex..Hz]) as chosen by xfd_create."""
+        cl = ctypes.c_int32()
+        n = (ctypes.c_int64 * 6)()
+        self._check(self._L.xfd_coef_layout(self._h, ctypes.byref(cl), n))
+        return bool(cl.value), list(n)
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.xfd_destroy(self._h)
@@ -278,6 +287,10 @@ class Simulation:
 
     def device_bytes(self) -> int:
         return int(self._L.xfd_sim_device_bytes(self._h))
+
+    def coef_classed(self) -> bool:
+        """True when the device stores coefficient class ids, not 18 arrays."""
+        return bool(self._L.xfd_sim_coef_classed(self._h))
 
     def grid_array(self, which: int, axis: int) -> np.ndarray:
         n = self._L.xfd_sim_grid_array(self._h, which, axis, None)

@@ -18,6 +18,9 @@ renders the scenario as line-oriented text, one directive per line:
     current_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
     voltage_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
     resistor NAME OX OY OZ SX SY SZ AXIS R
+    capacitor NAME OX OY OZ SX SY SZ AXIS C
+    inductor NAME OX OY OZ SX SY SZ AXIS L
+    pec_plane NAME OX OY OZ SX SY SZ
     movie FIELD OX OY OZ SX SY SZ INTERVAL NAME OUT_DIR
     field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
     WAVEFORM = gaussian TAU T0 AMP | cosine_modulated_gaussian TAU T0 F AMP
@@ -242,6 +245,45 @@ class Resistor:
     def line(self):
         return " ".join(["resistor", self.name] + self.cube.box_tokens() +
                         [self.xyz, _n(self.resistance)])
+
+
+@dataclass
+class Capacitor:
+    """``Capacitor(name, cube, xyz, capacitance)`` (capacitor.h)."""
+
+    name: str
+    cube: Cube
+    xyz: str
+    capacitance: float
+
+    def line(self):
+        return " ".join(["capacitor", self.name] + self.cube.box_tokens() +
+                        [self.xyz, _n(self.capacitance)])
+
+
+@dataclass
+class Inductor:
+    """``Inductor(name, cube, xyz, inductance)`` (inductor.h)."""
+
+    name: str
+    cube: Cube
+    xyz: str
+    inductance: float
+
+    def line(self):
+        return " ".join(["inductor", self.name] + self.cube.box_tokens() +
+                        [self.xyz, _n(self.inductance)])
+
+
+@dataclass
+class PecPlane:
+    """``PecPlane(name, cube)``: a zero-thickness cube (pec_plane.h)."""
+
+    name: str
+    cube: Cube
+
+    def line(self):
+        return " ".join(["pec_plane", self.name] + self.cube.box_tokens())
 
 
 @dataclass
