@@ -64,14 +64,21 @@ typedef struct {
   double ca[3], cb[3];                   /* TFSF::cax..caz, cbx..cbz */
 } xfd_tfsf_desc;
 
-/* Lumped voltage or current source: E += coff * w[t] over the element's
- * edges (src/object/lumped_element/lumped_element_corrrector.cpp:8-38). */
+/* Per-step lumped-element correction after the E update, in object add
+ * order (src/object/lumped_element/lumped_element_corrrector.cpp):
+ *   XFD_LUMPED_SOURCE   voltage or current source, E += coff * w[t] (:8-38);
+ *   XFD_LUMPED_INDUCTOR E += coff * J, then J += coff_j * E, with J a state
+ *                       per edge starting at zero (:40-99, inductor.cpp). */
+enum xfd_lumped_kind { XFD_LUMPED_SOURCE = 0, XFD_LUMPED_INDUCTOR = 1 };
+
 typedef struct {
   int32_t component;        /* 0..2: Ex, Ey, Ez */
   int32_t origin[3], size[3];  /* edge box in grid indices */
-  const double* coff;       /* size[0] x size[1] x size[2] */
+  const double* coff;       /* size[0] x size[1] x size[2]: coff_v / coff_i, or cecjc */
   int64_t nt;
-  const double* waveform;   /* nt samples, indexed by the time step */
+  const double* waveform;   /* source: nt samples, indexed by the time step */
+  int32_t kind;             /* enum xfd_lumped_kind */
+  const double* coff_j;     /* inductor: cjcec, same shape as coff */
 } xfd_lumped_desc;
 
 /* A recording: `count` copies of the box [origin, end) of one field, taken
@@ -82,6 +89,30 @@ typedef struct {
   int32_t origin[3], end[3];
   int64_t first, interval, count;
 } xfd_gather_desc;
+
+/* Linear dispersion by the auxiliary differential equation method: the
+ * reference's ADEMethodStorage (include/xfdtd/material/ade_method/
+ * ade_method.h) and E updators (src/include/updator/ade_updator/
+ * template_ade_update_scheme.h:14-88, src/updator/m_lor_ade_updator.cpp).
+ * The reference keeps the ADE coefficients per cell; they only depend on the
+ * cell's material, so they are passed once per material plus a per-cell
+ * material id.  The E coefficients of dispersive cells (cece, cecha, cechb)
+ * are already patched in xfd_desc.coeff, as after buildDispersiveSpace.
+ * Per material, `coeff` holds 2 + 7 * poles doubles:
+ *   cej_sum, cee_prev, then per pole
+ *   cj_en, cj_e, cj_ep, cj_j, cj_jp, cjsum_j, cjsum_jp
+ * (coeffEJSum, coeffEEPrev, coeffJENext, coeffJE, coeffJEPrev, coeffJJ,
+ * coeffJJPrev, coeffJSumJ, coeffJSumJPrev; entries a scheme does not use are
+ * ignored).  Pole currents, previous currents and previous E start at zero. */
+enum xfd_ade_scheme { XFD_ADE_DRUDE = 0, XFD_ADE_DEBYE = 1, XFD_ADE_MLORENTZ = 2 };
+
+typedef struct {
+  int32_t scheme;            /* enum xfd_ade_scheme (simulation.cpp:806-818) */
+  int32_t poles;             /* the storage's pole count */
+  int32_t n_materials;
+  const double* coeff;       /* n_materials x (2 + 7 * poles) */
+  const int16_t* cell_material;  /* nx x ny x nz cells; -1: not dispersive */
+} xfd_ade_desc;
 
 typedef struct {
   int32_t nx, ny, nz;
@@ -97,6 +128,7 @@ typedef struct {
   const xfd_tfsf_desc* tfsf;
   int32_t n_gather;
   const xfd_gather_desc* gather;
+  const xfd_ade_desc* ade;     /* NULL: no dispersive material */
 } xfd_desc;
 
 typedef struct xfd_ctx xfd_ctx;

@@ -128,6 +128,7 @@ class ElectroMagneticProperty {
  public:
   ElectroMagneticProperty(Real epsilon_r = 1, Real mu_r = 1, Real sigma_e = 0, Real sigma_m = 0);
   static ElectroMagneticProperty air() { return {1, 1, 0, 0}; }
+  static ElectroMagneticProperty pec() { return {1, 1, 1e10, 0}; }  // material.cpp:32-34
   Real epsilonR() const { return _eps_r; }
   Real muR() const { return _mu_r; }
   Real sigmaE() const { return _sigma_e; }
@@ -146,12 +147,70 @@ class Material {
   static std::unique_ptr<Material> createAir(std::string name = "air") {
     return std::make_unique<Material>(std::move(name), ElectroMagneticProperty::air());
   }
+  static std::unique_ptr<Material> createPec(std::string name = "pec") {
+    return std::make_unique<Material>(std::move(name), ElectroMagneticProperty::pec());
+  }
   const ElectroMagneticProperty& emProperty() const { return _p; }
   virtual bool dispersion() const { return false; }
 
  private:
   std::string _name;
   ElectroMagneticProperty _p;
+};
+
+// Linear dispersive material in the modified-Lorentz form
+// (dispersive_material.h, dispersive_material_equation.h):
+//   eps_r(w) = eps_inf + sum_p (a1_p jw + a0_p) / (b2_p (jw)^2 + b1_p jw + b0_p).
+// The subclasses are the reference's factories; which of them a scenario
+// uses decides the ADE scheme (Drude-only, Debye-only, else modified Lorentz).
+class LinearDispersiveMaterial : public Material {
+ public:
+  enum class Kind { MLorentz, Drude, Debye };
+  LinearDispersiveMaterial(std::string name, Real epsilon_inf, Kind kind, std::vector<Real> a0,
+                           std::vector<Real> a1, std::vector<Real> b0, std::vector<Real> b1, std::vector<Real> b2,
+                           ElectroMagneticProperty emp = ElectroMagneticProperty::air());
+  bool dispersion() const override { return true; }
+  Kind kind() const { return _kind; }
+  Real epsilonInf() const { return _epsilon_inf; }
+  std::size_t numPoles() const { return _a0.size(); }
+  const std::vector<Real>& a0() const { return _a0; }
+  const std::vector<Real>& a1() const { return _a1; }
+  const std::vector<Real>& b0() const { return _b0; }
+  const std::vector<Real>& b1() const { return _b1; }
+  const std::vector<Real>& b2() const { return _b2; }
+
+ private:
+  Kind _kind;
+  Real _epsilon_inf;
+  std::vector<Real> _a0, _a1, _b0, _b1, _b2;
+};
+
+// MLorentzMaterial::makeMLorentz (dipersive_material.cpp:47-56).
+class MLorentzMaterial : public LinearDispersiveMaterial {
+ public:
+  using LinearDispersiveMaterial::LinearDispersiveMaterial;
+  static std::unique_ptr<MLorentzMaterial> makeMLorentz(std::string name, Real epsilon_inf, std::vector<Real> a_0,
+                                                        std::vector<Real> a_1, std::vector<Real> b_0,
+                                                        std::vector<Real> b_1, std::vector<Real> b_2);
+};
+using LorentzMedium = MLorentzMaterial;
+
+// DrudeMedium::makeDrudeMedium: a0 = wp^2, a1 = 0, b0 = 0, b1 = gamma, b2 = 1
+// (drude_medium.cpp:9-14, dispersive_material_equation.cpp:55-59).
+class DrudeMedium : public LinearDispersiveMaterial {
+ public:
+  using LinearDispersiveMaterial::LinearDispersiveMaterial;
+  static std::unique_ptr<DrudeMedium> makeDrudeMedium(std::string name, Real eps_inf, std::vector<Real> omega_p,
+                                                      std::vector<Real> gamma);
+};
+
+// DebyeMedium::makeDebyeMedium: a0 = eps_s - eps_inf, a1 = 0, b0 = 1,
+// b1 = tau, b2 = 0 (debye_medium.cpp:10-17, dispersive_material_equation.cpp:48-53).
+class DebyeMedium : public LinearDispersiveMaterial {
+ public:
+  using LinearDispersiveMaterial::LinearDispersiveMaterial;
+  static std::unique_ptr<DebyeMedium> makeDebyeMedium(std::string name, Real epsilon_inf,
+                                                      std::vector<Real> epsilon_static, std::vector<Real> tau);
 };
 
 class Waveform {
@@ -209,6 +268,36 @@ class Resistor : public LumpedElement {
 
  private:
   Real _resistance;
+};
+
+// capacitor.h: coefficient-only element (capacitor.cpp:61-112).
+class Capacitor : public LumpedElement {
+ public:
+  Capacitor(std::string name, std::unique_ptr<Cube> cube, Axis::XYZ xyz, Real capacitance,
+            std::unique_ptr<Material> material = Material::createAir());
+  Real capacitance() const { return _capacitance; }
+
+ private:
+  Real _capacitance;
+};
+
+// inductor.h: a current state per edge updated after every E update
+// (inductor.cpp:76-135).  An inductance of 0 is replaced by 1e-10 H.
+class Inductor : public LumpedElement {
+ public:
+  Inductor(std::string name, std::unique_ptr<Cube> cube, Axis::XYZ xyz, Real inductance,
+           std::unique_ptr<Material> material = Material::createAir());
+  Real inductance() const { return _inductance; }
+
+ private:
+  Real _inductance;
+};
+
+// pec_plane.h: a zero-thickness cube whose tangential E edges get the PEC
+// sigma_e after material averaging (pec_plane.cpp:12-67).
+class PecPlane : public Object {
+ public:
+  PecPlane(std::string name, std::unique_ptr<Cube> cube);
 };
 
 class VoltageSource : public LumpedElement {

@@ -171,6 +171,162 @@ std::array<Index, 3> GridSpace::box(const Shape& s, Index* end) const {
   return o;
 }
 
+Material Material::makeLorentz(Real eps_inf, std::vector<Real> a0, std::vector<Real> a1, std::vector<Real> b0,
+                               std::vector<Real> b1, std::vector<Real> b2) {
+  // MLorentzEqDecision: all five parameter arrays have the pole count
+  if (a1.size() != a0.size() || b0.size() != a0.size() || b1.size() != a0.size() || b2.size() != a0.size())
+    throw Error("Modified Lorentz Model paramater doesn't have same size.");
+  Material m = make(1, 1, 0, 0);  // ElectroMagneticProperty::air()
+  m.kind = Lorentz;
+  m.eps_inf = eps_inf;
+  m.a0 = std::move(a0);
+  m.a1 = std::move(a1);
+  m.b0 = std::move(b0);
+  m.b1 = std::move(b1);
+  m.b2 = std::move(b2);
+  return m;
+}
+
+Material Material::makeDrude(Real eps_inf, std::vector<Real> omega_p, std::vector<Real> gamma) {
+  // DrudeEqDecision: a0 = wp*wp, a1 = 0, b0 = 0, b1 = gamma, b2 = 1
+  std::vector<Real> a0(omega_p.size()), zero(omega_p.size(), 0.0), one(omega_p.size(), 1.0);
+  for (Index p = 0; p < omega_p.size(); ++p) a0[p] = omega_p[p] * omega_p[p];
+  Material m = makeLorentz(eps_inf, a0, zero, zero, gamma, one);
+  m.kind = Drude;
+  m.omega_p = std::move(omega_p);
+  m.gamma = std::move(gamma);
+  return m;
+}
+
+Material Material::makeDebye(Real eps_inf, std::vector<Real> eps_static, std::vector<Real> tau) {
+  // DebyeEqDecision: a0 = eps_s - eps_inf, a1 = 0, b0 = 1, b1 = tau, b2 = 0
+  std::vector<Real> a0(eps_static.size()), zero(eps_static.size(), 0.0), one(eps_static.size(), 1.0);
+  for (Index p = 0; p < eps_static.size(); ++p) a0[p] = eps_static[p] - eps_inf;
+  Material m = makeLorentz(eps_inf, a0, zero, one, tau, zero);
+  m.kind = Debye;
+  m.eps_static = std::move(eps_static);
+  m.tau = std::move(tau);
+  return m;
+}
+
+// ---------------------------------------------------------------- ADE --
+ADEStorage::ADEStorage(Kind k, Index p, Index x, Index y, Index z) : kind(k), P(p), nx(x), ny(y), nz(z) {
+  // the arrays each storage allocates (ade_method constructors)
+  const Index n4 = x * y * z * p;
+  cjj.assign(n4, 0);
+  cje.assign(n4, 0);
+  cjsj.assign(n4, 0);
+  cej_sum = Arr3(x, y, z);
+  for (int c = 0; c < 3; ++c) j[c].assign(n4, 0);
+  if (k == MLorentz) {
+    cjjp.assign(n4, 0);
+    cjen.assign(n4, 0);
+    cjep.assign(n4, 0);
+    cjsjp.assign(n4, 0);
+    cee_p = Arr3(x, y, z);
+    for (int c = 0; c < 3; ++c) {
+      e_prev[c] = Arr3(x, y, z);
+      j_prev[c].assign(n4, 0);
+    }
+  }
+}
+
+void ADEStorage::correctCoeff(Sim& s, Index i, Index jj, Index k, const Material& m) {
+  if (P < m.numPoles()) throw Error("The number of poles is not enough.");
+  const Real dt = s.dt;
+  const Real sigma = m.sigma_e;
+  const Index np = m.numPoles();
+  Real cece, b_over;  // cece and the numerator of cecha / cechb
+  Real cecha_num, cechb_num;
+  bool lor_form = false;
+  if (kind == Debye || kind == Drude) {
+    // Debye- or Drude-only storage loops over the storage's pole count
+    // (debye_ade_method.cpp:68-72, drude_ade_method.cpp:79-83); a material
+    // with fewer poles would be read past its arrays, so require equality.
+    if (np != P) throw Error("Debye/Drude storage needs every material to have the storage's pole count");
+    std::vector<Real> kk(P), beta(P);
+    for (Index p = 0; p < P; ++p) {
+      if (kind == Debye) {  // DebyeADEMethodCoefficient (debye_ade_method.cpp:9-31)
+        const Real tau = m.b1[p], de = m.a0[p];
+        kk[p] = (2 * tau - dt) / (2 * tau + dt);
+        beta[p] = 2 * EPSILON_0 * de * dt / (2 * tau + dt);
+      } else {  // DrudeADEMethodCoefficient (drude_ade_method.cpp:9-38)
+        const Real g = m.b1[p], wp = std::sqrt(m.a0[p]);  // DrudeEqDecision::omegaP
+        kk[p] = (1 - g * dt / 2) / (1 + g * dt / 2);
+        beta[p] = (EPSILON_0 * wp * wp * dt / 2) / (1 + g * dt / 2);
+      }
+    }
+    Real sum_beta = 0;
+    for (Index p = 0; p < P; ++p) sum_beta += beta[p];
+    for (Index p = 0; p < P; ++p) {
+      cjj[at(i, jj, k, p)] = kk[p];
+      cje[at(i, jj, k, p)] = kind == Debye ? beta[p] / dt : beta[p];
+      cjsj[at(i, jj, k, p)] = 0.5 * (1 + cjj[at(i, jj, k, p)]);
+    }
+    const Real ei = m.eps_inf;
+    Real a, b;
+    if (kind == Debye) {
+      a = (2 * EPSILON_0 * ei + sum_beta - dt * sigma) / (2 * EPSILON_0 * ei + sum_beta + dt * sigma);
+      b = (2 * dt) / (2 * EPSILON_0 * ei + sum_beta + dt * sigma);
+    } else {  // the reference's Drude a/b subtract dt*sum_beta in both places
+      a = (2 * EPSILON_0 * ei - dt * sum_beta - dt * sigma) / (2 * EPSILON_0 * ei - dt * sum_beta + dt * sigma);
+      b = (2 * dt) / (2 * EPSILON_0 * ei - dt * sum_beta + dt * sigma);
+    }
+    cej_sum(i, jj, k) = -b;
+    cece = a;
+    b_over = b;
+    cecha_num = -b;
+    cechb_num = b;
+  } else {  // MLorentzADEMethodCoefficient (m_lor_ade_method.cpp:8-39, 66-159)
+    lor_form = true;
+    std::vector<Real> A(np), B(np), C(np), D(np), E(np), F(np);
+    for (Index p = 0; p < np; ++p) {
+      const Real a0 = m.a0[p], a1 = m.a1[p], b0 = m.b0[p], b1 = m.b1[p], b2 = m.b2[p];
+      A[p] = (a1 * EPSILON_0 / (dt * dt)) + (0.5 * a0 * EPSILON_0 / dt);
+      B[p] = (-2 * a1 * EPSILON_0) / (dt * dt);
+      C[p] = (a1 * EPSILON_0 / (dt * dt)) - (0.5 * a0 * EPSILON_0 / dt);
+      D[p] = (2 * b2 / (dt * dt)) - b0;
+      E[p] = (-b2 / (dt * dt)) + (b1 / (2 * dt));
+      F[p] = (b2 / (dt * dt)) + (b1 / (2 * dt));
+    }
+    for (Index p = 0; p < np; ++p) {
+      cjen[at(i, jj, k, p)] = A[p] / F[p];
+      cje[at(i, jj, k, p)] = B[p] / F[p];
+      cjep[at(i, jj, k, p)] = C[p] / F[p];
+      cjj[at(i, jj, k, p)] = D[p] / F[p];
+      cjjp[at(i, jj, k, p)] = E[p] / F[p];
+      cjsj[at(i, jj, k, p)] = 0.5 * (1 + D[p] / F[p]);
+      cjsjp[at(i, jj, k, p)] = 0.5 * (E[p] / F[p]);
+    }
+    Real sum_a_f = 0, sum_b_f = 0, sum_c_f = 0;
+    for (Index p = 0; p < np; ++p) sum_a_f += A[p] / F[p];
+    for (Index p = 0; p < np; ++p) sum_b_f += B[p] / F[p];
+    for (Index p = 0; p < np; ++p) sum_c_f += C[p] / F[p];
+    const Real ei = m.eps_inf;
+    const Real temp = (ei * EPSILON_0) / dt + 0.5 * sum_a_f + 0.5 * sigma;
+    cej_sum(i, jj, k) = -1 / temp;
+    cee_p(i, jj, k) = -0.5 * sum_c_f / temp;
+    cece = ((ei * EPSILON_0) / dt - 0.5 * sum_b_f - 0.5 * sigma) / temp;
+    b_over = temp;
+    cecha_num = -1;
+    cechb_num = 1;
+  }
+  // cece / cecha / cechb of all three components at (i, j, k), with the
+  // h-node sizes of the cell; (da, db) = (dy, dz), (dz, dx), (dx, dy)
+  const Real d[3] = {s.g.h_size[0](i), s.g.h_size[1](jj), s.g.h_size[2](k)};
+  for (int c = 0; c < 3; ++c) {
+    const Real da = d[(c + 1) % 3], db = d[(c + 2) % 3];
+    s.c.self(0, c)(i, jj, k) = cece;
+    if (lor_form) {
+      s.c.dualA(0, c)(i, jj, k) = cecha_num / (b_over * db);
+      s.c.dualB(0, c)(i, jj, k) = cechb_num / (b_over * da);
+    } else {
+      s.c.dualA(0, c)(i, jj, k) = cecha_num / db;
+      s.c.dualB(0, c)(i, jj, k) = cechb_num / da;
+    }
+  }
+}
+
 // ---------------------------------------------------------- coefficients --
 Arr3& Coeff::self(int h, int c) {
   if (!h) return c == 0 ? cexe : c == 1 ? ceye : ceze;
@@ -1161,6 +1317,35 @@ static void extentOf(int h, int c, Index nx, Index ny, Index nz, Index out[3]) {
   for (int a = 0; a < 3; ++a) out[a] = h ? (a == c ? n[a] + 1 : n[a]) : (a == c ? n[a] : n[a] + 1);
 }
 
+void Sim::buildDispersiveSpace() {  // simulation.cpp:745-829
+  ade.reset();
+  Index num_pole = 0;
+  bool is_drude = false, is_debye = false, is_lor = false;
+  for (const auto& m : material_array) {
+    if (!m.dispersive()) continue;
+    num_pole = std::max(num_pole, m.numPoles());
+    if (m.kind == Material::Drude) is_drude = true;
+    else if (m.kind == Material::Debye) is_debye = true;
+    else is_lor = true;
+  }
+  if (num_pole == 0) return;
+  ADEStorage::Kind k = ADEStorage::MLorentz;
+  if (is_drude && !is_debye && !is_lor) k = ADEStorage::Drude;
+  else if (!is_drude && is_debye && !is_lor) k = ADEStorage::Debye;
+  ade = std::make_unique<ADEStorage>(k, num_pole, g.nx(), g.ny(), g.nz());
+  // Object::handleDispersion (object.cpp:52-98): every cell whose material
+  // index is the object's gets the object's ADE coefficients
+  const Index nx = g.nx(), ny = g.ny(), nz = g.nz();
+  for (auto& o : objects) {
+    if (!o->material.dispersive()) continue;
+    for (Index i = 0; i < nx; ++i)
+      for (Index j = 0; j < ny; ++j)
+        for (Index kk = 0; kk < nz; ++kk)
+          if (material_grid[(i * ny + j) * nz + kk] == o->material_index)
+            ade->correctCoeff(*this, i, j, kk, o->material);
+  }
+}
+
 void Sim::generateMaterialSpace() {  // simulation.cpp:593-636
   const Index nx = g.nx(), ny = g.ny(), nz = g.nz();
   material_grid.assign(nx * ny * nz, std::numeric_limits<Index>::max());
@@ -1275,6 +1460,7 @@ void Sim::init() {  // simulation.cpp:319-371
   calculateCoefficient();
   for (auto& o : objects) o->correctUpdateCoefficient(*this);  // simulation.cpp:638-652
   for (auto& p : pmls) p->correctUpdateCoefficient(*this);
+  buildDispersiveSpace();  // simulation.cpp:651
   for (auto& m : monitors) m->init(*this);
 }
 
@@ -1319,9 +1505,72 @@ struct Domain {
                           c.chzey(i, j, k) * (f.ey(i + 1, j, k) - f.ey(i, j, k));
   }
 
+  // TemplateADEUpdateScheme::updateE (template_ade_update_scheme.h:14-88)
+  // for Drude / Debye storage and MLorentzUpdator::updateE
+  // (m_lor_ade_updator.cpp:21-87) for modified-Lorentz storage, over the
+  // same ranges as the plain update, for every cell.
+  void updateEAde(int comp) {
+    auto& f = s->emf;
+    auto& c = s->c;
+    ADEStorage& st = *s->ade;
+    const int A = (comp + 1) % 3, B = (comp + 2) % 3;
+    Arr3& e = f.f(0, comp);
+    const Arr3& ha = f.f(1, A);
+    const Arr3& hb = f.f(1, B);
+    const Arr3& cs = c.self(0, comp);
+    const Arr3& ca = c.dualA(0, comp);
+    const Arr3& cb = c.dualB(0, comp);
+    Index lo[3] = {task.x.s, task.y.s, task.z.s};
+    const Index hi[3] = {task.x.e, task.y.e, task.z.e};
+    if (lo[A] == 0) lo[A] = 1;
+    if (lo[B] == 0) lo[B] = 1;
+    const bool lor = st.kind == ADEStorage::MLorentz;
+    std::vector<Real>& J = st.j[comp];
+    for (Index i = lo[0]; i < hi[0]; ++i)
+      for (Index j = lo[1]; j < hi[1]; ++j)
+        for (Index k = lo[2]; k < hi[2]; ++k) {
+          Index ia[3] = {i, j, k}, ib[3] = {i, j, k};
+          ia[B] -= 1;  // dual A neighbour one back along B
+          ib[A] -= 1;  // dual B neighbour one back along A
+          Real j_sum = 0;
+          for (Index p = 0; p < st.P; ++p) {
+            const Index q = st.at(i, j, k, p);
+            if (lor)
+              j_sum += st.cjsj[q] * J[q] + st.cjsjp[q] * st.j_prev[comp][q];
+            else
+              j_sum += st.cjsj[q] * J[q];
+          }
+          const Real e_prev = lor ? st.e_prev[comp](i, j, k) : 0.0;
+          const Real cee_p = lor ? st.cee_p(i, j, k) : 0.0;
+          const Real e_cur = e(i, j, k);
+          const Real e_next = cs(i, j, k) * e(i, j, k) + ca(i, j, k) * (ha(i, j, k) - ha(ia[0], ia[1], ia[2])) +
+                              cb(i, j, k) * (hb(i, j, k) - hb(ib[0], ib[1], ib[2]));
+          e(i, j, k) = cee_p * e_prev + e_next + st.cej_sum(i, j, k) * j_sum;
+          const Real en = e(i, j, k);
+          for (Index p = 0; p < st.P; ++p) {
+            const Index q = st.at(i, j, k, p);
+            if (st.kind == ADEStorage::Debye) {  // debye_ade_updator.cpp:41-49
+              J[q] = st.cjj[q] * J[q] + st.cje[q] * (en - e_cur);
+            } else if (st.kind == ADEStorage::Drude) {  // drude_ade_updator.cpp:31-39
+              J[q] = st.cjj[q] * J[q] + st.cje[q] * (en + e_cur);
+            } else {  // m_lor_ade_updator.cpp:89-112
+              const Real jn = st.cjen[q] * en + st.cje[q] * e_cur + st.cjep[q] * e_prev + st.cjj[q] * J[q] +
+                              st.cjjp[q] * st.j_prev[comp][q];
+              st.j_prev[comp][q] = J[q];
+              J[q] = jn;
+            }
+          }
+          if (lor) st.e_prev[comp](i, j, k) = e_cur;
+        }
+  }
+
   void updateE() {  // basic_updator_3d.cpp:38-69
     auto& f = s->emf;
     auto& c = s->c;
+    if (s->ade) {
+      for (int comp = 0; comp < 3; ++comp) updateEAde(comp);
+      return;
+    }
     const Index xs = task.x.s, ys = task.y.s, zs = task.z.s;
     const Index xe = task.x.e, ye = task.y.e, ze = task.z.e;
     const Index xs1 = xs == 0 ? 1 : xs, ys1 = ys == 0 ? 1 : ys, zs1 = zs == 0 ? 1 : zs;

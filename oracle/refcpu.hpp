@@ -196,6 +196,33 @@ struct Material {
   bool dispersive() const { return kind != Plain; }
   Index numPoles() const { return a0.size(); }
   static Material make(Real eps_r, Real mu_r, Real se, Real sm);
+  // MLorentzMaterial::makeMLorentz (dipersive_material.cpp:47-56)
+  static Material makeLorentz(Real eps_inf, std::vector<Real> a0, std::vector<Real> a1,
+                              std::vector<Real> b0, std::vector<Real> b1, std::vector<Real> b2);
+  // DrudeMedium::makeDrudeMedium + DrudeEqDecision (drude_medium.cpp:9-14,
+  // dispersive_material_equation.cpp:55-59)
+  static Material makeDrude(Real eps_inf, std::vector<Real> omega_p, std::vector<Real> gamma);
+  // DebyeMedium::makeDebyeMedium + DebyeEqDecision (debye_medium.cpp:10-17,
+  // dispersive_material_equation.cpp:48-53)
+  static Material makeDebye(Real eps_inf, std::vector<Real> eps_static, std::vector<Real> tau);
+};
+
+struct Sim;
+
+// ADEMethodStorage and its Drude / Debye / modified-Lorentz forms
+// (ade_method.h, drude_ade_method.cpp, debye_ade_method.cpp,
+// m_lor_ade_method.cpp): per-cell (nx, ny, nz) and per-pole (p fastest)
+// coefficient and state arrays.
+struct ADEStorage {
+  enum Kind { Drude = 0, Debye = 1, MLorentz = 2 } kind;
+  Index P = 0, nx = 0, ny = 0, nz = 0;
+  std::vector<Real> cjj, cjjp, cjen, cje, cjep, cjsj, cjsjp;  // (nx, ny, nz, P)
+  Arr3 cej_sum, cee_p;                                       // (nx, ny, nz)
+  Arr3 e_prev[3];
+  std::vector<Real> j[3], j_prev[3];                         // (nx, ny, nz, P)
+  ADEStorage(Kind k, Index p, Index x, Index y, Index z);
+  Index at(Index i, Index jj, Index k, Index p) const { return ((i * ny + jj) * nz + k) * P + p; }
+  void correctCoeff(Sim& s, Index i, Index jj, Index k, const Material& m);
 };
 
 // ----------------------------------------------------------------- grid ---
@@ -423,6 +450,7 @@ struct Sim {
   MaterialParam mp;
   std::vector<Material> material_array;
   std::vector<Index> material_grid;  // per-cell material index (grid_space.cpp:357-368)
+  std::unique_ptr<ADEStorage> ade;   // Simulation::_ade_method_storage
   Real dt = 0;
   Index start_step = 0, size = 0, current = 0;
   double loop_seconds = 0;
@@ -434,6 +462,7 @@ struct Sim {
   void generateMaterialSpace();         // simulation.cpp:593-636
   void averageMaterial();               // calculation_param.cpp:43-168
   void calculateCoefficient();          // calculation_param.cpp:170-264
+  void buildDispersiveSpace();          // simulation.cpp:745-829, object.cpp:52-98
   void init();                          // simulation.cpp:319-371
   void initSteps(Index n);              // simulation.cpp:373-389
   void run(Index n);                    // simulation.cpp:284-289

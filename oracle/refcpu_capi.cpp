@@ -111,6 +111,27 @@ Material parseMaterial(std::istringstream& is) {
     Real er = num(is), mr = num(is), se = num(is), sm = num(is);
     return Material::make(er, mr, se, sm);
   }
+  auto list = [&](Index n) {
+    std::vector<Real> v(n);
+    for (auto& x : v) x = num(is);
+    return v;
+  };
+  if (kind == "lorentz" || kind == "drude" || kind == "debye") {
+    const Real eps_inf = num(is);
+    const Index n = static_cast<Index>(num(is));
+    if (kind == "drude") {
+      auto wp = list(n);
+      auto g = list(n);
+      return Material::makeDrude(eps_inf, wp, g);
+    }
+    if (kind == "debye") {
+      auto es = list(n);
+      auto tau = list(n);
+      return Material::makeDebye(eps_inf, es, tau);
+    }
+    auto a0 = list(n), a1 = list(n), b0 = list(n), b1 = list(n), b2 = list(n);
+    return Material::makeLorentz(eps_inf, a0, a1, b0, b1, b2);
+  }
   throw Error("unknown material kind " + kind);
 }
 

@@ -15,6 +15,7 @@ import pytest
 import eigenmode as em
 from oracle import RefSim, rel_l2
 from xfdtd_core_amd import native, scenarios
+from xfdtd_core_amd import spec as S
 
 pytestmark = pytest.mark.gpu
 
@@ -145,6 +146,42 @@ def test_lumped_oblique_tfsf_fp64():
 
     sp = _lumped_box().spec()
     out = _compare_run(sp, 250, (1, 1, 1), "lumped")
+    for k, (e, _) in out.items():
+        assert e <= TOL_F64, (k, e)
+
+
+def test_circuit_elements_fp64():
+    """Capacitors, inductors (per-edge current state on the device), PEC
+    planes, a resistor and a voltage source beside a lossy sphere and CPML."""
+    from test_precompute_parity import _circuit_box
+
+    sp = _circuit_box()
+    sp.addMonitor(S.MovieMonitor(S.FieldMonitor(S.Cube(S.Vector(-0.010, -0.011, 0.0), S.Vector(0.020, 0.022, 0.001)),
+                                                "EZ"), 25, "ez_xy", ""))
+    # a zero-thickness box records empty frames, as in the reference
+    sp.addMonitor(S.MovieMonitor(S.FieldMonitor(S.Cube(S.Vector(-0.010, -0.011, 0.0), S.Vector(0.020, 0.022, 0.0)),
+                                                "EY"), 50, "ey_empty", ""))
+    out = _compare_run(sp.spec(), 300, (1, 1, 1), "circuit")
+    for k, (e, _) in out.items():
+        assert e <= TOL_F64, (k, e)
+
+
+DISPERSIVE = {
+    "mixed": dict(n_total=48, kinds=("drude", "debye", "lorentz", "mlorentz")),
+    "drude_only": dict(n_total=40, kinds=("drude",)),
+    "debye_only": dict(n_total=40, kinds=("debye", "debye")),
+}
+
+
+@pytest.mark.parametrize("case", sorted(DISPERSIVE))
+def test_dispersive_ade_fp64(case):
+    """ADE dispersion on the device (per-cell material ids, pole currents):
+    the mixed scene runs the modified-Lorentz scheme, the others the
+    Drude-only and Debye-only schemes; TFSF, resistor and voltage source in
+    the same box."""
+    kw = DISPERSIVE[case]
+    sp = scenarios.dispersive_slabs(pml=5, slab=6, gap=2, tfsf_offset=7, snapshot_interval=100, **kw).spec()
+    out = _compare_run(sp, 300, (1, 1, 1), f"ade-{case}")
     for k, (e, _) in out.items():
         assert e <= TOL_F64, (k, e)
 

@@ -47,11 +47,41 @@ def _lumped_box():
     return s
 
 
+def _circuit_box():
+    """Capacitors, inductors and PEC planes (x, y and z planes) beside a
+    resistor and a voltage source, a dielectric sphere and PML."""
+    dl = 1e-3
+    s = S.Simulation(dl, dl, dl, 0.9)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-0.010, -0.011, -0.009), S.Vector(0.020, 0.022, 0.018)),
+                         S.Material.createAir()))
+    s.addObject(S.Object("ball", S.Sphere(S.Vector(0.001, 0.002, -0.001), 0.004),
+                         S.Material("m", S.ElectroMagneticProperty(3.0, 1.0, 0.01, 0.0))))
+    s.addObject(S.Capacitor("c1", S.Cube(S.Vector(0.003, -0.004, -0.003), S.Vector(0.002, 0.001, 0.003)), "Z", 2e-12))
+    s.addObject(S.Inductor("l1", S.Cube(S.Vector(-0.005, 0.004, 0.001), S.Vector(0.0, 0.002, 0.002)), "Y", 3e-9))
+    s.addObject(S.Inductor("l2", S.Cube(S.Vector(-0.002, -0.006, 0.003), S.Vector(0.003, 0.0, 0.0)), "X", 0.0))
+    s.addObject(S.Capacitor("c2", S.Cube(S.Vector(0.004, 0.005, -0.004), S.Vector(0.0, 0.0, 0.002)), "Z", 1e-13))
+    s.addObject(S.PecPlane("px", S.Cube(S.Vector(0.006, -0.008, -0.006), S.Vector(0.0, 0.006, 0.005))))
+    s.addObject(S.PecPlane("py", S.Cube(S.Vector(-0.007, -0.004, -0.005), S.Vector(0.004, 0.0, 0.004))))
+    s.addObject(S.PecPlane("pz", S.Cube(S.Vector(-0.006, 0.0, 0.005), S.Vector(0.005, 0.004, 0.0))))
+    s.addObject(S.Resistor("r1", S.Cube(S.Vector(0.0, 0.006, -0.002), S.Vector(0.0, 0.0, 0.003)), "Z", 50.0))
+    s.addObject(S.VoltageSource("v1", S.Cube(S.Vector(-0.003, -0.002, -0.004), S.Vector(0.0, 0.0, 0.002)), "ZP",
+                                50.0, S.Waveform.gaussian(2e-11, 9e-11, 1.0)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(5, d))
+    return s
+
+
 CASES = {
     "dielectric_sphere": lambda: scenarios.dielectric_sphere().spec(),
     "inhomogeneous_box_small": lambda: scenarios.inhomogeneous_box(n_total=40, pml=6, n_spheres=5,
                                                                    r_cells=(2, 6), seed=7).spec(),
     "lumped_box": lambda: _lumped_box().spec(),
+    "circuit_box": lambda: _circuit_box().spec(),
+    "dispersive_mixed": lambda: scenarios.dispersive_slabs(n_total=48, pml=5, slab=6, gap=2, tfsf_offset=7).spec(),
+    "dispersive_drude_only": lambda: scenarios.dispersive_slabs(n_total=40, pml=5, slab=6, gap=2, tfsf_offset=7,
+                                                                kinds=("drude",)).spec(),
+    "dispersive_debye_only": lambda: scenarios.dispersive_slabs(n_total=40, pml=5, slab=6, gap=2, tfsf_offset=7,
+                                                                kinds=("debye", "debye")).spec(),
 }
 
 

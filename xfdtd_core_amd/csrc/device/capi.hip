@@ -222,6 +222,13 @@ struct xfd_ctx {
   DevBuf pool;
   int64_t pool_rows = 0;  // smallest table row count: steps must stay below it
   DevBuf rec_e, seg_e, rec_h, seg_h;
+  DevBuf state;  // inductor currents, one per inductor edge
+  // dispersive materials (ADE): scheme or -1, per-cell material ids,
+  // per-material coefficient table, pole currents, previous currents and
+  // previous E (full padded arrays per component and pole)
+  int ade_scheme = -1;
+  int ade_poles = 0;
+  DevBuf ade_mat, ade_tab, ade_j[3], ade_jp[3], ade_eprev[3];
   int64_t nseg_e = 0, nseg_h = 0;
 
   struct Gather {
@@ -263,7 +270,9 @@ struct xfd_ctx {
       for (auto& x : s->psi) b += x.bytes;
       b += s->ae.bytes + s->be.bytes + s->ah.bytes + s->bh.bytes;
     }
-    b += pool.bytes + rec_e.bytes + seg_e.bytes + rec_h.bytes + seg_h.bytes;
+    b += pool.bytes + rec_e.bytes + seg_e.bytes + rec_h.bytes + seg_h.bytes + state.bytes;
+    b += ade_mat.bytes + ade_tab.bytes;
+    for (int q = 0; q < 3; ++q) b += ade_j[q].bytes + ade_jp[q].bytes + ade_eprev[q].bytes;
     for (auto& gth : gathers) b += gth->out.bytes;
     return b;
   }
@@ -563,6 +572,8 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   int64_t min_rows = INT64_MAX;
   for (int li = 0; li < d.n_lumped; ++li) {
     const auto& L = d.lumped[li];
+    require(L.kind == XFD_LUMPED_SOURCE || L.kind == XFD_LUMPED_INDUCTOR, "bad lumped element kind");
+    if (L.kind != XFD_LUMPED_SOURCE) continue;
     require(L.waveform != nullptr && L.nt > 0, "lumped waveform missing");
     lumped_off[li] = static_cast<int64_t>(pool.size());
     pool.insert(pool.end(), L.waveform, L.waveform + L.nt);
@@ -581,9 +592,12 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   uploadVector(pool, c.pool);
 
   std::vector<xfd::Record> re, rh;
+  int64_t nstate = 0;
   for (int li = 0; li < d.n_lumped; ++li) {  // object correctors first
     const auto& L = d.lumped[li];
     require(L.component >= 0 && L.component < 3 && L.coff != nullptr, "bad lumped element");
+    const bool inductor = L.kind == XFD_LUMPED_INDUCTOR;
+    require(!inductor || L.coff_j != nullptr, "inductor cjcec missing");
     int ext[3];
     extentOf(L.component, g.nx, g.ny, g.nz, ext);
     for (int q = 0; q < 3; ++q)
@@ -593,18 +607,30 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       for (int64_t j = 0; j < L.size[1]; ++j)
         for (int64_t k = 0; k < L.size[2]; ++k) {
           xfd::Record r{};
-          r.kind = 1;
+          const int64_t q = (i * L.size[1] + j) * L.size[2] + k;
           r.field = L.component;
           r.target = padIndex(g, L.origin[0] + i, L.origin[1] + j, L.origin[2] + k);
-          r.coef = L.coff[(i * L.size[1] + j) * L.size[2] + k];
-          r.pool = lumped_off[li];
-          r.row = 1;
+          r.coef = L.coff[q];
+          if (inductor) {
+            r.kind = 2;
+            r.w = L.coff_j[q];
+            r.pool = nstate++;  // this edge's J slot
+            r.row = 0;
+          } else {
+            r.kind = 1;
+            r.pool = lumped_off[li];
+            r.row = 1;
+          }
           re.push_back(r);
         }
   }
   for (int ti = 0; ti < d.n_tfsf; ++ti) {
     tfsfRecords(g, d.tfsf[ti], e_off[ti], h_off[ti], false, re);
     tfsfRecords(g, d.tfsf[ti], e_off[ti], h_off[ti], true, rh);
+  }
+  if (nstate) {  // inductor currents start at zero
+    std::vector<T> z(static_cast<size_t>(nstate), T(0));
+    uploadVector(z, c.state);
   }
   std::vector<int64_t> se, sh;
   segment(re, se);
@@ -615,6 +641,36 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   uploadVector(sh, c.seg_h);
   c.nseg_e = re.empty() ? 0 : static_cast<int64_t>(se.size()) - 1;
   c.nseg_h = rh.empty() ? 0 : static_cast<int64_t>(sh.size()) - 1;
+
+  if (d.ade) {
+    const auto& A = *d.ade;
+    require(A.scheme >= XFD_ADE_DRUDE && A.scheme <= XFD_ADE_MLORENTZ, "bad ADE scheme");
+    require(A.poles > 0 && A.n_materials > 0 && A.n_materials <= 32767, "bad ADE pole or material count");
+    require(A.coeff != nullptr && A.cell_material != nullptr, "ADE arrays missing");
+    const int stride = 2 + 7 * A.poles;
+    std::vector<T> tab(static_cast<size_t>(A.n_materials) * stride);
+    for (size_t q = 0; q < tab.size(); ++q) tab[q] = static_cast<T>(A.coeff[q]);
+    uploadVector(tab, c.ade_tab);
+    std::vector<int16_t> mat(static_cast<size_t>(g.total), int16_t(-1));
+    for (int64_t i = 0; i < g.nx; ++i)
+      for (int64_t j = 0; j < g.ny; ++j)
+        for (int64_t k = 0; k < g.nz; ++k) {
+          const int16_t m = A.cell_material[(i * g.ny + j) * g.nz + k];
+          require(m >= -1 && m < A.n_materials, "ADE cell material out of range");
+          mat[padIndex(g, i, j, k)] = m;
+        }
+    uploadVector(mat, c.ade_mat);
+    const bool lor = A.scheme == XFD_ADE_MLORENTZ;
+    for (int q = 0; q < 3; ++q) {
+      c.ade_j[q].alloc(static_cast<size_t>(A.poles) * g.total * sizeof(T));
+      if (lor) {
+        c.ade_jp[q].alloc(static_cast<size_t>(A.poles) * g.total * sizeof(T));
+        c.ade_eprev[q].alloc(static_cast<size_t>(g.total) * sizeof(T));
+      }
+    }
+    c.ade_scheme = A.scheme;
+    c.ade_poles = A.poles;
+  }
 
   for (int gi = 0; gi < d.n_gather; ++gi) {
     const auto& G = d.gather[gi];
@@ -709,6 +765,19 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
     K.id[q] = c.ids[q].as<uint16_t>();
     K.tab[q] = c.tabs[q].as<T>();
   }
+  xfd::AdeArgs<T> A{};
+  if (c.ade_scheme >= 0) {
+    A.poles = c.ade_poles;
+    A.stride = 2 + 7 * c.ade_poles;
+    A.total = c.g.total;
+    A.mat = c.ade_mat.as<int16_t>();
+    A.tab = c.ade_tab.as<T>();
+    for (int q = 0; q < 3; ++q) {
+      A.j[q] = c.ade_j[q].as<T>();
+      A.jp[q] = c.ade_jp[q].as<T>();
+      A.eprev[q] = c.ade_eprev[q].as<T>();
+    }
+  }
   std::vector<xfd::SlabArgs<T>> se, sh;
   for (auto& s : c.slabs) {
     se.push_back(slabArgs<T>(*s, true));
@@ -728,19 +797,25 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
     if (c.nseg_h)
       c.timed(K_REC_H, [&] {
         xfd::Launch<T>::records(st, F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h,
-                                c.pool.as<double>(), t);
+                                c.pool.as<double>(), c.state.as<T>(), t);
       });
     for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, false); });
     c.timed(K_UPDATE_E, [&] {
-      if (c.classed)
+      if (c.ade_scheme >= 0) {
+        if (c.classed)
+          xfd::Launch<T>::update_e_ade(st, c.g, F, K, A, c.ade_scheme, c.ichunk);
+        else
+          xfd::Launch<T>::update_e_ade(st, c.g, F, C, A, c.ade_scheme, c.ichunk);
+      } else if (c.classed) {
         xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk);
-      else
+      } else {
         xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk);
+      }
     });
     if (c.nseg_e)
       c.timed(K_REC_E, [&] {
         xfd::Launch<T>::records(st, F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e,
-                                c.pool.as<double>(), t);
+                                c.pool.as<double>(), c.state.as<T>(), t);
       });
     for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
     if (c.nprobe && t < c.probe_count)

@@ -150,12 +150,104 @@ __global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T
   }
 }
 
-// Lumped-source and TFSF corrections: one thread per corrected cell applies
-// that cell's records in the reference's corrector order.
+enum AdeScheme { kAdeDrude = 0, kAdeDebye = 1, kAdeMLorentz = 2 };
+
+// One dispersive E update after the plain curl update `en` of the old value
+// `cur`: TemplateADEUpdateScheme::updateE (template_ade_update_scheme.h:
+// 62-82) with DebyeADEUpdator / DrudeADEUpdator (debye_ade_updator.cpp:
+// 31-49, drude_ade_updator.cpp:21-39), or MLorentzUpdator::updateE / updateJ
+// (m_lor_ade_updator.cpp:66-135).  E = cee_prev * E_prev + en + cej * sum_J,
+// the pole sum taken before the currents advance.
+template <class T, int S>
+__device__ __forceinline__ T adeUpdate(const AdeArgs<T>& A, int m, int q, int64_t idx, T cur, T en) {
+  const T* c = A.tab + static_cast<int64_t>(m) * A.stride;
+  T jsum = T(0);
+  for (int p = 0; p < A.poles; ++p) {
+    const T* cp = c + 2 + 7 * p;
+    const int64_t o = p * A.total + idx;
+    if (S == kAdeMLorentz)
+      jsum += cp[5] * A.j[q][o] + cp[6] * A.jp[q][o];
+    else
+      jsum += cp[5] * A.j[q][o];
+  }
+  const T eprev = S == kAdeMLorentz ? A.eprev[q][idx] : T(0);
+  const T ceep = S == kAdeMLorentz ? c[1] : T(0);
+  const T enew = ceep * eprev + en + c[0] * jsum;
+  for (int p = 0; p < A.poles; ++p) {
+    const T* cp = c + 2 + 7 * p;
+    const int64_t o = p * A.total + idx;
+    const T jv = A.j[q][o];
+    if (S == kAdeDebye) {
+      A.j[q][o] = cp[3] * jv + cp[1] * (enew - cur);
+    } else if (S == kAdeDrude) {
+      A.j[q][o] = cp[3] * jv + cp[1] * (enew + cur);
+    } else {
+      const T jn = cp[0] * enew + cp[1] * cur + cp[2] * eprev + cp[3] * jv + cp[4] * A.jp[q][o];
+      A.jp[q][o] = jv;
+      A.j[q][o] = jn;
+    }
+  }
+  if (S == kAdeMLorentz) A.eprev[q][idx] = cur;
+  return enew;
+}
+
+// The E sweep of k_update_e with the dispersive update on cells whose
+// material id is set (the reference runs its ADE updator on every cell; on
+// a cell without dispersion its coefficients are zero and the result is the
+// plain update).
+template <class T, class Coef, int S>
+__global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, int ichunk) {
+  const int k = blockIdx.x * kLanesK + threadIdx.x;
+  const int j = blockIdx.y * kRowsJ + threadIdx.y;
+  const int i0 = blockIdx.z * ichunk;
+  if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
+  const int i1 = min(i0 + ichunk, g.nx);
+  const int64_t pz = g.pz, pl = g.plane;
+  int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  T* __restrict__ ex = F.f[0];
+  T* __restrict__ ey = F.f[1];
+  T* __restrict__ ez = F.f[2];
+  const T* __restrict__ hx = F.f[3];
+  const T* __restrict__ hy = F.f[4];
+  const T* __restrict__ hz = F.f[5];
+  T hz_p = i0 > 0 ? hz[idx - pl] : T(0);
+  T hy_p = i0 > 0 ? hy[idx - pl] : T(0);
+  const bool jk = j >= 1 && k >= 1;
+  for (int i = i0; i < i1; ++i, idx += pl) {
+    const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
+    const int m = A.mat[idx];
+    T s, a, b;
+    if (jk) {
+      C.get(0, idx, s, a, b);
+      const T cur = ex[idx];
+      const T en = s * cur + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
+      ex[idx] = m < 0 ? en : adeUpdate<T, S>(A, m, 0, idx, cur, en);
+    }
+    if (i >= 1 && k >= 1) {
+      C.get(1, idx, s, a, b);
+      const T cur = ey[idx];
+      const T en = s * cur + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
+      ey[idx] = m < 0 ? en : adeUpdate<T, S>(A, m, 1, idx, cur, en);
+    }
+    if (i >= 1 && j >= 1) {
+      C.get(2, idx, s, a, b);
+      const T cur = ez[idx];
+      const T en = s * cur + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
+      ez[idx] = m < 0 ? en : adeUpdate<T, S>(A, m, 2, idx, cur, en);
+    }
+    hz_p = hz0;
+    hy_p = hy0;
+  }
+}
+
+// Lumped-element and TFSF corrections: one thread per corrected cell applies
+// that cell's records in the reference's corrector order.  `state` holds the
+// inductor currents (one slot per inductor edge).
 template <class T>
 __global__ __launch_bounds__(256) void k_records(FieldPtrs<T> F, const Record* __restrict__ rec,
                                                  const int64_t* __restrict__ seg, int64_t nseg,
-                                                 const double* __restrict__ pool, int64_t t) {
+                                                 const double* __restrict__ pool, T* __restrict__ state,
+                                                 int64_t t) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   const int64_t r0 = seg[s], r1 = seg[s + 1];
@@ -164,15 +256,21 @@ __global__ __launch_bounds__(256) void k_records(FieldPtrs<T> F, const Record* _
   double acc = static_cast<double>(f[target]);
   for (int64_t r = r0; r < r1; ++r) {
     const Record R = rec[r];
-    const double* row = pool + R.pool + t * R.row;
     if (R.kind == 0) {
       // TFSFCorrector::exInc..hzInc (tfsf_corrector.cpp:16-122) times the
       // signed coefficient (tfsf_corrector.h:216-225)
+      const double* row = pool + R.pool + t * R.row;
       const double inc = R.tr * (R.omw * row[0] + R.w * row[1]);
       acc = acc + inc * R.coef;
-    } else {
+    } else if (R.kind == 1) {
       // Voltage/CurrentSourceCorrector: E += coff * w[t]
-      acc = acc + R.coef * row[0];
+      acc = acc + R.coef * pool[R.pool + t];
+    } else {
+      // InductorCorrector::correctE: E += cecjc * J, then J += cjcec * E
+      const double j = static_cast<double>(state[R.pool]);
+      acc = acc + R.coef * j;
+      if (sizeof(T) == sizeof(float)) acc = static_cast<double>(static_cast<T>(acc));
+      state[R.pool] = static_cast<T>(j + R.w * acc);
     }
     if (sizeof(T) == sizeof(float)) acc = static_cast<double>(static_cast<T>(acc));
   }
@@ -225,11 +323,15 @@ struct Launch {
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
   static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk);
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk);
+  static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
+                           const AdeArgs<T>& A, int scheme, int ichunk);
+  static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C,
+                           const AdeArgs<T>& A, int scheme, int ichunk);
   static void probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                      int64_t t);
   static void cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
   static void records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
-                      const double* pool, int64_t t);
+                      const double* pool, T* state, int64_t t);
   static void gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out);
   static void unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2);
   static void pack(hipStream_t s, const Grid& g, const T* src, double* dst, int n0, int n1, int n2);

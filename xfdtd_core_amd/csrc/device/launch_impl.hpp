@@ -36,6 +36,34 @@ void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, co
   launchUpdate<T>(false, s, g, F, C, ichunk);
 }
 
+template <class T, class Coef>
+void launchUpdateAde(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
+                     int scheme, int ichunk) {
+  dim3 block(kLanesK, kRowsJ);
+  dim3 grid(blocks_for(g.nz, kLanesK), blocks_for(g.ny, kRowsJ), blocks_for(g.nx, ichunk));
+  switch (scheme) {
+    case kAdeDrude:
+      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeDrude>), grid, block, 0, s, g, F, C, A, ichunk);
+      break;
+    case kAdeDebye:
+      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeDebye>), grid, block, 0, s, g, F, C, A, ichunk);
+      break;
+    default:
+      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeMLorentz>), grid, block, 0, s, g, F, C, A, ichunk);
+      break;
+  }
+}
+template <class T>
+void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
+                             const AdeArgs<T>& A, int scheme, int ichunk) {
+  launchUpdateAde<T>(s, g, F, C, A, scheme, ichunk);
+}
+template <class T>
+void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C,
+                             const AdeArgs<T>& A, int scheme, int ichunk) {
+  launchUpdateAde<T>(s, g, F, C, A, scheme, ichunk);
+}
+
 template <class T>
 void Launch<T>::probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                        int64_t t) {
@@ -55,9 +83,10 @@ void Launch<T>::cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const 
 
 template <class T>
 void Launch<T>::records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
-                        const double* pool, int64_t t) {
+                        const double* pool, T* state, int64_t t) {
   if (nseg <= 0) return;
-  hipLaunchKernelGGL(k_records<T>, dim3(blocks_for(nseg, 256)), dim3(256), 0, s, F, rec, seg, nseg, pool, t);
+  hipLaunchKernelGGL(k_records<T>, dim3(blocks_for(nseg, 256)), dim3(256), 0, s, F, rec, seg, nseg, pool, state,
+                     t);
 }
 
 template <class T>

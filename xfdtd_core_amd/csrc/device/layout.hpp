@@ -56,6 +56,25 @@ struct ClassCoef {
   }
 };
 
+// ADE state and per-material coefficients for the dispersive E update.
+// `mat` is a per-cell material id in the padded layout (-1: plain update);
+// material m's coefficients start at tab + m * stride:
+//   [0] cej_sum  [1] cee_prev  then per pole p at 2 + 7p:
+//   cj_en, cj_e, cj_ep, cj_j, cj_jp, cjsum_j, cjsum_jp.
+// j[q] / jp[q] hold `poles` padded arrays each (pole p at offset p * total);
+// jp and eprev are used by the modified-Lorentz scheme only.
+template <class T>
+struct AdeArgs {
+  int poles;
+  int stride;
+  int64_t total;
+  const int16_t* mat;
+  const T* tab;
+  T* j[3];
+  T* jp[3];
+  T* eprev[3];
+};
+
 // One CPML slab for one half-step.  The thread space is the union box of the
 // two corrected components; `ext_a` / `ext_b` give each component's extent
 // along the tangential axes a and b (the other component's is one larger).

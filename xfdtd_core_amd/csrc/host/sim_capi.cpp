@@ -122,6 +122,27 @@ std::unique_ptr<Material> material(Tokens& t, const std::string& name) {
     Real er = t.num(), mr = t.num(), se = t.num(), sm = t.num();
     return std::make_unique<Material>(name, ElectroMagneticProperty(er, mr, se, sm));
   }
+  if (kind == "lorentz" || kind == "drude" || kind == "debye") {
+    const Real eps_inf = t.num();
+    const auto n = static_cast<std::size_t>(t.num());
+    auto list = [&] {
+      std::vector<Real> v(n);
+      for (auto& x : v) x = t.num();
+      return v;
+    };
+    if (kind == "drude") {
+      auto wp = list();
+      auto g = list();
+      return DrudeMedium::makeDrudeMedium(name, eps_inf, wp, g);
+    }
+    if (kind == "debye") {
+      auto es = list();
+      auto tau = list();
+      return DebyeMedium::makeDebyeMedium(name, eps_inf, es, tau);
+    }
+    auto a0 = list(), a1 = list(), b0 = list(), b1 = list(), b2 = list();
+    return MLorentzMaterial::makeMLorentz(name, eps_inf, a0, a1, b0, b1, b2);
+  }
   throw XFDTDException("unknown material kind " + kind);
 }
 
@@ -179,6 +200,18 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
       auto cb = cube(t);
       auto a = parseAxis(t.word());
       s.addObject(std::make_shared<Resistor>(name, std::move(cb), a, t.num()));
+    } else if (d == "capacitor" || d == "inductor") {
+      std::string name = t.word();
+      auto cb = cube(t);
+      auto a = parseAxis(t.word());
+      const Real v = t.num();
+      if (d == "capacitor")
+        s.addObject(std::make_shared<Capacitor>(name, std::move(cb), a, v));
+      else
+        s.addObject(std::make_shared<Inductor>(name, std::move(cb), a, v));
+    } else if (d == "pec_plane") {
+      std::string name = t.word();
+      s.addObject(std::make_shared<PecPlane>(name, cube(t)));
     } else if (d == "movie") {
       auto f = parseField(t.word());
       auto cb = cube(t);

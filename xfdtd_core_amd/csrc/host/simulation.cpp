@@ -131,6 +131,48 @@ ElectroMagneticProperty::ElectroMagneticProperty(Real er, Real mr, Real se, Real
   if (_sigma_m == 0) _sigma_m = kSigmaZero;
 }
 
+LinearDispersiveMaterial::LinearDispersiveMaterial(std::string name, Real epsilon_inf, Kind kind,
+                                                   std::vector<Real> a0, std::vector<Real> a1,
+                                                   std::vector<Real> b0, std::vector<Real> b1,
+                                                   std::vector<Real> b2, ElectroMagneticProperty emp)
+    : Material(std::move(name), emp),
+      _kind(kind),
+      _epsilon_inf(epsilon_inf),
+      _a0(std::move(a0)),
+      _a1(std::move(a1)),
+      _b0(std::move(b0)),
+      _b1(std::move(b1)),
+      _b2(std::move(b2)) {
+  const auto n = _a0.size();
+  if (_a1.size() != n || _b0.size() != n || _b1.size() != n || _b2.size() != n)
+    throw XFDTDException("Modified Lorentz Model paramater doesn't have same size.");
+}
+std::unique_ptr<MLorentzMaterial> MLorentzMaterial::makeMLorentz(std::string name, Real epsilon_inf,
+                                                                 std::vector<Real> a_0, std::vector<Real> a_1,
+                                                                 std::vector<Real> b_0, std::vector<Real> b_1,
+                                                                 std::vector<Real> b_2) {
+  return std::make_unique<MLorentzMaterial>(std::move(name), epsilon_inf, Kind::MLorentz, std::move(a_0),
+                                            std::move(a_1), std::move(b_0), std::move(b_1), std::move(b_2));
+}
+std::unique_ptr<DrudeMedium> DrudeMedium::makeDrudeMedium(std::string name, Real eps_inf, std::vector<Real> omega_p,
+                                                          std::vector<Real> gamma) {
+  const auto n = omega_p.size();
+  std::vector<Real> a0(n);
+  for (std::size_t p = 0; p < n; ++p) a0[p] = omega_p[p] * omega_p[p];
+  return std::make_unique<DrudeMedium>(std::move(name), eps_inf, Kind::Drude, std::move(a0),
+                                       std::vector<Real>(n, 0.0), std::vector<Real>(n, 0.0), std::move(gamma),
+                                       std::vector<Real>(n, 1.0));
+}
+std::unique_ptr<DebyeMedium> DebyeMedium::makeDebyeMedium(std::string name, Real epsilon_inf,
+                                                          std::vector<Real> epsilon_static, std::vector<Real> tau) {
+  const auto n = epsilon_static.size();
+  std::vector<Real> a0(n);
+  for (std::size_t p = 0; p < n; ++p) a0[p] = epsilon_static[p] - epsilon_inf;
+  return std::make_unique<DebyeMedium>(std::move(name), epsilon_inf, Kind::Debye, std::move(a0),
+                                       std::vector<Real>(n, 0.0), std::vector<Real>(n, 1.0), std::move(tau),
+                                       std::vector<Real>(n, 0.0));
+}
+
 // ----------------------------------------------------------- waveforms ---
 void Waveform::init(const std::vector<Real>& time) {
   _value.assign(time.size(), 0);
@@ -164,6 +206,11 @@ LumpedElement::LumpedElement(std::string n, std::unique_ptr<Cube> c, Axis::XYZ x
     : Object(std::move(n), std::move(c), std::move(m)), _xyz(xyz) {}
 Resistor::Resistor(std::string n, std::unique_ptr<Cube> c, Axis::XYZ xyz, Real r, std::unique_ptr<Material> m)
     : LumpedElement(std::move(n), std::move(c), xyz, std::move(m)), _resistance(r == 0 ? 1e-10 : r) {}
+Capacitor::Capacitor(std::string n, std::unique_ptr<Cube> c, Axis::XYZ xyz, Real cap, std::unique_ptr<Material> m)
+    : LumpedElement(std::move(n), std::move(c), xyz, std::move(m)), _capacitance(cap) {}
+Inductor::Inductor(std::string n, std::unique_ptr<Cube> c, Axis::XYZ xyz, Real l, std::unique_ptr<Material> m)
+    : LumpedElement(std::move(n), std::move(c), xyz, std::move(m)), _inductance(l == 0 ? 1e-10 : l) {}
+PecPlane::PecPlane(std::string n, std::unique_ptr<Cube> c) : Object(std::move(n), std::move(c), Material::createPec()) {}
 static Axis::XYZ dirAxis(Axis::Direction d) { return static_cast<Axis::XYZ>(static_cast<int>(d) / 2); }
 static bool dirNeg(Axis::Direction d) { return static_cast<int>(d) % 2 == 0; }
 VoltageSource::VoltageSource(std::string n, std::unique_ptr<Cube> c, Axis::Direction d, Real r,
@@ -282,6 +329,14 @@ struct Simulation::Impl {
   // material (eps, mu, sigma_e, sigma_m per component) and coefficients
   Field3 eps[3], mu[3], se[3], sm[3];
   Field3 coef[18];
+  // Per-cell index (into `objects`) of the object whose shape took the cell
+  // in the staircase; kept only when some object is dispersive.
+  std::vector<int32_t> owner;
+  // ADE: scheme, pole count, per-object coefficient rows, per-cell ids
+  xfd_ade_desc ade_d{};
+  bool has_ade = false;
+  std::vector<Real> ade_coeff;
+  std::vector<int16_t> ade_cell;
   mutable std::map<std::string, Array3D<Real>> coef_view;
 
   // device hand-off
@@ -293,7 +348,7 @@ struct Simulation::Impl {
   std::vector<SlabData> slabs;
   struct LumpedData {
     xfd_lumped_desc d{};
-    std::vector<Real> coff;
+    std::vector<Real> coff, coff_j;
     std::vector<Real> wave;
   };
   std::vector<LumpedData> lumped;
@@ -429,7 +484,13 @@ struct Simulation::Impl {
     // wins.  Only cells whose centre lies inside the shape's bounding box
     // widened by the tolerance can pass the test, so only those are visited.
     const Real tol = tolerance();
-    for (auto& o : objects) {
+    bool any_dispersive = false;
+    for (auto& o : objects) any_dispersive = any_dispersive || o->material().dispersion();
+    owner.clear();
+    if (any_dispersive) owner.assign(N[0] * N[1] * N[2], -1);
+    for (std::size_t oi = 0; oi < objects.size(); ++oi) {
+      auto& o = objects[oi];
+      if (dynamic_cast<PecPlane*>(o.get())) continue;  // simulation.cpp:611-613
       const auto& p = o->material().emProperty();
       auto box = o->shape().wrappedCube();
       std::size_t lo[3], hi[3];
@@ -448,6 +509,7 @@ struct Simulation::Impl {
           for (std::size_t j = lo[1]; j < hi[1]; ++j)
             for (std::size_t k = lo[2]; k < hi[2]; ++k) {
               if (!shape.isInside(Vector(h_node[0][i], h_node[1][j], h_node[2][k]), tol)) continue;
+              if (!owner.empty()) owner[(i * N[1] + j) * N[2] + k] = static_cast<int32_t>(oi);
               for (int c = 0; c < 3; ++c) {
                 eps[c].at(i, j, k) = p.epsilon();
                 mu[c].at(i, j, k) = p.mu();
@@ -465,6 +527,34 @@ struct Simulation::Impl {
       averageEdges(se[c], c);
       averageFaces(mu[c], c);
       averageFaces(sm[c], c);
+    }
+    for (auto& o : objects)  // simulation.cpp:629-635
+      if (auto* pp = dynamic_cast<PecPlane*>(o.get())) applyPecPlane(*pp);
+  }
+
+  // PecPlane::correctMaterialSpace (pec_plane.cpp:12-67): the plane's two
+  // tangential E components take the material's sigma_e on the plane.
+  void applyPecPlane(const PecPlane& pp) {
+    std::size_t o[3], e[3];
+    gridBox(pp.shape(), o, e);
+    const Real sg = pp.material().emProperty().sigmaE();
+    auto fill = [&](Field3& f, std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0,
+                    std::size_t k1) {
+      for (std::size_t i = i0; i < i1; ++i)
+        for (std::size_t j = j0; j < j1; ++j)
+          for (std::size_t k = k0; k < k1; ++k) f.at(i, j, k) = sg;
+    };
+    if (o[0] == e[0]) {
+      fill(se[1], o[0], o[0] + 1, o[1], e[1], o[2], e[2] + 1);
+      fill(se[2], o[0], o[0] + 1, o[1], e[1] + 1, o[2], e[2]);
+    } else if (o[1] == e[1]) {
+      fill(se[2], o[0], e[0] + 1, o[1], o[1] + 1, o[2], e[2]);
+      fill(se[0], o[0], e[0], o[1], o[1] + 1, o[2], e[2] + 1);
+    } else if (o[2] == e[2]) {
+      fill(se[0], o[0], e[0], o[1], e[1] + 1, o[2], o[2] + 1);
+      fill(se[1], o[0], e[0] + 1, o[1], e[1], o[2], o[2] + 1);
+    } else {
+      throw XFDTDSimulationException(pp.name() + " PecPlane is not a plane!");
     }
   }
 
@@ -600,9 +690,57 @@ struct Simulation::Impl {
     (void)gmain;
     auto size_of = [&](int a, std::size_t idx) { return a == m ? e_size[a][idx] : h_size[a][idx]; };
 
+    if (auto* cap = dynamic_cast<Capacitor*>(&le)) {
+      // capacitor.cpp:24-25, 58, 73-81: C' = C nc / (nc nb), beta =
+      // 2 C' dc / (da db); the E coefficients with +beta in numerator and
+      // denominator
+      const Real cf = cap->capacitance() * gc / (gc * gb);
+      Field3& cece = coef[m * 3];
+      Field3& cecha = coef[m * 3 + 1];
+      Field3& cechb = coef[m * 3 + 2];
+      for (std::size_t i = 0; i < cnt[0]; ++i)
+        for (std::size_t j = 0; j < cnt[1]; ++j)
+          for (std::size_t k = 0; k < cnt[2]; ++k) {
+            const std::size_t g[3] = {lo[0] + i, lo[1] + j, lo[2] + k};
+            const Real da = size_of(A, g[A]), db = size_of(B, g[B]), dc = size_of(m, g[m]);
+            const Real beta = 2 * cf * dc / (da * db);
+            const Real e0 = eps[m].at(g[0], g[1], g[2]), s0 = se[m].at(g[0], g[1], g[2]);
+            cece.at(g[0], g[1], g[2]) = (2 * e0 - dt * s0 + beta) / (2 * e0 + dt * s0 + beta);
+            cecha.at(g[0], g[1], g[2]) = -2 * dt / ((2 * e0 + dt * s0 + beta) * db);
+            cechb.at(g[0], g[1], g[2]) = 2 * dt / ((2 * e0 + dt * s0 + beta) * da);
+          }
+      return;
+    }
+    if (auto* ind = dynamic_cast<Inductor*>(&le)) {
+      // inductor.cpp:29-31, 69, 84: L' = L na nb / nc, cjcec = dt dc /
+      // (L' da db), cecjc = -2 dt / (2 eps + dt sigma); no coefficient patch
+      const Real lf = ind->inductance() * ga * gb / gc;
+      LumpedData ld;
+      const std::size_t vol = cnt[0] * cnt[1] * cnt[2];
+      ld.coff.resize(vol);
+      ld.coff_j.resize(vol);
+      for (std::size_t i = 0; i < cnt[0]; ++i)
+        for (std::size_t j = 0; j < cnt[1]; ++j)
+          for (std::size_t k = 0; k < cnt[2]; ++k) {
+            const std::size_t g[3] = {lo[0] + i, lo[1] + j, lo[2] + k};
+            const Real da = size_of(A, g[A]), db = size_of(B, g[B]), dc = size_of(m, g[m]);
+            const std::size_t q = (i * cnt[1] + j) * cnt[2] + k;
+            ld.coff_j[q] = dt * dc / (lf * da * db);
+            ld.coff[q] = -(2 * dt) / (2 * eps[m].at(g[0], g[1], g[2]) + dt * se[m].at(g[0], g[1], g[2]));
+          }
+      ld.d.kind = XFD_LUMPED_INDUCTOR;
+      ld.d.component = m;
+      for (int a = 0; a < 3; ++a) {
+        ld.d.origin[a] = static_cast<int32_t>(lo[a]);
+        ld.d.size[a] = static_cast<int32_t>(cnt[a]);
+      }
+      lumped.push_back(std::move(ld));
+      return;
+    }
     auto* res = dynamic_cast<Resistor*>(&le);
     auto* vs = dynamic_cast<VoltageSource*>(&le);
     auto* cs = dynamic_cast<CurrentSource*>(&le);
+    if (!res && !vs && !cs) throw XFDTDSimulationException("unsupported lumped element " + le.name());
     const Real R = res ? res->resistance() : vs ? vs->resistance() : cs->resistance();
     const Real rf = R * ga * gb / gc;
     Real factor = 0;
@@ -651,6 +789,137 @@ struct Simulation::Impl {
 
   std::vector<Real> eTime() const { return linspaceAssign(1 * dt, steps * dt, steps); }
   std::vector<Real> hTime() const { return linspaceAssign(0.5 * dt, (steps + 0.5) * dt, steps); }
+
+  // ------------------------------------------------------------------- ADE --
+  // Simulation::buildDispersiveSpace (simulation.cpp:745-829) and
+  // Object::handleDispersion (object.cpp:52-98): the scheme follows the kinds
+  // of dispersive material present; each dispersive object's cells get the
+  // ADE E coefficients of its material (debye_ade_method.cpp:39-122,
+  // drude_ade_method.cpp:48-133, m_lor_ade_method.cpp:66-159).
+  void buildDispersive() {
+    has_ade = false;
+    ade_coeff.clear();
+    ade_cell.clear();
+    std::size_t poles = 0;
+    bool is_drude = false, is_debye = false, is_lor = false;
+    for (auto& o : objects) {
+      if (dynamic_cast<PecPlane*>(o.get())) continue;
+      auto* m = dynamic_cast<const LinearDispersiveMaterial*>(&o->material());
+      if (!m) continue;
+      poles = std::max(poles, m->numPoles());
+      if (m->kind() == LinearDispersiveMaterial::Kind::Drude) is_drude = true;
+      else if (m->kind() == LinearDispersiveMaterial::Kind::Debye) is_debye = true;
+      else is_lor = true;
+    }
+    if (poles == 0) return;
+    int scheme = XFD_ADE_MLORENTZ;
+    if (is_drude && !is_debye && !is_lor) scheme = XFD_ADE_DRUDE;
+    else if (!is_drude && is_debye && !is_lor) scheme = XFD_ADE_DEBYE;
+    const std::size_t stride = 2 + 7 * poles;
+    ade_cell.assign(N[0] * N[1] * N[2], int16_t(-1));
+    int16_t nmat = 0;
+    for (std::size_t oi = 0; oi < objects.size(); ++oi) {
+      auto* m = dynamic_cast<const LinearDispersiveMaterial*>(&objects[oi]->material());
+      if (!m || dynamic_cast<PecPlane*>(objects[oi].get())) continue;
+      if (nmat == 32767) throw XFDTDSimulationException("too many dispersive objects");
+      std::vector<Real> row(stride, 0.0);
+      Real cece = 0, num_a = 0, num_b = 0, den = 1;  // cecha = num_a / (den db), cechb = num_b / (den da)
+      const Real sigma = m->emProperty().sigmaE(), ei = m->epsilonInf();
+      const std::size_t np = m->numPoles();
+      if (scheme != XFD_ADE_MLORENTZ) {
+        if (np != poles)
+          throw XFDTDSimulationException("Debye/Drude storage needs every material to have the storage's pole count");
+        std::vector<Real> kk(np), beta(np);
+        for (std::size_t p = 0; p < np; ++p) {
+          if (scheme == XFD_ADE_DEBYE) {
+            const Real tau = m->b1()[p], de = m->a0()[p];
+            kk[p] = (2 * tau - dt) / (2 * tau + dt);
+            beta[p] = 2 * kEps0 * de * dt / (2 * tau + dt);
+          } else {
+            const Real g = m->b1()[p], wp = std::sqrt(m->a0()[p]);
+            kk[p] = (1 - g * dt / 2) / (1 + g * dt / 2);
+            beta[p] = (kEps0 * wp * wp * dt / 2) / (1 + g * dt / 2);
+          }
+        }
+        Real sum_beta = 0;
+        for (std::size_t p = 0; p < np; ++p) sum_beta += beta[p];
+        for (std::size_t p = 0; p < np; ++p) {
+          Real* r = row.data() + 2 + 7 * p;
+          r[1] = scheme == XFD_ADE_DEBYE ? beta[p] / dt : beta[p];
+          r[3] = kk[p];
+          r[5] = 0.5 * (1 + kk[p]);
+        }
+        Real a, b;
+        if (scheme == XFD_ADE_DEBYE) {
+          a = (2 * kEps0 * ei + sum_beta - dt * sigma) / (2 * kEps0 * ei + sum_beta + dt * sigma);
+          b = (2 * dt) / (2 * kEps0 * ei + sum_beta + dt * sigma);
+        } else {
+          a = (2 * kEps0 * ei - dt * sum_beta - dt * sigma) / (2 * kEps0 * ei - dt * sum_beta + dt * sigma);
+          b = (2 * dt) / (2 * kEps0 * ei - dt * sum_beta + dt * sigma);
+        }
+        row[0] = -b;
+        cece = a;
+        num_a = -b;
+        num_b = b;
+      } else {
+        std::vector<Real> A(np), B(np), C(np), D(np), E(np), F(np);
+        for (std::size_t p = 0; p < np; ++p) {
+          const Real a0 = m->a0()[p], a1 = m->a1()[p], b0 = m->b0()[p], b1 = m->b1()[p], b2 = m->b2()[p];
+          A[p] = (a1 * kEps0 / (dt * dt)) + (0.5 * a0 * kEps0 / dt);
+          B[p] = (-2 * a1 * kEps0) / (dt * dt);
+          C[p] = (a1 * kEps0 / (dt * dt)) - (0.5 * a0 * kEps0 / dt);
+          D[p] = (2 * b2 / (dt * dt)) - b0;
+          E[p] = (-b2 / (dt * dt)) + (b1 / (2 * dt));
+          F[p] = (b2 / (dt * dt)) + (b1 / (2 * dt));
+        }
+        for (std::size_t p = 0; p < np; ++p) {
+          Real* r = row.data() + 2 + 7 * p;
+          r[0] = A[p] / F[p];
+          r[1] = B[p] / F[p];
+          r[2] = C[p] / F[p];
+          r[3] = D[p] / F[p];
+          r[4] = E[p] / F[p];
+          r[5] = 0.5 * (1 + D[p] / F[p]);
+          r[6] = 0.5 * (E[p] / F[p]);
+        }
+        Real sum_a_f = 0, sum_b_f = 0, sum_c_f = 0;
+        for (std::size_t p = 0; p < np; ++p) sum_a_f += A[p] / F[p];
+        for (std::size_t p = 0; p < np; ++p) sum_b_f += B[p] / F[p];
+        for (std::size_t p = 0; p < np; ++p) sum_c_f += C[p] / F[p];
+        const Real temp = (ei * kEps0) / dt + 0.5 * sum_a_f + 0.5 * sigma;
+        row[0] = -1 / temp;
+        row[1] = -0.5 * sum_c_f / temp;
+        cece = ((ei * kEps0) / dt - 0.5 * sum_b_f - 0.5 * sigma) / temp;
+        num_a = -1;
+        num_b = 1;
+        den = temp;
+      }
+      const bool lor = scheme == XFD_ADE_MLORENTZ;
+      const int16_t id = nmat++;
+      ade_coeff.insert(ade_coeff.end(), row.begin(), row.end());
+      const int32_t me = static_cast<int32_t>(oi);
+      parallelFor(N[0], [&](std::size_t lo, std::size_t hi) {
+        for (std::size_t i = lo; i < hi; ++i)
+          for (std::size_t j = 0; j < N[1]; ++j)
+            for (std::size_t k = 0; k < N[2]; ++k) {
+              const std::size_t cell = (i * N[1] + j) * N[2] + k;
+              if (owner[cell] != me) continue;
+              ade_cell[cell] = id;
+              const Real d[3] = {h_size[0][i], h_size[1][j], h_size[2][k]};
+              for (int c = 0; c < 3; ++c) {
+                const Real da = d[(c + 1) % 3], db = d[(c + 2) % 3];
+                coef[3 * c].at(i, j, k) = cece;
+                coef[3 * c + 1].at(i, j, k) = lor ? num_a / (den * db) : num_a / db;
+                coef[3 * c + 2].at(i, j, k) = lor ? num_b / (den * da) : num_b / da;
+              }
+            }
+      });
+    }
+    ade_d.scheme = scheme;
+    ade_d.poles = static_cast<int32_t>(poles);
+    ade_d.n_materials = nmat;
+    has_ade = true;
+  }
 
   // ------------------------------------------------------------------ CPML --
   void buildPml(const PML& p) {
@@ -951,6 +1220,8 @@ struct Simulation::Impl {
       buildTfsf(*t);
     }
     stage("tfsf");
+    buildDispersive();  // simulation.cpp:651, after every other coefficient patch
+    stage("dispersive");
     buildMonitors();
     coef_view.clear();
     stage("monitors");
@@ -980,8 +1251,9 @@ struct Simulation::Impl {
     std::vector<xfd_lumped_desc> lp;
     for (auto& l : lumped) {
       l.d.coff = l.coff.data();
+      l.d.coff_j = l.coff_j.empty() ? nullptr : l.coff_j.data();
       l.d.nt = static_cast<int64_t>(l.wave.size());
-      l.d.waveform = l.wave.data();
+      l.d.waveform = l.wave.empty() ? nullptr : l.wave.data();
       lp.push_back(l.d);
     }
     std::vector<xfd_tfsf_desc> tp;
@@ -1006,6 +1278,11 @@ struct Simulation::Impl {
     dd.tfsf = tp.data();
     dd.n_gather = static_cast<int32_t>(gp.size());
     dd.gather = gp.data();
+    if (has_ade) {
+      ade_d.coeff = ade_coeff.data();
+      ade_d.cell_material = ade_cell.data();
+      dd.ade = &ade_d;
+    }
     if (xfd_create(&dd, &ctx) != XFD_OK) throw XFDTDSimulationException(std::string("xfd_create: ") + xfd_last_error());
     int64_t b = 0;
     xfd_device_bytes(ctx, &b);
@@ -1067,7 +1344,9 @@ struct Simulation::Impl {
         auto& frames = mv->frames();
         frames.clear();
         for (int64_t f = 0; f < g.d.count; ++f) {
-          Array3D<Real> a(n0, n1, n2);
+          // FieldMonitor::output (field_monitor.cpp:82-96) leaves the data
+          // unassigned (an empty array) when the box is empty
+          Array3D<Real> a = fe > 0 ? Array3D<Real>(n0, n1, n2) : Array3D<Real>(0, 0, 0);
           std::memcpy(a.data.data(), buf.data() + f * fe, fe * sizeof(Real));
           char name[32];
           std::snprintf(name, sizeof(name), "%05lld", static_cast<long long>(f * g.d.interval));

@@ -133,7 +133,8 @@ class LumpedDesc(ctypes.Structure):
     """xfd_lumped_desc."""
 
     _fields_ = [("component", ctypes.c_int32), ("origin", ctypes.c_int32 * 3), ("size", ctypes.c_int32 * 3),
-                ("coff", _dp), ("nt", ctypes.c_int64), ("waveform", _dp)]
+                ("coff", _dp), ("nt", ctypes.c_int64), ("waveform", _dp), ("kind", ctypes.c_int32),
+                ("coff_j", _dp)]
 
 
 class GatherDesc(ctypes.Structure):

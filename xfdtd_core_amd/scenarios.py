@@ -94,3 +94,53 @@ def inhomogeneous_box(n_total: int = 384, pml: int = 12, n_spheres: int = 64,
                     s.addMonitor(S.FieldTimeMonitor(f"probe_{a}{b}{c}", S.Cube(S.Vector(x, y, z), S.Vector(dl, dl, dl)),
                                                     "", "EZ"))
     return s
+
+
+def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab: int = 40, gap: int = 8,
+                     tfsf_offset: int = 14, lumped: bool = True, kinds=("drude", "debye", "lorentz", "mlorentz"),
+                     snapshot_interval: int = 0):
+    """Dispersive + lumped box: four z-slabs (`slab` cells thick, `gap`-cell
+    air gaps, centred, full transverse extent inside the CPML) of Drude
+    (eps_inf=1, wp=1.37e16, gamma=1e14), Debye (eps_inf=4, eps_s=50,
+    tau=1e-11 s), two-pole Lorentz (eps_inf=2, d_eps={1.5, 0.8},
+    w0={2e10, 6e10}, delta={1e9, 3e9}) and a modified Lorentz with a1 != 0
+    (eps_inf=2, a0=1.35e21, a1=5e9, b0=9e20, b1=1e10, b2=1; a0 b1 - a1 b0 > 0
+    and 4 b2 - b0 dt^2 > 0, the stability conditions of
+    dispersive_material_equation.cpp:62-99).  TFSF plane wave (theta=30 deg,
+    phi=45 deg) with a Gaussian of tau = 20 dl / 6e8; a 50-ohm Resistor and a
+    50-ohm, 1 V VoltageSource along z, each a 6-cell gap in the air below the
+    slabs.  CFL 0.95.  `kinds` selects and orders the slab materials."""
+    s = S.Simulation(dl, dl, dl, 0.95)
+    n = n_total - 2 * pml
+    half = n * dl / 2
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    mats = {
+        "drude": S.DrudeMedium.makeDrudeMedium("drude", 1.0, [1.37e16], [1e14]),
+        "debye": S.DebyeMedium.makeDebyeMedium("debye", 4.0, [50.0], [1e-11]),
+        "lorentz": S.LorentzMedium.makeMLorentz("lorentz", 2.0, [1.5 * 2e10 ** 2, 0.8 * 6e10 ** 2], [0.0, 0.0],
+                                                [2e10 ** 2, 6e10 ** 2], [2e9, 6e9], [1.0, 1.0]),
+        "mlorentz": S.MLorentzMaterial.makeMLorentz("mlorentz", 2.0, [1.35e21], [5e9], [9e20], [1e10], [1.0]),
+    }
+    total = len(kinds) * slab + (len(kinds) - 1) * gap
+    z0 = (n - total) // 2
+    for q, kind in enumerate(kinds):
+        lo = z0 + q * (slab + gap)
+        s.addObject(S.Object(f"slab_{kind}", S.Cube(S.Vector(-half, -half, -half + lo * dl),
+                                                    S.Vector(2 * half, 2 * half, slab * dl)), mats[kind]))
+    if lumped:
+        zl = -half + max(1, z0 // 2 - 3) * dl
+        s.addObject(S.Resistor("r50", S.Cube(S.Vector(-4 * dl, 0.0, zl), S.Vector(0.0, 0.0, 6 * dl)), "Z", 50.0))
+        tau_v = 20 * dl / 6e8
+        s.addObject(S.VoltageSource("v1", S.Cube(S.Vector(4 * dl, 0.0, zl), S.Vector(0.0, 0.0, 6 * dl)), "ZP", 50.0,
+                                    S.Waveform.gaussian(tau_v, 4.5 * tau_v, 1.0)))
+    tau = 20 * dl / 6e8
+    s.addWaveformSource(S.TFSF3D(tfsf_offset, tfsf_offset, tfsf_offset, math.radians(30), math.radians(45), 0.0,
+                                 S.Waveform.gaussian(tau, 4.5 * tau)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(pml, d))
+    if snapshot_interval:
+        whole = S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half))
+        for f in S.FIELDS:
+            s.addMonitor(S.MovieMonitor(S.FieldMonitor(whole, f), snapshot_interval, f"snap_{f}", ""))
+    return s

@@ -13,6 +13,9 @@ renders the scenario as line-oriented text, one directive per line:
         SHAPE    = cube OX OY OZ SX SY SZ | sphere CX CY CZ R
                  | cylinder CX CY CZ R H AXIS
         MATERIAL = plain EPS_R MU_R SIGMA_E SIGMA_M
+                 | lorentz EPS_INF P A0[P] A1[P] B0[P] B1[P] B2[P]
+                 | drude EPS_INF P OMEGA_P[P] GAMMA[P]
+                 | debye EPS_INF P EPS_STATIC[P] TAU[P]
     pml THICKNESS DIR [ORDER SIGMA_RATIO ALPHA_MIN ALPHA_MAX KAPPA_MAX]
     tfsf3d X Y Z THETA PHI PSI WAVEFORM...
     current_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
@@ -116,6 +119,75 @@ class Material:
     def tokens(self):
         p = self.em_property
         return ["plain", _n(p.epsilon_r), _n(p.mu_r), _n(p.sigma_e), _n(p.sigma_m)]
+
+
+@dataclass
+class LinearDispersiveMaterial(Material):
+    """Modified-Lorentz form of a linear dispersive material
+    (dispersive_material.h): eps_r(w) = eps_inf + sum_p (a1 jw + a0) /
+    (b2 (jw)^2 + b1 jw + b0).  `kind` is the reference class the material
+    was made with ("lorentz", "drude" or "debye"); `params` are that
+    factory's per-pole arrays."""
+
+    kind: str = "lorentz"
+    epsilon_inf: float = 1.0
+    params: List[List[float]] = field(default_factory=list)
+
+    def tokens(self):
+        n = len(self.params[0])
+        out = [self.kind, _n(self.epsilon_inf), str(n)]
+        for arr in self.params:
+            assert len(arr) == n, "every per-pole array has the pole count"
+            out += [_n(v) for v in arr]
+        return out
+
+    def susceptibility(self, omega, p):
+        """(a1 jw + a0) / (b2 (jw)^2 + b1 jw + b0) of pole p at angular
+        frequency `omega` (dispersive_material_equation.cpp:35-41)."""
+        a0, a1, b0, b1, b2 = self.lorentz_params()
+        jw = 1j * omega
+        return (a1[p] * jw + a0[p]) / (b2[p] * jw * jw + b1[p] * jw + b0[p])
+
+    def relative_permittivity(self, omega):
+        return self.epsilon_inf + sum(self.susceptibility(omega, p) for p in range(len(self.params[0])))
+
+    def lorentz_params(self):
+        """(a0, a1, b0, b1, b2) per pole, as the equation classes set them."""
+        n = len(self.params[0])
+        if self.kind == "lorentz":
+            return tuple(list(a) for a in self.params)
+        if self.kind == "drude":
+            wp, g = self.params
+            return ([w * w for w in wp], [0.0] * n, [0.0] * n, list(g), [1.0] * n)
+        es, tau = self.params
+        return ([e - self.epsilon_inf for e in es], [0.0] * n, [1.0] * n, list(tau), [0.0] * n)
+
+
+class MLorentzMaterial:
+    @staticmethod
+    def makeMLorentz(name, epsilon_inf, a_0, a_1, b_0, b_1, b_2):
+        """MLorentzMaterial::makeMLorentz (dipersive_material.cpp:47-56)."""
+        return LinearDispersiveMaterial(name, kind="lorentz", epsilon_inf=epsilon_inf,
+                                        params=[list(a_0), list(a_1), list(b_0), list(b_1), list(b_2)])
+
+
+LorentzMedium = MLorentzMaterial
+
+
+class DrudeMedium:
+    @staticmethod
+    def makeDrudeMedium(name, eps_inf, omega_p, gamma):
+        """DrudeMedium::makeDrudeMedium (drude_medium.cpp:9-14)."""
+        return LinearDispersiveMaterial(name, kind="drude", epsilon_inf=eps_inf,
+                                        params=[list(omega_p), list(gamma)])
+
+
+class DebyeMedium:
+    @staticmethod
+    def makeDebyeMedium(name, epsilon_inf, epsilon_static, tau):
+        """DebyeMedium::makeDebyeMedium (debye_medium.cpp:10-17)."""
+        return LinearDispersiveMaterial(name, kind="debye", epsilon_inf=epsilon_inf,
+                                        params=[list(epsilon_static), list(tau)])
 
 
 @dataclass
