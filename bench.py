@@ -55,6 +55,19 @@ def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int) -> float:
     return 0.0
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_inhomogeneous_box_384.json"
+
+
+def measured_traffic(precision, n, pml):
+    """HBM bytes per launch of each kernel, from the calibrated rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded for the 384^3 box (None for other
+    sizes, which were not profiled)."""
+    if n != 384 or pml != 12 or not TRAFFIC_FILE.exists():
+        return None
+    d = json.loads(TRAFFIC_FILE.read_text())
+    return {k: v["hbm_bytes"] for k, v in d["kernels"].get(precision, {}).items()}
+
+
 def step_alg_bytes(n: int, pml: int, s: int) -> float:
     """Compulsory bytes of one whole step: fused Yee (6 field reads, 6 writes,
     18 coefficient reads per cell = 30 s) plus CPML (12 s per slab cell per
@@ -94,6 +107,11 @@ def run_device(sim, precision, warmup, steps, n, pml):
                          "alg_GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     ach = kernels[dom]["alg_GBps"]
+    traffic = measured_traffic(precision, n, pml)
+    for name, k in kernels.items():
+        if traffic and name in traffic:
+            k["hbm_bytes_pmc"] = traffic[name]
+            k["hbm_GBps_pmc"] = round(traffic[name] / (k["avg_ms"] * 1e-3) / 1e9, 1)
     res = {
         "mcells_per_s": cells * steps / wall / 1e6,
         "ms_per_step": wall / steps * 1e3,
@@ -101,7 +119,9 @@ def run_device(sim, precision, warmup, steps, n, pml):
         "step_alg_GBps": round(step_alg_bytes(n, pml, s) * steps / wall / 1e9, 1),
         "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(ach / PEAK_HBM_GBS, 4) if ach else None, "traffic": None},
+                     "frac": round(ach / PEAK_HBM_GBS, 4) if ach else None,
+                     "traffic": traffic.get(dom) if traffic else None,
+                     "hbm_GBps_measured": kernels[dom].get("hbm_GBps_pmc")},
     }
     log(f"{precision}: {res['mcells_per_s']:.0f} Mcells/s, {res['ms_per_step']:.3f} ms/step, dominant {dom} "
         f"{ach} GB/s")

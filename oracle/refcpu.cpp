@@ -1226,6 +1226,221 @@ void FieldTimeMonitor::output(Sim& s) {
   writeNpy(out_dir + "/" + name + ".npy", {2, time.size()}, st.data());
 }
 
+// ----------------------------------------------------------------- NF2FF --
+namespace {
+// GCC's complex product for finite operands: (ac - bd, ad + bc)
+Cplx cmul(const Cplx& a, const Cplx& b) {
+  return {a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()};
+}
+// interpolate::interpolateSurfaceCenter (interpolate_scheme.h:18-138): the
+// tangential field component `f` (0..5) at the centre of the face of the
+// surface normal to axis c at (i, j, k)
+Real surfaceCenter(Sim& s, int c, int f, Index i, Index j, Index k) {
+  const Arr3& a = s.emf.byId(f);
+  const Real h = 0.5, q = 0.25;
+  if (c == 0) {
+    if (f == 5) return q * (a(i, j, k + 1) + a(i, j, k) + a(i - 1, j, k + 1) + a(i - 1, j, k));
+    if (f == 4) return q * (a(i, j + 1, k) + a(i, j, k) + a(i - 1, j + 1, k) + a(i - 1, j, k));
+    if (f == 2) return h * (a(i, j + 1, k) + a(i, j, k));
+    if (f == 1) return h * (a(i, j, k + 1) + a(i, j, k));
+  } else if (c == 1) {
+    if (f == 5) return q * (a(i, j, k + 1) + a(i, j, k) + a(i, j - 1, k + 1) + a(i, j - 1, k));
+    if (f == 3) return q * (a(i + 1, j, k) + a(i, j, k) + a(i + 1, j - 1, k) + a(i, j - 1, k));
+    if (f == 2) return h * (a(i + 1, j, k) + a(i, j, k));
+    if (f == 0) return h * (a(i, j, k + 1) + a(i, j, k));
+  } else {
+    if (f == 3) return q * (a(i + 1, j, k) + a(i, j, k) + a(i, j, k - 1) + a(i + 1, j, k - 1));
+    if (f == 4) return q * (a(i, j, k) + a(i, j + 1, k) + a(i, j, k - 1) + a(i, j + 1, k - 1));
+    if (f == 0) return h * (a(i, j + 1, k) + a(i, j, k));
+    if (f == 1) return h * (a(i + 1, j, k) + a(i, j, k));
+  }
+  throw Error("Invalid field");
+}
+}  // namespace
+
+void NF2FF::init(Sim& s) {  // nffft.cpp:73-223 with node == global
+  const Index n[3] = {s.g.nx(), s.g.ny(), s.g.nz()};
+  Index lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    if (dist[a] == 0) throw Error("distance == 0");
+    lo[a] = dist[a];
+    hi[a] = n[a] - dist[a];
+    if (hi[a] <= lo[a]) throw Error("Global box is not legal");
+  }
+  for (int d = 0; d < 6; ++d) {
+    const int c = d / 2;
+    const bool neg = d % 2 == 0;
+    Range r[3];
+    for (int a = 0; a < 3; ++a) r[a] = {lo[a], hi[a]};
+    const Index cs = neg ? lo[c] : hi[c];
+    // makeNodeAxisTask: a negative surface on plane 0 or 1, or a positive
+    // one on the last plane, is not used
+    if ((neg && cs <= 1) || (!neg && cs >= n[c])) {
+      surf[d] = Task{};
+      continue;
+    }
+    r[c] = {cs, cs + 1};
+    surf[d] = {r[0], r[1], r[2]};
+  }
+  bool any = false;
+  for (auto& t : surf) any = any || t.valid();
+  if (!any) throw Error("Invalid task");
+  planes.clear();
+  for (Real f : freqs) {
+    Plane p;
+    p.freq = f;
+    for (int d = 0; d < 6; ++d) {
+      const Index cells = surf[d].valid() ? surf[d].x.size() * surf[d].y.size() * surf[d].z.size() : 0;
+      p.ja[d].assign(cells, Cplx{});
+      p.jb[d].assign(cells, Cplx{});
+      p.ma[d].assign(cells, Cplx{});
+      p.mb[d].assign(cells, Cplx{});
+    }
+    planes.push_back(std::move(p));
+  }
+}
+
+void NF2FF::initTimeDependentVariable(Sim& s) {  // nffft_fd_data.cpp:65-76
+  const Real dt = s.dt;
+  const Cplx II{0, 1};
+  for (auto& p : planes) {
+    p.te.assign(s.size, Cplx{});
+    p.th.assign(s.size, Cplx{});
+    for (Index t = 0; t < s.size; ++t) {
+      p.te[t] = dt * std::exp(-II * static_cast<Real>(2.0) * PI * (p.freq * (t + 1) * dt));
+      p.th[t] = dt * std::exp(-II * static_cast<Real>(2.0) * PI * (p.freq * (t + static_cast<Real>(0.5)) * dt));
+    }
+  }
+}
+
+void NF2FF::update(Sim& s, Index t) {
+  // every FDPlaneData::update runs once per frequency of the NFFFT
+  // (nffft_frequency_domain.cpp:45-53); calculateJ then calculateM per
+  // surface (nffft_fd_data_update.cpp:7-122)
+  for (Index rep = 0; rep < freqs.size(); ++rep)
+    for (auto& p : planes)
+      for (int pass = 0; pass < 2; ++pass)
+        for (int d = 0; d < 6; ++d) {
+          const Task& tk = surf[d];
+          if (!tk.valid()) continue;
+          const int c = d / 2, A = (c + 1) % 3, B = (c + 2) % 3;
+          const bool pos = d % 2 == 1;
+          const bool is_j = pass == 0;
+          const Real ca = is_j ? (pos ? -1.0 : 1.0) : (pos ? 1.0 : -1.0);
+          const Real cb = is_j ? (pos ? 1.0 : -1.0) : (pos ? -1.0 : 1.0);
+          const int fa = (is_j ? 3 : 0) + A, fb = (is_j ? 3 : 0) + B;
+          const Cplx tr = is_j ? p.th[t] : p.te[t];
+          auto& va = is_j ? p.ja[d] : p.ma[d];
+          auto& vb = is_j ? p.jb[d] : p.mb[d];
+          Index q = 0;
+          for (Index i = tk.x.s; i < tk.x.e; ++i)
+            for (Index j = tk.y.s; j < tk.y.e; ++j)
+              for (Index k = tk.z.s; k < tk.z.e; ++k, ++q) {
+                va[q] += ca * surfaceCenter(s, c, fb, i, j, k) * tr;
+                vb[q] += cb * surfaceCenter(s, c, fa, i, j, k) * tr;
+              }
+        }
+}
+
+std::vector<Cplx> NF2FF::potential(Sim& s, Index fi, bool a_pot, bool theta_comp, const std::vector<Real>& theta,
+                                   const std::vector<Real>& phi, const Vec3& origin) const {
+  if (theta.empty() || phi.empty()) throw Error("Invalid theta or phi size");
+  if (theta.size() != 1 && phi.size() != 1) throw Error("There must be only one theta or phi");
+  const Index na = theta.size() * phi.size();
+  auto th = [&](Index n) { return theta[theta.size() == 1 ? 0 : n]; };
+  auto ph = [&](Index n) { return phi[phi.size() == 1 ? 0 : n]; };
+  // cCSToSCSTransformMatrix (transform.h:24-58) of axis x/y/z
+  auto tmat = [&](int axis, Index n) -> Real {
+    const Real t = th(n), p = ph(n);
+    if (theta_comp) {
+      if (axis == 0) return std::cos(t) * std::cos(p);
+      if (axis == 1) return std::cos(t) * std::sin(p);
+      return -1 * std::sin(t);
+    }
+    if (axis == 0) return -1 * std::sin(p);
+    if (axis == 1) return std::cos(p);
+    return 0.0;
+  };
+  const Plane& P = planes[fi];
+  const Real k = 2.0 * PI * P.freq / C_0;
+  const Cplx ik = Cplx{0, 1} * k;
+  std::vector<Cplx> total(na, Cplx{});
+  for (int d = 0; d < 6; ++d) {
+    std::vector<Cplx> data(na, Cplx{});
+    const Task& tk = surf[d];
+    if (tk.valid()) {
+      const int c = d / 2, A = (c + 1) % 3, B = (c + 2) % 3;
+      const auto& va = a_pot ? P.ja[d] : P.ma[d];
+      const auto& vb = a_pot ? P.jb[d] : P.mb[d];
+      std::vector<Real> st(na), ct(na), sp(na), cp(na), stsp(na), stcp(na);
+      std::vector<Cplx> ta(na), tb(na);
+      for (Index n = 0; n < na; ++n) {
+        ct[n] = std::cos(th(n));
+        st[n] = std::sin(th(n));
+        cp[n] = std::cos(ph(n));
+        sp[n] = std::sin(ph(n));
+        stsp[n] = st[n] * sp[n];
+        stcp[n] = st[n] * cp[n];
+        ta[n] = tmat(A, n);
+        tb[n] = tmat(B, n);
+      }
+      Index q = 0;
+      for (Index i = tk.x.s; i < tk.x.e; ++i)
+        for (Index j = tk.y.s; j < tk.y.e; ++j)
+          for (Index kk = tk.z.s; kk < tk.z.e; ++kk, ++q) {
+            // rVector / ds (nffft_fd_data_process.cpp:196-225)
+            Vec3 rv;
+            Real ds;
+            if (c == 0) {
+              rv = {s.g.e_node[0](i), s.g.h_node[1](j), s.g.h_node[2](kk)};
+              ds = s.g.e_size[1](j) * s.g.e_size[2](kk);
+            } else if (c == 1) {
+              rv = {s.g.h_node[0](i), s.g.e_node[1](j), s.g.h_node[2](kk)};
+              ds = s.g.e_size[2](kk) * s.g.e_size[0](i);
+            } else {
+              rv = {s.g.h_node[0](i), s.g.h_node[1](j), s.g.e_node[2](kk)};
+              ds = s.g.e_size[0](i) * s.g.e_size[1](j);
+            }
+            const Vec3 r = rv - origin;
+            for (Index n = 0; n < na; ++n) {
+              const Real v = r.x * stcp[n] + r.y * stsp[n] + r.z * ct[n];
+              const Cplx phase = std::exp(ik * v);
+              const Cplx sum = cmul(va[q], ta[n]) + cmul(vb[q], tb[n]);
+              data[n] += cmul(sum, phase) * ds;
+            }
+          }
+    }
+    for (Index n = 0; n < na; ++n) total[n] += data[n];
+  }
+  return total;
+}
+
+Real NF2FF::power(Sim& s, Index fi) const {
+  const Plane& P = planes[fi];
+  Cplx acc{0.0};
+  for (int d = 0; d < 6; ++d) {
+    const Task& tk = surf[d];
+    Cplx pw{0.0};
+    if (tk.valid()) {
+      const int c = d / 2;
+      Index q = 0;
+      for (Index i = tk.x.s; i < tk.x.e; ++i)
+        for (Index j = tk.y.s; j < tk.y.e; ++j)
+          for (Index kk = tk.z.s; kk < tk.z.e; ++kk, ++q) {
+            Real ds;
+            if (c == 0) ds = s.g.e_size[1](j) * s.g.e_size[2](kk);
+            else if (c == 1) ds = s.g.e_size[2](kk) * s.g.e_size[0](i);
+            else ds = s.g.e_size[0](i) * s.g.e_size[1](j);
+            pw += ds * (cmul(std::conj(P.ja[d][q]), P.mb[d][q]) - cmul(std::conj(P.jb[d][q]), P.ma[d][q]));
+          }
+      const Real negative = d % 2 == 0 ? -1.0 : 1.0;
+      pw = negative * pw;
+    }
+    acc = acc + pw;
+  }
+  return 0.5 * std::real(acc);
+}
+
 // ------------------------------------------------------------ simulation --
 Arr1 Sim::eTime() const {
   return linspace((start_step + 1) * dt, (start_step + size) * dt, size);
@@ -1462,6 +1677,7 @@ void Sim::init() {  // simulation.cpp:319-371
   for (auto& p : pmls) p->correctUpdateCoefficient(*this);
   buildDispersiveSpace();  // simulation.cpp:651
   for (auto& m : monitors) m->init(*this);
+  for (auto& n : nf2ffs) n->init(*this);  // simulation.cpp:350-358
 }
 
 void Sim::initSteps(Index n) {  // simulation.cpp:373-389
@@ -1471,6 +1687,7 @@ void Sim::initSteps(Index n) {  // simulation.cpp:373-389
   current = 0;
   for (auto& o : objects) o->initTimeDependentVariable(*this);
   for (auto& t : tfsfs) t->initTimeDependentVariable(*this);
+  for (auto& n : nf2ffs) n->initTimeDependentVariable(*this);  // simulation.cpp:373-389
   for (auto& m : monitors) m->initTimeDependentVariable(*this);
 }
 
@@ -1638,6 +1855,7 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
       if (d.master) {
         current = t;
         for (auto& m : monitors) m->update(*this, t);
+        for (auto& n : nf2ffs) n->update(*this, t);  // Domain::record (domain.cpp:176-184)
       }
       bar.arrive_and_wait();
       bar.arrive_and_wait();

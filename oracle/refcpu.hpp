@@ -27,6 +27,7 @@
 #include <barrier>
 #include <cmath>
 #include <cstddef>
+#include <complex>
 #include <cstdint>
 #include <functional>
 #include <memory>
@@ -435,6 +436,34 @@ struct FieldTimeMonitor : Monitor {
 };
 
 // ------------------------------------------------------------ simulation --
+// ---------------------------------------------------------------- NF2FF ---
+// NFFFTFrequencyDomain (nffft.cpp:39-236, nffft_frequency_domain.cpp) with
+// one FDPlaneData per frequency (nffft_fd_data.cpp, nffft_fd_data_update.cpp,
+// nffft_fd_data_process.cpp), one process: node surfaces == global ones.
+using Cplx = std::complex<Real>;
+struct NF2FF {
+  Index dist[3] = {1, 1, 1};
+  std::vector<Real> freqs;
+  std::string out_dir;
+  // surfaces XN XP YN YP ZN ZP (nffft.cpp:97-223); invalid ones are empty
+  Task surf[6];
+  struct Plane {
+    Real freq = 0;
+    // per surface: (ja, jb) and (ma, mb) on the surface task's shape
+    std::vector<Cplx> ja[6], jb[6], ma[6], mb[6];
+    std::vector<Cplx> te, th;  // DFT phase tables (nffft_fd_data.cpp:65-76)
+  };
+  std::vector<Plane> planes;
+  void init(Sim& s);
+  void initTimeDependentVariable(Sim& s);
+  void update(Sim& s, Index t);  // nffft_frequency_domain.cpp:45-53
+  // potentials (nffft_fd_data_process.cpp:44-160): A (J) or F (M),
+  // theta or phi component, over a theta / phi cut (one of them size 1)
+  std::vector<Cplx> potential(Sim& s, Index f, bool a_pot, bool theta_comp, const std::vector<Real>& theta,
+                              const std::vector<Real>& phi, const Vec3& origin) const;
+  Real power(Sim& s, Index f) const;  // nffft_fd_data_process.cpp:28-42, 162-194
+};
+
 struct Sim {
   Real dx, dy, dz, cfl;
   int tcfg[3] = {1, 1, 1};
@@ -442,6 +471,7 @@ struct Sim {
   std::vector<std::unique_ptr<PML>> pmls;
   std::vector<std::unique_ptr<TFSF3D>> tfsfs;
   std::vector<std::unique_ptr<Monitor>> monitors;
+  std::vector<std::unique_ptr<NF2FF>> nf2ffs;
   bool write_files = true;
 
   GridSpace g;

@@ -236,6 +236,14 @@ void parseSpec(Sim& s, const std::string& spec) {
       e->shape = parseCube(is);
       e->material = Material::make(1, 1, 1e10, 0);  // ElectroMagneticProperty::pec
       s.objects.push_back(std::move(e));
+    } else if (d == "nf2ff") {
+      auto n = std::make_unique<NF2FF>();
+      for (int a = 0; a < 3; ++a) n->dist[a] = static_cast<Index>(num(is));
+      const Index nf = static_cast<Index>(num(is));
+      for (Index f = 0; f < nf; ++f) n->freqs.push_back(num(is));
+      n->out_dir = word(is);
+      if (n->out_dir == "-") n->out_dir.clear();
+      s.nf2ffs.push_back(std::move(n));
     } else if (d == "movie") {
       auto m = std::make_unique<MovieMonitor>();
       auto f = std::make_unique<FieldMonitor>();
@@ -324,6 +332,56 @@ int ref_init(void* p, int64_t steps) {
 }
 
 double ref_dt(void* p) { return static_cast<Handle*>(p)->sim.dt; }
+int ref_nf2ff_count(void* p) { return static_cast<int>(static_cast<Handle*>(p)->sim.nf2ffs.size()); }
+
+// Surface task of NF2FF `n`, surface d (XN XP YN YP ZN ZP): lo[3], hi[3]
+// (hi == lo when the surface is not used).
+int ref_nf2ff_surface(void* p, int n, int d, int64_t* lo, int64_t* hi) {
+  return guard([&] {
+    auto& s = static_cast<Handle*>(p)->sim;
+    const Task& t = s.nf2ffs.at(n)->surf[d];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = static_cast<int64_t>(t.r(a).s);
+      hi[a] = static_cast<int64_t>(t.valid() ? t.r(a).e : t.r(a).s);
+    }
+  });
+}
+
+// Accumulated surface currents of frequency f, surface d, which = 0 ja,
+// 1 jb, 2 ma, 3 mb; interleaved (re, im) doubles.
+int ref_nf2ff_current(void* p, int n, int f, int d, int which, double* out) {
+  return guard([&] {
+    auto& pl = static_cast<Handle*>(p)->sim.nf2ffs.at(n)->planes.at(f);
+    const auto& v = which == 0 ? pl.ja[d] : which == 1 ? pl.jb[d] : which == 2 ? pl.ma[d] : pl.mb[d];
+    for (size_t q = 0; q < v.size(); ++q) {
+      out[2 * q] = v[q].real();
+      out[2 * q + 1] = v[q].imag();
+    }
+  });
+}
+
+// aTheta / aPhi (a_pot = 1) or fTheta / fPhi (a_pot = 0) over theta x phi
+// (one of them of size 1); out: interleaved complex.
+int ref_nf2ff_potential(void* p, int n, int f, int a_pot, int theta_comp, int nt, const double* theta, int np,
+                        const double* phi, const double* origin, double* out) {
+  return guard([&] {
+    auto& s = static_cast<Handle*>(p)->sim;
+    std::vector<Real> th(theta, theta + nt), ph(phi, phi + np);
+    auto v = s.nf2ffs.at(n)->potential(s, f, a_pot != 0, theta_comp != 0, th, ph, {origin[0], origin[1], origin[2]});
+    for (size_t q = 0; q < v.size(); ++q) {
+      out[2 * q] = v[q].real();
+      out[2 * q + 1] = v[q].imag();
+    }
+  });
+}
+
+int ref_nf2ff_power(void* p, int n, int f, double* out) {
+  return guard([&] {
+    auto& s = static_cast<Handle*>(p)->sim;
+    *out = s.nf2ffs.at(n)->power(s, f);
+  });
+}
+
 double ref_loop_seconds(void* p) { return static_cast<Handle*>(p)->sim.loop_seconds; }
 // Copies up to `cap` per-step end times (seconds since the loop start) of
 // the last loop; returns how many there are.

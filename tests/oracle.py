@@ -46,6 +46,12 @@ def lib():
         "ref_material": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
         "ref_pml_profile": (i64, [vp, ctypes.c_int, ctypes.c_int, vp]),
         "ref_tfsf_table": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
+        "ref_nf2ff_count": (ctypes.c_int, [vp]),
+        "ref_nf2ff_surface": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, p64]),
+        "ref_nf2ff_current": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
+        "ref_nf2ff_potential": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]),
+        "ref_nf2ff_power": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(dbl)]),
         "ref_monitor_count": (ctypes.c_int, [vp]),
         "ref_monitor_records": (ctypes.c_int, [vp, ctypes.c_int]),
         "ref_monitor_record": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
@@ -149,6 +155,35 @@ class RefSim:
         a = np.empty((shape[0], shape[1]))
         self._check(self.L.ref_tfsf_table(self.h, i, which, shape, a.ctypes.data))
         return a
+
+    # ------------------------------------------------------------ NF2FF --
+    def nf2ff_surface(self, n: int, d: int):
+        lo = (ctypes.c_int64 * 3)()
+        hi = (ctypes.c_int64 * 3)()
+        self._check(self.L.ref_nf2ff_surface(self.h, n, d, lo, hi))
+        return tuple(lo), tuple(hi)
+
+    def nf2ff_current(self, n: int, f: int, d: int, which: int) -> np.ndarray:
+        """Surface current array (complex) on surface d: which 0 ja, 1 jb, 2 ma, 3 mb."""
+        lo, hi = self.nf2ff_surface(n, d)
+        shape = tuple(h - l for l, h in zip(lo, hi))
+        out = np.zeros(int(np.prod(shape)) * 2)
+        self._check(self.L.ref_nf2ff_current(self.h, n, f, d, which, out.ctypes.data))
+        return (out[0::2] + 1j * out[1::2]).reshape(shape)
+
+    def nf2ff_potential(self, n: int, f: int, a_pot: bool, theta_comp: bool, theta, phi, origin=(0.0, 0.0, 0.0)):
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+        org = np.ascontiguousarray(origin, dtype=np.float64)
+        out = np.zeros(2 * th.size * ph.size)
+        self._check(self.L.ref_nf2ff_potential(self.h, n, f, int(a_pot), int(theta_comp), th.size, th.ctypes.data,
+                                               ph.size, ph.ctypes.data, org.ctypes.data, out.ctypes.data))
+        return out[0::2] + 1j * out[1::2]
+
+    def nf2ff_power(self, n: int, f: int) -> float:
+        v = ctypes.c_double()
+        self._check(self.L.ref_nf2ff_power(self.h, n, f, ctypes.byref(v)))
+        return v.value
 
     def monitor_records(self, m: int) -> List[np.ndarray]:
         out = []

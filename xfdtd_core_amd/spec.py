@@ -25,6 +25,7 @@ renders the scenario as line-oriented text, one directive per line:
     inductor NAME OX OY OZ SX SY SZ AXIS L
     pec_plane NAME OX OY OZ SX SY SZ
     movie FIELD OX OY OZ SX SY SZ INTERVAL NAME OUT_DIR
+    nf2ff DX DY DZ NF F1 .. FNF OUT_DIR
     field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
     WAVEFORM = gaussian TAU T0 AMP | cosine_modulated_gaussian TAU T0 F AMP
              | sine F AMP | cosine F AMP | step T0 AMP
@@ -397,6 +398,23 @@ class FieldTimeMonitor:
                         [self.output_dir or "-"])
 
 
+@dataclass
+class NFFFTFrequencyDomain:
+    """``NFFFTFrequencyDomain(distance_x, distance_y, distance_z, frequencies,
+    output_dir)`` (nffft_frequency_domain.h): a running DFT of the tangential
+    fields on the box `distance` cells inside the grid boundary."""
+
+    distance_x: int
+    distance_y: int
+    distance_z: int
+    frequencies: List[float]
+    output_dir: str = ""
+
+    def line(self):
+        return " ".join(["nf2ff", str(int(self.distance_x)), str(int(self.distance_y)), str(int(self.distance_z)),
+                         str(len(self.frequencies))] + [_n(f) for f in self.frequencies] + [self.output_dir or "-"])
+
+
 class Simulation:
     """``xfdtd::Simulation(dx, dy, dz, cfl, ThreadConfig)`` (simulation.h:39-40)."""
 
@@ -405,6 +423,7 @@ class Simulation:
         self.thread_config = tuple(thread_config)
         self._lines: List[str] = []
         self.monitors: List[object] = []
+        self.nf2ffs: List[object] = []
 
     def addObject(self, obj):
         self._lines.append(obj.line())
@@ -418,6 +437,10 @@ class Simulation:
     def addMonitor(self, m):
         self._lines.append(m.line())
         self.monitors.append(m)
+
+    def addNF2FF(self, n):
+        self._lines.append(n.line())
+        self.nf2ffs.append(n)
 
     def spec(self) -> str:
         head = [f"sim {_n(self.dx)} {_n(self.dy)} {_n(self.dz)} {_n(self.cfl)}",
