@@ -114,6 +114,22 @@ typedef struct {
   const int16_t* cell_material;  /* nx x ny x nz cells; -1: not dispersive */
 } xfd_ade_desc;
 
+/* Frequency-domain NF2FF running DFT (NFFFTFrequencyDomain::update,
+ * src/nffft/nffft_frequency_domain.cpp:45-53, FDPlaneData::calculateJ/M,
+ * src/nffft/nffft_fd_data_update.cpp:22-122), run after the recordings of
+ * every step.  Per frequency f and surface d the library accumulates four
+ * complex arrays on the surface's cells (k fastest):
+ *   ja, jb += (+-1) * face-centre(H_b, H_a) * th[f][t]
+ *   ma, mb += (+-1) * face-centre(E_b, E_a) * te[f][t]
+ * in FP64, `nf` times per step as the reference's update loop does. */
+typedef struct {
+  int32_t lo[6][3], hi[6][3];  /* surfaces XN XP YN YP ZN ZP; hi == lo: unused */
+  int32_t nf;                  /* frequencies (and update repeats per step) */
+  int64_t nt;                  /* rows of the phase tables */
+  const double* te;            /* nf x nt complex (re, im): dt exp(-j 2 pi f (t + 1) dt) */
+  const double* th;            /* nf x nt complex: dt exp(-j 2 pi f (t + 1/2) dt) */
+} xfd_nf2ff_desc;
+
 typedef struct {
   int32_t nx, ny, nz;
   int32_t dtype;               /* enum xfd_dtype: field and update precision */
@@ -129,6 +145,8 @@ typedef struct {
   int32_t n_gather;
   const xfd_gather_desc* gather;
   const xfd_ade_desc* ade;     /* NULL: no dispersive material */
+  int32_t n_nf2ff;
+  const xfd_nf2ff_desc* nf2ff;
 } xfd_desc;
 
 typedef struct xfd_ctx xfd_ctx;
@@ -158,6 +176,11 @@ int xfd_write_field(xfd_ctx* ctx, int32_t field, const double* src);
 
 /* Copies recording `g` (count x box volume doubles) to the host. */
 int xfd_read_gather(xfd_ctx* ctx, int32_t g, double* dst);
+
+/* Copies the accumulated surface current `which` (0 ja, 1 jb, 2 ma, 3 mb)
+ * of NF2FF `n`, frequency `f`, surface `d` (XN XP YN YP ZN ZP) to the host
+ * as interleaved (re, im) doubles in the surface's (i, j, k) order. */
+int xfd_read_nf2ff(xfd_ctx* ctx, int32_t n, int32_t f, int32_t d, int32_t which, double* dst);
 
 /* Kernel timing with HIP events on the library's stream.  When enabled,
  * xfd_step records an event pair around every launch; xfd_kernel_stats then

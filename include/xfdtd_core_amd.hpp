@@ -14,6 +14,7 @@
 #define XFDTD_CORE_AMD_HPP_
 
 #include <array>
+#include <complex>
 #include <cstddef>
 #include <functional>
 #include <memory>
@@ -440,6 +441,52 @@ class FieldTimeMonitor : public Monitor {
   std::vector<Real> _time, _data;
 };
 
+// Frequency-domain near-to-far-field transform (nffft_frequency_domain.h):
+// a running DFT of the tangential fields on the box `distance` cells inside
+// the grid boundary, run on the GPU every step; the far-field potentials and
+// the radiated power are formed on the host after the run.
+class NFFFTFrequencyDomain {
+ public:
+  NFFFTFrequencyDomain(std::size_t distance_x, std::size_t distance_y, std::size_t distance_z,
+                       std::vector<Real> frequencies, std::string output_dir = "");
+  ~NFFFTFrequencyDomain();
+  std::size_t freqCount() const { return _frequencies.size(); }
+  const std::vector<Real>& frequencies() const { return _frequencies; }
+  const std::string& outputDir() const { return _output_dir; }
+  std::size_t distance(int axis) const { return _distance[axis]; }
+
+  // nffft_frequency_domain.cpp:56-160: writes <f>GHz_a_theta.npy,
+  // _f_phi, _a_phi, _f_theta (complex) under outputDir()/sub_dir, and
+  // power.npy (frequencies; radiated power).
+  void processFarField(const std::vector<Real>& theta, Real phi, const std::string& sub_dir,
+                       const Vector& origin = Vector{0.0, 0.0, 0.0}) const;
+  void processFarField(Real theta, const std::vector<Real>& phi, const std::string& sub_dir,
+                       const Vector& origin = Vector{0.0, 0.0, 0.0}) const;
+  void outputRadiationPower() const;
+
+  // FDPlaneData::aTheta / aPhi / fTheta / fPhi and power()
+  // (nffft_fd_data_process.cpp) of frequency index f; one of theta and phi
+  // has a single element.
+  enum class Potential { A, F };
+  std::vector<std::complex<Real>> potential(std::size_t f, Potential p, bool theta_component,
+                                            const std::vector<Real>& theta, const std::vector<Real>& phi,
+                                            const Vector& origin = Vector{0.0, 0.0, 0.0}) const;
+  Real power(std::size_t f) const;
+  // Surface d (XN XP YN YP ZN ZP): index box [lo, hi) (empty when unused)
+  // and the accumulated currents, which = 0 ja, 1 jb, 2 ma, 3 mb.
+  std::array<std::size_t, 6> surface(int d) const;
+  const std::vector<std::complex<Real>>& current(std::size_t f, int d, int which) const;
+
+  struct Data;
+  Data& data() const { return *_data; }
+
+ private:
+  std::size_t _distance[3];
+  std::vector<Real> _frequencies;
+  std::string _output_dir;
+  std::unique_ptr<Data> _data;
+};
+
 struct ThreadConfig {  // accepted for API compatibility; the GPU path ignores it
   int x = 1, y = 1, z = 1;
 };
@@ -457,6 +504,7 @@ class Simulation {
   void addWaveformSource(std::shared_ptr<WaveformSource> source);
   void addBoundary(std::shared_ptr<Boundary> boundary);
   void addMonitor(std::shared_ptr<Monitor> monitor);
+  void addNF2FF(std::shared_ptr<NFFFTFrequencyDomain> nffft);
 
   // Field and update precision of the GPU loop (default FP64).  The
   // precompute is FP64 in both cases.

@@ -42,6 +42,23 @@ int xfd_sim_grid(void* sim, int64_t* n3);
 int64_t xfd_sim_device_bytes(void* sim);
 /* 1 when the device holds coefficient class ids (xfd_coef_layout), else 0. */
 int32_t xfd_sim_coef_classed(void* sim);
+
+/* NF2FF (NFFFTFrequencyDomain) results after run() or finish(), in the
+ * order the scenario adds them: surface boxes (d = XN XP YN YP ZN ZP),
+ * accumulated currents (which = 0 ja, 1 jb, 2 ma, 3 mb; interleaved complex),
+ * the potentials aTheta / aPhi (a_pot = 1) and fTheta / fPhi (a_pot = 0) over
+ * a theta or phi cut (nffft_fd_data_process.cpp:9-26), the radiated power,
+ * and the reference's output files (processFarField, outputRadiationPower:
+ * nffft_frequency_domain.cpp:56-160; nt == 1 selects the phi cut). */
+int32_t xfd_sim_nf2ff_count(void* sim);
+int xfd_sim_nf2ff_surface(void* sim, int32_t n, int32_t d, int64_t* lo, int64_t* hi);
+int xfd_sim_nf2ff_current(void* sim, int32_t n, int32_t f, int32_t d, int32_t which, double* dst);
+int xfd_sim_nf2ff_potential(void* sim, int32_t n, int32_t f, int32_t a_pot, int32_t theta_comp, int32_t nt,
+                            const double* theta, int32_t np, const double* phi, const double* origin, double* dst);
+int xfd_sim_nf2ff_power(void* sim, int32_t n, int32_t f, double* dst);
+int xfd_sim_nf2ff_process_far_field(void* sim, int32_t n, int32_t nt, const double* theta, int32_t np,
+                                    const double* phi, const char* sub_dir, const double* origin);
+int xfd_sim_nf2ff_output_power(void* sim, int32_t n);
 /* which = 0: e-node coordinates, otherwise h sizes; returns the length. */
 int64_t xfd_sim_grid_array(void* sim, int32_t which, int32_t axis, double* dst);
 /* Arrays of the reference extent; dst may be NULL to query the shape. */

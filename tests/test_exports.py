@@ -41,3 +41,34 @@ def test_symbol_exported(name):
 def test_version_string():
     L = native.load_library()
     assert b"gfx950" in L.xfd_version()
+
+
+def test_ctypes_descriptors_match_the_c_header(tmp_path):
+    """The Python descriptor structs have the C header's size and field
+    offsets (compiled with the host C compiler)."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {
+        "xfd_cpml_desc": native.CpmlDesc, "xfd_tfsf_desc": native.TfsfDesc, "xfd_lumped_desc": native.LumpedDesc,
+        "xfd_gather_desc": native.GatherDesc, "xfd_desc": native.Desc,
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xfdtd_core_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} __sizeof__ %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "__sizeof__")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

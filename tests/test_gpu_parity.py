@@ -186,6 +186,59 @@ def test_dispersive_ade_fp64(case):
         assert e <= TOL_F64, (k, e)
 
 
+def _compare_nf2ff(dev, ref, nf, label):
+    """Accumulated surface currents, potentials and power of NF2FF 0."""
+    theta = np.linspace(0.1, np.pi - 0.1, 17)
+    worst = 0.0
+    for f in range(nf):
+        for d in range(6):
+            for w in range(4):
+                a, b = dev.nf2ff_current(0, f, d, w), ref.nf2ff_current(0, f, d, w)
+                assert a.shape == b.shape
+                if b.size:
+                    worst = max(worst, rel_l2(a, b))
+        for a_pot in (True, False):
+            for th_comp in (True, False):
+                a = dev.nf2ff_potential(0, f, a_pot, th_comp, theta, 0.4)
+                b = ref.nf2ff_potential(0, f, a_pot, th_comp, theta, 0.4)
+                worst = max(worst, rel_l2(a, b))
+        pa, pb = dev.nf2ff_power(0, f), ref.nf2ff_power(0, f)
+        worst = max(worst, abs(pa - pb) / abs(pb))
+    say(f"  {label} nf2ff worst relative L2: {worst:.3e}")
+    return worst
+
+
+def test_nf2ff_dipole_fp64():
+    """NF2FF running DFT on the device (FP64 accumulators, the per-frequency
+    repeat): surface currents, far-field potentials and radiated power of a
+    radiating dipole equal the oracle's."""
+    from test_oracle_nf2ff import STEPS, _dipole
+
+    sp = _dipole([6e9, 9e9])
+    say("[gpu] nf2ff dipole: oracle")
+    ref = RefSim(sp)
+    ref.run(STEPS)
+    say("[gpu] nf2ff dipole: device")
+    dev = native.Simulation(sp, "f64")
+    dev.run(STEPS)
+    for f in FIELDS:
+        assert rel_l2(dev.field(f), ref.field(f)) <= TOL_F64, f
+    assert _compare_nf2ff(dev, ref, 2, "dipole") <= TOL_F64
+
+
+def test_nf2ff_dispersive_box_fp64():
+    """NF2FF (8 frequencies, 5-40 GHz) around the dispersive, lumped and
+    TFSF-driven box."""
+    sp = scenarios.dispersive_slabs(n_total=48, pml=5, slab=6, gap=2, tfsf_offset=7, nf2ff_distance=6).spec()
+    say("[gpu] nf2ff dispersive: oracle")
+    ref = RefSim(sp)
+    ref.run(200)
+    say("[gpu] nf2ff dispersive: device")
+    dev = native.Simulation(sp, "f64")
+    dev.run(200)
+    assert _compare_nf2ff(dev, ref, 8, "dispersive") <= TOL_F64
+
+
 def test_fp32_error_against_fp64_oracle():
     """FP32 fields and updates against the FP64 oracle on the scaled box.
 

@@ -1,0 +1,130 @@
+"""Pins the oracle's NF2FF (frequency-domain running DFT and far field).
+
+A one-edge z-directed current source at the centre of a vacuum box with CPML
+radiates as a Hertzian dipole.  Checks:
+
+* the surfaces sit `distance` cells inside the grid (nffft.cpp:97-133);
+* the far field built from both potentials, |eta A_theta + F_phi|, follows
+  sin(theta) and the cross-polarised combination eta A_phi - F_theta stays
+  small (Love's equivalence: only the correct J/M signs reproduce the dipole);
+* the radiated power from power() equals the far-field integral
+  k^2 / (32 pi^2 eta) * integral of (|eta A_theta + F_phi|^2 + |eta A_phi - F_theta|^2)
+  over the sphere, and the closed form of a Hertzian dipole,
+  eta k^2 |I dz|^2 / (12 pi) with I the DFT of the source current, times nf^2
+  for the per-frequency repeat below;
+* the per-step update runs once per frequency (nffft_frequency_domain.cpp:
+  45-53): with two frequencies the accumulated currents are twice those of a
+  one-frequency run.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import RefSim
+from xfdtd_core_amd import spec as S
+
+EPS0 = 8.854187817e-12
+MU0 = 4 * np.pi * 1e-7
+C0 = 1 / np.sqrt(EPS0 * MU0)
+ETA = np.sqrt(MU0 / EPS0)
+DL = 1e-3
+N = 40
+PML = 8
+DIST = 12
+STEPS = 700
+TAU = 20 * DL / (2 * C0)
+T0 = 4.5 * TAU
+
+
+def _dipole(freqs):
+    s = S.Simulation(DL, DL, DL, 0.9)
+    half = N * DL / 2
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    # a derivative-of-Gaussian current deposits no net charge, so no static
+    # field is left to be truncated by the finite DFT window
+    s.addObject(S.CurrentSource("src", S.Cube(S.Vector(0.0, 0.0, 0.0), S.Vector(0.0, 0.0, DL)), "ZP", 1e20,
+                                S.Waveform.gaussianDerivative(TAU, T0, 1.0)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(PML, d))
+    s.addNF2FF(S.NFFFTFrequencyDomain(DIST, DIST, DIST, list(freqs)))
+    return s.spec()
+
+
+@pytest.fixture(scope="module")
+def dipole():
+    ref = RefSim(_dipole([6e9, 9e9]))
+    ref.run(STEPS)
+    return ref
+
+
+def test_surfaces(dipole):
+    n = dipole.grid()
+    for d in range(6):
+        lo, hi = dipole.nf2ff_surface(0, d)
+        c, pos = d // 2, d % 2 == 1
+        assert lo[c] == (n[c] - DIST if pos else DIST) and hi[c] == lo[c] + 1
+        for a in range(3):
+            if a != c:
+                assert (lo[a], hi[a]) == (DIST, n[a] - DIST)
+
+
+def _far(ref, f, theta, phi):
+    at = ref.nf2ff_potential(0, f, True, True, theta, phi)
+    ap = ref.nf2ff_potential(0, f, True, False, theta, phi)
+    ft = ref.nf2ff_potential(0, f, False, True, theta, phi)
+    fp = ref.nf2ff_potential(0, f, False, False, theta, phi)
+    return ETA * at + fp, ETA * ap - ft
+
+
+@pytest.mark.parametrize("f", [0, 1])
+@pytest.mark.parametrize("phi", [0.0, np.pi / 4, 1.2])
+def test_dipole_pattern(dipole, f, phi):
+    theta = np.linspace(0.15, np.pi - 0.15, 31)
+    co, cross = _far(dipole, f, theta, phi)
+    pattern = np.abs(co) / np.abs(co).max()
+    ideal = np.sin(theta) / np.sin(theta).max()
+    np.testing.assert_allclose(pattern, ideal, atol=5e-3)
+    assert np.abs(cross).max() < 1e-2 * np.abs(co).max()
+    # the opposite relative sign of the two potentials is far off
+    at = dipole.nf2ff_potential(0, f, True, True, theta, phi)
+    fp = dipole.nf2ff_potential(0, f, False, False, theta, phi)
+    wrong = np.abs(ETA * at - fp)
+    assert np.abs(wrong / wrong.max() - ideal).max() > 2e-2
+
+
+def _source_current_dft(freq, dt):
+    """DFT (dt sum over steps, at the lumped sources' half-step times) of
+    the source current I(t) = -u exp(-u^2), u = (t - t0) / tau."""
+    t = (np.arange(STEPS) + 0.5) * dt
+    u = (t - T0) / TAU
+    return dt * np.sum(-u * np.exp(-u * u) * np.exp(-2j * np.pi * freq * t))
+
+
+@pytest.mark.parametrize("f,freq", [(0, 6e9), (1, 9e9)])
+def test_radiated_power(dipole, f, freq):
+    k = 2 * np.pi * freq / C0
+    p_surface = dipole.nf2ff_power(0, f)
+    theta = np.linspace(1e-3, np.pi - 1e-3, 721)
+    co, cross = _far(dipole, f, theta, 0.0)
+    integrand = (np.abs(co) ** 2 + np.abs(cross) ** 2) * np.sin(theta)
+    p_far = k ** 2 / (32 * np.pi ** 2 * ETA) * 2 * np.pi * np.trapezoid(integrand, theta)
+    i_f = _source_current_dft(freq, dipole.dt)
+    # every accumulator holds nf = 2 copies of the DFT (the per-frequency
+    # repeat of the update), so the power is nf^2 times the dipole's
+    p_dipole = 4 * ETA * k ** 2 * abs(i_f * DL) ** 2 / (12 * np.pi)
+    assert p_surface > 0
+    assert abs(p_far / p_surface - 1) < 2e-2, (p_far, p_surface)
+    assert abs(p_surface / p_dipole - 1) < 2e-2, (p_surface, p_dipole)
+
+
+def test_update_repeats_once_per_frequency():
+    one = RefSim(_dipole([6e9]))
+    one.run(200)
+    two = RefSim(_dipole([6e9, 9e9]))
+    two.run(200)
+    for d in range(6):
+        for which in range(4):
+            a = one.nf2ff_current(0, 0, d, which)
+            b = two.nf2ff_current(0, 0, d, which)
+            np.testing.assert_allclose(b, 2 * a, rtol=1e-12, atol=1e-30)

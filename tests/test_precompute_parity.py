@@ -109,3 +109,14 @@ def test_readme_grid_matches_survey_numbers():
     c0 = 1 / np.sqrt(8.854187817e-12 * 4 * np.pi * 1e-7)
     dl = 7.5e-3
     assert abs(ref.dt - 0.9 / (c0 * np.sqrt(3 / dl ** 2))) < 1e-24
+
+
+@pytest.mark.parametrize("dist", [(12, 12, 12), (1, 3, 2), (2, 15, 6)])
+def test_nf2ff_surfaces_match_oracle(dist):
+    """NFFFT surfaces (nffft.cpp:73-223), including the unused negative
+    surface at distance 1."""
+    sp = _circuit_box()
+    sp.addNF2FF(S.NFFFTFrequencyDomain(*dist, [5e9, 7e9]))
+    ref, prod = _both(sp.spec(), 10)
+    for d in range(6):
+        assert ref.nf2ff_surface(0, d) == prod.nf2ff_surface(0, d), d

@@ -42,6 +42,7 @@ UNITS = [
     (CSRC / "device" / "capi.hip", "hip", ["-ffp-contract=off"]),
     (CSRC / "host" / "simulation.cpp", "host", []),
     (CSRC / "host" / "sim_capi.cpp", "host", []),
+    (CSRC / "host" / "nf2ff.cpp", "host", []),
 ]
 
 

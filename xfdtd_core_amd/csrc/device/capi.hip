@@ -164,9 +164,9 @@ bool classify(const double* A, const double* B, const double* C, const int ext[3
   return true;
 }
 
-enum KernelClass { K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_COUNT };
-const char* kKernelNames[K_COUNT] = {"update_h", "update_e", "cpml_h", "cpml_e",
-                                     "records_h", "records_e", "gather"};
+enum KernelClass { K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_NF2FF, K_COUNT };
+const char* kKernelNames[K_COUNT] = {"update_h", "update_e", "cpml_h",  "cpml_e",
+                                     "records_h", "records_e", "gather", "nf2ff"};
 
 struct DevBuf {
   void* p = nullptr;
@@ -226,6 +226,12 @@ struct xfd_ctx {
   // dispersive materials (ADE): scheme or -1, per-cell material ids,
   // per-material coefficient table, pole currents, previous currents and
   // previous E (full padded arrays per component and pole)
+  // NF2FF running DFTs: surfaces, phase tables, FP64 accumulators
+  struct Nf2ff {
+    xfd::Nf2ffArgs a{};
+    DevBuf te, th, acc;
+  };
+  std::vector<std::unique_ptr<Nf2ff>> nf2ffs;
   int ade_scheme = -1;
   int ade_poles = 0;
   DevBuf ade_mat, ade_tab, ade_j[3], ade_jp[3], ade_eprev[3];
@@ -272,6 +278,7 @@ struct xfd_ctx {
     }
     b += pool.bytes + rec_e.bytes + seg_e.bytes + rec_h.bytes + seg_h.bytes + state.bytes;
     b += ade_mat.bytes + ade_tab.bytes;
+    for (auto& n : nf2ffs) b += n->te.bytes + n->th.bytes + n->acc.bytes;
     for (int q = 0; q < 3; ++q) b += ade_j[q].bytes + ade_jp[q].bytes + ade_eprev[q].bytes;
     for (auto& gth : gathers) b += gth->out.bytes;
     return b;
@@ -672,6 +679,37 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     c.ade_poles = A.poles;
   }
 
+  for (int ni = 0; ni < d.n_nf2ff; ++ni) {
+    const auto& N = d.nf2ff[ni];
+    require(N.nf > 0 && N.nt > 0 && N.te != nullptr && N.th != nullptr, "NF2FF tables missing");
+    auto nn = std::make_unique<xfd_ctx::Nf2ff>();
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    nn->a.off[0] = 0;
+    for (int s6 = 0; s6 < 6; ++s6) {
+      int64_t cells = 1;
+      for (int q = 0; q < 3; ++q) {
+        const int lo = N.lo[s6][q], hi = N.hi[s6][q];
+        // the face-centre stencils reach one cell down and one up
+        require(lo >= 1 && lo <= hi && hi <= n3[q], "NF2FF surface outside the grid interior");
+        nn->a.lo[s6][q] = lo;
+        nn->a.ext[s6][q] = hi - lo;
+        cells *= hi - lo;
+      }
+      nn->a.off[s6 + 1] = nn->a.off[s6] + cells;
+    }
+    nn->a.nf = N.nf;
+    nn->a.nt = N.nt;
+    const size_t tab = static_cast<size_t>(N.nf) * N.nt * 2;
+    uploadVector(std::vector<double>(N.te, N.te + tab), nn->te);
+    uploadVector(std::vector<double>(N.th, N.th + tab), nn->th);
+    nn->acc.alloc(static_cast<size_t>(N.nf) * 4 * nn->a.off[6] * 2 * sizeof(double));
+    nn->a.te = nn->te.as<double2>();
+    nn->a.th = nn->th.as<double2>();
+    nn->a.acc = nn->acc.as<double2>();
+    c.pool_rows = std::min(c.pool_rows, N.nt);
+    c.nf2ffs.push_back(std::move(nn));
+  }
+
   for (int gi = 0; gi < d.n_gather; ++gi) {
     const auto& G = d.gather[gi];
     require(G.field >= 0 && G.field < 6 && G.interval > 0 && G.count >= 0, "bad recording");
@@ -833,6 +871,8 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
         xfd::Launch<T>::gather(st, c.g, F, gp->a, gp->out.as<double>() + frame * gp->a.frame_elems);
       });
     }
+    for (auto& nn : c.nf2ffs)  // Domain::record: monitors, then NF2FF (domain.cpp:176-184)
+      c.timed(K_NF2FF, [&] { xfd::Launch<T>::nf2ff(st, c.g, F, nn->a, t); });
     XFD_HIP(hipGetLastError());
     if (c.prof && c.pending.size() > 4096) {
       XFD_HIP(hipStreamSynchronize(st));
@@ -980,6 +1020,18 @@ int xfd_kernel_stats(xfd_ctx* c, int32_t k, const char** name, int64_t* launches
     if (name) *name = kKernelNames[k];
     if (launches) *launches = c->launches[k];
     if (total_ms) *total_ms = c->total_ms[k];
+  });
+}
+
+int xfd_read_nf2ff(xfd_ctx* c, int32_t n, int32_t f, int32_t d, int32_t which, double* dst) {
+  return guard([&] {
+    require(c && dst && n >= 0 && n < static_cast<int32_t>(c->nf2ffs.size()), "bad NF2FF index");
+    auto& nn = *c->nf2ffs[n];
+    require(f >= 0 && f < nn.a.nf && d >= 0 && d < 6 && which >= 0 && which < 4, "bad NF2FF slice");
+    const int64_t cells = nn.a.off[6], cnt = nn.a.off[d + 1] - nn.a.off[d];
+    if (cnt == 0) return;
+    const double2* src = nn.acc.as<double2>() + (static_cast<int64_t>(f) * 4 + which) * cells + nn.a.off[d];
+    XFD_HIP(hipMemcpy(dst, src, cnt * sizeof(double2), hipMemcpyDeviceToHost));
   });
 }
 

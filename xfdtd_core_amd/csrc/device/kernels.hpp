@@ -240,6 +240,79 @@ __global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Co
   }
 }
 
+// interpolate::interpolateSurfaceCenter (interpolate_scheme.h:18-138): field
+// component f (0..5) at the centre of the face normal to axis c at the
+// padded index idx.
+template <class T>
+__device__ __forceinline__ T faceCentre(const Grid& g, const FieldPtrs<T>& F, int c, int f, int64_t idx) {
+  const T* a = F.f[f];
+  const int64_t sx = g.plane, sy = g.pz, sz = 1;
+  const T h = T(0.5), q = T(0.25);
+  if (c == 0) {
+    if (f == 5) return q * (a[idx + sz] + a[idx] + a[idx - sx + sz] + a[idx - sx]);
+    if (f == 4) return q * (a[idx + sy] + a[idx] + a[idx - sx + sy] + a[idx - sx]);
+    if (f == 2) return h * (a[idx + sy] + a[idx]);
+    return h * (a[idx + sz] + a[idx]);  // f == 1
+  }
+  if (c == 1) {
+    if (f == 5) return q * (a[idx + sz] + a[idx] + a[idx - sy + sz] + a[idx - sy]);
+    if (f == 3) return q * (a[idx + sx] + a[idx] + a[idx + sx - sy] + a[idx - sy]);
+    if (f == 2) return h * (a[idx + sx] + a[idx]);
+    return h * (a[idx + sz] + a[idx]);  // f == 0
+  }
+  if (f == 3) return q * (a[idx + sx] + a[idx] + a[idx - sz] + a[idx + sx - sz]);
+  if (f == 4) return q * (a[idx] + a[idx + sy] + a[idx - sz] + a[idx + sy - sz]);
+  if (f == 0) return h * (a[idx + sy] + a[idx]);
+  return h * (a[idx + sx] + a[idx]);  // f == 1
+}
+
+// NF2FF running DFT, one thread per surface cell: FDPlaneData::calculateJ
+// and calculateM (nffft_fd_data_update.cpp:22-122) for every frequency,
+// each accumulator advanced `nf` times as NFFFTFrequencyDomain::update
+// repeats the whole update once per frequency.  FP64 accumulators.
+template <class T>
+__global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs a, int64_t t) {
+  const int64_t cell = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (cell >= a.off[6]) return;
+  int d = 0;
+  while (cell >= a.off[d + 1]) ++d;
+  const int64_t q = cell - a.off[d];
+  const int e1 = a.ext[d][1], e2 = a.ext[d][2];
+  const int k = a.lo[d][2] + static_cast<int>(q % e2);
+  const int j = a.lo[d][1] + static_cast<int>((q / e2) % e1);
+  const int i = a.lo[d][0] + static_cast<int>(q / (static_cast<int64_t>(e2) * e1));
+  const int64_t idx = (static_cast<int64_t>(i) * g.NY + j) * g.pz + k;
+  const int c = d / 2, A = (c + 1) % 3, B = (c + 2) % 3;
+  const bool pos = d & 1;
+  // coff_ja / coff_jb (calculateJ) and coff_ma / coff_mb (calculateM)
+  const double xja = (pos ? -1.0 : 1.0) * static_cast<double>(faceCentre(g, F, c, 3 + B, idx));
+  const double xjb = (pos ? 1.0 : -1.0) * static_cast<double>(faceCentre(g, F, c, 3 + A, idx));
+  const double xma = (pos ? 1.0 : -1.0) * static_cast<double>(faceCentre(g, F, c, B, idx));
+  const double xmb = (pos ? -1.0 : 1.0) * static_cast<double>(faceCentre(g, F, c, A, idx));
+  const int64_t cells = a.off[6];
+  for (int f = 0; f < a.nf; ++f) {
+    const double2 th = a.th[f * a.nt + t], te = a.te[f * a.nt + t];
+    const double2 ija = {xja * th.x, xja * th.y}, ijb = {xjb * th.x, xjb * th.y};
+    const double2 ima = {xma * te.x, xma * te.y}, imb = {xmb * te.x, xmb * te.y};
+    double2* p = a.acc + static_cast<int64_t>(f) * 4 * cells + cell;
+    double2 ja = p[0], jb = p[cells], ma = p[2 * cells], mb = p[3 * cells];
+    for (int r = 0; r < a.nf; ++r) {
+      ja.x = ja.x + ija.x;
+      ja.y = ja.y + ija.y;
+      jb.x = jb.x + ijb.x;
+      jb.y = jb.y + ijb.y;
+      ma.x = ma.x + ima.x;
+      ma.y = ma.y + ima.y;
+      mb.x = mb.x + imb.x;
+      mb.y = mb.y + imb.y;
+    }
+    p[0] = ja;
+    p[cells] = jb;
+    p[2 * cells] = ma;
+    p[3 * cells] = mb;
+  }
+}
+
 // Lumped-element and TFSF corrections: one thread per corrected cell applies
 // that cell's records in the reference's corrector order.  `state` holds the
 // inductor currents (one slot per inductor edge).
@@ -333,6 +406,7 @@ struct Launch {
   static void records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
                       const double* pool, T* state, int64_t t);
   static void gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out);
+  static void nf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Nf2ffArgs& a, int64_t t);
   static void unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2);
   static void pack(hipStream_t s, const Grid& g, const T* src, double* dst, int n0, int n1, int n2);
 };

@@ -65,6 +65,12 @@ void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F
 }
 
 template <class T>
+void Launch<T>::nf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Nf2ffArgs& a, int64_t t) {
+  if (a.off[6] <= 0) return;
+  hipLaunchKernelGGL(k_nf2ff<T>, dim3(blocks_for(a.off[6], 256)), dim3(256), 0, s, g, F, a, t);
+}
+
+template <class T>
 void Launch<T>::probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                        int64_t t) {
   if (n <= 0) return;

@@ -75,6 +75,20 @@ struct AdeArgs {
   T* eprev[3];
 };
 
+// NF2FF running DFT: the six surfaces' cells flattened (surface d owns
+// [off[d], off[d+1])), each surface a box [lo, hi) with k fastest.
+// acc holds nf x 4 x cells complex values: (f * 4 + w) * cells + cell, w =
+// ja, jb, ma, mb.  te / th: nf x nt complex phase tables.
+struct Nf2ffArgs {
+  int lo[6][3], ext[6][3];
+  int64_t off[7];
+  int nf;
+  int64_t nt;
+  const double2* te;
+  const double2* th;
+  double2* acc;
+};
+
 // One CPML slab for one half-step.  The thread space is the union box of the
 // two corrected components; `ext_a` / `ext_b` give each component's extent
 // along the tangential axes a and b (the other component's is one larger).

@@ -1,0 +1,266 @@
+// Host side of the frequency-domain NF2FF: surfaces, DFT phase tables, and
+// the far-field potentials and radiated power formed from the surface
+// currents that the device accumulated (FDPlaneData, nffft_fd_data*.cpp).
+// Built like the rest of the host precompute (FP64, no FMA contraction).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <filesystem>
+#include <sstream>
+#include <iomanip>
+
+#include "../../../include/xfdtd_core_amd.hpp"
+#include "nf2ff_data.hpp"
+#include "npy.hpp"
+
+namespace xfdtd {
+
+namespace {
+using Cplx = std::complex<Real>;
+constexpr Real kPiN = 3.141592653589793238462643383279502884;
+const Real kC0N = 1 / std::sqrt(8.854187817e-12 * (4 * kPiN * 1e-7));
+
+// GCC's complex product for finite operands: (ac - bd, ad + bc)
+Cplx cmul(const Cplx& a, const Cplx& b) {
+  return {a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()};
+}
+}  // namespace
+
+NFFFTFrequencyDomain::NFFFTFrequencyDomain(std::size_t dx, std::size_t dy, std::size_t dz,
+                                           std::vector<Real> frequencies, std::string output_dir)
+    : _distance{dx, dy, dz},
+      _frequencies(std::move(frequencies)),
+      _output_dir(std::move(output_dir)),
+      _data(std::make_unique<Data>()) {
+  // nffft.cpp:19-33
+  if (dx == 0) throw XFDTDException("distance_x == 0");
+  if (dy == 0) throw XFDTDException("distance_y == 0");
+  if (dz == 0) throw XFDTDException("distance_z == 0");
+}
+NFFFTFrequencyDomain::~NFFFTFrequencyDomain() = default;
+
+void NFFFTFrequencyDomain::Data::setup(const std::size_t dist[3], const std::size_t n[3],
+                                       const std::vector<Real> node_e[3], const std::vector<Real> node_h[3],
+                                       const std::vector<Real> size_e[3], const std::vector<Real>& freq, Real dt,
+                                       std::size_t steps) {
+  // NFFFT::initGlobal / makeNodeAxisTask (nffft.cpp:73-223) for one process
+  std::size_t lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = dist[a];
+    hi[a] = n[a] > dist[a] ? n[a] - dist[a] : 0;
+    if (hi[a] <= lo[a]) throw XFDTDException("NFFFT: Global box is not legal");
+  }
+  bool any = false;
+  for (int d = 0; d < 6; ++d) {
+    const int c = d / 2;
+    const bool neg = d % 2 == 0;
+    const std::size_t cs = neg ? lo[c] : hi[c];
+    used[d] = !((neg && cs <= 1) || (!neg && cs >= n[c]));
+    for (int a = 0; a < 3; ++a) {  // an unused surface is the empty task, as IndexTask{}
+      box_lo[d][a] = used[d] ? (a == c ? cs : lo[a]) : 0;
+      box_hi[d][a] = used[d] ? (a == c ? cs + 1 : hi[a]) : 0;
+    }
+    any = any || used[d];
+  }
+  if (!any) throw XFDTDException("NFFFT: Invalid task");
+  for (int a = 0; a < 3; ++a) {
+    e_node[a] = node_e[a];
+    h_node[a] = node_h[a];
+    e_size[a] = size_e[a];
+  }
+  freqs = freq;
+  // FDPlaneData::initDFT (nffft_fd_data.cpp:65-76)
+  const Cplx II{0, 1};
+  te.assign(freqs.size() * steps, Cplx{});
+  th.assign(freqs.size() * steps, Cplx{});
+  for (std::size_t f = 0; f < freqs.size(); ++f)
+    for (std::size_t t = 0; t < steps; ++t) {
+      te[f * steps + t] = dt * std::exp(-II * static_cast<Real>(2.0) * kPiN * (freqs[f] * (t + 1) * dt));
+      th[f * steps + t] =
+          dt * std::exp(-II * static_cast<Real>(2.0) * kPiN * (freqs[f] * (t + static_cast<Real>(0.5)) * dt));
+    }
+  nt = steps;
+  cur.assign(freqs.size() * 6 * 4, {});
+  for (std::size_t f = 0; f < freqs.size(); ++f)
+    for (int d = 0; d < 6; ++d)
+      for (int w = 0; w < 4; ++w) cur[(f * 6 + d) * 4 + w].assign(cells(d), Cplx{});
+}
+
+std::size_t NFFFTFrequencyDomain::Data::cells(int d) const {
+  if (!used[d]) return 0;
+  return (box_hi[d][0] - box_lo[d][0]) * (box_hi[d][1] - box_lo[d][1]) * (box_hi[d][2] - box_lo[d][2]);
+}
+
+std::array<std::size_t, 6> NFFFTFrequencyDomain::surface(int d) const {
+  const auto& D = *_data;
+  return {D.box_lo[d][0], D.box_lo[d][1], D.box_lo[d][2], D.box_hi[d][0], D.box_hi[d][1], D.box_hi[d][2]};
+}
+
+const std::vector<std::complex<Real>>& NFFFTFrequencyDomain::current(std::size_t f, int d, int which) const {
+  if (f >= _frequencies.size() || d < 0 || d > 5 || which < 0 || which > 3)
+    throw XFDTDException("NFFFT: bad current index");
+  return _data->cur.at((f * 6 + d) * 4 + which);
+}
+
+std::vector<std::complex<Real>> NFFFTFrequencyDomain::potential(std::size_t fi, Potential pot, bool theta_comp,
+                                                                 const std::vector<Real>& theta,
+                                                                 const std::vector<Real>& phi,
+                                                                 const Vector& origin) const {
+  // FDPlaneData::getPotential / calculatePotential (nffft_fd_data_process.cpp:44-160)
+  if (theta.empty() || phi.empty()) throw XFDTDException("Invalid theta or phi size");
+  if (theta.size() != 1 && phi.size() != 1) throw XFDTDException("There must be only one theta or phi");
+  const auto& D = *_data;
+  const std::size_t na = theta.size() * phi.size();
+  auto th = [&](std::size_t n) { return theta[theta.size() == 1 ? 0 : n]; };
+  auto ph = [&](std::size_t n) { return phi[phi.size() == 1 ? 0 : n]; };
+  // cCSToSCSTransformMatrix (transform.h:24-58)
+  auto tmat = [&](int axis, std::size_t n) -> Real {
+    const Real t = th(n), p = ph(n);
+    if (theta_comp) {
+      if (axis == 0) return std::cos(t) * std::cos(p);
+      if (axis == 1) return std::cos(t) * std::sin(p);
+      return -1 * std::sin(t);
+    }
+    if (axis == 0) return -1 * std::sin(p);
+    if (axis == 1) return std::cos(p);
+    return 0.0;
+  };
+  const Real freq = _frequencies.at(fi);
+  const Real k = 2.0 * kPiN * freq / kC0N;
+  const Cplx ik = Cplx{0, 1} * k;
+  std::vector<Cplx> total(na, Cplx{});
+  std::vector<Real> st(na), ct(na), stsp(na), stcp(na);
+  for (std::size_t n = 0; n < na; ++n) {
+    ct[n] = std::cos(th(n));
+    st[n] = std::sin(th(n));
+    stsp[n] = st[n] * std::sin(ph(n));
+    stcp[n] = st[n] * std::cos(ph(n));
+  }
+  for (int d = 0; d < 6; ++d) {
+    std::vector<Cplx> data(na, Cplx{});
+    if (D.used[d]) {
+      const int c = d / 2, A = (c + 1) % 3, B = (c + 2) % 3;
+      const int wa = pot == Potential::A ? 0 : 2;
+      const auto& va = D.cur[(fi * 6 + d) * 4 + wa];
+      const auto& vb = D.cur[(fi * 6 + d) * 4 + wa + 1];
+      std::vector<Cplx> ta(na), tb(na);
+      for (std::size_t n = 0; n < na; ++n) {
+        ta[n] = tmat(A, n);
+        tb[n] = tmat(B, n);
+      }
+      std::size_t q = 0;
+      for (std::size_t i = D.box_lo[d][0]; i < D.box_hi[d][0]; ++i)
+        for (std::size_t j = D.box_lo[d][1]; j < D.box_hi[d][1]; ++j)
+          for (std::size_t kk = D.box_lo[d][2]; kk < D.box_hi[d][2]; ++kk, ++q) {
+            // rVector / ds (nffft_fd_data_process.cpp:196-225)
+            Real r0, r1, r2, ds;
+            if (c == 0) {
+              r0 = D.e_node[0][i];
+              r1 = D.h_node[1][j];
+              r2 = D.h_node[2][kk];
+              ds = D.e_size[1][j] * D.e_size[2][kk];
+            } else if (c == 1) {
+              r0 = D.h_node[0][i];
+              r1 = D.e_node[1][j];
+              r2 = D.h_node[2][kk];
+              ds = D.e_size[2][kk] * D.e_size[0][i];
+            } else {
+              r0 = D.h_node[0][i];
+              r1 = D.h_node[1][j];
+              r2 = D.e_node[2][kk];
+              ds = D.e_size[0][i] * D.e_size[1][j];
+            }
+            const Real rx = r0 - origin[0], ry = r1 - origin[1], rz = r2 - origin[2];
+            for (std::size_t n = 0; n < na; ++n) {
+              const Real v = rx * stcp[n] + ry * stsp[n] + rz * ct[n];
+              const Cplx phase = std::exp(ik * v);
+              const Cplx sum = cmul(va[q], ta[n]) + cmul(vb[q], tb[n]);
+              data[n] += cmul(sum, phase) * ds;
+            }
+          }
+    }
+    for (std::size_t n = 0; n < na; ++n) total[n] += data[n];
+  }
+  return total;
+}
+
+Real NFFFTFrequencyDomain::power(std::size_t fi) const {
+  // FDPlaneData::power / calculatePower (nffft_fd_data_process.cpp:28-42, 162-194)
+  const auto& D = *_data;
+  Cplx acc{0.0};
+  for (int d = 0; d < 6; ++d) {
+    Cplx pw{0.0};
+    if (D.used[d]) {
+      const int c = d / 2;
+      const auto& ja = D.cur[(fi * 6 + d) * 4 + 0];
+      const auto& jb = D.cur[(fi * 6 + d) * 4 + 1];
+      const auto& ma = D.cur[(fi * 6 + d) * 4 + 2];
+      const auto& mb = D.cur[(fi * 6 + d) * 4 + 3];
+      std::size_t q = 0;
+      for (std::size_t i = D.box_lo[d][0]; i < D.box_hi[d][0]; ++i)
+        for (std::size_t j = D.box_lo[d][1]; j < D.box_hi[d][1]; ++j)
+          for (std::size_t kk = D.box_lo[d][2]; kk < D.box_hi[d][2]; ++kk, ++q) {
+            Real ds;
+            if (c == 0) ds = D.e_size[1][j] * D.e_size[2][kk];
+            else if (c == 1) ds = D.e_size[2][kk] * D.e_size[0][i];
+            else ds = D.e_size[0][i] * D.e_size[1][j];
+            pw += ds * (cmul(std::conj(ja[q]), mb[q]) - cmul(std::conj(jb[q]), ma[q]));
+          }
+      const Real negative = d % 2 == 0 ? -1.0 : 1.0;
+      pw = negative * pw;
+    }
+    acc = acc + pw;
+  }
+  return 0.5 * std::real(acc);
+}
+
+void NFFFTFrequencyDomain::processFarField(const std::vector<Real>& theta, Real phi, const std::string& sub_dir,
+                                           const Vector& origin) const {
+  const std::vector<Real> ph{phi};
+  const auto dir = std::filesystem::path(_output_dir) / sub_dir;
+  for (std::size_t f = 0; f < _frequencies.size(); ++f) {
+    std::stringstream ss;
+    ss << std::fixed << std::setprecision(2) << _frequencies[f] / 1e9 << "GHz";
+    std::filesystem::create_directories(dir);
+    const char* names[4] = {"_a_theta.npy", "_f_phi.npy", "_a_phi.npy", "_f_theta.npy"};
+    const Potential pots[4] = {Potential::A, Potential::F, Potential::A, Potential::F};
+    const bool comps[4] = {true, false, false, true};
+    for (int q = 0; q < 4; ++q) {
+      auto v = potential(f, pots[q], comps[q], theta, ph, origin);
+      npy::writeComplex((dir / (ss.str() + names[q])).string(), {v.size()}, reinterpret_cast<const double*>(v.data()));
+    }
+  }
+}
+
+void NFFFTFrequencyDomain::processFarField(Real theta, const std::vector<Real>& phi, const std::string& sub_dir,
+                                           const Vector& origin) const {
+  const std::vector<Real> th{theta};
+  const auto dir = std::filesystem::path(_output_dir) / sub_dir;
+  for (std::size_t f = 0; f < _frequencies.size(); ++f) {
+    std::stringstream ss;
+    ss << std::fixed << std::setprecision(2) << _frequencies[f] / 1e9 << "GHz";
+    std::filesystem::create_directories(dir);
+    const char* names[4] = {"_a_theta.npy", "_f_phi.npy", "_a_phi.npy", "_f_theta.npy"};
+    const Potential pots[4] = {Potential::A, Potential::F, Potential::A, Potential::F};
+    const bool comps[4] = {true, false, false, true};
+    for (int q = 0; q < 4; ++q) {
+      auto v = potential(f, pots[q], comps[q], th, phi, origin);
+      npy::writeComplex((dir / (ss.str() + names[q])).string(), {v.size()}, reinterpret_cast<const double*>(v.data()));
+    }
+  }
+}
+
+void NFFFTFrequencyDomain::outputRadiationPower() const {
+  // nffft_frequency_domain.cpp:65-92: stack(frequencies, power) -> power.npy
+  const std::size_t nf = _frequencies.size();
+  std::vector<Real> out(2 * nf);
+  for (std::size_t f = 0; f < nf; ++f) {
+    out[f] = _frequencies[f];
+    out[nf + f] = power(f);
+  }
+  std::filesystem::create_directories(_output_dir.empty() ? "." : _output_dir);
+  npy::write((std::filesystem::path(_output_dir) / "power.npy").string(), {2, nf}, out.data());
+}
+
+}  // namespace xfdtd

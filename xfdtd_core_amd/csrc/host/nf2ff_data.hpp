@@ -1,0 +1,28 @@
+// State of one NFFFTFrequencyDomain on the host: the six surfaces, the DFT
+// phase tables handed to the device, a copy of the grid it needs for the far
+// field, and the surface currents copied back after the run.
+#pragma once
+
+#include <complex>
+#include <cstddef>
+#include <vector>
+
+#include "../../../include/xfdtd_core_amd.hpp"
+
+namespace xfdtd {
+
+struct NFFFTFrequencyDomain::Data {
+  bool used[6] = {false, false, false, false, false, false};
+  std::size_t box_lo[6][3] = {}, box_hi[6][3] = {};
+  std::vector<Real> e_node[3], h_node[3], e_size[3];
+  std::vector<Real> freqs;
+  std::vector<std::complex<Real>> te, th;  // nf x nt
+  std::size_t nt = 0;
+  std::vector<std::vector<std::complex<Real>>> cur;  // index (f * 6 + d) * 4 + which
+  void setup(const std::size_t dist[3], const std::size_t n[3], const std::vector<Real> node_e[3],
+             const std::vector<Real> node_h[3], const std::vector<Real> size_e[3], const std::vector<Real>& freq,
+             Real dt, std::size_t steps);
+  std::size_t cells(int d) const;
+};
+
+}  // namespace xfdtd

@@ -11,11 +11,12 @@
 
 namespace npy {
 
-inline void write(const std::string& path, const std::vector<std::size_t>& shape, const double* data) {
+inline void write(const std::string& path, const std::vector<std::size_t>& shape, const double* data,
+                  const char* descr = "<f8", std::size_t item_doubles = 1) {
   std::string sh = "(";
   for (std::size_t i = 0; i < shape.size(); ++i) sh += std::to_string(shape[i]) + (shape.size() == 1 || i + 1 < shape.size() ? ", " : "");
   sh += ")";
-  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': " + sh + ", }";
+  std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
   const std::size_t unpadded = 10 + hdr.size() + 1;
   hdr += std::string((64 - unpadded % 64) % 64, ' ');
   hdr += '\n';
@@ -28,8 +29,14 @@ inline void write(const std::string& path, const std::vector<std::size_t>& shape
   std::fwrite(hdr.data(), 1, hdr.size(), f);
   std::size_t n = 1;
   for (auto s : shape) n *= s;
-  if (n) std::fwrite(data, sizeof(double), n, f);
+  if (n) std::fwrite(data, sizeof(double), n * item_doubles, f);
   std::fclose(f);
+}
+
+// Complex128 ('<c16') array: interleaved (re, im) doubles, as xt::dump_npy
+// writes std::complex<double>.
+inline void writeComplex(const std::string& path, const std::vector<std::size_t>& shape, const double* data) {
+  write(path, shape, data, "<c16", 2);
 }
 
 }  // namespace npy

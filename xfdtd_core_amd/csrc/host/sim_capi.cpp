@@ -18,6 +18,7 @@ thread_local std::string g_err;
 struct SimHandle {
   std::unique_ptr<Simulation> sim;
   std::vector<std::shared_ptr<Monitor>> monitors;
+  std::vector<std::shared_ptr<NFFFTFrequencyDomain>> nf2ffs;
   Precision precision = Precision::FP64;
 };
 
@@ -212,6 +213,17 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
     } else if (d == "pec_plane") {
       std::string name = t.word();
       s.addObject(std::make_shared<PecPlane>(name, cube(t)));
+    } else if (d == "nf2ff") {
+      std::size_t dist[3];
+      for (auto& x : dist) x = static_cast<std::size_t>(t.num());
+      const auto nf = static_cast<std::size_t>(t.num());
+      std::vector<Real> fr(nf);
+      for (auto& f : fr) f = t.num();
+      std::string dir = t.word();
+      if (dir == "-") dir.clear();
+      auto n = std::make_shared<NFFFTFrequencyDomain>(dist[0], dist[1], dist[2], fr, dir);
+      h->nf2ffs.push_back(n);
+      s.addNF2FF(n);
     } else if (d == "movie") {
       auto f = parseField(t.word());
       auto cb = cube(t);
@@ -320,6 +332,56 @@ int xfd_sim_grid(void* p, int64_t* n3) {
 
 int64_t xfd_sim_device_bytes(void* p) { return H(p)->sim->deviceBytes(); }
 int32_t xfd_sim_coef_classed(void* p) { return H(p)->sim->classedCoefficients() ? 1 : 0; }
+
+int32_t xfd_sim_nf2ff_count(void* p) { return static_cast<int32_t>(H(p)->nf2ffs.size()); }
+
+int xfd_sim_nf2ff_surface(void* p, int32_t n, int32_t d, int64_t* lo, int64_t* hi) {
+  return guard([&] {
+    const auto b = H(p)->nf2ffs.at(n)->surface(d);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = static_cast<int64_t>(b[a]);
+      hi[a] = static_cast<int64_t>(b[3 + a]);
+    }
+  });
+}
+
+int xfd_sim_nf2ff_current(void* p, int32_t n, int32_t f, int32_t d, int32_t which, double* dst) {
+  return guard([&] {
+    const auto& v = H(p)->nf2ffs.at(n)->current(static_cast<std::size_t>(f), d, which);
+    std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  });
+}
+
+int xfd_sim_nf2ff_potential(void* p, int32_t n, int32_t f, int32_t a_pot, int32_t theta_comp, int32_t nt,
+                            const double* theta, int32_t np, const double* phi, const double* origin, double* dst) {
+  return guard([&] {
+    std::vector<Real> th(theta, theta + nt), ph(phi, phi + np);
+    auto v = H(p)->nf2ffs.at(n)->potential(
+        static_cast<std::size_t>(f), a_pot ? NFFFTFrequencyDomain::Potential::A : NFFFTFrequencyDomain::Potential::F,
+        theta_comp != 0, th, ph, Vector(origin[0], origin[1], origin[2]));
+    std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  });
+}
+
+int xfd_sim_nf2ff_power(void* p, int32_t n, int32_t f, double* dst) {
+  return guard([&] { *dst = H(p)->nf2ffs.at(n)->power(static_cast<std::size_t>(f)); });
+}
+
+int xfd_sim_nf2ff_process_far_field(void* p, int32_t n, int32_t nt, const double* theta, int32_t np,
+                                    const double* phi, const char* sub_dir, const double* origin) {
+  return guard([&] {
+    auto& nf = *H(p)->nf2ffs.at(n);
+    const Vector o(origin[0], origin[1], origin[2]);
+    if (nt == 1)
+      nf.processFarField(theta[0], std::vector<Real>(phi, phi + np), sub_dir, o);
+    else
+      nf.processFarField(std::vector<Real>(theta, theta + nt), phi[0], sub_dir, o);
+  });
+}
+
+int xfd_sim_nf2ff_output_power(void* p, int32_t n) {
+  return guard([&] { H(p)->nf2ffs.at(n)->outputRadiationPower(); });
+}
 
 int xfd_sim_field(void* p, int32_t field, int64_t* shape, double* dst) {
   return guard([&] {

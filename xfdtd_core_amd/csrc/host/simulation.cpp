@@ -18,6 +18,7 @@
 
 #include "../../../include/xfdtd_core_amd.h"
 #include "../../../include/xfdtd_core_amd.hpp"
+#include "nf2ff_data.hpp"
 #include "npy.hpp"
 
 namespace xfdtd {
@@ -317,6 +318,8 @@ struct Simulation::Impl {
   std::vector<std::shared_ptr<WaveformSource>> sources;
   std::vector<std::shared_ptr<Boundary>> boundaries;
   std::vector<std::shared_ptr<Monitor>> monitors;
+  std::vector<std::shared_ptr<NFFFTFrequencyDomain>> nf2ffs;
+  std::vector<xfd_nf2ff_desc> nf2ff_d;
 
   // grid (grid_space_generator.cpp, grid_space.cpp:526-544)
   std::size_t N[3] = {0, 0, 0};
@@ -790,6 +793,30 @@ struct Simulation::Impl {
   std::vector<Real> eTime() const { return linspaceAssign(1 * dt, steps * dt, steps); }
   std::vector<Real> hTime() const { return linspaceAssign(0.5 * dt, (steps + 0.5) * dt, steps); }
 
+  // ----------------------------------------------------------------- NF2FF --
+  // NFFFT::init + FDPlaneData::initDFT (nffft.cpp:73-223,
+  // nffft_fd_data.cpp:65-76): surfaces and phase tables of each NF2FF.
+  void buildNf2ff() {
+    nf2ff_d.clear();
+    for (auto& n : nf2ffs) {
+      const std::size_t dist[3] = {n->distance(0), n->distance(1), n->distance(2)};
+      auto& D = n->data();
+      D.setup(dist, N, e_node, h_node, e_size, n->frequencies(), dt, steps);
+      xfd_nf2ff_desc d{};
+      for (int s6 = 0; s6 < 6; ++s6)
+        for (int a = 0; a < 3; ++a) {
+          // unused surfaces are passed as an empty box at a legal position
+          const std::size_t lo = D.used[s6] ? D.box_lo[s6][a] : 1;
+          const std::size_t hi = D.used[s6] ? D.box_hi[s6][a] : 1;
+          d.lo[s6][a] = static_cast<int32_t>(lo);
+          d.hi[s6][a] = static_cast<int32_t>(hi);
+        }
+      d.nf = static_cast<int32_t>(n->freqCount());
+      d.nt = static_cast<int64_t>(D.nt);
+      nf2ff_d.push_back(d);
+    }
+  }
+
   // ------------------------------------------------------------------- ADE --
   // Simulation::buildDispersiveSpace (simulation.cpp:745-829) and
   // Object::handleDispersion (object.cpp:52-98): the scheme follows the kinds
@@ -1223,6 +1250,7 @@ struct Simulation::Impl {
     buildDispersive();  // simulation.cpp:651, after every other coefficient patch
     stage("dispersive");
     buildMonitors();
+    buildNf2ff();
     coef_view.clear();
     stage("monitors");
     precomputed = true;
@@ -1283,6 +1311,13 @@ struct Simulation::Impl {
       ade_d.cell_material = ade_cell.data();
       dd.ade = &ade_d;
     }
+    for (std::size_t q = 0; q < nf2ffs.size(); ++q) {
+      auto& D = nf2ffs[q]->data();
+      nf2ff_d[q].te = reinterpret_cast<const double*>(D.te.data());
+      nf2ff_d[q].th = reinterpret_cast<const double*>(D.th.data());
+    }
+    dd.n_nf2ff = static_cast<int32_t>(nf2ff_d.size());
+    dd.nf2ff = nf2ff_d.empty() ? nullptr : nf2ff_d.data();
     if (xfd_create(&dd, &ctx) != XFD_OK) throw XFDTDSimulationException(std::string("xfd_create: ") + xfd_last_error());
     int64_t b = 0;
     xfd_device_bytes(ctx, &b);
@@ -1368,6 +1403,18 @@ struct Simulation::Impl {
       if (xfd_read_field(ctx, f, dst[f]->data.data()) != XFD_OK)
         throw XFDTDSimulationException(std::string("xfd_read_field: ") + xfd_last_error());
     }
+    for (std::size_t q = 0; q < nf2ffs.size(); ++q) {  // the surface currents
+      auto& D = nf2ffs[q]->data();
+      for (std::size_t f = 0; f < D.freqs.size(); ++f)
+        for (int d = 0; d < 6; ++d)
+          for (int w = 0; w < 4; ++w) {
+            auto& v = D.cur[(f * 6 + d) * 4 + w];
+            if (v.empty()) continue;
+            if (xfd_read_nf2ff(ctx, static_cast<int32_t>(q), static_cast<int32_t>(f), d, w,
+                               reinterpret_cast<double*>(v.data())) != XFD_OK)
+              throw XFDTDSimulationException(std::string("xfd_read_nf2ff: ") + xfd_last_error());
+          }
+    }
   }
 };
 
@@ -1392,6 +1439,10 @@ void Simulation::addBoundary(std::shared_ptr<Boundary> b) {
 }
 void Simulation::addMonitor(std::shared_ptr<Monitor> m) {
   _impl->monitors.push_back(std::move(m));
+  _impl->precomputed = false;
+}
+void Simulation::addNF2FF(std::shared_ptr<NFFFTFrequencyDomain> n) {
+  _impl->nf2ffs.push_back(std::move(n));
   _impl->precomputed = false;
 }
 void Simulation::setPrecision(Precision p) { _impl->precision = p; }

@@ -54,6 +54,13 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_grid": (ctypes.c_int, [vp, p64]),
         "xfd_sim_device_bytes": (i64, [vp]),
         "xfd_sim_coef_classed": (ctypes.c_int32, [vp]),
+        "xfd_sim_nf2ff_count": (ctypes.c_int32, [vp]),
+        "xfd_sim_nf2ff_surface": (ctypes.c_int, [vp, i32, i32, p64, p64]),
+        "xfd_sim_nf2ff_current": (ctypes.c_int, [vp, i32, i32, i32, i32, vp]),
+        "xfd_sim_nf2ff_potential": (ctypes.c_int, [vp, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp]),
+        "xfd_sim_nf2ff_power": (ctypes.c_int, [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]),
+        "xfd_sim_nf2ff_process_far_field": (ctypes.c_int, [vp, i32, i32, vp, i32, vp, ctypes.c_char_p, vp]),
+        "xfd_sim_nf2ff_output_power": (ctypes.c_int, [vp, i32]),
         "xfd_sim_grid_array": (i64, [vp, i32, i32, vp]),
         "xfd_sim_field": (ctypes.c_int, [vp, i32, p64, vp]),
         "xfd_sim_coeff": (ctypes.c_int, [vp, cp, p64, vp]),
@@ -151,7 +158,8 @@ class Desc(ctypes.Structure):
                 ("coeff", _dp * 18), ("n_cpml", ctypes.c_int32), ("cpml", ctypes.POINTER(CpmlDesc)),
                 ("n_lumped", ctypes.c_int32), ("lumped", ctypes.POINTER(LumpedDesc)),
                 ("n_tfsf", ctypes.c_int32), ("tfsf", ctypes.POINTER(TfsfDesc)),
-                ("n_gather", ctypes.c_int32), ("gather", ctypes.POINTER(GatherDesc))]
+                ("n_gather", ctypes.c_int32), ("gather", ctypes.POINTER(GatherDesc)),
+                ("ade", ctypes.c_void_p), ("n_nf2ff", ctypes.c_int32), ("nf2ff", ctypes.c_void_p)]
 
 
 def _ptr(a: np.ndarray):
@@ -292,6 +300,50 @@ class Simulation:
     def coef_classed(self) -> bool:
         """True when the device stores coefficient class ids, not 18 arrays."""
         return bool(self._L.xfd_sim_coef_classed(self._h))
+
+    # ---------------------------------------------------------------- NF2FF --
+    def nf2ff_count(self) -> int:
+        return int(self._L.xfd_sim_nf2ff_count(self._h))
+
+    def nf2ff_surface(self, n: int, d: int):
+        lo = (ctypes.c_int64 * 3)()
+        hi = (ctypes.c_int64 * 3)()
+        self._check(self._L.xfd_sim_nf2ff_surface(self._h, n, d, lo, hi))
+        return tuple(lo), tuple(hi)
+
+    def nf2ff_current(self, n: int, f: int, d: int, which: int) -> np.ndarray:
+        """Accumulated surface current on surface d: which 0 ja, 1 jb, 2 ma, 3 mb."""
+        lo, hi = self.nf2ff_surface(n, d)
+        shape = tuple(h - l for l, h in zip(lo, hi))
+        out = np.zeros(int(np.prod(shape)), dtype=np.complex128)
+        self._check(self._L.xfd_sim_nf2ff_current(self._h, n, f, d, which, out.ctypes.data))
+        return out.reshape(shape)
+
+    def nf2ff_potential(self, n: int, f: int, a_pot: bool, theta_comp: bool, theta, phi, origin=(0.0, 0.0, 0.0)):
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+        org = np.ascontiguousarray(origin, dtype=np.float64)
+        out = np.zeros(th.size * ph.size, dtype=np.complex128)
+        self._check(self._L.xfd_sim_nf2ff_potential(self._h, n, f, int(a_pot), int(theta_comp), th.size,
+                                                    th.ctypes.data, ph.size, ph.ctypes.data, org.ctypes.data,
+                                                    out.ctypes.data))
+        return out
+
+    def nf2ff_power(self, n: int, f: int) -> float:
+        v = ctypes.c_double()
+        self._check(self._L.xfd_sim_nf2ff_power(self._h, n, f, ctypes.byref(v)))
+        return v.value
+
+    def nf2ff_process_far_field(self, n: int, theta, phi, sub_dir: str, origin=(0.0, 0.0, 0.0)):
+        """processFarField: one of theta / phi is a scalar."""
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+        org = np.ascontiguousarray(origin, dtype=np.float64)
+        self._check(self._L.xfd_sim_nf2ff_process_far_field(self._h, n, th.size, th.ctypes.data, ph.size,
+                                                            ph.ctypes.data, sub_dir.encode(), org.ctypes.data))
+
+    def nf2ff_output_power(self, n: int):
+        self._check(self._L.xfd_sim_nf2ff_output_power(self._h, n))
 
     def grid_array(self, which: int, axis: int) -> np.ndarray:
         n = self._L.xfd_sim_grid_array(self._h, which, axis, None)

@@ -98,7 +98,7 @@ def inhomogeneous_box(n_total: int = 384, pml: int = 12, n_spheres: int = 64,
 
 def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab: int = 40, gap: int = 8,
                      tfsf_offset: int = 14, lumped: bool = True, kinds=("drude", "debye", "lorentz", "mlorentz"),
-                     snapshot_interval: int = 0):
+                     snapshot_interval: int = 0, nf2ff_distance: int = 0, nf2ff_freqs=None):
     """Dispersive + lumped box: four z-slabs (`slab` cells thick, `gap`-cell
     air gaps, centred, full transverse extent inside the CPML) of Drude
     (eps_inf=1, wp=1.37e16, gamma=1e14), Debye (eps_inf=4, eps_s=50,
@@ -109,7 +109,9 @@ def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab
     dispersive_material_equation.cpp:62-99).  TFSF plane wave (theta=30 deg,
     phi=45 deg) with a Gaussian of tau = 20 dl / 6e8; a 50-ohm Resistor and a
     50-ohm, 1 V VoltageSource along z, each a 6-cell gap in the air below the
-    slabs.  CFL 0.95.  `kinds` selects and orders the slab materials."""
+    slabs.  CFL 0.95.  `kinds` selects and orders the slab materials.  With
+    `nf2ff_distance`, a frequency-domain NF2FF on the box that many cells
+    inside the grid (default frequencies: 8 from 5 to 40 GHz)."""
     s = S.Simulation(dl, dl, dl, 0.95)
     n = n_total - 2 * pml
     half = n * dl / 2
@@ -143,4 +145,7 @@ def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab
         whole = S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half))
         for f in S.FIELDS:
             s.addMonitor(S.MovieMonitor(S.FieldMonitor(whole, f), snapshot_interval, f"snap_{f}", ""))
+    if nf2ff_distance:
+        freqs = list(nf2ff_freqs) if nf2ff_freqs is not None else [float(f) for f in np.linspace(5e9, 40e9, 8)]
+        s.addNF2FF(S.NFFFTFrequencyDomain(nf2ff_distance, nf2ff_distance, nf2ff_distance, freqs))
     return s
