@@ -121,10 +121,13 @@ typedef struct {
  * complex arrays on the surface's cells (k fastest):
  *   ja, jb += (+-1) * face-centre(H_b, H_a) * th[f][t]
  *   ma, mb += (+-1) * face-centre(E_b, E_a) * te[f][t]
- * in FP64, `nf` times per step as the reference's update loop does. */
+ * in FP64, `repeat` times per step (the reference's update loop repeats the
+ * whole update once per frequency, so its currents carry a factor nf). */
 typedef struct {
   int32_t lo[6][3], hi[6][3];  /* surfaces XN XP YN YP ZN ZP; hi == lo: unused */
-  int32_t nf;                  /* frequencies (and update repeats per step) */
+  int32_t nf;                  /* frequencies */
+  int32_t repeat;              /* accumulations per step: nf reproduces the
+                                  reference's loop, 1 is the plain DFT */
   int64_t nt;                  /* rows of the phase tables */
   const double* te;            /* nf x nt complex (re, im): dt exp(-j 2 pi f (t + 1) dt) */
   const double* th;            /* nf x nt complex: dt exp(-j 2 pi f (t + 1/2) dt) */

@@ -454,6 +454,12 @@ class NFFFTFrequencyDomain {
   const std::vector<Real>& frequencies() const { return _frequencies; }
   const std::string& outputDir() const { return _output_dir; }
   std::size_t distance(int axis) const { return _distance[axis]; }
+  // The reference advances every frequency's DFT once per frequency each
+  // step (nffft_frequency_domain.cpp:45-53), so its currents carry a factor
+  // nf and its power nf^2.  true (the default) keeps that; false accumulates
+  // each step once, the plain DFT.
+  void setReferenceRepeat(bool on) { _reference_repeat = on; }
+  bool referenceRepeat() const { return _reference_repeat; }
 
   // nffft_frequency_domain.cpp:56-160: writes <f>GHz_a_theta.npy,
   // _f_phi, _a_phi, _f_theta (complex) under outputDir()/sub_dir, and
@@ -484,6 +490,7 @@ class NFFFTFrequencyDomain {
   std::size_t _distance[3];
   std::vector<Real> _frequencies;
   std::string _output_dir;
+  bool _reference_repeat = true;
   std::unique_ptr<Data> _data;
 };
 

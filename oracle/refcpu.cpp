@@ -1317,7 +1317,8 @@ void NF2FF::update(Sim& s, Index t) {
   // every FDPlaneData::update runs once per frequency of the NFFFT
   // (nffft_frequency_domain.cpp:45-53); calculateJ then calculateM per
   // surface (nffft_fd_data_update.cpp:7-122)
-  for (Index rep = 0; rep < freqs.size(); ++rep)
+  const Index repeats = reference_repeat ? freqs.size() : 1;
+  for (Index rep = 0; rep < repeats; ++rep)
     for (auto& p : planes)
       for (int pass = 0; pass < 2; ++pass)
         for (int d = 0; d < 6; ++d) {

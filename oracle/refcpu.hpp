@@ -445,6 +445,7 @@ struct NF2FF {
   Index dist[3] = {1, 1, 1};
   std::vector<Real> freqs;
   std::string out_dir;
+  bool reference_repeat = true;  // false: one accumulation per step
   // surfaces XN XP YN YP ZN ZP (nffft.cpp:97-223); invalid ones are empty
   Task surf[6];
   struct Plane {

@@ -243,6 +243,8 @@ void parseSpec(Sim& s, const std::string& spec) {
       for (Index f = 0; f < nf; ++f) n->freqs.push_back(num(is));
       n->out_dir = word(is);
       if (n->out_dir == "-") n->out_dir.clear();
+      std::string opt;
+      if (is >> opt && opt == "plain_dft") n->reference_repeat = false;
       s.nf2ffs.push_back(std::move(n));
     } else if (d == "movie") {
       auto m = std::make_unique<MovieMonitor>();

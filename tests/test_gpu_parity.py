@@ -226,6 +226,18 @@ def test_nf2ff_dipole_fp64():
     assert _compare_nf2ff(dev, ref, 2, "dipole") <= TOL_F64
 
 
+def test_nf2ff_plain_dft_fp64():
+    """The non-repeating DFT option on the device equals the oracle's."""
+    from test_oracle_nf2ff import _dipole
+
+    sp = _dipole([6e9, 9e9], plain=True)
+    ref = RefSim(sp)
+    ref.run(300)
+    dev = native.Simulation(sp, "f64")
+    dev.run(300)
+    assert _compare_nf2ff(dev, ref, 2, "plain-dft") <= TOL_F64
+
+
 def test_nf2ff_dispersive_box_fp64():
     """NF2FF (8 frequencies, 5-40 GHz) around the dispersive, lumped and
     TFSF-driven box."""

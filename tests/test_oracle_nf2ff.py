@@ -36,7 +36,7 @@ TAU = 20 * DL / (2 * C0)
 T0 = 4.5 * TAU
 
 
-def _dipole(freqs):
+def _dipole(freqs, plain=False):
     s = S.Simulation(DL, DL, DL, 0.9)
     half = N * DL / 2
     s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
@@ -47,7 +47,7 @@ def _dipole(freqs):
                                 S.Waveform.gaussianDerivative(TAU, T0, 1.0)))
     for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
         s.addBoundary(S.PML(PML, d))
-    s.addNF2FF(S.NFFFTFrequencyDomain(DIST, DIST, DIST, list(freqs)))
+    s.addNF2FF(S.NFFFTFrequencyDomain(DIST, DIST, DIST, list(freqs), reference_repeat=not plain))
     return s.spec()
 
 
@@ -128,3 +128,19 @@ def test_update_repeats_once_per_frequency():
             a = one.nf2ff_current(0, 0, d, which)
             b = two.nf2ff_current(0, 0, d, which)
             np.testing.assert_allclose(b, 2 * a, rtol=1e-12, atol=1e-30)
+
+
+def test_plain_dft_option():
+    """With the repeat switched off (``plain_dft``) each frequency is
+    accumulated once per step: the same currents as a one-frequency run, and
+    the Hertzian-dipole power without the nf^2 factor."""
+    one = RefSim(_dipole([6e9]))
+    one.run(STEPS)
+    plain = RefSim(_dipole([6e9, 9e9], plain=True))
+    plain.run(STEPS)
+    for d in range(6):
+        for which in range(4):
+            np.testing.assert_array_equal(plain.nf2ff_current(0, 0, d, which), one.nf2ff_current(0, 0, d, which))
+    k = 2 * np.pi * 9e9 / C0
+    p_dipole = ETA * k ** 2 * abs(_source_current_dft(9e9, plain.dt) * DL) ** 2 / (12 * np.pi)
+    assert abs(plain.nf2ff_power(0, 1) / p_dipole - 1) < 2e-2

@@ -697,7 +697,9 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       }
       nn->a.off[s6 + 1] = nn->a.off[s6] + cells;
     }
+    require(N.repeat >= 1, "NF2FF repeat must be at least 1");
     nn->a.nf = N.nf;
+    nn->a.repeat = N.repeat;
     nn->a.nt = N.nt;
     const size_t tab = static_cast<size_t>(N.nf) * N.nt * 2;
     uploadVector(std::vector<double>(N.te, N.te + tab), nn->te);

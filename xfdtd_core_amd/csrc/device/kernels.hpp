@@ -25,10 +25,12 @@ constexpr int kRowsJ = 4;
 // BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41): Hx, Hy, Hz
 // over [0,nx) x [0,ny) x [0,nz).  Coef is CoefPtrs (dense) or ClassCoef.
 template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, int ichunk) {
-  const int k = blockIdx.x * kLanesK + threadIdx.x;
-  const int j = blockIdx.y * kRowsJ + threadIdx.y;
-  const int i0 = blockIdx.z * ichunk;
+__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
+  const int k = bx * kLanesK + threadIdx.x;
+  const int j = by * kRowsJ + threadIdx.y;
+  const int i0 = bz * ichunk;
   if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
   const int i1 = min(i0 + ichunk, g.nx);
   const int64_t pz = g.pz, pl = g.plane;
@@ -61,10 +63,12 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C
 // Ex over [0,nx) x [1,ny) x [1,nz), Ey over [1,nx) x [0,ny) x [1,nz),
 // Ez over [1,nx) x [1,ny) x [0,nz).
 template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, int ichunk) {
-  const int k = blockIdx.x * kLanesK + threadIdx.x;
-  const int j = blockIdx.y * kRowsJ + threadIdx.y;
-  const int i0 = blockIdx.z * ichunk;
+__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
+  const int k = bx * kLanesK + threadIdx.x;
+  const int j = by * kRowsJ + threadIdx.y;
+  const int i0 = bz * ichunk;
   if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
   const int i1 = min(i0 + ichunk, g.nx);
   const int64_t pz = g.pz, pl = g.plane;
@@ -196,10 +200,13 @@ __device__ __forceinline__ T adeUpdate(const AdeArgs<T>& A, int m, int q, int64_
 // a cell without dispersion its coefficients are zero and the result is the
 // plain update).
 template <class T, class Coef, int S>
-__global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, int ichunk) {
-  const int k = blockIdx.x * kLanesK + threadIdx.x;
-  const int j = blockIdx.y * kRowsJ + threadIdx.y;
-  const int i0 = blockIdx.z * ichunk;
+__global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
+                                                      int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
+  const int k = bx * kLanesK + threadIdx.x;
+  const int j = by * kRowsJ + threadIdx.y;
+  const int i0 = bz * ichunk;
   if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
   const int i1 = min(i0 + ichunk, g.nx);
   const int64_t pz = g.pz, pl = g.plane;
@@ -268,8 +275,9 @@ __device__ __forceinline__ T faceCentre(const Grid& g, const FieldPtrs<T>& F, in
 
 // NF2FF running DFT, one thread per surface cell: FDPlaneData::calculateJ
 // and calculateM (nffft_fd_data_update.cpp:22-122) for every frequency,
-// each accumulator advanced `nf` times as NFFFTFrequencyDomain::update
-// repeats the whole update once per frequency.  FP64 accumulators.
+// each accumulator advanced `repeat` times (nf reproduces
+// NFFFTFrequencyDomain::update, which repeats the whole update once per
+// frequency).  FP64 accumulators.
 template <class T>
 __global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs a, int64_t t) {
   const int64_t cell = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs
     const double2 ima = {xma * te.x, xma * te.y}, imb = {xmb * te.x, xmb * te.y};
     double2* p = a.acc + static_cast<int64_t>(f) * 4 * cells + cell;
     double2 ja = p[0], jb = p[cells], ma = p[2 * cells], mb = p[3 * cells];
-    for (int r = 0; r < a.nf; ++r) {
+    for (int r = 0; r < a.repeat; ++r) {
       ja.x = ja.x + ija.x;
       ja.y = ja.y + ija.y;
       jb.x = jb.x + ijb.x;

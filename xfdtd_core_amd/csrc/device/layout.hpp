@@ -24,6 +24,31 @@ struct Grid {
   int64_t total;   // (nx + 1) * plane
 };
 
+// Work-group order of the Yee sweeps.  The sweep grid is nbx (k) x nby (j)
+// x nbz (i-chunk) work-groups launched as a 1D grid.  The dispatcher hands
+// consecutive work-group ids to the 8 XCDs in turn, and each XCD has its own
+// L2; with `xcd` set, hardware id h is renumbered so that each XCD receives
+// one contiguous run of logical ids (k fastest, then j, then i), i.e. whole
+// neighbourhoods of tiles, and the rows/planes that neighbouring tiles share
+// are served from one L2 instead of being fetched from HBM once per XCD.
+struct SweepMap {
+  int nbx, nby, nbz;
+  int xcd;  // 0: identity order, else the number of XCDs
+  __device__ __forceinline__ void coords(int& bx, int& by, int& bz) const {
+    const int n = nbx * nby * nbz;
+    const int h = blockIdx.x;
+    int logical = h;
+    if (xcd > 1) {
+      const int per = n / xcd;
+      if (h < per * xcd) logical = (h % xcd) * per + h / xcd;
+    }
+    bx = logical % nbx;
+    const int r = logical / nbx;
+    by = r % nby;
+    bz = r / nby;
+  }
+};
+
 template <class T>
 struct FieldPtrs {
   T* f[6];  // Ex Ey Ez Hx Hy Hz
@@ -83,6 +108,7 @@ struct Nf2ffArgs {
   int lo[6][3], ext[6][3];
   int64_t off[7];
   int nf;
+  int repeat;
   int64_t nt;
   const double2* te;
   const double2* th;

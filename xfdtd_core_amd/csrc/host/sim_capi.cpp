@@ -222,6 +222,7 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
       std::string dir = t.word();
       if (dir == "-") dir.clear();
       auto n = std::make_shared<NFFFTFrequencyDomain>(dist[0], dist[1], dist[2], fr, dir);
+      if (t.more() && t.word() == "plain_dft") n->setReferenceRepeat(false);  // the non-repeating DFT
       h->nf2ffs.push_back(n);
       s.addNF2FF(n);
     } else if (d == "movie") {

@@ -812,6 +812,7 @@ struct Simulation::Impl {
           d.hi[s6][a] = static_cast<int32_t>(hi);
         }
       d.nf = static_cast<int32_t>(n->freqCount());
+      d.repeat = n->referenceRepeat() ? d.nf : 1;
       d.nt = static_cast<int64_t>(D.nt);
       nf2ff_d.push_back(d);
     }

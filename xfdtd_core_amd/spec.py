@@ -25,7 +25,7 @@ renders the scenario as line-oriented text, one directive per line:
     inductor NAME OX OY OZ SX SY SZ AXIS L
     pec_plane NAME OX OY OZ SX SY SZ
     movie FIELD OX OY OZ SX SY SZ INTERVAL NAME OUT_DIR
-    nf2ff DX DY DZ NF F1 .. FNF OUT_DIR
+    nf2ff DX DY DZ NF F1 .. FNF OUT_DIR [plain_dft]
     field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
     WAVEFORM = gaussian TAU T0 AMP | cosine_modulated_gaussian TAU T0 F AMP
              | sine F AMP | cosine F AMP | step T0 AMP
@@ -409,10 +409,12 @@ class NFFFTFrequencyDomain:
     distance_z: int
     frequencies: List[float]
     output_dir: str = ""
+    reference_repeat: bool = True  # False: the plain DFT, one accumulation per step
 
     def line(self):
         return " ".join(["nf2ff", str(int(self.distance_x)), str(int(self.distance_y)), str(int(self.distance_z)),
-                         str(len(self.frequencies))] + [_n(f) for f in self.frequencies] + [self.output_dir or "-"])
+                         str(len(self.frequencies))] + [_n(f) for f in self.frequencies] + [self.output_dir or "-"] +
+                        ([] if self.reference_repeat else ["plain_dft"]))
 
 
 class Simulation:
