@@ -201,6 +201,43 @@ int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
  * environment variable XFD_COEF_MODE=dense forces the dense layout. */
 int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
 
+/* Zeroes a context's time-stepping state: fields, CPML auxiliaries, inductor
+ * currents, ADE state, NF2FF accumulators and recordings (a fresh
+ * Simulation::run without rebuilding the coefficients). */
+int xfd_reset(xfd_ctx* ctx);
+
+/* Batched stepping of many independent simulations.  No reference
+ * counterpart: the reference runs one Simulation per process (an MPI domain
+ * split of one grid, src/simulation/simulation.cpp:401-494).  A batch owns
+ * one context per descriptor (all of one dtype) and steps them together:
+ * every phase of Domain::run (updateH, H correctors, CPML H, updateE, E
+ * correctors, CPML E, recordings, NF2FF) is one persistent launch over the
+ * work of all cases, whatever their sizes, with the step index kept on the
+ * device, so one step is a short fixed sequence of launches that can be
+ * replayed as a HIP graph.  Cases with dense coefficients or dispersive
+ * materials are stepped one by one instead (xfd_batch_info: shared = 0).
+ * Results are bitwise those of stepping each context with xfd_step. */
+typedef struct xfd_batch xfd_batch;
+
+int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out);
+void xfd_batch_destroy(xfd_batch* batch);
+/* Borrowed pointer to case i (read fields, recordings, NF2FF through it). */
+int xfd_batch_case(xfd_batch* batch, int32_t i, xfd_ctx** ctx);
+/* shared: 1 when the batched launches are used; ichunk and grid: the current
+ * i-planes per work-group and work-groups per persistent launch. */
+int xfd_batch_info(xfd_batch* batch, int32_t* shared, int32_t* ichunk, int32_t* grid);
+/* on = 1: the first xfd_batch_step captures one step as a HIP graph and
+ * every step replays it. */
+int xfd_batch_set_graph(xfd_batch* batch, int32_t on);
+/* Autotuner: times `reps` Yee sweep pairs for each i-chunk (4, 8, 16, 32) and
+ * persistent grid size (4, 8, 16, 32 work-groups per CU), keeps the fastest,
+ * then zeroes every case's state (xfd_reset). */
+int xfd_batch_tune(xfd_batch* batch, int32_t reps);
+/* Runs steps t0 .. t0+n-1 of every case and waits for them. */
+int xfd_batch_step(xfd_batch* batch, int64_t t0, int64_t n);
+/* Device time of the last xfd_batch_step, in milliseconds. */
+int xfd_batch_last_ms(xfd_batch* batch, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -554,6 +554,50 @@ class Simulation {
   struct Impl;
 
  private:
+  friend class SimulationBatch;
+  std::unique_ptr<Impl> _impl;
+};
+
+// Many independent Simulations stepped together on one GPU.  No reference
+// counterpart (the reference runs one Simulation per process).  Each phase of
+// the time step runs as one launch over the work of every case, whatever
+// their sizes, and one step can be captured once as a HIP graph and
+// replayed.  Every Simulation keeps its own results (emf(), monitors, NF2FF)
+// after finish(), bitwise those of running it alone.  The Simulations must
+// share one precision and outlive the batch's use of them; their own
+// advance() is disabled while they belong to an initialised batch.
+class SimulationBatch {
+ public:
+  SimulationBatch();
+  ~SimulationBatch();
+  SimulationBatch(const SimulationBatch&) = delete;
+  SimulationBatch& operator=(const SimulationBatch&) = delete;
+
+  void add(Simulation& sim);
+  std::size_t size() const;
+  // Precomputes every case (as Simulation::precompute) and creates the batch.
+  void init(Index time_step);
+  // Capture one step as a HIP graph on the first advance() and replay it.
+  void setGraph(bool on);
+  // Picks the i-chunk and persistent grid size by timing the Yee sweeps,
+  // then clears every case's state.
+  void tune(int reps = 5);
+  void advance(Index t0, Index n);
+  void finish();  // monitor recordings, final fields and NF2FF currents of every case
+  void run();
+  void run(Index time_step);
+  double loopSeconds() const;         // host clock around the last advance()
+  double deviceMilliseconds() const;  // HIP events around the last advance()
+  struct Layout {
+    bool shared;  // false: cases with dense coefficients or ADE, stepped one by one
+    int ichunk;
+    int grid;
+  };
+  Layout layout() const;
+
+  struct Impl;
+
+ private:
   std::unique_ptr<Impl> _impl;
 };
 

@@ -71,6 +71,22 @@ int xfd_sim_kernel_count(void* sim);
 int xfd_sim_kernel_stat(void* sim, int32_t k, char* name, int32_t name_len, int64_t* launches,
                         double* total_ms);
 
+/* Batched stepping of several scenarios on one GPU (xfdtd::SimulationBatch;
+ * no reference counterpart).  The handles stay owned by the caller and must
+ * outlive the batch; results are read from each handle as after run().
+ * info: shared = 1 when the batched launches are used, the i-chunk and
+ * persistent grid size, host seconds and device milliseconds of the last
+ * advance. */
+void* xfd_sim_batch_create(int32_t n, void* const* sims);
+void xfd_sim_batch_destroy(void* batch);
+int xfd_sim_batch_init(void* batch, int64_t steps);
+int xfd_sim_batch_set_graph(void* batch, int32_t on);
+int xfd_sim_batch_tune(void* batch, int32_t reps);
+int xfd_sim_batch_advance(void* batch, int64_t t0, int64_t n);
+int xfd_sim_batch_finish(void* batch);
+int xfd_sim_batch_info(void* batch, int32_t* shared, int32_t* ichunk, int32_t* grid, double* loop_s,
+                       double* device_ms);
+
 #ifdef __cplusplus
 }
 #endif

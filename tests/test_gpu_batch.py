@@ -1,0 +1,165 @@
+"""Batched stepping (SimulationBatch / xfd_batch_*): many uneven cases in one
+set of persistent launches, optionally replayed as a HIP graph.
+
+The batched launches run the same per-cell code as a single context, so a
+case stepped in a batch must equal the same case stepped alone: bitwise in
+FP64 (no FMA contraction) and in FP32 within 1e-6 relative L2 (the compiler
+may contract the two instantiations differently).  A small FP64 batch is also
+compared case by case with the oracle (<= 1e-12), which pins the batched
+result independently of the single-context path.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import RefSim, rel_l2
+from xfdtd_core_amd import native, scenarios
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("EX", "EY", "EZ", "HX", "HY", "HZ")
+STEPS = 120
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _cases(n=5, seed=7, sizes=(24, 32, 40), pml=6):
+    return scenarios.design_sweep(n_cases=n, sizes=sizes, pml=pml, freqs=(1e10, 2e10), seed=seed)
+
+
+def _outputs(sim, nf=2):
+    out = {f: sim.field(f) for f in FIELDS}
+    for f in range(nf):
+        for d in range(6):
+            for w in range(4):
+                out[f"nf{f}{d}{w}"] = sim.nf2ff_current(0, f, d, w)
+    return out
+
+
+def _alone(specs, precision, steps=STEPS):
+    res = []
+    for sp in specs:
+        s = native.Simulation(sp.spec(), precision)
+        s.run(steps)
+        res.append(_outputs(s))
+        s.close()
+    return res
+
+
+def _batched(specs, precision, steps=STEPS, graph=False, tune=0, chunks=None):
+    sims = [native.Simulation(sp.spec(), precision) for sp in specs]
+    b = native.SimulationBatch(sims)
+    b.init(steps)
+    b.set_graph(graph)
+    if tune:
+        b.tune(tune)
+    t = 0
+    for c in chunks or [steps]:
+        b.advance(t, c)
+        t += c
+    b.finish()
+    info = b.info()
+    return sims, b, [_outputs(s) for s in sims], info
+
+
+def _compare(got, want, exact, label):
+    worst = 0.0
+    for i, (g, w) in enumerate(zip(got, want)):
+        for k in w:
+            assert g[k].shape == w[k].shape, (label, i, k)
+            if exact:
+                np.testing.assert_array_equal(g[k], w[k], err_msg=f"{label} case {i} {k}")
+            elif w[k].size:
+                worst = max(worst, rel_l2(g[k], w[k]))
+    say(f"  {label}: worst relative L2 {worst:.3e}")
+    return worst
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batch_matches_single_cases(precision):
+    specs = _cases()
+    say(f"[gpu] batch vs single cases {precision}")
+    want = _alone(specs, precision)
+    sims, b, got, info = _batched(specs, precision)
+    assert info["shared"], info
+    if precision == "f64":
+        _compare(got, want, True, "batch f64")
+    else:
+        assert _compare(got, want, False, "batch f32") <= 1e-6
+
+
+def test_batch_fp64_against_oracle():
+    specs = _cases(n=3, seed=11)
+    say("[gpu] batch fp64 vs oracle")
+    sims, b, got, info = _batched(specs, "f64")
+    for i, sp in enumerate(specs):
+        ref = RefSim(sp.spec())
+        ref.run(STEPS)
+        want = {f: ref.field(f) for f in FIELDS}
+        for f in range(2):
+            for d in range(6):
+                for w in range(4):
+                    want[f"nf{f}{d}{w}"] = ref.nf2ff_current(0, f, d, w)
+        for k, v in want.items():
+            assert got[i][k].shape == v.shape, (i, k)
+            if v.size:
+                assert rel_l2(got[i][k], v) <= 1e-12, (i, k, rel_l2(got[i][k], v))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batch_graph_replays_match_eager(precision):
+    """One captured step replayed many times, with the host reading every
+    case's outputs between replays, equals the eager batch after each chunk."""
+    specs = _cases(n=4, seed=5)
+    say(f"[gpu] batch graph replays {precision}")
+    chunks = [1, 1, 1, 3, 10, 24, 80]
+    sims_g = [native.Simulation(sp.spec(), precision) for sp in specs]
+    sims_e = [native.Simulation(sp.spec(), precision) for sp in specs]
+    bg, be = native.SimulationBatch(sims_g), native.SimulationBatch(sims_e)
+    bg.init(STEPS)
+    be.init(STEPS)
+    bg.set_graph(True)
+    t = 0
+    for c in chunks:
+        bg.advance(t, c)
+        be.advance(t, c)
+        t += c
+        bg.finish()
+        be.finish()
+        for i in range(len(specs)):
+            og = _outputs(sims_g[i])
+            for k, v in _outputs(sims_e[i]).items():
+                np.testing.assert_array_equal(og[k], v, err_msg=f"t={t} case {i} {k}")
+        say(f"  replay chunk ending at step {t}: equal")
+    assert t == STEPS
+
+
+def test_batch_tune_resets_state():
+    """The autotuner steps the sweeps to time them, then clears every case:
+    a tuned batch run equals the single cases."""
+    specs = _cases(n=4, seed=3)
+    say("[gpu] batch tune")
+    want = _alone(specs, "f64")
+    sims, b, got, info = _batched(specs, "f64", tune=2)
+    say(f"  tuned: ichunk={info['ichunk']} grid={info['grid']}")
+    assert info["ichunk"] in (4, 8, 16, 32)
+    _compare(got, want, True, "tuned batch")
+
+
+def test_batch_with_dispersive_case_steps_one_by_one():
+    """A case with an ADE material is outside the batched launches: the batch
+    falls back to stepping its contexts in turn, with the same results."""
+    specs = _cases(n=2, seed=9)
+    disp = scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9, lumped=False,
+                                      kinds=("drude", "lorentz"), nf2ff_distance=8, nf2ff_freqs=[1e10, 2e10])
+    specs = specs + [disp]
+    say("[gpu] batch with dispersive case")
+    want = _alone(specs, "f64", steps=60)
+    sims, b, got, info = _batched(specs, "f64", steps=60)
+    assert not info["shared"]
+    _compare(got, want, True, "fallback batch")

@@ -1,0 +1,167 @@
+// Batched launches: many independent simulations stepped together.
+//
+// Each phase of a time step (Yee H, TFSF/lumped records, CPML, Yee E, ...)
+// is one persistent launch over the work of every case: a table lists the
+// entries (a case's sweep, a CPML slab, a record list, an NF2FF) with the
+// first work-group id each entry owns, and a fixed grid of work-groups walks
+// the global work-group ids with a grid stride, finding its entry by binary
+// search.  Cases of uneven size share the launch, so small cases no longer
+// leave the GPU idle, and one step is a handful of launches that a hipGraph
+// replays.  The time step index lives in device memory (advanced by k_tick at
+// the end of every step), so the captured step is independent of t.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace xfd {
+
+// Entry e owns global work-groups [start[e], start[e+1]).
+__device__ __forceinline__ int findEntry(const int64_t* __restrict__ start, int n, int64_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T>
+struct BatchCase {
+  Grid g;
+  FieldPtrs<T> F;
+  ClassCoef<T> K;
+  int nbx, nby, nbz, ichunk;
+};
+
+template <class T>
+struct BatchSlab {
+  Grid g;
+  FieldPtrs<T> F;
+  SlabArgs<T> s;
+};
+
+template <class T>
+struct BatchRec {
+  FieldPtrs<T> F;
+  const Record* rec;
+  const int64_t* seg;
+  int64_t nseg;
+  const double* pool;
+  T* state;
+};
+
+template <class T>
+struct BatchNf2ff {
+  Grid g;
+  FieldPtrs<T> F;
+  Nf2ffArgs a;
+};
+
+template <class T, bool IsH>
+__global__ __launch_bounds__(256) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
+                                                     const int64_t* __restrict__ start, int n, int64_t total) {
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int e = findEntry(start, n, b);
+    const BatchCase<T> C = cases[e];
+    const int64_t local = b - start[e];
+    const int bx = static_cast<int>(local % C.nbx);
+    const int64_t r = local / C.nbx;
+    const int by = static_cast<int>(r % C.nby);
+    const int bz = static_cast<int>(r / C.nby);
+    if (IsH)
+      sweepH(C.g, C.F, C.K, bx, by, bz, C.ichunk);
+    else
+      sweepE(C.g, C.F, C.K, bx, by, bz, C.ichunk);
+  }
+}
+
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_batch_cpml(const BatchSlab<T>* __restrict__ slabs,
+                                                    const int64_t* __restrict__ start, int n, int64_t total) {
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int e = findEntry(start, n, b);
+    const BatchSlab<T> S = slabs[e];
+    cpmlCell<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_batch_records(const BatchRec<T>* __restrict__ recs,
+                                                       const int64_t* __restrict__ start, int n, int64_t total,
+                                                       const int64_t* __restrict__ tdev) {
+  const int64_t t = *tdev;
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int e = findEntry(start, n, b);
+    const BatchRec<T> R = recs[e];
+    recordSeg(R.F, R.rec, R.seg, R.nseg, R.pool, R.state, (b - start[e]) * blockDim.x + threadIdx.x, t);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_batch_nf2ff(const BatchNf2ff<T>* __restrict__ nfs,
+                                                     const int64_t* __restrict__ start, int n, int64_t total,
+                                                     const int64_t* __restrict__ tdev) {
+  const int64_t t = *tdev;
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int e = findEntry(start, n, b);
+    const BatchNf2ff<T> N = nfs[e];
+    nf2ffCell(N.g, N.F, N.a, (b - start[e]) * blockDim.x + threadIdx.x, t);
+  }
+}
+
+// Recordings with the schedule evaluated on the device: frame
+// (t - first) / interval of a box, when t is on the schedule.
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_at(Grid g, FieldPtrs<T> F, GatherArgs a, double* out, int64_t first,
+                                                   int64_t interval, int64_t count, const int64_t* __restrict__ tdev) {
+  const int64_t t = *tdev;
+  if (t < first || (t - first) % interval != 0) return;
+  const int64_t frame = (t - first) / interval;
+  if (frame >= count) return;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.frame_elems) return;
+  const int n2 = a.e[2] - a.o[2], n1 = a.e[1] - a.o[1];
+  const int w = static_cast<int>(q % n2);
+  const int v = static_cast<int>((q / n2) % n1);
+  const int u = static_cast<int>(q / (static_cast<int64_t>(n2) * n1));
+  const int64_t idx = (static_cast<int64_t>(a.o[0] + u) * g.NY + (a.o[1] + v)) * g.pz + (a.o[2] + w);
+  out[frame * a.frame_elems + q] = static_cast<double>(F.f[a.field][idx]);
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_probes_at(FieldPtrs<T> F, const Probe* __restrict__ p, int n, double* out,
+                                                  int64_t stride, const int64_t* __restrict__ tdev) {
+  const int64_t t = *tdev;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n || t >= stride) return;
+  out[q * stride + t] = static_cast<double>(F.f[p[q].field][p[q].idx]);
+}
+
+// Advances the device-side step counter (one lane writes it).  Templated on
+// the field type only so that each precision's unit holds its own copy.
+template <class T>
+__global__ void k_tick(int64_t* t) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *t = *t + 1;
+}
+
+template <class T>
+struct BatchLaunch {
+  static void sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
+                    int grid);
+  static void cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n, int64_t total,
+                   int grid);
+  static void records(hipStream_t s, const BatchRec<T>* recs, const int64_t* start, int n, int64_t total, int grid,
+                      const int64_t* tdev);
+  static void nf2ff(hipStream_t s, const BatchNf2ff<T>* nfs, const int64_t* start, int n, int64_t total, int grid,
+                    const int64_t* tdev);
+  static void gatherAt(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out,
+                       int64_t first, int64_t interval, int64_t count, const int64_t* tdev);
+  static void probesAt(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
+                       const int64_t* tdev);
+  static void tick(hipStream_t s, int64_t* t);
+};
+
+}  // namespace xfd

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../../include/xfdtd_core_amd.h"
+#include "batch_kernels.hpp"
 #include "kernels.hpp"
 
 namespace {
@@ -174,6 +175,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -885,6 +900,260 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
   if (c.prof) c.drainEvents();
 }
 
+// Zeroes every piece of time-stepping state of a case: fields, CPML psi,
+// inductor currents, ADE state, NF2FF accumulators and recordings.
+void resetState(xfd_ctx& c) {
+  auto zero = [&](DevBuf& b) {
+    if (b.p && b.bytes) XFD_HIP(hipMemsetAsync(b.p, 0, b.bytes, c.stream));
+  };
+  for (auto& f : c.fields) zero(f);
+  for (auto& sl : c.slabs)
+    for (auto& p : sl->psi) zero(p);
+  zero(c.state);
+  for (int q = 0; q < 3; ++q) {
+    zero(c.ade_j[q]);
+    zero(c.ade_jp[q]);
+    zero(c.ade_eprev[q]);
+  }
+  for (auto& n : c.nf2ffs) zero(n->acc);
+  for (auto& gg : c.gathers) zero(gg->out);
+  zero(c.probe_out);
+  XFD_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+// A batch of independent cases stepped by shared persistent launches (see
+// batch_kernels.hpp).  The cases are ordinary contexts (the batch owns them),
+// so fields and recordings are read through xfd_batch_case.
+struct xfd_batch {
+  int dtype = XFD_F64;
+  std::vector<std::unique_ptr<xfd_ctx>> cases;
+  bool shared = true;  // every case class-indexed and free of ADE: batched launches
+  hipStream_t stream = nullptr;
+  DevBuf tdev;
+  int ichunk = 16;
+  int grid = 2048;  // work-groups of each persistent launch
+  int cus = 256;
+  // One table per launch: entries and their first work-group ids.  CPML
+  // slabs of one case overlap at edges and corners, where the reference adds
+  // their corrections in boundary order, so slab o of every case goes into
+  // CPML launch o.
+  struct Table {
+    DevBuf tab, start;
+    int n = 0;
+    int64_t total = 0;
+  };
+  Table sweep, rec_e, rec_h, nf;
+  std::vector<Table> cpml_e, cpml_h;
+  int64_t pool_rows = INT64_MAX;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0;
+  ~xfd_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void dropGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+
+template <class E>
+void fillTable(xfd_batch::Table& t, const std::vector<E>& v, const std::vector<int64_t>& start, int64_t total) {
+  uploadVector(v, t.tab);
+  uploadVector(start, t.start);
+  t.n = static_cast<int>(v.size());
+  t.total = total;
+}
+
+// Collects entries with their first work-group ids.
+template <class E>
+struct Collect {
+  std::vector<E> v;
+  std::vector<int64_t> start;
+  int64_t total = 0;
+  void add(const E& e, int64_t groups) {
+    if (groups <= 0) return;
+    v.push_back(e);
+    start.push_back(total);
+    total += groups;
+  }
+  void to(xfd_batch::Table& t) const { fillTable(t, v, start, total); }
+};
+
+// Builds the entry tables of every batched launch for the current ichunk.
+template <class T>
+void batchTables(xfd_batch& b) {
+  Collect<xfd::BatchCase<T>> sw;
+  std::vector<Collect<xfd::BatchSlab<T>>> ce, ch;
+  Collect<xfd::BatchRec<T>> re, rh;
+  Collect<xfd::BatchNf2ff<T>> nf;
+  for (auto& up : b.cases) {
+    xfd_ctx& c = *up;
+    const auto F = fieldPtrs<T>(c);
+    xfd::BatchCase<T> bc{};
+    bc.g = c.g;
+    bc.F = F;
+    for (int q = 0; q < 6; ++q) {
+      bc.K.id[q] = c.ids[q].as<uint16_t>();
+      bc.K.tab[q] = c.tabs[q].as<T>();
+    }
+    bc.ichunk = b.ichunk;
+    bc.nbx = (c.g.nz + xfd::kLanesK - 1) / xfd::kLanesK;
+    bc.nby = (c.g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ;
+    bc.nbz = (c.g.nx + b.ichunk - 1) / b.ichunk;
+    sw.add(bc, static_cast<int64_t>(bc.nbx) * bc.nby * bc.nbz);
+    const int64_t stride[3] = {c.g.plane, c.g.pz, 1};
+    for (size_t o = 0; o < c.slabs.size(); ++o) {
+      if (ce.size() <= o) {
+        ce.resize(o + 1);
+        ch.resize(o + 1);
+      }
+      for (int is_e = 0; is_e < 2; ++is_e) {
+        xfd::BatchSlab<T> e{c.g, F, slabArgs<T>(*c.slabs[o], is_e != 0)};
+        e.s.stride_c = stride[e.s.axis];
+        const int64_t vol = static_cast<int64_t>(e.s.ext[0]) * e.s.ext[1] * e.s.ext[2];
+        (is_e ? ce : ch)[o].add(e, (vol + 255) / 256);
+      }
+    }
+    if (c.nseg_h)
+      rh.add({F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h, c.pool.as<double>(), c.state.as<T>()},
+             (c.nseg_h + 255) / 256);
+    if (c.nseg_e)
+      re.add({F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e, c.pool.as<double>(), c.state.as<T>()},
+             (c.nseg_e + 255) / 256);
+    for (auto& n : c.nf2ffs) nf.add({c.g, F, n->a}, (n->a.off[6] + 255) / 256);
+    b.pool_rows = std::min(b.pool_rows, c.pool_rows);
+  }
+  sw.to(b.sweep);
+  re.to(b.rec_e);
+  rh.to(b.rec_h);
+  nf.to(b.nf);
+  b.cpml_e.clear();
+  b.cpml_h.clear();
+  b.cpml_e.resize(ce.size());
+  b.cpml_h.resize(ch.size());
+  for (size_t o = 0; o < ce.size(); ++o) {
+    ce[o].to(b.cpml_e[o]);
+    ch[o].to(b.cpml_h[o]);
+  }
+}
+
+// One time step of every case, in Domain::run order, reading the step index
+// from b.tdev and advancing it at the end.
+template <class T>
+void batchEnqueueStep(xfd_batch& b) {
+  using L = xfd::BatchLaunch<T>;
+  hipStream_t st = b.stream;
+  const int64_t* t = b.tdev.as<int64_t>();
+  auto sweep = [&](bool is_h) {
+    L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n, b.sweep.total,
+             b.grid);
+  };
+  auto records = [&](xfd_batch::Table& r) {
+    if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
+  };
+  auto cpml = [&](bool is_e) {
+    for (auto& c : is_e ? b.cpml_e : b.cpml_h)
+      if (c.n) L::cpml(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
+  };
+  sweep(true);
+  records(b.rec_h);
+  cpml(false);
+  sweep(false);
+  records(b.rec_e);
+  cpml(true);
+  for (auto& up : b.cases) {  // recordings, on the device's schedule
+    xfd_ctx& c = *up;
+    const auto F = fieldPtrs<T>(c);
+    if (c.nprobe)
+      L::probesAt(st, F, c.probe_list.as<xfd::Probe>(), c.nprobe, c.probe_out.as<double>(), c.probe_count, t);
+    for (auto& gp : c.gathers)
+      if (gp->probe < 0)
+        L::gatherAt(st, c.g, F, gp->a, gp->out.as<double>(), gp->d.first, gp->d.interval, gp->d.count, t);
+  }
+  if (b.nf.n) L::nf2ff(st, b.nf.tab.as<xfd::BatchNf2ff<T>>(), b.nf.start.as<int64_t>(), b.nf.n, b.nf.total, b.grid, t);
+  L::tick(st, b.tdev.as<int64_t>());
+}
+
+template <class T>
+void batchStep(xfd_batch& b, int64_t t0, int64_t n) {
+  require(t0 >= 0 && n >= 0, "bad step range");
+  if (!b.shared) {  // cases the batched launches do not cover: step them one by one
+    for (auto& c : b.cases) stepTyped<T>(*c, t0, n);
+    return;
+  }
+  require(t0 + n <= b.pool_rows, "step beyond the source tables (run length given at init)");
+  XFD_HIP(hipMemcpy(b.tdev.p, &t0, sizeof(int64_t), hipMemcpyHostToDevice));
+  XFD_HIP(hipEventRecord(b.ev0, b.stream));
+  if (b.use_graph) {
+    if (!b.exec) {
+      XFD_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+      batchEnqueueStep<T>(b);
+      XFD_HIP(hipStreamEndCapture(b.stream, &b.graph));
+      XFD_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    }
+    for (int64_t s = 0; s < n; ++s) XFD_HIP(hipGraphLaunch(b.exec, b.stream));
+  } else {
+    for (int64_t s = 0; s < n; ++s) batchEnqueueStep<T>(b);
+  }
+  XFD_HIP(hipGetLastError());
+  XFD_HIP(hipEventRecord(b.ev1, b.stream));
+  XFD_HIP(hipStreamSynchronize(b.stream));
+  XFD_HIP(hipEventElapsedTime(&b.last_ms, b.ev0, b.ev1));
+}
+
+// Times `reps` pairs of batched Yee sweeps for each candidate (i-chunk,
+// work-groups per launch) and keeps the fastest.
+template <class T>
+void batchTune(xfd_batch& b, int reps) {
+  const int chunks[] = {4, 8, 16, 32};
+  const int mults[] = {4, 8, 16, 32};
+  float best = -1;
+  int best_chunk = b.ichunk, best_grid = b.grid;
+  for (int ch : chunks) {
+    b.ichunk = ch;
+    batchTables<T>(b);
+    for (int m : mults) {
+      b.grid = b.cus * m;
+      using L = xfd::BatchLaunch<T>;
+      auto pair = [&] {
+        for (bool is_h : {true, false})
+          L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
+                   b.sweep.total, b.grid);
+      };
+      pair();  // warm-up
+      XFD_HIP(hipEventRecord(b.ev0, b.stream));
+      for (int r = 0; r < reps; ++r) pair();
+      XFD_HIP(hipEventRecord(b.ev1, b.stream));
+      XFD_HIP(hipStreamSynchronize(b.stream));
+      float ms = 0;
+      XFD_HIP(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+      if (best < 0 || ms < best) {
+        best = ms;
+        best_chunk = ch;
+        best_grid = b.grid;
+      }
+    }
+  }
+  b.ichunk = best_chunk;
+  b.grid = best_grid;
+  batchTables<T>(b);
+  b.dropGraph();
+  for (auto& c : b.cases) resetState(*c);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1022,6 +1291,100 @@ int xfd_kernel_stats(xfd_ctx* c, int32_t k, const char** name, int64_t* launches
     if (name) *name = kKernelNames[k];
     if (launches) *launches = c->launches[k];
     if (total_ms) *total_ms = c->total_ms[k];
+  });
+}
+
+int xfd_reset(xfd_ctx* c) {
+  return guard([&] {
+    require(c != nullptr, "null context");
+    resetState(*c);
+  });
+}
+
+int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
+  return guard([&] {
+    require(n > 0 && descs != nullptr && out != nullptr, "bad argument");
+    *out = nullptr;
+    auto b = std::make_unique<xfd_batch>();
+    b->dtype = descs[0].dtype;
+    for (int i = 0; i < n; ++i) {
+      require(descs[i].dtype == b->dtype, "every case of a batch has the same dtype");
+      xfd_ctx* c = nullptr;
+      if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
+      b->cases.emplace_back(c);
+      b->shared = b->shared && c->classed && c->ade_scheme < 0;
+    }
+    XFD_HIP(hipStreamCreate(&b->stream));
+    XFD_HIP(hipEventCreate(&b->ev0));
+    XFD_HIP(hipEventCreate(&b->ev1));
+    b->tdev.alloc(sizeof(int64_t));
+    int dev = 0;
+    XFD_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    XFD_HIP(hipGetDeviceProperties(&prop, dev));
+    b->cus = std::max(1, prop.multiProcessorCount);
+    b->grid = b->cus * 8;
+    if (b->shared) {
+      if (b->dtype == XFD_F64)
+        batchTables<double>(*b);
+      else
+        batchTables<float>(*b);
+    }
+    *out = b.release();
+  });
+}
+
+void xfd_batch_destroy(xfd_batch* b) { delete b; }
+
+int xfd_batch_case(xfd_batch* b, int32_t i, xfd_ctx** ctx) {
+  return guard([&] {
+    require(b && ctx && i >= 0 && i < static_cast<int32_t>(b->cases.size()), "bad case index");
+    *ctx = b->cases[i].get();
+  });
+}
+
+int xfd_batch_info(xfd_batch* b, int32_t* shared, int32_t* ichunk, int32_t* grid) {
+  return guard([&] {
+    require(b != nullptr, "null batch");
+    if (shared) *shared = b->shared ? 1 : 0;
+    if (ichunk) *ichunk = b->ichunk;
+    if (grid) *grid = b->grid;
+  });
+}
+
+int xfd_batch_set_graph(xfd_batch* b, int32_t on) {
+  return guard([&] {
+    require(b != nullptr, "null batch");
+    b->use_graph = on != 0;
+    b->dropGraph();
+  });
+}
+
+int xfd_batch_tune(xfd_batch* b, int32_t reps) {
+  return guard([&] {
+    require(b != nullptr && reps > 0, "bad argument");
+    if (!b->shared) return;
+    if (b->dtype == XFD_F64)
+      batchTune<double>(*b, reps);
+    else
+      batchTune<float>(*b, reps);
+  });
+}
+
+int xfd_batch_step(xfd_batch* b, int64_t t0, int64_t n) {
+  return guard([&] {
+    require(b != nullptr, "null batch");
+    if (b->dtype == XFD_F64)
+      batchStep<double>(*b, t0, n);
+    else
+      batchStep<float>(*b, t0, n);
+  });
+}
+
+int xfd_batch_last_ms(xfd_batch* b, double* ms) {
+  return guard([&] {
+    require(b && ms, "bad argument");
+    *ms = b->last_ms;
   });
 }
 

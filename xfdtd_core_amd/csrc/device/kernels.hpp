@@ -24,10 +24,10 @@ constexpr int kRowsJ = 4;
 
 // BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41): Hx, Hy, Hz
 // over [0,nx) x [0,ny) x [0,nz).  Coef is CoefPtrs (dense) or ClassCoef.
+// One work-group tile: k-block bx, j-block by, i-chunk bz.
 template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
-  int bx, by, bz;
-  M.coords(bx, by, bz);
+__device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
+                                       int ichunk) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -59,13 +59,19 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C
   }
 }
 
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
+  sweepH(g, F, C, bx, by, bz, ichunk);
+}
+
 // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
 // Ex over [0,nx) x [1,ny) x [1,nz), Ey over [1,nx) x [0,ny) x [1,nz),
 // Ez over [1,nx) x [1,ny) x [0,nz).
 template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
-  int bx, by, bz;
-  M.coords(bx, by, bz);
+__device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
+                                       int ichunk) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -102,6 +108,13 @@ __global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C
   }
 }
 
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
+  sweepE(g, F, C, bx, by, bz, ichunk);
+}
+
 // Single-cell time series (FieldTimeMonitor sampling): one launch per step
 // for all of them; probe p writes out[p * stride + t].
 struct Probe {
@@ -123,9 +136,8 @@ __global__ __launch_bounds__(64) void k_probes(FieldPtrs<T> F, const Probe* __re
 // with q one plane below along the main axis; H: psi = b psi + a (q - p)
 // with q one plane above.  field += c_psi * psi.
 template <class T, bool IsE>
-__global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T> s) {
+__device__ __forceinline__ void cpmlCell(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s, int64_t t) {
   const int64_t vol = static_cast<int64_t>(s.ext[0]) * s.ext[1] * s.ext[2];
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= vol) return;
   const int w = static_cast<int>(t % s.ext[2]);
   const int v = static_cast<int>((t / s.ext[2]) % s.ext[1]);
@@ -152,6 +164,11 @@ __global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T
     T* f = F.f[s.fieldB];
     f[idx] += s.cpsiB[t] * psi;
   }
+}
+
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T> s) {
+  cpmlCell<T, IsE>(g, F, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 enum AdeScheme { kAdeDrude = 0, kAdeDebye = 1, kAdeMLorentz = 2 };
@@ -200,10 +217,8 @@ __device__ __forceinline__ T adeUpdate(const AdeArgs<T>& A, int m, int q, int64_
 // a cell without dispersion its coefficients are zero and the result is the
 // plain update).
 template <class T, class Coef, int S>
-__global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
-                                                      int ichunk) {
-  int bx, by, bz;
-  M.coords(bx, by, bz);
+__device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
+                                          int bx, int by, int bz, int ichunk) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -247,6 +262,14 @@ __global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Co
   }
 }
 
+template <class T, class Coef, int S>
+__global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
+                                                      int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
+  sweepEAde<T, Coef, S>(g, F, C, A, bx, by, bz, ichunk);
+}
+
 // interpolate::interpolateSurfaceCenter (interpolate_scheme.h:18-138): field
 // component f (0..5) at the centre of the face normal to axis c at the
 // padded index idx.
@@ -279,8 +302,8 @@ __device__ __forceinline__ T faceCentre(const Grid& g, const FieldPtrs<T>& F, in
 // NFFFTFrequencyDomain::update, which repeats the whole update once per
 // frequency).  FP64 accumulators.
 template <class T>
-__global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs a, int64_t t) {
-  const int64_t cell = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nf2ffCell(const Grid& g, const FieldPtrs<T>& F, const Nf2ffArgs& a, int64_t cell,
+                                          int64_t t) {
   if (cell >= a.off[6]) return;
   int d = 0;
   while (cell >= a.off[d + 1]) ++d;
@@ -321,15 +344,19 @@ __global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs
   }
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs a, int64_t t) {
+  nf2ffCell(g, F, a, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, t);
+}
+
 // Lumped-element and TFSF corrections: one thread per corrected cell applies
 // that cell's records in the reference's corrector order.  `state` holds the
 // inductor currents (one slot per inductor edge).
 template <class T>
-__global__ __launch_bounds__(256) void k_records(FieldPtrs<T> F, const Record* __restrict__ rec,
-                                                 const int64_t* __restrict__ seg, int64_t nseg,
-                                                 const double* __restrict__ pool, T* __restrict__ state,
-                                                 int64_t t) {
-  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void recordSeg(const FieldPtrs<T>& F, const Record* __restrict__ rec,
+                                          const int64_t* __restrict__ seg, int64_t nseg,
+                                          const double* __restrict__ pool, T* __restrict__ state, int64_t s,
+                                          int64_t t) {
   if (s >= nseg) return;
   const int64_t r0 = seg[s], r1 = seg[s + 1];
   T* f = F.f[rec[r0].field];
@@ -356,6 +383,14 @@ __global__ __launch_bounds__(256) void k_records(FieldPtrs<T> F, const Record* _
     if (sizeof(T) == sizeof(float)) acc = static_cast<double>(static_cast<T>(acc));
   }
   f[target] = static_cast<T>(acc);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_records(FieldPtrs<T> F, const Record* __restrict__ rec,
+                                                 const int64_t* __restrict__ seg, int64_t nseg,
+                                                 const double* __restrict__ pool, T* __restrict__ state,
+                                                 int64_t t) {
+  recordSeg(F, rec, seg, nseg, pool, state, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, t);
 }
 
 // Recording: copy the box [o, e) of one field into a double frame.

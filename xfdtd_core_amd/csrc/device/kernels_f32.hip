@@ -3,3 +3,4 @@
 #include "launch_impl.hpp"
 
 template struct xfd::Launch<float>;
+template struct xfd::BatchLaunch<float>;

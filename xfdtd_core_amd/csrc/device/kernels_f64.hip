@@ -3,3 +3,4 @@
 #include "launch_impl.hpp"
 
 template struct xfd::Launch<double>;
+template struct xfd::BatchLaunch<double>;

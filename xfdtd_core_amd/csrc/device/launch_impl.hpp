@@ -3,8 +3,10 @@
 // floating-point contraction flags.
 #pragma once
 
+#include <algorithm>
 #include <cstdlib>
 
+#include "batch_kernels.hpp"
 #include "kernels.hpp"
 
 namespace xfd {
@@ -129,6 +131,60 @@ void Launch<T>::pack(hipStream_t s, const Grid& g, const T* src, double* dst, in
   const int64_t vol = static_cast<int64_t>(n0) * n1 * n2;
   if (vol <= 0) return;
   hipLaunchKernelGGL(k_pack<T>, dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, src, dst, n0, n1, n2);
+}
+
+// ------------------------------------------------------------- batched --
+template <class T>
+void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n,
+                           int64_t total, int grid) {
+  if (total <= 0) return;
+  dim3 block(kLanesK, kRowsJ);
+  dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
+  if (is_h)
+    hipLaunchKernelGGL((k_batch_sweep<T, true>), g, block, 0, s, cases, start, n, total);
+  else
+    hipLaunchKernelGGL((k_batch_sweep<T, false>), g, block, 0, s, cases, start, n, total);
+}
+template <class T>
+void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,
+                          int64_t total, int grid) {
+  if (total <= 0) return;
+  dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
+  if (is_e)
+    hipLaunchKernelGGL((k_batch_cpml<T, true>), g, dim3(256), 0, s, slabs, start, n, total);
+  else
+    hipLaunchKernelGGL((k_batch_cpml<T, false>), g, dim3(256), 0, s, slabs, start, n, total);
+}
+template <class T>
+void BatchLaunch<T>::records(hipStream_t s, const BatchRec<T>* recs, const int64_t* start, int n, int64_t total,
+                             int grid, const int64_t* tdev) {
+  if (total <= 0) return;
+  dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
+  hipLaunchKernelGGL(k_batch_records<T>, g, dim3(256), 0, s, recs, start, n, total, tdev);
+}
+template <class T>
+void BatchLaunch<T>::nf2ff(hipStream_t s, const BatchNf2ff<T>* nfs, const int64_t* start, int n, int64_t total,
+                           int grid, const int64_t* tdev) {
+  if (total <= 0) return;
+  dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
+  hipLaunchKernelGGL(k_batch_nf2ff<T>, g, dim3(256), 0, s, nfs, start, n, total, tdev);
+}
+template <class T>
+void BatchLaunch<T>::gatherAt(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out,
+                              int64_t first, int64_t interval, int64_t count, const int64_t* tdev) {
+  if (a.frame_elems <= 0 || count <= 0) return;
+  hipLaunchKernelGGL(k_gather_at<T>, dim3(blocks_for(a.frame_elems, 256)), dim3(256), 0, s, g, F, a, out, first,
+                     interval, count, tdev);
+}
+template <class T>
+void BatchLaunch<T>::probesAt(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out,
+                              int64_t stride, const int64_t* tdev) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_probes_at<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, F, p, n, out, stride, tdev);
+}
+template <class T>
+void BatchLaunch<T>::tick(hipStream_t s, int64_t* t) {
+  hipLaunchKernelGGL(k_tick<T>, dim3(1), dim3(64), 0, s, t);
 }
 
 }  // namespace xfd

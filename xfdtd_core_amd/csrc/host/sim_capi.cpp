@@ -447,4 +447,54 @@ int xfd_sim_kernel_stat(void* p, int32_t k, char* name, int32_t name_len, int64_
 
 int xfd_sim_kernel_count(void* p) { return static_cast<int>(H(p)->sim->kernelStats().size()); }
 
+void* xfd_sim_batch_create(int32_t n, void* const* sims) {
+  std::unique_ptr<SimulationBatch> b;
+  if (guard([&] {
+        if (n <= 0 || !sims) throw XFDTDException("empty batch");
+        b = std::make_unique<SimulationBatch>();
+        for (int32_t i = 0; i < n; ++i) b->add(*H(sims[i])->sim);
+      }) != XFD_OK)
+    return nullptr;
+  return b.release();
+}
+
+void xfd_sim_batch_destroy(void* b) { delete static_cast<SimulationBatch*>(b); }
+
+static SimulationBatch* B(void* p) {
+  if (!p) throw XFDTDException("null batch");
+  return static_cast<SimulationBatch*>(p);
+}
+
+int xfd_sim_batch_init(void* p, int64_t steps) {
+  return guard([&] { B(p)->init(static_cast<Index>(steps)); });
+}
+
+int xfd_sim_batch_set_graph(void* p, int32_t on) {
+  return guard([&] { B(p)->setGraph(on != 0); });
+}
+
+int xfd_sim_batch_tune(void* p, int32_t reps) {
+  return guard([&] { B(p)->tune(reps); });
+}
+
+int xfd_sim_batch_advance(void* p, int64_t t0, int64_t n) {
+  return guard([&] { B(p)->advance(static_cast<Index>(t0), static_cast<Index>(n)); });
+}
+
+int xfd_sim_batch_finish(void* p) {
+  return guard([&] { B(p)->finish(); });
+}
+
+int xfd_sim_batch_info(void* p, int32_t* shared, int32_t* ichunk, int32_t* grid, double* loop_s, double* device_ms) {
+  return guard([&] {
+    auto* b = B(p);
+    const auto l = b->layout();
+    if (shared) *shared = l.shared ? 1 : 0;
+    if (ichunk) *ichunk = l.ichunk;
+    if (grid) *grid = l.grid;
+    if (loop_s) *loop_s = b->loopSeconds();
+    if (device_ms) *device_ms = b->deviceMilliseconds();
+  });
+}
+
 }  // extern "C"

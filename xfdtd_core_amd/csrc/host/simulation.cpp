@@ -309,6 +309,15 @@ void extent(bool h, int c, const std::size_t N[3], std::size_t out[3]) {
 const char* kCoefNames[18] = {"cexe", "cexhy", "cexhz", "ceye", "ceyhz", "ceyhx", "ceze", "cezhx", "cezhy",
                               "chxh", "chxey", "chxez", "chyh", "chyez", "chyex", "chzh", "chzex", "chzey"};
 
+// The device batch shared by a SimulationBatch and the Simulations whose
+// contexts are its cases; destroyed with the last of them.
+struct BatchCore {
+  xfd_batch* b = nullptr;
+  ~BatchCore() {
+    if (b) xfd_batch_destroy(b);
+  }
+};
+
 }  // namespace
 
 struct Simulation::Impl {
@@ -369,6 +378,8 @@ struct Simulation::Impl {
   std::vector<GatherPlan> gathers;
 
   xfd_ctx* ctx = nullptr;
+  // Set when ctx is a case of a SimulationBatch: the batch core owns it.
+  std::shared_ptr<BatchCore> batch_core;
   std::shared_ptr<EMF> emf = std::make_shared<EMF>();
   double loop_seconds = 0;
   bool profiling = false;
@@ -376,8 +387,12 @@ struct Simulation::Impl {
   long long dev_bytes = 0;
   bool coef_classed = false;
 
-  ~Impl() {
-    if (ctx) xfd_destroy(ctx);
+  ~Impl() { releaseDevice(); }
+
+  void releaseDevice() {
+    if (ctx && !batch_core) xfd_destroy(ctx);
+    ctx = nullptr;
+    batch_core.reset();
   }
 
   // ---------------------------------------------------------------- grid --
@@ -1199,10 +1214,7 @@ struct Simulation::Impl {
 
   void init(Index n) {
     if (!precomputed || precomputed_steps != n) precompute(n);
-    if (ctx) {
-      xfd_destroy(ctx);
-      ctx = nullptr;
-    }
+    releaseDevice();
     createDevice();
     initialised = true;
   }
@@ -1210,10 +1222,7 @@ struct Simulation::Impl {
   void precompute(Index n) {
     initialised = false;
     precomputed = false;
-    if (ctx) {
-      xfd_destroy(ctx);
-      ctx = nullptr;
-    }
+    releaseDevice();
     slabs.clear();
     lumped.clear();
     tfsf.clear();
@@ -1258,14 +1267,24 @@ struct Simulation::Impl {
     precomputed_steps = n;
   }
 
-  void createDevice() {
+  // Device descriptor of the precomputed scenario.  The arrays it points to
+  // are held here until the next call.
+  std::vector<xfd_cpml_desc> cp;
+  std::vector<xfd_lumped_desc> lp;
+  std::vector<xfd_tfsf_desc> tp;
+  std::vector<xfd_gather_desc> gp;
+
+  xfd_desc makeDesc() {
     xfd_desc dd{};
     dd.nx = static_cast<int32_t>(N[0]);
     dd.ny = static_cast<int32_t>(N[1]);
     dd.nz = static_cast<int32_t>(N[2]);
     dd.dtype = precision == Precision::FP64 ? XFD_F64 : XFD_F32;
     for (int s = 0; s < 18; ++s) dd.coeff[s] = coef[s].v.data();
-    std::vector<xfd_cpml_desc> cp;
+    cp.clear();
+    lp.clear();
+    tp.clear();
+    gp.clear();
     for (auto& s : slabs) {
       s.d.a_e = s.ae.data();
       s.d.b_e = s.be.data();
@@ -1277,7 +1296,6 @@ struct Simulation::Impl {
       s.d.c_hb_psi_ea = s.cpsi[3].data();
       cp.push_back(s.d);
     }
-    std::vector<xfd_lumped_desc> lp;
     for (auto& l : lumped) {
       l.d.coff = l.coff.data();
       l.d.coff_j = l.coff_j.empty() ? nullptr : l.coff_j.data();
@@ -1285,7 +1303,6 @@ struct Simulation::Impl {
       l.d.waveform = l.wave.empty() ? nullptr : l.wave.data();
       lp.push_back(l.d);
     }
-    std::vector<xfd_tfsf_desc> tp;
     for (auto& t : tfsf) {
       t.d.px_int = t.proj[0].data();
       t.d.py_int = t.proj[1].data();
@@ -1297,7 +1314,6 @@ struct Simulation::Impl {
       t.d.h_inc = t.h_inc.data();
       tp.push_back(t.d);
     }
-    std::vector<xfd_gather_desc> gp;
     for (auto& g : gathers) gp.push_back(g.d);
     dd.n_cpml = static_cast<int32_t>(cp.size());
     dd.cpml = cp.data();
@@ -1319,7 +1335,21 @@ struct Simulation::Impl {
     }
     dd.n_nf2ff = static_cast<int32_t>(nf2ff_d.size());
     dd.nf2ff = nf2ff_d.empty() ? nullptr : nf2ff_d.data();
-    if (xfd_create(&dd, &ctx) != XFD_OK) throw XFDTDSimulationException(std::string("xfd_create: ") + xfd_last_error());
+    return dd;
+  }
+
+  void createDevice() {
+    const xfd_desc dd = makeDesc();
+    xfd_ctx* c = nullptr;
+    if (xfd_create(&dd, &c) != XFD_OK) throw XFDTDSimulationException(std::string("xfd_create: ") + xfd_last_error());
+    attach(c, nullptr);
+  }
+
+  // Takes a device context: owned, or a case of the batch `core`.
+  void attach(xfd_ctx* c, std::shared_ptr<BatchCore> core) {
+    ctx = c;
+    batch_core = std::move(core);
+    initialised = true;
     int64_t b = 0;
     xfd_device_bytes(ctx, &b);
     dev_bytes = b;
@@ -1336,6 +1366,7 @@ struct Simulation::Impl {
   // Steps t0 .. t0+n-1 on the device; returns when they are done.
   void advance(Index t0, Index n) {
     if (!initialised || !ctx) throw XFDTDSimulationException("Simulation is not initialised");
+    if (batch_core) throw XFDTDSimulationException("this Simulation is stepped by its SimulationBatch");
     if (t0 + n > steps) throw XFDTDSimulationException("steps beyond the run length given to init()");
     xfd_set_profiling(ctx, profiling ? 1 : 0);
     const auto start = std::chrono::steady_clock::now();
@@ -1480,5 +1511,96 @@ void Simulation::setProfiling(bool on) { _impl->profiling = on; }
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
 bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
+
+// ------------------------------------------------------ SimulationBatch --
+struct SimulationBatch::Impl {
+  std::vector<Simulation*> sims;
+  std::shared_ptr<BatchCore> core;
+  Index steps = 0;
+  bool graph = false;
+  double loop_seconds = 0, device_ms = 0;
+
+  xfd_batch* batch() const {
+    if (!core) throw XFDTDSimulationException("SimulationBatch is not initialised");
+    return core->b;
+  }
+  static void check(int rc, const char* what) {
+    if (rc != XFD_OK) throw XFDTDSimulationException(std::string(what) + ": " + xfd_last_error());
+  }
+};
+
+SimulationBatch::SimulationBatch() : _impl(std::make_unique<Impl>()) {}
+SimulationBatch::~SimulationBatch() = default;
+
+void SimulationBatch::add(Simulation& sim) {
+  _impl->sims.push_back(&sim);
+  _impl->core.reset();
+}
+
+std::size_t SimulationBatch::size() const { return _impl->sims.size(); }
+
+void SimulationBatch::init(Index n) {
+  auto& I = *_impl;
+  if (I.sims.empty()) throw XFDTDSimulationException("empty SimulationBatch");
+  I.core.reset();
+  std::vector<xfd_desc> descs;
+  for (auto* s : I.sims) {
+    auto& S = *s->_impl;
+    if (!S.precomputed || S.precomputed_steps != n) S.precompute(n);
+    S.releaseDevice();
+    descs.push_back(S.makeDesc());
+  }
+  auto core = std::make_shared<BatchCore>();
+  Impl::check(xfd_batch_create(static_cast<int32_t>(descs.size()), descs.data(), &core->b), "xfd_batch_create");
+  for (std::size_t i = 0; i < I.sims.size(); ++i) {
+    xfd_ctx* c = nullptr;
+    Impl::check(xfd_batch_case(core->b, static_cast<int32_t>(i), &c), "xfd_batch_case");
+    I.sims[i]->_impl->attach(c, core);
+  }
+  Impl::check(xfd_batch_set_graph(core->b, I.graph ? 1 : 0), "xfd_batch_set_graph");
+  I.core = std::move(core);
+  I.steps = n;
+}
+
+void SimulationBatch::setGraph(bool on) {
+  _impl->graph = on;
+  if (_impl->core) Impl::check(xfd_batch_set_graph(_impl->batch(), on ? 1 : 0), "xfd_batch_set_graph");
+}
+
+void SimulationBatch::tune(int reps) { Impl::check(xfd_batch_tune(_impl->batch(), reps), "xfd_batch_tune"); }
+
+void SimulationBatch::advance(Index t0, Index n) {
+  auto& I = *_impl;
+  xfd_batch* b = I.batch();
+  if (t0 + n > I.steps) throw XFDTDSimulationException("steps beyond the run length given to init()");
+  const auto start = std::chrono::steady_clock::now();
+  Impl::check(xfd_batch_step(b, static_cast<int64_t>(t0), static_cast<int64_t>(n)), "xfd_batch_step");
+  I.loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  Impl::check(xfd_batch_last_ms(b, &I.device_ms), "xfd_batch_last_ms");
+}
+
+void SimulationBatch::finish() {
+  _impl->batch();
+  for (auto* s : _impl->sims) s->_impl->finish();
+}
+
+void SimulationBatch::run() {
+  advance(0, _impl->steps);
+  finish();
+}
+
+void SimulationBatch::run(Index n) {
+  init(n);
+  run();
+}
+
+double SimulationBatch::loopSeconds() const { return _impl->loop_seconds; }
+double SimulationBatch::deviceMilliseconds() const { return _impl->device_ms; }
+
+SimulationBatch::Layout SimulationBatch::layout() const {
+  int32_t shared = 0, ichunk = 0, grid = 0;
+  Impl::check(xfd_batch_info(_impl->batch(), &shared, &ichunk, &grid), "xfd_batch_info");
+  return {shared != 0, ichunk, grid};
+}
 
 }  // namespace xfdtd

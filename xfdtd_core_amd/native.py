@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes
 from pathlib import Path
-from typing import List, Optional, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -37,6 +37,7 @@ def load_library() -> ctypes.CDLL:
     L = ctypes.CDLL(str(_LIB_PATH))
     vp, i32, i64, dbl, cp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_char_p
     p64 = ctypes.POINTER(ctypes.c_int64)
+    p32 = ctypes.POINTER(ctypes.c_int32)
     sig = {
         "xfd_sim_last_error": (cp, []),
         "xfd_sim_create": (vp, [cp]),
@@ -84,6 +85,24 @@ def load_library() -> ctypes.CDLL:
         "xfd_kernel_stats": (ctypes.c_int, [vp, i32, ctypes.POINTER(cp), p64, ctypes.POINTER(ctypes.c_double)]),
         "xfd_device_bytes": (ctypes.c_int, [vp, p64]),
         "xfd_coef_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), p64]),
+        "xfd_reset": (ctypes.c_int, [vp]),
+        "xfd_batch_create": (ctypes.c_int, [i32, vp, ctypes.POINTER(vp)]),
+        "xfd_batch_destroy": (None, [vp]),
+        "xfd_batch_case": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
+        "xfd_batch_info": (ctypes.c_int, [vp, p32, p32, p32]),
+        "xfd_batch_set_graph": (ctypes.c_int, [vp, i32]),
+        "xfd_batch_tune": (ctypes.c_int, [vp, i32]),
+        "xfd_batch_step": (ctypes.c_int, [vp, i64, i64]),
+        "xfd_batch_last_ms": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "xfd_sim_batch_create": (vp, [i32, ctypes.POINTER(vp)]),
+        "xfd_sim_batch_destroy": (None, [vp]),
+        "xfd_sim_batch_init": (ctypes.c_int, [vp, i64]),
+        "xfd_sim_batch_set_graph": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_batch_tune": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_batch_advance": (ctypes.c_int, [vp, i64, i64]),
+        "xfd_sim_batch_finish": (ctypes.c_int, [vp]),
+        "xfd_sim_batch_info": (ctypes.c_int, [vp, p32, p32, p32, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -386,3 +405,55 @@ class Simulation:
             self._check(self._L.xfd_sim_kernel_stat(self._h, k, name, 64, ctypes.byref(launches), ctypes.byref(ms)))
             out[name.value.decode()] = (launches.value, ms.value)
         return out
+
+
+class SimulationBatch:
+    """Several scenarios stepped together on one GPU (xfdtd::SimulationBatch).
+
+    Every phase of a time step is one launch over the work of all cases; with
+    ``graph=True`` one step is captured as a HIP graph and replayed.  Results
+    are read from each Simulation as after ``run()``.
+    """
+
+    def __init__(self, sims: Sequence["Simulation"]):
+        self._L = load_library()
+        self.sims = list(sims)  # keeps the handles alive
+        arr = (ctypes.c_void_p * len(self.sims))(*[s._h for s in self.sims])
+        self._h = self._L.xfd_sim_batch_create(len(self.sims), arr)
+        if not self._h:
+            raise ValueError(self._L.xfd_sim_last_error().decode())
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise RuntimeError(self._L.xfd_sim_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.xfd_sim_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, steps: int):
+        self._check(self._L.xfd_sim_batch_init(self._h, steps))
+
+    def set_graph(self, on: bool = True):
+        self._check(self._L.xfd_sim_batch_set_graph(self._h, int(on)))
+
+    def tune(self, reps: int = 5):
+        self._check(self._L.xfd_sim_batch_tune(self._h, reps))
+
+    def advance(self, t0: int, n: int):
+        self._check(self._L.xfd_sim_batch_advance(self._h, t0, n))
+
+    def finish(self):
+        self._check(self._L.xfd_sim_batch_finish(self._h))
+
+    def info(self) -> dict:
+        sh, ic, gr = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ls, ms = ctypes.c_double(), ctypes.c_double()
+        self._check(self._L.xfd_sim_batch_info(self._h, ctypes.byref(sh), ctypes.byref(ic), ctypes.byref(gr),
+                                               ctypes.byref(ls), ctypes.byref(ms)))
+        return {"shared": bool(sh.value), "ichunk": ic.value, "grid": gr.value, "loop_seconds": ls.value,
+                "device_ms": ms.value}

@@ -149,3 +149,41 @@ def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab
         freqs = list(nf2ff_freqs) if nf2ff_freqs is not None else [float(f) for f in np.linspace(5e9, 40e9, 8)]
         s.addNF2FF(S.NFFFTFrequencyDomain(nf2ff_distance, nf2ff_distance, nf2ff_distance, freqs))
     return s
+
+
+def design_sweep(n_cases: int = 96, sizes=(48, 64, 96, 128, 160), pml: int = 8, dl: float = 1e-3,
+                 side=(0.2, 0.5), eps_r=(1.0, 8.0), freqs=(2.5e9, 5e9, 7.5e9, 1e10), seed: int = 2030,
+                 cfl: float = 0.9):
+    """Batched design sweep: `n_cases` independent boxes.  Per case and axis
+    the total cell count (CPML included) is drawn from `sizes`; one centred
+    dielectric cuboid spans a fraction U[side] of the interior along each axis
+    (eps_r ~ U[eps_r]); a one-edge Ez current source at the centre carries a
+    Gaussian (tau = 20 dl / 6e8, t0 = 4.5 tau); `pml`-cell CPML on all faces;
+    a plain (one accumulation per step) running DFT at `freqs` on the
+    cuboid's bounding faces.  Returns the list of scenarios."""
+    rng = np.random.default_rng(seed)
+    tau = 20 * dl / 6e8
+    cases = []
+    for c in range(n_cases):
+        n = [int(rng.choice(sizes)) for _ in range(3)]
+        n_in = [a - 2 * pml for a in n]
+        frac = float(rng.uniform(*side))
+        eps = float(rng.uniform(*eps_r))
+        # cuboid sides keep the parity of the interior, so its faces sit on
+        # grid planes `pml + d` cells from each end
+        cube = [max(2, 2 * int(round(frac * a / 2))) for a in n_in]
+        s = S.Simulation(dl, dl, dl, cfl)
+        half = [a * dl / 2 for a in n_in]
+        s.addObject(S.Object("domain", S.Cube(S.Vector(*[-h for h in half]), S.Vector(*[2 * h for h in half])),
+                             S.Material.createAir()))
+        s.addObject(S.Object("cube", S.Cube(S.Vector(*[-q * dl / 2 for q in cube]), S.Vector(*[q * dl for q in cube])),
+                             S.Material(f"m{c}", S.ElectroMagneticProperty(eps, 1.0, 0.0, 0.0))))
+        s.addObject(S.CurrentSource("dipole", S.Cube(S.Vector(0.0, 0.0, 0.0), S.Vector(0.0, 0.0, dl)), "ZP",
+                                    1e20, S.Waveform.gaussian(tau, 4.5 * tau)))
+        for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+            s.addBoundary(S.PML(pml, d))
+        dist = [pml + (a - q) // 2 for a, q in zip(n_in, cube)]
+        s.addNF2FF(S.NFFFTFrequencyDomain(dist[0], dist[1], dist[2], [float(f) for f in freqs],
+                                          reference_repeat=False))
+        cases.append(s)
+    return cases
