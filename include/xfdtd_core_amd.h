@@ -206,6 +206,44 @@ int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
  * Simulation::run without rebuilding the coefficients). */
 int xfd_reset(xfd_ctx* ctx);
 
+/* Reverse-time discrete adjoint (no reference counterpart): the gradient of
+ * an objective J(F) of recorded field values with respect to the
+ * permittivity of the E edges of a design box of cells.
+ *
+ * xfd_forward_checkpointed zeroes the state and runs steps 0 .. nt-1 like
+ * xfd_step (recordings included), saving the whole state (fields, CPML psi)
+ * at steps 0, K, 2K, ... in device memory.  xfd_backward then walks the
+ * segments from last to first: it restores a checkpoint, recomputes the
+ * segment while keeping the design edges' E, and runs the transposed step
+ * (adjoint Yee sweeps and CPML) backwards over it, accumulating in FP64
+ *   dJ/d eps_e = sum_n lambda_e^{n+1} (beta_e E_e^{n+1} + gamma_e E_e^n),
+ * where, for E = s E + a dH_a + b dH_b with s = (2 eps - sigma dt) /
+ * (2 eps + sigma dt) and a, b proportional to 1 / (2 eps + sigma dt),
+ *   beta = -2 / (2 eps + sigma dt),  gamma = 4 sigma dt / (2 eps + sigma dt)^2
+ *   - beta s.
+ * The objective enters as `source[p * nt + n]` = dJ/dF_p after step n (F_p
+ * the field value at point p once step n's corrections are applied).  The
+ * design edges must be updated by the plain Yee sweep only (no CPML, TFSF or
+ * lumped correction on them); ADE materials and inductors are not covered.
+ * After xfd_backward the fields hold a recomputed intermediate state. */
+typedef struct {
+  int32_t origin[3], size[3];  /* design cells [origin, origin + size) */
+  const double* beta;          /* per design edge (xfd_design_edges order) */
+  const double* gamma;
+  int32_t n_points;
+  const int32_t* points;       /* n_points x 4: field (enum xfd_field), i, j, k */
+  int64_t nt;                  /* steps of the checkpointed forward run */
+  const double* source;        /* n_points x nt */
+} xfd_adjoint_desc;
+
+int xfd_forward_checkpointed(xfd_ctx* ctx, int64_t nt, int64_t interval);
+/* Edge counts of the design box per component: Ex edges are the box
+ * [origin, origin + (size0, size1 + 1, size2 + 1)) of Ex indices, Ey and Ez
+ * likewise; the gradient lists Ex, then Ey, then Ez edges, each box k
+ * fastest. */
+int xfd_design_edges(const int32_t origin[3], const int32_t size[3], int64_t counts[3]);
+int xfd_backward(xfd_ctx* ctx, const xfd_adjoint_desc* desc, double* grad);
+
 /* Batched stepping of many independent simulations.  No reference
  * counterpart: the reference runs one Simulation per process (an MPI domain
  * split of one grid, src/simulation/simulation.cpp:401-494).  A batch owns

@@ -551,6 +551,26 @@ class Simulation {
   // 18 dense coefficient arrays (same values; chosen at init).
   bool classedCoefficients() const;
 
+  // ---- Adjoint (no reference counterpart) ----
+  // Per-cell relative permittivity of the design cells [origin, origin+size)
+  // (grid cell indices, k fastest).  It replaces the objects' permittivity
+  // there before the edge averaging; conductivity and permeability stay the
+  // objects'.
+  void setDesignRegion(const std::array<Index, 3>& origin, const std::array<Index, 3>& size,
+                       std::vector<Real> eps_r);
+  // Objective J = sum over the points p of |D_p|^2, where D_p = sum_n dt
+  // exp(-j 2 pi f (n+1) dt) F_p^{n+1} is the running DFT of E component
+  // `field` at its grid points [origin, origin+size) over the whole run.
+  void setDftObjective(EMF::Field field, const std::array<Index, 3>& origin, const std::array<Index, 3>& size,
+                       Real frequency);
+  // init(time_step), then the time loop with the state saved every
+  // `checkpoint_interval` steps; returns J.  Monitors record as in run().
+  Real forward(Index time_step, Index checkpoint_interval);
+  // dJ/d eps_r of every design cell (k fastest), by the reverse-time
+  // discrete adjoint from the checkpoints of the last forward().
+  std::vector<Real> backward();
+  std::vector<std::complex<Real>> objectiveDft() const;
+
   struct Impl;
 
  private:

@@ -71,6 +71,16 @@ int xfd_sim_kernel_count(void* sim);
 int xfd_sim_kernel_stat(void* sim, int32_t k, char* name, int32_t name_len, int64_t* launches,
                         double* total_ms);
 
+/* Adjoint (no reference counterpart; spec lines `design_region` and
+ * `dft_objective`, see xfdtd::Simulation::setDesignRegion/setDftObjective).
+ * forward: init + checkpointed loop, returns J in *value; backward: dJ/d eps_r
+ * per design cell (k fastest) into grad; objective_dft: the objective points'
+ * DFT values as interleaved complex. */
+int xfd_sim_set_design(void* sim, const int64_t* origin3, const int64_t* size3, const double* eps_r);
+int xfd_sim_forward(void* sim, int64_t steps, int64_t interval, double* value);
+int xfd_sim_backward(void* sim, double* grad);
+int xfd_sim_objective_dft(void* sim, double* dst);
+
 /* Batched stepping of several scenarios on one GPU (xfdtd::SimulationBatch;
  * no reference counterpart).  The handles stay owned by the caller and must
  * outlive the batch; results are read from each handle as after run().

@@ -1581,6 +1581,13 @@ void Sim::generateMaterialSpace() {  // simulation.cpp:593-636
     material_array.push_back(o->material);
     ++m_index;
   }
+  if (has_design)
+    for (Index i = 0; i < design_n[0]; ++i)
+      for (Index j = 0; j < design_n[1]; ++j)
+        for (Index k = 0; k < design_n[2]; ++k) {
+          const Real e = design_eps[(i * design_n[1] + j) * design_n[2] + k] * EPSILON_0;
+          for (int c = 0; c < 3; ++c) mp.eps[c](design_o[0] + i, design_o[1] + j, design_o[2] + k) = e;
+        }
   averageMaterial();
   for (auto& o : objects)
     if (o->isPecPlane()) o->correctMaterialSpace(*this, Index(-1));
@@ -1690,6 +1697,7 @@ void Sim::initSteps(Index n) {  // simulation.cpp:373-389
   for (auto& t : tfsfs) t->initTimeDependentVariable(*this);
   for (auto& n : nf2ffs) n->initTimeDependentVariable(*this);  // simulation.cpp:373-389
   for (auto& m : monitors) m->initTimeDependentVariable(*this);
+  if (objective) objective->dft.assign(objective->n[0] * objective->n[1] * objective->n[2], Cplx(0, 0));
 }
 
 namespace {
@@ -1857,6 +1865,16 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
         current = t;
         for (auto& m : monitors) m->update(*this, t);
         for (auto& n : nf2ffs) n->update(*this, t);  // Domain::record (domain.cpp:176-184)
+        if (objective) {
+          auto& ob = *objective;
+          const Real ph = -2 * PI * ob.freq * static_cast<Real>(t + 1) * dt;
+          const Cplx w(dt * std::cos(ph), dt * std::sin(ph));
+          Arr3& f = emf.byId(ob.field);
+          Index p = 0;
+          for (Index i = 0; i < ob.n[0]; ++i)
+            for (Index j = 0; j < ob.n[1]; ++j)
+              for (Index k = 0; k < ob.n[2]; ++k, ++p) ob.dft[p] += w * f(ob.o[0] + i, ob.o[1] + j, ob.o[2] + k);
+        }
       }
       bar.arrive_and_wait();
       bar.arrive_and_wait();

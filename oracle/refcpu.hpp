@@ -465,8 +465,30 @@ struct NF2FF {
   Real power(Sim& s, Index f) const;  // nffft_fd_data_process.cpp:28-42, 162-194
 };
 
+// The DFT objective of the adjoint form (no reference counterpart): over the
+// grid points [o, o+n) of one E component, D_p = sum_t dt exp(-j 2 pi f
+// (t+1) dt) F_p^{t+1}, accumulated after each step's corrections;
+// J = sum_p |D_p|^2.
+struct DftObjective {
+  int field = 2;
+  Index o[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+  Real freq = 0;
+  std::vector<Cplx> dft;
+  Real value() const {
+    Real v = 0;
+    for (auto& d : dft) v += std::norm(d);
+    return v;
+  }
+};
+
 struct Sim {
   Real dx, dy, dz, cfl;
+  // design cells of the adjoint form: relative permittivity per cell (k
+  // fastest), replacing the objects' permittivity before the averaging
+  bool has_design = false;
+  Index design_o[3] = {0, 0, 0}, design_n[3] = {0, 0, 0};
+  std::vector<Real> design_eps;
+  std::unique_ptr<DftObjective> objective;
   int tcfg[3] = {1, 1, 1};
   std::vector<std::unique_ptr<Object>> objects;
   std::vector<std::unique_ptr<PML>> pmls;

@@ -246,6 +246,19 @@ void parseSpec(Sim& s, const std::string& spec) {
       std::string opt;
       if (is >> opt && opt == "plain_dft") n->reference_repeat = false;
       s.nf2ffs.push_back(std::move(n));
+    } else if (d == "design_region") {  // ox oy oz nx ny nz eps_r... (cells, k fastest)
+      for (int a = 0; a < 3; ++a) s.design_o[a] = static_cast<Index>(num(is));
+      for (int a = 0; a < 3; ++a) s.design_n[a] = static_cast<Index>(num(is));
+      s.design_eps.resize(s.design_n[0] * s.design_n[1] * s.design_n[2]);
+      for (auto& e : s.design_eps) e = num(is);
+      s.has_design = true;
+    } else if (d == "dft_objective") {  // FIELD ox oy oz nx ny nz FREQ
+      auto ob = std::make_unique<DftObjective>();
+      ob->field = parseField(word(is));
+      for (int a = 0; a < 3; ++a) ob->o[a] = static_cast<Index>(num(is));
+      for (int a = 0; a < 3; ++a) ob->n[a] = static_cast<Index>(num(is));
+      ob->freq = num(is);
+      s.objective = std::move(ob);
     } else if (d == "movie") {
       auto m = std::make_unique<MovieMonitor>();
       auto f = std::make_unique<FieldMonitor>();
@@ -334,6 +347,16 @@ int ref_init(void* p, int64_t steps) {
 }
 
 double ref_dt(void* p) { return static_cast<Handle*>(p)->sim.dt; }
+// The DFT objective after run(): J into *value, the point DFTs (interleaved
+// complex) into dft when it is not null; returns the point count.
+int ref_objective(void* p, double* value, double* dft) {
+  auto& s = static_cast<Handle*>(p)->sim;
+  if (!s.objective) return -1;
+  if (value) *value = s.objective->value();
+  if (dft) std::memcpy(dft, s.objective->dft.data(), s.objective->dft.size() * sizeof(Cplx));
+  return static_cast<int>(s.objective->dft.size());
+}
+
 int ref_nf2ff_count(void* p) { return static_cast<int>(static_cast<Handle*>(p)->sim.nf2ffs.size()); }
 
 // Surface task of NF2FF `n`, surface d (XN XP YN YP ZN ZP): lo[3], hi[3]

@@ -52,6 +52,7 @@ def lib():
         "ref_nf2ff_potential": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]),
         "ref_nf2ff_power": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(dbl)]),
+        "ref_objective": (ctypes.c_int, [vp, ctypes.POINTER(dbl), vp]),
         "ref_monitor_count": (ctypes.c_int, [vp]),
         "ref_monitor_records": (ctypes.c_int, [vp, ctypes.c_int]),
         "ref_monitor_record": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
@@ -184,6 +185,16 @@ class RefSim:
         v = ctypes.c_double()
         self._check(self.L.ref_nf2ff_power(self.h, n, f, ctypes.byref(v)))
         return v.value
+
+    def objective(self):
+        """(J, per-point DFT values) of the scenario's DFT objective."""
+        n = self.L.ref_objective(self.h, None, None)
+        if n < 0:
+            raise RuntimeError("no objective in the scenario")
+        v = ctypes.c_double()
+        d = np.zeros(n, dtype=np.complex128)
+        self.L.ref_objective(self.h, ctypes.byref(v), d.ctypes.data)
+        return float(v.value), d
 
     def monitor_records(self, m: int) -> List[np.ndarray]:
         out = []

@@ -82,6 +82,8 @@ CASES = {
                                                                 kinds=("drude",)).spec(),
     "dispersive_debye_only": lambda: scenarios.dispersive_slabs(n_total=40, pml=5, slab=6, gap=2, tfsf_offset=7,
                                                                 kinds=("debye", "debye")).spec(),
+    "adjoint_design_region": lambda: scenarios.adjoint_design(n_total=40, pml=6, design=8, patch_gap=4,
+                                                              tfsf_margin=2)[0].spec(),
 }
 
 

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../../include/xfdtd_core_amd.h"
+#include "adjoint_kernels.hpp"
 #include "batch_kernels.hpp"
 #include "kernels.hpp"
 
@@ -264,6 +265,10 @@ struct xfd_ctx {
   int nprobe = 0;
   int64_t probe_count = 0;
 
+  // checkpoints of a forward run for the adjoint (xfd_forward_checkpointed)
+  DevBuf ckpt;
+  int64_t ck_interval = 0, ck_nt = 0, ck_bytes = 0;
+
   bool prof = false;
   int64_t launches[K_COUNT] = {};
   double total_ms[K_COUNT] = {};
@@ -296,6 +301,7 @@ struct xfd_ctx {
     for (auto& n : nf2ffs) b += n->te.bytes + n->th.bytes + n->acc.bytes;
     for (int q = 0; q < 3; ++q) b += ade_j[q].bytes + ade_jp[q].bytes + ade_eprev[q].bytes;
     for (auto& gth : gathers) b += gth->out.bytes;
+    b += ckpt.bytes;
     return b;
   }
 
@@ -810,7 +816,7 @@ xfd::SlabArgs<T> slabArgs(xfd_ctx::Slab& s, bool is_e) {
 }
 
 template <class T>
-void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
+void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   require(t0 >= 0 && n >= 0, "bad step range");
   require(t0 + n <= c.pool_rows, "step beyond the source tables (run length given at init)");
   const auto F = fieldPtrs<T>(c);
@@ -873,6 +879,10 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n) {
                                 c.pool.as<double>(), c.state.as<T>(), t);
       });
     for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
+    if (!record) {
+      XFD_HIP(hipGetLastError());
+      continue;
+    }
     if (c.nprobe && t < c.probe_count)
       c.timed(K_GATHER, [&] {
         xfd::Launch<T>::probes(st, F, c.probe_list.as<xfd::Probe>(), c.nprobe, c.probe_out.as<double>(),
@@ -919,6 +929,169 @@ void resetState(xfd_ctx& c) {
   for (auto& gg : c.gathers) zero(gg->out);
   zero(c.probe_out);
   XFD_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+namespace {
+
+// The arrays that make up the time-stepping state of a context (checkpoints
+// save and restore them).
+std::vector<DevBuf*> stateBuffers(xfd_ctx& c) {
+  std::vector<DevBuf*> v;
+  for (auto& f : c.fields) v.push_back(&f);
+  for (auto& sl : c.slabs)
+    for (auto& p : sl->psi) v.push_back(&p);
+  v.push_back(&c.state);
+  for (int q = 0; q < 3; ++q) {
+    v.push_back(&c.ade_j[q]);
+    v.push_back(&c.ade_jp[q]);
+    v.push_back(&c.ade_eprev[q]);
+  }
+  return v;
+}
+
+void checkpointCopy(xfd_ctx& c, int64_t slot, bool save) {
+  char* base = c.ckpt.as<char>() + slot * c.ck_bytes;
+  for (DevBuf* b : stateBuffers(c)) {
+    if (!b->bytes) continue;
+    if (save)
+      XFD_HIP(hipMemcpyAsync(base, b->p, b->bytes, hipMemcpyDeviceToDevice, c.stream));
+    else
+      XFD_HIP(hipMemcpyAsync(b->p, base, b->bytes, hipMemcpyDeviceToDevice, c.stream));
+    base += b->bytes;
+  }
+}
+
+template <class T>
+void forwardCheckpointed(xfd_ctx& c, int64_t nt, int64_t interval) {
+  require(nt > 0 && interval > 0, "bad run length or checkpoint interval");
+  int64_t bytes = 0;
+  for (DevBuf* b : stateBuffers(c)) bytes += static_cast<int64_t>(b->bytes);
+  const int64_t n_ck = (nt + interval - 1) / interval;
+  c.ckpt.alloc(static_cast<size_t>(bytes * n_ck));
+  c.ck_bytes = bytes;
+  c.ck_interval = interval;
+  c.ck_nt = nt;
+  resetState(c);
+  for (int64_t k = 0; k < n_ck; ++k) {
+    checkpointCopy(c, k, true);
+    const int64_t t0 = k * interval;
+    stepTyped<T>(c, t0, std::min(interval, nt - t0));
+  }
+}
+
+// Design edges of the cells [o, o + n): Ex o x (n0, n1+1, n2+1), Ey
+// (n0+1, n1, n2+1), Ez (n0+1, n1+1, n2).
+xfd::DesignArgs designArgs(const int32_t o[3], const int32_t n[3]) {
+  xfd::DesignArgs d{};
+  int64_t off = 0;
+  for (int q = 0; q < 3; ++q) {
+    for (int a = 0; a < 3; ++a) {
+      d.o[q][a] = o[a];
+      d.e[q][a] = a == q ? n[a] : n[a] + 1;
+    }
+    d.off[q] = off;
+    off += static_cast<int64_t>(d.e[q][0]) * d.e[q][1] * d.e[q][2];
+  }
+  d.off[3] = off;
+  return d;
+}
+
+template <class T>
+void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
+  require(c.ck_nt > 0 && c.ckpt.p, "no checkpointed forward run (xfd_forward_checkpointed)");
+  require(D.nt == c.ck_nt, "objective length differs from the forward run");
+  require(c.ade_scheme < 0, "the adjoint does not cover dispersive (ADE) materials");
+  require(c.state.bytes == 0, "the adjoint does not cover inductors");
+  for (int a = 0; a < 3; ++a) {
+    const int n = a == 0 ? c.g.nx : (a == 1 ? c.g.ny : c.g.nz);
+    require(D.size[a] > 0 && D.origin[a] >= 1 && D.origin[a] + D.size[a] <= n - 1, "design box outside the grid");
+  }
+  require(D.n_points >= 0 && (D.n_points == 0 || (D.points && D.source)), "bad objective points");
+  const xfd::DesignArgs d = designArgs(D.origin, D.size);
+  const int64_t nd = d.off[3];
+  require(D.beta && D.gamma && grad_out, "bad design coefficients");
+  hipStream_t st = c.stream;
+  const auto F = fieldPtrs<T>(c);
+  const auto C = coefPtrs<T>(c);
+  xfd::ClassCoef<T> K{};
+  for (int q = 0; q < 6; ++q) {
+    K.id[q] = c.ids[q].as<uint16_t>();
+    K.tab[q] = c.tabs[q].as<T>();
+  }
+  // adjoint fields, adjoint psi, history, design coefficients, objective
+  DevBuf lam[6];
+  xfd::FieldPtrs<T> L;
+  for (int q = 0; q < 6; ++q) {
+    lam[q].alloc(c.fields[q].bytes);
+    L.f[q] = lam[q].as<T>();
+  }
+  std::vector<std::unique_ptr<DevBuf[]>> mu;
+  std::vector<xfd::SlabArgs<T>> me, mh;
+  const int64_t stride[3] = {c.g.plane, c.g.pz, 1};
+  for (auto& sl : c.slabs) {
+    mu.emplace_back(new DevBuf[4]);
+    for (int q = 0; q < 4; ++q) mu.back()[q].alloc(sl->psi[q].bytes);
+    auto e = slabArgs<T>(*sl, true), h = slabArgs<T>(*sl, false);
+    e.stride_c = stride[e.axis];
+    h.stride_c = stride[h.axis];
+    e.psiA = mu.back()[0].as<T>();
+    e.psiB = mu.back()[1].as<T>();
+    h.psiA = mu.back()[2].as<T>();
+    h.psiB = mu.back()[3].as<T>();
+    me.push_back(e);
+    mh.push_back(h);
+  }
+  const int64_t K_int = c.ck_interval;
+  DevBuf hist, beta, gamma, grad, pts, src;
+  hist.alloc(static_cast<size_t>((K_int + 1) * nd) * sizeof(T));
+  uploadVector(std::vector<double>(D.beta, D.beta + nd), beta);
+  uploadVector(std::vector<double>(D.gamma, D.gamma + nd), gamma);
+  grad.alloc(static_cast<size_t>(nd) * sizeof(double));
+  std::vector<xfd::AdjPoint> hp;
+  for (int p = 0; p < D.n_points; ++p) {
+    const int32_t* q = D.points + 4 * p;
+    require(q[0] >= 0 && q[0] < 6, "bad objective field");
+    const int ext_ok = q[1] >= 0 && q[2] >= 0 && q[3] >= 0 && q[1] <= c.g.nx && q[2] <= c.g.ny && q[3] <= c.g.nz;
+    require(ext_ok, "objective point outside the grid");
+    hp.push_back({padIndex(c.g, q[1], q[2], q[3]), q[0], 0});
+  }
+  uploadVector(hp, pts);
+  if (D.n_points)
+    uploadVector(std::vector<double>(D.source, D.source + static_cast<int64_t>(D.n_points) * D.nt), src);
+  using A = xfd::AdjLaunch<T>;
+  const int64_t n_ck = (c.ck_nt + K_int - 1) / K_int;
+  T* H0 = hist.as<T>();
+  for (int64_t seg = n_ck - 1; seg >= 0; --seg) {
+    const int64_t t0 = seg * K_int, t1 = std::min(t0 + K_int, c.ck_nt);
+    checkpointCopy(c, seg, false);  // recompute the segment, keeping the design edges
+    A::hist(st, c.g, F, d, H0);
+    for (int64_t t = t0; t < t1; ++t) {
+      stepTyped<T>(c, t, 1, false);
+      A::hist(st, c.g, F, d, H0 + (t - t0 + 1) * nd);
+    }
+    for (int64_t t = t1 - 1; t >= t0; --t) {
+      A::source(st, L, pts.as<xfd::AdjPoint>(), D.n_points, src.as<double>(), D.nt, t);
+      A::grad(st, c.g, L, d, H0 + (t - t0) * nd, H0 + (t - t0 + 1) * nd, beta.as<double>(), gamma.as<double>(),
+              grad.as<double>());
+      for (auto& a : me) A::cpmlMu(st, c.g, L, a);
+      if (c.classed)
+        A::adjH(st, c.g, L, K);
+      else
+        A::adjH(st, c.g, L, C);
+      for (auto& a : me) A::cpmlGather(st, true, c.g, L, a);
+      for (auto& a : mh) A::cpmlMu(st, c.g, L, a);
+      if (c.classed)
+        A::adjE(st, c.g, L, K);
+      else
+        A::adjE(st, c.g, L, C);
+      for (auto& a : mh) A::cpmlGather(st, false, c.g, L, a);
+      XFD_HIP(hipGetLastError());
+    }
+  }
+  XFD_HIP(hipMemcpyAsync(grad_out, grad.p, static_cast<size_t>(nd) * sizeof(double), hipMemcpyDeviceToHost, st));
+  XFD_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -1291,6 +1464,34 @@ int xfd_kernel_stats(xfd_ctx* c, int32_t k, const char** name, int64_t* launches
     if (name) *name = kKernelNames[k];
     if (launches) *launches = c->launches[k];
     if (total_ms) *total_ms = c->total_ms[k];
+  });
+}
+
+int xfd_forward_checkpointed(xfd_ctx* c, int64_t nt, int64_t interval) {
+  return guard([&] {
+    require(c != nullptr, "null context");
+    if (c->dtype == XFD_F64)
+      forwardCheckpointed<double>(*c, nt, interval);
+    else
+      forwardCheckpointed<float>(*c, nt, interval);
+  });
+}
+
+int xfd_design_edges(const int32_t origin[3], const int32_t size[3], int64_t counts[3]) {
+  return guard([&] {
+    require(origin && size && counts, "bad argument");
+    const auto d = designArgs(origin, size);
+    for (int q = 0; q < 3; ++q) counts[q] = d.off[q + 1] - d.off[q];
+  });
+}
+
+int xfd_backward(xfd_ctx* c, const xfd_adjoint_desc* d, double* grad) {
+  return guard([&] {
+    require(c != nullptr && d != nullptr, "bad argument");
+    if (c->dtype == XFD_F64)
+      backwardTyped<double>(*c, *d, grad);
+    else
+      backwardTyped<float>(*c, *d, grad);
   });
 }
 

@@ -4,3 +4,4 @@
 
 template struct xfd::Launch<float>;
 template struct xfd::BatchLaunch<float>;
+template struct xfd::AdjLaunch<float>;

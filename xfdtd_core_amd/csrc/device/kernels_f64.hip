@@ -4,3 +4,4 @@
 
 template struct xfd::Launch<double>;
 template struct xfd::BatchLaunch<double>;
+template struct xfd::AdjLaunch<double>;

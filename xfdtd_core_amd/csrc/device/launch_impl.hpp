@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdlib>
 
+#include "adjoint_kernels.hpp"
 #include "batch_kernels.hpp"
 #include "kernels.hpp"
 
@@ -185,6 +186,61 @@ void BatchLaunch<T>::probesAt(hipStream_t s, const FieldPtrs<T>& F, const Probe*
 template <class T>
 void BatchLaunch<T>::tick(hipStream_t s, int64_t* t) {
   hipLaunchKernelGGL(k_tick<T>, dim3(1), dim3(64), 0, s, t);
+}
+
+// ---------------------------------------------------------------- adjoint --
+inline dim3 adjGrid(const Grid& g) {
+  return dim3(blocks_for(g.nz + 1, kLanesK), blocks_for(g.ny + 1, kRowsJ), static_cast<unsigned>(g.nx + 1));
+}
+template <class T>
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C) {
+  hipLaunchKernelGGL((k_adj_h<T, CoefPtrs<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+}
+template <class T>
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C) {
+  hipLaunchKernelGGL((k_adj_h<T, ClassCoef<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+}
+template <class T>
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C) {
+  hipLaunchKernelGGL((k_adj_e<T, CoefPtrs<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+}
+template <class T>
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C) {
+  hipLaunchKernelGGL((k_adj_e<T, ClassCoef<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+}
+template <class T>
+void AdjLaunch<T>::cpmlMu(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a) {
+  const int64_t vol = static_cast<int64_t>(a.ext[0]) * a.ext[1] * a.ext[2];
+  if (vol <= 0) return;
+  hipLaunchKernelGGL(k_adj_cpml_mu<T>, dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, L, a);
+}
+template <class T>
+void AdjLaunch<T>::cpmlGather(hipStream_t s, bool is_e, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a) {
+  int64_t vol = static_cast<int64_t>(a.ext[0]) * a.ext[1] * a.ext[2];
+  if (vol <= 0) return;
+  vol = vol / a.ext[a.axis] * (a.ext[a.axis] + 1);
+  if (is_e)
+    hipLaunchKernelGGL((k_adj_cpml_gather<T, true>), dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, L, a);
+  else
+    hipLaunchKernelGGL((k_adj_cpml_gather<T, false>), dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, L, a);
+}
+template <class T>
+void AdjLaunch<T>::hist(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const DesignArgs& d, T* h) {
+  if (d.off[3] <= 0) return;
+  hipLaunchKernelGGL(k_adj_hist<T>, dim3(blocks_for(d.off[3], 256)), dim3(256), 0, s, g, F, d, h);
+}
+template <class T>
+void AdjLaunch<T>::grad(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const DesignArgs& d, const T* h0,
+                        const T* h1, const double* beta, const double* gamma, double* gr) {
+  if (d.off[3] <= 0) return;
+  hipLaunchKernelGGL(k_adj_grad<T>, dim3(blocks_for(d.off[3], 256)), dim3(256), 0, s, g, L, d, h0, h1, beta, gamma,
+                     gr);
+}
+template <class T>
+void AdjLaunch<T>::source(hipStream_t s, const FieldPtrs<T>& L, const AdjPoint* pts, int n, const double* src,
+                          int64_t nt, int64_t step) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_adj_source<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, L, pts, n, src, nt, step);
 }
 
 }  // namespace xfd

@@ -225,6 +225,19 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
       if (t.more() && t.word() == "plain_dft") n->setReferenceRepeat(false);  // the non-repeating DFT
       h->nf2ffs.push_back(n);
       s.addNF2FF(n);
+    } else if (d == "design_region") {  // ox oy oz nx ny nz eps_r... (cells, k fastest)
+      std::array<Index, 3> o, n;
+      for (auto& x : o) x = static_cast<Index>(t.num());
+      for (auto& x : n) x = static_cast<Index>(t.num());
+      std::vector<Real> e(n[0] * n[1] * n[2]);
+      for (auto& x : e) x = t.num();
+      s.setDesignRegion(o, n, std::move(e));
+    } else if (d == "dft_objective") {  // FIELD ox oy oz nx ny nz FREQ
+      auto f = parseField(t.word());
+      std::array<Index, 3> o, n;
+      for (auto& x : o) x = static_cast<Index>(t.num());
+      for (auto& x : n) x = static_cast<Index>(t.num());
+      s.setDftObjective(f, o, n, t.num());
     } else if (d == "movie") {
       auto f = parseField(t.word());
       auto cb = cube(t);
@@ -446,6 +459,39 @@ int xfd_sim_kernel_stat(void* p, int32_t k, char* name, int32_t name_len, int64_
 }
 
 int xfd_sim_kernel_count(void* p) { return static_cast<int>(H(p)->sim->kernelStats().size()); }
+
+int xfd_sim_set_design(void* p, const int64_t* origin, const int64_t* size, const double* eps_r) {
+  return guard([&] {
+    std::array<Index, 3> o, n;
+    for (int a = 0; a < 3; ++a) {
+      if (origin[a] < 0 || size[a] <= 0) throw XFDTDException("bad design box");
+      o[a] = static_cast<Index>(origin[a]);
+      n[a] = static_cast<Index>(size[a]);
+    }
+    H(p)->sim->setDesignRegion(o, n, std::vector<Real>(eps_r, eps_r + n[0] * n[1] * n[2]));
+  });
+}
+
+int xfd_sim_forward(void* p, int64_t steps, int64_t interval, double* value) {
+  return guard([&] {
+    const Real v = H(p)->sim->forward(static_cast<Index>(steps), static_cast<Index>(interval));
+    if (value) *value = v;
+  });
+}
+
+int xfd_sim_backward(void* p, double* grad) {
+  return guard([&] {
+    auto g = H(p)->sim->backward();
+    std::memcpy(grad, g.data(), g.size() * sizeof(double));
+  });
+}
+
+int xfd_sim_objective_dft(void* p, double* dst) {
+  return guard([&] {
+    auto d = H(p)->sim->objectiveDft();
+    std::memcpy(dst, d.data(), d.size() * sizeof(std::complex<double>));
+  });
+}
 
 void* xfd_sim_batch_create(int32_t n, void* const* sims) {
   std::unique_ptr<SimulationBatch> b;

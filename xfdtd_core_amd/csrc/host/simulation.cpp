@@ -377,6 +377,21 @@ struct Simulation::Impl {
   };
   std::vector<GatherPlan> gathers;
 
+  // Adjoint: design cells with their relative permittivity, the DFT
+  // objective (a box of points of one E component at one frequency) and the
+  // results of the last forward().
+  bool has_design = false;
+  std::size_t design_o[3] = {0, 0, 0}, design_n[3] = {0, 0, 0};
+  std::vector<Real> design_eps;
+  bool has_objective = false;
+  int obj_field = 2;
+  std::size_t obj_o[3] = {0, 0, 0}, obj_n[3] = {0, 0, 0};
+  Real obj_freq = 0;
+  int obj_gather = -1;  // index in gathers
+  std::vector<std::complex<Real>> obj_dft;
+  Real obj_value = 0;
+  bool forward_done = false;
+
   xfd_ctx* ctx = nullptr;
   // Set when ctx is a case of a SimulationBatch: the batch core owns it.
   std::shared_ptr<BatchCore> batch_core;
@@ -537,6 +552,13 @@ struct Simulation::Impl {
             }
       });
     }
+    if (has_design)  // the design cells' permittivity replaces the objects'
+      for (std::size_t i = 0; i < design_n[0]; ++i)
+        for (std::size_t j = 0; j < design_n[1]; ++j)
+          for (std::size_t k = 0; k < design_n[2]; ++k) {
+            const Real e = design_eps[(i * design_n[1] + j) * design_n[2] + k] * kEps0;
+            for (int c = 0; c < 3; ++c) eps[c].at(design_o[0] + i, design_o[1] + j, design_o[2] + k) = e;
+          }
     // Edge averaging (calculation_param.cpp:43-168): eps/sigma_e the mean of
     // the four cells around an edge, mu/sigma_m the harmonic mean of the two
     // cells across a face; evaluated from the pre-averaging values.
@@ -1191,6 +1213,22 @@ struct Simulation::Impl {
       }
       gathers.push_back(gp);
     }
+    obj_gather = -1;
+    if (has_objective) {  // every step, the objective's box
+      GatherPlan gp;
+      gp.monitor = nullptr;
+      gp.time_series = false;
+      gp.d.field = obj_field;
+      for (int a = 0; a < 3; ++a) {
+        gp.d.origin[a] = static_cast<int32_t>(obj_o[a]);
+        gp.d.end[a] = static_cast<int32_t>(obj_o[a] + obj_n[a]);
+      }
+      gp.d.first = 0;
+      gp.d.interval = 1;
+      gp.d.count = static_cast<int64_t>(steps);
+      obj_gather = static_cast<int>(gathers.size());
+      gathers.push_back(gp);
+    }
   }
 
   static int fieldId(EMF::Field f) {
@@ -1388,9 +1426,122 @@ struct Simulation::Impl {
     collectOutputs();
   }
 
+  // ------------------------------------------------------------- adjoint --
+  void checkAdjointSetup() const {
+    if (!has_design) throw XFDTDException("no design region (setDesignRegion)");
+    if (!has_objective) throw XFDTDException("no objective (setDftObjective)");
+  }
+
+  // DFT weight of the value recorded after step n: dt exp(-j 2 pi f (n+1) dt).
+  std::complex<Real> dftWeight(Index n) const {
+    const Real ph = -2 * kPi * obj_freq * static_cast<Real>(n + 1) * dt;
+    return std::complex<Real>(dt * std::cos(ph), dt * std::sin(ph));
+  }
+
+  Real forward(Index n, Index interval) {
+    checkAdjointSetup();
+    if (!precomputed || precomputed_steps != n) precompute(n);
+    releaseDevice();
+    createDevice();
+    forward_done = false;
+    if (xfd_forward_checkpointed(ctx, static_cast<int64_t>(n), static_cast<int64_t>(interval)) != XFD_OK)
+      throw XFDTDSimulationException(std::string("xfd_forward_checkpointed: ") + xfd_last_error());
+    collectOutputs();
+    const std::size_t np = obj_n[0] * obj_n[1] * obj_n[2];
+    std::vector<Real> series(np * n);
+    if (!series.empty() && xfd_read_gather(ctx, obj_gather, series.data()) != XFD_OK)
+      throw XFDTDSimulationException(std::string("xfd_read_gather: ") + xfd_last_error());
+    obj_dft.assign(np, {0, 0});
+    for (Index t = 0; t < n; ++t) {
+      const auto w = dftWeight(t);
+      for (std::size_t p = 0; p < np; ++p) obj_dft[p] += w * series[t * np + p];
+    }
+    obj_value = 0;
+    for (auto& d : obj_dft) obj_value += std::norm(d);
+    forward_done = true;
+    return obj_value;
+  }
+
+  std::vector<Real> backward() {
+    if (!forward_done || !ctx) throw XFDTDSimulationException("backward() needs a forward() first");
+    const Index n = steps;
+    const std::size_t np = obj_n[0] * obj_n[1] * obj_n[2];
+    // dJ/dF_p after step t = 2 Re(conj(D_p) w_t)
+    std::vector<Real> src(np * n);
+    for (Index t = 0; t < n; ++t) {
+      const auto w = dftWeight(t);
+      for (std::size_t p = 0; p < np; ++p) src[p * n + t] = 2 * std::real(std::conj(obj_dft[p]) * w);
+    }
+    std::vector<int32_t> pts;
+    for (std::size_t i = 0; i < obj_n[0]; ++i)
+      for (std::size_t j = 0; j < obj_n[1]; ++j)
+        for (std::size_t k = 0; k < obj_n[2]; ++k) {
+          pts.push_back(obj_field);
+          pts.push_back(static_cast<int32_t>(obj_o[0] + i));
+          pts.push_back(static_cast<int32_t>(obj_o[1] + j));
+          pts.push_back(static_cast<int32_t>(obj_o[2] + k));
+        }
+    xfd_adjoint_desc d{};
+    int32_t eo[3][3], en[3][3];
+    for (int a = 0; a < 3; ++a) {
+      d.origin[a] = static_cast<int32_t>(design_o[a]);
+      d.size[a] = static_cast<int32_t>(design_n[a]);
+    }
+    // the design edges and their coefficient derivatives
+    std::vector<Real> beta, gamma;
+    for (int q = 0; q < 3; ++q) {
+      for (int a = 0; a < 3; ++a) {
+        eo[q][a] = d.origin[a];
+        en[q][a] = a == q ? d.size[a] : d.size[a] + 1;
+      }
+      for (int i = 0; i < en[q][0]; ++i)
+        for (int j = 0; j < en[q][1]; ++j)
+          for (int k = 0; k < en[q][2]; ++k) {
+            const std::size_t ii = eo[q][0] + i, jj = eo[q][1] + j, kk = eo[q][2] + k;
+            const Real e = eps[q].at(ii, jj, kk), sg = se[q].at(ii, jj, kk);
+            const Real den = 2 * e + sg * dt;
+            const Real b = -2 / den, sv = (2 * e - dt * sg) / den;
+            beta.push_back(b);
+            gamma.push_back(4 * sg * dt / (den * den) - b * sv);
+          }
+    }
+    d.beta = beta.data();
+    d.gamma = gamma.data();
+    d.n_points = static_cast<int32_t>(np);
+    d.points = pts.data();
+    d.nt = static_cast<int64_t>(n);
+    d.source = src.data();
+    std::vector<Real> ge(beta.size());
+    if (xfd_backward(ctx, &d, ge.data()) != XFD_OK)
+      throw XFDTDSimulationException(std::string("xfd_backward: ") + xfd_last_error());
+    // eps of an edge is the mean of its four cells (averageEdges), so each
+    // design cell collects eps0/4 of the gradient of its twelve edges
+    std::vector<Real> g(design_n[0] * design_n[1] * design_n[2], 0);
+    std::size_t off = 0;
+    for (int q = 0; q < 3; ++q) {
+      const int A = (q + 1) % 3, B = (q + 2) % 3;
+      for (int i = 0; i < en[q][0]; ++i)
+        for (int j = 0; j < en[q][1]; ++j)
+          for (int k = 0; k < en[q][2]; ++k, ++off) {
+            const int e[3] = {i, j, k};
+            for (int da = -1; da <= 0; ++da)
+              for (int db = -1; db <= 0; ++db) {
+                int c[3] = {e[0], e[1], e[2]};
+                c[A] += da;
+                c[B] += db;
+                if (c[A] < 0 || c[B] < 0 || c[A] >= d.size[A] || c[B] >= d.size[B]) continue;
+                g[(static_cast<std::size_t>(c[0]) * design_n[1] + c[1]) * design_n[2] + c[2]] +=
+                    0.25 * kEps0 * ge[off];
+              }
+          }
+    }
+    return g;
+  }
+
   void collectOutputs() {
     for (std::size_t gi = 0; gi < gathers.size(); ++gi) {
       auto& g = gathers[gi];
+      if (!g.monitor) continue;
       const std::size_t n0 = g.d.end[0] - g.d.origin[0], n1 = g.d.end[1] - g.d.origin[1],
                         n2 = g.d.end[2] - g.d.origin[2];
       const std::size_t fe = n0 * n1 * n2;
@@ -1511,6 +1662,44 @@ void Simulation::setProfiling(bool on) { _impl->profiling = on; }
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
 bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
+
+void Simulation::setDesignRegion(const std::array<Index, 3>& origin, const std::array<Index, 3>& size,
+                                 std::vector<Real> eps_r) {
+  auto& I = *_impl;
+  if (eps_r.size() != size[0] * size[1] * size[2] || eps_r.empty())
+    throw XFDTDException("design permittivity must have one value per design cell");
+  for (Real e : eps_r)
+    if (!(e > 0)) throw XFDTDException("design permittivity must be positive");
+  for (int a = 0; a < 3; ++a) {
+    I.design_o[a] = origin[a];
+    I.design_n[a] = size[a];
+  }
+  I.design_eps = std::move(eps_r);
+  I.has_design = true;
+  I.precomputed = false;
+}
+
+void Simulation::setDftObjective(EMF::Field field, const std::array<Index, 3>& origin,
+                                 const std::array<Index, 3>& size, Real frequency) {
+  auto& I = *_impl;
+  const int f = Impl::fieldId(field);
+  if (f >= 3) throw XFDTDException("the objective records an E component");
+  if (size[0] * size[1] * size[2] == 0) throw XFDTDException("empty objective box");
+  I.obj_field = f;
+  for (int a = 0; a < 3; ++a) {
+    I.obj_o[a] = origin[a];
+    I.obj_n[a] = size[a];
+  }
+  I.obj_freq = frequency;
+  I.has_objective = true;
+  I.precomputed = false;
+}
+
+Real Simulation::forward(Index time_step, Index checkpoint_interval) {
+  return _impl->forward(time_step, checkpoint_interval);
+}
+std::vector<Real> Simulation::backward() { return _impl->backward(); }
+std::vector<std::complex<Real>> Simulation::objectiveDft() const { return _impl->obj_dft; }
 
 // ------------------------------------------------------ SimulationBatch --
 struct SimulationBatch::Impl {

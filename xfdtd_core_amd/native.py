@@ -94,6 +94,13 @@ def load_library() -> ctypes.CDLL:
         "xfd_batch_tune": (ctypes.c_int, [vp, i32]),
         "xfd_batch_step": (ctypes.c_int, [vp, i64, i64]),
         "xfd_batch_last_ms": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "xfd_sim_set_design": (ctypes.c_int, [vp, p64, p64, vp]),
+        "xfd_sim_forward": (ctypes.c_int, [vp, i64, i64, ctypes.POINTER(ctypes.c_double)]),
+        "xfd_sim_backward": (ctypes.c_int, [vp, vp]),
+        "xfd_sim_objective_dft": (ctypes.c_int, [vp, vp]),
+        "xfd_forward_checkpointed": (ctypes.c_int, [vp, i64, i64]),
+        "xfd_design_edges": (ctypes.c_int, [vp, vp, vp]),
+        "xfd_backward": (ctypes.c_int, [vp, vp, vp]),
         "xfd_sim_batch_create": (vp, [i32, ctypes.POINTER(vp)]),
         "xfd_sim_batch_destroy": (None, [vp]),
         "xfd_sim_batch_init": (ctypes.c_int, [vp, i64]),
@@ -299,6 +306,33 @@ class Simulation:
     def finish(self):
         """Copies monitor recordings and final fields back."""
         self._check(self._L.xfd_sim_finish(self._h))
+
+    def set_design(self, origin, eps_r: np.ndarray):
+        """Replaces the design cells' relative permittivity (shape = design
+        box size, cells from `origin`)."""
+        e = np.ascontiguousarray(eps_r, dtype=np.float64)
+        o = (ctypes.c_int64 * 3)(*[int(v) for v in origin])
+        n = (ctypes.c_int64 * 3)(*[int(v) for v in e.shape])
+        self._check(self._L.xfd_sim_set_design(self._h, o, n, e.ctypes.data))
+
+    def forward(self, steps: int, checkpoint_interval: int) -> float:
+        """Adjoint-ready run: init + time loop keeping a state checkpoint every
+        `checkpoint_interval` steps; returns the DFT objective J (needs a
+        design region and an objective in the scenario)."""
+        v = ctypes.c_double()
+        self._check(self._L.xfd_sim_forward(self._h, steps, checkpoint_interval, ctypes.byref(v)))
+        return float(v.value)
+
+    def backward(self, design_shape) -> np.ndarray:
+        """dJ/d eps_r of every design cell, shaped `design_shape`."""
+        g = np.zeros(tuple(int(n) for n in design_shape), dtype=np.float64)
+        self._check(self._L.xfd_sim_backward(self._h, g.ctypes.data))
+        return g
+
+    def objective_dft(self, n_points: int) -> np.ndarray:
+        d = np.zeros(n_points, dtype=np.complex128)
+        self._check(self._L.xfd_sim_objective_dft(self._h, d.ctypes.data))
+        return d
 
     @property
     def dt(self) -> float:

@@ -187,3 +187,33 @@ def design_sweep(n_cases: int = 96, sizes=(48, 64, 96, 128, 160), pml: int = 8, 
                                           reference_repeat=False))
         cases.append(s)
     return cases
+
+
+def adjoint_design(n_total: int = 160, pml: int = 10, dl: float = 1e-3, design: int = 48, f0: float = 1e10,
+                   eps_r=(1.0, 4.0), patch=(4, 4, 1), patch_gap: int = 8, tfsf_margin: int = 3, seed: int = 2031,
+                   cfl: float = 0.9):
+    """Adjoint inverse-design workload: an `n_total`^3 grid (CPML included) at
+    `dl`, an x-directed TFSF plane wave (theta = 90 deg, phi = 0) carrying a
+    cosine-modulated Gaussian at f0 (tau = 1 / (2 f0 0.5), t0 = 4.5 tau),
+    a central `design`^3 block of cells with eps_r ~ U[eps_r] (the
+    differentiable parameter) and the objective sum |DFT_f0(Ez)|^2 over a
+    `patch` of Ez points `patch_gap` cells beyond the block along +x.
+    Returns (scenario, design origin, design eps_r array)."""
+    rng = np.random.default_rng(seed)
+    n_in = n_total - 2 * pml
+    half = n_in * dl / 2
+    s = S.Simulation(dl, dl, dl, cfl)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    tau = 1.0 / (2 * f0 * 0.5)
+    s.addWaveformSource(S.TFSF3D(pml + tfsf_margin, pml + tfsf_margin, pml + tfsf_margin, math.pi / 2, 0.0, 0.0,
+                                 S.Waveform.cosineModulatedGaussian(tau, 4.5 * tau, f0)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(pml, d))
+    o = (n_total - design) // 2
+    origin = (o, o, o)
+    eps = rng.uniform(eps_r[0], eps_r[1], size=(design, design, design))
+    s.setDesignRegion(origin, (design,) * 3, eps)
+    po = (o + design + patch_gap, n_total // 2 - patch[1] // 2, n_total // 2 - patch[2] // 2)
+    s.setDftObjective("EZ", po, patch, f0)
+    return s, origin, eps

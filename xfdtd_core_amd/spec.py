@@ -40,6 +40,8 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import List, Optional
 
+import numpy as np
+
 
 def _n(x: float) -> str:
     return repr(float(x))
@@ -444,7 +446,31 @@ class Simulation:
         self._lines.append(n.line())
         self.nf2ffs.append(n)
 
+    def setDesignRegion(self, origin, size, eps_r):
+        """Per-cell relative permittivity of the design cells [origin,
+        origin+size) (grid cell indices); `eps_r` has shape `size`."""
+        e = np.asarray(eps_r, dtype=np.float64).reshape(tuple(int(n) for n in size))
+        self._design = (tuple(int(o) for o in origin), tuple(int(n) for n in size), e)
+
+    def setDftObjective(self, field, origin, size, frequency):
+        """J = sum over the E-component grid points [origin, origin+size) of
+        |sum_n dt exp(-j 2 pi f (n+1) dt) F^{n+1}|^2."""
+        self._objective = (str(field), tuple(int(o) for o in origin), tuple(int(n) for n in size), float(frequency))
+
     def spec(self) -> str:
+        extra = []
+        design = getattr(self, "_design", None)
+        if design is not None:
+            o, n, e = design
+            extra.append("design_region " + " ".join(str(v) for v in (*o, *n)) + " " +
+                         " ".join(_n(v) for v in e.ravel()))
+        obj = getattr(self, "_objective", None)
+        if obj is not None:
+            f, o, n, fr = obj
+            extra.append(f"dft_objective {f} " + " ".join(str(v) for v in (*o, *n)) + f" {_n(fr)}")
+        return self._spec_base() + "".join(line + "\n" for line in extra)
+
+    def _spec_base(self) -> str:
         head = [f"sim {_n(self.dx)} {_n(self.dy)} {_n(self.dz)} {_n(self.cfl)}",
                 "threads " + " ".join(str(int(t)) for t in self.thread_config)]
         return "\n".join(head + self._lines) + "\n"
