@@ -201,6 +201,13 @@ int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
  * environment variable XFD_COEF_MODE=dense forces the dense layout. */
 int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
 
+/* *fused = 1 when the Yee sweeps apply the CPML corrections in the same
+ * pass (same operations in the same order, fewer bytes moved); 0 when the
+ * slabs run as separate launches: with ADE materials, when a TFSF or lumped
+ * correction lands inside a slab (the reference applies those first), or
+ * when the environment sets XFD_CPML_FUSED=0. */
+int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);
+
 /* Zeroes a context's time-stepping state: fields, CPML auxiliaries, inductor
  * currents, ADE state, NF2FF accumulators and recordings (a fresh
  * Simulation::run without rebuilding the coefficients). */

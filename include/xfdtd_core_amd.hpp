@@ -550,6 +550,8 @@ class Simulation {
   // True when the device holds per-cell coefficient class ids instead of the
   // 18 dense coefficient arrays (same values; chosen at init).
   bool classedCoefficients() const;
+  // True when the Yee sweeps apply the CPML corrections in the same pass.
+  bool fusedCpml() const;
 
   // ---- Adjoint (no reference counterpart) ----
   // Per-cell relative permittivity of the design cells [origin, origin+size)

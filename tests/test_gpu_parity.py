@@ -298,3 +298,30 @@ def test_fp32_error_against_fp64_oracle():
     for g in final:
         assert final[g][1] <= 1e-5, (g, final[g])
     assert max(probes) <= 1e-4
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
+    """The Yee sweeps with the CPML corrections applied in the same pass give
+    the separate-launch results: bitwise in FP64, and in FP32 within the
+    contraction freedom of the compiler.  Cases where a record lands in a
+    slab or ADE is present keep the separate launches."""
+    sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
+    say(f"[gpu] fused CPML vs separate {precision}")
+    fused = native.Simulation(sp, precision)
+    fused.run(200)
+    assert fused.cpml_fused()
+    monkeypatch.setenv("XFD_CPML_FUSED", "0")
+    sep = native.Simulation(sp, precision)
+    sep.run(200)
+    assert not sep.cpml_fused()
+    for f in FIELDS:
+        a, b = fused.field(f), sep.field(f)
+        if precision == "f64":
+            np.testing.assert_array_equal(a, b, err_msg=f)
+        else:
+            assert rel_l2(a, b) < 1e-5, (f, rel_l2(a, b))
+    disp = native.Simulation(scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9,
+                                                        lumped=False).spec(), precision)
+    disp.init(2)
+    assert not disp.cpml_fused()

@@ -33,6 +33,7 @@ struct BatchCase {
   Grid g;
   FieldPtrs<T> F;
   ClassCoef<T> K;
+  CpmlList<T> ph, pe;  // slabs applied inside the sweeps (empty: separate CPML launches)
   int nbx, nby, nbz, ichunk;
 };
 
@@ -72,9 +73,9 @@ __global__ __launch_bounds__(256) void k_batch_sweep(const BatchCase<T>* __restr
     const int by = static_cast<int>(r % C.nby);
     const int bz = static_cast<int>(r / C.nby);
     if (IsH)
-      sweepH(C.g, C.F, C.K, bx, by, bz, C.ichunk);
+      sweepH<T, ClassCoef<T>, true>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
     else
-      sweepE(C.g, C.F, C.K, bx, by, bz, C.ichunk);
+      sweepE<T, ClassCoef<T>, true>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
   }
 }
 
@@ -85,6 +86,17 @@ __global__ __launch_bounds__(256) void k_batch_cpml(const BatchSlab<T>* __restri
     const int e = findEntry(start, n, b);
     const BatchSlab<T> S = slabs[e];
     cpmlCell<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
+  }
+}
+
+// The far-plane points of every slab entry (the fused sweeps' remainder).
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_batch_cpml_faces(const BatchSlab<T>* __restrict__ slabs,
+                                                          const int64_t* __restrict__ start, int n, int64_t total) {
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int e = findEntry(start, n, b);
+    const BatchSlab<T> S = slabs[e];
+    cpmlFacePoint<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
   }
 }
 
@@ -153,6 +165,8 @@ struct BatchLaunch {
                     int grid);
   static void cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n, int64_t total,
                    int grid);
+  static void cpmlFaces(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,
+                        int64_t total, int grid);
   static void records(hipStream_t s, const BatchRec<T>* recs, const int64_t* start, int n, int64_t total, int grid,
                       const int64_t* tdev);
   static void nf2ff(hipStream_t s, const BatchNf2ff<T>* nfs, const int64_t* start, int n, int64_t total, int grid,

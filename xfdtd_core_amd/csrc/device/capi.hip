@@ -265,6 +265,11 @@ struct xfd_ctx {
   int nprobe = 0;
   int64_t probe_count = 0;
 
+  // CPML applied inside the Yee sweeps (see cpmlFused): the slab arguments
+  // of each half-step as device arrays
+  bool fused = false;
+  DevBuf cpml_list_e, cpml_list_h;
+
   // checkpoints of a forward run for the adjoint (xfd_forward_checkpointed)
   DevBuf ckpt;
   int64_t ck_interval = 0, ck_nt = 0, ck_bytes = 0;
@@ -479,6 +484,27 @@ void segment(std::vector<xfd::Record>& rec, std::vector<int64_t>& seg) {
   seg.push_back(static_cast<int64_t>(rec.size()));
 }
 
+// True when a record (TFSF or lumped correction) writes a component that a
+// CPML slab of the same half-step also corrects at that point: the
+// reference applies records before CPML there, so such a context keeps the
+// separate CPML launches.
+bool recordsMeetSlabs(const xfd_ctx& c, const std::vector<xfd::Record>& recs, bool is_e) {
+  for (const auto& r : recs) {
+    const int64_t i = r.target / c.g.plane, rem = r.target % c.g.plane;
+    const int64_t p[3] = {i, rem / c.g.pz, rem % c.g.pz};
+    for (const auto& sl : c.slabs) {
+      const int* lo = is_e ? sl->lo_e : sl->lo_h;
+      bool in = true;
+      for (int q = 0; q < 3; ++q) in = in && p[q] >= lo[q] && p[q] < lo[q] + sl->ext[q];
+      if (in && r.field % 3 != sl->axis) return true;
+    }
+  }
+  return false;
+}
+
+template <class T>
+xfd::SlabArgs<T> slabArgs(xfd_ctx::Slab& s, bool is_e);
+
 template <class T>
 void createTyped(xfd_ctx& c, const xfd_desc& d) {
   const xfd::Grid& g = c.g;
@@ -669,6 +695,7 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   uploadVector(sh, c.seg_h);
   c.nseg_e = re.empty() ? 0 : static_cast<int64_t>(se.size()) - 1;
   c.nseg_h = rh.empty() ? 0 : static_cast<int64_t>(sh.size()) - 1;
+  const bool records_meet_slabs = recordsMeetSlabs(c, re, true) || recordsMeetSlabs(c, rh, false);
 
   if (d.ade) {
     const auto& A = *d.ade;
@@ -767,6 +794,23 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     uploadVector(pl, c.probe_list);
     c.probe_out.alloc(static_cast<size_t>(c.nprobe * c.probe_count) * sizeof(double));
   }
+  {  // CPML inside the sweeps unless XFD_CPML_FUSED=0, ADE or records in a slab
+    const char* env = std::getenv("XFD_CPML_FUSED");
+    const bool off = env && env[0] == '0';
+    c.fused = !off && !c.slabs.empty() && c.ade_scheme < 0 && !records_meet_slabs;
+    if (c.fused) {
+      const int64_t stride[3] = {g.plane, g.pz, 1};
+      std::vector<xfd::SlabArgs<T>> le, lh;
+      for (auto& sl : c.slabs) {
+        le.push_back(slabArgs<T>(*sl, true));
+        lh.push_back(slabArgs<T>(*sl, false));
+        le.back().stride_c = stride[le.back().axis];
+        lh.back().stride_c = stride[lh.back().axis];
+      }
+      uploadVector(le, c.cpml_list_e);
+      uploadVector(lh, c.cpml_list_h);
+    }
+  }
   XFD_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -848,19 +892,27 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   for (auto& a : se) a.stride_c = stride[a.axis];
   for (auto& a : sh) a.stride_c = stride[a.axis];
   hipStream_t st = c.stream;
+  xfd::CpmlList<T> PE{}, PH{};
+  if (c.fused) {
+    PE = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
+    PH = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
+  }
   for (int64_t t = t0; t < t0 + n; ++t) {
     c.timed(K_UPDATE_H, [&] {
       if (c.classed)
-        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk);
+        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH);
       else
-        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk);
+        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk, PH);
     });
     if (c.nseg_h)
       c.timed(K_REC_H, [&] {
         xfd::Launch<T>::records(st, F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h,
                                 c.pool.as<double>(), c.state.as<T>(), t);
       });
-    for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, false); });
+    if (c.fused)
+      for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml_faces(st, c.g, F, a, false); });
+    else
+      for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, false); });
     c.timed(K_UPDATE_E, [&] {
       if (c.ade_scheme >= 0) {
         if (c.classed)
@@ -868,9 +920,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         else
           xfd::Launch<T>::update_e_ade(st, c.g, F, C, A, c.ade_scheme, c.ichunk);
       } else if (c.classed) {
-        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk);
+        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk, PE);
       } else {
-        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk);
+        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk, PE);
       }
     });
     if (c.nseg_e)
@@ -878,7 +930,10 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         xfd::Launch<T>::records(st, F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e,
                                 c.pool.as<double>(), c.state.as<T>(), t);
       });
-    for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
+    if (c.fused)
+      for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml_faces(st, c.g, F, a, true); });
+    else
+      for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
     if (!record) {
       XFD_HIP(hipGetLastError());
       continue;
@@ -1103,6 +1158,7 @@ struct xfd_batch {
   int dtype = XFD_F64;
   std::vector<std::unique_ptr<xfd_ctx>> cases;
   bool shared = true;  // every case class-indexed and free of ADE: batched launches
+  bool fused = true;   // every case applies its CPML inside the sweeps
   hipStream_t stream = nullptr;
   DevBuf tdev;
   int ichunk = 16;
@@ -1183,6 +1239,10 @@ void batchTables(xfd_batch& b) {
       bc.K.tab[q] = c.tabs[q].as<T>();
     }
     bc.ichunk = b.ichunk;
+    if (b.fused && c.fused) {
+      bc.ph = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
+      bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
+    }
     bc.nbx = (c.g.nz + xfd::kLanesK - 1) / xfd::kLanesK;
     bc.nby = (c.g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ;
     bc.nbz = (c.g.nx + b.ichunk - 1) / b.ichunk;
@@ -1196,7 +1256,11 @@ void batchTables(xfd_batch& b) {
       for (int is_e = 0; is_e < 2; ++is_e) {
         xfd::BatchSlab<T> e{c.g, F, slabArgs<T>(*c.slabs[o], is_e != 0)};
         e.s.stride_c = stride[e.s.axis];
-        const int64_t vol = static_cast<int64_t>(e.s.ext[0]) * e.s.ext[1] * e.s.ext[2];
+        int64_t vol = static_cast<int64_t>(e.s.ext[0]) * e.s.ext[1] * e.s.ext[2];
+        if (b.fused) {  // only the far-plane remainder
+          const auto fp = xfd::facePieces(c.g, e.s);
+          vol = fp.nX + fp.nY + fp.nZ;
+        }
         (is_e ? ce : ch)[o].add(e, (vol + 255) / 256);
       }
     }
@@ -1238,8 +1302,13 @@ void batchEnqueueStep(xfd_batch& b) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
   };
   auto cpml = [&](bool is_e) {
-    for (auto& c : is_e ? b.cpml_e : b.cpml_h)
-      if (c.n) L::cpml(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
+    for (auto& c : is_e ? b.cpml_e : b.cpml_h) {
+      if (!c.n) continue;
+      if (b.fused)
+        L::cpmlFaces(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
+      else
+        L::cpml(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
+    }
   };
   sweep(true);
   records(b.rec_h);
@@ -1495,6 +1564,13 @@ int xfd_backward(xfd_ctx* c, const xfd_adjoint_desc* d, double* grad) {
   });
 }
 
+int xfd_cpml_fused(xfd_ctx* c, int32_t* fused) {
+  return guard([&] {
+    require(c && fused, "bad argument");
+    *fused = c->fused ? 1 : 0;
+  });
+}
+
 int xfd_reset(xfd_ctx* c) {
   return guard([&] {
     require(c != nullptr, "null context");
@@ -1514,6 +1590,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
       b->cases.emplace_back(c);
       b->shared = b->shared && c->classed && c->ade_scheme < 0;
+      b->fused = b->fused && (c->fused || c->slabs.empty());
     }
     XFD_HIP(hipStreamCreate(&b->stream));
     XFD_HIP(hipEventCreate(&b->ev0));

@@ -22,12 +22,62 @@ namespace xfd {
 constexpr int kLanesK = 64;
 constexpr int kRowsJ = 4;
 
+// The CPML corrections of one point, fused into a Yee sweep: for every slab
+// of P containing the point (in boundary order), for its components A =
+// (c+1)%3 and B = (c+2)%3 of this half-step,
+//   psi = b psi + a d,  v += c_psi psi
+// with d the dual difference along the slab axis c, which is the Yee curl
+// difference the sweep already holds: dA[c] for component A, dB[c] for B.
+// Same operations in the same order as k_cpml after the sweep, so results
+// are unchanged; the field and its dual are not read again.
+template <class T>
+__device__ __forceinline__ void cpmlFused(const CpmlList<T>& P, int i, int j, int k, T& v0, T& v1, T& v2,
+                                          const T dA0, const T dA1, const T dA2, const T dB0, const T dB1,
+                                          const T dB2) {
+  const int p[3] = {i, j, k};
+  for (int n = 0; n < P.n; ++n) {
+    const SlabArgs<T>& S = P.s[n];
+    const int u = i - S.lo[0], v = j - S.lo[1], w = k - S.lo[2];
+    if (u < 0 || v < 0 || w < 0 || u >= S.ext[0] || v >= S.ext[1] || w >= S.ext[2]) continue;
+    const int c = S.axis, a = c == 2 ? 0 : c + 1, b = c == 0 ? 2 : c - 1;
+    const int64_t t = (static_cast<int64_t>(u) * S.ext[1] + v) * S.ext[2] + w;
+    const int layer = p[c] - S.start;
+    const T ca = S.ca[layer], cb = S.cb[layer];
+    if (p[a] < S.extA[0] && p[b] < S.extA[1]) {
+      const T d = c == 0 ? dA0 : (c == 1 ? dA1 : dA2);
+      const T psi = cb * S.psiA[t] + ca * d;
+      S.psiA[t] = psi;
+      const T add = S.cpsiA[t] * psi;
+      if (a == 0)
+        v0 = v0 + add;
+      else if (a == 1)
+        v1 = v1 + add;
+      else
+        v2 = v2 + add;
+    }
+    if (p[a] < S.extB[0] && p[b] < S.extB[1]) {
+      const T d = c == 0 ? dB0 : (c == 1 ? dB1 : dB2);
+      const T psi = cb * S.psiB[t] + ca * d;
+      S.psiB[t] = psi;
+      const T add = S.cpsiB[t] * psi;
+      if (b == 0)
+        v0 = v0 + add;
+      else if (b == 1)
+        v1 = v1 + add;
+      else
+        v2 = v2 + add;
+    }
+  }
+}
+
 // BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41): Hx, Hy, Hz
 // over [0,nx) x [0,ny) x [0,nz).  Coef is CoefPtrs (dense) or ClassCoef.
-// One work-group tile: k-block bx, j-block by, i-chunk bz.
-template <class T, class Coef>
+// One work-group tile: k-block bx, j-block by, i-chunk bz.  Fused: the CPML
+// H corrections of the visited points in the same pass (the H dual
+// differences along x, y, z are dual[idx + s_c] - dual[idx]).
+template <class T, class Coef, bool Fused = false>
 __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
-                                       int ichunk) {
+                                       int ichunk, const CpmlList<T>& P = CpmlList<T>{}) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -49,29 +99,39 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
     const T ez_j = ez[idx + pz], ez_n = ez[idx + pl];
     T s, a, b;
     C.get(3, idx, s, a, b);
-    hx[idx] = s * hx[idx] + a * (ey_k - ey_i) + b * (ez_j - ez_i);
+    T vx = s * hx[idx] + a * (ey_k - ey_i) + b * (ez_j - ez_i);
     C.get(4, idx, s, a, b);
-    hy[idx] = s * hy[idx] + a * (ez_n - ez_i) + b * (ex_k - ex0);
+    T vy = s * hy[idx] + a * (ez_n - ez_i) + b * (ex_k - ex0);
     C.get(5, idx, s, a, b);
-    hz[idx] = s * hz[idx] + a * (ex_j - ex0) + b * (ey_n - ey_i);
+    T vz = s * hz[idx] + a * (ex_j - ex0) + b * (ey_n - ey_i);
+    if (Fused)  // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
+      cpmlFused(P, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
+                ex_k - ex0);
+    hx[idx] = vx;
+    hy[idx] = vy;
+    hz[idx] = vz;
     ey_i = ey_n;
     ez_i = ez_n;
   }
 }
 
-template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
+template <class T, class Coef, bool Fused>
+__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
+                                                  CpmlList<T> P) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepH(g, F, C, bx, by, bz, ichunk);
+  sweepH<T, Coef, Fused>(g, F, C, bx, by, bz, ichunk, P);
 }
 
 // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
 // Ex over [0,nx) x [1,ny) x [1,nz), Ey over [1,nx) x [0,ny) x [1,nz),
-// Ez over [1,nx) x [1,ny) x [0,nz).
-template <class T, class Coef>
+// Ez over [1,nx) x [1,ny) x [0,nz).  Fused: the CPML E corrections of the
+// visited points (i < nx, j < ny, k < nz) in the same pass, including the
+// face points a Yee range leaves out; the dual differences along x, y, z are
+// dual[idx] - dual[idx - s_c].
+template <class T, class Coef, bool Fused = false>
 __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
-                                       int ichunk) {
+                                       int ichunk, const CpmlList<T>& P = CpmlList<T>{}) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -91,28 +151,57 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   for (int i = i0; i < i1; ++i, idx += pl) {
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
     T s, a, b;
-    if (jk) {
-      C.get(0, idx, s, a, b);
-      ex[idx] = s * ex[idx] + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
-    }
-    if (i >= 1 && k >= 1) {
-      C.get(1, idx, s, a, b);
-      ey[idx] = s * ey[idx] + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
-    }
-    if (i >= 1 && j >= 1) {
-      C.get(2, idx, s, a, b);
-      ez[idx] = s * ez[idx] + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
+    if (!Fused) {
+      if (jk) {
+        C.get(0, idx, s, a, b);
+        ex[idx] = s * ex[idx] + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
+      }
+      if (i >= 1 && k >= 1) {
+        C.get(1, idx, s, a, b);
+        ey[idx] = s * ey[idx] + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
+      }
+      if (i >= 1 && j >= 1) {
+        C.get(2, idx, s, a, b);
+        ez[idx] = s * ez[idx] + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
+      }
+    } else {
+      // curl differences: x uses the carried plane, y and z the j-1 / k-1
+      // neighbours where they exist (a slab's axis never starts at 0)
+      const T dzHy = k >= 1 ? hy0 - hy[idx - 1] : T(0);
+      const T dzHx = k >= 1 ? hx0 - hx[idx - 1] : T(0);
+      const T dyHz = j >= 1 ? hz0 - hz[idx - pz] : T(0);
+      const T dyHx = j >= 1 ? hx0 - hx[idx - pz] : T(0);
+      const T dxHz = hz0 - hz_p, dxHy = hy0 - hy_p;
+      T vx = ex[idx], vy = ey[idx], vz = ez[idx];
+      if (jk) {
+        C.get(0, idx, s, a, b);
+        vx = s * vx + a * dzHy + b * dyHz;
+      }
+      if (i >= 1 && k >= 1) {
+        C.get(1, idx, s, a, b);
+        vy = s * vy + a * dxHz + b * dzHx;
+      }
+      if (i >= 1 && j >= 1) {
+        C.get(2, idx, s, a, b);
+        vz = s * vz + a * dyHx + b * dxHy;
+      }
+      // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
+      cpmlFused(P, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+      ex[idx] = vx;
+      ey[idx] = vy;
+      ez[idx] = vz;
     }
     hz_p = hz0;
     hy_p = hy0;
   }
 }
 
-template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk) {
+template <class T, class Coef, bool Fused>
+__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
+                                                  CpmlList<T> P) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepE(g, F, C, bx, by, bz, ichunk);
+  sweepE<T, Coef, Fused>(g, F, C, bx, by, bz, ichunk, P);
 }
 
 // Single-cell time series (FieldTimeMonitor sampling): one launch per step
@@ -136,13 +225,8 @@ __global__ __launch_bounds__(64) void k_probes(FieldPtrs<T> F, const Probe* __re
 // with q one plane below along the main axis; H: psi = b psi + a (q - p)
 // with q one plane above.  field += c_psi * psi.
 template <class T, bool IsE>
-__device__ __forceinline__ void cpmlCell(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s, int64_t t) {
-  const int64_t vol = static_cast<int64_t>(s.ext[0]) * s.ext[1] * s.ext[2];
-  if (t >= vol) return;
-  const int w = static_cast<int>(t % s.ext[2]);
-  const int v = static_cast<int>((t / s.ext[2]) % s.ext[1]);
-  const int u = static_cast<int>(t / (static_cast<int64_t>(s.ext[2]) * s.ext[1]));
-  const int p[3] = {s.lo[0] + u, s.lo[1] + v, s.lo[2] + w};
+__device__ __forceinline__ void cpmlPoint(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s, const int p[3],
+                                          int64_t t) {
   const int c = s.axis, a = (c + 1) % 3, b = (c + 2) % 3;
   const int layer = p[c] - s.start;
   const T ca = s.ca[layer], cb = s.cb[layer];
@@ -167,8 +251,70 @@ __device__ __forceinline__ void cpmlCell(const Grid& g, const FieldPtrs<T>& F, c
 }
 
 template <class T, bool IsE>
+__device__ __forceinline__ void cpmlCell(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s, int64_t t) {
+  const int64_t vol = static_cast<int64_t>(s.ext[0]) * s.ext[1] * s.ext[2];
+  if (t >= vol) return;
+  const int w = static_cast<int>(t % s.ext[2]);
+  const int v = static_cast<int>((t / s.ext[2]) % s.ext[1]);
+  const int u = static_cast<int>(t / (static_cast<int64_t>(s.ext[2]) * s.ext[1]));
+  const int p[3] = {s.lo[0] + u, s.lo[1] + v, s.lo[2] + w};
+  cpmlPoint<T, IsE>(g, F, s, p, t);
+}
+
+template <class T, bool IsE>
 __global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T> s) {
   cpmlCell<T, IsE>(g, F, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// The points of a slab box that the fused sweeps do not visit: the far grid
+// planes i = nx, j = ny, k = nz.  Point q of the union of the three plane
+// pieces (X: i = nx; Y: j = ny, i < nx; Z: k = nz, i < nx, j < ny), or false.
+struct FacePieces {
+  int n0, n1;        // extents of the slab box clipped below nx / ny
+  int64_t nX, nY, nZ;
+};
+template <class T>
+__device__ __host__ __forceinline__ FacePieces facePieces(const Grid& g, const SlabArgs<T>& s) {
+  FacePieces f;
+  const int hi0 = s.lo[0] + s.ext[0], hi1 = s.lo[1] + s.ext[1], hi2 = s.lo[2] + s.ext[2];
+  f.n0 = max(0, min(hi0, g.nx) - s.lo[0]);
+  f.n1 = max(0, min(hi1, g.ny) - s.lo[1]);
+  f.nX = (s.lo[0] <= g.nx && g.nx < hi0) ? static_cast<int64_t>(s.ext[1]) * s.ext[2] : 0;
+  f.nY = (s.lo[1] <= g.ny && g.ny < hi1) ? static_cast<int64_t>(f.n0) * s.ext[2] : 0;
+  f.nZ = (s.lo[2] <= g.nz && g.nz < hi2) ? static_cast<int64_t>(f.n0) * f.n1 : 0;
+  return f;
+}
+
+template <class T, bool IsE>
+__device__ __forceinline__ void cpmlFacePoint(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s,
+                                              int64_t q) {
+  const FacePieces f = facePieces(g, s);
+  int p[3];
+  if (q < f.nX) {
+    p[0] = g.nx;
+    p[1] = s.lo[1] + static_cast<int>(q / s.ext[2]);
+    p[2] = s.lo[2] + static_cast<int>(q % s.ext[2]);
+  } else if (q < f.nX + f.nY) {
+    q -= f.nX;
+    p[0] = s.lo[0] + static_cast<int>(q / s.ext[2]);
+    p[1] = g.ny;
+    p[2] = s.lo[2] + static_cast<int>(q % s.ext[2]);
+  } else if (q < f.nX + f.nY + f.nZ) {
+    q -= f.nX + f.nY;
+    p[0] = s.lo[0] + static_cast<int>(q / f.n1);
+    p[1] = s.lo[1] + static_cast<int>(q % f.n1);
+    p[2] = g.nz;
+  } else {
+    return;
+  }
+  const int64_t t =
+      (static_cast<int64_t>(p[0] - s.lo[0]) * s.ext[1] + (p[1] - s.lo[1])) * s.ext[2] + (p[2] - s.lo[2]);
+  cpmlPoint<T, IsE>(g, F, s, p, t);
+}
+
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_cpml_faces(Grid g, FieldPtrs<T> F, SlabArgs<T> s) {
+  cpmlFacePoint<T, IsE>(g, F, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 enum AdeScheme { kAdeDrude = 0, kAdeDebye = 1, kAdeMLorentz = 2 };
@@ -435,10 +581,17 @@ __global__ __launch_bounds__(256) void k_pack(Grid g, const T* __restrict__ src,
 // Host-side launchers, instantiated once per precision.
 template <class T>
 struct Launch {
-  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
-  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk);
-  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk);
-  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk);
+  // P.n > 0: the sweep applies the slabs of P in the same pass (then
+  // cpml_faces completes them on the far grid planes)
+  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
+                       const CpmlList<T>& P = CpmlList<T>{});
+  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
+                       const CpmlList<T>& P = CpmlList<T>{});
+  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
+                       const CpmlList<T>& P = CpmlList<T>{});
+  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
+                       const CpmlList<T>& P = CpmlList<T>{});
+  static void cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
                            const AdeArgs<T>& A, int scheme, int ichunk);
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C,

@@ -27,31 +27,54 @@ inline SweepMap sweepMap(const Grid& g, int ichunk) {
 }
 
 template <class T, class Coef>
-void launchUpdate(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk) {
+void launchUpdate(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk,
+                  const CpmlList<T>& P) {
   dim3 block(kLanesK, kRowsJ);
   const SweepMap m = sweepMap(g, ichunk);
   dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz));
-  if (is_h)
-    hipLaunchKernelGGL((k_update_h<T, Coef>), grid, block, 0, s, g, F, C, m, ichunk);
-  else
-    hipLaunchKernelGGL((k_update_e<T, Coef>), grid, block, 0, s, g, F, C, m, ichunk);
+  const bool fused = P.n > 0;
+  if (is_h) {
+    if (fused)
+      hipLaunchKernelGGL((k_update_h<T, Coef, true>), grid, block, 0, s, g, F, C, m, ichunk, P);
+    else
+      hipLaunchKernelGGL((k_update_h<T, Coef, false>), grid, block, 0, s, g, F, C, m, ichunk, P);
+  } else {
+    if (fused)
+      hipLaunchKernelGGL((k_update_e<T, Coef, true>), grid, block, 0, s, g, F, C, m, ichunk, P);
+    else
+      hipLaunchKernelGGL((k_update_e<T, Coef, false>), grid, block, 0, s, g, F, C, m, ichunk, P);
+  }
 }
 
 template <class T>
-void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
-  launchUpdate<T>(true, s, g, F, C, ichunk);
+void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
+                         const CpmlList<T>& P) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P);
 }
 template <class T>
-void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk) {
-  launchUpdate<T>(false, s, g, F, C, ichunk);
+void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
+                         const CpmlList<T>& P) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P);
 }
 template <class T>
-void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk) {
-  launchUpdate<T>(true, s, g, F, C, ichunk);
+void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
+                         const CpmlList<T>& P) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P);
 }
 template <class T>
-void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk) {
-  launchUpdate<T>(false, s, g, F, C, ichunk);
+void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
+                         const CpmlList<T>& P) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P);
+}
+template <class T>
+void Launch<T>::cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e) {
+  const FacePieces f = facePieces(g, a);
+  const int64_t n = f.nX + f.nY + f.nZ;
+  if (n <= 0) return;
+  if (is_e)
+    hipLaunchKernelGGL((k_cpml_faces<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s, g, F, a);
+  else
+    hipLaunchKernelGGL((k_cpml_faces<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, g, F, a);
 }
 
 template <class T, class Coef>
@@ -155,6 +178,16 @@ void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, c
     hipLaunchKernelGGL((k_batch_cpml<T, true>), g, dim3(256), 0, s, slabs, start, n, total);
   else
     hipLaunchKernelGGL((k_batch_cpml<T, false>), g, dim3(256), 0, s, slabs, start, n, total);
+}
+template <class T>
+void BatchLaunch<T>::cpmlFaces(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,
+                               int64_t total, int grid) {
+  if (total <= 0) return;
+  dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
+  if (is_e)
+    hipLaunchKernelGGL((k_batch_cpml_faces<T, true>), g, dim3(256), 0, s, slabs, start, n, total);
+  else
+    hipLaunchKernelGGL((k_batch_cpml_faces<T, false>), g, dim3(256), 0, s, slabs, start, n, total);
 }
 template <class T>
 void BatchLaunch<T>::records(hipStream_t s, const BatchRec<T>* recs, const int64_t* start, int n, int64_t total,

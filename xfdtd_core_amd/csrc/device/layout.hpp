@@ -136,6 +136,14 @@ struct SlabArgs {
   const T* cb;
 };
 
+// The CPML slabs of one half-step for the sweeps that apply them in the same
+// pass (device array of `n` slabs in boundary add order).
+template <class T>
+struct CpmlList {
+  const SlabArgs<T>* s = nullptr;
+  int n = 0;
+};
+
 // Additive correction applied after a Yee half-step, in the reference's
 // corrector order (objects, then TFSF; src/simulation/simulation.cpp:437-487).
 struct Record {

@@ -401,6 +401,7 @@ struct Simulation::Impl {
   std::vector<KernelStat> stats;
   long long dev_bytes = 0;
   bool coef_classed = false;
+  bool cpml_fused = false;
 
   ~Impl() { releaseDevice(); }
 
@@ -1394,6 +1395,9 @@ struct Simulation::Impl {
     int32_t cl = 0;
     xfd_coef_layout(ctx, &cl, nullptr);
     coef_classed = cl != 0;
+    int32_t fu = 0;
+    xfd_cpml_fused(ctx, &fu);
+    cpml_fused = fu != 0;
   }
 
   void runLoop() {
@@ -1662,6 +1666,7 @@ void Simulation::setProfiling(bool on) { _impl->profiling = on; }
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
 bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
+bool Simulation::fusedCpml() const { return _impl->cpml_fused; }
 
 void Simulation::setDesignRegion(const std::array<Index, 3>& origin, const std::array<Index, 3>& size,
                                  std::vector<Real> eps_r) {

@@ -55,6 +55,8 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_grid": (ctypes.c_int, [vp, p64]),
         "xfd_sim_device_bytes": (i64, [vp]),
         "xfd_sim_coef_classed": (ctypes.c_int32, [vp]),
+        "xfd_sim_cpml_fused": (ctypes.c_int32, [vp]),
+        "xfd_cpml_fused": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32)]),
         "xfd_sim_nf2ff_count": (ctypes.c_int32, [vp]),
         "xfd_sim_nf2ff_surface": (ctypes.c_int, [vp, i32, i32, p64, p64]),
         "xfd_sim_nf2ff_current": (ctypes.c_int, [vp, i32, i32, i32, i32, vp]),
@@ -349,6 +351,9 @@ class Simulation:
 
     def device_bytes(self) -> int:
         return int(self._L.xfd_sim_device_bytes(self._h))
+
+    def cpml_fused(self) -> bool:
+        return bool(self._L.xfd_sim_cpml_fused(self._h))
 
     def coef_classed(self) -> bool:
         """True when the device stores coefficient class ids, not 18 arrays."""
