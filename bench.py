@@ -27,6 +27,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -39,19 +41,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int) -> float:
+def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: bool = False) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` on an n^3 grid.
 
     update_h / update_e: per cell 3 own-field reads + 3 writes, 3 dual-field
-    reads (neighbours reused) and 9 coefficient reads = 18 * s bytes.
-    cpml_h / cpml_e (one slab): per slab cell and component psi read + write,
-    c_psi read, field read + write and one new dual-field read = 6 * s,
-    two components per slab."""
+    reads (neighbours reused) and 9 coefficient reads = 18 * s bytes; with the
+    CPML applied in the same sweep, plus per slab cell and component psi read
+    + write and c_psi read (3 * s, two components per slab, six slabs).
+    cpml_h / cpml_e as separate launches (one slab): per slab cell and
+    component psi read + write, c_psi read, field read + write and one new
+    dual-field read = 6 * s, two components per slab.  With fusion these
+    launches only finish the far grid planes (not counted)."""
     cells = n ** 3
+    slab_cells = pml * (n + 1) * (n + 1)
     if kernel in ("update_h", "update_e"):
-        return 18.0 * s * cells
+        return 18.0 * s * cells + (2 * 3.0 * s * 6 * slab_cells if fused else 0.0)
     if kernel in ("cpml_h", "cpml_e"):
-        return 2 * 6.0 * s * pml * (n + 1) * (n + 1)
+        return 0.0 if fused else 2 * 6.0 * s * slab_cells
     return 0.0
 
 
@@ -96,13 +102,14 @@ def run_device(sim, precision, warmup, steps, n, pml):
     wall = time.perf_counter() - t0
     stats = sim.kernel_stats()
     s = 4 if precision == "f32" else 8
+    fused = sim.cpml_fused()
     cells = n ** 3
     kernels = {}
     for name, (launches, ms) in stats.items():
         if launches == 0:
             continue
         avg_ms = ms / launches
-        b = alg_bytes_per_launch(name, n, pml, s)
+        b = alg_bytes_per_launch(name, n, pml, s, fused)
         kernels[name] = {"launches": launches, "avg_ms": round(avg_ms, 5), "total_ms": round(ms, 3),
                          "alg_GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
@@ -134,6 +141,7 @@ def run_device(sim, precision, warmup, steps, n, pml):
                 kernels[k]["layout_bytes"] = ib
                 kernels[k]["layout_GBps"] = round(ib / (kernels[k]["avg_ms"] * 1e-3) / 1e9, 1)
     res["coefficients"] = "class ids" if sim.coef_classed() else "dense"
+    res["cpml"] = "inside the Yee sweeps" if fused else "separate slab launches"
     # recordings for the accuracy comparison: six snapshot series, then probes
     sim.finish()
     res["snapshots"] = {f: sim.monitor_records(i) for i, f in enumerate(FIELDS)}
@@ -193,6 +201,105 @@ def cpu_baseline(spec, n, steps):
                       f"(faster of {{1,1,{cores}}} and {{{cores},1,1}})"}
 
 
+def bench_design_sweep(args):
+    """Batched design sweep: 96 independent FP32 cases of uneven size in one
+    set of persistent launches per step, one step captured as a HIP graph and
+    replayed, tile shape from the autotuner.  value = total cell updates / s
+    over all cases."""
+    from xfdtd_core_amd import native, scenarios
+
+    cases = scenarios.design_sweep(n_cases=args.cases)
+    log(f"design sweep: {len(cases)} cases, precompute + upload")
+    t0 = time.perf_counter()
+    sims = [native.Simulation(c.spec(), "f32") for c in cases]
+    batch = native.SimulationBatch(sims)
+    batch.init(args.warmup + args.steps)
+    cells = sum(int(np.prod(s.grid())) for s in sims)
+    log(f"design sweep: init {time.perf_counter() - t0:.1f} s, {cells / 1e6:.1f} M cells")
+    batch.set_graph(True)
+    t0 = time.perf_counter()
+    batch.tune(3)
+    info = batch.info()
+    log(f"design sweep: tuned in {time.perf_counter() - t0:.1f} s: ichunk {info['ichunk']}, grid {info['grid']}")
+    batch.advance(0, args.warmup)
+    native.device_synchronize()
+    t0 = time.perf_counter()
+    batch.advance(args.warmup, args.steps)
+    native.device_synchronize()
+    wall = time.perf_counter() - t0
+    info = batch.info()
+    value = cells * args.steps / wall / 1e6
+    log(f"design sweep: {value:.0f} Mcells/s ({wall / args.steps * 1e3:.3f} ms/step; device "
+        f"{info['device_ms'] / args.steps:.3f} ms/step)")
+    batch.finish()
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random sizes, cubes and permittivities; no dataset)",
+        "config": {"workload": f"batched design sweep: {len(cases)} cases, per-axis size from "
+                               "{48,64,96,128,160} incl. 8-cell CPML, one dielectric cube each (eps_r U[1,8]), "
+                               "Ez Gaussian dipole, 4-frequency FP64 DFT on the cube faces, HIP-graph steps",
+                   "cells": cells, "parallelism": "one persistent launch per phase over all cases, 1 GPU"},
+        "batch": {"shared_launches": info["shared"], "ichunk": info["ichunk"], "grid": info["grid"],
+                  "device_ms_per_step": round(info["device_ms"] / args.steps, 4)},
+    }
+    for s_ in sims:
+        s_.close()
+    batch.close()
+    return line
+
+
+def bench_adjoint(args):
+    """Adjoint inverse design at full size: FP32 forward with checkpoints
+    every 100 steps and the reverse-time adjoint; optionally the FP64 adjoint
+    and central differences (FP64 forward runs) at --fd-cells random design
+    cells."""
+    from xfdtd_core_amd import native, scenarios
+
+    s, origin, eps = scenarios.adjoint_design()
+    steps, K = args.adjoint_steps, 100
+    out = {}
+    grads = {}
+    for prec in (["f32", "f64"] if args.fd_cells or args.adjoint_f64 else ["f32"]):
+        sim = native.Simulation(s.spec(), prec)
+        t0 = time.perf_counter()
+        J = sim.forward(steps, K)
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g = sim.backward(eps.shape)
+        tb = time.perf_counter() - t0
+        grads[prec] = g
+        out[prec] = {"J": J, "forward_s": round(tf, 3), "backward_s": round(tb, 3)}
+        log(f"adjoint {prec}: J {J:.6e}, forward {tf:.2f} s (incl. init), backward {tb:.2f} s")
+        sim.close()
+    res = {"steps": steps, "checkpoint_interval": K, "design_cells": int(eps.size), "runs": out}
+    if "f64" in grads:
+        d = grads["f32"] - grads["f64"]
+        res["fp32_vs_fp64_gradient_rel_l2"] = float(np.linalg.norm(d) / np.linalg.norm(grads["f64"]))
+        log(f"adjoint: FP32 vs FP64 gradient relative L2 {res['fp32_vs_fp64_gradient_rel_l2']:.2e}")
+    if args.fd_cells:
+        rng = np.random.default_rng(7)
+        h = 1e-3
+        errs = []
+        sim = native.Simulation(s.spec(), "f64")
+        for q in range(args.fd_cells):
+            c = tuple(int(v) for v in rng.integers(0, eps.shape[0], size=3))
+            vals = []
+            for sgn in (1, -1):
+                e = eps.copy()
+                e[c] += sgn * h
+                sim.set_design(origin, e)
+                vals.append(sim.forward(steps, steps))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            errs.append(abs(grads["f64"][c] - fd) / abs(fd))
+            log(f"adjoint fd cell {c}: adjoint {grads['f64'][c]:.6e} fd {fd:.6e} rel {errs[-1]:.2e}")
+        res["fd_cells"] = args.fd_cells
+        res["fd_rel_err_max"] = max(errs)
+        res["fd_rel_err_median"] = float(np.median(errs))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +310,12 @@ def main():
     ap.add_argument("--precision", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint"], default="box",
+                    help="box: the 384^3 headline; design_sweep: the batched form; adjoint: the gradient form")
+    ap.add_argument("--cases", type=int, default=96)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-f64", action="store_true")
+    ap.add_argument("--fd-cells", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -211,6 +324,13 @@ def main():
         sys.exit(2)
 
     from xfdtd_core_amd import native, scenarios
+
+    if args.workload == "design_sweep":
+        print(json.dumps(bench_design_sweep(args)), flush=True)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps({"workload": "adjoint inverse design 160^3", **bench_adjoint(args)}), flush=True)
+        return
 
     spec = scenarios.inhomogeneous_box(n_total=args.n, pml=args.pml, snapshot_interval=500).spec()
     dev_name, dev_bytes = native.device_info()
