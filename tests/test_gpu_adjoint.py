@@ -14,6 +14,7 @@ random design block, DFT objective on an Ez patch, CPML on all faces):
 * the torch.autograd.Function returns the same gradient.
 """
 
+import copy
 import sys
 
 import numpy as np
@@ -38,6 +39,7 @@ def _problem(seed=4):
 
 
 def _oracle_J(s, eps, origin):
+    s = copy.deepcopy(s)  # the shared scenario keeps its own permittivity
     s.setDesignRegion(origin, eps.shape, eps)
     r = RefSim(s.spec())
     r.run(STEPS)
