@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_batch_sweep(const BatchCase<T>* __restr
                                                      const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
-    const BatchCase<T> C = cases[e];
+    const BatchCase<T>& C = cases[e];
     const int64_t local = b - start[e];
     const int bx = static_cast<int>(local % C.nbx);
     const int64_t r = local / C.nbx;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_batch_cpml(const BatchSlab<T>* __restri
                                                     const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
-    const BatchSlab<T> S = slabs[e];
+    const BatchSlab<T>& S = slabs[e];
     cpmlCell<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
   }
 }
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_batch_cpml_faces(const BatchSlab<T>* __
                                                           const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
-    const BatchSlab<T> S = slabs[e];
+    const BatchSlab<T>& S = slabs[e];
     cpmlFacePoint<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
   }
 }
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_batch_records(const BatchRec<T>* __rest
   const int64_t t = *tdev;
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
-    const BatchRec<T> R = recs[e];
+    const BatchRec<T>& R = recs[e];
     recordSeg(R.F, R.rec, R.seg, R.nseg, R.pool, R.state, (b - start[e]) * blockDim.x + threadIdx.x, t);
   }
 }
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_batch_nf2ff(const BatchNf2ff<T>* __rest
   const int64_t t = *tdev;
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
-    const BatchNf2ff<T> N = nfs[e];
+    const BatchNf2ff<T>& N = nfs[e];
     nf2ffCell(N.g, N.F, N.a, (b - start[e]) * blockDim.x + threadIdx.x, t);
   }
 }

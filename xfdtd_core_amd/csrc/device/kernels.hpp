@@ -30,42 +30,66 @@ constexpr int kRowsJ = 4;
 // difference the sweep already holds: dA[c] for component A, dB[c] for B.
 // Same operations in the same order as k_cpml after the sweep, so results
 // are unchanged; the field and its dual are not read again.
+// Component c of a point, without a runtime-indexed local array (which the
+// compiler would place in scratch memory).
+__device__ __forceinline__ int pick3(int c, int x, int y, int z) { return c == 0 ? x : (c == 1 ? y : z); }
+
+// Bit n set: slab n of P contains (i, j, k) for some i of [i0, i1).  Taken
+// once per thread and i-chunk, so the sweep's inner loop only tests i.
 template <class T>
-__device__ __forceinline__ void cpmlFused(const CpmlList<T>& P, int i, int j, int k, T& v0, T& v1, T& v2,
-                                          const T dA0, const T dA1, const T dA2, const T dB0, const T dB1,
-                                          const T dB2) {
-  const int p[3] = {i, j, k};
+__device__ __forceinline__ uint32_t cpmlChunkMask(const CpmlList<T>& P, int i0, int i1, int j, int k) {
+  uint32_t m = 0;
   for (int n = 0; n < P.n; ++n) {
     const SlabArgs<T>& S = P.s[n];
-    const int u = i - S.lo[0], v = j - S.lo[1], w = k - S.lo[2];
-    if (u < 0 || v < 0 || w < 0 || u >= S.ext[0] || v >= S.ext[1] || w >= S.ext[2]) continue;
+    if (j >= S.lo[1] && j < S.lo[1] + S.ext[1] && k >= S.lo[2] && k < S.lo[2] + S.ext[2] && i1 > S.lo[0] &&
+        i0 < S.lo[0] + S.ext[0])
+      m |= 1u << n;
+  }
+  return m;
+}
+
+// The CPML corrections of one point, fused into a Yee sweep: for every slab
+// of P containing the point (in boundary order), for its components A =
+// (c+1)%3 and B = (c+2)%3 of this half-step,
+//   psi = b psi + a d,  v += c_psi psi
+// with d the dual difference along the slab axis c, which is the Yee curl
+// difference the sweep already holds: dA[c] for component A, dB[c] for B.
+// Same operations in the same order as k_cpml after the sweep, so results
+// are unchanged; the field and its dual are not read again.
+template <class T>
+__device__ __forceinline__ void cpmlFused(const CpmlList<T>& P, uint32_t mask, int i, int j, int k, T& v0, T& v1,
+                                          T& v2, const T dA0, const T dA1, const T dA2, const T dB0, const T dB1,
+                                          const T dB2) {
+  while (mask) {
+    const int n = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const SlabArgs<T>& S = P.s[n];
+    const int u = i - S.lo[0];
+    if (u < 0 || u >= S.ext[0]) continue;
     const int c = S.axis, a = c == 2 ? 0 : c + 1, b = c == 0 ? 2 : c - 1;
-    const int64_t t = (static_cast<int64_t>(u) * S.ext[1] + v) * S.ext[2] + w;
-    const int layer = p[c] - S.start;
+    const int64_t t = (static_cast<int64_t>(u) * S.ext[1] + (j - S.lo[1])) * S.ext[2] + (k - S.lo[2]);
+    const int layer = pick3(c, i, j, k) - S.start;
+    const int pa = pick3(a, i, j, k), pb = pick3(b, i, j, k);
     const T ca = S.ca[layer], cb = S.cb[layer];
-    if (p[a] < S.extA[0] && p[b] < S.extA[1]) {
+    if (pa < S.extA[0] && pb < S.extA[1]) {
       const T d = c == 0 ? dA0 : (c == 1 ? dA1 : dA2);
       const T psi = cb * S.psiA[t] + ca * d;
       S.psiA[t] = psi;
-      const T add = S.cpsiA[t] * psi;
-      if (a == 0)
-        v0 = v0 + add;
-      else if (a == 1)
-        v1 = v1 + add;
-      else
-        v2 = v2 + add;
+      const T add = S.cpsiA[t] * psi;  // value selects: no address of v0..v2 is taken
+      const T n0 = a == 0 ? v0 + add : v0, n1 = a == 1 ? v1 + add : v1, n2 = a == 2 ? v2 + add : v2;
+      v0 = n0;
+      v1 = n1;
+      v2 = n2;
     }
-    if (p[a] < S.extB[0] && p[b] < S.extB[1]) {
+    if (pa < S.extB[0] && pb < S.extB[1]) {
       const T d = c == 0 ? dB0 : (c == 1 ? dB1 : dB2);
       const T psi = cb * S.psiB[t] + ca * d;
       S.psiB[t] = psi;
       const T add = S.cpsiB[t] * psi;
-      if (b == 0)
-        v0 = v0 + add;
-      else if (b == 1)
-        v1 = v1 + add;
-      else
-        v2 = v2 + add;
+      const T n0 = b == 0 ? v0 + add : v0, n1 = b == 1 ? v1 + add : v1, n2 = b == 2 ? v2 + add : v2;
+      v0 = n0;
+      v1 = n1;
+      v2 = n2;
     }
   }
 }
@@ -93,6 +117,7 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   T* __restrict__ hz = F.f[5];
   T ey_i = ey[idx];
   T ez_i = ez[idx];
+  const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
   for (int i = i0; i < i1; ++i, idx += pl) {
     const T ex0 = ex[idx], ex_j = ex[idx + pz], ex_k = ex[idx + 1];
     const T ey_k = ey[idx + 1], ey_n = ey[idx + pl];
@@ -104,8 +129,8 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
     T vy = s * hy[idx] + a * (ez_n - ez_i) + b * (ex_k - ex0);
     C.get(5, idx, s, a, b);
     T vz = s * hz[idx] + a * (ex_j - ex0) + b * (ey_n - ey_i);
-    if (Fused)  // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
-      cpmlFused(P, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
+    if (Fused && mask)  // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
+      cpmlFused(P, mask, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
                 ex_k - ex0);
     hx[idx] = vx;
     hy[idx] = vy;
@@ -148,10 +173,11 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   T hz_p = i0 > 0 ? hz[idx - pl] : T(0);
   T hy_p = i0 > 0 ? hy[idx - pl] : T(0);
   const bool jk = j >= 1 && k >= 1;
+  const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
   for (int i = i0; i < i1; ++i, idx += pl) {
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
     T s, a, b;
-    if (!Fused) {
+    if (!Fused || !mask) {
       if (jk) {
         C.get(0, idx, s, a, b);
         ex[idx] = s * ex[idx] + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
@@ -186,7 +212,7 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
         vz = s * vz + a * dyHx + b * dxHy;
       }
       // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
-      cpmlFused(P, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+      cpmlFused(P, mask, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
       ex[idx] = vx;
       ey[idx] = vy;
       ez[idx] = vz;
@@ -225,14 +251,15 @@ __global__ __launch_bounds__(64) void k_probes(FieldPtrs<T> F, const Probe* __re
 // with q one plane below along the main axis; H: psi = b psi + a (q - p)
 // with q one plane above.  field += c_psi * psi.
 template <class T, bool IsE>
-__device__ __forceinline__ void cpmlPoint(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s, const int p[3],
-                                          int64_t t) {
-  const int c = s.axis, a = (c + 1) % 3, b = (c + 2) % 3;
-  const int layer = p[c] - s.start;
+__device__ __forceinline__ void cpmlPoint(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s, int p0, int p1,
+                                          int p2, int64_t t) {
+  const int c = s.axis, a = c == 2 ? 0 : c + 1, b = c == 0 ? 2 : c - 1;
+  const int layer = pick3(c, p0, p1, p2) - s.start;
   const T ca = s.ca[layer], cb = s.cb[layer];
-  const int64_t idx = (static_cast<int64_t>(p[0]) * g.NY + p[1]) * g.pz + p[2];
+  const int64_t idx = (static_cast<int64_t>(p0) * g.NY + p1) * g.pz + p2;
   const int64_t q = IsE ? idx - s.stride_c : idx + s.stride_c;
-  if (p[a] < s.extA[0] && p[b] < s.extA[1]) {
+  const int pa = pick3(a, p0, p1, p2), pb = pick3(b, p0, p1, p2);
+  if (pa < s.extA[0] && pb < s.extA[1]) {
     const T* dual = F.f[s.dualB];
     T psi = s.psiA[t];
     psi = IsE ? cb * psi + ca * (dual[idx] - dual[q]) : cb * psi + ca * (dual[q] - dual[idx]);
@@ -240,7 +267,7 @@ __device__ __forceinline__ void cpmlPoint(const Grid& g, const FieldPtrs<T>& F, 
     T* f = F.f[s.fieldA];
     f[idx] += s.cpsiA[t] * psi;
   }
-  if (p[a] < s.extB[0] && p[b] < s.extB[1]) {
+  if (pa < s.extB[0] && pb < s.extB[1]) {
     const T* dual = F.f[s.dualA];
     T psi = s.psiB[t];
     psi = IsE ? cb * psi + ca * (dual[idx] - dual[q]) : cb * psi + ca * (dual[q] - dual[idx]);
@@ -257,8 +284,7 @@ __device__ __forceinline__ void cpmlCell(const Grid& g, const FieldPtrs<T>& F, c
   const int w = static_cast<int>(t % s.ext[2]);
   const int v = static_cast<int>((t / s.ext[2]) % s.ext[1]);
   const int u = static_cast<int>(t / (static_cast<int64_t>(s.ext[2]) * s.ext[1]));
-  const int p[3] = {s.lo[0] + u, s.lo[1] + v, s.lo[2] + w};
-  cpmlPoint<T, IsE>(g, F, s, p, t);
+  cpmlPoint<T, IsE>(g, F, s, s.lo[0] + u, s.lo[1] + v, s.lo[2] + w, t);
 }
 
 template <class T, bool IsE>
@@ -289,27 +315,26 @@ template <class T, bool IsE>
 __device__ __forceinline__ void cpmlFacePoint(const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& s,
                                               int64_t q) {
   const FacePieces f = facePieces(g, s);
-  int p[3];
+  int p0, p1, p2;
   if (q < f.nX) {
-    p[0] = g.nx;
-    p[1] = s.lo[1] + static_cast<int>(q / s.ext[2]);
-    p[2] = s.lo[2] + static_cast<int>(q % s.ext[2]);
+    p0 = g.nx;
+    p1 = s.lo[1] + static_cast<int>(q / s.ext[2]);
+    p2 = s.lo[2] + static_cast<int>(q % s.ext[2]);
   } else if (q < f.nX + f.nY) {
     q -= f.nX;
-    p[0] = s.lo[0] + static_cast<int>(q / s.ext[2]);
-    p[1] = g.ny;
-    p[2] = s.lo[2] + static_cast<int>(q % s.ext[2]);
+    p0 = s.lo[0] + static_cast<int>(q / s.ext[2]);
+    p1 = g.ny;
+    p2 = s.lo[2] + static_cast<int>(q % s.ext[2]);
   } else if (q < f.nX + f.nY + f.nZ) {
     q -= f.nX + f.nY;
-    p[0] = s.lo[0] + static_cast<int>(q / f.n1);
-    p[1] = s.lo[1] + static_cast<int>(q % f.n1);
-    p[2] = g.nz;
+    p0 = s.lo[0] + static_cast<int>(q / f.n1);
+    p1 = s.lo[1] + static_cast<int>(q % f.n1);
+    p2 = g.nz;
   } else {
     return;
   }
-  const int64_t t =
-      (static_cast<int64_t>(p[0] - s.lo[0]) * s.ext[1] + (p[1] - s.lo[1])) * s.ext[2] + (p[2] - s.lo[2]);
-  cpmlPoint<T, IsE>(g, F, s, p, t);
+  const int64_t t = (static_cast<int64_t>(p0 - s.lo[0]) * s.ext[1] + (p1 - s.lo[1])) * s.ext[2] + (p2 - s.lo[2]);
+  cpmlPoint<T, IsE>(g, F, s, p0, p1, p2, t);
 }
 
 template <class T, bool IsE>
