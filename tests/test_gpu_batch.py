@@ -80,17 +80,42 @@ def _compare(got, want, exact, label):
     return worst
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_batch_matches_single_cases(precision):
+ETA0 = 376.730313668
+
+
+def _state_err(a, b):
+    """Relative L2 of the whole field state (E, eta0 H): the Gaussian dipole
+    leaves a static E field while H decays to roundoff, so a component's own
+    norm is no scale for H."""
+    da = np.concatenate([a[f].ravel() * (ETA0 if f[0] == "H" else 1.0) for f in FIELDS])
+    db = np.concatenate([b[f].ravel() * (ETA0 if f[0] == "H" else 1.0) for f in FIELDS])
+    return float(np.linalg.norm(da - db) / np.linalg.norm(db))
+
+
+def test_batch_matches_single_cases_fp64():
     specs = _cases()
-    say(f"[gpu] batch vs single cases {precision}")
-    want = _alone(specs, precision)
-    sims, b, got, info = _batched(specs, precision)
+    say("[gpu] batch vs single cases f64")
+    want = _alone(specs, "f64")
+    sims, b, got, info = _batched(specs, "f64")
     assert info["shared"], info
-    if precision == "f64":
-        _compare(got, want, True, "batch f64")
-    else:
-        assert _compare(got, want, False, "batch f32") <= 1e-6
+    _compare(got, want, True, "batch f64")
+
+
+def test_batch_fp32_accuracy_matches_single_cases():
+    """FP32: the batched and the single-context sweeps differ only in the
+    compiler's contraction choices, so the batch is as close to the FP64 run
+    as each case run alone."""
+    specs = _cases()
+    say("[gpu] batch vs single cases f32")
+    ref = _alone(specs, "f64")
+    single = _alone(specs, "f32")
+    sims, b, got, info = _batched(specs, "f32")
+    assert info["shared"], info
+    for i in range(len(specs)):
+        eb, es, ebs = _state_err(got[i], ref[i]), _state_err(single[i], ref[i]), _state_err(got[i], single[i])
+        say(f"  case {i}: batch-vs-f64 {eb:.2e}, single-vs-f64 {es:.2e}, batch-vs-single {ebs:.2e}")
+        assert eb <= 2 * es + 1e-6
+        assert ebs <= 1e-4
 
 
 def test_batch_fp64_against_oracle():

@@ -268,6 +268,7 @@ struct xfd_ctx {
   // CPML applied inside the Yee sweeps (see cpmlFused): the slab arguments
   // of each half-step as device arrays
   bool fused = false;
+  int n_fused = 0;  // slabs [0, n_fused) run inside the sweeps, the rest separately
   DevBuf cpml_list_e, cpml_list_h;
 
   // checkpoints of a forward run for the adjoint (xfd_forward_checkpointed)
@@ -797,11 +798,19 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   {  // CPML inside the sweeps unless XFD_CPML_FUSED=0, ADE or records in a slab
     const char* env = std::getenv("XFD_CPML_FUSED");
     const bool off = env && env[0] == '0';
-    c.fused = !off && !c.slabs.empty() && c.ade_scheme < 0 && !records_meet_slabs;
+    // Only x and y slabs: their membership is uniform over a wavefront (one i,
+    // one j row), while a z slab splits the 64 k-lanes of a wave and would
+    // make it run both paths.  The fused slabs are the leading x/y slabs in
+    // boundary order, so the slabs' relative order is kept.
+    int prefix = 0;
+    while (prefix < static_cast<int>(c.slabs.size()) && c.slabs[prefix]->axis != 2) ++prefix;
+    c.fused = !off && prefix > 0 && c.ade_scheme < 0 && !records_meet_slabs;
+    c.n_fused = c.fused ? prefix : 0;
     if (c.fused) {
       const int64_t stride[3] = {g.plane, g.pz, 1};
       std::vector<xfd::SlabArgs<T>> le, lh;
-      for (auto& sl : c.slabs) {
+      for (int n = 0; n < prefix; ++n) {
+        auto& sl = c.slabs[n];
         le.push_back(slabArgs<T>(*sl, true));
         lh.push_back(slabArgs<T>(*sl, false));
         le.back().stride_c = stride[le.back().axis];
@@ -894,8 +903,8 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   hipStream_t st = c.stream;
   xfd::CpmlList<T> PE{}, PH{};
   if (c.fused) {
-    PE = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
-    PH = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
+    PE = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
+    PH = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), c.n_fused};
   }
   for (int64_t t = t0; t < t0 + n; ++t) {
     c.timed(K_UPDATE_H, [&] {
@@ -909,10 +918,13 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         xfd::Launch<T>::records(st, F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h,
                                 c.pool.as<double>(), c.state.as<T>(), t);
       });
-    if (c.fused)
-      for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml_faces(st, c.g, F, a, false); });
-    else
-      for (auto& a : sh) c.timed(K_CPML_H, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, false); });
+    for (size_t q = 0; q < sh.size(); ++q)
+      c.timed(K_CPML_H, [&] {
+        if (static_cast<int>(q) < c.n_fused)
+          xfd::Launch<T>::cpml_faces(st, c.g, F, sh[q], false);
+        else
+          xfd::Launch<T>::cpml(st, c.g, F, sh[q], false);
+      });
     c.timed(K_UPDATE_E, [&] {
       if (c.ade_scheme >= 0) {
         if (c.classed)
@@ -930,10 +942,13 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         xfd::Launch<T>::records(st, F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e,
                                 c.pool.as<double>(), c.state.as<T>(), t);
       });
-    if (c.fused)
-      for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml_faces(st, c.g, F, a, true); });
-    else
-      for (auto& a : se) c.timed(K_CPML_E, [&] { xfd::Launch<T>::cpml(st, c.g, F, a, true); });
+    for (size_t q = 0; q < se.size(); ++q)
+      c.timed(K_CPML_E, [&] {
+        if (static_cast<int>(q) < c.n_fused)
+          xfd::Launch<T>::cpml_faces(st, c.g, F, se[q], true);
+        else
+          xfd::Launch<T>::cpml(st, c.g, F, se[q], true);
+      });
     if (!record) {
       XFD_HIP(hipGetLastError());
       continue;
@@ -1158,7 +1173,7 @@ struct xfd_batch {
   int dtype = XFD_F64;
   std::vector<std::unique_ptr<xfd_ctx>> cases;
   bool shared = true;  // every case class-indexed and free of ADE: batched launches
-  bool fused = true;   // every case applies its CPML inside the sweeps
+
   hipStream_t stream = nullptr;
   DevBuf tdev;
   int ichunk = 16;
@@ -1174,7 +1189,8 @@ struct xfd_batch {
     int64_t total = 0;
   };
   Table sweep, rec_e, rec_h, nf;
-  std::vector<Table> cpml_e, cpml_h;
+  std::vector<Table> cpml_e, cpml_h;            // slabs stepped by separate launches
+  std::vector<Table> faces_e, faces_h;          // far planes of slabs fused into the sweeps
   int64_t pool_rows = INT64_MAX;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -1225,7 +1241,7 @@ struct Collect {
 template <class T>
 void batchTables(xfd_batch& b) {
   Collect<xfd::BatchCase<T>> sw;
-  std::vector<Collect<xfd::BatchSlab<T>>> ce, ch;
+  std::vector<Collect<xfd::BatchSlab<T>>> ce, ch, fe, fh;
   Collect<xfd::BatchRec<T>> re, rh;
   Collect<xfd::BatchNf2ff<T>> nf;
   for (auto& up : b.cases) {
@@ -1239,9 +1255,9 @@ void batchTables(xfd_batch& b) {
       bc.K.tab[q] = c.tabs[q].as<T>();
     }
     bc.ichunk = b.ichunk;
-    if (b.fused && c.fused) {
-      bc.ph = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
-      bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), static_cast<int>(c.slabs.size())};
+    if (c.fused) {
+      bc.ph = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), c.n_fused};
+      bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
     }
     bc.nbx = (c.g.nz + xfd::kLanesK - 1) / xfd::kLanesK;
     bc.nby = (c.g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ;
@@ -1252,16 +1268,19 @@ void batchTables(xfd_batch& b) {
       if (ce.size() <= o) {
         ce.resize(o + 1);
         ch.resize(o + 1);
+        fe.resize(o + 1);
+        fh.resize(o + 1);
       }
       for (int is_e = 0; is_e < 2; ++is_e) {
         xfd::BatchSlab<T> e{c.g, F, slabArgs<T>(*c.slabs[o], is_e != 0)};
         e.s.stride_c = stride[e.s.axis];
-        int64_t vol = static_cast<int64_t>(e.s.ext[0]) * e.s.ext[1] * e.s.ext[2];
-        if (b.fused) {  // only the far-plane remainder
+        if (static_cast<int>(o) < c.n_fused) {  // only the far-plane remainder
           const auto fp = xfd::facePieces(c.g, e.s);
-          vol = fp.nX + fp.nY + fp.nZ;
+          (is_e ? fe : fh)[o].add(e, (fp.nX + fp.nY + fp.nZ + 255) / 256);
+        } else {
+          const int64_t vol = static_cast<int64_t>(e.s.ext[0]) * e.s.ext[1] * e.s.ext[2];
+          (is_e ? ce : ch)[o].add(e, (vol + 255) / 256);
         }
-        (is_e ? ce : ch)[o].add(e, (vol + 255) / 256);
       }
     }
     if (c.nseg_h)
@@ -1277,13 +1296,15 @@ void batchTables(xfd_batch& b) {
   re.to(b.rec_e);
   rh.to(b.rec_h);
   nf.to(b.nf);
-  b.cpml_e.clear();
-  b.cpml_h.clear();
-  b.cpml_e.resize(ce.size());
-  b.cpml_h.resize(ch.size());
+  for (auto* v : {&b.cpml_e, &b.cpml_h, &b.faces_e, &b.faces_h}) {
+    v->clear();
+    v->resize(ce.size());
+  }
   for (size_t o = 0; o < ce.size(); ++o) {
     ce[o].to(b.cpml_e[o]);
     ch[o].to(b.cpml_h[o]);
+    fe[o].to(b.faces_e[o]);
+    fh[o].to(b.faces_h[o]);
   }
 }
 
@@ -1301,13 +1322,14 @@ void batchEnqueueStep(xfd_batch& b) {
   auto records = [&](xfd_batch::Table& r) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
   };
-  auto cpml = [&](bool is_e) {
-    for (auto& c : is_e ? b.cpml_e : b.cpml_h) {
-      if (!c.n) continue;
-      if (b.fused)
-        L::cpmlFaces(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
-      else
-        L::cpml(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
+  auto cpml = [&](bool is_e) {  // slab ordinal by ordinal: boundary order within each case
+    auto& full = is_e ? b.cpml_e : b.cpml_h;
+    auto& faces = is_e ? b.faces_e : b.faces_h;
+    for (size_t o = 0; o < full.size(); ++o) {
+      auto& f = faces[o];
+      if (f.n) L::cpmlFaces(st, is_e, f.tab.as<xfd::BatchSlab<T>>(), f.start.as<int64_t>(), f.n, f.total, b.grid);
+      auto& c = full[o];
+      if (c.n) L::cpml(st, is_e, c.tab.as<xfd::BatchSlab<T>>(), c.start.as<int64_t>(), c.n, c.total, b.grid);
     }
   };
   sweep(true);
@@ -1590,7 +1612,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
       b->cases.emplace_back(c);
       b->shared = b->shared && c->classed && c->ade_scheme < 0;
-      b->fused = b->fused && (c->fused || c->slabs.empty());
+
     }
     XFD_HIP(hipStreamCreate(&b->stream));
     XFD_HIP(hipEventCreate(&b->ev0));
