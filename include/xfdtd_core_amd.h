@@ -201,11 +201,12 @@ int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
  * environment variable XFD_COEF_MODE=dense forces the dense layout. */
 int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
 
-/* *fused = 1 when the Yee sweeps apply the CPML corrections in the same
- * pass (same operations in the same order, fewer bytes moved); 0 when the
- * slabs run as separate launches: with ADE materials, when a TFSF or lumped
- * correction lands inside a slab (the reference applies those first), or
- * when the environment sets XFD_CPML_FUSED=0. */
+/* *fused = 1 when the Yee sweeps apply the leading x/y CPML slabs in the
+ * same pass (same operations in the same order, fewer bytes moved; z slabs
+ * and the far grid planes stay separate launches).  Opt-in with the
+ * environment variable XFD_CPML_FUSED=1 (the separate launches are faster on
+ * MI355X); never with ADE materials or when a TFSF or lumped correction lands
+ * inside a slab (the reference applies those first). */
 int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);
 
 /* Zeroes a context's time-stepping state: fields, CPML auxiliaries, inductor

@@ -92,9 +92,12 @@ def _state_err(a, b):
     return float(np.linalg.norm(da - db) / np.linalg.norm(db))
 
 
-def test_batch_matches_single_cases_fp64():
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_batch_matches_single_cases_fp64(monkeypatch, fused):
+    """Also with the x/y CPML slabs applied inside the batched sweeps."""
+    monkeypatch.setenv("XFD_CPML_FUSED", fused)
     specs = _cases()
-    say("[gpu] batch vs single cases f64")
+    say(f"[gpu] batch vs single cases f64 (CPML in sweeps: {fused})")
     want = _alone(specs, "f64")
     sims, b, got, info = _batched(specs, "f64")
     assert info["shared"], info

@@ -308,6 +308,7 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
     slab or ADE is present keep the separate launches."""
     sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
     say(f"[gpu] fused CPML vs separate {precision}")
+    monkeypatch.setenv("XFD_CPML_FUSED", "1")
     fused = native.Simulation(sp, precision)
     fused.run(200)
     assert fused.cpml_fused()
@@ -323,6 +324,7 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
             # contraction differences grow while the field decays; the FP32
             # accuracy target (1e-4) bounds them
             assert rel_l2(a, b) < 1e-4, (f, rel_l2(a, b))
+    monkeypatch.setenv("XFD_CPML_FUSED", "1")
     disp = native.Simulation(scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9,
                                                         lumped=False).spec(), precision)
     disp.init(2)
