@@ -61,7 +61,7 @@ struct BatchNf2ff {
   Nf2ffArgs a;
 };
 
-template <class T, bool IsH>
+template <class T, bool IsH, bool Fused>
 __global__ __launch_bounds__(256) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
                                                      const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
@@ -73,9 +73,9 @@ __global__ __launch_bounds__(256) void k_batch_sweep(const BatchCase<T>* __restr
     const int by = static_cast<int>(r % C.nby);
     const int bz = static_cast<int>(r / C.nby);
     if (IsH)
-      sweepH<T, ClassCoef<T>, true>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
+      sweepH<T, ClassCoef<T>, Fused>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
     else
-      sweepE<T, ClassCoef<T>, true>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
+      sweepE<T, ClassCoef<T>, Fused>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
   }
 }
 
@@ -161,8 +161,9 @@ __global__ void k_tick(int64_t* t) {
 
 template <class T>
 struct BatchLaunch {
+  // fused: some case applies CPML slabs inside its sweeps
   static void sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
-                    int grid);
+                    int grid, bool fused);
   static void cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n, int64_t total,
                    int grid);
   static void cpmlFaces(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,

@@ -1194,6 +1194,7 @@ struct xfd_batch {
   Table sweep, rec_e, rec_h, nf;
   std::vector<Table> cpml_e, cpml_h;            // slabs stepped by separate launches
   std::vector<Table> faces_e, faces_h;          // far planes of slabs fused into the sweeps
+  bool any_fused = false;                       // some case applies slabs inside its sweeps
   int64_t pool_rows = INT64_MAX;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -1243,6 +1244,7 @@ struct Collect {
 // Builds the entry tables of every batched launch for the current ichunk.
 template <class T>
 void batchTables(xfd_batch& b) {
+  b.any_fused = false;
   Collect<xfd::BatchCase<T>> sw;
   std::vector<Collect<xfd::BatchSlab<T>>> ce, ch, fe, fh;
   Collect<xfd::BatchRec<T>> re, rh;
@@ -1258,6 +1260,7 @@ void batchTables(xfd_batch& b) {
       bc.K.tab[q] = c.tabs[q].as<T>();
     }
     bc.ichunk = b.ichunk;
+    b.any_fused = b.any_fused || c.fused;
     if (c.fused) {
       bc.ph = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), c.n_fused};
       bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
@@ -1320,7 +1323,7 @@ void batchEnqueueStep(xfd_batch& b) {
   const int64_t* t = b.tdev.as<int64_t>();
   auto sweep = [&](bool is_h) {
     L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n, b.sweep.total,
-             b.grid);
+             b.grid, b.any_fused);
   };
   auto records = [&](xfd_batch::Table& r) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
@@ -1398,7 +1401,7 @@ void batchTune(xfd_batch& b, int reps) {
       auto pair = [&] {
         for (bool is_h : {true, false})
           L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
-                   b.sweep.total, b.grid);
+                   b.sweep.total, b.grid, b.any_fused);
       };
       pair();  // warm-up
       XFD_HIP(hipEventRecord(b.ev0, b.stream));

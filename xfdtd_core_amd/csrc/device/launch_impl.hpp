@@ -160,14 +160,18 @@ void Launch<T>::pack(hipStream_t s, const Grid& g, const T* src, double* dst, in
 // ------------------------------------------------------------- batched --
 template <class T>
 void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n,
-                           int64_t total, int grid) {
+                           int64_t total, int grid, bool fused) {
   if (total <= 0) return;
   dim3 block(kLanesK, kRowsJ);
   dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
-  if (is_h)
-    hipLaunchKernelGGL((k_batch_sweep<T, true>), g, block, 0, s, cases, start, n, total);
+  if (is_h && fused)
+    hipLaunchKernelGGL((k_batch_sweep<T, true, true>), g, block, 0, s, cases, start, n, total);
+  else if (is_h)
+    hipLaunchKernelGGL((k_batch_sweep<T, true, false>), g, block, 0, s, cases, start, n, total);
+  else if (fused)
+    hipLaunchKernelGGL((k_batch_sweep<T, false, true>), g, block, 0, s, cases, start, n, total);
   else
-    hipLaunchKernelGGL((k_batch_sweep<T, false>), g, block, 0, s, cases, start, n, total);
+    hipLaunchKernelGGL((k_batch_sweep<T, false, false>), g, block, 0, s, cases, start, n, total);
 }
 template <class T>
 void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,
