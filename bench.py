@@ -244,9 +244,27 @@ def bench_design_sweep(args):
         "batch": {"shared_launches": info["shared"], "ichunk": info["ichunk"], "grid": info["grid"],
                   "device_ms_per_step": round(info["device_ms"] / args.steps, 4)},
     }
+    batch.close()
+    del batch
+    if args.sequential:
+        # the same cases stepped one context at a time (each its own launches)
+        sub = sims[: args.sequential]
+        cells_sub = sum(int(np.prod(s_.grid())) for s_ in sub)
+        n_seq = min(args.steps, 100)
+        t_seq = 0.0
+        for s_ in sub:
+            s_.init(n_seq)
+            native.device_synchronize()
+            t0 = time.perf_counter()
+            s_.advance(0, n_seq)
+            native.device_synchronize()
+            t_seq += time.perf_counter() - t0
+        seq = cells_sub * n_seq / t_seq / 1e6
+        log(f"design sweep: the first {len(sub)} cases one context at a time: {seq:.0f} Mcells/s")
+        line["sequential_single_contexts"] = {"cases": len(sub), "steps": n_seq, "value": round(seq, 1),
+                                              "unit": "Mcells/s", "batch_speedup": round(value / seq, 2)}
     for s_ in sims:
         s_.close()
-    batch.close()
     return line
 
 
@@ -313,6 +331,8 @@ def main():
     ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint"], default="box",
                     help="box: the 384^3 headline; design_sweep: the batched form; adjoint: the gradient form")
     ap.add_argument("--cases", type=int, default=96)
+    ap.add_argument("--sequential", type=int, default=24,
+                    help="design_sweep: also time this many cases stepped one context at a time")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-f64", action="store_true")
     ap.add_argument("--fd-cells", type=int, default=0)
