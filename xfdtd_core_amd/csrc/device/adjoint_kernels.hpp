@@ -69,7 +69,7 @@ __device__ __forceinline__ T coefB(const Coef& C, int q, int64_t idx) {
 // Ey(e) = .. + a_ey (Hz(e) - Hz(e-i)) + b_ey (Hx(e) - Hx(e-k)),
 // Ez(e) = .. + a_ez (Hx(e) - Hx(e-j)) + b_ez (Hy(e) - Hy(e-i)).
 template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_adj_h(Grid g, FieldPtrs<T> L, Coef C) {
+__global__ __launch_bounds__(kTileThreads) void k_adj_h(Grid g, FieldPtrs<T> L, Coef C) {
   const int k = blockIdx.x * kLanesK + threadIdx.x;
   const int j = blockIdx.y * kRowsJ + threadIdx.y;
   const int i = blockIdx.z;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_adj_h(Grid g, FieldPtrs<T> L, Coef C) {
 // Hy(m) = .. + a_hy (Ez(m+i) - Ez(m)) + b_hy (Ex(m+k) - Ex(m)),
 // Hz(m) = .. + a_hz (Ex(m+j) - Ex(m)) + b_hz (Ey(m+i) - Ey(m)).
 template <class T, class Coef>
-__global__ __launch_bounds__(256) void k_adj_e(Grid g, FieldPtrs<T> L, Coef C) {
+__global__ __launch_bounds__(kTileThreads) void k_adj_e(Grid g, FieldPtrs<T> L, Coef C) {
   const int k = blockIdx.x * kLanesK + threadIdx.x;
   const int j = blockIdx.y * kRowsJ + threadIdx.y;
   const int i = blockIdx.z;
@@ -267,7 +267,7 @@ __device__ __forceinline__ int64_t designIndex(const Grid& g, const DesignArgs& 
 
 // Copies the design edges' E values into history slot `slot`.
 template <class T>
-__global__ __launch_bounds__(256) void k_adj_hist(Grid g, FieldPtrs<T> F, DesignArgs d, T* hist) {
+__global__ __launch_bounds__(kTileThreads) void k_adj_hist(Grid g, FieldPtrs<T> F, DesignArgs d, T* hist) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= d.off[3]) return;
   int q;

@@ -28,6 +28,12 @@ __device__ __forceinline__ int findEntry(const int64_t* __restrict__ start, int 
   return lo;
 }
 
+// Work-items of an entry per global work-group id for the point-wise batched
+// launches (CPML, far planes, records, DFT): one entry lookup serves this many
+// 256-lane passes.
+constexpr int kBatchPasses = 4;
+constexpr int64_t kBatchGroupItems = 256 * kBatchPasses;
+
 template <class T>
 struct BatchCase {
   Grid g;
@@ -62,7 +68,7 @@ struct BatchNf2ff {
 };
 
 template <class T, bool IsH, bool Fused>
-__global__ __launch_bounds__(256) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
+__global__ __launch_bounds__(kTileThreads) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
                                                      const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
@@ -85,7 +91,8 @@ __global__ __launch_bounds__(256) void k_batch_cpml(const BatchSlab<T>* __restri
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
     const BatchSlab<T>& S = slabs[e];
-    cpmlCell<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
+    const int64_t base = (b - start[e]) * kBatchGroupItems + threadIdx.x;
+    for (int r = 0; r < kBatchPasses; ++r) cpmlCell<T, IsE>(S.g, S.F, S.s, base + r * 256);
   }
 }
 
@@ -96,7 +103,8 @@ __global__ __launch_bounds__(256) void k_batch_cpml_faces(const BatchSlab<T>* __
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
     const BatchSlab<T>& S = slabs[e];
-    cpmlFacePoint<T, IsE>(S.g, S.F, S.s, (b - start[e]) * blockDim.x + threadIdx.x);
+    const int64_t base = (b - start[e]) * kBatchGroupItems + threadIdx.x;
+    for (int r = 0; r < kBatchPasses; ++r) cpmlFacePoint<T, IsE>(S.g, S.F, S.s, base + r * 256);
   }
 }
 
@@ -108,7 +116,8 @@ __global__ __launch_bounds__(256) void k_batch_records(const BatchRec<T>* __rest
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
     const BatchRec<T>& R = recs[e];
-    recordSeg(R.F, R.rec, R.seg, R.nseg, R.pool, R.state, (b - start[e]) * blockDim.x + threadIdx.x, t);
+    const int64_t base = (b - start[e]) * kBatchGroupItems + threadIdx.x;
+    for (int r = 0; r < kBatchPasses; ++r) recordSeg(R.F, R.rec, R.seg, R.nseg, R.pool, R.state, base + r * 256, t);
   }
 }
 
@@ -120,7 +129,8 @@ __global__ __launch_bounds__(256) void k_batch_nf2ff(const BatchNf2ff<T>* __rest
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
     const BatchNf2ff<T>& N = nfs[e];
-    nf2ffCell(N.g, N.F, N.a, (b - start[e]) * blockDim.x + threadIdx.x, t);
+    const int64_t base = (b - start[e]) * kBatchGroupItems + threadIdx.x;
+    for (int r = 0; r < kBatchPasses; ++r) nf2ffCell(N.g, N.F, N.a, base + r * 256, t);
   }
 }
 

@@ -1241,6 +1241,9 @@ struct Collect {
   void to(xfd_batch::Table& t) const { fillTable(t, v, start, total); }
 };
 
+// Global work-group ids an entry of `items` point-wise work-items owns.
+int64_t groupsOf(int64_t items) { return (items + xfd::kBatchGroupItems - 1) / xfd::kBatchGroupItems; }
+
 // Builds the entry tables of every batched launch for the current ichunk.
 template <class T>
 void batchTables(xfd_batch& b) {
@@ -1282,20 +1285,20 @@ void batchTables(xfd_batch& b) {
         e.s.stride_c = stride[e.s.axis];
         if (static_cast<int>(o) < c.n_fused) {  // only the far-plane remainder
           const auto fp = xfd::facePieces(c.g, e.s);
-          (is_e ? fe : fh)[o].add(e, (fp.nX + fp.nY + fp.nZ + 255) / 256);
+          (is_e ? fe : fh)[o].add(e, groupsOf(fp.nX + fp.nY + fp.nZ));
         } else {
           const int64_t vol = static_cast<int64_t>(e.s.ext[0]) * e.s.ext[1] * e.s.ext[2];
-          (is_e ? ce : ch)[o].add(e, (vol + 255) / 256);
+          (is_e ? ce : ch)[o].add(e, groupsOf(vol));
         }
       }
     }
     if (c.nseg_h)
       rh.add({F, c.rec_h.as<xfd::Record>(), c.seg_h.as<int64_t>(), c.nseg_h, c.pool.as<double>(), c.state.as<T>()},
-             (c.nseg_h + 255) / 256);
+             groupsOf(c.nseg_h));
     if (c.nseg_e)
       re.add({F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e, c.pool.as<double>(), c.state.as<T>()},
-             (c.nseg_e + 255) / 256);
-    for (auto& n : c.nf2ffs) nf.add({c.g, F, n->a}, (n->a.off[6] + 255) / 256);
+             groupsOf(c.nseg_e));
+    for (auto& n : c.nf2ffs) nf.add({c.g, F, n->a}, groupsOf(n->a.off[6]));
     b.pool_rows = std::min(b.pool_rows, c.pool_rows);
   }
   sw.to(b.sweep);

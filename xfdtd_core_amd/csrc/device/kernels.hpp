@@ -19,8 +19,18 @@
 
 namespace xfd {
 
-constexpr int kLanesK = 64;
-constexpr int kRowsJ = 4;
+// Sweep tile: kLanesK k-lanes (a multiple of the 64-wide wavefront) by
+// kRowsJ j-rows; XFD_TILE_K / XFD_TILE_J override them at build time.
+#ifndef XFD_TILE_K
+#define XFD_TILE_K 64
+#endif
+#ifndef XFD_TILE_J
+#define XFD_TILE_J 4
+#endif
+constexpr int kLanesK = XFD_TILE_K;
+constexpr int kRowsJ = XFD_TILE_J;
+constexpr int kTileThreads = kLanesK * kRowsJ;
+static_assert(kLanesK % 64 == 0 && kTileThreads <= 1024, "sweep tile");
 
 // The CPML corrections of one point, fused into a Yee sweep: for every slab
 // of P containing the point (in boundary order), for its components A =
@@ -141,7 +151,7 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
 }
 
 template <class T, class Coef, bool Fused>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
+__global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
                                                   CpmlList<T> P) {
   int bx, by, bz;
   M.coords(bx, by, bz);
@@ -223,7 +233,7 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
 }
 
 template <class T, class Coef, bool Fused>
-__global__ __launch_bounds__(256) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
+__global__ __launch_bounds__(kTileThreads) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
                                                   CpmlList<T> P) {
   int bx, by, bz;
   M.coords(bx, by, bz);
@@ -434,7 +444,7 @@ __device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, 
 }
 
 template <class T, class Coef, int S>
-__global__ __launch_bounds__(256) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
+__global__ __launch_bounds__(kTileThreads) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
                                                       int ichunk) {
   int bx, by, bz;
   M.coords(bx, by, bz);
