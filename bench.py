@@ -42,22 +42,22 @@ def log(msg):
 
 
 def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: bool = False) -> float:
-    """Algorithmic HBM bytes of one launch of `kernel` on an n^3 grid.
-
-    update_h / update_e: per cell 3 own-field reads + 3 writes, 3 dual-field
-    reads (neighbours reused) and 9 coefficient reads = 18 * s bytes; with the
-    CPML applied in the same sweep, plus per slab cell and component psi read
-    + write and c_psi read (3 * s, two components per slab, six slabs).
-    cpml_h / cpml_e as separate launches (one slab): per slab cell and
-    component psi read + write, c_psi read, field read + write and one new
-    dual-field read = 6 * s, two components per slab.  With fusion these
+    """Algorithmic HBM bytes of one launch of `kernel` on an n^3 grid, the
+    per-cell-step figures of the leapfrog step split over its launches:
+    B_yee = (6 field reads + 6 field writes + 18 coefficient reads) * s per
+    cell and step, so each of the two Yee sweeps owns 15 * s per cell (its
+    own 3 reads, 3 writes and 9 coefficients; the 3 dual-field reads a
+    separate sweep adds are actual, not algorithmic, traffic); CPML = 12 * s
+    per slab cell and step (4 components x psi read + write + c_psi read), so
+    one slab launch (2 components) owns 6 * s per slab cell.  With the CPML
+    inside the sweeps the sweep also owns its slabs' share and the separate
     launches only finish the far grid planes (not counted)."""
     cells = n ** 3
     slab_cells = pml * (n + 1) * (n + 1)
     if kernel in ("update_h", "update_e"):
-        return 18.0 * s * cells + (2 * 3.0 * s * 6 * slab_cells if fused else 0.0)
+        return 15.0 * s * cells + (6.0 * s * 6 * slab_cells if fused else 0.0)
     if kernel in ("cpml_h", "cpml_e"):
-        return 0.0 if fused else 2 * 6.0 * s * slab_cells
+        return 0.0 if fused else 6.0 * s * slab_cells
     return 0.0
 
 
@@ -75,9 +75,9 @@ def measured_traffic(precision, n, pml):
 
 
 def step_alg_bytes(n: int, pml: int, s: int) -> float:
-    """Compulsory bytes of one whole step: fused Yee (6 field reads, 6 writes,
-    18 coefficient reads per cell = 30 s) plus CPML (12 s per slab cell per
-    slab: 4 components x psi read/write + c_psi read)."""
+    """Algorithmic bytes of one whole step: B_yee = 30 s per cell (6 field
+    reads, 6 writes, 18 coefficient reads) plus CPML 12 s per slab cell and
+    slab (4 components x psi read/write + c_psi read)."""
     return 30.0 * s * n ** 3 + 12.0 * s * 6 * pml * n * n
 
 
@@ -119,11 +119,16 @@ def run_device(sim, precision, warmup, steps, n, pml):
         if traffic and name in traffic:
             k["hbm_bytes_pmc"] = traffic[name]
             k["hbm_GBps_pmc"] = round(traffic[name] / (k["avg_ms"] * 1e-3) / 1e9, 1)
+    step_b = step_alg_bytes(n, pml, s)
+    step_gbps = step_b * steps / wall / 1e9
     res = {
         "mcells_per_s": cells * steps / wall / 1e6,
         "ms_per_step": wall / steps * 1e3,
         "init_s": round(init_s, 2),
-        "step_alg_GBps": round(step_alg_bytes(n, pml, s) * steps / wall / 1e9, 1),
+        "step_alg_GBps": round(step_gbps, 1),
+        "step_roofline": {"alg_bytes_per_step": step_b, "achieved_GBps": round(step_gbps, 1),
+                          "frac": round(step_gbps / PEAK_HBM_GBS, 4),
+                          "mcells_at_peak": round(cells / (step_b / (PEAK_HBM_GBS * 1e9)) / 1e6, 1)},
         "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(ach / PEAK_HBM_GBS, 4) if ach else None,
@@ -382,6 +387,7 @@ def main():
                                f"Ez current-source dipole, CFL 0.95, snapshots every 500 steps + 32 Ez probes",
                    "cells": args.n ** 3, "parallelism": "single domain, 1 GPU", "device": dev_name},
         "hbm_GBps_alg_step": head["step_alg_GBps"],
+        "step_roofline": head["step_roofline"],
         "roofline": head["roofline"],
         "coefficients": head["coefficients"],
         "kernels": head["kernels"],
@@ -391,6 +397,7 @@ def main():
         f64 = out["f64"]
         line["fp64"] = {"value": round(f64["mcells_per_s"], 1), "unit": "Mcells/s",
                         "ms_per_step": round(f64["ms_per_step"], 4), "hbm_GBps_alg_step": f64["step_alg_GBps"],
+                        "step_roofline": f64["step_roofline"],
                         "roofline": f64["roofline"], "kernels": f64["kernels"]}
         line["fp32_over_fp64"] = round(out["f32"]["mcells_per_s"] / f64["mcells_per_s"], 3)
     for r in out.values():
