@@ -68,69 +68,59 @@ __device__ __forceinline__ T coefB(const Coef& C, int q, int64_t idx) {
 // Forward E reads (sweepE): Ex(e) = .. + a_ex (Hy(e) - Hy(e-k)) + b_ex (Hz(e) - Hz(e-j)),
 // Ey(e) = .. + a_ey (Hz(e) - Hz(e-i)) + b_ey (Hx(e) - Hx(e-k)),
 // Ez(e) = .. + a_ez (Hx(e) - Hx(e-j)) + b_ez (Hy(e) - Hy(e-i)).
-// Like the forward sweeps, a thread marches a chunk of i-planes; the
-// (coefficient x lambda) products of the i+1 plane are carried to the next
-// plane.  Points cover the whole padded box [0,nx] x [0,ny] x [0,nz].
 template <class T, class Coef>
-__global__ __launch_bounds__(kTileThreads) void k_adj_h(Grid g, FieldPtrs<T> L, Coef C, int ichunk) {
+__global__ __launch_bounds__(kTileThreads) void k_adj_h(Grid g, FieldPtrs<T> L, Coef C) {
   const int k = blockIdx.x * kLanesK + threadIdx.x;
   const int j = blockIdx.y * kRowsJ + threadIdx.y;
-  const int i0 = blockIdx.z * ichunk;
-  if (k > g.nz || j > g.ny || i0 > g.nx) return;
-  const int i1 = min(i0 + ichunk, g.nx + 1);
+  const int i = blockIdx.z;
+  if (k > g.nz || j > g.ny || i > g.nx) return;
   const int64_t pz = g.pz, pl = g.plane;
-  int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  const int64_t idx = (static_cast<int64_t>(i) * g.NY + j) * pz + k;
   const T* ex = L.f[0];
   const T* ey = L.f[1];
   const T* ez = L.f[2];
-  // products of plane i that the i-neighbour terms need: b_ez lambdaEz, a_ey lambdaEy
-  T pez = inRE(g, 2, i0, j, k) ? coefB<T>(C, 2, idx) * ez[idx] : T(0);
-  T qey = inRE(g, 1, i0, j, k) ? coefA<T>(C, 1, idx) * ey[idx] : T(0);
-  for (int i = i0; i < i1; ++i, idx += pl) {
-    const T pez_n = inRE(g, 2, i + 1, j, k) ? coefB<T>(C, 2, idx + pl) * ez[idx + pl] : T(0);
-    const T qey_n = inRE(g, 1, i + 1, j, k) ? coefA<T>(C, 1, idx + pl) * ey[idx + pl] : T(0);
-    const bool rh = inRH(g, i, j, k);
-    if (inExt(g, true, 0, i, j, k)) {  // Hx
-      T v = L.f[3][idx];
-      if (rh) {
-        T s, a, b;
-        C.get(3, idx, s, a, b);
-        v = s * v;
-      }
-      if (inRE(g, 1, i, j, k)) v += coefB<T>(C, 1, idx) * ey[idx];
-      if (inRE(g, 1, i, j, k + 1)) v -= coefB<T>(C, 1, idx + 1) * ey[idx + 1];
-      if (inRE(g, 2, i, j, k)) v += coefA<T>(C, 2, idx) * ez[idx];
-      if (inRE(g, 2, i, j + 1, k)) v -= coefA<T>(C, 2, idx + pz) * ez[idx + pz];
-      L.f[3][idx] = v;
+  const bool rh = inRH(g, i, j, k);
+  // Hx
+  if (inExt(g, true, 0, i, j, k)) {
+    T v = L.f[3][idx];
+    if (rh) {
+      T s, a, b;
+      C.get(3, idx, s, a, b);
+      v = s * v;
     }
-    if (inExt(g, true, 1, i, j, k)) {  // Hy
-      T v = L.f[4][idx];
-      if (rh) {
-        T s, a, b;
-        C.get(4, idx, s, a, b);
-        v = s * v;
-      }
-      if (inRE(g, 0, i, j, k)) v += coefA<T>(C, 0, idx) * ex[idx];
-      if (inRE(g, 0, i, j, k + 1)) v -= coefA<T>(C, 0, idx + 1) * ex[idx + 1];
-      v += pez;
-      v -= pez_n;
-      L.f[4][idx] = v;
+    if (inRE(g, 1, i, j, k)) v += coefB<T>(C, 1, idx) * ey[idx];
+    if (inRE(g, 1, i, j, k + 1)) v -= coefB<T>(C, 1, idx + 1) * ey[idx + 1];
+    if (inRE(g, 2, i, j, k)) v += coefA<T>(C, 2, idx) * ez[idx];
+    if (inRE(g, 2, i, j + 1, k)) v -= coefA<T>(C, 2, idx + pz) * ez[idx + pz];
+    L.f[3][idx] = v;
+  }
+  // Hy
+  if (inExt(g, true, 1, i, j, k)) {
+    T v = L.f[4][idx];
+    if (rh) {
+      T s, a, b;
+      C.get(4, idx, s, a, b);
+      v = s * v;
     }
-    if (inExt(g, true, 2, i, j, k)) {  // Hz
-      T v = L.f[5][idx];
-      if (rh) {
-        T s, a, b;
-        C.get(5, idx, s, a, b);
-        v = s * v;
-      }
-      if (inRE(g, 0, i, j, k)) v += coefB<T>(C, 0, idx) * ex[idx];
-      if (inRE(g, 0, i, j + 1, k)) v -= coefB<T>(C, 0, idx + pz) * ex[idx + pz];
-      v += qey;
-      v -= qey_n;
-      L.f[5][idx] = v;
+    if (inRE(g, 0, i, j, k)) v += coefA<T>(C, 0, idx) * ex[idx];
+    if (inRE(g, 0, i, j, k + 1)) v -= coefA<T>(C, 0, idx + 1) * ex[idx + 1];
+    if (inRE(g, 2, i, j, k)) v += coefB<T>(C, 2, idx) * ez[idx];
+    if (inRE(g, 2, i + 1, j, k)) v -= coefB<T>(C, 2, idx + pl) * ez[idx + pl];
+    L.f[4][idx] = v;
+  }
+  // Hz
+  if (inExt(g, true, 2, i, j, k)) {
+    T v = L.f[5][idx];
+    if (rh) {
+      T s, a, b;
+      C.get(5, idx, s, a, b);
+      v = s * v;
     }
-    pez = pez_n;
-    qey = qey_n;
+    if (inRE(g, 0, i, j, k)) v += coefB<T>(C, 0, idx) * ex[idx];
+    if (inRE(g, 0, i, j + 1, k)) v -= coefB<T>(C, 0, idx + pz) * ex[idx + pz];
+    if (inRE(g, 1, i, j, k)) v += coefA<T>(C, 1, idx) * ey[idx];
+    if (inRE(g, 1, i + 1, j, k)) v -= coefA<T>(C, 1, idx + pl) * ey[idx + pl];
+    L.f[5][idx] = v;
   }
 }
 
@@ -141,63 +131,55 @@ __global__ __launch_bounds__(kTileThreads) void k_adj_h(Grid g, FieldPtrs<T> L, 
 // Forward H reads (sweepH): Hx(m) = .. + a_hx (Ey(m+k) - Ey(m)) + b_hx (Ez(m+j) - Ez(m)),
 // Hy(m) = .. + a_hy (Ez(m+i) - Ez(m)) + b_hy (Ex(m+k) - Ex(m)),
 // Hz(m) = .. + a_hz (Ex(m+j) - Ex(m)) + b_hz (Ey(m+i) - Ey(m)).
-// The i-1 plane's products b_hz lambdaHz and a_hy lambdaHy are carried.
 template <class T, class Coef>
-__global__ __launch_bounds__(kTileThreads) void k_adj_e(Grid g, FieldPtrs<T> L, Coef C, int ichunk) {
+__global__ __launch_bounds__(kTileThreads) void k_adj_e(Grid g, FieldPtrs<T> L, Coef C) {
   const int k = blockIdx.x * kLanesK + threadIdx.x;
   const int j = blockIdx.y * kRowsJ + threadIdx.y;
-  const int i0 = blockIdx.z * ichunk;
-  if (k > g.nz || j > g.ny || i0 > g.nx) return;
-  const int i1 = min(i0 + ichunk, g.nx + 1);
+  const int i = blockIdx.z;
+  if (k > g.nz || j > g.ny || i > g.nx) return;
   const int64_t pz = g.pz, pl = g.plane;
-  int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  const int64_t idx = (static_cast<int64_t>(i) * g.NY + j) * pz + k;
   const T* hx = L.f[3];
   const T* hy = L.f[4];
   const T* hz = L.f[5];
-  T rhz_p = inRH(g, i0 - 1, j, k) ? coefB<T>(C, 5, idx - pl) * hz[idx - pl] : T(0);
-  T shy_p = inRH(g, i0 - 1, j, k) ? coefA<T>(C, 4, idx - pl) * hy[idx - pl] : T(0);
-  for (int i = i0; i < i1; ++i, idx += pl) {
-    const bool rh = inRH(g, i, j, k);
-    const T rhz = rh ? coefB<T>(C, 5, idx) * hz[idx] : T(0);
-    const T shy = rh ? coefA<T>(C, 4, idx) * hy[idx] : T(0);
-    if (inExt(g, false, 0, i, j, k)) {  // Ex
-      T v = L.f[0][idx];
-      if (inRE(g, 0, i, j, k)) {
-        T s, a, b;
-        C.get(0, idx, s, a, b);
-        v = s * v;
-      }
-      if (rh) v -= coefB<T>(C, 4, idx) * hy[idx] + coefA<T>(C, 5, idx) * hz[idx];
-      if (inRH(g, i, j, k - 1)) v += coefB<T>(C, 4, idx - 1) * hy[idx - 1];
-      if (inRH(g, i, j - 1, k)) v += coefA<T>(C, 5, idx - pz) * hz[idx - pz];
-      L.f[0][idx] = v;
+  // Ex
+  if (inExt(g, false, 0, i, j, k)) {
+    T v = L.f[0][idx];
+    if (inRE(g, 0, i, j, k)) {
+      T s, a, b;
+      C.get(0, idx, s, a, b);
+      v = s * v;
     }
-    if (inExt(g, false, 1, i, j, k)) {  // Ey
-      T v = L.f[1][idx];
-      if (inRE(g, 1, i, j, k)) {
-        T s, a, b;
-        C.get(1, idx, s, a, b);
-        v = s * v;
-      }
-      if (rh) v -= coefA<T>(C, 3, idx) * hx[idx] + rhz;
-      if (inRH(g, i, j, k - 1)) v += coefA<T>(C, 3, idx - 1) * hx[idx - 1];
-      v += rhz_p;
-      L.f[1][idx] = v;
+    if (inRH(g, i, j, k)) v -= coefB<T>(C, 4, idx) * hy[idx] + coefA<T>(C, 5, idx) * hz[idx];
+    if (inRH(g, i, j, k - 1)) v += coefB<T>(C, 4, idx - 1) * hy[idx - 1];
+    if (inRH(g, i, j - 1, k)) v += coefA<T>(C, 5, idx - pz) * hz[idx - pz];
+    L.f[0][idx] = v;
+  }
+  // Ey
+  if (inExt(g, false, 1, i, j, k)) {
+    T v = L.f[1][idx];
+    if (inRE(g, 1, i, j, k)) {
+      T s, a, b;
+      C.get(1, idx, s, a, b);
+      v = s * v;
     }
-    if (inExt(g, false, 2, i, j, k)) {  // Ez
-      T v = L.f[2][idx];
-      if (inRE(g, 2, i, j, k)) {
-        T s, a, b;
-        C.get(2, idx, s, a, b);
-        v = s * v;
-      }
-      if (rh) v -= coefB<T>(C, 3, idx) * hx[idx] + shy;
-      if (inRH(g, i, j - 1, k)) v += coefB<T>(C, 3, idx - pz) * hx[idx - pz];
-      v += shy_p;
-      L.f[2][idx] = v;
+    if (inRH(g, i, j, k)) v -= coefA<T>(C, 3, idx) * hx[idx] + coefB<T>(C, 5, idx) * hz[idx];
+    if (inRH(g, i, j, k - 1)) v += coefA<T>(C, 3, idx - 1) * hx[idx - 1];
+    if (inRH(g, i - 1, j, k)) v += coefB<T>(C, 5, idx - pl) * hz[idx - pl];
+    L.f[1][idx] = v;
+  }
+  // Ez
+  if (inExt(g, false, 2, i, j, k)) {
+    T v = L.f[2][idx];
+    if (inRE(g, 2, i, j, k)) {
+      T s, a, b;
+      C.get(2, idx, s, a, b);
+      v = s * v;
     }
-    rhz_p = rhz;
-    shy_p = shy;
+    if (inRH(g, i, j, k)) v -= coefB<T>(C, 3, idx) * hx[idx] + coefA<T>(C, 4, idx) * hy[idx];
+    if (inRH(g, i, j - 1, k)) v += coefB<T>(C, 3, idx - pz) * hx[idx - pz];
+    if (inRH(g, i - 1, j, k)) v += coefA<T>(C, 4, idx - pl) * hy[idx - pl];
+    L.f[2][idx] = v;
   }
 }
 
@@ -324,10 +306,10 @@ __global__ __launch_bounds__(64) void k_adj_source(FieldPtrs<T> L, const AdjPoin
 
 template <class T>
 struct AdjLaunch {
-  static void adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk);
-  static void adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk);
-  static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk);
-  static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk);
+  static void adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C);
+  static void adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C);
+  static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C);
+  static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C);
   static void cpmlMu(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a);
   static void cpmlGather(hipStream_t s, bool is_e, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a);
   static void hist(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const DesignArgs& d, T* hist);

@@ -1150,15 +1150,15 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
               grad.as<double>());
       for (auto& a : me) A::cpmlMu(st, c.g, L, a);
       if (c.classed)
-        A::adjH(st, c.g, L, K, c.ichunk);
+        A::adjH(st, c.g, L, K);
       else
-        A::adjH(st, c.g, L, C, c.ichunk);
+        A::adjH(st, c.g, L, C);
       for (auto& a : me) A::cpmlGather(st, true, c.g, L, a);
       for (auto& a : mh) A::cpmlMu(st, c.g, L, a);
       if (c.classed)
-        A::adjE(st, c.g, L, K, c.ichunk);
+        A::adjE(st, c.g, L, K);
       else
-        A::adjE(st, c.g, L, C, c.ichunk);
+        A::adjE(st, c.g, L, C);
       for (auto& a : mh) A::cpmlGather(st, false, c.g, L, a);
       XFD_HIP(hipGetLastError());
     }

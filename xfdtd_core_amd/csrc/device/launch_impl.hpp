@@ -226,24 +226,24 @@ void BatchLaunch<T>::tick(hipStream_t s, int64_t* t) {
 }
 
 // ---------------------------------------------------------------- adjoint --
-inline dim3 adjGrid(const Grid& g, int ichunk) {
-  return dim3(blocks_for(g.nz + 1, kLanesK), blocks_for(g.ny + 1, kRowsJ), blocks_for(g.nx + 1, ichunk));
+inline dim3 adjGrid(const Grid& g) {
+  return dim3(blocks_for(g.nz + 1, kLanesK), blocks_for(g.ny + 1, kRowsJ), static_cast<unsigned>(g.nx + 1));
 }
 template <class T>
-void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk) {
-  hipLaunchKernelGGL((k_adj_h<T, CoefPtrs<T>>), adjGrid(g, ichunk), dim3(kLanesK, kRowsJ), 0, s, g, L, C, ichunk);
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C) {
+  hipLaunchKernelGGL((k_adj_h<T, CoefPtrs<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
 }
 template <class T>
-void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk) {
-  hipLaunchKernelGGL((k_adj_h<T, ClassCoef<T>>), adjGrid(g, ichunk), dim3(kLanesK, kRowsJ), 0, s, g, L, C, ichunk);
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C) {
+  hipLaunchKernelGGL((k_adj_h<T, ClassCoef<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
 }
 template <class T>
-void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk) {
-  hipLaunchKernelGGL((k_adj_e<T, CoefPtrs<T>>), adjGrid(g, ichunk), dim3(kLanesK, kRowsJ), 0, s, g, L, C, ichunk);
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C) {
+  hipLaunchKernelGGL((k_adj_e<T, CoefPtrs<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
 }
 template <class T>
-void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk) {
-  hipLaunchKernelGGL((k_adj_e<T, ClassCoef<T>>), adjGrid(g, ichunk), dim3(kLanesK, kRowsJ), 0, s, g, L, C, ichunk);
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C) {
+  hipLaunchKernelGGL((k_adj_e<T, ClassCoef<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
 }
 template <class T>
 void AdjLaunch<T>::cpmlMu(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a) {
