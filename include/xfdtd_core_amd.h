@@ -185,6 +185,20 @@ int xfd_read_gather(xfd_ctx* ctx, int32_t g, double* dst);
  * as interleaved (re, im) doubles in the surface's (i, j, k) order. */
 int xfd_read_nf2ff(xfd_ctx* ctx, int32_t n, int32_t f, int32_t d, int32_t which, double* dst);
 
+/* Far-field potentials of NF2FF `n`, frequency `f`, from its accumulated
+ * surface currents, on the device (FDPlaneData::getPotential /
+ * calculatePotential, nffft_fd_data_process.cpp:9-26, 44-160): for each of
+ * the `na` directions (theta[q], phi[q]) the sums over all surface cells of
+ * (cur_a t_a + cur_b t_b) exp(j k r.rhat) ds for A (ja, jb) and F (ma, mb),
+ * with the theta and the phi rows of the transform (transform.h:24-58).
+ * k = 2 pi f / c.  e_node[a] (N_a + 1 values), h_node[a] and e_size[a] (N_a
+ * values each) are the grid's node coordinates, cell centres and cell sizes
+ * (GridSpace).  dst: na x 4 complex (A_theta, A_phi, F_theta, F_phi) as
+ * interleaved (re, im) doubles. */
+int xfd_nf2ff_far_field(xfd_ctx* ctx, int32_t n, int32_t f, int32_t na, const double* theta, const double* phi,
+                        const double* origin, double k, const double* const* e_node, const double* const* h_node,
+                        const double* const* e_size, double* dst);
+
 /* Kernel timing with HIP events on the library's stream.  When enabled,
  * xfd_step records an event pair around every launch; xfd_kernel_stats then
  * reports, for kernel class `k`, its name, launch count and summed time. */

@@ -443,8 +443,8 @@ class FieldTimeMonitor : public Monitor {
 
 // Frequency-domain near-to-far-field transform (nffft_frequency_domain.h):
 // a running DFT of the tangential fields on the box `distance` cells inside
-// the grid boundary, run on the GPU every step; the far-field potentials and
-// the radiated power are formed on the host after the run.
+// the grid boundary, run on the GPU every step; after the run the far-field
+// potentials are summed on the GPU and the radiated power on the host.
 class NFFFTFrequencyDomain {
  public:
   NFFFTFrequencyDomain(std::size_t distance_x, std::size_t distance_y, std::size_t distance_z,
@@ -477,6 +477,14 @@ class NFFFTFrequencyDomain {
   std::vector<std::complex<Real>> potential(std::size_t f, Potential p, bool theta_component,
                                             const std::vector<Real>& theta, const std::vector<Real>& phi,
                                             const Vector& origin = Vector{0.0, 0.0, 0.0}) const;
+  // A_theta, A_phi, F_theta, F_phi over the same directions in one pass.
+  std::array<std::vector<std::complex<Real>>, 4> potentials(std::size_t f, const std::vector<Real>& theta,
+                                                            const std::vector<Real>& phi,
+                                                            const Vector& origin = Vector{0.0, 0.0, 0.0}) const;
+  // True while the currents live on a device context: the potentials are
+  // then summed there (xfd_nf2ff_far_field); XFD_FARFIELD=host forces the
+  // host sums.
+  bool deviceFarField() const;
   Real power(std::size_t f) const;
   // Surface d (XN XP YN YP ZN ZP): index box [lo, hi) (empty when unused)
   // and the accumulated currents, which = 0 ja, 1 jb, 2 ma, 3 mb.

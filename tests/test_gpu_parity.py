@@ -329,3 +329,27 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
                                                         lumped=False).spec(), precision)
     disp.init(2)
     assert not disp.cpml_fused()
+
+
+def test_far_field_on_device_matches_host(monkeypatch):
+    """The far-field potentials summed on the GPU (xfd_nf2ff_far_field) equal
+    the host sums over the same currents, for a theta cut and a phi cut
+    (measured against the largest potential of the cut: the cross-polarised
+    ones are cancellations of much larger terms)."""
+    from test_oracle_nf2ff import _dipole
+
+    dev = native.Simulation(_dipole([6e9, 9e9]), "f64")
+    dev.run(300)
+    theta = np.linspace(0.0, np.pi, 37)
+    phi = np.linspace(0.0, 2 * np.pi, 25)
+    combos = [(a, t) for a in (True, False) for t in (True, False)]
+    for f in range(2):
+        for th, ph in ((theta, 0.7), (1.1, phi)):
+            monkeypatch.delenv("XFD_FARFIELD", raising=False)
+            g = [dev.nf2ff_potential(0, f, a, t, th, ph) for a, t in combos]
+            monkeypatch.setenv("XFD_FARFIELD", "host")
+            h = [dev.nf2ff_potential(0, f, a, t, th, ph) for a, t in combos]
+            scale = max(np.abs(x).max() for x in h)
+            assert scale > 0
+            for gi, hi in zip(g, h):
+                assert np.abs(gi - hi).max() <= 1e-12 * scale

@@ -1595,6 +1595,43 @@ int xfd_backward(xfd_ctx* c, const xfd_adjoint_desc* d, double* grad) {
   });
 }
 
+int xfd_nf2ff_far_field(xfd_ctx* c, int32_t n, int32_t f, int32_t na, const double* theta, const double* phi,
+                        const double* origin, double k, const double* const* e_node, const double* const* h_node,
+                        const double* const* e_size, double* dst) {
+  return guard([&] {
+    require(c != nullptr && dst != nullptr && theta != nullptr && phi != nullptr && origin != nullptr,
+            "bad argument");
+    require(n >= 0 && n < static_cast<int32_t>(c->nf2ffs.size()), "bad NF2FF index");
+    auto& N = *c->nf2ffs[n];
+    require(f >= 0 && f < N.a.nf, "bad frequency index");
+    require(na >= 0, "bad direction count");
+    if (na == 0) return;
+    const int dims[3] = {c->g.nx, c->g.ny, c->g.nz};
+    DevBuf geo[9], ang[2], out;
+    xfd::FarArgs fa{};
+    for (int a = 0; a < 3; ++a) {
+      require(e_node[a] && h_node[a] && e_size[a], "grid arrays missing");
+      uploadVector(std::vector<double>(e_node[a], e_node[a] + dims[a] + 1), geo[a]);
+      uploadVector(std::vector<double>(h_node[a], h_node[a] + dims[a]), geo[3 + a]);
+      uploadVector(std::vector<double>(e_size[a], e_size[a] + dims[a]), geo[6 + a]);
+      fa.e_node[a] = geo[a].as<double>();
+      fa.h_node[a] = geo[3 + a].as<double>();
+      fa.e_size[a] = geo[6 + a].as<double>();
+      fa.origin[a] = origin[a];
+    }
+    uploadVector(std::vector<double>(theta, theta + na), ang[0]);
+    uploadVector(std::vector<double>(phi, phi + na), ang[1]);
+    fa.theta = ang[0].as<double>();
+    fa.phi = ang[1].as<double>();
+    fa.k = k;
+    out.alloc(static_cast<size_t>(na) * 8 * sizeof(double));
+    xfd::Launch<double>::far_field(c->stream, N.a, f, fa, na, out.as<double>());
+    XFD_HIP(hipGetLastError());
+    XFD_HIP(hipMemcpyAsync(dst, out.p, out.bytes, hipMemcpyDeviceToHost, c->stream));
+    XFD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
 int xfd_cpml_fused(xfd_ctx* c, int32_t* fused) {
   return guard([&] {
     require(c && fused, "bad argument");

@@ -530,6 +530,90 @@ __global__ __launch_bounds__(256) void k_nf2ff(Grid g, FieldPtrs<T> F, Nf2ffArgs
   nf2ffCell(g, F, a, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, t);
 }
 
+// Far-field potentials of one frequency (FDPlaneData::calculatePotential,
+// nffft_fd_data_process.cpp:44-160, rVector / ds :196-225): work-group n
+// reduces over every surface cell
+//   P(n) += (cur_a t_a(n) + cur_b t_b(n)) exp(j k r.rhat(n)) ds
+// for A (ja, jb) and F (ma, mb), each with the theta and the phi rows of the
+// Cartesian-to-spherical transform (transform.h:24-58).
+struct FarArgs {
+  const double* e_node[3];  // per axis, N + 1 nodes
+  const double* h_node[3];  // per axis, N cell centres
+  const double* e_size[3];  // per axis, N cell sizes
+  const double* theta;
+  const double* phi;
+  double origin[3];
+  double k;                 // 2 pi f / c
+};
+
+// Templated on the field type only so that each precision's unit holds its own copy.
+template <class T>
+__global__ __launch_bounds__(256) void k_far_field(Nf2ffArgs a, int f, FarArgs fa, double* out) {
+  __shared__ double red[8][256];
+  const int n = blockIdx.x;
+  const double th = fa.theta[n], ph = fa.phi[n];
+  const double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+  const double stcp = st * cp, stsp = st * sp;
+  const double tt0 = ct * cp, tt1 = ct * sp, tt2 = -1 * st;  // theta row
+  const double tp0 = -1 * sp, tp1 = cp;                      // phi row (z entry 0)
+  const int64_t cells = a.off[6];
+  const double2* acc = a.acc + static_cast<int64_t>(f) * 4 * cells;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // A_theta, A_phi, F_theta, F_phi (re, im)
+  for (int64_t q = threadIdx.x; q < cells; q += blockDim.x) {
+    int d = 0;
+    while (q >= a.off[d + 1]) ++d;
+    const int64_t l = q - a.off[d];
+    const int e1 = a.ext[d][1], e2 = a.ext[d][2];
+    const int kk = a.lo[d][2] + static_cast<int>(l % e2);
+    const int j = a.lo[d][1] + static_cast<int>((l / e2) % e1);
+    const int i = a.lo[d][0] + static_cast<int>(l / (static_cast<int64_t>(e2) * e1));
+    const int c = d / 2;
+    double r0, r1, r2, ds;
+    if (c == 0) {
+      r0 = fa.e_node[0][i];
+      r1 = fa.h_node[1][j];
+      r2 = fa.h_node[2][kk];
+      ds = fa.e_size[1][j] * fa.e_size[2][kk];
+    } else if (c == 1) {
+      r0 = fa.h_node[0][i];
+      r1 = fa.e_node[1][j];
+      r2 = fa.h_node[2][kk];
+      ds = fa.e_size[2][kk] * fa.e_size[0][i];
+    } else {
+      r0 = fa.h_node[0][i];
+      r1 = fa.h_node[1][j];
+      r2 = fa.e_node[2][kk];
+      ds = fa.e_size[0][i] * fa.e_size[1][j];
+    }
+    const double rx = r0 - fa.origin[0], ry = r1 - fa.origin[1], rz = r2 - fa.origin[2];
+    const double v = rx * stcp + ry * stsp + rz * ct;
+    double sn, cs;
+    sincos(fa.k * v, &sn, &cs);
+    const int A = c == 2 ? 0 : c + 1, B = c == 0 ? 2 : c - 1;
+    const double tta = A == 0 ? tt0 : (A == 1 ? tt1 : tt2), ttb = B == 0 ? tt0 : (B == 1 ? tt1 : tt2);
+    const double tpa = A == 0 ? tp0 : (A == 1 ? tp1 : 0.0), tpb = B == 0 ? tp0 : (B == 1 ? tp1 : 0.0);
+    const double2 ja = acc[q], jb = acc[cells + q], ma = acc[2 * cells + q], mb = acc[3 * cells + q];
+    // (cur_a t_a + cur_b t_b) * phase * ds, phase = (cs, sn)
+    auto add = [&](double* out2, const double2 ca, const double2 cb, double ta, double tb) {
+      const double re = ca.x * ta + cb.x * tb, im = ca.y * ta + cb.y * tb;
+      out2[0] += (re * cs - im * sn) * ds;
+      out2[1] += (re * sn + im * cs) * ds;
+    };
+    add(s + 0, ja, jb, tta, ttb);
+    add(s + 2, ja, jb, tpa, tpb);
+    add(s + 4, ma, mb, tta, ttb);
+    add(s + 6, ma, mb, tpa, tpb);
+  }
+  for (int w = 0; w < 8; ++w) red[w][threadIdx.x] = s[w];
+  __syncthreads();
+  for (int half = 128; half > 0; half >>= 1) {
+    if (threadIdx.x < half)
+      for (int w = 0; w < 8; ++w) red[w][threadIdx.x] += red[w][threadIdx.x + half];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) out[static_cast<int64_t>(n) * 8 + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // Lumped-element and TFSF corrections: one thread per corrected cell applies
 // that cell's records in the reference's corrector order.  `state` holds the
 // inductor currents (one slot per inductor edge).
@@ -638,6 +722,8 @@ struct Launch {
                       const double* pool, T* state, int64_t t);
   static void gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out);
   static void nf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Nf2ffArgs& a, int64_t t);
+  // na directions, out: na x 8 doubles (A_theta, A_phi, F_theta, F_phi as re, im)
+  static void far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* out);
   static void unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2);
   static void pack(hipStream_t s, const Grid& g, const T* src, double* dst, int n0, int n1, int n2);
 };

@@ -225,6 +225,12 @@ void BatchLaunch<T>::tick(hipStream_t s, int64_t* t) {
   hipLaunchKernelGGL(k_tick<T>, dim3(1), dim3(64), 0, s, t);
 }
 
+template <class T>
+void Launch<T>::far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* out) {
+  if (na <= 0) return;
+  hipLaunchKernelGGL(k_far_field<T>, dim3(na), dim3(256), 0, s, a, f, fa, out);
+}
+
 // ---------------------------------------------------------------- adjoint --
 inline dim3 adjGrid(const Grid& g) {
   return dim3(blocks_for(g.nz + 1, kLanesK), blocks_for(g.ny + 1, kRowsJ), static_cast<unsigned>(g.nx + 1));

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstdio>
 #include <filesystem>
 #include <sstream>
@@ -103,6 +104,11 @@ const std::vector<std::complex<Real>>& NFFFTFrequencyDomain::current(std::size_t
   return _data->cur.at((f * 6 + d) * 4 + which);
 }
 
+bool NFFFTFrequencyDomain::deviceFarField() const {
+  const char* env = std::getenv("XFD_FARFIELD");
+  return _data->device && !(env && std::string(env) == "host");
+}
+
 std::vector<std::complex<Real>> NFFFTFrequencyDomain::potential(std::size_t fi, Potential pot, bool theta_comp,
                                                                  const std::vector<Real>& theta,
                                                                  const std::vector<Real>& phi,
@@ -128,6 +134,17 @@ std::vector<std::complex<Real>> NFFFTFrequencyDomain::potential(std::size_t fi, 
   };
   const Real freq = _frequencies.at(fi);
   const Real k = 2.0 * kPiN * freq / kC0N;
+  if (deviceFarField()) {  // the same sums on the GPU (xfd_nf2ff_far_field)
+    std::vector<Real> t(na), p(na);
+    for (std::size_t n = 0; n < na; ++n) {
+      t[n] = th(n);
+      p[n] = ph(n);
+    }
+    std::vector<Cplx> all;
+    D.device(fi, k, t, p, origin, all);
+    const std::size_t w = (pot == Potential::A ? 0 : 2) + (theta_comp ? 0 : 1);
+    return std::vector<Cplx>(all.begin() + w * na, all.begin() + (w + 1) * na);
+  }
   const Cplx ik = Cplx{0, 1} * k;
   std::vector<Cplx> total(na, Cplx{});
   std::vector<Real> st(na), ct(na), stsp(na), stcp(na);
@@ -185,6 +202,32 @@ std::vector<std::complex<Real>> NFFFTFrequencyDomain::potential(std::size_t fi, 
   return total;
 }
 
+std::array<std::vector<std::complex<Real>>, 4> NFFFTFrequencyDomain::potentials(std::size_t f,
+                                                                                 const std::vector<Real>& theta,
+                                                                                 const std::vector<Real>& phi,
+                                                                                 const Vector& origin) const {
+  std::array<std::vector<Cplx>, 4> r;
+  if (deviceFarField()) {
+    const std::size_t na = theta.size() * phi.size();
+    if (theta.empty() || phi.empty()) throw XFDTDException("Invalid theta or phi size");
+    if (theta.size() != 1 && phi.size() != 1) throw XFDTDException("There must be only one theta or phi");
+    std::vector<Real> t(na), p(na);
+    for (std::size_t n = 0; n < na; ++n) {
+      t[n] = theta[theta.size() == 1 ? 0 : n];
+      p[n] = phi[phi.size() == 1 ? 0 : n];
+    }
+    std::vector<Cplx> all;
+    _data->device(f, 2.0 * kPiN * _frequencies.at(f) / kC0N, t, p, origin, all);
+    for (int w = 0; w < 4; ++w) r[w].assign(all.begin() + w * na, all.begin() + (w + 1) * na);
+    return r;
+  }
+  r[0] = potential(f, Potential::A, true, theta, phi, origin);
+  r[1] = potential(f, Potential::A, false, theta, phi, origin);
+  r[2] = potential(f, Potential::F, true, theta, phi, origin);
+  r[3] = potential(f, Potential::F, false, theta, phi, origin);
+  return r;
+}
+
 Real NFFFTFrequencyDomain::power(std::size_t fi) const {
   // FDPlaneData::power / calculatePower (nffft_fd_data_process.cpp:28-42, 162-194)
   const auto& D = *_data;
@@ -224,10 +267,10 @@ void NFFFTFrequencyDomain::processFarField(const std::vector<Real>& theta, Real 
     ss << std::fixed << std::setprecision(2) << _frequencies[f] / 1e9 << "GHz";
     std::filesystem::create_directories(dir);
     const char* names[4] = {"_a_theta.npy", "_f_phi.npy", "_a_phi.npy", "_f_theta.npy"};
-    const Potential pots[4] = {Potential::A, Potential::F, Potential::A, Potential::F};
-    const bool comps[4] = {true, false, false, true};
+    const int order[4] = {0, 3, 1, 2};  // a_theta, f_phi, a_phi, f_theta in potentials() order
+    const auto all = potentials(f, theta, ph, origin);
     for (int q = 0; q < 4; ++q) {
-      auto v = potential(f, pots[q], comps[q], theta, ph, origin);
+      const auto& v = all[order[q]];
       npy::writeComplex((dir / (ss.str() + names[q])).string(), {v.size()}, reinterpret_cast<const double*>(v.data()));
     }
   }
@@ -242,10 +285,10 @@ void NFFFTFrequencyDomain::processFarField(Real theta, const std::vector<Real>& 
     ss << std::fixed << std::setprecision(2) << _frequencies[f] / 1e9 << "GHz";
     std::filesystem::create_directories(dir);
     const char* names[4] = {"_a_theta.npy", "_f_phi.npy", "_a_phi.npy", "_f_theta.npy"};
-    const Potential pots[4] = {Potential::A, Potential::F, Potential::A, Potential::F};
-    const bool comps[4] = {true, false, false, true};
+    const int order[4] = {0, 3, 1, 2};  // a_theta, f_phi, a_phi, f_theta in potentials() order
+    const auto all = potentials(f, th, phi, origin);
     for (int q = 0; q < 4; ++q) {
-      auto v = potential(f, pots[q], comps[q], th, phi, origin);
+      const auto& v = all[order[q]];
       npy::writeComplex((dir / (ss.str() + names[q])).string(), {v.size()}, reinterpret_cast<const double*>(v.data()));
     }
   }

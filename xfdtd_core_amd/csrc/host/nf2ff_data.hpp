@@ -5,6 +5,7 @@
 
 #include <complex>
 #include <cstddef>
+#include <functional>
 #include <vector>
 
 #include "../../../include/xfdtd_core_amd.hpp"
@@ -19,6 +20,12 @@ struct NFFFTFrequencyDomain::Data {
   std::vector<std::complex<Real>> te, th;  // nf x nt
   std::size_t nt = 0;
   std::vector<std::vector<std::complex<Real>>> cur;  // index (f * 6 + d) * 4 + which
+  // Set while a device context holds this NF2FF's currents: computes, for
+  // frequency f and the directions (theta[q], phi[q]), the potentials
+  // A_theta, A_phi, F_theta, F_phi (out: 4 x na, in that order) on the GPU.
+  std::function<void(std::size_t f, Real k, const std::vector<Real>& theta, const std::vector<Real>& phi,
+                     const Vector& origin, std::vector<std::complex<Real>>& out)>
+      device;
   void setup(const std::size_t dist[3], const std::size_t n[3], const std::vector<Real> node_e[3],
              const std::vector<Real> node_h[3], const std::vector<Real> size_e[3], const std::vector<Real>& freq,
              Real dt, std::size_t steps);

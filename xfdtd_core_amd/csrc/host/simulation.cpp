@@ -406,6 +406,7 @@ struct Simulation::Impl {
   ~Impl() { releaseDevice(); }
 
   void releaseDevice() {
+    for (auto& n : nf2ffs) n->data().device = nullptr;
     if (ctx && !batch_core) xfd_destroy(ctx);
     ctx = nullptr;
     batch_core.reset();
@@ -1389,6 +1390,25 @@ struct Simulation::Impl {
     ctx = c;
     batch_core = std::move(core);
     initialised = true;
+    for (std::size_t q = 0; q < nf2ffs.size(); ++q) {  // far-field sums on the device
+      auto& D = nf2ffs[q]->data();
+      const NFFFTFrequencyDomain::Data* Dp = &D;
+      D.device = [this, q, Dp](std::size_t f, Real k, const std::vector<Real>& t, const std::vector<Real>& p,
+                               const Vector& o, std::vector<std::complex<Real>>& out) {
+        const std::size_t na = t.size();
+        out.assign(4 * na, {0, 0});
+        std::vector<double> buf(8 * na);
+        const double org[3] = {o[0], o[1], o[2]};
+        const double* en[3] = {Dp->e_node[0].data(), Dp->e_node[1].data(), Dp->e_node[2].data()};
+        const double* hn[3] = {Dp->h_node[0].data(), Dp->h_node[1].data(), Dp->h_node[2].data()};
+        const double* es[3] = {Dp->e_size[0].data(), Dp->e_size[1].data(), Dp->e_size[2].data()};
+        if (xfd_nf2ff_far_field(ctx, static_cast<int32_t>(q), static_cast<int32_t>(f), static_cast<int32_t>(na),
+                                t.data(), p.data(), org, k, en, hn, es, buf.data()) != XFD_OK)
+          throw XFDTDSimulationException(std::string("xfd_nf2ff_far_field: ") + xfd_last_error());
+        for (std::size_t n = 0; n < na; ++n)
+          for (int w = 0; w < 4; ++w) out[w * na + n] = {buf[n * 8 + 2 * w], buf[n * 8 + 2 * w + 1]};
+      };
+    }
     int64_t b = 0;
     xfd_device_bytes(ctx, &b);
     dev_bytes = b;
