@@ -273,6 +273,83 @@ def bench_design_sweep(args):
     return line
 
 
+def bench_dispersive(args):
+    """Dispersive + lumped box at full size: 256^3 (10-cell CPML), four ADE
+    slabs (Drude, Debye, two-pole Lorentz, modified Lorentz), oblique TFSF,
+    50-ohm resistor and voltage source, 8-frequency NF2FF 15 cells inside the
+    grid; FP64 and FP32 runs of the same steps, the FP32 error against FP64
+    (final fields, far-field potentials, radiated power) and the far field
+    summed on the GPU vs the host."""
+    from xfdtd_core_amd import native, scenarios
+
+    spec = scenarios.dispersive_slabs(nf2ff_distance=15).spec()
+    steps = args.dispersive_steps
+    out = {}
+    sims = {}
+    for prec in ("f64", "f32"):
+        sim = native.Simulation(spec, prec)
+        t0 = time.perf_counter()
+        sim.init(steps)
+        init_s = time.perf_counter() - t0
+        native.device_synchronize()
+        t0 = time.perf_counter()
+        sim.advance(0, steps)
+        native.device_synchronize()
+        wall = time.perf_counter() - t0
+        sim.finish()
+        g = sim.grid()
+        cells = int(np.prod(g))
+        out[prec] = {"mcells_per_s": round(cells * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+                     "init_s": round(init_s, 2)}
+        log(f"dispersive {prec}: {out[prec]['mcells_per_s']:.0f} Mcells/s ({out[prec]['ms_per_step']:.3f} ms/step)")
+        sims[prec] = sim
+    a, b = sims["f32"], sims["f64"]
+    eta = 376.730313668
+    num = den = 0.0
+    for f in FIELDS:
+        w = eta if f[0] == "H" else 1.0
+        x, y = a.field(f) * w, b.field(f) * w
+        num += float(np.sum((x - y) ** 2))
+        den += float(np.sum(y ** 2))
+    theta = np.linspace(0.0, np.pi, 181)
+    nf = 8
+    pot_err, pow_err = 0.0, 0.0
+    t0 = time.perf_counter()
+    for f in range(nf):
+        for ap in (True, False):
+            for tc in (True, False):
+                pa = a.nf2ff_potential(0, f, ap, tc, theta, 0.0)
+                pb = b.nf2ff_potential(0, f, ap, tc, theta, 0.0)
+                pot_err = max(pot_err, float(np.abs(pa - pb).max() / np.abs(pb).max()))
+        pow_err = max(pow_err, abs(a.nf2ff_power(0, f) / b.nf2ff_power(0, f) - 1))
+    dev_s = (time.perf_counter() - t0) / 2  # both precisions' potentials were summed
+    # the same potentials on the host for one frequency and a 19-direction cut
+    os.environ["XFD_FARFIELD"] = "host"
+    th_small = np.linspace(0.0, np.pi, 19)
+    t0 = time.perf_counter()
+    for ap in (True, False):
+        for tc in (True, False):
+            b.nf2ff_potential(0, 0, ap, tc, th_small, 0.0)
+    host_s = time.perf_counter() - t0
+    del os.environ["XFD_FARFIELD"]
+    host_per_dir = host_s / (4 * 19)
+    res = {
+        "workload": "dispersive + lumped 256^3 (10-cell CPML), Drude/Debye/Lorentz/M-Lorentz slabs, oblique TFSF, "
+                    "50-ohm resistor + voltage source, 8-frequency NF2FF",
+        "steps": steps, "fp64": out["f64"], "fp32": out["f32"],
+        "fp32_over_fp64": round(out["f32"]["mcells_per_s"] / out["f64"]["mcells_per_s"], 3),
+        "fp32_error": {"field_state_rel_l2": float(np.sqrt(num / den)),
+                       "far_field_potential_rel_max": pot_err, "radiated_power_rel_max": pow_err},
+        "far_field": {"directions": 181, "frequencies": nf, "device_s_for_all": round(dev_s, 4),
+                      "host_s_per_direction_and_potential": round(host_per_dir, 5),
+                      "host_s_estimate_for_all": round(host_per_dir * 4 * 181 * nf, 1)},
+    }
+    log(f"dispersive fp32 error: fields {res['fp32_error']['field_state_rel_l2']:.2e}, far field {pot_err:.2e}, "
+        f"power {pow_err:.2e}; far field on device {dev_s:.3f} s, host estimate "
+        f"{res['far_field']['host_s_estimate_for_all']:.1f} s")
+    return res
+
+
 def bench_adjoint(args):
     """Adjoint inverse design at full size: FP32 forward with checkpoints
     every 100 steps and the reverse-time adjoint; optionally the FP64 adjoint
@@ -333,8 +410,10 @@ def main():
     ap.add_argument("--precision", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint"], default="box",
-                    help="box: the 384^3 headline; design_sweep: the batched form; adjoint: the gradient form")
+    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint", "dispersive"], default="box",
+                    help="box: the 384^3 headline; design_sweep: the batched form; adjoint: the gradient form; "
+                         "dispersive: the 256^3 ADE + lumped + NF2FF box")
+    ap.add_argument("--dispersive-steps", type=int, default=3000)
     ap.add_argument("--cases", type=int, default=96)
     ap.add_argument("--sequential", type=int, default=24,
                     help="design_sweep: also time this many cases stepped one context at a time")
@@ -352,6 +431,9 @@ def main():
 
     if args.workload == "design_sweep":
         print(json.dumps(bench_design_sweep(args)), flush=True)
+        return
+    if args.workload == "dispersive":
+        print(json.dumps(bench_dispersive(args)), flush=True)
         return
     if args.workload == "adjoint":
         print(json.dumps({"workload": "adjoint inverse design 160^3", **bench_adjoint(args)}), flush=True)
