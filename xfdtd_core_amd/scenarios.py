@@ -98,7 +98,8 @@ def inhomogeneous_box(n_total: int = 384, pml: int = 12, n_spheres: int = 64,
 
 def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab: int = 40, gap: int = 8,
                      tfsf_offset: int = 14, lumped: bool = True, kinds=("drude", "debye", "lorentz", "mlorentz"),
-                     snapshot_interval: int = 0, nf2ff_distance: int = 0, nf2ff_freqs=None):
+                     snapshot_interval: int = 0, nf2ff_distance: int = 0, nf2ff_freqs=None,
+                     drude_wp: float = 1.37e16):
     """Dispersive + lumped box: four z-slabs (`slab` cells thick, `gap`-cell
     air gaps, centred, full transverse extent inside the CPML) of Drude
     (eps_inf=1, wp=1.37e16, gamma=1e14), Debye (eps_inf=4, eps_s=50,
@@ -111,14 +112,17 @@ def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab
     50-ohm, 1 V VoltageSource along z, each a 6-cell gap in the air below the
     slabs.  CFL 0.95.  `kinds` selects and orders the slab materials.  With
     `nf2ff_distance`, a frequency-domain NF2FF on the box that many cells
-    inside the grid (default frequencies: 8 from 5 to 40 GHz)."""
+    inside the grid (default frequencies: 8 from 5 to 40 GHz).  With the
+    default wp = 1.37e16 rad/s (wp dt ~ 1e4 at dl = 0.5 mm) the reference's
+    first-order Drude scheme diverges (the oracle shows the same growth);
+    `drude_wp` sets another plasma frequency."""
     s = S.Simulation(dl, dl, dl, 0.95)
     n = n_total - 2 * pml
     half = n * dl / 2
     s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
                          S.Material.createAir()))
     mats = {
-        "drude": S.DrudeMedium.makeDrudeMedium("drude", 1.0, [1.37e16], [1e14]),
+        "drude": S.DrudeMedium.makeDrudeMedium("drude", 1.0, [drude_wp], [1e14]),
         "debye": S.DebyeMedium.makeDebyeMedium("debye", 4.0, [50.0], [1e-11]),
         "lorentz": S.LorentzMedium.makeMLorentz("lorentz", 2.0, [1.5 * 2e10 ** 2, 0.8 * 6e10 ** 2], [0.0, 0.0],
                                                 [2e10 ** 2, 6e10 ** 2], [2e9, 6e9], [1.0, 1.0]),
