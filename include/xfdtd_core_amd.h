@@ -133,6 +133,21 @@ typedef struct {
   const double* th;            /* nf x nt complex: dt exp(-j 2 pi f (t + 1/2) dt) */
 } xfd_nf2ff_desc;
 
+/* A weighted-sum recording: VoltageMonitor / CurrentMonitor
+ * (voltage_monitor.cpp:191-227, current_monitor.cpp:272-385).  After the
+ * corrections of every step t < count the library records, in FP64,
+ *   scale * (g0 + g1),  g_q = sum over the terms of group q, in term order,
+ *   of weight * F[field](i, j, k)
+ * (the reference accumulates its integrals term by term; a current monitor
+ * keeps its two loop sides in the two groups). */
+typedef struct {
+  int32_t n_terms;
+  const int32_t* terms;    /* n_terms x 5: field (enum xfd_field), i, j, k, group (0 or 1) */
+  const double* weights;   /* n_terms */
+  double scale;
+  int64_t count;
+} xfd_line_desc;
+
 typedef struct {
   int32_t nx, ny, nz;
   int32_t dtype;               /* enum xfd_dtype: field and update precision */
@@ -150,6 +165,8 @@ typedef struct {
   const xfd_ade_desc* ade;     /* NULL: no dispersive material */
   int32_t n_nf2ff;
   const xfd_nf2ff_desc* nf2ff;
+  int32_t n_line;
+  const xfd_line_desc* line;
 } xfd_desc;
 
 typedef struct xfd_ctx xfd_ctx;
@@ -179,6 +196,9 @@ int xfd_write_field(xfd_ctx* ctx, int32_t field, const double* src);
 
 /* Copies recording `g` (count x box volume doubles) to the host. */
 int xfd_read_gather(xfd_ctx* ctx, int32_t g, double* dst);
+
+/* Copies weighted-sum recording `m` (count doubles) to the host. */
+int xfd_read_line(xfd_ctx* ctx, int32_t m, double* dst);
 
 /* Copies the accumulated surface current `which` (0 ja, 1 jb, 2 ma, 3 mb)
  * of NF2FF `n`, frequency `f`, surface `d` (XN XP YN YP ZN ZP) to the host

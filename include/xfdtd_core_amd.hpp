@@ -441,6 +441,42 @@ class FieldTimeMonitor : public Monitor {
   std::vector<Real> _time, _data;
 };
 
+// VoltageMonitor (voltage_monitor.h): every step, -(line integral of E
+// along `direction`) through the cube, averaged over its cross-section
+// (voltage_monitor.cpp:21-227).  time() holds the E times.
+class VoltageMonitor : public Monitor {
+ public:
+  VoltageMonitor(std::string name, std::unique_ptr<Cube> cube, Axis::Direction direction,
+                 std::string output_dir = "");
+  const Cube& cube() const { return *_cube; }
+  Axis::Direction direction() const { return _direction; }
+  std::vector<Real>& time() { return _time; }
+  std::vector<Real>& data() { return _data; }
+
+ private:
+  std::unique_ptr<Cube> _cube;
+  Axis::Direction _direction;
+  std::vector<Real> _time, _data;
+};
+
+// CurrentMonitor (current_monitor.h): every step, the loop integral of H
+// around the cube's last cross-section along `direction`
+// (current_monitor.cpp:18-385).  time() holds the H times.
+class CurrentMonitor : public Monitor {
+ public:
+  CurrentMonitor(std::string name, std::unique_ptr<Cube> cube, Axis::Direction direction,
+                 std::string output_dir = "");
+  const Cube& cube() const { return *_cube; }
+  Axis::Direction direction() const { return _direction; }
+  std::vector<Real>& time() { return _time; }
+  std::vector<Real>& data() { return _data; }
+
+ private:
+  std::unique_ptr<Cube> _cube;
+  Axis::Direction _direction;
+  std::vector<Real> _time, _data;
+};
+
 // Frequency-domain near-to-far-field transform (nffft_frequency_domain.h):
 // a running DFT of the tangential fields on the box `distance` cells inside
 // the grid boundary, run on the GPU every step; after the run the far-field

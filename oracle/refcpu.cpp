@@ -1226,6 +1226,173 @@ void FieldTimeMonitor::output(Sim& s) {
   writeNpy(out_dir + "/" + name + ".npy", {2, time.size()}, st.data());
 }
 
+// ------------------------------------------------ voltage / current ----
+namespace {
+// xfdtd::transform::xYZToABC for a direction's axis: (a, b, c) =
+// X: (y, z, x), Y: (z, x, y), Z: (x, y, z).
+void abcAxes(int axis, int& a, int& b, int& c) {
+  c = axis;
+  a = (axis + 1) % 3;
+  b = (axis + 2) % 3;
+}
+
+void writeTimeSeries(Sim& s, const std::string& dir, const std::string& name, const Arr1& time, const Arr1& data) {
+  if (!s.write_files || dir.empty()) return;
+  std::vector<Real> st(2 * time.size());
+  for (Index i = 0; i < time.size(); ++i) {
+    st[i] = time(i);
+    st[time.size() + i] = data(i);
+  }
+  std::filesystem::create_directories(dir);
+  writeNpy(dir + "/" + name + ".npy", {2, time.size()}, st.data());
+}
+}  // namespace
+
+void VoltageMonitor::init(Sim& s) {  // voltage_monitor.cpp:21-189, one process
+  Index end[3];
+  const auto org = s.g.box(*shape, end);
+  int a, b, c;
+  abcAxes(dir / 2, a, b, c);
+  const Index n[3] = {s.g.nx(), s.g.ny(), s.g.nz()};
+  auto clip = [](Index lo, Index up, Index& st, Index& en) {  // new_range
+    if (st <= lo) st = lo + 1;
+    if (up <= en) en = up - 1;
+  };
+  // a and b: the cube's range extended by one, kept off the grid faces
+  for (int q : {a, b}) {
+    r[q][0] = org[q];
+    r[q][1] = end[q] + 1;
+    clip(0, n[q], r[q][0], r[q][1]);
+  }
+  r[c][0] = org[c];  // c: the cube's range (the node clipping is undone at the global ends)
+  r[c][1] = end[c];
+  for (int q = 0; q < 3; ++q)
+    if (r[q][0] >= r[q][1]) throw Error(name + " globalTask is not valid");
+  const Index cnt = (r[a][1] - r[a][0]) * (r[b][1] - r[b][0]);
+  coff = Arr1(r[c][1] - r[c][0]);
+  for (Index q = 0; q < coff.size(); ++q) {
+    coff(q) = -s.g.h_size[c](r[c][0] + q) / static_cast<Real>(cnt);
+    if (dir % 2 == 0) coff(q) *= -1.0;  // negative direction
+  }
+}
+
+void VoltageMonitor::initTimeDependentVariable(Sim& s) {  // voltage_monitor.cpp:229-233
+  time = s.eTime();
+  data = Arr1(time.size());
+}
+
+void VoltageMonitor::update(Sim& s, Index t) {  // voltage_monitor.cpp:191-227
+  const int c = dir / 2;
+  Arr3& f = s.emf.byId(c);
+  for (Index i = r[0][0]; i < r[0][1]; ++i)
+    for (Index j = r[1][0]; j < r[1][1]; ++j)
+      for (Index k = r[2][0]; k < r[2][1]; ++k) {
+        const Index p[3] = {i, j, k};
+        data(t) += coff(p[c] - r[c][0]) * f(i, j, k);
+      }
+}
+
+void VoltageMonitor::output(Sim& s) { writeTimeSeries(s, out_dir, name, time, data); }
+
+void CurrentMonitor::init(Sim& s) {  // current_monitor.cpp:18-270, one process
+  Index end[3];
+  const auto org = s.g.box(*shape, end);
+  int a, b, c;
+  abcAxes(dir / 2, a, b, c);
+  const Index n[3] = {s.g.nx(), s.g.ny(), s.g.nz()};
+  if (end[c] == 0) throw Error("CurrentMonitor::init(): global_c_e is 0");
+  if (org[a] == 0 || end[a] == n[a]) throw Error("CurrentMonitor::init(): global x range is invalid");
+  if (org[b] == 0 || end[b] == n[b]) throw Error("CurrentMonitor::init(): global y range is invalid");
+  r[a][0] = org[a];
+  r[a][1] = end[a];
+  r[b][0] = org[b];
+  r[b][1] = end[b];
+  r[c][0] = end[c] - 1;  // the last plane along the direction
+  r[c][1] = end[c];
+  // make_ranges (:138-169): the loop side along one axis, present when the
+  // other axis' face lies strictly inside the grid
+  auto ranges = [](Index b_start, Index b_end, Index b_upper, Index a_upper, Index st, Index en, Index* rn,
+                   Index* rp) {
+    if (st <= 0) st = 1;
+    en = en + 1;
+    if (a_upper <= en) en = a_upper - 1;
+    rn[0] = rn[1] = rp[0] = rp[1] = 0;
+    if (b_start != 0 && b_start != b_upper - 1 && b_start != b_upper) {
+      rn[0] = st;
+      rn[1] = en;
+    }
+    if (b_end != b_upper && b_end != b_upper - 1 && b_end != 0) {
+      rp[0] = st;
+      rp[1] = en;
+    }
+  };
+  ranges(org[b], end[b], n[b], n[a], org[a], end[a], ha_n, ha_p);
+  ranges(org[a], end[a], n[a], n[b], org[b], end[b], hb_n, hb_p);
+  auto valid = [](const Index* q) { return q[0] < q[1]; };
+  auto sizes = [&](int axis, const Index* q) {
+    Arr1 v(q[1] - q[0]);
+    for (Index i = 0; i < v.size(); ++i) v(i) = s.g.e_size[axis](q[0] + i);
+    return v;
+  };
+  // da along a, db along b (eSize of those axes)
+  if (valid(ha_n))
+    da = sizes(a, ha_n);
+  else if (valid(ha_p))
+    da = sizes(a, ha_p);
+  if (valid(hb_n))
+    db = sizes(b, hb_n);
+  else if (valid(hb_p))
+    db = sizes(b, hb_p);
+  positive = dir % 2 == 1 ? 1.0 : -1.0;
+}
+
+void CurrentMonitor::initTimeDependentVariable(Sim& s) {  // current_monitor.cpp:387-391
+  time = s.hTime();
+  data = Arr1(time.size());
+}
+
+void CurrentMonitor::update(Sim& s, Index t) {  // current_monitor.cpp:272-385
+  auto valid = [](const Index* q) { return q[0] < q[1]; };
+  if (!(valid(ha_n) || valid(ha_p) || valid(hb_n) || valid(hb_p))) return;
+  const Index is = r[0][0], ie = r[0][1], js = r[1][0], je = r[1][1], ks = r[2][0], ke = r[2][1];
+  auto& e = s.emf;
+  Real ix = 0, iy = 0, iz = 0;
+  const int c = dir / 2;
+  if (c == 0) {
+    if (valid(ha_n))
+      for (Index j = ha_n[0]; j < ha_n[1]; ++j) iy += e.hy(ie - 1, j, ks - 1) * da(j - ha_n[0]);
+    if (valid(ha_p))
+      for (Index j = ha_p[0]; j < ha_p[1]; ++j) iy -= e.hy(ie - 1, j, ke) * da(j - ha_p[0]);
+    if (valid(hb_n))
+      for (Index k = hb_n[0]; k < hb_n[1]; ++k) iz -= e.hz(ie - 1, js - 1, k) * db(k - hb_n[0]);
+    if (valid(hb_p))
+      for (Index k = hb_p[0]; k < hb_p[1]; ++k) iz += e.hz(ie - 1, je, k) * db(k - hb_p[0]);
+    data(t) = positive * (iz + iy);
+  } else if (c == 1) {
+    if (valid(ha_n))
+      for (Index k = ha_n[0]; k < ha_n[1]; ++k) iz += e.hz(is - 1, je - 1, k) * da(k - ha_n[0]);
+    if (valid(ha_p))
+      for (Index k = ha_p[0]; k < ha_p[1]; ++k) iz -= e.hz(ie, je - 1, k) * da(k - ha_p[0]);
+    if (valid(hb_n))
+      for (Index i = hb_n[0]; i < hb_n[1]; ++i) ix -= e.hx(i, je - 1, ks - 1) * db(i - hb_n[0]);
+    if (valid(hb_p))
+      for (Index i = hb_p[0]; i < hb_p[1]; ++i) ix += e.hx(i, je - 1, ke) * db(i - hb_p[0]);
+    data(t) = positive * (ix + iz);
+  } else {
+    if (valid(ha_n))
+      for (Index i = ha_n[0]; i < ha_n[1]; ++i) ix += e.hx(i, js - 1, ke - 1) * da(i - ha_n[0]);
+    if (valid(ha_p))
+      for (Index i = ha_p[0]; i < ha_p[1]; ++i) ix -= e.hx(i, je, ke - 1) * da(i - ha_p[0]);
+    if (valid(hb_n))
+      for (Index j = hb_n[0]; j < hb_n[1]; ++j) iy -= e.hy(is - 1, j, ke - 1) * db(j - hb_n[0]);
+    if (valid(hb_p))
+      for (Index j = hb_p[0]; j < hb_p[1]; ++j) iy += e.hy(ie, j, ke - 1) * db(j - hb_p[0]);
+    data(t) = positive * (ix + iy);
+  }
+}
+
+void CurrentMonitor::output(Sim& s) { writeTimeSeries(s, out_dir, name, time, data); }
+
 // ----------------------------------------------------------------- NF2FF --
 namespace {
 // GCC's complex product for finite operands: (ac - bd, ad + bc)

@@ -435,6 +435,35 @@ struct FieldTimeMonitor : Monitor {
   void output(Sim& s) override;                     // time_monitor.cpp:11-21
 };
 
+// VoltageMonitor (voltage_monitor.cpp:15-235) and CurrentMonitor
+// (current_monitor.cpp:12-385) for one process: the node task is the global
+// task.  `dir` is an Axis::Direction (XN XP YN YP ZN ZP = 0..5).
+struct VoltageMonitor : Monitor {
+  std::unique_ptr<Shape> shape;
+  int dir = 5;
+  Index r[3][2]{};  // is ie, js je, ks ke
+  Arr1 coff;
+  Arr1 time, data;
+  void init(Sim& s) override;
+  void initTimeDependentVariable(Sim& s) override;
+  void update(Sim& s, Index t) override;
+  void output(Sim& s) override;  // time_monitor.cpp:11-21
+};
+
+struct CurrentMonitor : Monitor {
+  std::unique_ptr<Shape> shape;
+  int dir = 5;
+  Index r[3][2]{};                 // is ie, js je, ks ke
+  Index ha_n[2]{}, ha_p[2]{}, hb_n[2]{}, hb_p[2]{};  // [start, end), empty when start >= end
+  Arr1 da, db;
+  Real positive = 1;
+  Arr1 time, data;
+  void init(Sim& s) override;
+  void initTimeDependentVariable(Sim& s) override;
+  void update(Sim& s, Index t) override;
+  void output(Sim& s) override;
+};
+
 // ------------------------------------------------------------ simulation --
 // ---------------------------------------------------------------- NF2FF ---
 // NFFFTFrequencyDomain (nffft.cpp:39-236, nffft_frequency_domain.cpp) with

@@ -270,6 +270,26 @@ void parseSpec(Sim& s, const std::string& spec) {
       if (m->out_dir == "-") m->out_dir.clear();
       m->frame = std::move(f);
       s.monitors.push_back(std::move(m));
+    } else if (d == "voltage_monitor" || d == "current_monitor") {  // NAME CUBE DIR OUT_DIR
+      std::unique_ptr<Monitor> mon;
+      std::string name = word(is);
+      auto cube = parseCube(is);
+      const int dir = static_cast<int>(parseDir(word(is)));
+      if (d == "voltage_monitor") {
+        auto m = std::make_unique<VoltageMonitor>();
+        m->shape = std::move(cube);
+        m->dir = dir;
+        mon = std::move(m);
+      } else {
+        auto m = std::make_unique<CurrentMonitor>();
+        m->shape = std::move(cube);
+        m->dir = dir;
+        mon = std::move(m);
+      }
+      mon->name = name;
+      mon->out_dir = word(is);
+      if (mon->out_dir == "-") mon->out_dir.clear();
+      s.monitors.push_back(std::move(mon));
     } else if (d == "field_time") {
       auto m = std::make_unique<FieldTimeMonitor>();
       m->name = word(is);
@@ -498,7 +518,9 @@ int ref_monitor_records(void* p, int m) {
   auto& s = static_cast<Handle*>(p)->sim;
   Monitor* mo = s.monitors.at(m).get();
   if (auto mv = dynamic_cast<MovieMonitor*>(mo)) return static_cast<int>(mv->frames.size());
-  if (dynamic_cast<FieldTimeMonitor*>(mo)) return 1;
+  if (dynamic_cast<FieldTimeMonitor*>(mo) || dynamic_cast<VoltageMonitor*>(mo) ||
+      dynamic_cast<CurrentMonitor*>(mo))
+    return 1;
   return 0;
 }
 
@@ -517,6 +539,26 @@ int ref_monitor_record(void* p, int m, int r, int64_t* shape, double* dst) {
     if (dst) {
       std::memcpy(dst, ft->time.v.data(), nt * sizeof(double));
       std::memcpy(dst + nt, ft->data.v.data(), nt * sizeof(double));
+    }
+    return 0;
+  }
+  const Arr1* time = nullptr;
+  const Arr1* data = nullptr;
+  if (auto vm = dynamic_cast<VoltageMonitor*>(mo)) {
+    time = &vm->time;
+    data = &vm->data;
+  } else if (auto cm = dynamic_cast<CurrentMonitor*>(mo)) {
+    time = &cm->time;
+    data = &cm->data;
+  }
+  if (time) {
+    const Index nt = time->size();
+    shape[0] = 2;
+    shape[1] = static_cast<int64_t>(nt);
+    shape[2] = 1;
+    if (dst) {
+      std::memcpy(dst, time->v.data(), nt * sizeof(double));
+      std::memcpy(dst + nt, data->v.data(), nt * sizeof(double));
     }
     return 0;
   }

@@ -353,3 +353,53 @@ def test_far_field_on_device_matches_host(monkeypatch):
             assert scale > 0
             for gi, hi in zip(g, h):
                 assert np.abs(gi - hi).max() <= 1e-12 * scale
+
+
+def _line_monitor_spec():
+    """A 50-ohm voltage source and a 50-ohm resistor (6-cell z gaps) in a
+    CPML box, with a voltage and a current monitor on each and a voltage
+    monitor across a 2 x 3 x 4 block along y."""
+    dl = 1e-3
+    s = S.Simulation(dl, dl, dl, 0.9)
+    half = 12 * dl
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    tau = 15 * dl / 6e8
+    src = S.Cube(S.Vector(-3 * dl, 0.0, -3 * dl), S.Vector(0.0, 0.0, 6 * dl))
+    res = S.Cube(S.Vector(4 * dl, 0.0, -3 * dl), S.Vector(0.0, 0.0, 6 * dl))
+    s.addObject(S.VoltageSource("v1", src, "ZP", 50.0, S.Waveform.gaussian(tau, 4.5 * tau, 1.0)))
+    s.addObject(S.Resistor("r1", res, "Z", 50.0))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(6, d))
+    s.addMonitor(S.VoltageMonitor("v_src", src, "ZP"))
+    s.addMonitor(S.CurrentMonitor("i_src", src, "ZP"))
+    s.addMonitor(S.VoltageMonitor("v_res", res, "ZN"))
+    s.addMonitor(S.CurrentMonitor("i_res", res, "ZN"))
+    s.addMonitor(S.VoltageMonitor("v_blk", S.Cube(S.Vector(-dl, -2 * dl, dl), S.Vector(2 * dl, 3 * dl, 4 * dl)), "YP"))
+    s.addMonitor(S.CurrentMonitor("i_blk", S.Cube(S.Vector(-dl, -2 * dl, dl), S.Vector(2 * dl, 3 * dl, 4 * dl)),
+                                  "XN"))
+    return s.spec()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_voltage_current_monitors(precision):
+    """VoltageMonitor / CurrentMonitor series on the device equal the
+    oracle's: bitwise in FP64; FP32 within 1e-4 of each series' peak."""
+    sp = _line_monitor_spec()
+    steps = 400
+    ref = RefSim(sp)
+    ref.run(steps)
+    dev = native.Simulation(sp, precision)
+    dev.run(steps)
+    assert dev.monitor_count() == 6
+    for m in range(6):
+        a, b = dev.monitor_records(m)[0], ref.monitor_records(m)[0]
+        assert a.shape == b.shape == (2, steps)
+        np.testing.assert_array_equal(a[0], b[0])  # times
+        assert np.abs(b[1]).max() > 0, m
+        if precision == "f64":
+            np.testing.assert_array_equal(a[1], b[1], err_msg=f"monitor {m}")
+        else:
+            err = np.abs(a[1] - b[1]).max() / np.abs(b[1]).max()
+            say(f"  monitor {m}: fp32 error {err:.2e} of the peak")
+            assert err < 1e-4, (m, err)

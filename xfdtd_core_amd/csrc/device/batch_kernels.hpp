@@ -162,6 +162,16 @@ __global__ __launch_bounds__(64) void k_probes_at(FieldPtrs<T> F, const Probe* _
   out[q * stride + t] = static_cast<double>(F.f[p[q].field][p[q].idx]);
 }
 
+template <class T>
+__global__ __launch_bounds__(64) void k_lines_at(FieldPtrs<T> F, const LineTerm* __restrict__ terms,
+                                                 const int64_t* __restrict__ off, const double* __restrict__ scale,
+                                                 int n, double* out, int64_t stride, const int64_t* __restrict__ tdev) {
+  const int64_t t = *tdev;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n || t >= stride) return;
+  lineSum(F, terms, off, scale, m, out, stride, t);
+}
+
 // Advances the device-side step counter (one lane writes it).  Templated on
 // the field type only so that each precision's unit holds its own copy.
 template <class T>
@@ -186,6 +196,8 @@ struct BatchLaunch {
                        int64_t first, int64_t interval, int64_t count, const int64_t* tdev);
   static void probesAt(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                        const int64_t* tdev);
+  static void linesAt(hipStream_t s, const FieldPtrs<T>& F, const LineTerm* terms, const int64_t* off,
+                      const double* scale, int n, double* out, int64_t stride, const int64_t* tdev);
   static void tick(hipStream_t s, int64_t* t);
 };
 

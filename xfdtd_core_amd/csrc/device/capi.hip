@@ -260,6 +260,10 @@ struct xfd_ctx {
     int probe = -1;  // row in the batched probe buffer, or -1
   };
   std::vector<std::unique_ptr<Gather>> gathers;
+  // weighted-sum recordings (voltage / current monitors): one launch per step
+  DevBuf line_terms, line_off, line_scale, line_out;
+  int n_line = 0;
+  int64_t line_count = 0;
   // single-cell, every-step recordings share one launch per step
   DevBuf probe_list, probe_out;
   int nprobe = 0;
@@ -308,6 +312,7 @@ struct xfd_ctx {
     for (int q = 0; q < 3; ++q) b += ade_j[q].bytes + ade_jp[q].bytes + ade_eprev[q].bytes;
     for (auto& gth : gathers) b += gth->out.bytes;
     b += ckpt.bytes;
+    b += line_terms.bytes + line_off.bytes + line_scale.bytes + line_out.bytes;
     return b;
   }
 
@@ -795,6 +800,36 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     uploadVector(pl, c.probe_list);
     c.probe_out.alloc(static_cast<size_t>(c.nprobe * c.probe_count) * sizeof(double));
   }
+  if (d.n_line > 0) {  // weighted-sum recordings
+    require(d.line != nullptr, "line recordings missing");
+    std::vector<xfd::LineTerm> terms;
+    std::vector<int64_t> off{0};
+    std::vector<double> scale;
+    int64_t count = -1;
+    for (int m = 0; m < d.n_line; ++m) {
+      const auto& L = d.line[m];
+      require(L.n_terms >= 0 && (L.n_terms == 0 || (L.terms && L.weights)), "bad line recording");
+      require(count < 0 || L.count == count, "line recordings share one length");
+      count = L.count;
+      for (int q = 0; q < L.n_terms; ++q) {
+        const int32_t* tq = L.terms + 5 * q;
+        require(tq[0] >= 0 && tq[0] < 6 && (tq[4] == 0 || tq[4] == 1), "bad line term");
+        int ext[3];
+        extentOf(tq[0], g.nx, g.ny, g.nz, ext);
+        require(tq[1] >= 0 && tq[2] >= 0 && tq[3] >= 0 && tq[1] < ext[0] && tq[2] < ext[1] && tq[3] < ext[2],
+                "line term outside the field");
+        terms.push_back({padIndex(g, tq[1], tq[2], tq[3]), L.weights[q], tq[0], tq[4]});
+      }
+      off.push_back(static_cast<int64_t>(terms.size()));
+      scale.push_back(L.scale);
+    }
+    uploadVector(terms, c.line_terms);
+    uploadVector(off, c.line_off);
+    uploadVector(scale, c.line_scale);
+    c.n_line = d.n_line;
+    c.line_count = std::max<int64_t>(0, count);
+    c.line_out.alloc(static_cast<size_t>(c.n_line * c.line_count) * sizeof(double));
+  }
   {  // CPML inside the sweeps when XFD_CPML_FUSED=1, without ADE or records in a slab.
     // Off by default: measured on MI355X the separate slab launches are faster
     // (384^3 FP32 step 1.51 ms separate vs 1.83 ms fused), because the fused
@@ -971,6 +1006,11 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         xfd::Launch<T>::gather(st, c.g, F, gp->a, gp->out.as<double>() + frame * gp->a.frame_elems);
       });
     }
+    if (c.n_line && t < c.line_count)
+      c.timed(K_GATHER, [&] {
+        xfd::Launch<T>::lines(st, F, c.line_terms.as<xfd::LineTerm>(), c.line_off.as<int64_t>(),
+                              c.line_scale.as<double>(), c.n_line, c.line_out.as<double>(), c.line_count, t);
+      });
     for (auto& nn : c.nf2ffs)  // Domain::record: monitors, then NF2FF (domain.cpp:176-184)
       c.timed(K_NF2FF, [&] { xfd::Launch<T>::nf2ff(st, c.g, F, nn->a, t); });
     XFD_HIP(hipGetLastError());
@@ -1001,6 +1041,7 @@ void resetState(xfd_ctx& c) {
   for (auto& n : c.nf2ffs) zero(n->acc);
   for (auto& gg : c.gathers) zero(gg->out);
   zero(c.probe_out);
+  zero(c.line_out);
   XFD_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -1355,6 +1396,9 @@ void batchEnqueueStep(xfd_batch& b) {
     for (auto& gp : c.gathers)
       if (gp->probe < 0)
         L::gatherAt(st, c.g, F, gp->a, gp->out.as<double>(), gp->d.first, gp->d.interval, gp->d.count, t);
+    if (c.n_line)
+      L::linesAt(st, F, c.line_terms.as<xfd::LineTerm>(), c.line_off.as<int64_t>(), c.line_scale.as<double>(),
+                 c.n_line, c.line_out.as<double>(), c.line_count, t);
   }
   if (b.nf.n) L::nf2ff(st, b.nf.tab.as<xfd::BatchNf2ff<T>>(), b.nf.start.as<int64_t>(), b.nf.n, b.nf.total, b.grid, t);
   L::tick(st, b.tdev.as<int64_t>());
@@ -1731,6 +1775,15 @@ int xfd_batch_last_ms(xfd_batch* b, double* ms) {
   return guard([&] {
     require(b && ms, "bad argument");
     *ms = b->last_ms;
+  });
+}
+
+int xfd_read_line(xfd_ctx* c, int32_t m, double* dst) {
+  return guard([&] {
+    require(c && dst && m >= 0 && m < c->n_line, "bad line recording index");
+    if (!c->line_count) return;
+    XFD_HIP(hipMemcpy(dst, c->line_out.as<double>() + m * c->line_count, c->line_count * sizeof(double),
+                      hipMemcpyDeviceToHost));
   });
 }
 

@@ -256,6 +256,43 @@ __global__ __launch_bounds__(64) void k_probes(FieldPtrs<T> F, const Probe* __re
   out[q * stride + t] = static_cast<double>(F.f[p[q].field][p[q].idx]);
 }
 
+// Weighted-sum recordings (VoltageMonitor::update, voltage_monitor.cpp:
+// 191-227; CurrentMonitor::update, current_monitor.cpp:272-385): monitor m
+// sums, in term order, w * F[field][idx] into one of two partial sums and
+// records scale * (g0 + g1) for step t, in FP64.  One thread per monitor
+// keeps the reference's summation order.
+struct LineTerm {
+  int64_t idx;
+  double w;
+  int32_t field;
+  int32_t group;
+};
+
+template <class T>
+__device__ __forceinline__ void lineSum(const FieldPtrs<T>& F, const LineTerm* __restrict__ terms,
+                                        const int64_t* __restrict__ off, const double* __restrict__ scale, int m,
+                                        double* out, int64_t stride, int64_t t) {
+  double g0 = 0.0, g1 = 0.0;
+  for (int64_t q = off[m]; q < off[m + 1]; ++q) {
+    const LineTerm L = terms[q];
+    const double v = static_cast<double>(F.f[L.field][L.idx]) * L.w;
+    if (L.group == 0)
+      g0 = g0 + v;
+    else
+      g1 = g1 + v;
+  }
+  out[m * stride + t] = scale[m] * (g0 + g1);
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_lines(FieldPtrs<T> F, const LineTerm* __restrict__ terms,
+                                              const int64_t* __restrict__ off, const double* __restrict__ scale,
+                                              int n, double* out, int64_t stride, int64_t t) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n || t >= stride) return;
+  lineSum(F, terms, off, scale, m, out, stride, t);
+}
+
 // correctPML (src/include/boundary/pml_corrector.h:14-52) for both
 // components of one slab and one half-step.  IsE: psi = b psi + a (p - q)
 // with q one plane below along the main axis; H: psi = b psi + a (q - p)
@@ -721,6 +758,8 @@ struct Launch {
   static void records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
                       const double* pool, T* state, int64_t t);
   static void gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out);
+  static void lines(hipStream_t s, const FieldPtrs<T>& F, const LineTerm* terms, const int64_t* off,
+                    const double* scale, int n, double* out, int64_t stride, int64_t t);
   static void nf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Nf2ffArgs& a, int64_t t);
   // na directions, out: na x 8 doubles (A_theta, A_phi, F_theta, F_phi as re, im)
   static void far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* out);

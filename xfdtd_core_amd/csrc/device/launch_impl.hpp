@@ -221,6 +221,19 @@ void BatchLaunch<T>::probesAt(hipStream_t s, const FieldPtrs<T>& F, const Probe*
   hipLaunchKernelGGL(k_probes_at<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, F, p, n, out, stride, tdev);
 }
 template <class T>
+void BatchLaunch<T>::linesAt(hipStream_t s, const FieldPtrs<T>& F, const LineTerm* terms, const int64_t* off,
+                             const double* scale, int n, double* out, int64_t stride, const int64_t* tdev) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lines_at<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, F, terms, off, scale, n, out, stride,
+                     tdev);
+}
+template <class T>
+void Launch<T>::lines(hipStream_t s, const FieldPtrs<T>& F, const LineTerm* terms, const int64_t* off,
+                      const double* scale, int n, double* out, int64_t stride, int64_t t) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lines<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, F, terms, off, scale, n, out, stride, t);
+}
+template <class T>
 void BatchLaunch<T>::tick(hipStream_t s, int64_t* t) {
   hipLaunchKernelGGL(k_tick<T>, dim3(1), dim3(64), 0, s, t);
 }

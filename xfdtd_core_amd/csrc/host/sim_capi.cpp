@@ -248,6 +248,19 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
                                               name, dir);
       s.addMonitor(m);
       h->monitors.push_back(m);
+    } else if (d == "voltage_monitor" || d == "current_monitor") {  // NAME CUBE DIR OUT_DIR
+      std::string name = t.word();
+      auto cb = cube(t);
+      auto dr = parseDir(t.word());
+      std::string dir = t.word();
+      if (dir == "-") dir.clear();
+      std::shared_ptr<Monitor> m;
+      if (d == "voltage_monitor")
+        m = std::make_shared<VoltageMonitor>(name, std::move(cb), dr, dir);
+      else
+        m = std::make_shared<CurrentMonitor>(name, std::move(cb), dr, dir);
+      s.addMonitor(m);
+      h->monitors.push_back(m);
     } else if (d == "field_time") {
       std::string name = t.word();
       auto f = parseField(t.word());
@@ -423,7 +436,9 @@ int xfd_sim_monitor_count(void* p) { return static_cast<int>(H(p)->monitors.size
 int xfd_sim_monitor_records(void* p, int32_t m) {
   auto& mon = H(p)->monitors.at(m);
   if (auto* mv = dynamic_cast<MovieMonitor*>(mon.get())) return static_cast<int>(mv->frames().size());
-  if (dynamic_cast<FieldTimeMonitor*>(mon.get())) return 1;
+  if (dynamic_cast<FieldTimeMonitor*>(mon.get()) || dynamic_cast<VoltageMonitor*>(mon.get()) ||
+      dynamic_cast<CurrentMonitor*>(mon.get()))
+    return 1;
   return 0;
 }
 
@@ -432,6 +447,26 @@ int xfd_sim_monitor_record(void* p, int32_t m, int32_t r, int64_t* shape, double
     auto& mon = H(p)->monitors.at(m);
     if (auto* mv = dynamic_cast<MovieMonitor*>(mon.get())) {
       copyArray(mv->frames().at(r).second, shape, dst);
+      return;
+    }
+    std::vector<Real>* lt = nullptr;
+    std::vector<Real>* ld = nullptr;
+    if (auto* vm = dynamic_cast<VoltageMonitor*>(mon.get())) {
+      lt = &vm->time();
+      ld = &vm->data();
+    } else if (auto* cm = dynamic_cast<CurrentMonitor*>(mon.get())) {
+      lt = &cm->time();
+      ld = &cm->data();
+    }
+    if (lt) {  // (2, nt): time row, value row
+      const std::size_t nt = lt->size();
+      shape[0] = 2;
+      shape[1] = static_cast<int64_t>(nt);
+      shape[2] = 1;
+      if (dst) {
+        std::memcpy(dst, lt->data(), nt * sizeof(double));
+        std::memcpy(dst + nt, ld->data(), std::min(nt, ld->size()) * sizeof(double));
+      }
       return;
     }
     if (auto* ft = dynamic_cast<FieldTimeMonitor*>(mon.get())) {

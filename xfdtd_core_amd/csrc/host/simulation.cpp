@@ -256,6 +256,10 @@ MovieMonitor::MovieMonitor(std::unique_ptr<Monitor> frame, std::size_t interval,
 }
 FieldTimeMonitor::FieldTimeMonitor(std::string n, std::unique_ptr<Cube> c, std::string d, EMF::Field f)
     : Monitor(std::move(n), std::move(d)), _cube(std::move(c)), _field(f) {}
+VoltageMonitor::VoltageMonitor(std::string n, std::unique_ptr<Cube> c, Axis::Direction dir, std::string d)
+    : Monitor(std::move(n), std::move(d)), _cube(std::move(c)), _direction(dir) {}
+CurrentMonitor::CurrentMonitor(std::string n, std::unique_ptr<Cube> c, Axis::Direction dir, std::string d)
+    : Monitor(std::move(n), std::move(d)), _cube(std::move(c)), _direction(dir) {}
 
 // ---------------------------------------------------------- simulation ---
 namespace {
@@ -376,6 +380,15 @@ struct Simulation::Impl {
     bool time_series;
   };
   std::vector<GatherPlan> gathers;
+  // voltage / current monitors as weighted sums (xfd_line_desc)
+  struct LinePlan {
+    Monitor* monitor;
+    std::vector<int32_t> terms;  // field, i, j, k, group
+    std::vector<Real> weights;
+    Real scale = 1;
+  };
+  std::vector<LinePlan> lines;
+  std::vector<xfd_line_desc> ld;
 
   // Adjoint: design cells with their relative permittivity, the DFT
   // objective (a box of points of one E component at one frequency) and the
@@ -1182,7 +1195,120 @@ struct Simulation::Impl {
     }
   }
 
+  // VoltageMonitor::init / update (voltage_monitor.cpp:21-227) for one
+  // domain: E along the direction's axis c over the cube's range, the a and
+  // b ranges extended by one and kept off the grid faces, each term weighted
+  // by -dc / (na nb) (negated for a negative direction), summed in i, j, k
+  // order.
+  LinePlan voltagePlan(VoltageMonitor& vm) {
+    LinePlan p;
+    p.monitor = &vm;
+    std::size_t o[3], e[3];
+    gridBox(vm.cube(), o, e);
+    const int dir = static_cast<int>(vm.direction()), c = dir / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    std::size_t r[3][2];
+    for (int q : {a, b}) {
+      r[q][0] = o[q];
+      r[q][1] = e[q] + 1;
+      if (r[q][0] <= 0) r[q][0] = 1;
+      if (N[q] <= r[q][1]) r[q][1] = N[q] - 1;
+    }
+    r[c][0] = o[c];
+    r[c][1] = e[c];
+    for (int q = 0; q < 3; ++q)
+      if (r[q][0] >= r[q][1]) throw XFDTDException(vm.name() + " globalTask is not valid");
+    const std::size_t cnt = (r[a][1] - r[a][0]) * (r[b][1] - r[b][0]);
+    for (std::size_t i = r[0][0]; i < r[0][1]; ++i)
+      for (std::size_t j = r[1][0]; j < r[1][1]; ++j)
+        for (std::size_t k = r[2][0]; k < r[2][1]; ++k) {
+          const std::size_t pc = c == 0 ? i : (c == 1 ? j : k);
+          Real w = -h_size[c][pc] / static_cast<Real>(cnt);
+          if (dir % 2 == 0) w *= -1.0;
+          p.terms.insert(p.terms.end(), {c, static_cast<int32_t>(i), static_cast<int32_t>(j), static_cast<int32_t>(k), 0});
+          p.weights.push_back(w);
+        }
+    p.scale = 1.0;
+    vm.time() = eTime();
+    vm.data().assign(steps, 0);
+    return p;
+  }
+
+  // CurrentMonitor::init / update (current_monitor.cpp:18-385) for one
+  // domain: the H samples around the cube's last plane along the direction,
+  // weighted by +-e_size; the two loop sides are the two groups.
+  LinePlan currentPlan(CurrentMonitor& cm) {
+    LinePlan p;
+    p.monitor = &cm;
+    std::size_t o[3], e[3];
+    gridBox(cm.cube(), o, e);
+    const int dir = static_cast<int>(cm.direction()), c = dir / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    if (e[c] == 0) throw XFDTDException("CurrentMonitor::init(): global_c_e is 0");
+    if (o[a] == 0 || e[a] == N[a]) throw XFDTDException("CurrentMonitor::init(): global x range is invalid");
+    if (o[b] == 0 || e[b] == N[b]) throw XFDTDException("CurrentMonitor::init(): global y range is invalid");
+    // make_ranges (:138-169): the side along one axis exists when the other
+    // axis' face lies strictly inside the grid
+    auto ranges = [](std::size_t b_start, std::size_t b_end, std::size_t b_upper, std::size_t a_upper,
+                     std::size_t st, std::size_t en, std::size_t rn[2], std::size_t rp[2]) {
+      if (st <= 0) st = 1;
+      en = en + 1;
+      if (a_upper <= en) en = a_upper - 1;
+      rn[0] = rn[1] = rp[0] = rp[1] = 0;
+      if (b_start != 0 && b_start != b_upper - 1 && b_start != b_upper) {
+        rn[0] = st;
+        rn[1] = en;
+      }
+      if (b_end != b_upper && b_end != b_upper - 1 && b_end != 0) {
+        rp[0] = st;
+        rp[1] = en;
+      }
+    };
+    std::size_t ha_n[2], ha_p[2], hb_n[2], hb_p[2];
+    ranges(o[b], e[b], N[b], N[a], o[a], e[a], ha_n, ha_p);
+    ranges(o[a], e[a], N[a], N[b], o[b], e[b], hb_n, hb_p);
+    auto valid = [](const std::size_t* q) { return q[0] < q[1]; };
+    // da / db: e_size along a / b from the first valid side (the
+    // reference's views)
+    const std::size_t* ra = valid(ha_n) ? ha_n : ha_p;
+    const std::size_t* rb = valid(hb_n) ? hb_n : hb_p;
+    auto da = [&](std::size_t x) { return e_size[a][ra[0] + x]; };
+    auto db = [&](std::size_t x) { return e_size[b][rb[0] + x]; };
+    // node ranges: a, b the cube's; c its last plane
+    std::size_t lo[3], hi[3];
+    lo[a] = o[a];
+    hi[a] = e[a];
+    lo[b] = o[b];
+    hi[b] = e[b];
+    lo[c] = e[c] - 1;
+    hi[c] = e[c];
+    auto add = [&](int field, std::size_t i, std::size_t j, std::size_t k, Real w, int group) {
+      p.terms.insert(p.terms.end(), {field, static_cast<int32_t>(i), static_cast<int32_t>(j), static_cast<int32_t>(k), group});
+      p.weights.push_back(w);
+    };
+    const std::size_t is = lo[0], ie = hi[0], js = lo[1], je = hi[1], ks = lo[2], ke = hi[2];
+    if (c == 0) {  // integral_y (group 0) then integral_z (group 1)
+      if (valid(ha_n)) for (std::size_t j = ha_n[0]; j < ha_n[1]; ++j) add(4, ie - 1, j, ks - 1, da(j - ha_n[0]), 0);
+      if (valid(ha_p)) for (std::size_t j = ha_p[0]; j < ha_p[1]; ++j) add(4, ie - 1, j, ke, -da(j - ha_p[0]), 0);
+      if (valid(hb_n)) for (std::size_t k = hb_n[0]; k < hb_n[1]; ++k) add(5, ie - 1, js - 1, k, -db(k - hb_n[0]), 1);
+      if (valid(hb_p)) for (std::size_t k = hb_p[0]; k < hb_p[1]; ++k) add(5, ie - 1, je, k, db(k - hb_p[0]), 1);
+    } else if (c == 1) {  // integral_z (group 0) then integral_x (group 1)
+      if (valid(ha_n)) for (std::size_t k = ha_n[0]; k < ha_n[1]; ++k) add(5, is - 1, je - 1, k, da(k - ha_n[0]), 0);
+      if (valid(ha_p)) for (std::size_t k = ha_p[0]; k < ha_p[1]; ++k) add(5, ie, je - 1, k, -da(k - ha_p[0]), 0);
+      if (valid(hb_n)) for (std::size_t i = hb_n[0]; i < hb_n[1]; ++i) add(3, i, je - 1, ks - 1, -db(i - hb_n[0]), 1);
+      if (valid(hb_p)) for (std::size_t i = hb_p[0]; i < hb_p[1]; ++i) add(3, i, je - 1, ke, db(i - hb_p[0]), 1);
+    } else {  // integral_x (group 0) then integral_y (group 1)
+      if (valid(ha_n)) for (std::size_t i = ha_n[0]; i < ha_n[1]; ++i) add(3, i, js - 1, ke - 1, da(i - ha_n[0]), 0);
+      if (valid(ha_p)) for (std::size_t i = ha_p[0]; i < ha_p[1]; ++i) add(3, i, je, ke - 1, -da(i - ha_p[0]), 0);
+      if (valid(hb_n)) for (std::size_t j = hb_n[0]; j < hb_n[1]; ++j) add(4, is - 1, j, ke - 1, -db(j - hb_n[0]), 1);
+      if (valid(hb_p)) for (std::size_t j = hb_p[0]; j < hb_p[1]; ++j) add(4, ie, j, ke - 1, db(j - hb_p[0]), 1);
+    }
+    p.scale = dir % 2 == 1 ? 1.0 : -1.0;
+    cm.time() = hTime();
+    cm.data().assign(steps, 0);
+    return p;
+  }
+
   void buildMonitors() {
+    lines.clear();
     for (auto& m : monitors) {
       GatherPlan gp;
       gp.monitor = m.get();
@@ -1210,6 +1336,12 @@ struct Simulation::Impl {
         gp.time_series = true;
         ft->time() = ft->field() <= EMF::Field::EM ? eTime() : hTime();
         ft->data().assign(steps, 0);
+      } else if (auto* vm = dynamic_cast<VoltageMonitor*>(m.get())) {
+        lines.push_back(voltagePlan(*vm));
+        continue;
+      } else if (auto* cm = dynamic_cast<CurrentMonitor*>(m.get())) {
+        lines.push_back(currentPlan(*cm));
+        continue;
       } else {
         throw XFDTDException("unsupported monitor type");
       }
@@ -1375,6 +1507,18 @@ struct Simulation::Impl {
     }
     dd.n_nf2ff = static_cast<int32_t>(nf2ff_d.size());
     dd.nf2ff = nf2ff_d.empty() ? nullptr : nf2ff_d.data();
+    ld.clear();
+    for (auto& l : lines) {
+      xfd_line_desc x{};
+      x.n_terms = static_cast<int32_t>(l.weights.size());
+      x.terms = l.terms.data();
+      x.weights = l.weights.data();
+      x.scale = l.scale;
+      x.count = static_cast<int64_t>(steps);
+      ld.push_back(x);
+    }
+    dd.n_line = static_cast<int32_t>(ld.size());
+    dd.line = ld.empty() ? nullptr : ld.data();
     return dd;
   }
 
@@ -1599,6 +1743,26 @@ struct Simulation::Impl {
           if (f + 1 == g.d.count) mv->frame().data() = a;
           frames.emplace_back(name, std::move(a));
         }
+      }
+    }
+    for (std::size_t m = 0; m < lines.size(); ++m) {  // voltage / current monitors
+      std::vector<Real> buf(steps);
+      if (!buf.empty() && xfd_read_line(ctx, static_cast<int32_t>(m), buf.data()) != XFD_OK)
+        throw XFDTDSimulationException(std::string("xfd_read_line: ") + xfd_last_error());
+      std::vector<Real>* time = nullptr;
+      if (auto* vm = dynamic_cast<VoltageMonitor*>(lines[m].monitor)) {
+        vm->data() = buf;
+        time = &vm->time();
+      } else if (auto* cm = dynamic_cast<CurrentMonitor*>(lines[m].monitor)) {
+        cm->data() = buf;
+        time = &cm->time();
+      }
+      const auto& dir = lines[m].monitor->outputDir();
+      if (time && !dir.empty()) {  // time_monitor.cpp:11-21: stack(time, data)
+        std::vector<Real> st(*time);
+        st.insert(st.end(), buf.begin(), buf.end());
+        std::filesystem::create_directories(dir);
+        npy::write(dir + "/" + lines[m].monitor->name() + ".npy", {2, time->size()}, st.data());
       }
     }
     // final fields (Simulation::emf)

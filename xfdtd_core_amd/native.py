@@ -88,6 +88,8 @@ def load_library() -> ctypes.CDLL:
         "xfd_device_bytes": (ctypes.c_int, [vp, p64]),
         "xfd_coef_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), p64]),
         "xfd_reset": (ctypes.c_int, [vp]),
+        "xfd_read_line": (ctypes.c_int, [vp, i32, vp]),
+        "xfd_nf2ff_far_field": (ctypes.c_int, [vp, i32, i32, i32, vp, vp, vp, dbl, vp, vp, vp, vp]),
         "xfd_batch_create": (ctypes.c_int, [i32, vp, ctypes.POINTER(vp)]),
         "xfd_batch_destroy": (None, [vp]),
         "xfd_batch_case": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
@@ -187,7 +189,8 @@ class Desc(ctypes.Structure):
                 ("n_lumped", ctypes.c_int32), ("lumped", ctypes.POINTER(LumpedDesc)),
                 ("n_tfsf", ctypes.c_int32), ("tfsf", ctypes.POINTER(TfsfDesc)),
                 ("n_gather", ctypes.c_int32), ("gather", ctypes.POINTER(GatherDesc)),
-                ("ade", ctypes.c_void_p), ("n_nf2ff", ctypes.c_int32), ("nf2ff", ctypes.c_void_p)]
+                ("ade", ctypes.c_void_p), ("n_nf2ff", ctypes.c_int32), ("nf2ff", ctypes.c_void_p),
+                ("n_line", ctypes.c_int32), ("line", ctypes.c_void_p)]
 
 
 def _ptr(a: np.ndarray):

@@ -27,6 +27,10 @@ renders the scenario as line-oriented text, one directive per line:
     movie FIELD OX OY OZ SX SY SZ INTERVAL NAME OUT_DIR
     nf2ff DX DY DZ NF F1 .. FNF OUT_DIR [plain_dft]
     field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
+    voltage_monitor NAME OX OY OZ SX SY SZ DIR OUT_DIR
+    current_monitor NAME OX OY OZ SX SY SZ DIR OUT_DIR
+    design_region OI OJ OK NI NJ NK EPS_R...      (cells; adjoint form)
+    dft_objective FIELD OI OJ OK NI NJ NK FREQ    (grid points; adjoint form)
     WAVEFORM = gaussian TAU T0 AMP | cosine_modulated_gaussian TAU T0 F AMP
              | sine F AMP | cosine F AMP | step T0 AMP
              | gaussian_derivative TAU T0 AMP
@@ -398,6 +402,38 @@ class FieldTimeMonitor:
     def line(self):
         return " ".join(["field_time", self.name, self.field] + self.cube.box_tokens() +
                         [self.output_dir or "-"])
+
+
+@dataclass
+class VoltageMonitor:
+    """``VoltageMonitor(name, cube, direction, output_dir)``
+    (monitor/voltage_monitor.h): every step, the line integral of E along
+    `direction` through the cube, averaged over its cross-section."""
+
+    name: str
+    cube: Cube
+    direction: str
+    output_dir: str = ""
+
+    def line(self):
+        return " ".join(["voltage_monitor", self.name] + self.cube.box_tokens() +
+                        [self.direction, self.output_dir or "-"])
+
+
+@dataclass
+class CurrentMonitor:
+    """``CurrentMonitor(name, cube, direction, output_dir)``
+    (monitor/current_monitor.h): every step, the loop integral of H around
+    the cube's last cross-section along `direction`."""
+
+    name: str
+    cube: Cube
+    direction: str
+    output_dir: str = ""
+
+    def line(self):
+        return " ".join(["current_monitor", self.name] + self.cube.box_tokens() +
+                        [self.direction, self.output_dir or "-"])
 
 
 @dataclass
