@@ -133,18 +133,18 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
     const T ey_k = ey[idx + 1], ey_n = ey[idx + pl];
     const T ez_j = ez[idx + pz], ez_n = ez[idx + pl];
     T s, a, b;
-    C.get(3, idx, s, a, b);
-    T vx = s * hx[idx] + a * (ey_k - ey_i) + b * (ez_j - ez_i);
-    C.get(4, idx, s, a, b);
-    T vy = s * hy[idx] + a * (ez_n - ez_i) + b * (ex_k - ex0);
-    C.get(5, idx, s, a, b);
-    T vz = s * hz[idx] + a * (ex_j - ex0) + b * (ey_n - ey_i);
+    C.getOwn(3, idx, s, a, b);
+    T vx = s * ldOwn(hx + idx) + a * (ey_k - ey_i) + b * (ez_j - ez_i);
+    C.getOwn(4, idx, s, a, b);
+    T vy = s * ldOwn(hy + idx) + a * (ez_n - ez_i) + b * (ex_k - ex0);
+    C.getOwn(5, idx, s, a, b);
+    T vz = s * ldOwn(hz + idx) + a * (ex_j - ex0) + b * (ey_n - ey_i);
     if (Fused && mask)  // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
       cpmlFused(P, mask, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
                 ex_k - ex0);
-    hx[idx] = vx;
-    hy[idx] = vy;
-    hz[idx] = vz;
+    stOwn(hx + idx, vx);
+    stOwn(hy + idx, vy);
+    stOwn(hz + idx, vz);
     ey_i = ey_n;
     ez_i = ez_n;
   }
@@ -189,16 +189,16 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
     T s, a, b;
     if (!Fused || !mask) {
       if (jk) {
-        C.get(0, idx, s, a, b);
-        ex[idx] = s * ex[idx] + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
+        C.getOwn(0, idx, s, a, b);
+        stOwn(ex + idx, s * ldOwn(ex + idx) + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]));
       }
       if (i >= 1 && k >= 1) {
-        C.get(1, idx, s, a, b);
-        ey[idx] = s * ey[idx] + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
+        C.getOwn(1, idx, s, a, b);
+        stOwn(ey + idx, s * ldOwn(ey + idx) + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]));
       }
       if (i >= 1 && j >= 1) {
-        C.get(2, idx, s, a, b);
-        ez[idx] = s * ez[idx] + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
+        C.getOwn(2, idx, s, a, b);
+        stOwn(ez + idx, s * ldOwn(ez + idx) + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p));
       }
     } else {
       // curl differences: x uses the carried plane, y and z the j-1 / k-1
@@ -210,15 +210,15 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
       const T dxHz = hz0 - hz_p, dxHy = hy0 - hy_p;
       T vx = ex[idx], vy = ey[idx], vz = ez[idx];
       if (jk) {
-        C.get(0, idx, s, a, b);
+        C.getOwn(0, idx, s, a, b);
         vx = s * vx + a * dzHy + b * dyHz;
       }
       if (i >= 1 && k >= 1) {
-        C.get(1, idx, s, a, b);
+        C.getOwn(1, idx, s, a, b);
         vy = s * vy + a * dxHz + b * dzHx;
       }
       if (i >= 1 && j >= 1) {
-        C.get(2, idx, s, a, b);
+        C.getOwn(2, idx, s, a, b);
         vz = s * vz + a * dyHx + b * dxHy;
       }
       // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
@@ -308,19 +308,19 @@ __device__ __forceinline__ void cpmlPoint(const Grid& g, const FieldPtrs<T>& F, 
   const int pa = pick3(a, p0, p1, p2), pb = pick3(b, p0, p1, p2);
   if (pa < s.extA[0] && pb < s.extA[1]) {
     const T* dual = F.f[s.dualB];
-    T psi = s.psiA[t];
+    T psi = ldOwn(s.psiA + t);
     psi = IsE ? cb * psi + ca * (dual[idx] - dual[q]) : cb * psi + ca * (dual[q] - dual[idx]);
-    s.psiA[t] = psi;
+    stOwn(s.psiA + t, psi);
     T* f = F.f[s.fieldA];
-    f[idx] += s.cpsiA[t] * psi;
+    f[idx] += ldOwn(s.cpsiA + t) * psi;
   }
   if (pa < s.extB[0] && pb < s.extB[1]) {
     const T* dual = F.f[s.dualA];
-    T psi = s.psiB[t];
+    T psi = ldOwn(s.psiB + t);
     psi = IsE ? cb * psi + ca * (dual[idx] - dual[q]) : cb * psi + ca * (dual[q] - dual[idx]);
-    s.psiB[t] = psi;
+    stOwn(s.psiB + t, psi);
     T* f = F.f[s.fieldB];
-    f[idx] += s.cpsiB[t] * psi;
+    f[idx] += ldOwn(s.cpsiB + t) * psi;
   }
 }
 
@@ -458,19 +458,19 @@ __device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, 
     const int m = A.mat[idx];
     T s, a, b;
     if (jk) {
-      C.get(0, idx, s, a, b);
+      C.getOwn(0, idx, s, a, b);
       const T cur = ex[idx];
       const T en = s * cur + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
       ex[idx] = m < 0 ? en : adeUpdate<T, S>(A, m, 0, idx, cur, en);
     }
     if (i >= 1 && k >= 1) {
-      C.get(1, idx, s, a, b);
+      C.getOwn(1, idx, s, a, b);
       const T cur = ey[idx];
       const T en = s * cur + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
       ey[idx] = m < 0 ? en : adeUpdate<T, S>(A, m, 1, idx, cur, en);
     }
     if (i >= 1 && j >= 1) {
-      C.get(2, idx, s, a, b);
+      C.getOwn(2, idx, s, a, b);
       const T cur = ez[idx];
       const T en = s * cur + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
       ez[idx] = m < 0 ? en : adeUpdate<T, S>(A, m, 2, idx, cur, en);
