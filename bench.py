@@ -41,7 +41,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: bool = False) -> float:
+def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` on an n^3 grid, the
     per-cell-step figures of the leapfrog step split over its launches:
     B_yee = (6 field reads + 6 field writes + 18 coefficient reads) * s per
@@ -49,15 +49,18 @@ def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: bool = Fa
     own 3 reads, 3 writes and 9 coefficients; the 3 dual-field reads a
     separate sweep adds are actual, not algorithmic, traffic); CPML = 12 * s
     per slab cell and step (4 components x psi read + write + c_psi read), so
-    one slab launch (2 components) owns 6 * s per slab cell.  With the CPML
-    inside the sweeps the sweep also owns its slabs' share and the separate
-    launches only finish the far grid planes (not counted)."""
+    one slab launch (2 components) owns 6 * s per slab cell.  Slabs applied
+    inside the sweeps (`fused` 1: the four x/y slabs, 2: the two z slabs)
+    move their share to the sweep; their separate launches only finish the
+    far grid planes (not counted), and the CPML figure is the mean over the
+    six launches of a half-step."""
     cells = n ** 3
     slab_cells = pml * (n + 1) * (n + 1)
+    inside = {0: 0, 1: 4, 2: 2}[int(fused)]
     if kernel in ("update_h", "update_e"):
-        return 15.0 * s * cells + (6.0 * s * 6 * slab_cells if fused else 0.0)
+        return 15.0 * s * cells + 6.0 * s * inside * slab_cells
     if kernel in ("cpml_h", "cpml_e"):
-        return 0.0 if fused else 6.0 * s * slab_cells
+        return 6.0 * s * slab_cells * (6 - inside) / 6.0
     return 0.0
 
 
@@ -146,7 +149,8 @@ def run_device(sim, precision, warmup, steps, n, pml):
                 kernels[k]["layout_bytes"] = ib
                 kernels[k]["layout_GBps"] = round(ib / (kernels[k]["avg_ms"] * 1e-3) / 1e9, 1)
     res["coefficients"] = "class ids" if sim.coef_classed() else "dense"
-    res["cpml"] = "inside the Yee sweeps" if fused else "separate slab launches"
+    res["cpml"] = {0: "separate slab launches", 1: "x/y slabs inside the Yee sweeps",
+                   2: "z slabs inside the Yee sweeps, x/y slabs separate"}[int(fused)]
     # recordings for the accuracy comparison: six snapshot series, then probes
     sim.finish()
     res["snapshots"] = {f: sim.monitor_records(i) for i, f in enumerate(FIELDS)}

@@ -235,12 +235,19 @@ int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
  * environment variable XFD_COEF_MODE=dense forces the dense layout. */
 int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
 
-/* *fused = 1 when the Yee sweeps apply the leading x/y CPML slabs in the
- * same pass (same operations in the same order, fewer bytes moved; z slabs
- * and the far grid planes stay separate launches).  Opt-in with the
- * environment variable XFD_CPML_FUSED=1 (the separate launches are faster on
- * MI355X); never with ADE materials or when a TFSF or lumped correction lands
- * inside a slab (the reference applies those first). */
+/* Which CPML slabs the Yee sweeps apply in the same pass (the far grid
+ * planes always stay separate launches); no reference counterpart, the
+ * reference runs every slab as its own corrector (pml_corrector.h:14-52).
+ *   *fused = 0: none, every slab is a separate launch;
+ *   *fused = 1: the leading x/y slabs (same operations in the same order, so
+ *     FP64 stays bitwise); opt-in with XFD_CPML_FUSED=1 (slower on MI355X);
+ *   *fused = 2: the z slabs, FP32 contexts only: by default when a k-row
+ *     spans at least 4 sweep tiles (256 cells), where it is faster;
+ *     XFD_CPML_ZFUSED=1 forces it, =0 turns it off.  Their corrections then
+ *     precede the x/y slabs', which changes only the FP32 rounding where
+ *     slabs overlap.
+ * Never with ADE materials or when a TFSF or lumped correction lands inside
+ * a slab (the reference applies those first). */
 int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);
 
 /* Zeroes a context's time-stepping state: fields, CPML auxiliaries, inductor

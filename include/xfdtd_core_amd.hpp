@@ -594,8 +594,10 @@ class Simulation {
   // True when the device holds per-cell coefficient class ids instead of the
   // 18 dense coefficient arrays (same values; chosen at init).
   bool classedCoefficients() const;
-  // True when the Yee sweeps apply the CPML corrections in the same pass.
-  bool fusedCpml() const;
+  // Which CPML slabs the Yee sweeps apply in the same pass: 0 none, 1 the
+  // leading x/y slabs (XFD_CPML_FUSED=1), 2 the z slabs (FP32, nz > 192 or
+  // XFD_CPML_ZFUSED=1).
+  int fusedCpml() const;
 
   // ---- Adjoint (no reference counterpart) ----
   // Per-cell relative permittivity of the design cells [origin, origin+size)

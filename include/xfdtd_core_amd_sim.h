@@ -42,7 +42,7 @@ int xfd_sim_grid(void* sim, int64_t* n3);
 int64_t xfd_sim_device_bytes(void* sim);
 /* 1 when the device holds coefficient class ids (xfd_coef_layout), else 0. */
 int32_t xfd_sim_coef_classed(void* sim);
-/* 1 when the CPML corrections run inside the Yee sweeps (xfd_cpml_fused). */
+/* Which CPML slabs run inside the Yee sweeps: 0, 1 or 2 as xfd_cpml_fused. */
 int32_t xfd_sim_cpml_fused(void* sim);
 
 /* NF2FF (NFFFTFrequencyDomain) results after run() or finish(), in the

@@ -313,9 +313,10 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
     fused.run(200)
     assert fused.cpml_fused()
     monkeypatch.setenv("XFD_CPML_FUSED", "0")
+    monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
     sep = native.Simulation(sp, precision)
     sep.run(200)
-    assert not sep.cpml_fused()
+    assert sep.cpml_fused() == 0
     for f in FIELDS:
         a, b = fused.field(f), sep.field(f)
         if precision == "f64":
@@ -328,7 +329,72 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
     disp = native.Simulation(scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9,
                                                         lumped=False).spec(), precision)
     disp.init(2)
-    assert not disp.cpml_fused()
+    assert disp.cpml_fused() == 0
+
+
+def _tall_box(nz_total, pml=8, n_xy=24):
+    """A vacuum box of n_xy x n_xy x nz_total cells including the CPML."""
+    dl = 1e-3
+    s = S.Simulation(dl, dl, dl, 0.9)
+    n_in = [n_xy - 2 * pml, n_xy - 2 * pml, nz_total - 2 * pml]
+    s.addObject(S.Object("domain", S.Cube(S.Vector(*[-a * dl / 2 for a in n_in]), S.Vector(*[a * dl for a in n_in])),
+                         S.Material.createAir()))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(pml, d))
+    return s.spec()
+
+
+def test_z_cpml_inside_fp32_sweeps(monkeypatch):
+    """FP32 contexts apply the z CPML slabs inside the Yee sweeps when a
+    k-row spans at least 4 sweep tiles (or when forced; only the far grid
+    planes stay separate launches).  Forced on a 48^3 box, where one
+    wavefront holds lanes of both z slabs: against the all-separate FP32 run
+    the fields differ by rounding only, and both are as close to the FP64 run
+    (bitwise equal to the oracle), in the whole box and in the z boundary
+    layers alone.  FP64 contexts never do it, and contexts with ADE keep
+    separate slabs."""
+    sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
+    say("[gpu] z CPML inside the FP32 sweeps vs separate")
+    monkeypatch.delenv("XFD_CPML_FUSED", raising=False)
+    monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
+    for nz, want in ((192, 0), (193, 2)):  # 3 and 4 tiles of 64 k-lanes
+        tall = native.Simulation(_tall_box(nz), "f32")
+        tall.init(2)
+        assert tall.cpml_fused() == want, (nz, tall.cpml_fused())
+        tall.close()
+    auto = native.Simulation(sp, "f32")
+    auto.init(2)
+    assert auto.cpml_fused() == 0
+    auto.close()
+    monkeypatch.setenv("XFD_CPML_ZFUSED", "1")
+    zf = native.Simulation(sp, "f32")
+    zf.run(200)
+    assert zf.cpml_fused() == 2
+    ref = native.Simulation(sp, "f64")
+    ref.run(200)
+    assert ref.cpml_fused() == 0
+    monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
+    sep = native.Simulation(sp, "f32")
+    sep.run(200)
+    assert sep.cpml_fused() == 0
+
+    def layers(x):  # the z slabs' 8 layers at both ends
+        return np.concatenate([x[:, :, :9].ravel(), x[:, :, -9:].ravel()])
+
+    for f in FIELDS:
+        a, b, r = zf.field(f), sep.field(f), ref.field(f)
+        d, e_sep, e_zf = rel_l2(a, b), rel_l2(b, r), rel_l2(a, r)
+        l_sep, l_zf = rel_l2(layers(b), layers(r)), rel_l2(layers(a), layers(r))
+        say(f"  {f}: z-fused vs separate {d:.2e}; vs f64: separate {e_sep:.2e}, z-fused {e_zf:.2e}; "
+            f"z layers: separate {l_sep:.2e}, z-fused {l_zf:.2e}")
+        assert d < 1e-4, (f, d)
+        assert e_zf <= 2 * e_sep + 1e-6, (f, e_zf, e_sep)
+        assert l_zf <= 2 * l_sep + 1e-6, (f, l_zf, l_sep)
+    monkeypatch.setenv("XFD_CPML_ZFUSED", "1")
+    disp = native.Simulation(scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9,
+                                                        lumped=False).spec(), "f32")
+    disp.init(2)
+    assert disp.cpml_fused() == 0
 
 
 def test_far_field_on_device_matches_host(monkeypatch):

@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -273,6 +274,9 @@ struct xfd_ctx {
   // of each half-step as device arrays
   bool fused = false;
   int n_fused = 0;  // slabs [0, n_fused) run inside the sweeps, the rest separately
+  // FP32: the z slabs run inside the sweeps (zApply), the rest separately
+  bool zfused = false;
+  std::vector<char> zslab;  // per slab: applied inside the sweeps
   DevBuf cpml_list_e, cpml_list_h;
 
   // checkpoints of a forward run for the adjoint (xfd_forward_checkpointed)
@@ -857,6 +861,38 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       uploadVector(le, c.cpml_list_e);
       uploadVector(lh, c.cpml_list_h);
     }
+    // FP32: the z slabs inside the sweeps.  A z slab holds 8-12 cells of each
+    // 128-byte k-row, so a separate launch moves 2-2.5x its bytes in partial
+    // lines, while inside the sweep the field and dual values are already in
+    // registers and only psi and c_psi are read.  The slab lanes' extra work
+    // slows the wavefronts that carry them, so it pays only when few do:
+    // measured, the 384^3 box (6 k-tiles per row) steps 1.363 ms with it and
+    // 1.401 ms without, while the batched design sweep (nz 48-160, 1-3
+    // k-tiles) steps 3.66 ms with it and 3.41 ms without.  Default: rows of at
+    // least 4 k-tiles; XFD_CPML_ZFUSED=1 forces it, =0 turns it off.  The
+    // corrections then precede the x/y slabs' instead of following them,
+    // which changes only the rounding where slabs overlap: FP64 keeps the
+    // reference's order (and its bitwise results).  Never together with
+    // XFD_CPML_FUSED=1: a batched sweep applies one kind.
+    const char* zenv = std::getenv("XFD_CPML_ZFUSED");
+    const bool zoff = zenv && zenv[0] == '0';
+    const bool zforce = zenv && zenv[0] == '1';
+    const bool zwide = (g.nz + xfd::kLanesK - 1) / xfd::kLanesK >= 4;
+    std::vector<int> zq;
+    for (size_t q = 0; q < c.slabs.size(); ++q)
+      if (c.slabs[q]->axis == 2) zq.push_back(static_cast<int>(q));
+    bool disjoint = true;
+    if (zq.size() == 2) {
+      const auto& s0 = *c.slabs[zq[0]];
+      const auto& s1 = *c.slabs[zq[1]];
+      disjoint = (s0.e_start + s0.n <= s1.e_start || s1.e_start + s1.n <= s0.e_start) &&
+                 (s0.h_start + s0.n <= s1.h_start || s1.h_start + s1.n <= s0.h_start);
+    }
+    c.zfused = std::is_same<T, float>::value && !zoff && (zforce || zwide) && off && !zq.empty() &&
+               zq.size() <= 2 && disjoint && c.ade_scheme < 0 && !records_meet_slabs;
+    c.zslab.assign(c.slabs.size(), 0);
+    if (c.zfused)
+      for (int q : zq) c.zslab[q] = 1;
   }
   XFD_HIP(hipStreamSynchronize(c.stream));
 }
@@ -906,6 +942,43 @@ xfd::SlabArgs<T> slabArgs(xfd_ctx::Slab& s, bool is_e) {
   return a;
 }
 
+// The z slabs a context applies inside its FP32 sweeps, per half-step.  The
+// sweep's z path assumes a slab spanning the whole x/y range whose two
+// components cover every sweep point of its layers; the slabs built from an
+// xfd_cpml_desc always do (box lo 0 and extent N + 1 across, component
+// extents N or N + 1), and this checks it.
+template <class T>
+xfd::ZSlab<T> zSlab(const xfd::SlabArgs<T>& a, const xfd::Grid& g) {
+  require(a.axis == 2 && a.lo[0] == 0 && a.lo[1] == 0 && a.ext[0] == g.nx + 1 && a.ext[1] == g.ny + 1 &&
+              a.extA[0] >= g.nx && a.extA[1] >= g.ny && a.extB[0] >= g.nx && a.extB[1] >= g.ny &&
+              a.lo[2] == a.start && a.ext[2] == a.n,
+          "z CPML slab shape");
+  require(static_cast<int64_t>(a.ext[0]) * a.ext[1] * a.ext[2] < (int64_t{1} << 31), "z CPML slab too large");
+  xfd::ZSlab<T> z{};
+  z.klo = a.start;
+  z.khi = a.start + a.n;
+  z.ext1 = a.ext[1];
+  z.ext2 = a.ext[2];
+  z.cpsiA = a.cpsiA;
+  z.cpsiB = a.cpsiB;
+  z.psiA = a.psiA;
+  z.psiB = a.psiB;
+  z.ca = a.ca;
+  z.cb = a.cb;
+  return z;
+}
+
+template <class T>
+void zCpml(xfd_ctx& c, xfd::ZCpml<T>& ze, xfd::ZCpml<T>& zh) {
+  ze = {};
+  zh = {};
+  for (size_t q = 0; q < c.slabs.size(); ++q) {
+    if (!c.zslab[q]) continue;
+    ze.z[ze.n++] = zSlab<T>(slabArgs<T>(*c.slabs[q], true), c.g);
+    zh.z[zh.n++] = zSlab<T>(slabArgs<T>(*c.slabs[q], false), c.g);
+  }
+}
+
 template <class T>
 void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   require(t0 >= 0 && n >= 0, "bad step range");
@@ -944,12 +1017,15 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
     PE = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
     PH = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), c.n_fused};
   }
+  xfd::ZCpml<T> ZE{}, ZH{};
+  if (c.zfused) zCpml<T>(c, ZE, ZH);
+  auto inSweep = [&](size_t q) { return static_cast<int>(q) < c.n_fused || (c.zfused && c.zslab[q]); };
   for (int64_t t = t0; t < t0 + n; ++t) {
     c.timed(K_UPDATE_H, [&] {
       if (c.classed)
-        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH);
+        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH, ZH);
       else
-        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk, PH);
+        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk, PH, ZH);
     });
     if (c.nseg_h)
       c.timed(K_REC_H, [&] {
@@ -958,7 +1034,7 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (size_t q = 0; q < sh.size(); ++q)
       c.timed(K_CPML_H, [&] {
-        if (static_cast<int>(q) < c.n_fused)
+        if (inSweep(q))
           xfd::Launch<T>::cpml_faces(st, c.g, F, sh[q], false);
         else
           xfd::Launch<T>::cpml(st, c.g, F, sh[q], false);
@@ -970,9 +1046,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         else
           xfd::Launch<T>::update_e_ade(st, c.g, F, C, A, c.ade_scheme, c.ichunk);
       } else if (c.classed) {
-        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk, PE);
+        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk, PE, ZE);
       } else {
-        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk, PE);
+        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk, PE, ZE);
       }
     });
     if (c.nseg_e)
@@ -982,7 +1058,7 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (size_t q = 0; q < se.size(); ++q)
       c.timed(K_CPML_E, [&] {
-        if (static_cast<int>(q) < c.n_fused)
+        if (inSweep(q))
           xfd::Launch<T>::cpml_faces(st, c.g, F, se[q], true);
         else
           xfd::Launch<T>::cpml(st, c.g, F, se[q], true);
@@ -1235,7 +1311,7 @@ struct xfd_batch {
   Table sweep, rec_e, rec_h, nf;
   std::vector<Table> cpml_e, cpml_h;            // slabs stepped by separate launches
   std::vector<Table> faces_e, faces_h;          // far planes of slabs fused into the sweeps
-  bool any_fused = false;                       // some case applies slabs inside its sweeps
+  int fused_mode = 0;  // sweeps with slabs inside: 1 leading slabs (some case), 2 z slabs
   int64_t pool_rows = INT64_MAX;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -1288,7 +1364,7 @@ int64_t groupsOf(int64_t items) { return (items + xfd::kBatchGroupItems - 1) / x
 // Builds the entry tables of every batched launch for the current ichunk.
 template <class T>
 void batchTables(xfd_batch& b) {
-  b.any_fused = false;
+  b.fused_mode = 0;
   Collect<xfd::BatchCase<T>> sw;
   std::vector<Collect<xfd::BatchSlab<T>>> ce, ch, fe, fh;
   Collect<xfd::BatchRec<T>> re, rh;
@@ -1304,11 +1380,16 @@ void batchTables(xfd_batch& b) {
       bc.K.tab[q] = c.tabs[q].as<T>();
     }
     bc.ichunk = b.ichunk;
-    b.any_fused = b.any_fused || c.fused;
     if (c.fused) {
+      b.fused_mode = 1;
       bc.ph = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), c.n_fused};
       bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
+    } else if (c.zfused) {
+      b.fused_mode = 2;
+      zCpml<T>(c, bc.ze, bc.zh);
     }
+    require(!(c.fused && b.fused_mode == 2) && !(c.zfused && b.fused_mode == 1),
+            "cases mix CPML-in-sweep kinds");
     bc.nbx = (c.g.nz + xfd::kLanesK - 1) / xfd::kLanesK;
     bc.nby = (c.g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ;
     bc.nbz = (c.g.nx + b.ichunk - 1) / b.ichunk;
@@ -1324,7 +1405,7 @@ void batchTables(xfd_batch& b) {
       for (int is_e = 0; is_e < 2; ++is_e) {
         xfd::BatchSlab<T> e{c.g, F, slabArgs<T>(*c.slabs[o], is_e != 0)};
         e.s.stride_c = stride[e.s.axis];
-        if (static_cast<int>(o) < c.n_fused) {  // only the far-plane remainder
+        if (static_cast<int>(o) < c.n_fused || (c.zfused && c.zslab[o])) {  // only the far-plane remainder
           const auto fp = xfd::facePieces(c.g, e.s);
           (is_e ? fe : fh)[o].add(e, groupsOf(fp.nX + fp.nY + fp.nZ));
         } else {
@@ -1367,7 +1448,7 @@ void batchEnqueueStep(xfd_batch& b) {
   const int64_t* t = b.tdev.as<int64_t>();
   auto sweep = [&](bool is_h) {
     L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n, b.sweep.total,
-             b.grid, b.any_fused);
+             b.grid, b.fused_mode);
   };
   auto records = [&](xfd_batch::Table& r) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
@@ -1448,7 +1529,7 @@ void batchTune(xfd_batch& b, int reps) {
       auto pair = [&] {
         for (bool is_h : {true, false})
           L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
-                   b.sweep.total, b.grid, b.any_fused);
+                   b.sweep.total, b.grid, b.fused_mode);
       };
       pair();  // warm-up
       XFD_HIP(hipEventRecord(b.ev0, b.stream));
@@ -1679,7 +1760,7 @@ int xfd_nf2ff_far_field(xfd_ctx* c, int32_t n, int32_t f, int32_t na, const doub
 int xfd_cpml_fused(xfd_ctx* c, int32_t* fused) {
   return guard([&] {
     require(c && fused, "bad argument");
-    *fused = c->fused ? 1 : 0;
+    *fused = c->fused ? 1 : (c->zfused ? 2 : 0);
   });
 }
 

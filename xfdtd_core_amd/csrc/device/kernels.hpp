@@ -104,14 +104,97 @@ __device__ __forceinline__ void cpmlFused(const CpmlList<T>& P, uint32_t mask, i
   }
 }
 
+// A sweep lane's view of the z slabs (ZCpml): whether its k lies in one and
+// in which, that layer's coefficients, and the box-local psi index of the
+// lane's point, advanced by one x-plane per i.  The slab arrays are picked
+// at use with value selects between the two uniform pointers (no
+// runtime-indexed aggregate, which would go to scratch memory, and no
+// long-lived per-lane pointers).
+template <class T>
+struct ZLane {
+  bool in = false, sel = false;
+  int t = 0, step = 0;
+  T ca = T(0), cb = T(0);
+};
+
+template <class T>
+__device__ __forceinline__ ZLane<T> zLane(const ZCpml<T>& Z, int i0, int j, int k) {
+  ZLane<T> L;
+  const ZSlab<T>& s0 = Z.z[0];
+  const ZSlab<T>& s1 = Z.z[1];
+  const bool in0 = Z.n > 0 && k >= s0.klo && k < s0.khi;
+  const bool in1 = Z.n > 1 && k >= s1.klo && k < s1.khi;
+  L.in = in0 || in1;
+  L.sel = in1;
+  if (!L.in) return L;
+  const int klo = in1 ? s1.klo : s0.klo;
+  const int ext1 = in1 ? s1.ext1 : s0.ext1, ext2 = in1 ? s1.ext2 : s0.ext2;
+  const T* ca = in1 ? s1.ca : s0.ca;
+  const T* cb = in1 ? s1.cb : s0.cb;
+  L.ca = ca[k - klo];
+  L.cb = cb[k - klo];
+  L.step = ext1 * ext2;
+  L.t = (i0 * ext1 + j) * ext2 + (k - klo);
+  return L;
+}
+
+// The z-slab corrections of the sweep point (PMLCorrector order within the
+// slab: component A, then B):
+//   psi = b psi + a d,  v += c_psi psi,
+// with d the dual difference along z.  zLoad issues the point's psi and c_psi
+// reads (and, for E, the k-1 duals) at the top of the plane, beside the Yee
+// reads, so a slab lane waits for memory once per plane, not twice; zStore
+// finishes the corrections once the Yee values are known.  psi and c_psi are
+// plain (cached) accesses: one 128-byte line of them serves the 4 j-rows of
+// a work-group (measured: the non-temporal hint costs 8 % of the H sweep).
+template <class T>
+struct ZVals {
+  T pA, pB, cA, cB, dualA, dualB;
+};
+
+template <class T, bool IsE>
+__device__ __forceinline__ ZVals<T> zLoad(const ZCpml<T>& Z, const ZLane<T>& L, const T* __restrict__ duA,
+                                          const T* __restrict__ duB, int64_t idx) {
+  ZVals<T> v{T(0), T(0), T(0), T(0), T(0), T(0)};
+  if (L.in) {
+    const ZSlab<T>& s0 = Z.z[0];
+    const ZSlab<T>& s1 = Z.z[1];
+    v.pA = (L.sel ? s1.psiA : s0.psiA)[L.t];
+    v.cA = (L.sel ? s1.cpsiA : s0.cpsiA)[L.t];
+    v.pB = (L.sel ? s1.psiB : s0.psiB)[L.t];
+    v.cB = (L.sel ? s1.cpsiB : s0.cpsiB)[L.t];
+    if (IsE) {
+      v.dualA = duA[idx - 1];
+      v.dualB = duB[idx - 1];
+    }
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ void zStore(const ZCpml<T>& Z, const ZLane<T>& L, const ZVals<T>& v, T& vA, T& vB, T dA,
+                                       T dB) {
+  if (L.in) {
+    const ZSlab<T>& s0 = Z.z[0];
+    const ZSlab<T>& s1 = Z.z[1];
+    const T psiA = L.cb * v.pA + L.ca * dA;
+    const T psiB = L.cb * v.pB + L.ca * dB;
+    (L.sel ? s1.psiA : s0.psiA)[L.t] = psiA;
+    (L.sel ? s1.psiB : s0.psiB)[L.t] = psiB;
+    vA += v.cA * psiA;
+    vB += v.cB * psiB;
+  }
+}
+
 // BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41): Hx, Hy, Hz
 // over [0,nx) x [0,ny) x [0,nz).  Coef is CoefPtrs (dense) or ClassCoef.
 // One work-group tile: k-block bx, j-block by, i-chunk bz.  Fused: the CPML
 // H corrections of the visited points in the same pass (the H dual
 // differences along x, y, z are dual[idx + s_c] - dual[idx]).
-template <class T, class Coef, bool Fused = false>
+template <class T, class Coef, bool Fused = false, bool ZF = false>
 __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
-                                       int ichunk, const CpmlList<T>& P = CpmlList<T>{}) {
+                                       int ichunk, const CpmlList<T>& P = CpmlList<T>{},
+                                       const ZCpml<T>& Z = ZCpml<T>{}) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -128,7 +211,11 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   T ey_i = ey[idx];
   T ez_i = ez[idx];
   const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
+  ZLane<T> zl;
+  if (ZF) zl = zLane(Z, i0, j, k);
   for (int i = i0; i < i1; ++i, idx += pl) {
+    ZVals<T> zv;
+    if (ZF) zv = zLoad<T, false>(Z, zl, nullptr, nullptr, idx);
     const T ex0 = ex[idx], ex_j = ex[idx + pz], ex_k = ex[idx + 1];
     const T ey_k = ey[idx + 1], ey_n = ey[idx + pl];
     const T ez_j = ez[idx + pz], ez_n = ez[idx + pl];
@@ -142,6 +229,10 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
     if (Fused && mask)  // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
       cpmlFused(P, mask, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
                 ex_k - ex0);
+    if (ZF) {  // z slab: Hx <- Ey, Hy <- Ex
+      zStore(Z, zl, zv, vx, vy, ey_k - ey_i, ex_k - ex0);
+      zl.t += zl.step;
+    }
     stOwn(hx + idx, vx);
     stOwn(hy + idx, vy);
     stOwn(hz + idx, vz);
@@ -150,12 +241,12 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   }
 }
 
-template <class T, class Coef, bool Fused>
+template <class T, class Coef, bool Fused, bool ZF>
 __global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
-                                                  CpmlList<T> P) {
+                                                  CpmlList<T> P, ZCpml<T> Z) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepH<T, Coef, Fused>(g, F, C, bx, by, bz, ichunk, P);
+  sweepH<T, Coef, Fused, ZF>(g, F, C, bx, by, bz, ichunk, P, Z);
 }
 
 // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
@@ -164,9 +255,10 @@ __global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> 
 // visited points (i < nx, j < ny, k < nz) in the same pass, including the
 // face points a Yee range leaves out; the dual differences along x, y, z are
 // dual[idx] - dual[idx - s_c].
-template <class T, class Coef, bool Fused = false>
+template <class T, class Coef, bool Fused = false, bool ZF = false>
 __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
-                                       int ichunk, const CpmlList<T>& P = CpmlList<T>{}) {
+                                       int ichunk, const CpmlList<T>& P = CpmlList<T>{},
+                                       const ZCpml<T>& Z = ZCpml<T>{}) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -184,7 +276,11 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   T hy_p = i0 > 0 ? hy[idx - pl] : T(0);
   const bool jk = j >= 1 && k >= 1;
   const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
+  ZLane<T> zl;
+  if (ZF) zl = zLane(Z, i0, j, k);
   for (int i = i0; i < i1; ++i, idx += pl) {
+    ZVals<T> zv;
+    if (ZF) zv = zLoad<T, true>(Z, zl, hy, hx, idx);  // Ex <- Hy, Ey <- Hx (k >= 1 in a z slab)
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
     T s, a, b;
     if (!Fused || !mask) {
@@ -203,6 +299,10 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
       if (i >= 1 && j >= 1) {
         C.getOwn(2, idx, s, a, b);
         vz = s * vz + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
+      }
+      if (ZF) {
+        zStore(Z, zl, zv, vx, vy, hy0 - zv.dualA, hx0 - zv.dualB);
+        zl.t += zl.step;
       }
       stOwn(ex + idx, vx);
       stOwn(ey + idx, vy);
@@ -239,12 +339,12 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   }
 }
 
-template <class T, class Coef, bool Fused>
+template <class T, class Coef, bool Fused, bool ZF>
 __global__ __launch_bounds__(kTileThreads) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
-                                                  CpmlList<T> P) {
+                                                  CpmlList<T> P, ZCpml<T> Z) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepE<T, Coef, Fused>(g, F, C, bx, by, bz, ichunk, P);
+  sweepE<T, Coef, Fused, ZF>(g, F, C, bx, by, bz, ichunk, P, Z);
 }
 
 // Single-cell time series (FieldTimeMonitor sampling): one launch per step
@@ -746,16 +846,17 @@ __global__ __launch_bounds__(256) void k_pack(Grid g, const T* __restrict__ src,
 // Host-side launchers, instantiated once per precision.
 template <class T>
 struct Launch {
-  // P.n > 0: the sweep applies the slabs of P in the same pass (then
-  // cpml_faces completes them on the far grid planes)
+  // P.n > 0: the sweep applies the slabs of P in the same pass; Z.n > 0:
+  // the sweep applies the z slabs of Z (then cpml_faces completes either on
+  // the far grid planes)
   static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
   static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
   static void cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
                            const AdeArgs<T>& A, int scheme, int ichunk);

@@ -28,43 +28,47 @@ inline SweepMap sweepMap(const Grid& g, int ichunk) {
 
 template <class T, class Coef>
 void launchUpdate(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk,
-                  const CpmlList<T>& P) {
+                  const CpmlList<T>& P, const ZCpml<T>& Z) {
   dim3 block(kLanesK, kRowsJ);
   const SweepMap m = sweepMap(g, ichunk);
   dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz));
-  const bool fused = P.n > 0;
+  const bool fused = P.n > 0, zf = !fused && Z.n > 0;
   if (is_h) {
     if (fused)
-      hipLaunchKernelGGL((k_update_h<T, Coef, true>), grid, block, 0, s, g, F, C, m, ichunk, P);
+      hipLaunchKernelGGL((k_update_h<T, Coef, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+    else if (zf)
+      hipLaunchKernelGGL((k_update_h<T, Coef, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
     else
-      hipLaunchKernelGGL((k_update_h<T, Coef, false>), grid, block, 0, s, g, F, C, m, ichunk, P);
+      hipLaunchKernelGGL((k_update_h<T, Coef, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
   } else {
     if (fused)
-      hipLaunchKernelGGL((k_update_e<T, Coef, true>), grid, block, 0, s, g, F, C, m, ichunk, P);
+      hipLaunchKernelGGL((k_update_e<T, Coef, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+    else if (zf)
+      hipLaunchKernelGGL((k_update_e<T, Coef, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
     else
-      hipLaunchKernelGGL((k_update_e<T, Coef, false>), grid, block, 0, s, g, F, C, m, ichunk, P);
+      hipLaunchKernelGGL((k_update_e<T, Coef, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
   }
 }
 
 template <class T>
 void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                         const CpmlList<T>& P) {
-  launchUpdate<T>(true, s, g, F, C, ichunk, P);
+                         const CpmlList<T>& P, const ZCpml<T>& Z) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z);
 }
 template <class T>
 void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                         const CpmlList<T>& P) {
-  launchUpdate<T>(false, s, g, F, C, ichunk, P);
+                         const CpmlList<T>& P, const ZCpml<T>& Z) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z);
 }
 template <class T>
 void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                         const CpmlList<T>& P) {
-  launchUpdate<T>(true, s, g, F, C, ichunk, P);
+                         const CpmlList<T>& P, const ZCpml<T>& Z) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z);
 }
 template <class T>
 void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                         const CpmlList<T>& P) {
-  launchUpdate<T>(false, s, g, F, C, ichunk, P);
+                         const CpmlList<T>& P, const ZCpml<T>& Z) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z);
 }
 template <class T>
 void Launch<T>::cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e) {
@@ -160,18 +164,25 @@ void Launch<T>::pack(hipStream_t s, const Grid& g, const T* src, double* dst, in
 // ------------------------------------------------------------- batched --
 template <class T>
 void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n,
-                           int64_t total, int grid, bool fused) {
+                           int64_t total, int grid, int mode) {
   if (total <= 0) return;
   dim3 block(kLanesK, kRowsJ);
   dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
-  if (is_h && fused)
-    hipLaunchKernelGGL((k_batch_sweep<T, true, true>), g, block, 0, s, cases, start, n, total);
-  else if (is_h)
-    hipLaunchKernelGGL((k_batch_sweep<T, true, false>), g, block, 0, s, cases, start, n, total);
-  else if (fused)
-    hipLaunchKernelGGL((k_batch_sweep<T, false, true>), g, block, 0, s, cases, start, n, total);
-  else
-    hipLaunchKernelGGL((k_batch_sweep<T, false, false>), g, block, 0, s, cases, start, n, total);
+  if (is_h) {
+    if (mode == 1)
+      hipLaunchKernelGGL((k_batch_sweep<T, true, true, false>), g, block, 0, s, cases, start, n, total);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_batch_sweep<T, true, false, true>), g, block, 0, s, cases, start, n, total);
+    else
+      hipLaunchKernelGGL((k_batch_sweep<T, true, false, false>), g, block, 0, s, cases, start, n, total);
+  } else {
+    if (mode == 1)
+      hipLaunchKernelGGL((k_batch_sweep<T, false, true, false>), g, block, 0, s, cases, start, n, total);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_batch_sweep<T, false, false, true>), g, block, 0, s, cases, start, n, total);
+    else
+      hipLaunchKernelGGL((k_batch_sweep<T, false, false, false>), g, block, 0, s, cases, start, n, total);
+  }
 }
 template <class T>
 void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,

@@ -181,6 +181,32 @@ struct CpmlList {
   int n = 0;
 };
 
+// A z-normal CPML slab applied inside a Yee sweep.  A sweep lane keeps one k
+// for its whole i-march, so it lies in at most one such slab and its layer
+// coefficients are loaded once; only psi and c_psi are read per cell, and the
+// field and dual values come from the sweep's registers.  The slab spans the
+// whole x/y range (box-local psi index t = (i * ext1 + j) * ext2 + k - klo)
+// and both of its components cover every sweep point (i < nx, j < ny) of its
+// layers [klo, khi); the far planes i = nx, j = ny are left to k_cpml_faces.
+template <class T>
+struct ZSlab {
+  int klo, khi;  // corrected k planes (layer = k - klo)
+  int ext1, ext2;
+  const T* cpsiA;
+  const T* cpsiB;
+  T* psiA;
+  T* psiB;
+  const T* ca;
+  const T* cb;
+};
+
+// The z slabs (at most two, disjoint in k) of one half-step.
+template <class T>
+struct ZCpml {
+  ZSlab<T> z[2];
+  int n = 0;
+};
+
 // Additive correction applied after a Yee half-step, in the reference's
 // corrector order (objects, then TFSF; src/simulation/simulation.cpp:437-487).
 struct Record {

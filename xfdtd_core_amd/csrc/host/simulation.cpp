@@ -414,7 +414,7 @@ struct Simulation::Impl {
   std::vector<KernelStat> stats;
   long long dev_bytes = 0;
   bool coef_classed = false;
-  bool cpml_fused = false;
+  int cpml_fused = 0;
 
   ~Impl() { releaseDevice(); }
 
@@ -1561,7 +1561,7 @@ struct Simulation::Impl {
     coef_classed = cl != 0;
     int32_t fu = 0;
     xfd_cpml_fused(ctx, &fu);
-    cpml_fused = fu != 0;
+    cpml_fused = fu;
   }
 
   void runLoop() {
@@ -1850,7 +1850,7 @@ void Simulation::setProfiling(bool on) { _impl->profiling = on; }
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
 bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
-bool Simulation::fusedCpml() const { return _impl->cpml_fused; }
+int Simulation::fusedCpml() const { return _impl->cpml_fused; }
 
 void Simulation::setDesignRegion(const std::array<Index, 3>& origin, const std::array<Index, 3>& size,
                                  std::vector<Real> eps_r) {

@@ -355,8 +355,11 @@ class Simulation:
     def device_bytes(self) -> int:
         return int(self._L.xfd_sim_device_bytes(self._h))
 
-    def cpml_fused(self) -> bool:
-        return bool(self._L.xfd_sim_cpml_fused(self._h))
+    def cpml_fused(self) -> int:
+        """Which CPML slabs the Yee sweeps apply in the same pass: 0 none,
+        1 the leading x/y slabs (XFD_CPML_FUSED=1), 2 the z slabs (FP32
+        when nz > 192; XFD_CPML_ZFUSED=1 forces it, =0 turns it off)."""
+        return int(self._L.xfd_sim_cpml_fused(self._h))
 
     def coef_classed(self) -> bool:
         """True when the device stores coefficient class ids, not 18 arrays."""
