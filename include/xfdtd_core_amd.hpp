@@ -477,6 +477,66 @@ class CurrentMonitor : public Monitor {
   std::vector<Real> _time, _data;
 };
 
+// A port of an S-parameter network (network/port.h): a current and a
+// voltage monitor and a reference impedance z.  calculateSParameters
+// (port.cpp:44-130) transforms both recordings (util/dft.h:9-26, the current
+// shifted by -1.5 dt) and forms the power waves
+//   a = 0.5 (V + z I) / sqrt(Re z),   b = 0.5 (V - conj(z) I) / sqrt(Re z).
+// The recordings come from the device loop; the transform runs on the host
+// after the run.
+class Port {
+ public:
+  Port(std::size_t index, bool is_source, std::complex<Real> impedance,
+       std::shared_ptr<CurrentMonitor> current_monitor, std::shared_ptr<VoltageMonitor> voltage_monitor);
+  std::size_t index() const { return _index; }
+  bool isSource() const { return _is_source; }
+  std::complex<Real> impedance() const { return _impedance; }
+  const std::shared_ptr<CurrentMonitor>& currentMonitor() const { return _current; }
+  const std::shared_ptr<VoltageMonitor>& voltageMonitor() const { return _voltage; }
+  const std::vector<std::complex<Real>>& a() const { return _a; }
+  const std::vector<std::complex<Real>>& b() const { return _b; }
+  void init(Real dt) { _dt = dt; }  // Port::init keeps the time step (port.cpp:37-42)
+  void calculateSParameters(const std::vector<Real>& frequencies);
+
+ private:
+  std::size_t _index;
+  bool _is_source;
+  std::complex<Real> _impedance;
+  std::shared_ptr<CurrentMonitor> _current;
+  std::shared_ptr<VoltageMonitor> _voltage;
+  Real _dt = 0;
+  std::vector<std::complex<Real>> _a, _b;
+};
+
+// An S-parameter network (network/network.h): output() computes every port's
+// power waves and, for each source port i and each port j,
+// S(j, i) = b_j / a_i, written as "s{10 j + i}.npy" (complex128) beside
+// "frequencies.npy" (network.cpp:47-84).  Port indexes must be unique.
+class Network {
+ public:
+  Network() = default;
+  explicit Network(std::string output_dir) : _output_dir(std::move(output_dir)) {}
+  Network(std::vector<std::shared_ptr<Port>> ports, std::vector<Real> frequencies, std::string output_dir)
+      : _ports(std::move(ports)), _frequencies(std::move(frequencies)), _output_dir(std::move(output_dir)) {}
+  void addPort(std::shared_ptr<Port> port) { _ports.emplace_back(std::move(port)); }
+  void setFrequencies(std::vector<Real> frequencies) { _frequencies = std::move(frequencies); }
+  void setOutputDir(std::string output_dir) { _output_dir = std::move(output_dir); }
+  const std::vector<std::shared_ptr<Port>>& ports() const { return _ports; }
+  const std::vector<Real>& frequencies() const { return _frequencies; }
+  const std::string& outputDir() const { return _output_dir; }
+  void init(Real dt);  // network.cpp:32-45
+  // Computes the S-parameters; writes them when the output directory is set.
+  void output();
+  // Key 10 j + i -> S(j, i) over the frequencies, after output().
+  const std::vector<std::pair<int, std::vector<std::complex<Real>>>>& sParameters() const { return _s; }
+
+ private:
+  std::vector<std::shared_ptr<Port>> _ports;
+  std::vector<Real> _frequencies;
+  std::string _output_dir;
+  std::vector<std::pair<int, std::vector<std::complex<Real>>>> _s;
+};
+
 // Frequency-domain near-to-far-field transform (nffft_frequency_domain.h):
 // a running DFT of the tangential fields on the box `distance` cells inside
 // the grid boundary, run on the GPU every step; after the run the far-field
@@ -556,6 +616,8 @@ class Simulation {
   void addBoundary(std::shared_ptr<Boundary> boundary);
   void addMonitor(std::shared_ptr<Monitor> monitor);
   void addNF2FF(std::shared_ptr<NFFFTFrequencyDomain> nffft);
+  void addNetwork(std::shared_ptr<Network> network);  // simulation.cpp:79-81
+  const std::vector<std::shared_ptr<Network>>& networks() const;
 
   // Field and update precision of the GPU loop (default FP64).  The
   // precompute is FP64 in both cases.

@@ -53,6 +53,19 @@ int32_t xfd_sim_cpml_fused(void* sim);
  * and the reference's output files (processFarField, outputRadiationPower:
  * nffft_frequency_domain.cpp:56-160; nt == 1 selects the phi cut). */
 int32_t xfd_sim_nf2ff_count(void* sim);
+
+/* S-parameter networks (spec line `network`; Network / Port, network.cpp,
+ * port.cpp) in spec order.  xfd_sim_network_output runs Network::output
+ * (network.cpp:47-84) on the recordings of the last run: every port's power
+ * waves, S(j, i) = b_j / a_i for each source port i keyed 10 j + i, and the
+ * .npy files when the network has an output directory.  keys: up to cap
+ * keys, returns their count (-1 on error).  xfd_sim_network_s copies one
+ * S-parameter as nf interleaved (re, im) doubles. */
+int32_t xfd_sim_network_count(void* sim);
+int32_t xfd_sim_network_nf(void* sim, int32_t n); /* frequency count, -1 if n is out of range */
+int xfd_sim_network_output(void* sim, int32_t n);
+int32_t xfd_sim_network_keys(void* sim, int32_t n, int32_t* keys, int32_t cap);
+int xfd_sim_network_s(void* sim, int32_t n, int32_t key, double* dst);
 int xfd_sim_nf2ff_surface(void* sim, int32_t n, int32_t d, int64_t* lo, int64_t* hi);
 int xfd_sim_nf2ff_current(void* sim, int32_t n, int32_t f, int32_t d, int32_t which, double* dst);
 int xfd_sim_nf2ff_potential(void* sim, int32_t n, int32_t f, int32_t a_pot, int32_t theta_comp, int32_t nt,

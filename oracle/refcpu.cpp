@@ -1852,6 +1852,7 @@ void Sim::init() {  // simulation.cpp:319-371
   for (auto& p : pmls) p->correctUpdateCoefficient(*this);
   buildDispersiveSpace();  // simulation.cpp:651
   for (auto& m : monitors) m->init(*this);
+  for (auto& n : networks) n->init(*this);  // simulation.cpp:353-355
   for (auto& n : nf2ffs) n->init(*this);  // simulation.cpp:350-358
 }
 
@@ -2063,15 +2064,72 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
   for (auto& m : monitors) m->output(*this);
 }
 
+// --------------------------------------------------------------- network --
+std::vector<std::complex<Real>> seriesDft(const std::vector<Real>& x, Real dt, const std::vector<Real>& freqs,
+                                          Real shift) {
+  const std::complex<Real> II(0.0, 1.0);  // constant.h:23
+  std::vector<std::complex<Real>> res(freqs.size());
+  for (std::size_t i = 0; i < freqs.size(); ++i) {
+    std::complex<Real> sum{0.0, 0.0};
+    for (std::size_t j = 0; j < x.size(); ++j) {
+      auto t = (j + 1) * dt + shift;
+      sum += x[j] * std::exp(-static_cast<Real>(2.0) * II * PI * freqs[i] * t);
+    }
+    res[i] = sum * dt;
+  }
+  return res;
+}
+
+void Port::calculateSParameters(const std::vector<Real>& freqs) {
+  if (!current || !voltage) throw Error("Port: monitor missing");
+  auto k = std::sqrt(std::real(z));
+  auto i = seriesDft(current->data.v, dt, freqs, -1.5 * dt);  // port.cpp:122-124
+  auto v = seriesDft(voltage->data.v, dt, freqs);
+  a.assign(freqs.size(), {});
+  b.assign(freqs.size(), {});
+  for (std::size_t f = 0; f < freqs.size(); ++f) {
+    a[f] = Real{0.5} * (v[f] + z * i[f]) / k;
+    b[f] = Real{0.5} * (v[f] - std::conj(z) * i[f]) / k;
+  }
+}
+
+void Network::init(Sim& sim) {
+  std::map<Index, int> seen;
+  for (auto& p : ports) {
+    p.dt = sim.dt;  // Port::init (port.cpp:37-42)
+    if (!seen.emplace(p.index, 0).second) throw Error("Port index must be unique");
+  }
+}
+
+void Network::output() {
+  for (auto& p : ports) p.calculateSParameters(freqs);
+  s.clear();
+  for (std::size_t i = 0; i < ports.size(); ++i) {
+    if (!ports[i].is_source) continue;
+    for (std::size_t j = 0; j < ports.size(); ++j) {
+      std::vector<std::complex<Real>> sv(freqs.size());
+      for (std::size_t f = 0; f < freqs.size(); ++f) sv[f] = ports[j].b[f] / ports[i].a[f];
+      s[static_cast<int>(ports[j].index * 10 + ports[i].index)] = sv;
+    }
+  }
+  if (out_dir.empty()) return;
+  std::filesystem::create_directories(out_dir);
+  for (const auto& e : s)
+    writeNpyComplex(out_dir + "/s" + std::to_string(e.first) + ".npy", {e.second.size()}, e.second.data());
+  writeNpy(out_dir + "/frequencies.npy", {freqs.size()}, freqs.data());
+}
+
 // ------------------------------------------------------------------- npy --
-void writeNpy(const std::string& path, const std::vector<Index>& shape, const Real* data) {
+namespace {
+void writeNpyRaw(const std::string& path, const std::vector<Index>& shape, const char* descr, const void* data,
+                 std::size_t item_bytes) {
   std::string sh = "(";
   for (Index i = 0; i < shape.size(); ++i) {
     sh += std::to_string(shape[i]);
     if (shape.size() == 1 || i + 1 < shape.size()) sh += ", ";
   }
   sh += ")";
-  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': " + sh + ", }";
+  std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
   Index total = 10 + hdr.size() + 1;
   Index pad = (64 - total % 64) % 64;
   hdr += std::string(pad, ' ') + "\n";
@@ -2084,8 +2142,18 @@ void writeNpy(const std::string& path, const std::vector<Index>& shape, const Re
   std::fwrite(hdr.data(), 1, hdr.size(), fp);
   Index n = 1;
   for (auto v : shape) n *= v;
-  std::fwrite(data, sizeof(Real), n, fp);
+  std::fwrite(data, item_bytes, n, fp);
   std::fclose(fp);
+}
+}  // namespace
+
+void writeNpy(const std::string& path, const std::vector<Index>& shape, const Real* data) {
+  writeNpyRaw(path, shape, "<f8", data, sizeof(Real));
+}
+
+// complex128, interleaved (re, im) as xt::dump_npy writes std::complex<double>
+void writeNpyComplex(const std::string& path, const std::vector<Index>& shape, const std::complex<Real>* data) {
+  writeNpyRaw(path, shape, "<c16", data, sizeof(std::complex<Real>));
 }
 
 }  // namespace refcpu

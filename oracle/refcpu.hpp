@@ -30,6 +30,7 @@
 #include <complex>
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -465,6 +466,38 @@ struct CurrentMonitor : Monitor {
 };
 
 // ------------------------------------------------------------ simulation --
+// ------------------------------------------------------------ network ----
+// dft (include/xfdtd/util/dft.h:9-26): X(f) = dt * sum_j x_j
+// exp(-2 i pi f ((j + 1) dt + shift)), terms in the reference's order.
+std::vector<std::complex<Real>> seriesDft(const std::vector<Real>& x, Real dt, const std::vector<Real>& freqs,
+                                          Real shift = 0);
+
+// Port (network/port.h, port.cpp:44-130) for one process: the DFT of the
+// current record (shifted by -1.5 dt) and of the voltage record, and the
+// power waves a = 0.5 (V + z I) / sqrt(Re z), b = 0.5 (V - conj(z) I) /
+// sqrt(Re z).
+struct Port {
+  Index index = 0;
+  bool is_source = false;
+  std::complex<Real> z{50, 0};
+  const CurrentMonitor* current = nullptr;
+  const VoltageMonitor* voltage = nullptr;
+  Real dt = 0;
+  std::vector<std::complex<Real>> a, b;
+  void calculateSParameters(const std::vector<Real>& freqs);
+};
+
+// Network (network.cpp:32-84): S(j, i) = b_j / a_i for every source port i,
+// keyed 10 j + i; "s{key}.npy" (complex128) and "frequencies.npy".
+struct Network {
+  std::string out_dir;
+  std::vector<Real> freqs;
+  std::vector<Port> ports;
+  std::map<int, std::vector<std::complex<Real>>> s;
+  void init(Sim& sim);  // network.cpp:32-45
+  void output();        // network.cpp:47-84
+};
+
 // ---------------------------------------------------------------- NF2FF ---
 // NFFFTFrequencyDomain (nffft.cpp:39-236, nffft_frequency_domain.cpp) with
 // one FDPlaneData per frequency (nffft_fd_data.cpp, nffft_fd_data_update.cpp,
@@ -524,6 +557,7 @@ struct Sim {
   std::vector<std::unique_ptr<TFSF3D>> tfsfs;
   std::vector<std::unique_ptr<Monitor>> monitors;
   std::vector<std::unique_ptr<NF2FF>> nf2ffs;
+  std::vector<std::unique_ptr<Network>> networks;
   bool write_files = true;
 
   GridSpace g;
@@ -555,5 +589,7 @@ struct Sim {
 
 void writeNpy(const std::string& path, const std::vector<Index>& shape,
               const Real* data);
+void writeNpyComplex(const std::string& path, const std::vector<Index>& shape,
+                     const std::complex<Real>* data);
 
 }  // namespace refcpu

@@ -290,6 +290,31 @@ void parseSpec(Sim& s, const std::string& spec) {
       mon->out_dir = word(is);
       if (mon->out_dir == "-") mon->out_dir.clear();
       s.monitors.push_back(std::move(mon));
+    } else if (d == "network") {  // NAME OUT_DIR NF F.. NP (INDEX SOURCE ZRE ZIM CURRENT VOLTAGE)..
+      word(is);
+      auto nw = std::make_unique<Network>();
+      nw->out_dir = word(is);
+      if (nw->out_dir == "-") nw->out_dir.clear();
+      const Index nf = static_cast<Index>(num(is));
+      for (Index q = 0; q < nf; ++q) nw->freqs.push_back(num(is));
+      const Index np = static_cast<Index>(num(is));
+      auto byName = [&](const std::string& n) -> Monitor* {
+        for (auto& m : s.monitors)
+          if (m->name == n) return m.get();
+        throw Error("network: no monitor " + n);
+      };
+      for (Index q = 0; q < np; ++q) {
+        Port p;
+        p.index = static_cast<Index>(num(is));
+        p.is_source = num(is) != 0;
+        const Real zr = num(is), zi = num(is);
+        p.z = {zr, zi};
+        p.current = dynamic_cast<CurrentMonitor*>(byName(word(is)));
+        p.voltage = dynamic_cast<VoltageMonitor*>(byName(word(is)));
+        if (!p.current || !p.voltage) throw Error("network: port needs a current and a voltage monitor");
+        nw->ports.push_back(p);
+      }
+      s.networks.push_back(std::move(nw));
     } else if (d == "field_time") {
       auto m = std::make_unique<FieldTimeMonitor>();
       m->name = word(is);
@@ -378,6 +403,47 @@ int ref_objective(void* p, double* value, double* dft) {
 }
 
 int ref_nf2ff_count(void* p) { return static_cast<int>(static_cast<Handle*>(p)->sim.nf2ffs.size()); }
+
+// Network n: Network::output on the last run's recordings, then S(j, i)
+// by key 10 j + i as nf interleaved (re, im) doubles.
+int ref_network_output(void* p, int n) {
+  return guard([&] { static_cast<Handle*>(p)->sim.networks.at(n)->output(); });
+}
+int ref_network_s(void* p, int n, int key, double* dst) {
+  return guard([&] {
+    const auto& s = static_cast<Handle*>(p)->sim.networks.at(n)->s;
+    auto it = s.find(key);
+    if (it == s.end()) throw Error("no S-parameter " + std::to_string(key));
+    std::memcpy(dst, it->second.data(), it->second.size() * sizeof(std::complex<Real>));
+  });
+}
+
+// The restated dft and power waves on given series, for the closed-form
+// checks: out = nf complex values (interleaved); a / b likewise.
+int ref_series_dft(const double* x, int64_t n, double dt, const double* freqs, int64_t nf, double shift,
+                   double* out) {
+  return guard([&] {
+    auto r = seriesDft(std::vector<Real>(x, x + n), dt, std::vector<Real>(freqs, freqs + nf), shift);
+    std::memcpy(out, r.data(), r.size() * sizeof(std::complex<Real>));
+  });
+}
+int ref_port_waves(const double* current, const double* voltage, int64_t n, double dt, const double* freqs,
+                   int64_t nf, double z_re, double z_im, double* a, double* b) {
+  return guard([&] {
+    CurrentMonitor cm;
+    VoltageMonitor vm;
+    cm.data.v.assign(current, current + n);
+    vm.data.v.assign(voltage, voltage + n);
+    Port port;
+    port.z = {z_re, z_im};
+    port.current = &cm;
+    port.voltage = &vm;
+    port.dt = dt;
+    port.calculateSParameters(std::vector<Real>(freqs, freqs + nf));
+    std::memcpy(a, port.a.data(), port.a.size() * sizeof(std::complex<Real>));
+    std::memcpy(b, port.b.data(), port.b.size() * sizeof(std::complex<Real>));
+  });
+}
 
 // Surface task of NF2FF `n`, surface d (XN XP YN YP ZN ZP): lo[3], hi[3]
 // (hi == lo when the surface is not used).

@@ -56,6 +56,10 @@ def lib():
         "ref_monitor_count": (ctypes.c_int, [vp]),
         "ref_monitor_records": (ctypes.c_int, [vp, ctypes.c_int]),
         "ref_monitor_record": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
+        "ref_network_output": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ref_network_s": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp]),
+        "ref_series_dft": (ctypes.c_int, [vp, i64, dbl, vp, i64, dbl, vp]),
+        "ref_port_waves": (ctypes.c_int, [vp, vp, i64, dbl, vp, i64, dbl, dbl, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -63,6 +67,31 @@ def lib():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def series_dft(x, dt: float, freqs, shift: float = 0.0) -> np.ndarray:
+    """The oracle's dft (util/dft.h:9-26) of a real series."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    out = np.zeros(f.size, dtype=np.complex128)
+    L = lib()
+    if L.ref_series_dft(x.ctypes.data, x.size, dt, f.ctypes.data, f.size, shift, out.ctypes.data) != 0:
+        raise RuntimeError(L.ref_last_error().decode())
+    return out
+
+
+def port_waves(current, voltage, dt: float, freqs, z: complex):
+    """The oracle's Port::calculateSParameters on given records: (a, b)."""
+    c = np.ascontiguousarray(current, dtype=np.float64)
+    v = np.ascontiguousarray(voltage, dtype=np.float64)
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    a = np.zeros(f.size, dtype=np.complex128)
+    b = np.zeros(f.size, dtype=np.complex128)
+    L = lib()
+    if L.ref_port_waves(c.ctypes.data, v.ctypes.data, c.size, dt, f.ctypes.data, f.size, complex(z).real,
+                        complex(z).imag, a.ctypes.data, b.ctypes.data) != 0:
+        raise RuntimeError(L.ref_last_error().decode())
+    return a, b
 
 
 class RefSim:
@@ -77,6 +106,16 @@ class RefSim:
     def _check(self, rc):
         if rc != 0:
             raise RuntimeError(self.L.ref_last_error().decode())
+
+    def network_s(self, n: int, keys, nf: int) -> dict:
+        """Network::output on the last run, then {key: S} for `keys`."""
+        self._check(self.L.ref_network_output(self.h, n))
+        out = {}
+        for k in keys:
+            a = np.zeros(nf, dtype=np.complex128)
+            self._check(self.L.ref_network_s(self.h, n, int(k), a.ctypes.data))
+            out[int(k)] = a
+        return out
 
     def set_threads(self, tx: int, ty: int, tz: int):
         """ThreadConfig of the next loop (Simulation's thread layout)."""

@@ -421,10 +421,15 @@ def test_far_field_on_device_matches_host(monkeypatch):
                 assert np.abs(gi - hi).max() <= 1e-12 * scale
 
 
-def _line_monitor_spec():
+NET_FREQS = [1e9, 2.5e9, 4e9, 6e9]
+
+
+def _line_monitor_spec(network_dir=None):
     """A 50-ohm voltage source and a 50-ohm resistor (6-cell z gaps) in a
     CPML box, with a voltage and a current monitor on each and a voltage
-    monitor across a 2 x 3 x 4 block along y."""
+    monitor across a 2 x 3 x 4 block along y.  With `network_dir` (a path,
+    or "" for no files), also a two-port network: port 1 (source) on the
+    voltage source, port 2 on the resistor."""
     dl = 1e-3
     s = S.Simulation(dl, dl, dl, 0.9)
     half = 12 * dl
@@ -437,13 +442,15 @@ def _line_monitor_spec():
     s.addObject(S.Resistor("r1", res, "Z", 50.0))
     for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
         s.addBoundary(S.PML(6, d))
-    s.addMonitor(S.VoltageMonitor("v_src", src, "ZP"))
-    s.addMonitor(S.CurrentMonitor("i_src", src, "ZP"))
-    s.addMonitor(S.VoltageMonitor("v_res", res, "ZN"))
-    s.addMonitor(S.CurrentMonitor("i_res", res, "ZN"))
-    s.addMonitor(S.VoltageMonitor("v_blk", S.Cube(S.Vector(-dl, -2 * dl, dl), S.Vector(2 * dl, 3 * dl, 4 * dl)), "YP"))
-    s.addMonitor(S.CurrentMonitor("i_blk", S.Cube(S.Vector(-dl, -2 * dl, dl), S.Vector(2 * dl, 3 * dl, 4 * dl)),
-                                  "XN"))
+    mons = [S.VoltageMonitor("v_src", src, "ZP"), S.CurrentMonitor("i_src", src, "ZP"),
+            S.VoltageMonitor("v_res", res, "ZN"), S.CurrentMonitor("i_res", res, "ZN"),
+            S.VoltageMonitor("v_blk", S.Cube(S.Vector(-dl, -2 * dl, dl), S.Vector(2 * dl, 3 * dl, 4 * dl)), "YP"),
+            S.CurrentMonitor("i_blk", S.Cube(S.Vector(-dl, -2 * dl, dl), S.Vector(2 * dl, 3 * dl, 4 * dl)), "XN")]
+    for m in mons:
+        s.addMonitor(m)
+    if network_dir is not None:
+        s.addNetwork(S.Network([S.Port(1, True, 50.0, mons[1], mons[0]), S.Port(2, False, 50.0, mons[3], mons[2])],
+                               NET_FREQS, network_dir))
     return s.spec()
 
 
@@ -469,3 +476,33 @@ def test_voltage_current_monitors(precision):
             err = np.abs(a[1] - b[1]).max() / np.abs(b[1]).max()
             say(f"  monitor {m}: fp32 error {err:.2e} of the peak")
             assert err < 1e-4, (m, err)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_network_s_parameters(tmp_path, precision):
+    """Network::output on the device-recorded port series: S11 and S21 of a
+    two-port (source, resistor) equal the oracle's bitwise in FP64 (same
+    recordings, same host transform); FP32 within 1e-3.  The product writes
+    s11.npy / s21.npy (complex128) and frequencies.npy."""
+    out = tmp_path / precision
+    sp = _line_monitor_spec(str(out))
+    steps = 400
+    say(f"[gpu] network S-parameters {precision}")
+    ref = RefSim(_line_monitor_spec(""))
+    ref.run(steps)
+    want = ref.network_s(0, (11, 21), len(NET_FREQS))
+    dev = native.Simulation(sp, precision)
+    dev.run(steps)
+    assert dev.network_count() == 1
+    got = dev.network_output(0)
+    assert sorted(got) == [11, 21]
+    for k in (11, 21):
+        if precision == "f64":
+            np.testing.assert_array_equal(got[k], want[k], err_msg=f"s{k}")
+        else:
+            err = np.abs(got[k] - want[k]).max()
+            say(f"  s{k}: fp32 error {err:.2e} (|S| up to {np.abs(want[k]).max():.3f})")
+            assert err < 1e-3, (k, err)
+        np.testing.assert_array_equal(np.load(out / f"s{k}.npy"), got[k])
+    np.testing.assert_array_equal(np.load(out / "frequencies.npy"), np.array(NET_FREQS))
+    assert np.abs(want[11]).max() > 0.1 and np.abs(want[21]).max() > 1e-4  # |S21| ~ 1e-3 here

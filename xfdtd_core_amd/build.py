@@ -66,7 +66,12 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, UNITS))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lpthread"]
+    # -rtlib=libgcc: the host code's compiler helpers (complex multiply and
+    # divide, __muldc3 / __divdc3) come from libgcc as in a g++ build of the
+    # reference, not from clang's runtime, whose complex division rounds
+    # differently in the last bits.
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lpthread",
+           "-rtlib=libgcc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

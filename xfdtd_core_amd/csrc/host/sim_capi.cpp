@@ -19,6 +19,7 @@ struct SimHandle {
   std::unique_ptr<Simulation> sim;
   std::vector<std::shared_ptr<Monitor>> monitors;
   std::vector<std::shared_ptr<NFFFTFrequencyDomain>> nf2ffs;
+  std::vector<std::shared_ptr<Network>> networks;
   Precision precision = Precision::FP64;
 };
 
@@ -261,6 +262,32 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
         m = std::make_shared<CurrentMonitor>(name, std::move(cb), dr, dir);
       s.addMonitor(m);
       h->monitors.push_back(m);
+    } else if (d == "network") {  // NAME OUT_DIR NF F.. NP (INDEX SOURCE ZRE ZIM CURRENT VOLTAGE)..
+      t.word();
+      std::string dir = t.word();
+      if (dir == "-") dir.clear();
+      const int nf = static_cast<int>(t.num());
+      std::vector<Real> freqs(nf);
+      for (auto& f : freqs) f = t.num();
+      const int np = static_cast<int>(t.num());
+      auto find = [&](const std::string& name) {
+        for (auto& m : h->monitors)
+          if (m->name() == name) return m;
+        throw XFDTDException("network: no monitor " + name);
+      };
+      auto nw = std::make_shared<Network>(dir);
+      nw->setFrequencies(freqs);
+      for (int q = 0; q < np; ++q) {
+        const auto index = static_cast<std::size_t>(t.num());
+        const bool source = t.num() != 0;
+        const Real zr = t.num(), zi = t.num();
+        auto cm = std::dynamic_pointer_cast<CurrentMonitor>(find(t.word()));
+        auto vm = std::dynamic_pointer_cast<VoltageMonitor>(find(t.word()));
+        if (!cm || !vm) throw XFDTDException("network: port needs a current and a voltage monitor");
+        nw->addPort(std::make_shared<Port>(index, source, std::complex<Real>(zr, zi), cm, vm));
+      }
+      s.addNetwork(nw);
+      h->networks.push_back(nw);
     } else if (d == "field_time") {
       std::string name = t.word();
       auto f = parseField(t.word());
@@ -362,6 +389,43 @@ int32_t xfd_sim_coef_classed(void* p) { return H(p)->sim->classedCoefficients() 
 int32_t xfd_sim_cpml_fused(void* p) { return H(p)->sim->fusedCpml(); }
 
 int32_t xfd_sim_nf2ff_count(void* p) { return static_cast<int32_t>(H(p)->nf2ffs.size()); }
+
+int32_t xfd_sim_network_count(void* p) { return static_cast<int32_t>(H(p)->networks.size()); }
+
+int32_t xfd_sim_network_nf(void* p, int32_t n) {
+  if (n < 0 || n >= static_cast<int32_t>(H(p)->networks.size())) return -1;
+  return static_cast<int32_t>(H(p)->networks[n]->frequencies().size());
+}
+
+int xfd_sim_network_output(void* p, int32_t n) {
+  return guard([&] {
+    if (n < 0 || n >= static_cast<int32_t>(H(p)->networks.size())) throw XFDTDException("bad network");
+    H(p)->networks[n]->output();
+  });
+}
+
+int32_t xfd_sim_network_keys(void* p, int32_t n, int32_t* keys, int32_t cap) {
+  int32_t count = -1;
+  guard([&] {
+    if (n < 0 || n >= static_cast<int32_t>(H(p)->networks.size())) throw XFDTDException("bad network");
+    const auto& s = H(p)->networks[n]->sParameters();
+    count = static_cast<int32_t>(s.size());
+    for (int32_t q = 0; q < count && q < cap; ++q) keys[q] = s[q].first;
+  });
+  return count;
+}
+
+int xfd_sim_network_s(void* p, int32_t n, int32_t key, double* dst) {
+  return guard([&] {
+    if (n < 0 || n >= static_cast<int32_t>(H(p)->networks.size())) throw XFDTDException("bad network");
+    for (const auto& e : H(p)->networks[n]->sParameters())
+      if (e.first == key) {
+        std::memcpy(dst, e.second.data(), e.second.size() * sizeof(std::complex<Real>));
+        return;
+      }
+    throw XFDTDException("network: no S-parameter " + std::to_string(key));
+  });
+}
 
 int xfd_sim_nf2ff_surface(void* p, int32_t n, int32_t d, int64_t* lo, int64_t* hi) {
   return guard([&] {

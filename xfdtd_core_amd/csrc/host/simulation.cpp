@@ -261,6 +261,86 @@ VoltageMonitor::VoltageMonitor(std::string n, std::unique_ptr<Cube> c, Axis::Dir
 CurrentMonitor::CurrentMonitor(std::string n, std::unique_ptr<Cube> c, Axis::Direction dir, std::string d)
     : Monitor(std::move(n), std::move(d)), _cube(std::move(c)), _direction(dir) {}
 
+// ------------------------------------------------------------- network ---
+namespace {
+// dft (util/dft.h:9-26): X(f) = dt sum_j x_j exp(-2 i pi f ((j + 1) dt + shift)),
+// each term in the reference's evaluation order.
+std::vector<std::complex<Real>> seriesDft(const std::vector<Real>& x, Real dt, const std::vector<Real>& freqs,
+                                          Real shift) {
+  const std::complex<Real> II(0.0, 1.0);
+  std::vector<std::complex<Real>> res(freqs.size());
+  for (std::size_t i = 0; i < freqs.size(); ++i) {
+    std::complex<Real> sum{0.0, 0.0};
+    for (std::size_t j = 0; j < x.size(); ++j) {
+      const Real t = static_cast<Real>(j + 1) * dt + shift;
+      sum += x[j] * std::exp(-static_cast<Real>(2.0) * II * kPi * freqs[i] * t);
+    }
+    res[i] = sum * dt;
+  }
+  return res;
+}
+}  // namespace
+
+Port::Port(std::size_t index, bool is_source, std::complex<Real> impedance,
+           std::shared_ptr<CurrentMonitor> current_monitor, std::shared_ptr<VoltageMonitor> voltage_monitor)
+    : _index(index),
+      _is_source(is_source),
+      _impedance(impedance),
+      _current(std::move(current_monitor)),
+      _voltage(std::move(voltage_monitor)) {}
+
+// Port::calculateSParameters (port.cpp:44-130), one process.
+void Port::calculateSParameters(const std::vector<Real>& frequencies) {
+  if (!_current || !_voltage) throw XFDTDException("Port: monitor missing");
+  const auto z = _impedance;
+  const Real k = std::sqrt(std::real(z));
+  const auto i = seriesDft(_current->data(), _dt, frequencies, -1.5 * _dt);
+  const auto v = seriesDft(_voltage->data(), _dt, frequencies, 0);
+  _a.resize(frequencies.size());
+  _b.resize(frequencies.size());
+  for (std::size_t f = 0; f < frequencies.size(); ++f) {
+    _a[f] = Real{0.5} * (v[f] + z * i[f]) / k;
+    _b[f] = Real{0.5} * (v[f] - std::conj(z) * i[f]) / k;
+  }
+}
+
+void Network::init(Real dt) {
+  std::vector<std::size_t> seen;
+  for (auto& p : _ports) {
+    if (!p) throw XFDTDException("Network: null port");
+    p->init(dt);
+    if (std::find(seen.begin(), seen.end(), p->index()) != seen.end())
+      throw XFDTDException("Port index must be unique");
+    seen.push_back(p->index());
+  }
+}
+
+// Network::output (network.cpp:47-84): S(j, i) = b_j / a_i for every source
+// port i, keyed 10 j + i.
+void Network::output() {
+  for (auto& p : _ports) p->calculateSParameters(_frequencies);
+  _s.clear();
+  for (std::size_t i = 0; i < _ports.size(); ++i) {
+    if (!_ports[i]->isSource()) continue;
+    for (std::size_t j = 0; j < _ports.size(); ++j) {
+      const int key = static_cast<int>(_ports[j]->index() * 10 + _ports[i]->index());
+      std::vector<std::complex<Real>> sv(_frequencies.size());
+      for (std::size_t f = 0; f < sv.size(); ++f) sv[f] = _ports[j]->b()[f] / _ports[i]->a()[f];
+      auto it = std::find_if(_s.begin(), _s.end(), [&](const auto& e) { return e.first == key; });
+      if (it != _s.end())
+        it->second = std::move(sv);
+      else
+        _s.emplace_back(key, std::move(sv));
+    }
+  }
+  if (_output_dir.empty()) return;
+  std::filesystem::create_directories(_output_dir);
+  for (const auto& e : _s)
+    npy::writeComplex(_output_dir + "/s" + std::to_string(e.first) + ".npy", {e.second.size()},
+                      reinterpret_cast<const double*>(e.second.data()));
+  npy::write(_output_dir + "/frequencies.npy", {_frequencies.size()}, _frequencies.data());
+}
+
 // ---------------------------------------------------------- simulation ---
 namespace {
 
@@ -332,6 +412,7 @@ struct Simulation::Impl {
   std::vector<std::shared_ptr<Boundary>> boundaries;
   std::vector<std::shared_ptr<Monitor>> monitors;
   std::vector<std::shared_ptr<NFFFTFrequencyDomain>> nf2ffs;
+  std::vector<std::shared_ptr<Network>> networks;
   std::vector<xfd_nf2ff_desc> nf2ff_d;
 
   // grid (grid_space_generator.cpp, grid_space.cpp:526-544)
@@ -1435,6 +1516,7 @@ struct Simulation::Impl {
     buildNf2ff();
     coef_view.clear();
     stage("monitors");
+    for (auto& nw : networks) nw->init(dt);  // Simulation::init (simulation.cpp:353-355)
     precomputed = true;
     precomputed_steps = n;
   }
@@ -1816,6 +1898,11 @@ void Simulation::addNF2FF(std::shared_ptr<NFFFTFrequencyDomain> n) {
   _impl->nf2ffs.push_back(std::move(n));
   _impl->precomputed = false;
 }
+void Simulation::addNetwork(std::shared_ptr<Network> n) {
+  _impl->networks.push_back(std::move(n));
+  _impl->precomputed = false;
+}
+const std::vector<std::shared_ptr<Network>>& Simulation::networks() const { return _impl->networks; }
 void Simulation::setPrecision(Precision p) { _impl->precision = p; }
 void Simulation::init(Index n) { _impl->init(n); }
 void Simulation::precompute(Index n) { _impl->precompute(n); }

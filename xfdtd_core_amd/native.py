@@ -58,6 +58,11 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_cpml_fused": (ctypes.c_int32, [vp]),
         "xfd_cpml_fused": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32)]),
         "xfd_sim_nf2ff_count": (ctypes.c_int32, [vp]),
+        "xfd_sim_network_count": (ctypes.c_int32, [vp]),
+        "xfd_sim_network_nf": (ctypes.c_int32, [vp, i32]),
+        "xfd_sim_network_output": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_network_keys": (ctypes.c_int32, [vp, i32, vp, i32]),
+        "xfd_sim_network_s": (ctypes.c_int, [vp, i32, i32, vp]),
         "xfd_sim_nf2ff_surface": (ctypes.c_int, [vp, i32, i32, p64, p64]),
         "xfd_sim_nf2ff_current": (ctypes.c_int, [vp, i32, i32, i32, i32, vp]),
         "xfd_sim_nf2ff_potential": (ctypes.c_int, [vp, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp]),
@@ -364,6 +369,31 @@ class Simulation:
     def coef_classed(self) -> bool:
         """True when the device stores coefficient class ids, not 18 arrays."""
         return bool(self._L.xfd_sim_coef_classed(self._h))
+
+    # -------------------------------------------------------------- networks --
+    def network_count(self) -> int:
+        return int(self._L.xfd_sim_network_count(self._h))
+
+    def network_output(self, n: int = 0) -> dict:
+        """Network::output on the last run's recordings (writes the .npy
+        files when the network has an output directory); returns
+        {10 j + i: S(j, i) over the frequencies}."""
+        self._check(self._L.xfd_sim_network_output(self._h, n))
+        cnt = int(self._L.xfd_sim_network_keys(self._h, n, None, 0))
+        if cnt < 0:
+            self._check(-1)
+        keys = (ctypes.c_int32 * max(cnt, 1))()
+        self._L.xfd_sim_network_keys(self._h, n, keys, cnt)
+        out = {}
+        for q in range(cnt):
+            nf = self._network_nf(n)
+            a = np.zeros(nf, dtype=np.complex128)
+            self._check(self._L.xfd_sim_network_s(self._h, n, int(keys[q]), a.ctypes.data))
+            out[int(keys[q])] = a
+        return out
+
+    def _network_nf(self, n: int) -> int:
+        return int(self._L.xfd_sim_network_nf(self._h, n))
 
     # ---------------------------------------------------------------- NF2FF --
     def nf2ff_count(self) -> int:

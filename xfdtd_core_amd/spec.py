@@ -29,6 +29,7 @@ renders the scenario as line-oriented text, one directive per line:
     field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
     voltage_monitor NAME OX OY OZ SX SY SZ DIR OUT_DIR
     current_monitor NAME OX OY OZ SX SY SZ DIR OUT_DIR
+    network NAME OUT_DIR NF F1 .. FNF NP (INDEX SOURCE Z_RE Z_IM CURRENT VOLTAGE)...
     design_region OI OJ OK NI NJ NK EPS_R...      (cells; adjoint form)
     dft_objective FIELD OI OJ OK NI NJ NK FREQ    (grid points; adjoint form)
     WAVEFORM = gaussian TAU T0 AMP | cosine_modulated_gaussian TAU T0 F AMP
@@ -455,6 +456,41 @@ class NFFFTFrequencyDomain:
                         ([] if self.reference_repeat else ["plain_dft"]))
 
 
+@dataclass
+class Port:
+    """``Port(index, is_source, impedance, current_monitor, voltage_monitor)``
+    (network/port.h): a port of an S-parameter network."""
+
+    index: int
+    is_source: bool
+    impedance: complex
+    current_monitor: CurrentMonitor
+    voltage_monitor: VoltageMonitor
+
+    def tokens(self):
+        z = complex(self.impedance)
+        return [str(int(self.index)), "1" if self.is_source else "0", _n(z.real), _n(z.imag),
+                self.current_monitor.name, self.voltage_monitor.name]
+
+
+@dataclass
+class Network:
+    """``Network(ports, frequencies, output_dir)`` (network/network.h):
+    ``output()`` gives S(j, i) = b_j / a_i for every source port i, keyed
+    10 j + i (network.cpp:47-84).  The monitors must be added to the
+    simulation before the network."""
+
+    ports: List[Port]
+    frequencies: List[float]
+    output_dir: str = ""
+    name: str = "network"
+
+    def line(self):
+        f = [float(v) for v in self.frequencies]
+        return " ".join(["network", self.name, self.output_dir or "-", str(len(f))] + [_n(v) for v in f] +
+                        [str(len(self.ports))] + [t for p in self.ports for t in p.tokens()])
+
+
 class Simulation:
     """``xfdtd::Simulation(dx, dy, dz, cfl, ThreadConfig)`` (simulation.h:39-40)."""
 
@@ -481,6 +517,9 @@ class Simulation:
     def addNF2FF(self, n):
         self._lines.append(n.line())
         self.nf2ffs.append(n)
+
+    def addNetwork(self, n):
+        self._lines.append(n.line())
 
     def setDesignRegion(self, origin, size, eps_r):
         """Per-cell relative permittivity of the design cells [origin,
