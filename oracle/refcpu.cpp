@@ -1425,7 +1425,11 @@ Real surfaceCenter(Sim& s, int c, int f, Index i, Index j, Index k) {
 }
 }  // namespace
 
-void NF2FF::init(Sim& s) {  // nffft.cpp:73-223 with node == global
+namespace {
+// NFFFT::initGlobal / makeNodeAxisTask (nffft.cpp:88-223) with node ==
+// global: the global surfaces are the box faces; a node surface is empty
+// when it lies on plane 0 or 1 (negative side) or on the last plane.
+void nffftSurfaces(Sim& s, const Index dist[3], Task node[6], Task global[6]) {
   const Index n[3] = {s.g.nx(), s.g.ny(), s.g.nz()};
   Index lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
@@ -1440,18 +1444,19 @@ void NF2FF::init(Sim& s) {  // nffft.cpp:73-223 with node == global
     Range r[3];
     for (int a = 0; a < 3; ++a) r[a] = {lo[a], hi[a]};
     const Index cs = neg ? lo[c] : hi[c];
-    // makeNodeAxisTask: a negative surface on plane 0 or 1, or a positive
-    // one on the last plane, is not used
-    if ((neg && cs <= 1) || (!neg && cs >= n[c])) {
-      surf[d] = Task{};
-      continue;
-    }
     r[c] = {cs, cs + 1};
-    surf[d] = {r[0], r[1], r[2]};
+    global[d] = {r[0], r[1], r[2]};
+    node[d] = ((neg && cs <= 1) || (!neg && cs >= n[c])) ? Task{} : global[d];
   }
   bool any = false;
-  for (auto& t : surf) any = any || t.valid();
+  for (int d = 0; d < 6; ++d) any = any || node[d].valid();
   if (!any) throw Error("Invalid task");
+}
+}  // namespace
+
+void NF2FF::init(Sim& s) {  // nffft.cpp:73-223 with node == global
+  Task global[6];
+  nffftSurfaces(s, dist, surf, global);
   planes.clear();
   for (Real f : freqs) {
     Plane p;
@@ -1854,6 +1859,7 @@ void Sim::init() {  // simulation.cpp:319-371
   for (auto& m : monitors) m->init(*this);
   for (auto& n : networks) n->init(*this);  // simulation.cpp:353-355
   for (auto& n : nf2ffs) n->init(*this);  // simulation.cpp:350-358
+  for (auto& n : nf2ff_tds) n->init(*this);
 }
 
 void Sim::initSteps(Index n) {  // simulation.cpp:373-389
@@ -1864,6 +1870,7 @@ void Sim::initSteps(Index n) {  // simulation.cpp:373-389
   for (auto& o : objects) o->initTimeDependentVariable(*this);
   for (auto& t : tfsfs) t->initTimeDependentVariable(*this);
   for (auto& n : nf2ffs) n->initTimeDependentVariable(*this);  // simulation.cpp:373-389
+  for (auto& n : nf2ff_tds) n->initTimeDependentVariable(*this);
   for (auto& m : monitors) m->initTimeDependentVariable(*this);
   if (objective) objective->dft.assign(objective->n[0] * objective->n[1] * objective->n[2], Cplx(0, 0));
 }
@@ -2033,6 +2040,7 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
         current = t;
         for (auto& m : monitors) m->update(*this, t);
         for (auto& n : nf2ffs) n->update(*this, t);  // Domain::record (domain.cpp:176-184)
+        for (auto& n : nf2ff_tds) n->update(*this, t);
         if (objective) {
           auto& ob = *objective;
           const Real ph = -2 * PI * ob.freq * static_cast<Real>(t + 1) * dt;
@@ -2062,6 +2070,129 @@ void Sim::loop() {  // simulation.cpp:291-317 with the Domain::run bodies
   loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   current = start_step + size;
   for (auto& m : monitors) m->output(*this);
+}
+
+// ------------------------------------------------------------ NF2FF (TD) --
+namespace {
+// rVector (nffft_td_data.h:22-35): E node along the surface normal, H nodes
+// across it (the same for E and H)
+Vec3 tdR(Sim& s, int c, Index i, Index j, Index k) {
+  if (c == 0) return {s.g.e_node[0](i), s.g.h_node[1](j), s.g.h_node[2](k)};
+  if (c == 1) return {s.g.h_node[0](i), s.g.e_node[1](j), s.g.h_node[2](k)};
+  return {s.g.h_node[0](i), s.g.h_node[1](j), s.g.e_node[2](k)};
+}
+// surfaceArea (nffft_td_data.h:37-47)
+Real tdArea(Sim& s, int c, Index i, Index j, Index k) {
+  if (c == 0) return s.g.e_size[1](j) * s.g.e_size[2](k);
+  if (c == 1) return s.g.e_size[2](k) * s.g.e_size[0](i);
+  return s.g.e_size[0](i) * s.g.e_size[1](j);
+}
+Vec3 tdDirection(Real theta, Real phi) {  // observationDirection (:520-522)
+  return {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+}
+}  // namespace
+
+void NF2FFTD::init(Sim& s) {
+  Task global[6];
+  nffftSurfaces(s, dist, surf, global);
+  // distanceRange (nfffft_time_domain.cpp:410-480) over the global surfaces
+  const Vec3 u = tdDirection(theta, phi);
+  dmin = std::numeric_limits<Real>::max();
+  dmax = std::numeric_limits<Real>::lowest();
+  for (int d = 0; d < 6; ++d) {
+    const Task& tk = global[d];
+    for (Index k = tk.z.s; k < tk.z.e; ++k)
+      for (Index j = tk.y.s; j < tk.y.e; ++j)
+        for (Index i = tk.x.s; i < tk.x.e; ++i) {
+          const Vec3 r = tdR(s, d / 2, i, j, k);
+          const Real dis = r.x * u.x + r.y * u.y + r.z * u.z;
+          dmin = std::min(dmin, dis);
+          dmax = std::max(dmax, dis);
+        }
+  }
+  for (int d = 0; d < 6; ++d) {
+    const Index cells = surf[d].valid() ? surf[d].x.size() * surf[d].y.size() * surf[d].z.size() : 0;
+    for (int w = 0; w < 4; ++w) prev[d][w].assign(cells, 0.0);
+  }
+}
+
+void NF2FFTD::initTimeDependentVariable(Sim& s) {  // nfffft_time_domain.cpp:53-66
+  const Real aux = (dmax - dmin) / (C_0 * s.dt);
+  num = static_cast<Index>(std::ceil(aux)) + s.size + 4;
+  for (int d = 0; d < 6; ++d)
+    for (int w = 0; w < 4; ++w) pot[d][w].assign(num, 0.0);
+}
+
+void NF2FFTD::update(Sim& s, Index t) {
+  const Vec3 u = tdDirection(theta, phi);
+  for (int attr = 0; attr < 2; ++attr)  // E on every surface, then H
+    for (int d = 0; d < 6; ++d) {
+      const Task& tk = surf[d];
+      if (!tk.valid()) continue;
+      const int c = d / 2, A = (c + 1) % 3, B = (c + 2) % 3;
+      const bool is_e = attr == 0, neg = d % 2 == 0;
+      const int fa = (is_e ? 0 : 3) + A, fb = (is_e ? 0 : 3) + B;
+      const Real offset = is_e ? 0.5 : 0.0;
+      Real coeff_a = is_e ? 1.0 : -1.0;
+      if (neg) coeff_a *= -1.0;
+      Real coeff_b = is_e ? -1.0 : 1.0;
+      if (neg) coeff_b *= -1.0;
+      auto& pa = pot[d][is_e ? 0 : 2];
+      auto& pb = pot[d][is_e ? 1 : 3];
+      auto& prev_a = prev[d][is_e ? 0 : 2];
+      auto& prev_b = prev[d][is_e ? 1 : 3];
+      const Real dt = s.dt;
+      Index q = 0;
+      for (Index i = tk.x.s; i < tk.x.e; ++i)
+        for (Index j = tk.y.s; j < tk.y.e; ++j)
+          for (Index k = tk.z.s; k < tk.z.e; ++k, ++q) {
+            const Vec3 r = tdR(s, c, i, j, k);
+            const Real time_delay = dmax - (r.x * u.x + r.y * u.y + r.z * u.z);
+            const Real time_step_delay = time_delay / (C_0 * dt);
+            const Index nn = static_cast<Index>(std::floor(time_step_delay + offset + t));
+            const Real coeff_f = time_step_delay + offset + t - nn;
+            const Real ds = tdArea(s, c, i, j, k);
+            const Real a_avg = surfaceCenter(s, c, fa, i, j, k);
+            const Real b_avg = surfaceCenter(s, c, fb, i, j, k);
+            const Real delta_a = a_avg - prev_a[q];
+            const Real delta_b = b_avg - prev_b[q];
+            pa.at(nn) += coeff_a * (1 - coeff_f) * ds * delta_b;
+            pa.at(nn + 1) += coeff_a * coeff_f * ds * delta_b;
+            pb.at(nn) += coeff_b * (1 - coeff_f) * ds * delta_a;
+            pb.at(nn + 1) += coeff_b * coeff_f * ds * delta_a;
+            prev_a[q] = a_avg;
+            prev_b[q] = b_avg;
+          }
+    }
+}
+
+void NF2FFTD::farField(Sim& s, std::vector<Real> out[4]) const {
+  // wa() ... ub(): the potentials / (C_0 dt 4 pi) (nffft_td_data.h:239-261)
+  auto sc = [&](int d, int w) {
+    std::vector<Real> v(num);
+    for (Index n = 0; n < num; ++n) v[n] = pot[d][w][n] / (C_0 * s.dt * 4 * PI);
+    return v;
+  };
+  // wx = zn.wa + zp.wa + yn.wb + yp.wb, ... (nfffft_time_domain.cpp:484-512)
+  auto sum4 = [&](std::vector<Real> a, const std::vector<Real>& b, const std::vector<Real>& c,
+                  const std::vector<Real>& d) {
+    for (Index n = 0; n < num; ++n) a[n] = a[n] + b[n] + c[n] + d[n];
+    return a;
+  };
+  const auto wx = sum4(sc(ZN, 2), sc(ZP, 2), sc(YN, 3), sc(YP, 3));
+  const auto wy = sum4(sc(XN, 2), sc(XP, 2), sc(ZN, 3), sc(ZP, 3));
+  const auto wz = sum4(sc(XN, 3), sc(XP, 3), sc(YN, 2), sc(YP, 2));
+  const auto ux = sum4(sc(ZN, 0), sc(ZP, 0), sc(YN, 1), sc(YP, 1));
+  const auto uy = sum4(sc(XN, 0), sc(XP, 0), sc(ZN, 1), sc(ZP, 1));
+  const auto uz = sum4(sc(XN, 1), sc(XP, 1), sc(YN, 0), sc(YP, 0));
+  const Real ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+  for (int w = 0; w < 4; ++w) out[w].assign(num, 0.0);
+  for (Index n = 0; n < num; ++n) {
+    out[0][n] = wx[n] * ct * cp + wy[n] * ct * sp - wz[n] * st;
+    out[1][n] = -wx[n] * sp + wy[n] * cp;
+    out[2][n] = ux[n] * ct * cp + uy[n] * ct * sp - uz[n] * st;
+    out[3][n] = -ux[n] * sp + uy[n] * cp;
+  }
 }
 
 // --------------------------------------------------------------- network --

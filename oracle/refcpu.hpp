@@ -527,6 +527,27 @@ struct NF2FF {
   Real power(Sim& s, Index f) const;  // nffft_fd_data_process.cpp:28-42, 162-194
 };
 
+// NFFFTTimeDomain (nfffft_time_domain.cpp, src/include/nffft/nffft_td_data.h)
+// for one process: per step, the time derivative of each surface cell's
+// face-centre tangential E and H, scaled by its area, binned by the
+// retarded time along one observation direction (theta, phi) into the
+// surface's potential arrays (ua, ub from E; wa, wb from H).
+struct NF2FFTD {
+  Index dist[3] = {1, 1, 1};
+  Real theta = 0, phi = 0;
+  std::string out_dir;
+  Task surf[6];  // node tasks (nffft.cpp:152-223), invalid ones empty
+  Real dmin = 0, dmax = 0;  // distanceRange over the global surfaces (:410-480)
+  Index num = 0;            // potential length (:53-66)
+  std::vector<Real> prev[6][4];  // per surface: ea, eb, ha, hb previous values, loop order
+  std::vector<Real> pot[6][4];   // per surface: ua, ub, wa, wb
+  void init(Sim& s);
+  void initTimeDependentVariable(Sim& s);
+  void update(Sim& s, Index t);  // nfffft_time_domain.cpp:68-82, nffft_td_data.h:159-231
+  // processFarField (nfffft_time_domain.cpp:84-160): w_theta, w_phi, u_theta, u_phi
+  void farField(Sim& s, std::vector<Real> out[4]) const;
+};
+
 // The DFT objective of the adjoint form (no reference counterpart): over the
 // grid points [o, o+n) of one E component, D_p = sum_t dt exp(-j 2 pi f
 // (t+1) dt) F_p^{t+1}, accumulated after each step's corrections;
@@ -558,6 +579,7 @@ struct Sim {
   std::vector<std::unique_ptr<Monitor>> monitors;
   std::vector<std::unique_ptr<NF2FF>> nf2ffs;
   std::vector<std::unique_ptr<Network>> networks;
+  std::vector<std::unique_ptr<NF2FFTD>> nf2ff_tds;
   bool write_files = true;
 
   GridSpace g;

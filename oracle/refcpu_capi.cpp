@@ -290,6 +290,14 @@ void parseSpec(Sim& s, const std::string& spec) {
       mon->out_dir = word(is);
       if (mon->out_dir == "-") mon->out_dir.clear();
       s.monitors.push_back(std::move(mon));
+    } else if (d == "nf2ff_td") {  // DX DY DZ THETA PHI OUT_DIR
+      auto n = std::make_unique<NF2FFTD>();
+      for (auto& v : n->dist) v = static_cast<Index>(num(is));
+      n->theta = num(is);
+      n->phi = num(is);
+      n->out_dir = word(is);
+      if (n->out_dir == "-") n->out_dir.clear();
+      s.nf2ff_tds.push_back(std::move(n));
     } else if (d == "network") {  // NAME OUT_DIR NF F.. NP (INDEX SOURCE ZRE ZIM CURRENT VOLTAGE)..
       word(is);
       auto nw = std::make_unique<Network>();
@@ -403,6 +411,29 @@ int ref_objective(void* p, double* value, double* dft) {
 }
 
 int ref_nf2ff_count(void* p) { return static_cast<int>(static_cast<Handle*>(p)->sim.nf2ffs.size()); }
+
+// Time-domain NF2FF n: potential length, the raw potential `which` (ua ub
+// wa wb) of surface d, and the far-field arrays (w_theta w_phi u_theta
+// u_phi) of processFarField.
+int64_t ref_nf2ff_td_num(void* p, int n) {
+  int64_t r = -1;
+  guard([&] { r = static_cast<int64_t>(static_cast<Handle*>(p)->sim.nf2ff_tds.at(n)->num); });
+  return r;
+}
+int ref_nf2ff_td_potential(void* p, int n, int d, int which, double* dst) {
+  return guard([&] {
+    const auto& v = static_cast<Handle*>(p)->sim.nf2ff_tds.at(n)->pot[d][which];
+    std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  });
+}
+int ref_nf2ff_td_far_field(void* p, int n, double* dst) {
+  return guard([&] {
+    auto* h = static_cast<Handle*>(p);
+    std::vector<Real> out[4];
+    h->sim.nf2ff_tds.at(n)->farField(h->sim, out);
+    for (int w = 0; w < 4; ++w) std::memcpy(dst + w * out[w].size(), out[w].data(), out[w].size() * sizeof(double));
+  });
+}
 
 // Network n: Network::output on the last run's recordings, then S(j, i)
 // by key 10 j + i as nf interleaved (re, im) doubles.

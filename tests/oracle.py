@@ -57,6 +57,9 @@ def lib():
         "ref_monitor_records": (ctypes.c_int, [vp, ctypes.c_int]),
         "ref_monitor_record": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, p64, vp]),
         "ref_network_output": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ref_nf2ff_td_num": (i64, [vp, ctypes.c_int]),
+        "ref_nf2ff_td_potential": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
+        "ref_nf2ff_td_far_field": (ctypes.c_int, [vp, ctypes.c_int, vp]),
         "ref_network_s": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp]),
         "ref_series_dft": (ctypes.c_int, [vp, i64, dbl, vp, i64, dbl, vp]),
         "ref_port_waves": (ctypes.c_int, [vp, vp, i64, dbl, vp, i64, dbl, dbl, vp, vp]),
@@ -106,6 +109,20 @@ class RefSim:
     def _check(self, rc):
         if rc != 0:
             raise RuntimeError(self.L.ref_last_error().decode())
+
+    def nf2ff_td_potential(self, n: int, d: int, which: int) -> np.ndarray:
+        """Raw potential `which` (0 ua, 1 ub, 2 wa, 3 wb) of surface d."""
+        num = int(self.L.ref_nf2ff_td_num(self.h, n))
+        out = np.zeros(num)
+        self._check(self.L.ref_nf2ff_td_potential(self.h, n, d, which, out.ctypes.data))
+        return out
+
+    def nf2ff_td_far_field(self, n: int) -> np.ndarray:
+        """processFarField: rows w_theta, w_phi, u_theta, u_phi."""
+        num = int(self.L.ref_nf2ff_td_num(self.h, n))
+        out = np.zeros((4, num))
+        self._check(self.L.ref_nf2ff_td_far_field(self.h, n, out.ctypes.data))
+        return out
 
     def network_s(self, n: int, keys, nf: int) -> dict:
         """Network::output on the last run, then {key: S} for `keys`."""

@@ -26,6 +26,7 @@ renders the scenario as line-oriented text, one directive per line:
     pec_plane NAME OX OY OZ SX SY SZ
     movie FIELD OX OY OZ SX SY SZ INTERVAL NAME OUT_DIR
     nf2ff DX DY DZ NF F1 .. FNF OUT_DIR [plain_dft]
+    nf2ff_td DX DY DZ THETA PHI OUT_DIR
     field_time NAME FIELD OX OY OZ SX SY SZ OUT_DIR
     voltage_monitor NAME OX OY OZ SX SY SZ DIR OUT_DIR
     current_monitor NAME OX OY OZ SX SY SZ DIR OUT_DIR
@@ -454,6 +455,24 @@ class NFFFTFrequencyDomain:
         return " ".join(["nf2ff", str(int(self.distance_x)), str(int(self.distance_y)), str(int(self.distance_z)),
                          str(len(self.frequencies))] + [_n(f) for f in self.frequencies] + [self.output_dir or "-"] +
                         ([] if self.reference_repeat else ["plain_dft"]))
+
+
+@dataclass
+class NFFFTTimeDomain:
+    """``NFFFTTimeDomain(distance_x, distance_y, distance_z, output_dir,
+    theta, phi)`` (nffft/nffft_time_domain.h): time-domain near-to-far-field
+    potentials along one observation direction."""
+
+    distance_x: int
+    distance_y: int
+    distance_z: int
+    output_dir: str = ""
+    theta: float = 0.0
+    phi: float = 0.0
+
+    def line(self):
+        return " ".join(["nf2ff_td", str(int(self.distance_x)), str(int(self.distance_y)), str(int(self.distance_z)),
+                         _n(self.theta), _n(self.phi), self.output_dir or "-"])
 
 
 @dataclass
