@@ -133,6 +133,25 @@ typedef struct {
   const double* th;            /* nf x nt complex: dt exp(-j 2 pi f (t + 1/2) dt) */
 } xfd_nf2ff_desc;
 
+/* One time-domain NF2FF (NFFFTTimeDomain, src/nffft/nfffft_time_domain.cpp;
+ * TDPlaneData::update, src/include/nffft/nffft_td_data.h:159-231), run
+ * after the recordings of every step t.  For each used surface d (XN XP YN
+ * YP ZN ZP) and each of its cells (i, j, k; k fastest) the library keeps the
+ * previous face-centre tangential E (a, b) and H (a, b) and adds, with
+ * f = s + t - floor(s + t) and nn = floor(s + t),
+ *   E: ua[nn] += ca (1 - f) ds dEb, ua[nn + 1] += ca f ds dEb,
+ *      ub likewise with cb and dEa;   H: wa, wb with dHb, dHa,
+ * in FP64 and in the reference's order (ca, cb = +-1 by attribute and side).
+ * s_e / s_h: per cell of the used surfaces in order, the time-step delay
+ * (dmax - r . u) / (c dt) plus 0.5 (E) or 0 (H); ds: the face areas. */
+typedef struct {
+  int32_t lo[6][3], hi[6][3];  /* surfaces XN XP YN YP ZN ZP; hi == lo: unused */
+  const double* s_e;
+  const double* s_h;
+  const double* ds;
+  int64_t num;                 /* potential length */
+} xfd_tdnf_desc;
+
 /* A weighted-sum recording: VoltageMonitor / CurrentMonitor
  * (voltage_monitor.cpp:191-227, current_monitor.cpp:272-385).  After the
  * corrections of every step t < count the library records, in FP64,
@@ -167,6 +186,8 @@ typedef struct {
   const xfd_nf2ff_desc* nf2ff;
   int32_t n_line;
   const xfd_line_desc* line;
+  int32_t n_tdnf;
+  const xfd_tdnf_desc* tdnf;
 } xfd_desc;
 
 typedef struct xfd_ctx xfd_ctx;
@@ -204,6 +225,10 @@ int xfd_read_line(xfd_ctx* ctx, int32_t m, double* dst);
  * of NF2FF `n`, frequency `f`, surface `d` (XN XP YN YP ZN ZP) to the host
  * as interleaved (re, im) doubles in the surface's (i, j, k) order. */
 int xfd_read_nf2ff(xfd_ctx* ctx, int32_t n, int32_t f, int32_t d, int32_t which, double* dst);
+
+/* Copies potential `which` (0 ua, 1 ub, 2 wa, 3 wb) of surface `d` of
+ * time-domain NF2FF `n` (num doubles) to the host. */
+int xfd_read_td_nf2ff(xfd_ctx* ctx, int32_t n, int32_t d, int32_t which, double* dst);
 
 /* Far-field potentials of NF2FF `n`, frequency `f`, from its accumulated
  * surface currents, on the device (FDPlaneData::getPotential /

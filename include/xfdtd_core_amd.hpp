@@ -598,6 +598,41 @@ class NFFFTFrequencyDomain {
   std::unique_ptr<Data> _data;
 };
 
+// Time-domain near-to-far-field transform (nffft/nffft_time_domain.h): for
+// one observation direction (theta, phi), the potentials of the surface
+// currents' time derivatives binned by retarded time (TDPlaneData,
+// src/include/nffft/nffft_td_data.h), accumulated on the GPU every step.
+class NFFFTTimeDomain {
+ public:
+  NFFFTTimeDomain(std::size_t distance_x, std::size_t distance_y, std::size_t distance_z, Real theta, Real phi);
+  NFFFTTimeDomain(std::size_t distance_x, std::size_t distance_y, std::size_t distance_z, std::string output_dir,
+                  Real theta, Real phi);
+  ~NFFFTTimeDomain();
+  Real theta() const { return _theta; }
+  Real phi() const { return _phi; }
+  std::size_t distance(int axis) const { return _distance[axis]; }
+  const std::string& outputDir() const { return _output_dir; }
+  void setOutputDir(std::string d) { _output_dir = std::move(d); }
+  // processFarField (nfffft_time_domain.cpp:84-160): writes w_theta.npy,
+  // w_phi.npy, u_theta.npy and u_phi.npy under outputDir().
+  void processFarField() const;
+  // The same four arrays (w_theta, w_phi, u_theta, u_phi) after run().
+  std::array<std::vector<Real>, 4> farField() const;
+  // Potential `which` (0 ua, 1 ub, 2 wa, 3 wb) of surface d after run().
+  const std::vector<Real>& potential(int d, int which) const;
+  // Length of the potentials, ceil((dmax - dmin) / (c dt)) + steps + 4 (after init).
+  std::size_t length() const;
+
+  struct Data;
+  Data& data() const { return *_data; }
+
+ private:
+  std::size_t _distance[3];
+  std::string _output_dir;
+  Real _theta, _phi;
+  std::unique_ptr<Data> _data;
+};
+
 struct ThreadConfig {  // accepted for API compatibility; the GPU path ignores it
   int x = 1, y = 1, z = 1;
 };
@@ -616,6 +651,7 @@ class Simulation {
   void addBoundary(std::shared_ptr<Boundary> boundary);
   void addMonitor(std::shared_ptr<Monitor> monitor);
   void addNF2FF(std::shared_ptr<NFFFTFrequencyDomain> nffft);
+  void addNF2FF(std::shared_ptr<NFFFTTimeDomain> nffft);
   void addNetwork(std::shared_ptr<Network> network);  // simulation.cpp:79-81
   const std::vector<std::shared_ptr<Network>>& networks() const;
 

@@ -61,6 +61,16 @@ int32_t xfd_sim_nf2ff_count(void* sim);
  * .npy files when the network has an output directory.  keys: up to cap
  * keys, returns their count (-1 on error).  xfd_sim_network_s copies one
  * S-parameter as nf interleaved (re, im) doubles. */
+/* Time-domain NF2FFs (spec line `nf2ff_td`) in spec order, after a run:
+ * the potential length, potential `which` (0 ua, 1 ub, 2 wa, 3 wb) of
+ * surface d, and the far field (NFFFTTimeDomain::processFarField: w_theta,
+ * w_phi, u_theta, u_phi, 4 x num doubles; the .npy files too when the
+ * NF2FF has an output directory). */
+int32_t xfd_sim_nf2ff_td_count(void* sim);
+int64_t xfd_sim_nf2ff_td_num(void* sim, int32_t n);
+int xfd_sim_nf2ff_td_potential(void* sim, int32_t n, int32_t d, int32_t which, double* dst);
+int xfd_sim_nf2ff_td_far_field(void* sim, int32_t n, double* dst);
+
 int32_t xfd_sim_network_count(void* sim);
 int32_t xfd_sim_network_nf(void* sim, int32_t n); /* frequency count, -1 if n is out of range */
 int xfd_sim_network_output(void* sim, int32_t n);

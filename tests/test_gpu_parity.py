@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("EX", "EY", "EZ", "HX", "HY", "HZ")
 TOL_F64 = 1e-12
+ETA0 = 376.730313668
 
 
 def say(msg):
@@ -506,3 +507,40 @@ def test_network_s_parameters(tmp_path, precision):
         np.testing.assert_array_equal(np.load(out / f"s{k}.npy"), got[k])
     np.testing.assert_array_equal(np.load(out / "frequencies.npy"), np.array(NET_FREQS))
     assert np.abs(want[11]).max() > 0.1 and np.abs(want[21]).max() > 1e-4  # |S21| ~ 1e-3 here
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_time_domain_nf2ff(tmp_path, precision):
+    """Time-domain NF2FF on the device (per-cell bins, then each bin summed
+    in the reference's cell order): every surface potential and the far
+    field equal the oracle's in FP64; FP32 within 1e-4 of the far field's
+    co-polarised peak.  The far-field .npy files carry the same values."""
+    from test_oracle_nf2ff_td import STEPS, dipole_td
+
+    theta, phi = np.pi / 3, 1.1
+    say(f"[gpu] time-domain NF2FF {precision}")
+    ref = RefSim(dipole_td(theta, phi))
+    ref.run(STEPS)
+    out = tmp_path / precision
+    dev = native.Simulation(dipole_td(theta, phi, str(out)), precision)
+    dev.run(STEPS)
+    far = dev.nf2ff_td_far_field(0)
+    want = ref.nf2ff_td_far_field(0)
+    assert far.shape == want.shape
+    if precision == "f64":
+        for d in range(6):
+            for w in range(4):
+                a, b = dev.nf2ff_td_potential(0, d, w), ref.nf2ff_td_potential(0, d, w)
+                assert np.abs(b).max() > 0, (d, w)
+                err = rel_l2(a, b)
+                say(f"  surface {d} potential {w}: rel_l2 {err:.2e}, bitwise {np.mean(a == b):.4f}")
+                assert err <= TOL_F64, (d, w, err)
+        for w in range(4):
+            assert rel_l2(far[w], want[w]) <= TOL_F64, w
+    else:
+        peak = np.abs(ETA0 * want[0] + want[3]).max()
+        err = max(np.abs(far[w] - want[w]).max() * (ETA0 if w < 2 else 1.0) for w in range(4)) / peak
+        say(f"  fp32 far-field error {err:.2e} of the co-polarised peak")
+        assert err < 1e-4, err
+    for w, name in enumerate(("w_theta", "w_phi", "u_theta", "u_phi")):
+        np.testing.assert_array_equal(np.load(out / f"{name}.npy"), far[w])

@@ -249,6 +249,14 @@ struct xfd_ctx {
     DevBuf te, th, acc;
   };
   std::vector<std::unique_ptr<Nf2ff>> nf2ffs;
+  // time-domain NF2FFs: per group cell s, ds, previous values, this step's
+  // bins and contributions; delay buckets; FP64 potentials
+  struct TdNf2ff {
+    xfd::TdNf2ffArgs a{};
+    int64_t omax = 0;  // largest bucket over the groups (bins reach t + omax + 2)
+    DevBuf s, ds, bstart, order, prev_a, prev_b, nn, ca1, ca2, cb1, cb2, pot;
+  };
+  std::vector<std::unique_ptr<TdNf2ff>> tdnfs;
   int ade_scheme = -1;
   int ade_poles = 0;
   DevBuf ade_mat, ade_tab, ade_j[3], ade_jp[3], ade_eprev[3];
@@ -313,6 +321,10 @@ struct xfd_ctx {
     b += pool.bytes + rec_e.bytes + seg_e.bytes + rec_h.bytes + seg_h.bytes + state.bytes;
     b += ade_mat.bytes + ade_tab.bytes;
     for (auto& n : nf2ffs) b += n->te.bytes + n->th.bytes + n->acc.bytes;
+    for (auto& n : tdnfs)
+      for (DevBuf* x : {&n->s, &n->ds, &n->bstart, &n->order, &n->prev_a, &n->prev_b, &n->nn, &n->ca1, &n->ca2,
+                        &n->cb1, &n->cb2, &n->pot})
+        b += x->bytes;
     for (int q = 0; q < 3; ++q) b += ade_j[q].bytes + ade_jp[q].bytes + ade_eprev[q].bytes;
     for (auto& gth : gathers) b += gth->out.bytes;
     b += ckpt.bytes;
@@ -770,6 +782,97 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     c.nf2ffs.push_back(std::move(nn));
   }
 
+  for (int ni = 0; ni < d.n_tdnf; ++ni) {
+    const auto& N = d.tdnf[ni];
+    require(N.s_e && N.s_h && N.ds && N.num > 0, "time-domain NF2FF arrays missing");
+    auto td = std::make_unique<xfd_ctx::TdNf2ff>();
+    auto& A = td->a;
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    int64_t cells[6];
+    int64_t total = 0;
+    for (int s6 = 0; s6 < 6; ++s6) {
+      cells[s6] = 1;
+      for (int q = 0; q < 3; ++q) {
+        const int lo = N.lo[s6][q], hi = N.hi[s6][q];
+        require(lo >= 1 && lo <= hi && hi <= n3[q], "NF2FF surface outside the grid interior");
+        A.lo[s6][q] = lo;
+        A.ext[s6][q] = hi - lo;
+        cells[s6] *= hi - lo;
+      }
+      total += cells[s6];
+    }
+    // groups g = 2 d + attr, each a copy of surface d's cells
+    std::vector<double> s(2 * total), ds(2 * total);
+    A.off[0] = 0;
+    int64_t src = 0;
+    for (int s6 = 0; s6 < 6; ++s6) {
+      for (int at = 0; at < 2; ++at) {
+        const int gr = 2 * s6 + at;
+        A.off[gr + 1] = A.off[gr] + cells[s6];
+        for (int64_t q = 0; q < cells[s6]; ++q) {
+          s[A.off[gr] + q] = (at == 0 ? N.s_e : N.s_h)[src + q];
+          ds[A.off[gr] + q] = N.ds[src + q];
+        }
+      }
+      src += cells[s6];
+    }
+    // delay buckets floor(s), cells sorted by (bucket, cell) per group
+    std::vector<int64_t> bstart, order;
+    A.boff[0] = 0;
+    A.bin_off[0] = 0;
+    for (int gr = 0; gr < 12; ++gr) {
+      const int64_t c0 = A.off[gr], nc = A.off[gr + 1] - c0;
+      int64_t omin = 0, omax = -1;
+      std::vector<int64_t> o(nc);
+      for (int64_t q = 0; q < nc; ++q) {
+        require(s[c0 + q] >= 0 && s[c0 + q] < 1e15, "time-domain NF2FF delay out of range");
+        o[q] = static_cast<int64_t>(std::floor(s[c0 + q]));
+        omin = q == 0 ? o[q] : std::min(omin, o[q]);
+        omax = q == 0 ? o[q] : std::max(omax, o[q]);
+      }
+      const int64_t nb = nc ? omax - omin + 1 : 0;
+      A.omin[gr] = omin;
+      A.nbk[gr] = nb;
+      td->omax = std::max(td->omax, omax);
+      std::vector<int64_t> cnt(nb + 1, 0);
+      for (int64_t q = 0; q < nc; ++q) ++cnt[o[q] - omin + 1];
+      const int64_t base = static_cast<int64_t>(order.size());
+      for (int64_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+      for (int64_t b = 0; b <= nb; ++b) bstart.push_back(base + cnt[b]);
+      A.boff[gr + 1] = static_cast<int64_t>(bstart.size());
+      order.resize(base + nc);
+      std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+      for (int64_t q = 0; q < nc; ++q) order[base + fill[o[q] - omin]++] = q;  // ascending q within a bucket
+      A.bin_off[gr + 1] = A.bin_off[gr] + (nc ? nb + 2 : 0);
+    }
+    A.num = N.num;
+    uploadVector(s, td->s);
+    uploadVector(ds, td->ds);
+    uploadVector(bstart, td->bstart);
+    uploadVector(order, td->order);
+    const size_t nb = static_cast<size_t>(std::max<int64_t>(A.off[12], 1));
+    for (DevBuf* x : {&td->prev_a, &td->prev_b, &td->ca1, &td->ca2, &td->cb1, &td->cb2}) {
+      x->alloc(nb * sizeof(double));
+      XFD_HIP(hipMemsetAsync(x->p, 0, x->bytes, c.stream));
+    }
+    td->nn.alloc(nb * sizeof(int64_t));
+    td->pot.alloc(static_cast<size_t>(24) * N.num * sizeof(double));
+    XFD_HIP(hipMemsetAsync(td->pot.p, 0, td->pot.bytes, c.stream));
+    A.s = td->s.as<double>();
+    A.ds = td->ds.as<double>();
+    A.bstart = td->bstart.as<int64_t>();
+    A.order = td->order.as<int64_t>();
+    A.prev_a = td->prev_a.as<double>();
+    A.prev_b = td->prev_b.as<double>();
+    A.nn = td->nn.as<int64_t>();
+    A.ca1 = td->ca1.as<double>();
+    A.ca2 = td->ca2.as<double>();
+    A.cb1 = td->cb1.as<double>();
+    A.cb2 = td->cb2.as<double>();
+    A.pot = td->pot.as<double>();
+    c.tdnfs.push_back(std::move(td));
+  }
+
   for (int gi = 0; gi < d.n_gather; ++gi) {
     const auto& G = d.gather[gi];
     require(G.field >= 0 && G.field < 6 && G.interval > 0 && G.count >= 0, "bad recording");
@@ -1089,6 +1192,12 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (auto& nn : c.nf2ffs)  // Domain::record: monitors, then NF2FF (domain.cpp:176-184)
       c.timed(K_NF2FF, [&] { xfd::Launch<T>::nf2ff(st, c.g, F, nn->a, t); });
+    for (auto& td : c.tdnfs) {
+      // TDPlaneData::update writes potential.at(nn + 1): the arrays end
+      // before this step's last bin
+      require(t + td->omax + 2 < td->a.num, "time-domain NF2FF potential too short for this step");
+      c.timed(K_NF2FF, [&] { xfd::Launch<T>::tdNf2ff(st, c.g, F, td->a, t); });
+    }
     XFD_HIP(hipGetLastError());
     if (c.prof && c.pending.size() > 4096) {
       XFD_HIP(hipStreamSynchronize(st));
@@ -1115,6 +1224,11 @@ void resetState(xfd_ctx& c) {
     zero(c.ade_eprev[q]);
   }
   for (auto& n : c.nf2ffs) zero(n->acc);
+  for (auto& n : c.tdnfs) {
+    zero(n->prev_a);
+    zero(n->prev_b);
+    zero(n->pot);
+  }
   for (auto& gg : c.gathers) zero(gg->out);
   zero(c.probe_out);
   zero(c.line_out);
@@ -1782,7 +1896,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       xfd_ctx* c = nullptr;
       if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
       b->cases.emplace_back(c);
-      b->shared = b->shared && c->classed && c->ade_scheme < 0;
+      b->shared = b->shared && c->classed && c->ade_scheme < 0 && c->tdnfs.empty();
 
     }
     XFD_HIP(hipStreamCreate(&b->stream));
@@ -1877,6 +1991,16 @@ int xfd_read_nf2ff(xfd_ctx* c, int32_t n, int32_t f, int32_t d, int32_t which, d
     if (cnt == 0) return;
     const double2* src = nn.acc.as<double2>() + (static_cast<int64_t>(f) * 4 + which) * cells + nn.a.off[d];
     XFD_HIP(hipMemcpy(dst, src, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+  });
+}
+
+int xfd_read_td_nf2ff(xfd_ctx* c, int32_t n, int32_t d, int32_t which, double* dst) {
+  return guard([&] {
+    require(c && dst && n >= 0 && n < static_cast<int32_t>(c->tdnfs.size()), "bad time-domain NF2FF index");
+    require(d >= 0 && d < 6 && which >= 0 && which < 4, "bad time-domain NF2FF slice");
+    auto& td = *c->tdnfs[n];
+    XFD_HIP(hipMemcpy(dst, td.pot.as<double>() + (static_cast<int64_t>(d) * 4 + which) * td.a.num,
+                      td.a.num * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 

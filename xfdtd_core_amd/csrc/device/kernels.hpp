@@ -843,6 +843,112 @@ __global__ __launch_bounds__(256) void k_pack(Grid g, const T* __restrict__ src,
   dst[t] = static_cast<double>(src[(static_cast<int64_t>(u) * g.NY + v) * g.pz + w]);
 }
 
+// Time-domain NF2FF, phase 1, one thread per (group, cell):
+// TDPlaneData::update (nffft_td_data.h:159-231) up to the potential adds --
+// the face-centre tangential values, their change since the previous step,
+// the bin nn = floor(s + t) with weight f = s + t - nn, and the four
+// contributions coeff (1 - f) ds delta / coeff f ds delta to bins nn, nn + 1.
+template <class T>
+__device__ __forceinline__ void tdCell(const Grid& g, const FieldPtrs<T>& F, const TdNf2ffArgs& a, int64_t cell,
+                                       int64_t t) {
+  if (cell >= a.off[12]) return;
+  int gr = 0;
+  while (cell >= a.off[gr + 1]) ++gr;
+  const int d = gr >> 1;
+  const bool is_e = (gr & 1) == 0;
+  const int64_t q = cell - a.off[gr];
+  const int e1 = a.ext[d][1], e2 = a.ext[d][2];
+  const int k = a.lo[d][2] + static_cast<int>(q % e2);
+  const int j = a.lo[d][1] + static_cast<int>((q / e2) % e1);
+  const int i = a.lo[d][0] + static_cast<int>(q / (static_cast<int64_t>(e2) * e1));
+  const int64_t idx = (static_cast<int64_t>(i) * g.NY + j) * g.pz + k;
+  const int c = d / 2, A = (c + 1) % 3, B = (c + 2) % 3;
+  const bool neg = (d & 1) == 0;
+  double coeff_a = is_e ? 1.0 : -1.0;
+  if (neg) coeff_a *= -1.0;
+  double coeff_b = is_e ? -1.0 : 1.0;
+  if (neg) coeff_b *= -1.0;
+  const int base = is_e ? 0 : 3;
+  const double a_avg = static_cast<double>(faceCentre(g, F, c, base + A, idx));
+  const double b_avg = static_cast<double>(faceCentre(g, F, c, base + B, idx));
+  const double delta_a = a_avg - a.prev_a[cell];
+  const double delta_b = b_avg - a.prev_b[cell];
+  a.prev_a[cell] = a_avg;
+  a.prev_b[cell] = b_avg;
+  const double v = a.s[cell] + static_cast<double>(t);
+  const int64_t nn = static_cast<int64_t>(floor(v));
+  const double coeff_f = v - static_cast<double>(nn);
+  const double ds = a.ds[cell];
+  a.nn[cell] = nn;
+  a.ca1[cell] = coeff_a * (1 - coeff_f) * ds * delta_b;
+  a.ca2[cell] = coeff_a * coeff_f * ds * delta_b;
+  a.cb1[cell] = coeff_b * (1 - coeff_f) * ds * delta_a;
+  a.cb2[cell] = coeff_b * coeff_f * ds * delta_a;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_td_cells(Grid g, FieldPtrs<T> F, TdNf2ffArgs a, int64_t t) {
+  tdCell(g, F, a, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, t);
+}
+
+// Phase 2, one thread per (group, bin of this step's window): bin n adds the
+// contributions of the cells with nn == n (first) or nn + 1 == n (second),
+// in cell order, to the running potential sums -- the reference's order.
+__device__ __forceinline__ void tdBin(const TdNf2ffArgs& a, int64_t id, int64_t t) {
+  if (id >= a.bin_off[12]) return;
+  int gr = 0;
+  while (id >= a.bin_off[gr + 1]) ++gr;
+  const int64_t local = id - a.bin_off[gr];
+  const int64_t n = t + a.omin[gr] + local;
+  if (n < 0 || n >= a.num) return;
+  const int64_t* bs = a.bstart + a.boff[gr];
+  const int64_t nb = a.nbk[gr];
+  // buckets local - 2, local - 1, local (clamped to the group's range)
+  int64_t p[3], e[3];
+  for (int r = 0; r < 3; ++r) {
+    const int64_t b = local - 2 + r;
+    if (b < 0 || b >= nb) {
+      p[r] = e[r] = 0;
+    } else {
+      p[r] = bs[b];
+      e[r] = bs[b + 1];
+    }
+  }
+  const int64_t c0 = a.off[gr];
+  const int d = gr >> 1;
+  const int w = (gr & 1) ? 2 : 0;
+  double* pa = a.pot + (static_cast<int64_t>(d) * 4 + w) * a.num;
+  double* pb = pa + a.num;
+  double va = pa[n], vb = pb[n];
+  while (true) {
+    int r = -1;
+    int64_t best = 0;
+    for (int x = 0; x < 3; ++x)
+      if (p[x] < e[x] && (r < 0 || a.order[p[x]] < best)) {
+        r = x;
+        best = a.order[p[x]];
+      }
+    if (r < 0) break;
+    ++p[r];
+    const int64_t cell = c0 + best;
+    const int64_t nn = a.nn[cell];
+    if (nn == n) {
+      va += a.ca1[cell];
+      vb += a.cb1[cell];
+    } else if (nn + 1 == n) {
+      va += a.ca2[cell];
+      vb += a.cb2[cell];
+    }
+  }
+  pa[n] = va;
+  pb[n] = vb;
+}
+
+template <class T>  // one instance per precision's translation unit
+__global__ __launch_bounds__(256) void k_td_bins(TdNf2ffArgs a, int64_t t) {
+  tdBin(a, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, t);
+}
+
 // Host-side launchers, instantiated once per precision.
 template <class T>
 struct Launch {
@@ -871,6 +977,8 @@ struct Launch {
   static void lines(hipStream_t s, const FieldPtrs<T>& F, const LineTerm* terms, const int64_t* off,
                     const double* scale, int n, double* out, int64_t stride, int64_t t);
   static void nf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Nf2ffArgs& a, int64_t t);
+  // time-domain NF2FF: both phases of step t
+  static void tdNf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const TdNf2ffArgs& a, int64_t t);
   // na directions, out: na x 8 doubles (A_theta, A_phi, F_theta, F_phi as re, im)
   static void far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* out);
   static void unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2);

@@ -117,6 +117,13 @@ void Launch<T>::nf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const
 }
 
 template <class T>
+void Launch<T>::tdNf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const TdNf2ffArgs& a, int64_t t) {
+  if (a.off[12] <= 0) return;
+  hipLaunchKernelGGL(k_td_cells<T>, dim3(blocks_for(a.off[12], 256)), dim3(256), 0, s, g, F, a, t);
+  hipLaunchKernelGGL(k_td_bins<T>, dim3(blocks_for(a.bin_off[12], 256)), dim3(256), 0, s, a, t);
+}
+
+template <class T>
 void Launch<T>::probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                        int64_t t) {
   if (n <= 0) return;

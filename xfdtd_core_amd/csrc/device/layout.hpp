@@ -152,6 +152,41 @@ struct Nf2ffArgs {
   double2* acc;
 };
 
+// Time-domain NF2FF (NFFFTTimeDomain): 12 groups g = 2 d + attr of surface
+// d (XN..ZP) and attribute attr (0: E -> potentials ua, ub; 1: H -> wa, wb),
+// each the cells of the surface's box in the reference's loop order (k
+// fastest).  Group g owns cells [off[g], off[g + 1]); per group cell s (the
+// time step delay plus the attribute's offset: the bin of step t is
+// floor(s + t)) and ds (the face area).  Per step, k_td_cells computes every cell's bin nn
+// and its four contributions; k_td_bins then adds them bin by bin.  A cell's
+// bin at step t is o + t with o = floor(s) or floor(s) + 1 (rounding of
+// s + t), so the cells are bucketed by floor(s) (`order`, sorted by bucket
+// then cell; group g's buckets at bstart + boff[g], its nbk[g] buckets
+// starting at omin[g]); bin n of step t draws on buckets n - t - 2 .. n - t,
+// merged in cell order, so every bin sums in the reference's order.
+// pot holds per surface d: ua, ub, wa, wb of num values (FP64).
+struct TdNf2ffArgs {
+  int lo[6][3], ext[6][3];
+  int64_t off[13];
+  int64_t omin[12];
+  int64_t nbk[12];
+  int64_t boff[13];      // group g's bucket starts: bstart[boff[g] .. boff[g] + nbk[g]]
+  int64_t bin_off[13];   // thread ids of group g's bins: nbk[g] + 2 bins per step
+  int64_t num;
+  const double* s;
+  const double* ds;
+  const int64_t* bstart;
+  const int64_t* order;  // cell ids within the group
+  double* prev_a;
+  double* prev_b;
+  int64_t* nn;
+  double* ca1;
+  double* ca2;
+  double* cb1;
+  double* cb2;
+  double* pot;
+};
+
 // One CPML slab for one half-step.  The thread space is the union box of the
 // two corrected components; `ext_a` / `ext_b` give each component's extent
 // along the tangential axes a and b (the other component's is one larger).

@@ -10,6 +10,7 @@
 #include <filesystem>
 #include <sstream>
 #include <iomanip>
+#include <limits>
 
 #include "../../../include/xfdtd_core_amd.hpp"
 #include "nf2ff_data.hpp"
@@ -304,6 +305,149 @@ void NFFFTFrequencyDomain::outputRadiationPower() const {
   }
   std::filesystem::create_directories(_output_dir.empty() ? "." : _output_dir);
   npy::write((std::filesystem::path(_output_dir) / "power.npy").string(), {2, nf}, out.data());
+}
+
+// ------------------------------------------------------ time domain ------
+NFFFTTimeDomain::NFFFTTimeDomain(std::size_t dx, std::size_t dy, std::size_t dz, Real theta, Real phi)
+    : NFFFTTimeDomain(dx, dy, dz, "", theta, phi) {}
+
+NFFFTTimeDomain::NFFFTTimeDomain(std::size_t dx, std::size_t dy, std::size_t dz, std::string output_dir, Real theta,
+                                 Real phi)
+    : _distance{dx, dy, dz}, _output_dir(std::move(output_dir)), _theta(theta), _phi(phi),
+      _data(std::make_unique<Data>()) {
+  if (dx == 0) throw XFDTDException("distance_x == 0");  // nffft.cpp:19-33
+  if (dy == 0) throw XFDTDException("distance_y == 0");
+  if (dz == 0) throw XFDTDException("distance_z == 0");
+}
+NFFFTTimeDomain::~NFFFTTimeDomain() = default;
+
+// NFFFTTimeDomain::init / initTimeDependentVariable (nfffft_time_domain.cpp:
+// 35-66): the node surfaces (as the frequency-domain NF2FF), the distance
+// range of the six global surfaces along the observation direction
+// (:410-480), and per node-surface cell the time-step delay
+// (dmax - r . u) / (c dt) plus the E (0.5) or H (0) offset and the face
+// area (nffft_td_data.h:22-47, 178-199), in the reference's expressions.
+void NFFFTTimeDomain::Data::setup(const std::size_t dist[3], const std::size_t n[3],
+                                  const std::vector<Real> node_e[3], const std::vector<Real> node_h[3],
+                                  const std::vector<Real> size_e[3], Real theta, Real phi, Real dt_,
+                                  std::size_t steps) {
+  std::size_t lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = dist[a];
+    hi[a] = n[a] > dist[a] ? n[a] - dist[a] : 0;
+    if (hi[a] <= lo[a]) throw XFDTDException("NFFFT: Global box is not legal");
+  }
+  std::size_t glo[6][3], ghi[6][3];
+  bool any = false;
+  for (int d = 0; d < 6; ++d) {
+    const int c = d / 2;
+    const bool neg = d % 2 == 0;
+    const std::size_t cs = neg ? lo[c] : hi[c];
+    used[d] = !((neg && cs <= 1) || (!neg && cs >= n[c]));
+    for (int a = 0; a < 3; ++a) {
+      glo[d][a] = a == c ? cs : lo[a];
+      ghi[d][a] = a == c ? cs + 1 : hi[a];
+      box_lo[d][a] = used[d] ? glo[d][a] : 0;
+      box_hi[d][a] = used[d] ? ghi[d][a] : 0;
+    }
+    any = any || used[d];
+  }
+  if (!any) throw XFDTDException("NFFFT: Invalid task");
+  // observationDirection (:520-522)
+  const Real ux = std::sin(theta) * std::cos(phi), uy = std::sin(theta) * std::sin(phi), uz = std::cos(theta);
+  auto r = [&](int c, std::size_t i, std::size_t j, std::size_t k, Real& x, Real& y, Real& z) {
+    x = c == 0 ? node_e[0][i] : node_h[0][i];
+    y = c == 1 ? node_e[1][j] : node_h[1][j];
+    z = c == 2 ? node_e[2][k] : node_h[2][k];
+  };
+  Real dmin = std::numeric_limits<Real>::max(), dmax = std::numeric_limits<Real>::lowest();
+  for (int d = 0; d < 6; ++d)
+    for (std::size_t k = glo[d][2]; k < ghi[d][2]; ++k)
+      for (std::size_t j = glo[d][1]; j < ghi[d][1]; ++j)
+        for (std::size_t i = glo[d][0]; i < ghi[d][0]; ++i) {
+          Real x, y, z;
+          r(d / 2, i, j, k, x, y, z);
+          const Real dis = x * ux + y * uy + z * uz;
+          dmin = std::min(dmin, dis);
+          dmax = std::max(dmax, dis);
+        }
+  dt = dt_;
+  const Real aux = (dmax - dmin) / (kC0N * dt);
+  num = static_cast<std::size_t>(std::ceil(aux)) + steps + 4;
+  s_e.clear();
+  s_h.clear();
+  ds.clear();
+  for (int d = 0; d < 6; ++d) {
+    if (!used[d]) continue;
+    const int c = d / 2;
+    for (std::size_t i = box_lo[d][0]; i < box_hi[d][0]; ++i)
+      for (std::size_t j = box_lo[d][1]; j < box_hi[d][1]; ++j)
+        for (std::size_t k = box_lo[d][2]; k < box_hi[d][2]; ++k) {
+          Real x, y, z;
+          r(c, i, j, k, x, y, z);
+          const Real time_delay = dmax - (x * ux + y * uy + z * uz);
+          const Real time_step_delay = time_delay / (kC0N * dt);
+          s_e.push_back(time_step_delay + 0.5);
+          s_h.push_back(time_step_delay + 0.0);
+          ds.push_back(c == 0 ? size_e[1][j] * size_e[2][k]
+                              : (c == 1 ? size_e[2][k] * size_e[0][i] : size_e[0][i] * size_e[1][j]));
+        }
+  }
+  for (auto& p : pot)
+    for (auto& v : p) v.clear();
+}
+
+std::size_t NFFFTTimeDomain::length() const { return _data->num; }
+
+const std::vector<Real>& NFFFTTimeDomain::potential(int d, int which) const {
+  if (d < 0 || d >= 6 || which < 0 || which >= 4) throw XFDTDException("NFFFTTimeDomain: bad potential");
+  return _data->pot[d][which];
+}
+
+// processFarField (nfffft_time_domain.cpp:84-160) with wa() .. ub() =
+// potential / (c dt 4 pi) (nffft_td_data.h:239-261) and wx .. uz the sums of
+// the surfaces' potentials along each axis (nfffft_time_domain.cpp:484-512).
+std::array<std::vector<Real>, 4> NFFFTTimeDomain::farField() const {
+  const Data& D = *_data;
+  const std::size_t num = D.num;
+  for (auto& p : D.pot)
+    for (auto& v : p)
+      if (v.size() != num) throw XFDTDException("NFFFTTimeDomain: no potentials (run the simulation first)");
+  auto sc = [&](int d, int w) {
+    std::vector<Real> v(num);
+    for (std::size_t q = 0; q < num; ++q) v[q] = D.pot[d][w][q] / (kC0N * D.dt * 4 * kPiN);
+    return v;
+  };
+  auto sum4 = [&](std::vector<Real> a, const std::vector<Real>& b, const std::vector<Real>& c,
+                  const std::vector<Real>& d) {
+    for (std::size_t q = 0; q < num; ++q) a[q] = a[q] + b[q] + c[q] + d[q];
+    return a;
+  };
+  enum { XN, XP, YN, YP, ZN, ZP };
+  const auto wx = sum4(sc(ZN, 2), sc(ZP, 2), sc(YN, 3), sc(YP, 3));
+  const auto wy = sum4(sc(XN, 2), sc(XP, 2), sc(ZN, 3), sc(ZP, 3));
+  const auto wz = sum4(sc(XN, 3), sc(XP, 3), sc(YN, 2), sc(YP, 2));
+  const auto ux = sum4(sc(ZN, 0), sc(ZP, 0), sc(YN, 1), sc(YP, 1));
+  const auto uy = sum4(sc(XN, 0), sc(XP, 0), sc(ZN, 1), sc(ZP, 1));
+  const auto uz = sum4(sc(XN, 1), sc(XP, 1), sc(YN, 0), sc(YP, 0));
+  const Real ct = std::cos(_theta), st = std::sin(_theta), cp = std::cos(_phi), sp = std::sin(_phi);
+  std::array<std::vector<Real>, 4> out;
+  for (auto& o : out) o.assign(num, 0.0);
+  for (std::size_t q = 0; q < num; ++q) {
+    out[0][q] = wx[q] * ct * cp + wy[q] * ct * sp - wz[q] * st;
+    out[1][q] = -wx[q] * sp + wy[q] * cp;
+    out[2][q] = ux[q] * ct * cp + uy[q] * ct * sp - uz[q] * st;
+    out[3][q] = -ux[q] * sp + uy[q] * cp;
+  }
+  return out;
+}
+
+void NFFFTTimeDomain::processFarField() const {
+  const auto out = farField();
+  const char* names[4] = {"w_theta.npy", "w_phi.npy", "u_theta.npy", "u_phi.npy"};
+  std::filesystem::create_directories(_output_dir.empty() ? "." : _output_dir);
+  for (int w = 0; w < 4; ++w)
+    npy::write((std::filesystem::path(_output_dir) / names[w]).string(), {out[w].size()}, out[w].data());
 }
 
 }  // namespace xfdtd

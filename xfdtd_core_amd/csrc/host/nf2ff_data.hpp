@@ -32,4 +32,19 @@ struct NFFFTFrequencyDomain::Data {
   std::size_t cells(int d) const;
 };
 
+// State of one NFFFTTimeDomain on the host: the surfaces, per cell of the
+// used surfaces (loop order) the time-step delays plus offsets and the face
+// areas handed to the device, and the potentials copied back after the run.
+struct NFFFTTimeDomain::Data {
+  bool used[6] = {false, false, false, false, false, false};
+  std::size_t box_lo[6][3] = {}, box_hi[6][3] = {};
+  std::vector<Real> s_e, s_h, ds;
+  std::size_t num = 0;
+  Real dt = 0;
+  std::vector<Real> pot[6][4];  // ua ub wa wb per surface
+  void setup(const std::size_t dist[3], const std::size_t n[3], const std::vector<Real> node_e[3],
+             const std::vector<Real> node_h[3], const std::vector<Real> size_e[3], Real theta, Real phi, Real dt,
+             std::size_t steps);
+};
+
 }  // namespace xfdtd

@@ -19,6 +19,7 @@ struct SimHandle {
   std::unique_ptr<Simulation> sim;
   std::vector<std::shared_ptr<Monitor>> monitors;
   std::vector<std::shared_ptr<NFFFTFrequencyDomain>> nf2ffs;
+  std::vector<std::shared_ptr<NFFFTTimeDomain>> nf2ff_tds;
   std::vector<std::shared_ptr<Network>> networks;
   Precision precision = Precision::FP64;
 };
@@ -226,6 +227,15 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
       if (t.more() && t.word() == "plain_dft") n->setReferenceRepeat(false);  // the non-repeating DFT
       h->nf2ffs.push_back(n);
       s.addNF2FF(n);
+    } else if (d == "nf2ff_td") {  // DX DY DZ THETA PHI OUT_DIR
+      std::size_t dist[3];
+      for (auto& x : dist) x = static_cast<std::size_t>(t.num());
+      const Real theta = t.num(), phi = t.num();
+      std::string dir = t.word();
+      if (dir == "-") dir.clear();
+      auto n = std::make_shared<NFFFTTimeDomain>(dist[0], dist[1], dist[2], dir, theta, phi);
+      h->nf2ff_tds.push_back(n);
+      s.addNF2FF(n);
     } else if (d == "design_region") {  // ox oy oz nx ny nz eps_r... (cells, k fastest)
       std::array<Index, 3> o, n;
       for (auto& x : o) x = static_cast<Index>(t.num());
@@ -389,6 +399,31 @@ int32_t xfd_sim_coef_classed(void* p) { return H(p)->sim->classedCoefficients() 
 int32_t xfd_sim_cpml_fused(void* p) { return H(p)->sim->fusedCpml(); }
 
 int32_t xfd_sim_nf2ff_count(void* p) { return static_cast<int32_t>(H(p)->nf2ffs.size()); }
+
+int32_t xfd_sim_nf2ff_td_count(void* p) { return static_cast<int32_t>(H(p)->nf2ff_tds.size()); }
+
+int64_t xfd_sim_nf2ff_td_num(void* p, int32_t n) {
+  if (n < 0 || n >= static_cast<int32_t>(H(p)->nf2ff_tds.size())) return -1;
+  return static_cast<int64_t>(H(p)->nf2ff_tds[n]->length());
+}
+
+int xfd_sim_nf2ff_td_potential(void* p, int32_t n, int32_t d, int32_t which, double* dst) {
+  return guard([&] {
+    if (n < 0 || n >= static_cast<int32_t>(H(p)->nf2ff_tds.size())) throw XFDTDException("bad NF2FF index");
+    const auto& v = H(p)->nf2ff_tds[n]->potential(d, which);
+    std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int xfd_sim_nf2ff_td_far_field(void* p, int32_t n, double* dst) {
+  return guard([&] {
+    if (n < 0 || n >= static_cast<int32_t>(H(p)->nf2ff_tds.size())) throw XFDTDException("bad NF2FF index");
+    const auto& nf = *H(p)->nf2ff_tds[n];
+    const auto out = nf.farField();
+    for (int w = 0; w < 4; ++w) std::memcpy(dst + w * out[w].size(), out[w].data(), out[w].size() * sizeof(double));
+    if (!nf.outputDir().empty()) nf.processFarField();
+  });
+}
 
 int32_t xfd_sim_network_count(void* p) { return static_cast<int32_t>(H(p)->networks.size()); }
 

@@ -414,6 +414,8 @@ struct Simulation::Impl {
   std::vector<std::shared_ptr<NFFFTFrequencyDomain>> nf2ffs;
   std::vector<std::shared_ptr<Network>> networks;
   std::vector<xfd_nf2ff_desc> nf2ff_d;
+  std::vector<std::shared_ptr<NFFFTTimeDomain>> nf2ff_tds;
+  std::vector<xfd_tdnf_desc> tdnf_d;
 
   // grid (grid_space_generator.cpp, grid_space.cpp:526-544)
   std::size_t N[3] = {0, 0, 0};
@@ -948,6 +950,20 @@ struct Simulation::Impl {
       d.repeat = n->referenceRepeat() ? d.nf : 1;
       d.nt = static_cast<int64_t>(D.nt);
       nf2ff_d.push_back(d);
+    }
+    tdnf_d.clear();
+    for (auto& n : nf2ff_tds) {  // time-domain NF2FFs
+      const std::size_t dist[3] = {n->distance(0), n->distance(1), n->distance(2)};
+      auto& D = n->data();
+      D.setup(dist, N, e_node, h_node, e_size, n->theta(), n->phi(), dt, steps);
+      xfd_tdnf_desc d{};
+      for (int s6 = 0; s6 < 6; ++s6)
+        for (int a = 0; a < 3; ++a) {
+          d.lo[s6][a] = static_cast<int32_t>(D.used[s6] ? D.box_lo[s6][a] : 1);
+          d.hi[s6][a] = static_cast<int32_t>(D.used[s6] ? D.box_hi[s6][a] : 1);
+        }
+      d.num = static_cast<int64_t>(D.num);
+      tdnf_d.push_back(d);
     }
   }
 
@@ -1589,6 +1605,14 @@ struct Simulation::Impl {
     }
     dd.n_nf2ff = static_cast<int32_t>(nf2ff_d.size());
     dd.nf2ff = nf2ff_d.empty() ? nullptr : nf2ff_d.data();
+    for (std::size_t q = 0; q < nf2ff_tds.size(); ++q) {
+      auto& D = nf2ff_tds[q]->data();
+      tdnf_d[q].s_e = D.s_e.data();
+      tdnf_d[q].s_h = D.s_h.data();
+      tdnf_d[q].ds = D.ds.data();
+    }
+    dd.n_tdnf = static_cast<int32_t>(tdnf_d.size());
+    dd.tdnf = tdnf_d.empty() ? nullptr : tdnf_d.data();
     ld.clear();
     for (auto& l : lines) {
       xfd_line_desc x{};
@@ -1868,6 +1892,15 @@ struct Simulation::Impl {
               throw XFDTDSimulationException(std::string("xfd_read_nf2ff: ") + xfd_last_error());
           }
     }
+    for (std::size_t q = 0; q < nf2ff_tds.size(); ++q) {  // time-domain potentials
+      auto& D = nf2ff_tds[q]->data();
+      for (int d = 0; d < 6; ++d)
+        for (int w = 0; w < 4; ++w) {
+          D.pot[d][w].assign(D.num, 0.0);
+          if (xfd_read_td_nf2ff(ctx, static_cast<int32_t>(q), d, w, D.pot[d][w].data()) != XFD_OK)
+            throw XFDTDSimulationException(std::string("xfd_read_td_nf2ff: ") + xfd_last_error());
+        }
+    }
   }
 };
 
@@ -1896,6 +1929,10 @@ void Simulation::addMonitor(std::shared_ptr<Monitor> m) {
 }
 void Simulation::addNF2FF(std::shared_ptr<NFFFTFrequencyDomain> n) {
   _impl->nf2ffs.push_back(std::move(n));
+  _impl->precomputed = false;
+}
+void Simulation::addNF2FF(std::shared_ptr<NFFFTTimeDomain> n) {
+  _impl->nf2ff_tds.push_back(std::move(n));
   _impl->precomputed = false;
 }
 void Simulation::addNetwork(std::shared_ptr<Network> n) {

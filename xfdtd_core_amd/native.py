@@ -58,6 +58,10 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_cpml_fused": (ctypes.c_int32, [vp]),
         "xfd_cpml_fused": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32)]),
         "xfd_sim_nf2ff_count": (ctypes.c_int32, [vp]),
+        "xfd_sim_nf2ff_td_count": (ctypes.c_int32, [vp]),
+        "xfd_sim_nf2ff_td_num": (ctypes.c_int64, [vp, i32]),
+        "xfd_sim_nf2ff_td_potential": (ctypes.c_int, [vp, i32, i32, i32, vp]),
+        "xfd_sim_nf2ff_td_far_field": (ctypes.c_int, [vp, i32, vp]),
         "xfd_sim_network_count": (ctypes.c_int32, [vp]),
         "xfd_sim_network_nf": (ctypes.c_int32, [vp, i32]),
         "xfd_sim_network_output": (ctypes.c_int, [vp, i32]),
@@ -195,7 +199,8 @@ class Desc(ctypes.Structure):
                 ("n_tfsf", ctypes.c_int32), ("tfsf", ctypes.POINTER(TfsfDesc)),
                 ("n_gather", ctypes.c_int32), ("gather", ctypes.POINTER(GatherDesc)),
                 ("ade", ctypes.c_void_p), ("n_nf2ff", ctypes.c_int32), ("nf2ff", ctypes.c_void_p),
-                ("n_line", ctypes.c_int32), ("line", ctypes.c_void_p)]
+                ("n_line", ctypes.c_int32), ("line", ctypes.c_void_p),
+                ("n_tdnf", ctypes.c_int32), ("tdnf", ctypes.c_void_p)]
 
 
 def _ptr(a: np.ndarray):
@@ -369,6 +374,26 @@ class Simulation:
     def coef_classed(self) -> bool:
         """True when the device stores coefficient class ids, not 18 arrays."""
         return bool(self._L.xfd_sim_coef_classed(self._h))
+
+    # ------------------------------------------------------- time-domain NF2FF --
+    def nf2ff_td_potential(self, n: int, d: int, which: int) -> np.ndarray:
+        """Potential `which` (0 ua, 1 ub, 2 wa, 3 wb) of surface d after a run."""
+        num = int(self._L.xfd_sim_nf2ff_td_num(self._h, n))
+        if num < 0:
+            raise IndexError("bad time-domain NF2FF index")
+        out = np.zeros(num)
+        self._check(self._L.xfd_sim_nf2ff_td_potential(self._h, n, d, which, out.ctypes.data))
+        return out
+
+    def nf2ff_td_far_field(self, n: int = 0) -> np.ndarray:
+        """processFarField: rows w_theta, w_phi, u_theta, u_phi (and the .npy
+        files when the NF2FF has an output directory)."""
+        num = int(self._L.xfd_sim_nf2ff_td_num(self._h, n))
+        if num < 0:
+            raise IndexError("bad time-domain NF2FF index")
+        out = np.zeros((4, num))
+        self._check(self._L.xfd_sim_nf2ff_td_far_field(self._h, n, out.ctypes.data))
+        return out
 
     # -------------------------------------------------------------- networks --
     def network_count(self) -> int:
