@@ -376,6 +376,49 @@ def _dispersive_run(spec, steps, label, wp):
     return res
 
 
+def bench_nf2ff_td(args):
+    """Time-domain NF2FF on a 256^3 box (10-cell CPML, radiating dipole),
+    surfaces 15 cells inside the grid, one oblique direction: the step with
+    and without it (FP64 and FP32), the NF2FF kernels' share from the HIP
+    event timers, and the FP32 far field against FP64."""
+    from xfdtd_core_amd import native, scenarios
+
+    steps = args.td_steps
+    res = {"workload": "time-domain NF2FF: 256^3 vacuum box, 10-cell CPML, Ez dipole, surfaces 15 cells inside, "
+                       "theta 60 deg phi 45 deg", "steps": steps}
+    far = {}
+    for prec in ("f64", "f32"):
+        row = {}
+        for label, td in (("with", (15, np.pi / 3, np.pi / 4)), ("without", None)):
+            sim = native.Simulation(scenarios.radiating_dipole(td_nf2ff=td).spec(), prec)
+            sim.init(steps)
+            sim.set_profiling(True)
+            native.device_synchronize()
+            t0 = time.perf_counter()
+            sim.advance(0, steps)
+            native.device_synchronize()
+            wall = time.perf_counter() - t0
+            sim.finish()
+            stats = sim.kernel_stats()
+            launches, ms = stats.get("nf2ff", (0, 0.0))
+            row[label] = {"ms_per_step": round(wall / steps * 1e3, 4),
+                          "nf2ff_ms_per_step": round(ms / steps, 5) if launches else 0.0}
+            if td is not None:
+                far[prec] = sim.nf2ff_td_far_field(0)
+            sim.close()
+        row["surface_cells"] = int(6 * (256 - 30) ** 2)
+        res[prec] = row
+        log(f"nf2ff_td {prec}: {row['with']['ms_per_step']:.3f} ms/step with, {row['without']['ms_per_step']:.3f} "
+            f"without; NF2FF kernels {row['with']['nf2ff_ms_per_step'] * 1e3:.1f} us/step")
+    eta = 376.730313668
+    a, b = far["f32"], far["f64"]
+    peak = float(np.abs(eta * b[0] + b[3]).max())
+    err = max(float(np.abs(a[w] - b[w]).max()) * (eta if w < 2 else 1.0) for w in range(4)) / peak
+    res["fp32_far_field_error_of_copolar_peak"] = err
+    log(f"nf2ff_td fp32 far-field error {err:.2e} of the co-polarised peak")
+    return res
+
+
 def bench_adjoint(args):
     """Adjoint inverse design at full size: FP32 forward with checkpoints
     every 100 steps and the reverse-time adjoint; optionally the FP64 adjoint
@@ -436,9 +479,10 @@ def main():
     ap.add_argument("--precision", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint", "dispersive"], default="box",
+    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint", "dispersive", "nf2ff_td"], default="box",
                     help="box: the 384^3 headline; design_sweep: the batched form; adjoint: the gradient form; "
-                         "dispersive: the 256^3 ADE + lumped + NF2FF box")
+                         "dispersive: the 256^3 ADE + lumped + NF2FF box; nf2ff_td: the time-domain NF2FF cost")
+    ap.add_argument("--td-steps", type=int, default=1000)
     ap.add_argument("--dispersive-steps", type=int, default=3000)
     ap.add_argument("--cases", type=int, default=96)
     ap.add_argument("--sequential", type=int, default=24,
@@ -460,6 +504,9 @@ def main():
         return
     if args.workload == "dispersive":
         print(json.dumps(bench_dispersive(args)), flush=True)
+        return
+    if args.workload == "nf2ff_td":
+        print(json.dumps(bench_nf2ff_td(args)), flush=True)
         return
     if args.workload == "adjoint":
         print(json.dumps({"workload": "adjoint inverse design 160^3", **bench_adjoint(args)}), flush=True)

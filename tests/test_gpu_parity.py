@@ -509,16 +509,22 @@ def test_network_s_parameters(tmp_path, precision):
     assert np.abs(want[11]).max() > 0.1 and np.abs(want[21]).max() > 1e-4  # |S21| ~ 1e-3 here
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_time_domain_nf2ff(tmp_path, precision):
+@pytest.mark.parametrize("precision,merge", [("f64", False), ("f64", True), ("f32", False)])
+def test_time_domain_nf2ff(tmp_path, monkeypatch, precision, merge):
     """Time-domain NF2FF on the device (per-cell bins, then each bin summed
-    in the reference's cell order): every surface potential and the far
-    field equal the oracle's in FP64; FP32 within 1e-4 of the far field's
-    co-polarised peak.  The far-field .npy files carry the same values."""
+    in the reference's cell order: fixed runs, or with XFD_TDNF_MERGE=1 the
+    merge of the delay buckets that the general case needs): every surface
+    potential and the far field equal the oracle's in FP64; FP32 within 1e-4
+    of the far field's co-polarised peak.  The far-field .npy files carry
+    the same values."""
     from test_oracle_nf2ff_td import STEPS, dipole_td
 
+    if merge:
+        monkeypatch.setenv("XFD_TDNF_MERGE", "1")
+    else:
+        monkeypatch.delenv("XFD_TDNF_MERGE", raising=False)
     theta, phi = np.pi / 3, 1.1
-    say(f"[gpu] time-domain NF2FF {precision}")
+    say(f"[gpu] time-domain NF2FF {precision}{' merging bins' if merge else ''}")
     ref = RefSim(dipole_td(theta, phi))
     ref.run(STEPS)
     out = tmp_path / precision

@@ -255,6 +255,7 @@ struct xfd_ctx {
     xfd::TdNf2ffArgs a{};
     int64_t omax = 0;  // largest bucket over the groups (bins reach t + omax + 2)
     DevBuf s, ds, bstart, order, prev_a, prev_b, nn, ca1, ca2, cb1, cb2, pot;
+    DevBuf slot, lstart, va, vb;  // fixed-order runs
   };
   std::vector<std::unique_ptr<TdNf2ff>> tdnfs;
   int ade_scheme = -1;
@@ -323,7 +324,7 @@ struct xfd_ctx {
     for (auto& n : nf2ffs) b += n->te.bytes + n->th.bytes + n->acc.bytes;
     for (auto& n : tdnfs)
       for (DevBuf* x : {&n->s, &n->ds, &n->bstart, &n->order, &n->prev_a, &n->prev_b, &n->nn, &n->ca1, &n->ca2,
-                        &n->cb1, &n->cb2, &n->pot})
+                        &n->cb1, &n->cb2, &n->pot, &n->slot, &n->lstart, &n->va, &n->vb})
         b += x->bytes;
     for (int q = 0; q < 3; ++q) b += ade_j[q].bytes + ade_jp[q].bytes + ade_eprev[q].bytes;
     for (auto& gth : gathers) b += gth->out.bytes;
@@ -816,8 +817,18 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       }
       src += cells[s6];
     }
+    // fixed order: no s + t of the run (t < num) can round up to the next
+    // integer when 1 - frac(s) exceeds an ulp of s + num, so every cell's
+    // bin is floor(s) + t
+    double smax = 0;
+    for (double v : s) smax = std::max(smax, v);
+    const double ulp = std::ldexp(1.0, std::ilogb(smax + static_cast<double>(N.num)) - 52);
+    bool fixed = std::getenv("XFD_TDNF_MERGE") == nullptr;  // =1: always the merging bins
+    for (double v : s) fixed = fixed && (std::floor(v) + 1 - v) > 2 * ulp;
+    A.fixed = fixed ? 1 : 0;
     // delay buckets floor(s), cells sorted by (bucket, cell) per group
     std::vector<int64_t> bstart, order;
+    std::vector<int64_t> slot(2 * A.off[12], 0), lstart;  // two per group cell
     A.boff[0] = 0;
     A.bin_off[0] = 0;
     for (int gr = 0; gr < 12; ++gr) {
@@ -843,9 +854,32 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       order.resize(base + nc);
       std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
       for (int64_t q = 0; q < nc; ++q) order[base + fill[o[q] - omin]++] = q;  // ascending q within a bucket
-      A.bin_off[gr + 1] = A.bin_off[gr] + (nc ? nb + 2 : 0);
+      // fixed-order runs: bin L merges bucket L (first contributions) and
+      // bucket L - 1 (second) by cell
+      A.loff[gr] = static_cast<int64_t>(lstart.size());
+      int64_t pos = 2 * c0;
+      for (int64_t L = 0; nc && L <= nb; ++L) {
+        lstart.push_back(pos);
+        int64_t p1 = L < nb ? bstart[A.boff[gr] + L] : 0, e1 = L < nb ? bstart[A.boff[gr] + L + 1] : 0;
+        int64_t p2 = L > 0 ? bstart[A.boff[gr] + L - 1] : 0, e2 = L > 0 ? bstart[A.boff[gr] + L] : 0;
+        while (p1 < e1 || p2 < e2) {
+          const bool first = p2 >= e2 || (p1 < e1 && order[p1] < order[p2]);
+          const int64_t q = first ? order[p1++] : order[p2++];
+          slot[2 * (c0 + q) + (first ? 0 : 1)] = pos++;
+        }
+      }
+      if (nc) lstart.push_back(pos);
+      A.bin_off[gr + 1] = A.bin_off[gr] + (nc ? nb + (fixed ? 1 : 2) : 0);
     }
     A.num = N.num;
+    uploadVector(slot, td->slot);
+    uploadVector(lstart.empty() ? std::vector<int64_t>{0} : lstart, td->lstart);
+    td->va.alloc(static_cast<size_t>(std::max<int64_t>(2 * A.off[12], 1)) * sizeof(double));
+    td->vb.alloc(static_cast<size_t>(std::max<int64_t>(2 * A.off[12], 1)) * sizeof(double));
+    A.slot = td->slot.as<int64_t>();
+    A.lstart = td->lstart.as<int64_t>();
+    A.va = td->va.as<double>();
+    A.vb = td->vb.as<double>();
     uploadVector(s, td->s);
     uploadVector(ds, td->ds);
     uploadVector(bstart, td->bstart);

@@ -879,11 +879,23 @@ __device__ __forceinline__ void tdCell(const Grid& g, const FieldPtrs<T>& F, con
   const int64_t nn = static_cast<int64_t>(floor(v));
   const double coeff_f = v - static_cast<double>(nn);
   const double ds = a.ds[cell];
-  a.nn[cell] = nn;
-  a.ca1[cell] = coeff_a * (1 - coeff_f) * ds * delta_b;
-  a.ca2[cell] = coeff_a * coeff_f * ds * delta_b;
-  a.cb1[cell] = coeff_b * (1 - coeff_f) * ds * delta_a;
-  a.cb2[cell] = coeff_b * coeff_f * ds * delta_a;
+  const double ca1 = coeff_a * (1 - coeff_f) * ds * delta_b;
+  const double ca2 = coeff_a * coeff_f * ds * delta_b;
+  const double cb1 = coeff_b * (1 - coeff_f) * ds * delta_a;
+  const double cb2 = coeff_b * coeff_f * ds * delta_a;
+  if (a.fixed) {
+    const int64_t p1 = a.slot[2 * cell], p2 = a.slot[2 * cell + 1];
+    a.va[p1] = ca1;
+    a.va[p2] = ca2;
+    a.vb[p1] = cb1;
+    a.vb[p2] = cb2;
+  } else {
+    a.nn[cell] = nn;
+    a.ca1[cell] = ca1;
+    a.ca2[cell] = ca2;
+    a.cb1[cell] = cb1;
+    a.cb2[cell] = cb2;
+  }
 }
 
 template <class T>
@@ -901,6 +913,20 @@ __device__ __forceinline__ void tdBin(const TdNf2ffArgs& a, int64_t id, int64_t 
   const int64_t local = id - a.bin_off[gr];
   const int64_t n = t + a.omin[gr] + local;
   if (n < 0 || n >= a.num) return;
+  if (a.fixed) {  // this bin's contributions, already in order
+    const int d = gr >> 1;
+    double* pa = a.pot + (static_cast<int64_t>(d) * 4 + ((gr & 1) ? 2 : 0)) * a.num;
+    double* pb = pa + a.num;
+    double va = pa[n], vb = pb[n];
+    const int64_t q0 = a.lstart[a.loff[gr] + local], q1 = a.lstart[a.loff[gr] + local + 1];
+    for (int64_t q = q0; q < q1; ++q) {
+      va += a.va[q];
+      vb += a.vb[q];
+    }
+    pa[n] = va;
+    pb[n] = vb;
+    return;
+  }
   const int64_t* bs = a.bstart + a.boff[gr];
   const int64_t nb = a.nbk[gr];
   // buckets local - 2, local - 1, local (clamped to the group's range)
@@ -947,6 +973,55 @@ __device__ __forceinline__ void tdBin(const TdNf2ffArgs& a, int64_t id, int64_t 
 template <class T>  // one instance per precision's translation unit
 __global__ __launch_bounds__(256) void k_td_bins(TdNf2ffArgs a, int64_t t) {
   tdBin(a, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, t);
+}
+
+// Lane l's value of v (l a compile-time lane index after unrolling): two
+// v_readlane of the 32-bit halves.
+__device__ __forceinline__ double laneValue(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffull), l));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), l));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// The fixed-order bins, one 64-lane wavefront per bin: the lanes read 64
+// consecutive contributions of the bin's run at a time (one coalesced load)
+// and the wavefront adds them in lane order, so the sum is the same
+// sequence of additions as one thread walking the run.  Lanes past the run
+// hold +0.0: a running sum that starts at +0.0 never becomes -0.0 (round to
+// nearest), so adding +0.0 leaves it bitwise unchanged.
+__device__ __forceinline__ void tdBinWave(const TdNf2ffArgs& a, int64_t id, int lane, int64_t t) {
+  if (id >= a.bin_off[12]) return;  // uniform over the wavefront
+  int gr = 0;
+  while (id >= a.bin_off[gr + 1]) ++gr;
+  const int64_t local = id - a.bin_off[gr];
+  const int64_t n = t + a.omin[gr] + local;
+  if (n < 0 || n >= a.num) return;
+  const int d = gr >> 1;
+  double* pa = a.pot + (static_cast<int64_t>(d) * 4 + ((gr & 1) ? 2 : 0)) * a.num;
+  double* pb = pa + a.num;
+  double va = pa[n], vb = pb[n];
+  const int64_t q0 = a.lstart[a.loff[gr] + local], q1 = a.lstart[a.loff[gr] + local + 1];
+  for (int64_t q = q0; q < q1; q += 64) {
+    const bool in = q + lane < q1;
+    const double xa = in ? a.va[q + lane] : 0.0;
+    const double xb = in ? a.vb[q + lane] : 0.0;
+#pragma unroll
+    for (int l = 0; l < 64; ++l) {
+      va += laneValue(xa, l);
+      vb += laneValue(xb, l);
+    }
+  }
+  if (lane == 0) {
+    pa[n] = va;
+    pb[n] = vb;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_td_bins_wave(TdNf2ffArgs a, int64_t t) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  tdBinWave(a, g / 64, static_cast<int>(g % 64), t);
 }
 
 // Host-side launchers, instantiated once per precision.

@@ -120,7 +120,10 @@ template <class T>
 void Launch<T>::tdNf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const TdNf2ffArgs& a, int64_t t) {
   if (a.off[12] <= 0) return;
   hipLaunchKernelGGL(k_td_cells<T>, dim3(blocks_for(a.off[12], 256)), dim3(256), 0, s, g, F, a, t);
-  hipLaunchKernelGGL(k_td_bins<T>, dim3(blocks_for(a.bin_off[12], 256)), dim3(256), 0, s, a, t);
+  if (a.fixed)
+    hipLaunchKernelGGL(k_td_bins_wave<T>, dim3(blocks_for(a.bin_off[12] * 64, 256)), dim3(256), 0, s, a, t);
+  else
+    hipLaunchKernelGGL(k_td_bins<T>, dim3(blocks_for(a.bin_off[12], 256)), dim3(256), 0, s, a, t);
 }
 
 template <class T>

@@ -164,6 +164,14 @@ struct Nf2ffArgs {
 // then cell; group g's buckets at bstart + boff[g], its nbk[g] buckets
 // starting at omin[g]); bin n of step t draws on buckets n - t - 2 .. n - t,
 // merged in cell order, so every bin sums in the reference's order.
+// When no s + t of the run can round up to the next integer (`fixed`, the
+// usual case; the host checks it), a cell's bin is exactly floor(s) + t, so
+// the order of the contributions to every bin is the same each step: bin L
+// of group g (L = n - t - omin[g], 0..nbk[g]) takes bucket L's first and
+// bucket L - 1's second contributions merged by cell.  `slot` gives each
+// cell's two positions in that order (va / vb, one run per bin starting at
+// lstart[loff[g] + L]); k_td_cells writes there and k_td_bins streams each
+// run, nbk[g] + 1 bins per group and step.
 // pot holds per surface d: ua, ub, wa, wb of num values (FP64).
 struct TdNf2ffArgs {
   int lo[6][3], ext[6][3];
@@ -185,6 +193,12 @@ struct TdNf2ffArgs {
   double* cb1;
   double* cb2;
   double* pot;
+  int fixed;
+  const int64_t* slot;    // 2 per cell: first, second contribution
+  const int64_t* lstart;  // per group, nbk + 2 run starts
+  int64_t loff[12];
+  double* va;
+  double* vb;
 };
 
 // One CPML slab for one half-step.  The thread space is the union box of the

@@ -155,6 +155,29 @@ def dispersive_slabs(n_total: int = 256, pml: int = 10, dl: float = 0.5e-3, slab
     return s
 
 
+def radiating_dipole(n_total: int = 256, pml: int = 10, dl: float = 1e-3, td_nf2ff=(15, np.pi / 3, np.pi / 4),
+                     out_dir: str = ""):
+    """A one-edge Ez current source (derivative-of-Gaussian, tau = 20 dl /
+    (2 c)) at the centre of an n_total^3 vacuum box including a `pml`-cell
+    CPML, CFL 0.9; with `td_nf2ff` = (distance, theta, phi) a time-domain
+    NF2FF `distance` cells inside the grid along that direction."""
+    n_in = n_total - 2 * pml
+    half = n_in * dl / 2
+    s = S.Simulation(dl, dl, dl, 0.9)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-half, -half, -half), S.Vector(2 * half, 2 * half, 2 * half)),
+                         S.Material.createAir()))
+    c0 = 299792458.0
+    tau = 20 * dl / (2 * c0)
+    s.addObject(S.CurrentSource("src", S.Cube(S.Vector(0.0, 0.0, 0.0), S.Vector(0.0, 0.0, dl)), "ZP", 1e20,
+                                S.Waveform.gaussianDerivative(tau, 4.5 * tau, 1.0)))
+    for d in ("XN", "XP", "YN", "YP", "ZN", "ZP"):
+        s.addBoundary(S.PML(pml, d))
+    if td_nf2ff is not None:
+        dist, theta, phi = td_nf2ff
+        s.addNF2FF(S.NFFFTTimeDomain(dist, dist, dist, out_dir, theta, phi))
+    return s
+
+
 def design_sweep(n_cases: int = 96, sizes=(48, 64, 96, 128, 160), pml: int = 8, dl: float = 1e-3,
                  side=(0.2, 0.5), eps_r=(1.0, 8.0), freqs=(2.5e9, 5e9, 7.5e9, 1e10), seed: int = 2030,
                  cfl: float = 0.9):
