@@ -248,6 +248,11 @@ int xfd_nf2ff_far_field(xfd_ctx* ctx, int32_t n, int32_t f, int32_t na, const do
  * xfd_step records an event pair around every launch; xfd_kernel_stats then
  * reports, for kernel class `k`, its name, launch count and summed time. */
 int xfd_set_profiling(xfd_ctx* ctx, int32_t on);
+/* The same for the kernel classes whose bit is set in `mask` (bit k: class
+ * k of xfd_kernel_stats) -- an event pair costs about 10 us of idle GPU time
+ * between launches, so timing only the class of interest keeps the step
+ * close to its unprofiled time.  No reference counterpart. */
+int xfd_set_profiling_classes(xfd_ctx* ctx, uint32_t mask);
 int xfd_kernel_stats(xfd_ctx* ctx, int32_t k, const char** name, int64_t* launches,
                      double* total_ms);
 

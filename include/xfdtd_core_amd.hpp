@@ -687,6 +687,8 @@ class Simulation {
     double total_ms;
   };
   void setProfiling(bool on);
+  // Times only the kernel classes whose bit is set (bit k: kernelStats()[k]).
+  void setProfilingClasses(unsigned mask);
   std::vector<KernelStat> kernelStats() const;
   long long deviceBytes() const;
   // True when the device holds per-cell coefficient class ids instead of the

@@ -25,6 +25,8 @@ void xfd_sim_destroy(void* sim);
 /* 0 = FP64 (default), 1 = FP32 fields and updates. */
 int xfd_sim_set_precision(void* sim, int32_t precision);
 int xfd_sim_set_profiling(void* sim, int32_t on);
+/* Times only the kernel classes whose bit is set (xfd_set_profiling_classes). */
+int xfd_sim_set_profiling_classes(void* sim, uint32_t mask);
 /* Host precompute only (no GPU): Simulation::init without the device. */
 int xfd_sim_precompute(void* sim, int64_t steps);
 /* Simulation::init(steps): precompute and upload to the GPU. */

@@ -550,3 +550,22 @@ def test_time_domain_nf2ff(tmp_path, monkeypatch, precision, merge):
         assert err < 1e-4, err
     for w, name in enumerate(("w_theta", "w_phi", "u_theta", "u_phi")):
         np.testing.assert_array_equal(np.load(out / f"{name}.npy"), far[w])
+
+
+def test_profiling_selected_kernel_classes():
+    """HIP-event timing can be limited to chosen kernel classes: only those
+    classes report launches, and with every class the counts match the
+    launches of the steps."""
+    sp = scenarios.inhomogeneous_box(n_total=40, pml=6, n_spheres=3, r_cells=(3, 5), seed=3).spec()
+    say("[gpu] profiling selected kernel classes")
+    sim = native.Simulation(sp, "f64")
+    sim.init(40)
+    sim.set_profiling(True, classes=["update_e"])
+    sim.advance(0, 20)
+    st = sim.kernel_stats()
+    assert st["update_e"][0] == 20 and st["update_e"][1] > 0
+    assert all(v[0] == 0 for k, v in st.items() if k != "update_e"), st
+    sim.set_profiling(True)
+    sim.advance(20, 20)
+    st = sim.kernel_stats()
+    assert st["update_e"][0] == 20 and st["update_h"][0] == 20 and st["cpml_e"][0] == 20 * 6, st

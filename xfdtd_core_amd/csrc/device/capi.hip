@@ -293,6 +293,7 @@ struct xfd_ctx {
   int64_t ck_interval = 0, ck_nt = 0, ck_bytes = 0;
 
   bool prof = false;
+  uint32_t prof_mask = 0;  // kernel classes timed when prof
   int64_t launches[K_COUNT] = {};
   double total_ms[K_COUNT] = {};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -346,7 +347,7 @@ struct xfd_ctx {
 
   template <class F>
   void timed(int cls, F&& launch) {
-    if (!prof) {
+    if (!prof || !(prof_mask & (1u << cls))) {
       launch();
       return;
     }
@@ -1828,10 +1829,13 @@ int xfd_read_gather(xfd_ctx* c, int32_t gi, double* dst) {
   });
 }
 
-int xfd_set_profiling(xfd_ctx* c, int32_t on) {
+int xfd_set_profiling(xfd_ctx* c, int32_t on) { return xfd_set_profiling_classes(c, on ? ~0u : 0u); }
+
+int xfd_set_profiling_classes(xfd_ctx* c, uint32_t mask) {
   return guard([&] {
     require(c != nullptr, "null context");
-    c->prof = on != 0;
+    c->prof_mask = mask & ((1u << K_COUNT) - 1);
+    c->prof = c->prof_mask != 0;
     for (int k = 0; k < K_COUNT; ++k) {
       c->launches[k] = 0;
       c->total_ms[k] = 0;

@@ -356,6 +356,10 @@ int xfd_sim_set_precision(void* p, int32_t precision) {
   });
 }
 
+int xfd_sim_set_profiling_classes(void* p, uint32_t mask) {
+  return guard([&] { H(p)->sim->setProfilingClasses(mask); });
+}
+
 int xfd_sim_set_profiling(void* p, int32_t on) {
   return guard([&] { H(p)->sim->setProfiling(on != 0); });
 }

@@ -494,6 +494,7 @@ struct Simulation::Impl {
   std::shared_ptr<EMF> emf = std::make_shared<EMF>();
   double loop_seconds = 0;
   bool profiling = false;
+  uint32_t profiling_classes = ~0u;
   std::vector<KernelStat> stats;
   long long dev_bytes = 0;
   bool coef_classed = false;
@@ -1680,7 +1681,7 @@ struct Simulation::Impl {
     if (!initialised || !ctx) throw XFDTDSimulationException("Simulation is not initialised");
     if (batch_core) throw XFDTDSimulationException("this Simulation is stepped by its SimulationBatch");
     if (t0 + n > steps) throw XFDTDSimulationException("steps beyond the run length given to init()");
-    xfd_set_profiling(ctx, profiling ? 1 : 0);
+    xfd_set_profiling_classes(ctx, profiling ? profiling_classes : 0u);
     const auto start = std::chrono::steady_clock::now();
     if (xfd_step(ctx, static_cast<int64_t>(t0), static_cast<int64_t>(n)) != XFD_OK)
       throw XFDTDSimulationException(std::string("xfd_step: ") + xfd_last_error());
@@ -1970,7 +1971,14 @@ const Array3D<Real>& Simulation::coefficient(const std::string& name) const {
 const std::vector<Real>& Simulation::eNode(int a) const { return _impl->e_node[a]; }
 const std::vector<Real>& Simulation::hSize(int a) const { return _impl->h_size[a]; }
 double Simulation::loopSeconds() const { return _impl->loop_seconds; }
-void Simulation::setProfiling(bool on) { _impl->profiling = on; }
+void Simulation::setProfiling(bool on) {
+  _impl->profiling = on;
+  _impl->profiling_classes = ~0u;
+}
+void Simulation::setProfilingClasses(unsigned mask) {
+  _impl->profiling = mask != 0;
+  _impl->profiling_classes = mask;
+}
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
 bool Simulation::classedCoefficients() const { return _impl->coef_classed; }

@@ -44,6 +44,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_destroy": (None, [vp]),
         "xfd_sim_set_precision": (ctypes.c_int, [vp, i32]),
         "xfd_sim_set_profiling": (ctypes.c_int, [vp, i32]),
+        "xfd_sim_set_profiling_classes": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "xfd_sim_precompute": (ctypes.c_int, [vp, i64]),
         "xfd_sim_init": (ctypes.c_int, [vp, i64]),
         "xfd_sim_run_loop": (ctypes.c_int, [vp]),
@@ -93,6 +94,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_write_field": (ctypes.c_int, [vp, i32, vp]),
         "xfd_read_gather": (ctypes.c_int, [vp, i32, vp]),
         "xfd_set_profiling": (ctypes.c_int, [vp, i32]),
+        "xfd_set_profiling_classes": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "xfd_kernel_stats": (ctypes.c_int, [vp, i32, ctypes.POINTER(cp), p64, ctypes.POINTER(ctypes.c_double)]),
         "xfd_device_bytes": (ctypes.c_int, [vp, p64]),
         "xfd_coef_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), p64]),
@@ -188,6 +190,10 @@ class GatherDesc(ctypes.Structure):
 
     _fields_ = [("field", ctypes.c_int32), ("origin", ctypes.c_int32 * 3), ("end", ctypes.c_int32 * 3),
                 ("first", ctypes.c_int64), ("interval", ctypes.c_int64), ("count", ctypes.c_int64)]
+
+
+# kernel classes of xfd_kernel_stats, in bit order
+KERNEL_CLASSES = ("update_h", "update_e", "cpml_h", "cpml_e", "records_h", "records_e", "gather", "nf2ff")
 
 
 class Desc(ctypes.Structure):
@@ -299,8 +305,16 @@ class Simulation:
             raise ValueError("precision must be 'f64' or 'f32'")
         self._check(self._L.xfd_sim_set_precision(self._h, 1 if precision == "f32" else 0))
 
-    def set_profiling(self, on: bool = True):
-        self._check(self._L.xfd_sim_set_profiling(self._h, int(on)))
+    def set_profiling(self, on: bool = True, classes=None):
+        """HIP-event timing of every kernel class, or only of `classes`
+        (names as in kernel_stats(), e.g. ["update_e"])."""
+        if classes is None:
+            self._check(self._L.xfd_sim_set_profiling(self._h, int(on)))
+            return
+        mask = 0
+        for c in classes:
+            mask |= 1 << KERNEL_CLASSES.index(c)
+        self._check(self._L.xfd_sim_set_profiling_classes(self._h, mask if on else 0))
 
     def precompute(self, steps: int):
         self._check(self._L.xfd_sim_precompute(self._h, steps))
