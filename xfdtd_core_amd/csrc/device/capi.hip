@@ -974,8 +974,9 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   }
   {  // CPML inside the sweeps when XFD_CPML_FUSED=1, without ADE or records in a slab.
     // Off by default: measured on MI355X the separate slab launches are faster
-    // (384^3 FP32 step 1.51 ms separate vs 1.83 ms fused), because the fused
-    // sweep reads the slab parameters per cell on the slab tiles.
+    // (384^3 step FP32 1.192 ms vs 1.454 ms fused, FP64 2.246 vs 2.352 ms),
+    // because the fused sweep reads the slab parameters per cell on the slab
+    // tiles.
     const char* env = std::getenv("XFD_CPML_FUSED");
     const bool off = !(env && env[0] == '1');
     // Only x and y slabs: their membership is uniform over a wavefront (one i,
@@ -1004,8 +1005,8 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     // lines, while inside the sweep the field and dual values are already in
     // registers and only psi and c_psi are read.  The slab lanes' extra work
     // slows the wavefronts that carry them, so it pays only when few do:
-    // measured, the 384^3 box (6 k-tiles per row) steps 1.363 ms with it and
-    // 1.401 ms without, while the batched design sweep (nz 48-160, 1-3
+    // measured, the 384^3 box (6 k-tiles per row) steps 1.192 ms with it and
+    // 1.244 ms without, while the batched design sweep (nz 48-160, 1-3
     // k-tiles) steps 3.66 ms with it and 3.41 ms without.  Default: rows of at
     // least 4 k-tiles; XFD_CPML_ZFUSED=1 forces it, =0 turns it off.  The
     // corrections then precede the x/y slabs' instead of following them,
