@@ -271,11 +271,13 @@ int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
  *   *fused = 0: none, every slab is a separate launch;
  *   *fused = 1: the leading x/y slabs (same operations in the same order, so
  *     FP64 stays bitwise); opt-in with XFD_CPML_FUSED=1 (slower on MI355X);
- *   *fused = 2: the z slabs, FP32 contexts only: by default when a k-row
- *     spans at least 4 sweep tiles (256 cells), where it is faster;
- *     XFD_CPML_ZFUSED=1 forces it, =0 turns it off.  Their corrections then
- *     precede the x/y slabs', which changes only the FP32 rounding where
- *     slabs overlap.
+ *   *fused = 2: the z slabs.  FP32: by default when a k-row spans at
+ *     least 4 sweep tiles (256 cells), where it is faster; the z
+ *     corrections then precede the x/y slabs' where they overlap (FP32
+ *     rounding only).  FP64: only when forced; the sweeps correct only the
+ *     points in no x or y slab, and the frame where slabs overlap follows
+ *     the x/y slabs in one launch, so the reference's order holds (bitwise).
+ *     XFD_CPML_ZFUSED=1 forces it, =0 turns it off.
  * Never with ADE materials or when a TFSF or lumped correction lands inside
  * a slab (the reference applies those first). */
 int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);

@@ -695,8 +695,8 @@ class Simulation {
   // 18 dense coefficient arrays (same values; chosen at init).
   bool classedCoefficients() const;
   // Which CPML slabs the Yee sweeps apply in the same pass: 0 none, 1 the
-  // leading x/y slabs (XFD_CPML_FUSED=1), 2 the z slabs (FP32, nz > 192 or
-  // XFD_CPML_ZFUSED=1).
+  // leading x/y slabs (XFD_CPML_FUSED=1), 2 the z slabs (FP32 when nz > 192,
+  // either precision with XFD_CPML_ZFUSED=1).
   int fusedCpml() const;
 
   // ---- Adjoint (no reference counterpart) ----

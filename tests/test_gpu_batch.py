@@ -108,14 +108,14 @@ def test_batch_matches_single_cases_fp64(monkeypatch, fused):
 def test_batch_fp32_accuracy_matches_single_cases(monkeypatch, zfused):
     """FP32: the batched and the single-context sweeps differ only in the
     compiler's contraction choices, so the batch is as close to the FP64 run
-    as each case run alone.  Also with the z CPML slabs forced inside the
-    sweeps (XFD_CPML_ZFUSED=1; these small cases do not take it by default)."""
+    as each case run alone.  Batched cases keep their z CPML slabs separate
+    even when XFD_CPML_ZFUSED=1 asks single contexts to fuse them."""
     if zfused == "1":
         monkeypatch.setenv("XFD_CPML_ZFUSED", "1")
     else:
         monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
     specs = _cases()
-    say(f"[gpu] batch vs single cases f32 (z CPML in sweeps: {zfused})")
+    say(f"[gpu] batch vs single cases f32 (XFD_CPML_ZFUSED: {zfused})")
     ref = _alone(specs, "f64")
     single = _alone(specs, "f32")
     sims, b, got, info = _batched(specs, "f32")
@@ -125,7 +125,7 @@ def test_batch_fp32_accuracy_matches_single_cases(monkeypatch, zfused):
         say(f"  case {i}: batch-vs-f64 {eb:.2e}, single-vs-f64 {es:.2e}, batch-vs-single {ebs:.2e}")
         assert eb <= 2 * es + 1e-6
         assert ebs <= 1e-4
-        assert es <= 1e-4  # pins the z-CPML-in-sweep runs to FP64 as well
+        assert es <= 1e-4
 
 
 def test_batch_fp64_against_oracle():

@@ -345,55 +345,62 @@ def _tall_box(nz_total, pml=8, n_xy=24):
     return s.spec()
 
 
-def test_z_cpml_inside_fp32_sweeps(monkeypatch):
-    """FP32 contexts apply the z CPML slabs inside the Yee sweeps when a
-    k-row spans at least 4 sweep tiles (or when forced; only the far grid
-    planes stay separate launches).  Forced on a 48^3 box, where one
-    wavefront holds lanes of both z slabs: against the all-separate FP32 run
-    the fields differ by rounding only, and both are as close to the FP64 run
-    (bitwise equal to the oracle), in the whole box and in the z boundary
-    layers alone.  FP64 contexts never do it, and contexts with ADE keep
-    separate slabs."""
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_z_cpml_inside_sweeps(monkeypatch, precision):
+    """The sweeps apply the z CPML slabs: FP32 by default when a k-row spans
+    at least 4 sweep tiles, the whole slab; FP64 only when forced, and only
+    where no x/y slab corrects the point (the frame follows the x/y slabs in
+    a launch of its own, keeping the reference's order).  Forced on a 48^3
+    box, where one wavefront holds lanes of both z slabs: FP64 equals the
+    all-separate run and the oracle bitwise; FP32 is as close to FP64 as the
+    separate run.  Contexts with ADE keep separate slabs."""
     sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
-    say("[gpu] z CPML inside the FP32 sweeps vs separate")
+    say(f"[gpu] z CPML inside the sweeps vs separate {precision}")
     monkeypatch.delenv("XFD_CPML_FUSED", raising=False)
     monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
-    for nz, want in ((192, 0), (193, 2)):  # 3 and 4 tiles of 64 k-lanes
-        tall = native.Simulation(_tall_box(nz), "f32")
+    for nz, want in ((192, 0), (193, 2 if precision == "f32" else 0)):  # 3 and 4 tiles of 64 k-lanes
+        tall = native.Simulation(_tall_box(nz), precision)
         tall.init(2)
         assert tall.cpml_fused() == want, (nz, tall.cpml_fused())
         tall.close()
-    auto = native.Simulation(sp, "f32")
+    auto = native.Simulation(sp, precision)
     auto.init(2)
     assert auto.cpml_fused() == 0
     auto.close()
     monkeypatch.setenv("XFD_CPML_ZFUSED", "1")
-    zf = native.Simulation(sp, "f32")
+    zf = native.Simulation(sp, precision)
     zf.run(200)
     assert zf.cpml_fused() == 2
-    ref = native.Simulation(sp, "f64")
-    ref.run(200)
-    assert ref.cpml_fused() == 0
     monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
-    sep = native.Simulation(sp, "f32")
+    sep = native.Simulation(sp, precision)
     sep.run(200)
     assert sep.cpml_fused() == 0
+    if precision == "f64":
+        ref = RefSim(sp)
+        ref.run(200)
+        for f in FIELDS:
+            np.testing.assert_array_equal(zf.field(f), sep.field(f), err_msg=f)
+            np.testing.assert_array_equal(zf.field(f), ref.field(f), err_msg=f)
+    else:
+        monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
+        ref = native.Simulation(sp, "f64")
+        ref.run(200)
 
-    def layers(x):  # the z slabs' 8 layers at both ends
-        return np.concatenate([x[:, :, :9].ravel(), x[:, :, -9:].ravel()])
+        def layers(x):  # the z slabs' 8 layers at both ends
+            return np.concatenate([x[:, :, :9].ravel(), x[:, :, -9:].ravel()])
 
-    for f in FIELDS:
-        a, b, r = zf.field(f), sep.field(f), ref.field(f)
-        d, e_sep, e_zf = rel_l2(a, b), rel_l2(b, r), rel_l2(a, r)
-        l_sep, l_zf = rel_l2(layers(b), layers(r)), rel_l2(layers(a), layers(r))
-        say(f"  {f}: z-fused vs separate {d:.2e}; vs f64: separate {e_sep:.2e}, z-fused {e_zf:.2e}; "
-            f"z layers: separate {l_sep:.2e}, z-fused {l_zf:.2e}")
-        assert d < 1e-4, (f, d)
-        assert e_zf <= 2 * e_sep + 1e-6, (f, e_zf, e_sep)
-        assert l_zf <= 2 * l_sep + 1e-6, (f, l_zf, l_sep)
+        for f in FIELDS:
+            a, b, r = zf.field(f), sep.field(f), ref.field(f)
+            d, e_sep, e_zf = rel_l2(a, b), rel_l2(b, r), rel_l2(a, r)
+            l_sep, l_zf = rel_l2(layers(b), layers(r)), rel_l2(layers(a), layers(r))
+            say(f"  {f}: z-fused vs separate {d:.2e}; vs f64: separate {e_sep:.2e}, z-fused {e_zf:.2e}; "
+                f"z layers: separate {l_sep:.2e}, z-fused {l_zf:.2e}")
+            assert d < 1e-4, (f, d)
+            assert e_zf <= 2 * e_sep + 1e-6, (f, e_zf, e_sep)
+            assert l_zf <= 2 * l_sep + 1e-6, (f, l_zf, l_sep)
     monkeypatch.setenv("XFD_CPML_ZFUSED", "1")
     disp = native.Simulation(scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9,
-                                                        lumped=False).spec(), "f32")
+                                                        lumped=False).spec(), precision)
     disp.init(2)
     assert disp.cpml_fused() == 0
 

@@ -40,7 +40,6 @@ struct BatchCase {
   FieldPtrs<T> F;
   ClassCoef<T> K;
   CpmlList<T> ph, pe;  // slabs applied inside the sweeps (empty: separate CPML launches)
-  ZCpml<T> zh, ze;     // z slabs applied inside the sweeps (n = 0: separate launches)
   int nbx, nby, nbz, ichunk;
 };
 
@@ -68,7 +67,7 @@ struct BatchNf2ff {
   Nf2ffArgs a;
 };
 
-template <class T, bool IsH, bool Fused, bool ZF>
+template <class T, bool IsH, bool Fused>
 __global__ __launch_bounds__(kTileThreads) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
                                                      const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
@@ -80,9 +79,9 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_sweep(const BatchCase<T>
     const int by = static_cast<int>(r % C.nby);
     const int bz = static_cast<int>(r / C.nby);
     if (IsH)
-      sweepH<T, ClassCoef<T>, Fused, ZF>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph, C.zh);
+      sweepH<T, ClassCoef<T>, Fused>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
     else
-      sweepE<T, ClassCoef<T>, Fused, ZF>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe, C.ze);
+      sweepE<T, ClassCoef<T>, Fused>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
   }
 }
 
@@ -182,10 +181,9 @@ __global__ void k_tick(int64_t* t) {
 
 template <class T>
 struct BatchLaunch {
-  // mode 1: some case applies leading CPML slabs inside its sweeps (ph / pe);
-  // mode 2: some case applies its z slabs inside its sweeps (zh / ze)
+  // fused: some case applies leading CPML slabs inside its sweeps (ph / pe)
   static void sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
-                    int grid, int mode);
+                    int grid, bool fused);
   static void cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n, int64_t total,
                    int grid);
   static void cpmlFaces(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,

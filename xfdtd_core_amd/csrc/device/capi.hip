@@ -283,7 +283,7 @@ struct xfd_ctx {
   // of each half-step as device arrays
   bool fused = false;
   int n_fused = 0;  // slabs [0, n_fused) run inside the sweeps, the rest separately
-  // FP32: the z slabs run inside the sweeps (zApply), the rest separately
+  // the z slabs run inside the sweeps (ZCpml), the rest separately
   bool zfused = false;
   std::vector<char> zslab;  // per slab: applied inside the sweeps
   DevBuf cpml_list_e, cpml_list_h;
@@ -1000,19 +1000,23 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       uploadVector(le, c.cpml_list_e);
       uploadVector(lh, c.cpml_list_h);
     }
-    // FP32: the z slabs inside the sweeps.  A z slab holds 8-12 cells of each
+    // The z slabs inside the sweeps.  A z slab holds 8-12 cells of each
     // 128-byte k-row, so a separate launch moves 2-2.5x its bytes in partial
     // lines, while inside the sweep the field and dual values are already in
     // registers and only psi and c_psi are read.  The slab lanes' extra work
     // slows the wavefronts that carry them, so it pays only when few do:
     // measured, the 384^3 box (6 k-tiles per row) steps 1.192 ms with it and
     // 1.244 ms without, while the batched design sweep (nz 48-160, 1-3
-    // k-tiles) steps 3.66 ms with it and 3.41 ms without.  Default: rows of at
-    // least 4 k-tiles; XFD_CPML_ZFUSED=1 forces it, =0 turns it off.  The
-    // corrections then precede the x/y slabs' instead of following them,
-    // which changes only the rounding where slabs overlap: FP64 keeps the
-    // reference's order (and its bitwise results).  Never together with
-    // XFD_CPML_FUSED=1: a batched sweep applies one kind.
+    // k-tiles) steps 3.66 ms with it and 3.41 ms without.  FP32 applies the
+    // whole slab in the sweep (its corrections then precede the x/y slabs'
+    // where they overlap: FP32 rounding only); default on rows of at least 4
+    // k-tiles.  FP64 keeps the reference's order of corrections: the sweeps
+    // apply a z slab only where no x/y slab corrects the point, and the frame
+    // where they overlap follows the x/y slabs at the z slab's place
+    // (k_cpml_zrest), bitwise; the frame's launch costs more than the fusion
+    // saves (384^3: 2.40 ms against 2.26 ms), so FP64 takes it only when
+    // forced.  XFD_CPML_ZFUSED=1 forces it, =0 turns it off.  Never together
+    // with XFD_CPML_FUSED=1.
     const char* zenv = std::getenv("XFD_CPML_ZFUSED");
     const bool zoff = zenv && zenv[0] == '0';
     const bool zforce = zenv && zenv[0] == '1';
@@ -1027,8 +1031,11 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       disjoint = (s0.e_start + s0.n <= s1.e_start || s1.e_start + s1.n <= s0.e_start) &&
                  (s0.h_start + s0.n <= s1.h_start || s1.h_start + s1.n <= s0.h_start);
     }
-    c.zfused = std::is_same<T, float>::value && !zoff && (zforce || zwide) && off && !zq.empty() &&
-               zq.size() <= 2 && disjoint && c.ade_scheme < 0 && !records_meet_slabs;
+    int per_axis[2] = {0, 0};
+    for (auto& sl : c.slabs)
+      if (sl->axis < 2) ++per_axis[sl->axis];
+    c.zfused = !zoff && (zforce || (zwide && c.elem == 4)) && off && !zq.empty() && zq.size() <= 2 && disjoint &&
+               per_axis[0] <= 4 && per_axis[1] <= 4 && c.ade_scheme < 0 && !records_meet_slabs;
     c.zslab.assign(c.slabs.size(), 0);
     if (c.zfused)
       for (int q : zq) c.zslab[q] = 1;
@@ -1112,9 +1119,23 @@ void zCpml(xfd_ctx& c, xfd::ZCpml<T>& ze, xfd::ZCpml<T>& zh) {
   ze = {};
   zh = {};
   for (size_t q = 0; q < c.slabs.size(); ++q) {
-    if (!c.zslab[q]) continue;
-    ze.z[ze.n++] = zSlab<T>(slabArgs<T>(*c.slabs[q], true), c.g);
-    zh.z[zh.n++] = zSlab<T>(slabArgs<T>(*c.slabs[q], false), c.g);
+    if (c.zslab[q]) {
+      ze.z[ze.n++] = zSlab<T>(slabArgs<T>(*c.slabs[q], true), c.g);
+      zh.z[zh.n++] = zSlab<T>(slabArgs<T>(*c.slabs[q], false), c.g);
+      continue;
+    }
+    const int ax = c.slabs[q]->axis;  // FP64, an x or y slab: its range joins the frame
+    if (ax == 2 || sizeof(T) == 4) continue;
+    for (int is_e = 0; is_e < 2; ++is_e) {
+      const auto a = slabArgs<T>(*c.slabs[q], is_e != 0);
+      xfd::ZFrame& fr = is_e ? ze.fr : zh.fr;
+      int& n = ax == 0 ? fr.nx : fr.ny;
+      require(n < 4, "more than four CPML slabs along one axis");
+      auto& r = ax == 0 ? fr.xr[n] : fr.yr[n];
+      r[0] = a.lo[ax];
+      r[1] = a.lo[ax] + a.ext[ax];
+      ++n;
+    }
   }
 }
 
@@ -1158,7 +1179,7 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   }
   xfd::ZCpml<T> ZE{}, ZH{};
   if (c.zfused) zCpml<T>(c, ZE, ZH);
-  auto inSweep = [&](size_t q) { return static_cast<int>(q) < c.n_fused || (c.zfused && c.zslab[q]); };
+  auto inSweep = [&](size_t q) { return static_cast<int>(q) < c.n_fused; };
   for (int64_t t = t0; t < t0 + n; ++t) {
     c.timed(K_UPDATE_H, [&] {
       if (c.classed)
@@ -1173,7 +1194,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (size_t q = 0; q < sh.size(); ++q)
       c.timed(K_CPML_H, [&] {
-        if (inSweep(q))
+        if (c.zfused && c.zslab[q] && sizeof(T) == 8)
+          xfd::Launch<T>::cpml_zrest(st, c.g, F, sh[q], ZH.fr, false);
+        else if (inSweep(q) || (c.zfused && c.zslab[q]))
           xfd::Launch<T>::cpml_faces(st, c.g, F, sh[q], false);
         else
           xfd::Launch<T>::cpml(st, c.g, F, sh[q], false);
@@ -1197,7 +1220,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (size_t q = 0; q < se.size(); ++q)
       c.timed(K_CPML_E, [&] {
-        if (inSweep(q))
+        if (c.zfused && c.zslab[q] && sizeof(T) == 8)
+          xfd::Launch<T>::cpml_zrest(st, c.g, F, se[q], ZE.fr, true);
+        else if (inSweep(q) || (c.zfused && c.zslab[q]))
           xfd::Launch<T>::cpml_faces(st, c.g, F, se[q], true);
         else
           xfd::Launch<T>::cpml(st, c.g, F, se[q], true);
@@ -1461,7 +1486,7 @@ struct xfd_batch {
   Table sweep, rec_e, rec_h, nf;
   std::vector<Table> cpml_e, cpml_h;            // slabs stepped by separate launches
   std::vector<Table> faces_e, faces_h;          // far planes of slabs fused into the sweeps
-  int fused_mode = 0;  // sweeps with slabs inside: 1 leading slabs (some case), 2 z slabs
+  bool any_fused = false;  // some case applies leading slabs inside its sweeps
   int64_t pool_rows = INT64_MAX;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -1514,7 +1539,7 @@ int64_t groupsOf(int64_t items) { return (items + xfd::kBatchGroupItems - 1) / x
 // Builds the entry tables of every batched launch for the current ichunk.
 template <class T>
 void batchTables(xfd_batch& b) {
-  b.fused_mode = 0;
+  b.any_fused = false;
   Collect<xfd::BatchCase<T>> sw;
   std::vector<Collect<xfd::BatchSlab<T>>> ce, ch, fe, fh;
   Collect<xfd::BatchRec<T>> re, rh;
@@ -1531,15 +1556,11 @@ void batchTables(xfd_batch& b) {
     }
     bc.ichunk = b.ichunk;
     if (c.fused) {
-      b.fused_mode = 1;
+      b.any_fused = true;
       bc.ph = {c.cpml_list_h.as<xfd::SlabArgs<T>>(), c.n_fused};
       bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
-    } else if (c.zfused) {
-      b.fused_mode = 2;
-      zCpml<T>(c, bc.ze, bc.zh);
     }
-    require(!(c.fused && b.fused_mode == 2) && !(c.zfused && b.fused_mode == 1),
-            "cases mix CPML-in-sweep kinds");
+    require(!c.zfused, "batched cases keep their z slabs separate");
     bc.nbx = (c.g.nz + xfd::kLanesK - 1) / xfd::kLanesK;
     bc.nby = (c.g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ;
     bc.nbz = (c.g.nx + b.ichunk - 1) / b.ichunk;
@@ -1555,7 +1576,7 @@ void batchTables(xfd_batch& b) {
       for (int is_e = 0; is_e < 2; ++is_e) {
         xfd::BatchSlab<T> e{c.g, F, slabArgs<T>(*c.slabs[o], is_e != 0)};
         e.s.stride_c = stride[e.s.axis];
-        if (static_cast<int>(o) < c.n_fused || (c.zfused && c.zslab[o])) {  // only the far-plane remainder
+        if (static_cast<int>(o) < c.n_fused) {  // only the far-plane remainder
           const auto fp = xfd::facePieces(c.g, e.s);
           (is_e ? fe : fh)[o].add(e, groupsOf(fp.nX + fp.nY + fp.nZ));
         } else {
@@ -1598,7 +1619,7 @@ void batchEnqueueStep(xfd_batch& b) {
   const int64_t* t = b.tdev.as<int64_t>();
   auto sweep = [&](bool is_h) {
     L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n, b.sweep.total,
-             b.grid, b.fused_mode);
+             b.grid, b.any_fused);
   };
   auto records = [&](xfd_batch::Table& r) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
@@ -1679,7 +1700,7 @@ void batchTune(xfd_batch& b, int reps) {
       auto pair = [&] {
         for (bool is_h : {true, false})
           L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
-                   b.sweep.total, b.grid, b.fused_mode);
+                   b.sweep.total, b.grid, b.any_fused);
       };
       pair();  // warm-up
       XFD_HIP(hipEventRecord(b.ev0, b.stream));
@@ -1943,6 +1964,11 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       xfd_ctx* c = nullptr;
       if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
       b->cases.emplace_back(c);
+      // batched cases keep their z slabs as separate launches: with most
+      // wavefronts of these small boxes carrying slab lanes, the sweeps were
+      // measured slower with them inside
+      c->zfused = false;
+      std::fill(c->zslab.begin(), c->zslab.end(), 0);
       b->shared = b->shared && c->classed && c->ade_scheme < 0 && c->tdnfs.empty();
 
     }

@@ -124,7 +124,8 @@ __device__ __forceinline__ ZLane<T> zLane(const ZCpml<T>& Z, int i0, int j, int 
   const ZSlab<T>& s1 = Z.z[1];
   const bool in0 = Z.n > 0 && k >= s0.klo && k < s0.khi;
   const bool in1 = Z.n > 1 && k >= s1.klo && k < s1.khi;
-  L.in = in0 || in1;
+  // FP64: a j row in a y slab is frame throughout (FP32 has no frame)
+  L.in = (in0 || in1) && (sizeof(T) == 4 || !Z.fr.inY(j));
   L.sel = in1;
   if (!L.in) return L;
   const int klo = in1 ? s1.klo : s0.klo;
@@ -141,22 +142,25 @@ __device__ __forceinline__ ZLane<T> zLane(const ZCpml<T>& Z, int i0, int j, int 
 // The z-slab corrections of the sweep point (PMLCorrector order within the
 // slab: component A, then B):
 //   psi = b psi + a d,  v += c_psi psi,
-// with d the dual difference along z.  zLoad issues the point's psi and c_psi
-// reads (and, for E, the k-1 duals) at the top of the plane, beside the Yee
-// reads, so a slab lane waits for memory once per plane, not twice; zStore
-// finishes the corrections once the Yee values are known.  psi and c_psi are
-// plain (cached) accesses: one 128-byte line of them serves the 4 j-rows of
-// a work-group (measured: the non-temporal hint costs 8 % of the H sweep).
+// with d the dual difference along z, off the frame.  zLoad issues the
+// point's psi and c_psi reads (and, for E, the k-1 duals) at the top of the
+// plane, beside the Yee reads, so a slab lane waits for memory once per
+// plane, not twice; zStore finishes the corrections once the Yee values are
+// known.  psi and c_psi are plain (cached) accesses: one 128-byte line of
+// them serves the 4 j-rows of a work-group (measured: the non-temporal hint
+// costs 8 % of the H sweep).
 template <class T>
 struct ZVals {
   T pA, pB, cA, cB, dualA, dualB;
+  bool act;
 };
 
 template <class T, bool IsE>
-__device__ __forceinline__ ZVals<T> zLoad(const ZCpml<T>& Z, const ZLane<T>& L, const T* __restrict__ duA,
+__device__ __forceinline__ ZVals<T> zLoad(const ZCpml<T>& Z, const ZLane<T>& L, int i, const T* __restrict__ duA,
                                           const T* __restrict__ duB, int64_t idx) {
-  ZVals<T> v{T(0), T(0), T(0), T(0), T(0), T(0)};
-  if (L.in) {
+  ZVals<T> v{T(0), T(0), T(0), T(0), T(0), T(0), false};
+  v.act = L.in && (sizeof(T) == 4 || !Z.fr.inX(i));
+  if (v.act) {
     const ZSlab<T>& s0 = Z.z[0];
     const ZSlab<T>& s1 = Z.z[1];
     v.pA = (L.sel ? s1.psiA : s0.psiA)[L.t];
@@ -174,7 +178,7 @@ __device__ __forceinline__ ZVals<T> zLoad(const ZCpml<T>& Z, const ZLane<T>& L, 
 template <class T>
 __device__ __forceinline__ void zStore(const ZCpml<T>& Z, const ZLane<T>& L, const ZVals<T>& v, T& vA, T& vB, T dA,
                                        T dB) {
-  if (L.in) {
+  if (v.act) {
     const ZSlab<T>& s0 = Z.z[0];
     const ZSlab<T>& s1 = Z.z[1];
     const T psiA = L.cb * v.pA + L.ca * dA;
@@ -215,7 +219,7 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   if (ZF) zl = zLane(Z, i0, j, k);
   for (int i = i0; i < i1; ++i, idx += pl) {
     ZVals<T> zv;
-    if (ZF) zv = zLoad<T, false>(Z, zl, nullptr, nullptr, idx);
+    if (ZF) zv = zLoad<T, false>(Z, zl, i, nullptr, nullptr, idx);
     const T ex0 = ex[idx], ex_j = ex[idx + pz], ex_k = ex[idx + 1];
     const T ey_k = ey[idx + 1], ey_n = ey[idx + pl];
     const T ez_j = ez[idx + pz], ez_n = ez[idx + pl];
@@ -280,7 +284,7 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   if (ZF) zl = zLane(Z, i0, j, k);
   for (int i = i0; i < i1; ++i, idx += pl) {
     ZVals<T> zv;
-    if (ZF) zv = zLoad<T, true>(Z, zl, hy, hx, idx);  // Ex <- Hy, Ey <- Hx (k >= 1 in a z slab)
+    if (ZF) zv = zLoad<T, true>(Z, zl, i, hy, hx, idx);  // Ex <- Hy, Ey <- Hx (k >= 1 in a z slab)
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
     T s, a, b;
     if (!Fused || !mask) {
@@ -494,6 +498,21 @@ __device__ __forceinline__ void cpmlFacePoint(const Grid& g, const FieldPtrs<T>&
 template <class T, bool IsE>
 __global__ __launch_bounds__(256) void k_cpml_faces(Grid g, FieldPtrs<T> F, SlabArgs<T> s) {
   cpmlFacePoint<T, IsE>(g, F, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// The points of a z slab the sweeps leave: its frame (ZFrame) and the far
+// planes i = nx, j = ny, at the slab's place in boundary order.
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_cpml_zrest(Grid g, FieldPtrs<T> F, SlabArgs<T> s, ZFrame fr) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t vol = static_cast<int64_t>(s.ext[0]) * s.ext[1] * s.ext[2];
+  if (t >= vol) return;
+  const int w = static_cast<int>(t % s.ext[2]);
+  const int v = static_cast<int>((t / s.ext[2]) % s.ext[1]);
+  const int u = static_cast<int>(t / (static_cast<int64_t>(s.ext[2]) * s.ext[1]));
+  const int p0 = s.lo[0] + u, p1 = s.lo[1] + v;
+  if (p0 < g.nx && p1 < g.ny && !fr.inX(p0) && !fr.inY(p1)) return;
+  cpmlPoint<T, IsE>(g, F, s, p0, p1, s.lo[2] + w, t);
 }
 
 enum AdeScheme { kAdeDrude = 0, kAdeDebye = 1, kAdeMLorentz = 2 };
@@ -1039,6 +1058,8 @@ struct Launch {
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
                        const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
   static void cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
+  static void cpml_zrest(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a,
+                         const ZFrame& fr, bool is_e);
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
                            const AdeArgs<T>& A, int scheme, int ichunk);
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C,

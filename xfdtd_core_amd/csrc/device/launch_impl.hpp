@@ -81,6 +81,17 @@ void Launch<T>::cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, 
     hipLaunchKernelGGL((k_cpml_faces<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, g, F, a);
 }
 
+template <class T>
+void Launch<T>::cpml_zrest(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a,
+                           const ZFrame& fr, bool is_e) {
+  const int64_t n = static_cast<int64_t>(a.ext[0]) * a.ext[1] * a.ext[2];
+  if (n <= 0) return;
+  if (is_e)
+    hipLaunchKernelGGL((k_cpml_zrest<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s, g, F, a, fr);
+  else
+    hipLaunchKernelGGL((k_cpml_zrest<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, g, F, a, fr);
+}
+
 template <class T, class Coef>
 void launchUpdateAde(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
                      int scheme, int ichunk) {
@@ -174,25 +185,18 @@ void Launch<T>::pack(hipStream_t s, const Grid& g, const T* src, double* dst, in
 // ------------------------------------------------------------- batched --
 template <class T>
 void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n,
-                           int64_t total, int grid, int mode) {
+                           int64_t total, int grid, bool fused) {
   if (total <= 0) return;
   dim3 block(kLanesK, kRowsJ);
   dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
-  if (is_h) {
-    if (mode == 1)
-      hipLaunchKernelGGL((k_batch_sweep<T, true, true, false>), g, block, 0, s, cases, start, n, total);
-    else if (mode == 2)
-      hipLaunchKernelGGL((k_batch_sweep<T, true, false, true>), g, block, 0, s, cases, start, n, total);
-    else
-      hipLaunchKernelGGL((k_batch_sweep<T, true, false, false>), g, block, 0, s, cases, start, n, total);
-  } else {
-    if (mode == 1)
-      hipLaunchKernelGGL((k_batch_sweep<T, false, true, false>), g, block, 0, s, cases, start, n, total);
-    else if (mode == 2)
-      hipLaunchKernelGGL((k_batch_sweep<T, false, false, true>), g, block, 0, s, cases, start, n, total);
-    else
-      hipLaunchKernelGGL((k_batch_sweep<T, false, false, false>), g, block, 0, s, cases, start, n, total);
-  }
+  if (is_h && fused)
+    hipLaunchKernelGGL((k_batch_sweep<T, true, true>), g, block, 0, s, cases, start, n, total);
+  else if (is_h)
+    hipLaunchKernelGGL((k_batch_sweep<T, true, false>), g, block, 0, s, cases, start, n, total);
+  else if (fused)
+    hipLaunchKernelGGL((k_batch_sweep<T, false, true>), g, block, 0, s, cases, start, n, total);
+  else
+    hipLaunchKernelGGL((k_batch_sweep<T, false, false>), g, block, 0, s, cases, start, n, total);
 }
 template <class T>
 void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,

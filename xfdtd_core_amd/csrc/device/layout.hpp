@@ -236,7 +236,8 @@ struct CpmlList {
 // field and dual values come from the sweep's registers.  The slab spans the
 // whole x/y range (box-local psi index t = (i * ext1 + j) * ext2 + k - klo)
 // and both of its components cover every sweep point (i < nx, j < ny) of its
-// layers [klo, khi); the far planes i = nx, j = ny are left to k_cpml_faces.
+// layers [klo, khi); the far planes i = nx, j = ny (k_cpml_faces) and, in
+// FP64, the frame (ZFrame, k_cpml_zrest) are left to a launch of their own.
 template <class T>
 struct ZSlab {
   int klo, khi;  // corrected k planes (layer = k - klo)
@@ -249,11 +250,32 @@ struct ZSlab {
   const T* cb;
 };
 
+// The points of a z slab that x or y slabs also correct (the frame): i in
+// one of the x slabs' ranges or j in one of the y slabs'.  In FP64 the sweep
+// applies a z slab only off the frame, where it is the one correction of the
+// point, so the reference's order is kept; the frame follows the x/y slabs
+// in a launch of its own (k_cpml_zrest).  FP32 leaves the frame empty.
+struct ZFrame {
+  int nx = 0, ny = 0;
+  int xr[4][2] = {}, yr[4][2] = {};  // [lo, hi)
+  __device__ __forceinline__ bool inX(int i) const {
+    bool r = false;
+    for (int q = 0; q < nx; ++q) r = r || (i >= xr[q][0] && i < xr[q][1]);
+    return r;
+  }
+  __device__ __forceinline__ bool inY(int j) const {
+    bool r = false;
+    for (int q = 0; q < ny; ++q) r = r || (j >= yr[q][0] && j < yr[q][1]);
+    return r;
+  }
+};
+
 // The z slabs (at most two, disjoint in k) of one half-step.
 template <class T>
 struct ZCpml {
   ZSlab<T> z[2];
   int n = 0;
+  ZFrame fr;
 };
 
 // Additive correction applied after a Yee half-step, in the reference's

@@ -382,7 +382,8 @@ class Simulation:
     def cpml_fused(self) -> int:
         """Which CPML slabs the Yee sweeps apply in the same pass: 0 none,
         1 the leading x/y slabs (XFD_CPML_FUSED=1), 2 the z slabs (FP32
-        when nz > 192; XFD_CPML_ZFUSED=1 forces it, =0 turns it off)."""
+        when nz > 192; either precision with XFD_CPML_ZFUSED=1; =0 turns it
+        off)."""
         return int(self._L.xfd_sim_cpml_fused(self._h))
 
     def coef_classed(self) -> bool:
