@@ -122,3 +122,16 @@ def test_nf2ff_surfaces_match_oracle(dist):
     ref, prod = _both(sp.spec(), 10)
     for d in range(6):
         assert ref.nf2ff_surface(0, d) == prod.nf2ff_surface(0, d), d
+
+
+def test_two_dimensional_domain_is_refused():
+    """An object of infinite z extent makes the reference build a 2D grid
+    space (grid_space_generator.cpp:23-32) with the TE updator; this library
+    steps 3D grids only and says so instead of sizing a grid from inf."""
+    dl = 1e-3
+    s = S.Simulation(dl, dl, 1.0, 0.8)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-0.01, -0.01, -np.inf), S.Vector(0.02, 0.02, np.inf)),
+                         S.Material.createAir()))
+    prod = native.Simulation(s.spec())
+    with pytest.raises(RuntimeError, match="1D/2D"):
+        prod.precompute(4)

@@ -568,6 +568,11 @@ struct Simulation::Impl {
     if (any_pml)
       dom = Cube(Vector(plo[0], plo[1], plo[2]), Vector(phi[0] - plo[0], phi[1] - plo[1], phi[2] - plo[2]));
     Real mn[3], mx[3];
+    for (int a = 0; a < 3; ++a)
+      // an infinite extent makes the reference build a 1D/2D grid space
+      // (grid_space_generator.cpp), whose TE/TEM updators this library lacks
+      if (!std::isfinite(dom.origin()[a]) || !std::isfinite(dom.end()[a]))
+        throw XFDTDSimulationException("1D/2D grid spaces (objects of infinite extent) are not supported");
     for (int a = 0; a < 3; ++a) {
       mn[a] = dom.origin()[a];
       mx[a] = dom.end()[a];
