@@ -103,8 +103,8 @@ def test_coefficients_bitwise(case):
         assert bad.size == 0, f"{name}: {len(bad)} cells differ, first {bad[:3].tolist()}"
 
 
-def test_readme_grid_matches_survey_numbers():
-    """README example: 63^3 cells and dt = 1.299937e-11 s (CFL 0.9, dl 7.5 mm)."""
+def test_readme_example_grid_and_time_step():
+    """The reference README's example: 63^3 cells and dt = 1.299937e-11 s (CFL 0.9, dl 7.5 mm)."""
     ref = RefSim(scenarios.dielectric_sphere().spec())
     ref.init(1)
     assert ref.grid() == (63, 63, 63)
