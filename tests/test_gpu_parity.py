@@ -576,3 +576,45 @@ def test_profiling_selected_kernel_classes():
     sim.advance(20, 20)
     st = sim.kernel_stats()
     assert st["update_e"][0] == 20 and st["update_h"][0] == 20 and st["cpml_e"][0] == 20 * 6, st
+
+
+def test_full_size_box_is_repeatable_and_linear():
+    """The 384^3 inhomogeneous box (64 spheres, 12-cell CPML, the measured
+    workload) at full size, through properties that hold at any size: two
+    runs give bitwise the same fields (no race in any launch), FP64 and FP32;
+    and doubling the waveform amplitude multiplies every FP64 field by 4.
+    Each step is linear in the source current, and the reference's current
+    source applies the amplitude twice: in the waveform's values
+    (waveform.cpp:31) and in its amplitude factor (current_source.cpp:63).
+    Scaling by a power of 2 is exact; values in the denormal range at the
+    edge of the light cone may round differently, hence the 1e-14 bound
+    instead of equality."""
+    base = scenarios.inhomogeneous_box(monitors=False).spec()
+
+    def spec(amplitude):  # the waveform amplitude is the source line's last token
+        lines = base.splitlines()
+        src = [n for n, ln in enumerate(lines) if ln.startswith("current_source ")]
+        assert len(src) == 1 and lines[src[0]].split()[-1] == "1.0"
+        lines[src[0]] = " ".join(lines[src[0]].split()[:-1] + [repr(amplitude)])
+        return "\n".join(lines) + "\n"
+
+    steps = 150
+    for precision in ("f64", "f32"):
+        say(f"[gpu] 384^3 box, {steps} steps, {precision}: repeatability" + (" and linearity" if precision == "f64" else ""))
+        runs = []
+        for amp in ((1.0, 1.0, 2.0) if precision == "f64" else (1.0, 1.0)):
+            sim = native.Simulation(spec(amp), precision)
+            sim.run(steps)
+            runs.append(sim)
+        for f in FIELDS:
+            a, b = runs[0].field(f), runs[1].field(f)
+            assert np.abs(a).max() > 0, f
+            np.testing.assert_array_equal(a, b, err_msg=f)
+            if precision == "f64":
+                c = runs[2].field(f)
+                e = np.linalg.norm(c - 4 * a) / np.linalg.norm(4 * a)
+                say(f"  {f}: doubled-amplitude residue {e:.1e}, exact in {np.mean(c == 4 * a):.6f} of the points")
+                assert e < 1e-14, (f, e)
+            del a, b
+        for sim in runs:
+            sim.close()
