@@ -618,3 +618,24 @@ def test_full_size_box_is_repeatable_and_linear():
             del a, b
         for sim in runs:
             sim.close()
+
+
+def test_full_size_dispersive_box_is_repeatable():
+    """The 256^3 dispersive + lumped box (Drude, Debye, Lorentz and modified
+    Lorentz slabs, TFSF plane wave, lumped elements; the measured workload)
+    at full size: two FP64 runs give bitwise the same fields, so the ADE,
+    record and CPML launches carry no race at this size."""
+    sp = scenarios.dispersive_slabs().spec()
+    steps = 200
+    say(f"[gpu] 256^3 dispersive box, {steps} steps, f64: repeatability")
+    runs = []
+    for _ in range(2):
+        sim = native.Simulation(sp, "f64")
+        sim.run(steps)
+        runs.append(sim)
+    for f in FIELDS:
+        a, b = runs[0].field(f), runs[1].field(f)
+        assert np.abs(a).max() > 0, f
+        np.testing.assert_array_equal(a, b, err_msg=f)
+    for sim in runs:
+        sim.close()
