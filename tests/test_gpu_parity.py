@@ -151,6 +151,8 @@ def test_lumped_oblique_tfsf_fp64():
         assert e <= TOL_F64, (k, e)
 
 
+@pytest.mark.filterwarnings("ignore:Mean of empty slice:RuntimeWarning")
+@pytest.mark.filterwarnings("ignore:invalid value encountered:RuntimeWarning")
 def test_circuit_elements_fp64():
     """Capacitors, inductors (per-edge current state on the device), PEC
     planes, a resistor and a voltage source beside a lossy sphere and CPML."""
