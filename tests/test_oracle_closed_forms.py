@@ -77,7 +77,6 @@ def test_staircase_and_edge_averaging_brute_force():
     # coefficient closed form at one averaged Ex edge
     dt = ref.dt
     e = ex[4, 3, 1]
-    s = 0.25 * (0.3 + 0.3 + 1e-20 + 1e-20) if False else None  # sigma checked below
     sig = ref.material(2, 0)[4, 3, 1]
     cexe = (2 * e - dt * sig) / (2 * e + dt * sig)
     assert ref.coeff("cexe")[4, 3, 1] == cexe
