@@ -150,6 +150,9 @@ def run_device(sim, precision, warmup, steps, n, pml):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(ach / PEAK_HBM_GBS, 4) if ach else None,
                      "traffic": traffic.get(dom) if traffic else None,
+                     "traffic_source": (f"stored rocprofv3 FETCH_SIZE/WRITE_SIZE profile "
+                                        f"({TRAFFIC_FILE.relative_to(ROOT)}), not measured in this run"
+                                        if traffic else None),
                      "hbm_GBps_measured": kernels[dom].get("hbm_GBps_pmc")},
     }
     log(f"{precision}: {res['mcells_per_s']:.0f} Mcells/s, {res['ms_per_step']:.3f} ms/step, dominant {dom} "
@@ -165,31 +168,73 @@ def run_device(sim, precision, warmup, steps, n, pml):
     res["coefficients"] = "class ids" if sim.coef_classed() else "dense"
     res["cpml"] = {0: "separate slab launches", 1: "x/y slabs inside the Yee sweeps",
                    2: "z slabs inside the Yee sweeps, x/y slabs separate"}[int(fused)]
-    # recordings for the accuracy comparison: six snapshot series, then probes
     sim.finish()
-    res["snapshots"] = {f: sim.monitor_records(i) for i, f in enumerate(FIELDS)}
-    res["probes"] = [sim.monitor_records(m)[0][1] for m in range(len(FIELDS), sim.monitor_count())]
     return res
 
 
-def fp32_error(f32, f64, interval):
-    """FP32 snapshots and probes against the FP64 run of the same scenario
-    (the FP64 device path is bitwise equal to the oracle on every parity
-    case, so it stands in for the oracle at this size)."""
+ACCURACY_STEPS = 1000
+
+
+def accuracy_leg(sim, steps, interval=500):
+    """FP32 against FP64 on the headline box, outside the timed region and
+    independent of --steps: one untimed run per precision of `steps` + 1
+    steps, so the scenario's snapshots land on steps 0, 500, ..., `steps`,
+    plus the 32 probe series.  The FP64 device run is the error reference:
+    it equals the oracle bitwise on every parity case, and at this full size
+    the GPU test test_full_size_fp64_matches_oracle checks it against the
+    oracle over 100 steps."""
     from xfdtd_core_amd import accuracy
 
+    runs = {}
+    for prec in ("f64", "f32"):
+        t0 = time.perf_counter()
+        sim.set_precision(prec)
+        sim.init(steps + 1)
+        sim.set_profiling(False)
+        sim.advance(0, steps + 1)
+        sim.finish()
+        runs[prec] = {"snapshots": {f: sim.monitor_records(i) for i, f in enumerate(FIELDS)},
+                      "probes": [sim.monitor_records(m)[0][1] for m in range(len(FIELDS), sim.monitor_count())]}
+        log(f"accuracy leg: {prec} {steps + 1} steps in {time.perf_counter() - t0:.1f} s")
+    f32, f64 = runs["f32"], runs["f64"]
     n = len(f64["snapshots"]["EX"])
     rows = accuracy.snapshot_errors(f32["snapshots"], f64["snapshots"], [k * interval for k in range(n)])
     probes = [accuracy.series_error(a, b) for a, b in zip(f32["probes"], f64["probes"])]
-    out = {"reference": "FP64 run of the same scenario on the GPU",
+    seen = [p for p in probes if p is not None]
+    out = {"reference": "FP64 run of the same scenario on the GPU (equal to the oracle; see the docstring)",
+           "steps": steps,
            "snapshots": [{"step": r["step"], "E_rel_l2": r["E"]["own"], "H_rel_l2": r["H"]["own"],
                           "E_rel_peak": r["E"]["peak"], "H_rel_peak": r["H"]["peak"]} for r in rows if r["step"] > 0],
-           "probes_rel_l2_max": max(probes) if probes else None}
+           "probes_rel_l2_max": max(seen) if seen else None,
+           "probes_compared": len(seen), "probes": len(probes)}
     for r in out["snapshots"]:
         log(f"fp32 error step {r['step']}: E {r['E_rel_l2']:.2e} (peak-normalised {r['E_rel_peak']:.2e}), "
             f"H {r['H_rel_l2']:.2e} ({r['H_rel_peak']:.2e})")
-    log(f"fp32 error probes: max relative L2 {out['probes_rel_l2_max']:.2e}")
+    if seen:
+        log(f"fp32 error probes: max relative L2 {out['probes_rel_l2_max']:.2e} over {len(seen)} probes")
+    del runs, f32, f64
+    gc.collect()
     return out
+
+
+def physical_cores():
+    """Physical cores of this host from lscpu (cores per socket x sockets),
+    or None when lscpu is unavailable."""
+    import subprocess
+
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    kv = {}
+    for line in txt.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        return int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(spec, n, steps):
@@ -199,8 +244,17 @@ def cpu_baseline(spec, n, steps):
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle import RefSim  # CPU restatement of the reference path
 
-    cores = max(1, min(16, os.cpu_count() or 1))
-    log(f"cpu baseline: oracle init on {n}^3 (untimed), then 1 + {steps} steps per layout on {cores} threads")
+    # Threads: this process's CPU share (its affinity set, capped by the
+    # OMP_NUM_THREADS the host sets for one GPU), never more than the host's
+    # physical cores.
+    phys = physical_cores()
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        share = min(share, int(omp))
+    cores = max(1, min(share, phys or share))
+    log(f"cpu baseline: host has {phys} physical cores (lscpu), CPU share {share}; oracle init on {n}^3 "
+        f"(untimed), then 1 + {steps} steps per layout on {cores} threads")
     t0 = time.perf_counter()
     ref = RefSim(spec)
     ref.init(steps + 1)
@@ -219,6 +273,7 @@ def cpu_baseline(spec, n, steps):
     gc.collect()
     rate, layout, sec = best
     return {"value": round(rate, 2), "unit": "Mcells/s", "cores": cores, "kind": "port",
+            "host_physical_cores_lscpu": phys, "process_cpu_share": share,
             "sample": f"same {n}^3 scenario, FP64 oracle, {steps} steps timed after 1 warm-up step "
                       f"({sec:.1f} s), ThreadConfig{{{layout[0]},{layout[1]},{layout[2]}}} "
                       f"(faster of {{1,1,{cores}}} and {{{cores},1,1}})"}
@@ -292,37 +347,41 @@ def bench_design_sweep(args):
 
 
 def bench_dispersive(args):
-    """Dispersive + lumped box at full size: 256^3 (10-cell CPML), four ADE
-    slabs (Drude, Debye, two-pole Lorentz, modified Lorentz), oblique TFSF,
-    50-ohm resistor and voltage source, 8-frequency NF2FF 15 cells inside the
-    grid; FP64 and FP32 runs of the same steps, the FP32 error against FP64
-    (final fields, far-field potentials, radiated power) and the far field
-    summed on the GPU vs the host."""
+    """Dispersive + lumped box at full size: 256^3 (10-cell CPML), ADE slabs,
+    oblique TFSF, 50-ohm resistor and voltage source, 8-frequency NF2FF 15
+    cells inside the grid.
+
+    Throughput (FP64 and FP32, `--dispersive-steps` steps each) on two
+    material sets: the four slabs as specified (Drude wp = 1.37e16 rad/s,
+    Debye, two-pole Lorentz, modified Lorentz) and the same box without the
+    Debye slab.  With the Debye slab every dispersive cell runs the
+    modified-Lorentz scheme with b2 = 0 for Debye, outside that scheme's
+    stability condition 4 b2 - b0 dt^2 >= 0 (MLorentzEqDecision::
+    stabilityCondition, which the reference defines but never calls): the
+    fields grow without bound in FP64 exactly as in the oracle.  The FP32
+    error is stated on the box without Debye, with the specified stiff Drude
+    slab (wp dt ~ 1.25e4): FP64 and FP32 runs advance side by side and are
+    compared every `--dispersive-check` steps (field state, far-field
+    potentials, radiated power), together with the FP64 run's largest |E|,
+    which shows where the reference's own scheme stops being bounded."""
     from xfdtd_core_amd import native, scenarios
 
     steps = args.dispersive_steps
     res = {"workload": "dispersive + lumped 256^3 (10-cell CPML), Drude/Debye/Lorentz/M-Lorentz slabs, oblique TFSF, "
                        "50-ohm resistor + voltage source, 8-frequency NF2FF", "steps": steps}
-    # Throughput on the workload as specified.  Its fields diverge, in FP64
-    # exactly as in the oracle: with mixed kinds the reference runs every
-    # material through the modified-Lorentz scheme, and the Debye slab there
-    # has b2 = 0, outside that scheme's stability condition 4 b2 - b0 dt^2 >= 0
-    # (MLorentzEqDecision::stabilityCondition, which the reference defines but
-    # never calls).  The FP32 error is measured on the same box with the stable
-    # slabs only: Drude (wp = 1.9e11 rad/s), two-pole Lorentz, modified Lorentz.
-    for label, kinds, wp in (("as_specified", ("drude", "debye", "lorentz", "mlorentz"), 1.37e16),
-                             ("stable_slabs", ("drude", "lorentz", "mlorentz"), 1.9e11)):
-        sp = scenarios.dispersive_slabs(nf2ff_distance=15, drude_wp=wp, kinds=kinds).spec()
-        res[label] = _dispersive_run(sp, steps, label, wp)
-        res[label]["kinds"] = list(kinds)
+    sets = {"as_specified": ("drude", "debye", "lorentz", "mlorentz"), "without_debye": ("drude", "lorentz", "mlorentz")}
+    for label, kinds in sets.items():
+        sp = scenarios.dispersive_slabs(nf2ff_distance=15, kinds=kinds).spec()
+        res[label] = {"kinds": list(kinds), "drude_wp": 1.37e16, **_dispersive_throughput(sp, steps, label)}
+    sp = scenarios.dispersive_slabs(nf2ff_distance=15, kinds=sets["without_debye"]).spec()
+    res["fp32_error"] = _dispersive_error(sp, steps, args.dispersive_check)
     return res
 
 
-def _dispersive_run(spec, steps, label, wp):
+def _dispersive_throughput(spec, steps, label):
     from xfdtd_core_amd import native
 
     out = {}
-    sims = {}
     for prec in ("f64", "f32"):
         sim = native.Simulation(spec, prec)
         t0 = time.perf_counter()
@@ -334,60 +393,74 @@ def _dispersive_run(spec, steps, label, wp):
         native.device_synchronize()
         wall = time.perf_counter() - t0
         sim.finish()
-        g = sim.grid()
-        cells = int(np.prod(g))
-        out[prec] = {"mcells_per_s": round(cells * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
-                     "init_s": round(init_s, 2)}
+        cells = int(np.prod(sim.grid()))
         finite = all(bool(np.isfinite(sim.field(f)).all()) for f in FIELDS)
-        out[prec]["fields_finite"] = finite
+        out[prec] = {"mcells_per_s": round(cells * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+                     "init_s": round(init_s, 2), "fields_finite": finite,
+                     "max_abs_E": float(max(np.abs(sim.field(f)).max() for f in FIELDS[:3]))}
         log(f"dispersive {label} {prec}: {out[prec]['mcells_per_s']:.0f} Mcells/s ({out[prec]['ms_per_step']:.3f} "
-            f"ms/step), fields finite: {finite}")
-        sims[prec] = sim
-    a, b = sims["f32"], sims["f64"]
-    base = {"drude_wp": wp, "fp64": out["f64"], "fp32": out["f32"],
-            "fp32_over_fp64": round(out["f32"]["mcells_per_s"] / out["f64"]["mcells_per_s"], 3)}
-    if not (out["f64"]["fields_finite"] and out["f32"]["fields_finite"]):
-        return base
+            f"ms/step), max |E| at the end {out[prec]['max_abs_E']:.3e}")
+        sim.close()
+    out["fp32_over_fp64"] = round(out["f32"]["mcells_per_s"] / out["f64"]["mcells_per_s"], 3)
+    return out
+
+
+def _dispersive_error(spec, steps, check):
+    """FP64 and FP32 contexts advanced side by side; every `check` steps the
+    FP32 field state (E and eta0 H together), far-field potentials (each of
+    the four against the largest at its frequency, 181 directions) and
+    radiated power are compared with FP64."""
+    from xfdtd_core_amd import native
+
     eta = 376.730313668
-    num = den = 0.0
-    for f in FIELDS:
-        w = eta if f[0] == "H" else 1.0
-        x, y = a.field(f) * w, b.field(f) * w
-        num += float(np.sum((x - y) ** 2))
-        den += float(np.sum(y ** 2))
+    sims = {p: native.Simulation(spec, p) for p in ("f64", "f32")}
+    for sim in sims.values():
+        sim.init(steps)
     theta = np.linspace(0.0, np.pi, 181)
-    nf = 8
-    pot_err, pow_err = 0.0, 0.0
-    t0 = time.perf_counter()
-    for f in range(nf):  # each potential against the largest of the four at that frequency
-        pa = [a.nf2ff_potential(0, f, ap, tc, theta, 0.0) for ap in (True, False) for tc in (True, False)]
-        pb = [b.nf2ff_potential(0, f, ap, tc, theta, 0.0) for ap in (True, False) for tc in (True, False)]
-        scale = max(float(np.abs(x).max()) for x in pb)
-        pot_err = max([pot_err] + [float(np.abs(x - y).max()) / scale for x, y in zip(pa, pb)])
-        pow_err = max(pow_err, abs(a.nf2ff_power(0, f) / b.nf2ff_power(0, f) - 1))
-    dev_s = (time.perf_counter() - t0) / 2  # both precisions' potentials were summed
-    # the same potentials on the host for one frequency and a 19-direction cut
-    os.environ["XFD_FARFIELD"] = "host"
-    th_small = np.linspace(0.0, np.pi, 19)
-    t0 = time.perf_counter()
-    for ap in (True, False):
-        for tc in (True, False):
-            b.nf2ff_potential(0, 0, ap, tc, th_small, 0.0)
-    host_s = time.perf_counter() - t0
-    del os.environ["XFD_FARFIELD"]
-    host_per_dir = host_s / (4 * 19)
-    res = {
-        **base,
-        "fp32_error": {"field_state_rel_l2": float(np.sqrt(num / den)),
-                       "far_field_potential_rel_max": pot_err, "radiated_power_rel_max": pow_err},
-        "far_field": {"directions": 181, "frequencies": nf, "device_s_for_all": round(dev_s, 4),
-                      "host_s_per_direction_and_potential": round(host_per_dir, 5),
-                      "host_s_estimate_for_all": round(host_per_dir * 4 * 181 * nf, 1)},
-    }
-    log(f"dispersive {label} fp32 error: fields {res['fp32_error']['field_state_rel_l2']:.2e}, far field {pot_err:.2e}, "
-        f"power {pow_err:.2e}; far field on device {dev_s:.3f} s, host estimate "
-        f"{res['far_field']['host_s_estimate_for_all']:.1f} s")
-    return res
+    rows = []
+    t = 0
+    while t < steps:
+        n = min(check, steps - t)
+        for sim in sims.values():
+            sim.advance(t, n)
+        t += n
+        a, b = sims["f32"], sims["f64"]
+        num = den = 0.0
+        emax = 0.0
+        for f in FIELDS:
+            w = eta if f[0] == "H" else 1.0
+            x, y = a.field(f) * w, b.field(f) * w
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(y ** 2))
+            if f[0] == "E":
+                emax = max(emax, float(np.abs(y).max()))
+        row = {"step": t, "fp64_max_abs_E": emax,
+               "field_state_rel_l2": float(np.sqrt(num / den)) if den > 0 and np.isfinite(num) else None}
+        if np.isfinite(emax) and row["field_state_rel_l2"] is not None:
+            pot_err, pow_err = 0.0, 0.0
+            for f in range(8):
+                pa = [a.nf2ff_potential(0, f, ap, tc, theta, 0.0) for ap in (True, False) for tc in (True, False)]
+                pb = [b.nf2ff_potential(0, f, ap, tc, theta, 0.0) for ap in (True, False) for tc in (True, False)]
+                scale = max(float(np.abs(x).max()) for x in pb)
+                pot_err = max([pot_err] + [float(np.abs(x - y).max()) / scale for x, y in zip(pa, pb)])
+                pow_err = max(pow_err, abs(a.nf2ff_power(0, f) / b.nf2ff_power(0, f) - 1))
+            row["far_field_potential_rel_max"] = pot_err
+            row["radiated_power_rel_max"] = pow_err
+        rows.append(row)
+        log(f"dispersive error step {t}: fields {row['field_state_rel_l2']}, far field "
+            f"{row.get('far_field_potential_rel_max')}, power {row.get('radiated_power_rel_max')}, "
+            f"FP64 max |E| {emax:.3e}")
+    for sim in sims.values():
+        sim.close()
+    # bounded: the FP64 field's maximum stays within twice the largest value
+    # it took during the first third of the run (the pulse and the sources)
+    early = max(r["fp64_max_abs_E"] for r in rows if r["step"] <= steps // 3) if rows else 0.0
+    bounded = [r for r in rows if r["fp64_max_abs_E"] <= 2 * early and r["field_state_rel_l2"] is not None]
+    last = bounded[-1] if bounded else None
+    return {"reference": "FP64 run of the same scenario on the GPU (equal to the oracle)",
+            "kinds": ["drude", "lorentz", "mlorentz"], "drude_wp": 1.37e16, "check_interval": check,
+            "timeline": rows, "last_bounded_step": last["step"] if last else None,
+            "at_last_bounded_step": last}
 
 
 def bench_nf2ff_td(args):
@@ -500,6 +573,7 @@ def main():
                          "dispersive: the 256^3 ADE + lumped + NF2FF box; nf2ff_td: the time-domain NF2FF cost")
     ap.add_argument("--td-steps", type=int, default=1000)
     ap.add_argument("--dispersive-steps", type=int, default=3000)
+    ap.add_argument("--dispersive-check", type=int, default=250)
     ap.add_argument("--cases", type=int, default=96)
     ap.add_argument("--sequential", type=int, default=24,
                     help="design_sweep: also time this many cases stepped one context at a time")
@@ -536,9 +610,6 @@ def main():
     order = ["f64", "f32"] if args.precision == "both" else [args.precision]
     for p in order:
         out[p] = run_device(sim, p, args.warmup, args.steps, args.n, args.pml)
-    sim.close()
-    del sim
-    gc.collect()
 
     head = out.get("f32") or out["f64"]
     line = {
@@ -564,16 +635,15 @@ def main():
         "kernels": head["kernels"],
     }
     if "f64" in out and "f32" in out:
-        line["fp32_error"] = fp32_error(out["f32"], out["f64"], 500)
+        line["fp32_error"] = accuracy_leg(sim, ACCURACY_STEPS)
         f64 = out["f64"]
         line["fp64"] = {"value": round(f64["mcells_per_s"], 1), "unit": "Mcells/s",
                         "ms_per_step": round(f64["ms_per_step"], 4), "hbm_GBps_alg_step": f64["step_alg_GBps"],
                         "step_roofline": f64["step_roofline"],
                         "roofline": f64["roofline"], "kernels": f64["kernels"]}
         line["fp32_over_fp64"] = round(out["f32"]["mcells_per_s"] / f64["mcells_per_s"], 3)
-    for r in out.values():
-        r.pop("snapshots", None)
-        r.pop("probes", None)
+    sim.close()
+    del sim
     gc.collect()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec, args.n, args.cpu_steps)

@@ -14,10 +14,13 @@
 #define XFDTD_CORE_AMD_HPP_
 
 #include <array>
+#include <cmath>
 #include <complex>
 #include <cstddef>
 #include <functional>
+#include <limits>
 #include <memory>
+#include <numbers>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,6 +29,21 @@ namespace xfdtd {
 
 using Real = double;
 using Index = std::size_t;
+
+// include/xfdtd/common/constant.h:11-23: the reference's constants, with
+// eps0 and mu0 exactly as it writes them (not CODATA 2018).
+namespace constant {
+inline constexpr Real PI{std::numbers::pi_v<Real>};
+inline constexpr Real EPSILON_0{8.854187817e-12};
+inline constexpr Real MU_0{4 * PI * 1e-7};
+inline const Real C_0{1 / std::sqrt(EPSILON_0 * MU_0)};
+inline const Real Z_0{std::sqrt(MU_0 / EPSILON_0)};
+inline constexpr Real SIGMA_E_ZERO_APPROX{1e-20};
+inline constexpr Real SIGMA_M_ZERO_APPROX{1e-20};
+inline constexpr Real INF{std::numeric_limits<Real>::infinity()};
+inline constexpr Real NEG_INF{-1 * std::numeric_limits<Real>::infinity()};
+constexpr auto II = std::complex<Real>(0.0, 1.0);
+}  // namespace constant
 
 // include/xfdtd/exception/exception.h
 class XFDTDException : public std::runtime_error {
@@ -385,26 +403,38 @@ struct EMF {
   Array3D<Real>& field(Field f);
 };
 
+// include/xfdtd/monitor/monitor.h.  output() writes the monitor's result to
+// <outputDir()>/<name()>.npy (monitor.cpp:42-54): `<f8`, C order, as
+// xt::dump_npy does; the data it writes is the one copied back by
+// Simulation::run().
 class Monitor {
  public:
-  Monitor(std::string name, std::string output_dir) : _name(std::move(name)), _output_dir(std::move(output_dir)) {}
+  Monitor(std::string name, std::string output_dir = "xfdtd_output")
+      : _name(std::move(name)), _output_dir(std::move(output_dir)) {}
   virtual ~Monitor() = default;
   const std::string& name() const { return _name; }
   const std::string& outputDir() const { return _output_dir; }
   void setName(std::string n) { _name = std::move(n); }
   void setOutputDir(std::string d) { _output_dir = std::move(d); }
+  virtual void output() {}
 
  private:
   std::string _name, _output_dir;
 };
 
+// A box of one field (field_monitor.h).  Inside a MovieMonitor it is the
+// frame; added on its own, output() writes the box of the final fields
+// (field_monitor.cpp:82-96: update() records nothing, output() copies the
+// current view).
 class FieldMonitor : public Monitor {
  public:
-  FieldMonitor(std::unique_ptr<Shape> shape, EMF::Field field, std::string name, std::string output_dir);
+  FieldMonitor(std::unique_ptr<Shape> shape, EMF::Field field, std::string name = "feild_monitor",
+               std::string output_dir = "xfdtd_output");
   const Shape& shape() const { return *_shape; }
   EMF::Field field() const { return _field; }
   // Last frame written (field_monitor.cpp:82-96).
   Array3D<Real>& data() { return _data; }
+  void output() override;
 
  private:
   std::unique_ptr<Shape> _shape;
@@ -412,10 +442,12 @@ class FieldMonitor : public Monitor {
   Array3D<Real> _data;
 };
 
+// movie_monitor.h: frame files "%05d.npy" of the record count are written
+// while the loop runs (movie_monitor.cpp:26-34); output() writes nothing.
 class MovieMonitor : public Monitor {
  public:
-  MovieMonitor(std::unique_ptr<Monitor> frame, std::size_t frame_interval, std::string name,
-               std::string output_dir);
+  MovieMonitor(std::unique_ptr<Monitor> frame, std::size_t frame_interval, std::string name = "movie_monitor",
+               std::string output_dir = "xfdtd_output");
   FieldMonitor& frame() const { return *_frame; }
   std::size_t frameInterval() const { return _interval; }
   // Frames in recording order, named "%05d" of the record count.
@@ -434,6 +466,8 @@ class FieldTimeMonitor : public Monitor {
   EMF::Field field() const { return _field; }
   std::vector<Real>& time() { return _time; }
   std::vector<Real>& data() { return _data; }
+  // time_monitor.cpp:11-21: the (2, nt) stack of time and data.
+  void output() override;
 
  private:
   std::unique_ptr<Cube> _cube;
@@ -452,6 +486,8 @@ class VoltageMonitor : public Monitor {
   Axis::Direction direction() const { return _direction; }
   std::vector<Real>& time() { return _time; }
   std::vector<Real>& data() { return _data; }
+  // time_monitor.cpp:11-21: the (2, nt) stack of time and data.
+  void output() override;
 
  private:
   std::unique_ptr<Cube> _cube;
@@ -470,6 +506,8 @@ class CurrentMonitor : public Monitor {
   Axis::Direction direction() const { return _direction; }
   std::vector<Real>& time() { return _time; }
   std::vector<Real>& data() { return _data; }
+  // time_monitor.cpp:11-21: the (2, nt) stack of time and data.
+  void output() override;
 
  private:
   std::unique_ptr<Cube> _cube;
@@ -639,12 +677,75 @@ struct ThreadConfig {  // accepted for API compatibility; the GPU path ignores i
 
 enum class Precision { FP64, FP32 };
 
+// include/xfdtd/simulation/simulation_flag.h:8-31.  Simulation::run(n) sends
+// SimulationStart / SimulationEnd around itself and UpdateStart / UpdateEnd
+// around the time loop (simulation.cpp:284-289, domain.cpp:80-137).  The
+// loop runs on the GPU, so the per-phase iterator flags (UpdateHSStart ..
+// UpdateEEnd) have no host-side moment and are not sent; NextStep is sent
+// with cur = the number of steps done after each piece of the loop the host
+// waits for (about 100 pieces per run when a visitor is registered, one
+// otherwise), start = 0 and end = the run length.
+enum class SimulationInitFlag { SimulationStart, SimulationEnd, InitStart, InitEnd, UpdateStart, UpdateEnd };
+enum class SimulationIteratorFlag { UpdateHSStart, UpdateHSEnd, UpdateEStart, UpdateEEnd, NextStep };
+
+class SimulationFlagVisitor {
+ public:
+  virtual ~SimulationFlagVisitor() = default;
+  virtual auto initStep(SimulationInitFlag flag) -> void = 0;
+  virtual auto iteratorStep(SimulationIteratorFlag flag, Index cur, Index start, Index end) -> void = 0;
+};
+
+// include/xfdtd/calculation_param/time_param.h: the run's time axis, valid
+// after Simulation::init().  eTime() = (n + 1) dt and hTime() = (n + 0.5) dt
+// for n = 0 .. size - 1 (time_param.cpp:63-71).
+class TimeParam {
+ public:
+  TimeParam(Real dt, Index size, Index start_time_step = 0);
+  // time_param.cpp:10-21
+  static Real calculateDt(Real cfl, Real dx, Real dy, Real dz);
+  static Real calculateDt(Real cfl, Real dx, Real dy);
+  static Real calculateDt(Real cfl, Real dz);
+  Real dt() const { return _dt; }
+  Index startTimeStep() const { return _start; }
+  Index size() const { return _size; }
+  Index endTimeStep() const { return _start + _size; }
+  // The step the loop is at: endTimeStep() once run() has returned.
+  Index currentTimeStep() const { return _current; }
+  const Array1D<Real>& eTime() const { return _e_time; }
+  const Array1D<Real>& hTime() const { return _h_time; }
+  void setCurrentTimeStep(Index t) { _current = t; }
+
+ private:
+  Real _dt;
+  Index _start, _size, _current;
+  Array1D<Real> _e_time, _h_time;
+};
+
+// include/xfdtd/calculation_param/calculation_param.h (its time part).
+class CalculationParam {
+ public:
+  explicit CalculationParam(std::shared_ptr<TimeParam> time_param) : _time(std::move(time_param)) {}
+  const std::shared_ptr<TimeParam>& timeParam() const { return _time; }
+
+ private:
+  std::shared_ptr<TimeParam> _time;
+};
+
 class Simulation {
  public:
   Simulation(Real dx, Real dy, Real dz, Real cfl, ThreadConfig thread_config = ThreadConfig{});
   ~Simulation();
   Simulation(const Simulation&) = delete;
   Simulation& operator=(const Simulation&) = delete;
+
+  // simulation.h:64-66, simulation.cpp:87-91, 148-268: flag visitors; the
+  // default one prints the reference's start/end lines and progress bar to
+  // stderr.
+  auto addVisitor(std::shared_ptr<SimulationFlagVisitor> visitor) -> void;
+  auto addDefaultVisitor() -> void;
+  auto visitors() -> std::vector<std::shared_ptr<SimulationFlagVisitor>>;
+  // simulation.h:78: dt and the run's E/H times (after init()).
+  const std::shared_ptr<CalculationParam>& calculationParam() const;
 
   void addObject(std::shared_ptr<Object> object);
   void addWaveformSource(std::shared_ptr<WaveformSource> source);

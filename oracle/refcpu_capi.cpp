@@ -270,6 +270,14 @@ void parseSpec(Sim& s, const std::string& spec) {
       if (m->out_dir == "-") m->out_dir.clear();
       m->frame = std::move(f);
       s.monitors.push_back(std::move(m));
+    } else if (d == "field_monitor") {  // FIELD CUBE NAME OUT_DIR: a FieldMonitor on its own
+      auto f = std::make_unique<FieldMonitor>();
+      f->field = parseField(word(is));
+      f->shape = parseCube(is);
+      f->name = word(is);
+      f->out_dir = word(is);
+      if (f->out_dir == "-") f->out_dir.clear();
+      s.monitors.push_back(std::move(f));
     } else if (d == "voltage_monitor" || d == "current_monitor") {  // NAME CUBE DIR OUT_DIR
       std::unique_ptr<Monitor> mon;
       std::string name = word(is);
@@ -616,7 +624,7 @@ int ref_monitor_records(void* p, int m) {
   Monitor* mo = s.monitors.at(m).get();
   if (auto mv = dynamic_cast<MovieMonitor*>(mo)) return static_cast<int>(mv->frames.size());
   if (dynamic_cast<FieldTimeMonitor*>(mo) || dynamic_cast<VoltageMonitor*>(mo) ||
-      dynamic_cast<CurrentMonitor*>(mo))
+      dynamic_cast<CurrentMonitor*>(mo) || dynamic_cast<FieldMonitor*>(mo))
     return 1;
   return 0;
 }
@@ -627,6 +635,10 @@ int ref_monitor_record(void* p, int m, int r, int64_t* shape, double* dst) {
   if (auto mv = dynamic_cast<MovieMonitor*>(mo)) {
     const Arr3& a = mv->frames.at(r).second;
     return copy3(a, shape, dst);
+  }
+  if (auto fm = dynamic_cast<FieldMonitor*>(mo)) {  // output() after the run: the box of the fields
+    fm->output(s);
+    return copy3(fm->data, shape, dst);
   }
   if (auto ft = dynamic_cast<FieldTimeMonitor*>(mo)) {
     Index nt = ft->time.size();

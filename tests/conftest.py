@@ -28,6 +28,11 @@ def _ensure_built():
         from xfdtd_core_amd import build as b
 
         b.build()
+    prog = ROOT / "tests" / "cpp" / "build" / "sphere_scatter"
+    if not prog.exists():
+        from xfdtd_core_amd import build as b
+
+        b.build_cpp_programs()
 
 
 _ensure_built()

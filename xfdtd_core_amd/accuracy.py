@@ -34,8 +34,8 @@ def snapshot_errors(test: Dict[str, Sequence[np.ndarray]], ref: Dict[str, Sequen
     """`test` / `ref` map a field name to its list of snapshot frames.
 
     Returns one entry per frame: {"frame": f, "step": s, "E": {"own", "peak"},
-    "H": {...}}.  Frames whose reference norm is zero report own = 0 when the
-    difference is zero too, else inf."""
+    "H": {...}}.  A figure whose reference norm is zero (a frame before the
+    source has fired) is None: there is nothing to be relative to."""
     nframes = min(len(ref[f]) for g in GROUPS.values() for f in g if f in ref)
     out = []
     peaks = {}
@@ -53,17 +53,18 @@ def snapshot_errors(test: Dict[str, Sequence[np.ndarray]], ref: Dict[str, Sequen
             d = np.sqrt(sum(_norm2(np.asarray(test[f][i], np.float64) - np.asarray(ref[f][i], np.float64))
                             for f in fs))
             n = np.sqrt(sum(_norm2(ref[f][i]) for f in fs))
-            own = d / n if n > 0 else (0.0 if d == 0 else float("inf"))
-            peak = d / peaks[g] if peaks[g] > 0 else (0.0 if d == 0 else float("inf"))
-            row[g] = {"own": float(own), "peak": float(peak)}
+            own = float(d / n) if n > 0 else None
+            peak = float(d / peaks[g]) if peaks[g] > 0 else None
+            row[g] = {"own": own, "peak": peak}
         out.append(row)
     return out
 
 
-def series_error(test: np.ndarray, ref: np.ndarray) -> float:
-    """Relative L2 of a probe's whole time series."""
+def series_error(test: np.ndarray, ref: np.ndarray):
+    """Relative L2 of a probe's whole time series; None when the reference
+    series is all zero (the probe never saw the field)."""
     t = np.asarray(test, np.float64).ravel()
     r = np.asarray(ref, np.float64).ravel()
     n = np.sqrt(_norm2(r))
     d = np.sqrt(_norm2(t - r))
-    return float(d / n) if n > 0 else (0.0 if d == 0 else float("inf"))
+    return float(d / n) if n > 0 else None

@@ -81,10 +81,35 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+CPP_TESTS = ROOT / "tests" / "cpp"
+
+
+def build_cpp_programs(verbose: bool = False):
+    """Compiles the C++ scenario programs under tests/cpp/ (user code of the
+    C++ API, include/xfdtd_core_amd.hpp) with g++ against the library, into
+    tests/cpp/build/.  The run path points at the library relative to the
+    program, so the tree can move (it does on the way to a GPU host)."""
+    out = CPP_TESTS / "build"
+    out.mkdir(exist_ok=True)
+    built = []
+    for src in sorted(CPP_TESTS.glob("*.cpp")):
+        exe = out / src.stem
+        cmd = ["g++", "-std=c++20", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+               f"-L{LIB_DIR}", "-lxfdtd_core_amd", "-Wl,-rpath,$ORIGIN/../../../xfdtd_core_amd/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        built.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return built
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
 
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_programs(verbose=True)
     sys.exit(0)

@@ -259,6 +259,14 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
                                               name, dir);
       s.addMonitor(m);
       h->monitors.push_back(m);
+    } else if (d == "field_monitor") {  // FIELD CUBE NAME OUT_DIR: a FieldMonitor on its own
+      auto f = parseField(t.word());
+      auto cb = cube(t);
+      std::string name = t.word(), dir = t.word();
+      if (dir == "-") dir.clear();
+      auto m = std::make_shared<FieldMonitor>(std::move(cb), f, name, dir);
+      s.addMonitor(m);
+      h->monitors.push_back(m);
     } else if (d == "voltage_monitor" || d == "current_monitor") {  // NAME CUBE DIR OUT_DIR
       std::string name = t.word();
       auto cb = cube(t);
@@ -536,11 +544,15 @@ int64_t xfd_sim_grid_array(void* p, int32_t which, int32_t axis, double* dst) {
 
 int xfd_sim_monitor_count(void* p) { return static_cast<int>(H(p)->monitors.size()); }
 
+int xfd_sim_monitor_output(void* p, int32_t m) {
+  return guard([&] { H(p)->monitors.at(m)->output(); });
+}
+
 int xfd_sim_monitor_records(void* p, int32_t m) {
   auto& mon = H(p)->monitors.at(m);
   if (auto* mv = dynamic_cast<MovieMonitor*>(mon.get())) return static_cast<int>(mv->frames().size());
   if (dynamic_cast<FieldTimeMonitor*>(mon.get()) || dynamic_cast<VoltageMonitor*>(mon.get()) ||
-      dynamic_cast<CurrentMonitor*>(mon.get()))
+      dynamic_cast<CurrentMonitor*>(mon.get()) || dynamic_cast<FieldMonitor*>(mon.get()))
     return 1;
   return 0;
 }
@@ -550,6 +562,10 @@ int xfd_sim_monitor_record(void* p, int32_t m, int32_t r, int64_t* shape, double
     auto& mon = H(p)->monitors.at(m);
     if (auto* mv = dynamic_cast<MovieMonitor*>(mon.get())) {
       copyArray(mv->frames().at(r).second, shape, dst);
+      return;
+    }
+    if (auto* fm = dynamic_cast<FieldMonitor*>(mon.get())) {  // the box of the final fields
+      copyArray(fm->data(), shape, dst);
       return;
     }
     std::vector<Real>* lt = nullptr;

@@ -261,6 +261,103 @@ VoltageMonitor::VoltageMonitor(std::string n, std::unique_ptr<Cube> c, Axis::Dir
 CurrentMonitor::CurrentMonitor(std::string n, std::unique_ptr<Cube> c, Axis::Direction dir, std::string d)
     : Monitor(std::move(n), std::move(d)), _cube(std::move(c)), _direction(dir) {}
 
+namespace {
+// Monitor::output (monitor.cpp:42-54): <dir>/<name>.npy, the directory
+// created when missing; an empty directory name is the working directory.
+std::string outputPath(const Monitor& m) {
+  if (m.outputDir().empty()) return m.name() + ".npy";
+  std::filesystem::create_directories(m.outputDir());
+  return (std::filesystem::path(m.outputDir()) / (m.name() + ".npy")).string();
+}
+
+// TimeMonitor::output (time_monitor.cpp:11-21): xt::stack(time, data), a
+// (2, nt) array.
+void writeTimeSeries(const Monitor& m, const std::vector<Real>& time, const std::vector<Real>& data) {
+  if (time.size() != data.size()) throw XFDTDException(m.name() + ": no recording to output (run first)");
+  std::vector<Real> st(time);
+  st.insert(st.end(), data.begin(), data.end());
+  npy::write(outputPath(m), {2, time.size()}, st.data());
+}
+}  // namespace
+
+void FieldMonitor::output() {
+  npy::write(outputPath(*this), {_data.shape[0], _data.shape[1], _data.shape[2]}, _data.data.data());
+}
+void FieldTimeMonitor::output() { writeTimeSeries(*this, _time, _data); }
+void VoltageMonitor::output() { writeTimeSeries(*this, _time, _data); }
+void CurrentMonitor::output() { writeTimeSeries(*this, _time, _data); }
+
+// time_param.cpp:10-21
+Real TimeParam::calculateDt(Real cfl, Real dx, Real dy, Real dz) {
+  return cfl / (constant::C_0 * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
+}
+Real TimeParam::calculateDt(Real cfl, Real dx, Real dy) {
+  return cfl / (constant::C_0 * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy)));
+}
+Real TimeParam::calculateDt(Real cfl, Real dz) { return cfl / (constant::C_0 * std::sqrt(1.0 / (dz * dz))); }
+
+TimeParam::TimeParam(Real dt, Index size, Index start_time_step)
+    : _dt(dt), _start(start_time_step), _size(size), _current(start_time_step) {
+  // time_param.cpp:63-71: linspace((start + 1) dt, (end) dt) and the half steps
+  const Real a = static_cast<Real>(start_time_step), b = static_cast<Real>(start_time_step + size);
+  _e_time.data = linspaceAssign((a + 1) * dt, b * dt, size);
+  _h_time.data = linspaceAssign((a + 0.5) * dt, (b + 0.5) * dt, size);
+}
+
+namespace {
+// The reference's default visitor (simulation.cpp:148-268): start / end
+// lines with elapsed times and a progress bar on stderr.
+class DefaultSimulationFlagVisitor : public SimulationFlagVisitor {
+ public:
+  auto initStep(SimulationInitFlag flag) -> void override {
+    using clock = std::chrono::high_resolution_clock;
+    auto elapsed = [](clock::time_point t0) {
+      return std::to_string(std::chrono::duration_cast<std::chrono::milliseconds>(clock::now() - t0).count()) +
+             " ms";
+    };
+    switch (flag) {
+      case SimulationInitFlag::SimulationStart:
+        _sim = clock::now();
+        std::fprintf(stderr, "Simulation start\n");
+        break;
+      case SimulationInitFlag::SimulationEnd:
+        std::fprintf(stderr, "Simulation end\nElapsed time: %s\n", elapsed(_sim).c_str());
+        break;
+      case SimulationInitFlag::InitStart:
+        _init = clock::now();
+        std::fprintf(stderr, "Simulation init start\n");
+        break;
+      case SimulationInitFlag::InitEnd:
+        std::fprintf(stderr, "Simulation init end\nElapsed time: %s\n", elapsed(_init).c_str());
+        break;
+      case SimulationInitFlag::UpdateStart:
+        _update = clock::now();
+        std::fprintf(stderr, "Simulation update start\n");
+        break;
+      case SimulationInitFlag::UpdateEnd:
+        std::fprintf(stderr, "Simulation update end\nElapsed time: %s\n", elapsed(_update).c_str());
+        break;
+    }
+  }
+  auto iteratorStep(SimulationIteratorFlag flag, Index cur, Index start, Index end) -> void override {
+    if (flag != SimulationIteratorFlag::NextStep || end == 0) return;
+    using clock = std::chrono::high_resolution_clock;
+    const double p = static_cast<double>(cur) / static_cast<double>(end);
+    const int width = 50, pos = static_cast<int>(width * p);
+    std::string bar(width, ' ');
+    for (int i = 0; i < width; ++i) bar[i] = i < pos ? '=' : (i == pos ? '>' : ' ');
+    const double ms = std::chrono::duration<double, std::milli>(clock::now() - _update).count();
+    const double left = cur > start ? ms * static_cast<double>(end - cur) / static_cast<double>(cur - start) : 0.0;
+    std::fprintf(stderr, "\rProgress: [%s] %3d%% %zu/%zu Elapsed time: %.0f ms. Estimated remaining time: %.0f ms%s",
+                 bar.c_str(), static_cast<int>(p * 100.0), cur, end, ms, left, cur == end ? "\n" : "");
+    std::fflush(stderr);
+  }
+
+ private:
+  std::chrono::high_resolution_clock::time_point _sim, _init, _update;
+};
+}  // namespace
+
 // ------------------------------------------------------------- network ---
 namespace {
 // dft (util/dft.h:9-26): X(f) = dt sum_j x_j exp(-2 i pi f ((j + 1) dt + shift)),
@@ -487,6 +584,16 @@ struct Simulation::Impl {
   std::vector<std::complex<Real>> obj_dft;
   Real obj_value = 0;
   bool forward_done = false;
+
+  // flag visitors (simulation.h:64-66), the run's time axis and the
+  // FieldMonitors added on their own with their node box
+  std::vector<std::shared_ptr<SimulationFlagVisitor>> visitors;
+  std::shared_ptr<CalculationParam> calc;
+  struct FieldBox {
+    FieldMonitor* monitor;
+    int32_t o[3], e[3];
+  };
+  std::vector<FieldBox> field_boxes;
 
   xfd_ctx* ctx = nullptr;
   // Set when ctx is a case of a SimulationBatch: the batch core owns it.
@@ -1412,6 +1519,7 @@ struct Simulation::Impl {
 
   void buildMonitors() {
     lines.clear();
+    field_boxes.clear();
     for (auto& m : monitors) {
       GatherPlan gp;
       gp.monitor = m.get();
@@ -1444,6 +1552,14 @@ struct Simulation::Impl {
         continue;
       } else if (auto* cm = dynamic_cast<CurrentMonitor*>(m.get())) {
         lines.push_back(currentPlan(*cm));
+        continue;
+      } else if (auto* fm = dynamic_cast<FieldMonitor*>(m.get())) {
+        // on its own a FieldMonitor records nothing while the loop runs
+        // (field_monitor.cpp:80): its box of the final fields is taken after
+        // the run
+        FieldBox fb{fm, {0, 0, 0}, {0, 0, 0}};
+        monitorBox(fm->shape(), fb.o, fb.e);
+        field_boxes.push_back(fb);
         continue;
       } else {
         throw XFDTDException("unsupported monitor type");
@@ -1539,6 +1655,7 @@ struct Simulation::Impl {
     coef_view.clear();
     stage("monitors");
     for (auto& nw : networks) nw->init(dt);  // Simulation::init (simulation.cpp:353-355)
+    calc = std::make_shared<CalculationParam>(std::make_shared<TimeParam>(dt, steps, 0));
     precomputed = true;
     precomputed_steps = n;
   }
@@ -1676,9 +1793,25 @@ struct Simulation::Impl {
     cpml_fused = fu;
   }
 
+  void sendFlag(SimulationInitFlag f) {
+    for (auto& v : visitors) v->initStep(f);
+  }
+
+  // The time loop (Domain::run, domain.cpp:80-137): with visitors it is
+  // submitted in about 100 pieces, each followed by NextStep.
   void runLoop() {
-    advance(0, steps);
+    sendFlag(SimulationInitFlag::UpdateStart);
+    const Index piece = visitors.empty() ? std::max<Index>(steps, 1) : std::max<Index>(1, (steps + 99) / 100);
+    double total = 0;
+    for (Index t = 0; t < steps; t += piece) {
+      const Index n = std::min(piece, steps - t);
+      advance(t, n);
+      total += loop_seconds;
+      for (auto& v : visitors) v->iteratorStep(SimulationIteratorFlag::NextStep, t + n, 0, steps);
+    }
+    loop_seconds = total;
     finish();
+    sendFlag(SimulationInitFlag::UpdateEnd);
   }
 
   // Steps t0 .. t0+n-1 on the device; returns when they are done.
@@ -1691,6 +1824,7 @@ struct Simulation::Impl {
     if (xfd_step(ctx, static_cast<int64_t>(t0), static_cast<int64_t>(n)) != XFD_OK)
       throw XFDTDSimulationException(std::string("xfd_step: ") + xfd_last_error());
     loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    if (calc) calc->timeParam()->setCurrentTimeStep(t0 + n);
     stats.clear();
     for (int k = 0;; ++k) {
       const char* name = nullptr;
@@ -1828,15 +1962,9 @@ struct Simulation::Impl {
       std::vector<Real> buf(fe * static_cast<std::size_t>(g.d.count));
       if (!buf.empty() && xfd_read_gather(ctx, static_cast<int32_t>(gi), buf.data()) != XFD_OK)
         throw XFDTDSimulationException(std::string("xfd_read_gather: ") + xfd_last_error());
-      if (g.time_series) {
+      if (g.time_series) {  // its file is written by output() (time_monitor.cpp:11-21)
         auto* ft = static_cast<FieldTimeMonitor*>(g.monitor);
         if (fe == 1) ft->data() = buf;
-        if (!ft->outputDir().empty()) {  // time_monitor.cpp:11-21: stack(time, data)
-          std::vector<Real> st(ft->time());
-          st.insert(st.end(), ft->data().begin(), ft->data().end());
-          std::filesystem::create_directories(ft->outputDir());
-          npy::write(ft->outputDir() + "/" + ft->name() + ".npy", {2, ft->time().size()}, st.data());
-        }
       } else {
         auto* mv = static_cast<MovieMonitor*>(g.monitor);
         auto& frames = mv->frames();
@@ -1861,21 +1989,11 @@ struct Simulation::Impl {
       std::vector<Real> buf(steps);
       if (!buf.empty() && xfd_read_line(ctx, static_cast<int32_t>(m), buf.data()) != XFD_OK)
         throw XFDTDSimulationException(std::string("xfd_read_line: ") + xfd_last_error());
-      std::vector<Real>* time = nullptr;
-      if (auto* vm = dynamic_cast<VoltageMonitor*>(lines[m].monitor)) {
+      // files are written by output() (time_monitor.cpp:11-21)
+      if (auto* vm = dynamic_cast<VoltageMonitor*>(lines[m].monitor))
         vm->data() = buf;
-        time = &vm->time();
-      } else if (auto* cm = dynamic_cast<CurrentMonitor*>(lines[m].monitor)) {
+      else if (auto* cm = dynamic_cast<CurrentMonitor*>(lines[m].monitor))
         cm->data() = buf;
-        time = &cm->time();
-      }
-      const auto& dir = lines[m].monitor->outputDir();
-      if (time && !dir.empty()) {  // time_monitor.cpp:11-21: stack(time, data)
-        std::vector<Real> st(*time);
-        st.insert(st.end(), buf.begin(), buf.end());
-        std::filesystem::create_directories(dir);
-        npy::write(dir + "/" + lines[m].monitor->name() + ".npy", {2, time->size()}, st.data());
-      }
     }
     // final fields (Simulation::emf)
     Array3D<Real>* dst[6] = {&emf->ex, &emf->ey, &emf->ez, &emf->hx, &emf->hy, &emf->hz};
@@ -1885,6 +2003,16 @@ struct Simulation::Impl {
       *dst[f] = Array3D<Real>(ext[0], ext[1], ext[2]);
       if (xfd_read_field(ctx, f, dst[f]->data.data()) != XFD_OK)
         throw XFDTDSimulationException(std::string("xfd_read_field: ") + xfd_last_error());
+    }
+    for (auto& fb : field_boxes) {  // FieldMonitor::output's view (field_monitor.cpp:82-96)
+      const Array3D<Real>& src = emf->field(fb.monitor->field());
+      const std::size_t n0 = fb.e[0] > fb.o[0] ? fb.e[0] - fb.o[0] : 0, n1 = fb.e[1] > fb.o[1] ? fb.e[1] - fb.o[1] : 0,
+                        n2 = fb.e[2] > fb.o[2] ? fb.e[2] - fb.o[2] : 0;
+      Array3D<Real> a(n0, n1, n2);
+      for (std::size_t i = 0; i < n0; ++i)
+        for (std::size_t j = 0; j < n1; ++j)
+          for (std::size_t k = 0; k < n2; ++k) a(i, j, k) = src(fb.o[0] + i, fb.o[1] + j, fb.o[2] + k);
+      fb.monitor->data() = std::move(a);
     }
     for (std::size_t q = 0; q < nf2ffs.size(); ++q) {  // the surface currents
       auto& D = nf2ffs[q]->data();
@@ -1952,10 +2080,20 @@ void Simulation::precompute(Index n) { _impl->precompute(n); }
 void Simulation::run() { _impl->runLoop(); }
 void Simulation::advance(Index t0, Index n) { _impl->advance(t0, n); }
 void Simulation::finish() { _impl->finish(); }
-void Simulation::run(Index n) {
+void Simulation::run(Index n) {  // simulation.cpp:284-289
+  _impl->sendFlag(SimulationInitFlag::SimulationStart);
   init(n);
   run();
+  _impl->sendFlag(SimulationInitFlag::SimulationEnd);
 }
+auto Simulation::addVisitor(std::shared_ptr<SimulationFlagVisitor> v) -> void {
+  _impl->visitors.push_back(std::move(v));
+}
+auto Simulation::addDefaultVisitor() -> void {
+  _impl->visitors.push_back(std::make_shared<DefaultSimulationFlagVisitor>());
+}
+auto Simulation::visitors() -> std::vector<std::shared_ptr<SimulationFlagVisitor>> { return _impl->visitors; }
+const std::shared_ptr<CalculationParam>& Simulation::calculationParam() const { return _impl->calc; }
 Real Simulation::dt() const { return _impl->dt; }
 std::array<Index, 3> Simulation::gridSize() const { return {_impl->N[0], _impl->N[1], _impl->N[2]}; }
 const std::shared_ptr<EMF>& Simulation::emf() const { return _impl->emf; }

@@ -78,6 +78,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_field": (ctypes.c_int, [vp, i32, p64, vp]),
         "xfd_sim_coeff": (ctypes.c_int, [vp, cp, p64, vp]),
         "xfd_sim_monitor_count": (ctypes.c_int, [vp]),
+        "xfd_sim_monitor_output": (ctypes.c_int, [vp, i32]),
         "xfd_sim_monitor_records": (ctypes.c_int, [vp, i32]),
         "xfd_sim_monitor_record": (ctypes.c_int, [vp, i32, i32, p64, vp]),
         "xfd_sim_kernel_count": (ctypes.c_int, [vp]),
@@ -510,6 +511,10 @@ class Simulation:
 
     def monitor_count(self) -> int:
         return self._L.xfd_sim_monitor_count(self._h)
+
+    def monitor_output(self, m: int):
+        """Monitor::output() of monitor m: writes its .npy file(s)."""
+        self._check(self._L.xfd_sim_monitor_output(self._h, m))
 
     def kernel_stats(self):
         out = {}

@@ -369,12 +369,18 @@ class PecPlane:
 
 @dataclass
 class FieldMonitor:
-    """``FieldMonitor(shape, field, name, output_dir)`` (monitor/field_monitor.h)."""
+    """``FieldMonitor(shape, field, name, output_dir)`` (monitor/field_monitor.h).
+    Inside a MovieMonitor it is the frame; added on its own it records the box
+    of the final fields (field_monitor.cpp:80-96)."""
 
     shape: Cube
     field: str
-    name: str = ""
+    name: str = "feild_monitor"
     output_dir: str = ""
+
+    def line(self):
+        return " ".join(["field_monitor", self.field] + self.shape.box_tokens() +
+                        [self.name or "feild_monitor", self.output_dir or "-"])
 
 
 @dataclass
