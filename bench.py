@@ -423,6 +423,7 @@ def _dispersive_error(spec, steps, check):
         n = min(check, steps - t)
         for sim in sims.values():
             sim.advance(t, n)
+            sim.finish()  # fields and NF2FF currents back to the host
         t += n
         a, b = sims["f32"], sims["f64"]
         num = den = 0.0

@@ -696,8 +696,10 @@ class SimulationFlagVisitor {
 };
 
 // include/xfdtd/calculation_param/time_param.h: the run's time axis, valid
-// after Simulation::init().  eTime() = (n + 1) dt and hTime() = (n + 0.5) dt
-// for n = 0 .. size - 1 (time_param.cpp:63-71).
+// after Simulation::init().  As time_param.cpp:63-71 writes them, eTime() is
+// linspace((start + 1) dt, (start + size) dt, size) and hTime() is
+// linspace((start + 0.5) dt, (start + size + 0.5) dt, size), whose spacing
+// is size / (size - 1) dt.
 class TimeParam {
  public:
   TimeParam(Real dt, Index size, Index start_time_step = 0);
@@ -792,6 +794,10 @@ class Simulation {
   void setProfilingClasses(unsigned mask);
   std::vector<KernelStat> kernelStats() const;
   long long deviceBytes() const;
+  // The device context (xfd_ctx* of include/xfdtd_core_amd.h) this
+  // Simulation steps, for direct calls into the C-ABI; nullptr before init().
+  // Borrowed: it lives until the next init() or the Simulation's end.
+  void* deviceContext() const;
   // True when the device holds per-cell coefficient class ids instead of the
   // 18 dense coefficient arrays (same values; chosen at init).
   bool classedCoefficients() const;

@@ -92,6 +92,9 @@ int64_t xfd_sim_grid_array(void* sim, int32_t which, int32_t axis, double* dst);
 int xfd_sim_field(void* sim, int32_t field, int64_t* shape3, double* dst);
 int xfd_sim_coeff(void* sim, const char* name, int64_t* shape3, double* dst);
 int xfd_sim_monitor_count(void* sim);
+/* The device context (xfd_ctx*, include/xfdtd_core_amd.h) the simulation
+ * steps, for direct C-ABI calls; NULL before init.  Borrowed. */
+void* xfd_sim_device_context(void* sim);
 /* Monitor::output() of monitor m (monitor.h:65): writes its .npy file(s) in
  * its output directory; a movie's frames are written by the run itself. */
 int xfd_sim_monitor_output(void* sim, int32_t monitor);

@@ -98,22 +98,28 @@ def test_scenario_program_outputs_match_oracle(tmp_path):
     for line in ("Simulation start", "Simulation update start", "Simulation update end", "Simulation end",
                  "Progress: ["):
         assert line in r.stderr, line
-    flags = dict(re.findall(r"(\w+) (\d+)", r.stdout.split("flags:")[1]))
+    lines = {ln.split(":", 1)[0]: ln.split(":", 1)[1] for ln in r.stdout.splitlines() if ":" in ln}
+    flags = dict(re.findall(r"(\w+) (\d+)", lines["flags"]))
     assert flags["sim_start"] == flags["sim_end"] == flags["update_start"] == flags["update_end"] == "1"
     assert flags["last"] == flags["end"] == flags["current"] == str(steps)
     assert flags["monotonic"] == "1" and int(flags["next"]) >= 50
-    tok = r.stdout.split("dt:")[1].splitlines()[0].split()  # dt dt_check c0: C z0: Z
-    dt, dt_check, c0, z0 = float(tok[0]), float(tok[1]), float(tok[3]), float(tok[5])
-    assert dt == dt_check
+    tok = lines["dt"].split()  # dt, the nominal-dl dt, "c0:", C, "z0:", Z
+    dt, dt_nominal, c0, z0 = float(tok[0]), float(tok[1]), float(tok[3]), float(tok[5])
+    # the run's dt comes from the generated grid's smallest cell (0.35 m over
+    # 47 cells), TimeParam::calculateDt(0.9, dl, dl, dl) from the nominal dl
+    assert dt != dt_nominal and abs(dt / dt_nominal - 1) < 1e-2
     assert z0 == np.sqrt(4 * np.pi * 1e-7 / 8.854187817e-12)
     assert c0 == 1 / np.sqrt(8.854187817e-12 * 4 * np.pi * 1e-7)
-    n_t, h0, h1, e0, e1 = (float(v) for v in r.stdout.split("time:")[1].split())
-    assert n_t == steps and h0 == 0.5 * dt and abs(h1 - (steps - 0.5) * dt) < 1e-12 * h1
-    assert e0 == dt and abs(e1 - steps * dt) < 1e-12 * e1
+    n_t, h0, h1, e0, e1 = (float(v) for v in lines["time"].split())
+    # time_param.cpp:63-71: hTime = linspace(0.5 dt, (size + 0.5) dt, size), whose last sample is
+    # (size + 0.5) dt
+    assert n_t == steps and h0 == 0.5 * dt and h1 == (steps + 0.5) * dt
+    assert e0 == dt and e1 == steps * dt
 
     print("[gpu] C++ scenario program: oracle", flush=True)
     ref = RefSim(_oracle_spec())
     ref.run(steps)
+    assert dt == ref.dt
     frames = ref.monitor_records(0)
     assert len(frames) == steps // 10
     for q, want in enumerate(frames):

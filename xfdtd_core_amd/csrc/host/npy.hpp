@@ -13,9 +13,10 @@ namespace npy {
 
 inline void write(const std::string& path, const std::vector<std::size_t>& shape, const double* data,
                   const char* descr = "<f8", std::size_t item_doubles = 1) {
+  // the shape as a Python tuple: "(n,)" for one dimension, "(a, b, c)" else
   std::string sh = "(";
-  for (std::size_t i = 0; i < shape.size(); ++i) sh += std::to_string(shape[i]) + (shape.size() == 1 || i + 1 < shape.size() ? ", " : "");
-  sh += ")";
+  for (std::size_t i = 0; i < shape.size(); ++i) sh += (i ? ", " : "") + std::to_string(shape[i]);
+  sh += shape.size() == 1 ? ",)" : ")";
   std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
   const std::size_t unpadded = 10 + hdr.size() + 1;
   hdr += std::string((64 - unpadded % 64) % 64, ' ');

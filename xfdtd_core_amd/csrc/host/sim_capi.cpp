@@ -542,6 +542,8 @@ int64_t xfd_sim_grid_array(void* p, int32_t which, int32_t axis, double* dst) {
   return static_cast<int64_t>(v.size());
 }
 
+void* xfd_sim_device_context(void* p) { return H(p)->sim->deviceContext(); }
+
 int xfd_sim_monitor_count(void* p) { return static_cast<int>(H(p)->monitors.size()); }
 
 int xfd_sim_monitor_output(void* p, int32_t m) {

@@ -2124,6 +2124,7 @@ void Simulation::setProfilingClasses(unsigned mask) {
 }
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
+void* Simulation::deviceContext() const { return _impl->ctx; }
 bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
 int Simulation::fusedCpml() const { return _impl->cpml_fused; }
 
