@@ -142,11 +142,18 @@ Material Material::make(Real er, Real mr, Real se, Real sm) {
 }
 
 // ----------------------------------------------------------------- grid ---
-Real GridSpace::eps() const {
+Real GridSpace::eps() const {  // grid_space.cpp:459-468
+  if (dim == 1) return min_d[2] / 1000;
+  if (dim == 2) return std::min(min_d[0], min_d[1]) / 1000;
   return std::min({min_d[0], min_d[1], min_d[2]}) / 1000;
 }
 
 Index GridSpace::nearest(int axis, Real x) const {
+  // The one-cell axes of 1D/2D grids: handleTransformZ of GridSpace2D
+  // (grid_space_2d.cpp:116-122) and handleTransformX/Y of GridSpace1D
+  // (grid_space_1d.cpp:117-135) give 0 below +inf, else 1.
+  if ((dim == 2 && axis == 2) || (dim == 1 && axis < 2))
+    return x < std::numeric_limits<Real>::infinity() ? 0 : 1;
   // xt::argmin(xt::abs(e_node - x)): first index of the minimum
   const Arr1& e = e_node[axis];
   Index best = 0;
@@ -989,7 +996,8 @@ void TFSF3D::init(Sim& s) {
   pz_half = get_half(box_size[2] + 2, box_origin[2], inc[2], k.z);
 
   const Real dt = s.dt;
-  scaled_dl = s.g.based[2] / ratio_delta;
+  // tfsf.cpp:153-161: based dz in 1D and 3D, based dx in 2D
+  scaled_dl = (s.g.dim == 2 ? s.g.based[0] : s.g.based[2]) / ratio_delta;
   ceie = 1;
   chih = 1;
   ceihi = -(dt / (EPSILON_0 * scaled_dl));
@@ -1103,6 +1111,17 @@ struct TFSFCorrector : Corrector {  // tfsf_corrector.h:12-351, tfsf_corrector.c
   }
 
   void correctE(Index t) override {  // tfsf_corrector.cpp:177-203
+    if (f->kind == 1) {  // TFSF1DCorrector::correctE (tfsf_corrector.cpp:124-136)
+      correct(f->forward ? ZN : ZP, false, X, t);
+      return;
+    }
+    if (f->kind == 2) {  // TFSF2DCorrector::correctE (tfsf_corrector.cpp:153-163)
+      correct(XN, false, Z, t);
+      correct(XP, false, Z, t);
+      correct(YN, false, Z, t);
+      correct(YP, false, Z, t);
+      return;
+    }
     correct(XN, false, Z, t);
     correct(XP, false, Z, t);
     correct(YN, false, Z, t);
@@ -1117,6 +1136,17 @@ struct TFSFCorrector : Corrector {  // tfsf_corrector.h:12-351, tfsf_corrector.c
     correct(YP, false, X, t);
   }
   void correctH(Index t) override {  // tfsf_corrector.cpp:205-232
+    if (f->kind == 1) {  // TFSF1DCorrector::correctH (tfsf_corrector.cpp:138-151)
+      correct(f->forward ? ZN : ZP, true, Y, t);
+      return;
+    }
+    if (f->kind == 2) {  // TFSF2DCorrector::correctH (tfsf_corrector.cpp:165-175)
+      correct(XN, true, Y, t);
+      correct(XP, true, Y, t);
+      correct(YN, true, X, t);
+      correct(YP, true, X, t);
+      return;
+    }
     correct(XN, true, Y, t);
     correct(XP, true, Y, t);
     correct(YN, true, X, t);
@@ -1164,7 +1194,10 @@ static void monitorBox(Sim& s, const Shape& sh, Index o[3], Index e[3]) {
   for (int a = 0; a < 3; ++a) {
     o[a] = org[a];
     e[a] = end[a];
-    if (o[a] == 0) o[a] = 1;  // origin +1, size -1: end unchanged
+    // origin +1, size -1 (end unchanged) on x and y unless the grid is 1D,
+    // on z only in 3D (field_monitor.cpp:29-38, field_time_monitor.cpp:43-52)
+    const bool shift = a < 2 ? s.g.dim != 1 : s.g.dim == 3;
+    if (o[a] == 0 && shift) o[a] = 1;
   }
 }
 
@@ -1638,20 +1671,105 @@ void Sim::generateGridSpace() {
     }
   }
   Cube dom(Vec3{mn[0], mn[1], mn[2]}, Vec3{mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]});
+  // GridSpaceGenerator::decideDimension (grid_space_generator.cpp:101-124):
+  // x and y infinite -> 1D, only z infinite -> 2D, none -> 3D
+  auto decide = [](const Cube& c) {
+    const bool ix = std::isinf(c.size.x), iy = std::isinf(c.size.y), iz = std::isinf(c.size.z);
+    if (ix && iy && !iz) return 1;
+    if (!ix && !iy && iz) return 2;
+    if (!ix && !iy && !iz) return 3;
+    return 0;
+  };
   // GridSpaceGenerator::extendDomain (grid_space_generator.cpp:36-99)
   Real d[3] = {dx, dy, dz};
   if (!pmls.empty()) {
+    const int dim0 = decide(dom);
     Real lo[3] = {dom.origin.x, dom.origin.y, dom.origin.z};
     Real hi[3] = {dom.end.x, dom.end.y, dom.end.z};
     for (auto& p : pmls) {
-      if (p->thickness < 0) continue;
       int a = dirAxis(p->dir);
+      if (dim0 == 1 && a != Z) throw Error("PML has to be in Z direction");
+      if (dim0 == 2 && a == Z) throw Error("PML has to be in X or Y direction");
+      if (p->thickness < 0) continue;
       if (dirNegative(p->dir))
         lo[a] -= p->thickness * d[a];
       else
         hi[a] += p->thickness * d[a];
     }
     dom = Cube(Vec3{lo[0], lo[1], lo[2]}, Vec3{hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+  }
+  g.dim = decide(dom);  // generateUniformGridSpace (:16-34)
+  if (g.dim == 0) throw Error("GridSpace dimension is undefined");
+  const Real inf = std::numeric_limits<Real>::infinity();
+  // per-axis node arrays (h_node, e_size, h_size) of an axis with cells:
+  // GridSpace::correctGridSpaceForOne (grid_space.cpp:517-544)
+  auto correctOne = [&](int a, Real dl) {
+    const Arr1& e = g.e_node[a];
+    Index m = e.size() - 1;
+    g.h_node[a] = Arr1(m);
+    g.e_size[a] = Arr1(m);
+    for (Index i = 0; i < m; ++i) {
+      g.h_node[a](i) = (e(i) + e(i + 1)) / 2;
+      g.e_size[a](i) = e(i + 1) - e(i);
+    }
+    std::vector<Real> tmp(m + 2);
+    tmp[0] = e(0) - dl / 2;
+    for (Index i = 0; i < m; ++i) tmp[i + 1] = g.h_node[a](i);
+    tmp[m + 1] = e(m) + dl / 2;
+    g.h_size[a] = Arr1(m + 1);
+    for (Index i = 0; i < m + 1; ++i) g.h_size[a](i) = tmp[i + 1] - tmp[i];
+    Real md = std::numeric_limits<Real>::max();
+    for (Index i = 0; i < m; ++i) md = std::min(md, g.e_size[a](i));
+    g.min_d[a] = md;
+  };
+  // the one-cell axis of a 1D/2D grid (grid_space_2d.cpp:96-106,
+  // grid_space_1d.cpp:64-75): nodes {-inf, inf}, one cell of size 1 centred
+  // at 0
+  auto oneCell = [&](int a) {
+    g.e_node[a] = Arr1(2);
+    g.e_node[a](0) = -inf;
+    g.e_node[a](1) = inf;
+    g.h_node[a] = Arr1(1, 0.0);
+    g.e_size[a] = Arr1(1, 1.0);
+    g.h_size[a] = Arr1(1, 1.0);
+    g.min_d[a] = 1;
+  };
+  if (g.dim == 1) {
+    // generateGridSpace1D (grid_space_generator.cpp:126-150): z only; the
+    // GridSpace1D base sizes are (1, 1, dz) (grid_space_1d.cpp:12-19)
+    const Real min_z = dom.origin.z, max_z0 = dom.end.z;
+    const Index nz = static_cast<Index>(std::round((max_z0 - min_z) / dz));
+    if (nz <= 1) throw Error("GridSpace is too small");
+    Cube region(Vec3{-inf, -inf, min_z}, Vec3{inf, inf, nz * dz});
+    g.based[0] = 1;
+    g.based[1] = 1;
+    g.based[2] = dz;
+    g.e_node[2] = linspace(region.origin.z, region.end.z, nz + 1);
+    correctOne(2, g.based[2]);
+    oneCell(0);
+    oneCell(1);
+    return;
+  }
+  if (g.dim == 2) {
+    // generateGridSpace2D (grid_space_generator.cpp:152-181); the GridSpace2D
+    // constructor passes based_dx for both in-plane sizes and 1 for z
+    // (grid_space_2d.cpp:14-23)
+    Real lo[2] = {dom.origin.x, dom.origin.y};
+    Index nn[2];
+    for (int a = 0; a < 2; ++a) {
+      nn[a] = static_cast<Index>(std::round(((a == 0 ? dom.end.x : dom.end.y) - lo[a]) / d[a]));
+    }
+    if (nn[0] <= 1 || nn[1] <= 1) throw Error("GridSpace is too small");
+    Cube region(Vec3{lo[0], lo[1], -inf}, Vec3{nn[0] * dx, nn[1] * dy, inf});
+    g.based[0] = dx;
+    g.based[1] = dx;
+    g.based[2] = 1;
+    g.e_node[0] = linspace(region.origin.x, region.end.x, nn[0] + 1);
+    g.e_node[1] = linspace(region.origin.y, region.end.y, nn[1] + 1);
+    correctOne(0, g.based[0]);
+    correctOne(1, g.based[1]);
+    oneCell(2);
+    return;
   }
   // generateGridSpace3D (grid_space_generator.cpp:183-217)
   Real lo[3] = {dom.origin.x, dom.origin.y, dom.origin.z};
@@ -1668,25 +1786,7 @@ void Sim::generateGridSpace() {
     g.e_node[a] = linspace(region.origin[a], region.end[a], nn[a] + 1);
   }
   // GridSpace3D::correctGridSpace (grid_space_3d.cpp:111-137, grid_space.cpp:526-544)
-  for (int a = 0; a < 3; ++a) {
-    const Arr1& e = g.e_node[a];
-    Index m = e.size() - 1;
-    g.h_node[a] = Arr1(m);
-    g.e_size[a] = Arr1(m);
-    for (Index i = 0; i < m; ++i) {
-      g.h_node[a](i) = (e(i) + e(i + 1)) / 2;
-      g.e_size[a](i) = e(i + 1) - e(i);
-    }
-    std::vector<Real> tmp(m + 2);
-    tmp[0] = e(0) - d[a] / 2;
-    for (Index i = 0; i < m; ++i) tmp[i + 1] = g.h_node[a](i);
-    tmp[m + 1] = e(m) + d[a] / 2;
-    g.h_size[a] = Arr1(m + 1);
-    for (Index i = 0; i < m + 1; ++i) g.h_size[a](i) = tmp[i + 1] - tmp[i];
-    Real md = std::numeric_limits<Real>::max();
-    for (Index i = 0; i < m; ++i) md = std::min(md, g.e_size[a](i));
-    g.min_d[a] = md;
-  }
+  for (int a = 0; a < 3; ++a) correctOne(a, d[a]);
 }
 
 void Sim::generateEMF() {
@@ -1845,9 +1945,15 @@ void Sim::init() {  // simulation.cpp:319-371
   if (objects.empty()) throw Error("No object is added");
   generateGridSpace();
   generateEMF();
-  // TimeParam::calculateDt (time_param.cpp:10-13)
-  dt = cfl / (C_0 * std::sqrt(1.0 / (g.min_d[0] * g.min_d[0]) + 1.0 / (g.min_d[1] * g.min_d[1]) +
-                              1.0 / (g.min_d[2] * g.min_d[2])));
+  // Simulation::makeTimeParam (simulation.cpp:654-676) and
+  // TimeParam::calculateDt (time_param.cpp:10-21) for the grid's dimension
+  if (g.dim == 1)
+    dt = cfl / (C_0 * std::sqrt(1.0 / (g.min_d[2] * g.min_d[2])));
+  else if (g.dim == 2)
+    dt = cfl / (C_0 * std::sqrt(1.0 / (g.min_d[0] * g.min_d[0]) + 1.0 / (g.min_d[1] * g.min_d[1])));
+  else
+    dt = cfl / (C_0 * std::sqrt(1.0 / (g.min_d[0] * g.min_d[0]) + 1.0 / (g.min_d[1] * g.min_d[1]) +
+                                1.0 / (g.min_d[2] * g.min_d[2])));
   for (auto& o : objects) o->init(*this);
   for (auto& p : pmls) p->init(*this);
   for (auto& t : tfsfs) t->init(*this);
@@ -1856,6 +1962,13 @@ void Sim::init() {  // simulation.cpp:319-371
   for (auto& o : objects) o->correctUpdateCoefficient(*this);  // simulation.cpp:638-652
   for (auto& p : pmls) p->correctUpdateCoefficient(*this);
   buildDispersiveSpace();  // simulation.cpp:651
+  // Simulation::makeUpdator (simulation.cpp:688-743): the ADE updators are
+  // 3D only; any other dispersive case throws
+  if (ade && g.dim != 3) throw Error("don't support this type of updator yet.");
+  // the NF2FF surfaces (nffft.cpp:97-223) assume a 3D box; a 1D/2D grid has
+  // no closed surface to integrate over, so this restatement refuses it
+  if (g.dim != 3 && (!nf2ffs.empty() || !nf2ff_tds.empty()))
+    throw Error("NF2FF needs a 3D grid space");
   for (auto& m : monitors) m->init(*this);
   for (auto& n : networks) n->init(*this);  // simulation.cpp:353-355
   for (auto& n : nf2ffs) n->init(*this);  // simulation.cpp:350-358
@@ -1975,6 +2088,16 @@ struct Domain {
     const Index xs = task.x.s, ys = task.y.s, zs = task.z.s;
     const Index xe = task.x.e, ye = task.y.e, ze = task.z.e;
     const Index xs1 = xs == 0 ? 1 : xs, ys1 = ys == 0 ? 1 : ys, zs1 = zs == 0 ? 1 : zs;
+    if (s->g.dim == 1) {
+      // BasicUpdatorTEM::updateE (basic_updator.cpp:90-109): Ex on the one
+      // x/y edge, its Hz term written as zeros
+      for (Index k = zs1; k < ze; ++k)
+        f.ex(0, 0, k) = c.cexe(0, 0, k) * f.ex(0, 0, k) + c.cexhy(0, 0, k) * (f.hy(0, 0, k) - f.hy(0, 0, k - 1)) +
+                        0.0 * (0.0 - 0.0);
+      return;
+    }
+    // a 2D grid (nz = 1) runs BasicUpdatorTE::updateE (basic_updator_te.cpp:
+    // 23-38): the Ez update below; the Ex and Ey ranges [1, nz) are empty
     for (Index i = xs; i < xe; ++i)
       for (Index j = ys1; j < ye; ++j)
         for (Index k = zs1; k < ze; ++k)

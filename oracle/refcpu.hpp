@@ -229,6 +229,10 @@ struct ADEStorage {
 
 // ----------------------------------------------------------------- grid ---
 struct GridSpace {
+  // GridSpace::Dimension (grid_space.h): 3, or 2 (z infinite: one cell
+  // along z, grid_space_2d.cpp) or 1 (x and y infinite: one cell along each,
+  // grid_space_1d.cpp)
+  int dim = 3;
   Real based[3];
   Real min_d[3];
   Arr1 e_node[3], h_node[3], e_size[3], h_size[3];
@@ -381,6 +385,11 @@ struct PML {
 struct TFSF3D {
   Index x, y, z;
   Real theta, phi, psi;
+  // 3: TFSF3D; 2: TFSF2D(x, y, phi) = TFSF(x, y, 0, pi/2, phi, 0)
+  // (tfsf_2d.cpp:10-13); 1: TFSF1D(z, forward) = TFSF(0, 0, z, forward ? 0 :
+  // pi, 0, 0) (tfsf_1d.cpp:14-23).  The kind picks the corrector calls.
+  int kind = 3;
+  bool forward = true;
   Waveform waveform;
   Real st, ct, sp, cp, ss, cs;
   Vec3 k, transform_e, transform_h;

@@ -192,6 +192,29 @@ void parseSpec(Sim& s, const std::string& spec) {
       t->psi = num(is);
       t->waveform = parseWaveform(is);
       s.tfsfs.push_back(std::move(t));
+    } else if (d == "tfsf2d") {  // TFSF2D (tfsf_2d.cpp:10-13)
+      auto t = std::make_unique<TFSF3D>();
+      t->kind = 2;
+      t->x = static_cast<Index>(num(is));
+      t->y = static_cast<Index>(num(is));
+      t->z = 0;
+      t->theta = PI * 0.5;
+      t->phi = num(is);
+      t->psi = 0;
+      t->waveform = parseWaveform(is);
+      s.tfsfs.push_back(std::move(t));
+    } else if (d == "tfsf1d") {  // TFSF1D (tfsf_1d.cpp:14-23)
+      auto t = std::make_unique<TFSF3D>();
+      t->kind = 1;
+      t->x = 0;
+      t->y = 0;
+      t->z = static_cast<Index>(num(is));
+      t->forward = num(is) != 0;
+      t->theta = t->forward ? 0 : PI;
+      t->phi = 0;
+      t->psi = 0;
+      t->waveform = parseWaveform(is);
+      s.tfsfs.push_back(std::move(t));
     } else if (d == "current_source" || d == "voltage_source") {
       auto e = std::make_unique<SourceElement>();
       e->voltage = d == "voltage_source";

@@ -126,3 +126,24 @@ def test_torch_autograd_function(fp64_grad):
     val.backward()
     assert abs(val.item() / J - 1) <= 1e-12
     np.testing.assert_allclose(x.grad.numpy(), g, rtol=1e-12, atol=0)
+
+
+def test_torch_backward_of_an_earlier_forward(fp64_grad):
+    """Two objective() calls on one problem before the first backward (a line
+    search does this): the first graph's gradient still belongs to its own
+    eps_r, recomputed from the saved parameters, not to the second's."""
+    import torch
+
+    from xfdtd_core_amd.adjoint import DesignProblem
+
+    s, origin, eps, J, g = fp64_grad
+    prob = DesignProblem(s.spec(), origin, eps.shape, STEPS, checkpoint_interval=50, precision="f64")
+    x = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    y = torch.tensor(eps * 1.25, dtype=torch.float64, requires_grad=True)
+    vx = prob.objective(x)
+    vy = prob.objective(y)
+    assert vy.item() != vx.item()
+    vx.backward()
+    np.testing.assert_allclose(x.grad.numpy(), g, rtol=1e-12, atol=0)
+    vy.backward()
+    assert np.abs(y.grad.numpy() - g).max() > 0

@@ -25,15 +25,27 @@ class DesignProblem:
         self.steps = int(steps)
         self.interval = int(checkpoint_interval)
         self.last_value = None
+        # counts forward runs; the checkpoints on the device belong to the
+        # latest one only
+        self.generation = 0
 
     def run_forward(self, eps_r: np.ndarray) -> float:
         if tuple(eps_r.shape) != self.shape:
             raise ValueError(f"eps_r shape {eps_r.shape} != design shape {self.shape}")
         self.sim.set_design(self.origin, eps_r)
         self.last_value = self.sim.forward(self.steps, self.interval)
+        self.generation += 1
         return self.last_value
 
-    def run_backward(self) -> np.ndarray:
+    def run_backward(self, generation: int | None = None, eps_r: np.ndarray | None = None) -> np.ndarray:
+        """Gradient of the forward run `generation` (default: the latest).
+        The device keeps the checkpoints of the latest forward only, so an
+        older generation is run forward again from its saved eps_r first."""
+        if generation is not None and generation != self.generation:
+            if eps_r is None:
+                raise RuntimeError(f"backward of forward run {generation}, but the checkpoints on the device "
+                                   f"belong to run {self.generation} and no eps_r was saved to recompute them")
+            self.run_forward(eps_r)
         return self.sim.backward(self.shape)
 
     def objective(self, eps_r: torch.Tensor) -> torch.Tensor:
@@ -46,11 +58,13 @@ class _DesignObjective(torch.autograd.Function):
         e = eps_r.detach().to("cpu", torch.float64).numpy()
         value = problem.run_forward(e)
         ctx.problem = problem
+        ctx.generation = problem.generation
+        ctx.eps_r = e
         ctx.dtype = eps_r.dtype
         return torch.tensor(value, dtype=torch.float64, device=eps_r.device)
 
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
-        g = ctx.problem.run_backward()
+        g = ctx.problem.run_backward(ctx.generation, ctx.eps_r)
         gt = torch.from_numpy(g).to(device=grad_out.device, dtype=torch.float64)
         return (grad_out * gt).to(ctx.dtype), None

@@ -18,6 +18,8 @@ renders the scenario as line-oriented text, one directive per line:
                  | debye EPS_INF P EPS_STATIC[P] TAU[P]
     pml THICKNESS DIR [ORDER SIGMA_RATIO ALPHA_MIN ALPHA_MAX KAPPA_MAX]
     tfsf3d X Y Z THETA PHI PSI WAVEFORM...
+    tfsf2d X Y PHI WAVEFORM...
+    tfsf1d Z FORWARD(0|1) WAVEFORM...
     current_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
     voltage_source NAME OX OY OZ SX SY SZ DIR R WAVEFORM...
     resistor NAME OX OY OZ SX SY SZ AXIS R
@@ -282,6 +284,33 @@ class TFSF3D:
     def line(self):
         return " ".join(["tfsf3d", str(int(self.x)), str(int(self.y)), str(int(self.z)),
                          _n(self.theta), _n(self.phi), _n(self.psi)] + self.waveform.tokens())
+
+
+@dataclass
+class TFSF2D:
+    """``xfdtd::TFSF2D(distance_x, distance_y, phi, waveform)`` (waveform_source/tfsf_2d.h): the
+    plane wave of a 2D (TE) grid, TFSF with theta = pi/2, psi = 0 (tfsf_2d.cpp:10-13)."""
+
+    x: int
+    y: int
+    phi: float
+    waveform: Waveform
+
+    def line(self):
+        return " ".join(["tfsf2d", str(int(self.x)), str(int(self.y)), _n(self.phi)] + self.waveform.tokens())
+
+
+@dataclass
+class TFSF1D:
+    """``xfdtd::TFSF1D(z, forward, waveform)`` (waveform_source/tfsf_1d.h): the plane wave of a
+    1D (TEM) grid travelling towards +z (forward) or -z (tfsf_1d.cpp:14-23)."""
+
+    z: int
+    forward: bool
+    waveform: Waveform
+
+    def line(self):
+        return " ".join(["tfsf1d", str(int(self.z)), "1" if self.forward else "0"] + self.waveform.tokens())
 
 
 @dataclass
