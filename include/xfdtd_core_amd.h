@@ -62,6 +62,11 @@ typedef struct {
   const double *e_inc, *h_inc;           /* nt x aux, nt x (aux-1) */
   double transform_e[3], transform_h[3];
   double ca[3], cb[3];                   /* TFSF::cax..caz, cbx..cbz */
+  /* corrector calls per half-step: 0 or 3 TFSF3DCorrector (the 12 faces x
+   * components of tfsf_corrector.cpp:177-232), 2 TFSF2DCorrector (:153-175),
+   * 1 TFSF1DCorrector (:124-151) on the ZN face (forward) or ZP */
+  int32_t kind;
+  int32_t forward;
 } xfd_tfsf_desc;
 
 /* Per-step lumped-element correction after the E update, in object add
@@ -188,6 +193,10 @@ typedef struct {
   const xfd_line_desc* line;
   int32_t n_tdnf;
   const xfd_tdnf_desc* tdnf;
+  /* grid dimension (GridSpace::Dimension): 0 or 3 a 3D grid; 2 a 2D grid
+   * (nz = 1: BasicUpdatorTE, Ez from Hx/Hy); 1 a 1D grid (nx = ny = 1:
+   * BasicUpdatorTEM, Ex from Hy) */
+  int32_t dim;
 } xfd_desc;
 
 typedef struct xfd_ctx xfd_ctx;

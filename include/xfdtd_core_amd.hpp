@@ -54,6 +54,11 @@ class XFDTDSimulationException : public XFDTDException {
  public:
   using XFDTDException::XFDTDException;
 };
+// include/xfdtd/grid_space/grid_space.h
+class XFDTDGridSpaceException : public XFDTDException {
+ public:
+  using XFDTDException::XFDTDException;
+};
 
 template <class T>
 struct Array1D {
@@ -381,20 +386,55 @@ class WaveformSource {
   std::unique_ptr<Waveform> _waveform;
 };
 
-class TFSF3D : public WaveformSource {
+// include/xfdtd/waveform_source/tfsf.h: the plane-wave source; the box is
+// the grid shrunk by (x, y, z) cells on each side.  TFSF3D / TFSF2D / TFSF1D
+// pick the corrector calls of their grid's dimension.
+class TFSF : public WaveformSource {
  public:
-  TFSF3D(std::size_t x, std::size_t y, std::size_t z, Real theta, Real phi, Real psi,
-         std::unique_ptr<Waveform> waveform);
+  enum class Kind { ONE = 1, TWO = 2, THREE = 3 };
+  TFSF(std::size_t x, std::size_t y, std::size_t z, Real theta, Real phi, Real psi,
+       std::unique_ptr<Waveform> waveform, Kind kind)
+      : WaveformSource(std::move(waveform)), _x(x), _y(y), _z(z), _theta(theta), _phi(phi), _psi(psi),
+        _kind(kind) {}
   std::size_t x() const { return _x; }
   std::size_t y() const { return _y; }
   std::size_t z() const { return _z; }
   Real theta() const { return _theta; }
   Real phi() const { return _phi; }
   Real psi() const { return _psi; }
+  Kind kind() const { return _kind; }
 
  private:
   std::size_t _x, _y, _z;
   Real _theta, _phi, _psi;
+  Kind _kind;
+};
+
+// include/xfdtd/waveform_source/tfsf_3d.h
+class TFSF3D : public TFSF {
+ public:
+  TFSF3D(std::size_t x, std::size_t y, std::size_t z, Real theta, Real phi, Real psi,
+         std::unique_ptr<Waveform> waveform);
+};
+
+// include/xfdtd/waveform_source/tfsf_2d.h: a 2D (TE) grid's plane wave,
+// theta = pi/2 and psi = 0 (src/waveform_source/tfsf_2d.cpp:10-13)
+class TFSF2D : public TFSF {
+ public:
+  TFSF2D(std::size_t distance_x, std::size_t distance_y, Real phi, std::unique_ptr<Waveform> waveform);
+};
+
+// include/xfdtd/waveform_source/tfsf_1d.h: a 1D (TEM) grid's plane wave
+// towards +z (forward) or -z (src/waveform_source/tfsf_1d.cpp:14-23)
+class TFSF1D : public TFSF {
+ public:
+  TFSF1D(Index z, bool forward, std::unique_ptr<Waveform> waveform);
+  bool forward() const { return _forward; }
+  Index start() const { return _start; }
+
+ private:
+  bool _forward;
+  Index _start;
 };
 
 struct EMF {

@@ -1925,8 +1925,12 @@ void Sim::calculateCoefficient() {  // calculation_param.cpp:170-264
         for (Index j = 0; j < ext[1]; ++j)
           for (Index k = 0; k < ext[2]; ++k) {
             Index id[3] = {i, j, k};
-            Real da = (*sz[A])(id[A]);
-            Real db = (*sz[B])(id[B]);
+            // xt::meshgrid of the size arrays broadcasts against the field
+            // extent: the one-cell axis of a 1D/2D grid holds a single size
+            // (1) that every index along it takes, also where the component
+            // extends to 2 (Ex/Ey/Hz along z in 2D, Ey/Ez/Hx along x in 1D)
+            Real da = (*sz[A])(sz[A]->size() == 1 ? 0 : id[A]);
+            Real db = (*sz[B])(sz[B]->size() == 1 ? 0 : id[B]);
             Real e = m(i, j, k), s = sg(i, j, k);
             if (!h) {
               cs(i, j, k) = (2 * e - dt * s) / (2 * e + dt * s);

@@ -84,6 +84,13 @@ CASES = {
                                                                 kinds=("debye", "debye")).spec(),
     "adjoint_design_region": lambda: scenarios.adjoint_design(n_total=40, pml=6, design=8, patch_gap=4,
                                                               tfsf_margin=2)[0].spec(),
+    # 2D (TE) grids: z infinite, one cell along z, based sizes (dx, dx, 1)
+    "te_pec_cylinder": lambda: scenarios.cylinder_scatter_2d(nx=60, ny=64, movie_interval=10, probes=3, tfsf=15,
+                                                             radius=0.01).spec(),
+    "te_dielectric_cylinder": lambda: scenarios.cylinder_scatter_2d(nx=61, ny=50, movie_interval=0, probes=2,
+                                                                    eps_r=3.5, tfsf=12, radius=0.012).spec(),
+    # 1D (TEM) grid: x and y infinite, based sizes (1, 1, dz)
+    "tem_slab_line": lambda: scenarios.slab_line_1d(nz=120, slab=(40, 60), sigma=0.02).spec(),
 }
 
 
@@ -124,14 +131,39 @@ def test_nf2ff_surfaces_match_oracle(dist):
         assert ref.nf2ff_surface(0, d) == prod.nf2ff_surface(0, d), d
 
 
-def test_two_dimensional_domain_is_refused():
-    """An object of infinite z extent makes the reference build a 2D grid
-    space (grid_space_generator.cpp:23-32) with the TE updator; this library
-    steps 3D grids only and says so instead of sizing a grid from inf."""
-    dl = 1e-3
-    s = S.Simulation(dl, dl, 1.0, 0.8)
-    s.addObject(S.Object("domain", S.Cube(S.Vector(-0.01, -0.01, -np.inf), S.Vector(0.02, 0.02, np.inf)),
-                         S.Material.createAir()))
-    prod = native.Simulation(s.spec())
-    with pytest.raises(RuntimeError, match="1D/2D"):
-        prod.precompute(4)
+def test_dimension_errors_match_oracle():
+    """The reference's refusals on 1D/2D grids (grid_space_generator.cpp:
+    52-65, simulation.cpp:688-743): PML along a one-cell axis, a grid that is
+    neither 1D, 2D nor 3D, and dispersive material outside 3D -- raised by
+    the product as by the oracle."""
+    inf = np.inf
+
+    def plane():
+        s = S.Simulation(1e-3, 1e-3, 1.0, 0.8)
+        s.addObject(S.Object("d", S.Cube(S.Vector(0, 0, -inf), S.Vector(0.02, 0.016, inf)), S.Material.createAir()))
+        return s
+
+    def line():
+        s = S.Simulation(1e-3, 1e-3, 1e-3, 0.9)
+        s.addObject(S.Object("d", S.Cube(S.Vector(-inf, -inf, 0), S.Vector(inf, inf, 0.03)), S.Material.createAir()))
+        return s
+
+    cases = []
+    s = plane()
+    s.addBoundary(S.PML(4, "ZN"))
+    cases.append((s, "X or Y"))
+    s = line()
+    s.addBoundary(S.PML(4, "XN"))
+    cases.append((s, "Z direction"))
+    s = S.Simulation(1e-3, 1e-3, 1e-3, 0.9)
+    s.addObject(S.Object("d", S.Cube(S.Vector(-inf, 0, 0), S.Vector(inf, 0.02, 0.02)), S.Material.createAir()))
+    cases.append((s, "undefined"))
+    s = line()
+    s.addObject(S.Object("dr", S.Cube(S.Vector(-inf, -inf, 0.01), S.Vector(inf, inf, 0.01)),
+                         S.DrudeMedium.makeDrudeMedium("dr", 1.0, [1e11], [1e9])))
+    cases.append((s, "updator"))
+    for sim, msg in cases:
+        with pytest.raises(Exception, match=msg):
+            RefSim(sim.spec()).init(1)
+        with pytest.raises(RuntimeError, match=msg):
+            native.Simulation(sim.spec()).precompute(1)

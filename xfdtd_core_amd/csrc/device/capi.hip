@@ -281,6 +281,8 @@ struct xfd_ctx {
 
   // CPML applied inside the Yee sweeps (see cpmlFused): the slab arguments
   // of each half-step as device arrays
+  // grid dimension: 3, 2 (TE, nz = 1) or 1 (TEM, nx = ny = 1)
+  int dim = 3;
   bool fused = false;
   int n_fused = 0;  // slabs [0, n_fused) run inside the sweeps, the rest separately
   // the z slabs run inside the sweeps (ZCpml), the rest separately
@@ -451,8 +453,25 @@ void tfsfRecords(const xfd::Grid& g, const xfd_tfsf_desc& d, int64_t e_off, int6
                                    {0, 1}, {1, 1}, {4, 0}, {5, 0}, {2, 0}, {3, 0}};
   static const Call h_calls[12] = {{0, 1}, {1, 1}, {2, 0}, {3, 0}, {4, 0}, {5, 0},
                                    {0, 2}, {1, 2}, {2, 2}, {3, 2}, {4, 1}, {5, 1}};
+  // TFSF2DCorrector (tfsf_corrector.cpp:153-175): the four x/y faces, Ez
+  // and Hy / Hx; TFSF1DCorrector (:124-151): the ZN (forward) or ZP face,
+  // Ex and Hy
+  static const Call e_calls_2d[4] = {{0, 2}, {1, 2}, {2, 2}, {3, 2}};
+  static const Call h_calls_2d[4] = {{0, 1}, {1, 1}, {2, 0}, {3, 0}};
+  const Call e_call_1d[1] = {{d.forward ? 4 : 5, 0}};
+  const Call h_call_1d[1] = {{d.forward ? 4 : 5, 1}};
   const Call* calls = is_h ? h_calls : e_calls;
-  for (int ci = 0; ci < 12; ++ci) {
+  int n_calls = 12;
+  if (d.kind == 2) {
+    calls = is_h ? h_calls_2d : e_calls_2d;
+    n_calls = 4;
+  } else if (d.kind == 1) {
+    calls = is_h ? h_call_1d : e_call_1d;
+    n_calls = 1;
+  } else {
+    require(d.kind == 0 || d.kind == 3, "bad TFSF kind");
+  }
+  for (int ci = 0; ci < n_calls; ++ci) {
     const int dir = calls[ci].dir, fcomp = calls[ci].comp;
     const int xyz = dir / 2;
     const bool neg = dir % 2 == 0;
@@ -721,6 +740,8 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   c.nseg_h = rh.empty() ? 0 : static_cast<int64_t>(sh.size()) - 1;
   const bool records_meet_slabs = recordsMeetSlabs(c, re, true) || recordsMeetSlabs(c, rh, false);
 
+  require(!d.ade || c.dim == 3, "ADE materials need a 3D grid");
+  require((d.n_nf2ff == 0 && d.n_tdnf == 0) || c.dim == 3, "NF2FF needs a 3D grid");
   if (d.ade) {
     const auto& A = *d.ade;
     require(A.scheme >= XFD_ADE_DRUDE && A.scheme <= XFD_ADE_MLORENTZ, "bad ADE scheme");
@@ -985,7 +1006,7 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     // boundary order, so the slabs' relative order is kept.
     int prefix = 0;
     while (prefix < static_cast<int>(c.slabs.size()) && c.slabs[prefix]->axis != 2) ++prefix;
-    c.fused = !off && prefix > 0 && c.ade_scheme < 0 && !records_meet_slabs;
+    c.fused = !off && prefix > 0 && c.ade_scheme < 0 && !records_meet_slabs && c.dim == 3;
     c.n_fused = c.fused ? prefix : 0;
     if (c.fused) {
       const int64_t stride[3] = {g.plane, g.pz, 1};
@@ -1035,7 +1056,7 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     for (auto& sl : c.slabs)
       if (sl->axis < 2) ++per_axis[sl->axis];
     c.zfused = !zoff && (zforce || (zwide && c.elem == 4)) && off && !zq.empty() && zq.size() <= 2 && disjoint &&
-               per_axis[0] <= 4 && per_axis[1] <= 4 && c.ade_scheme < 0 && !records_meet_slabs;
+               per_axis[0] <= 4 && per_axis[1] <= 4 && c.ade_scheme < 0 && !records_meet_slabs && c.dim == 3;
     c.zslab.assign(c.slabs.size(), 0);
     if (c.zfused)
       for (int q : zq) c.zslab[q] = 1;
@@ -1182,10 +1203,16 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   auto inSweep = [&](size_t q) { return static_cast<int>(q) < c.n_fused; };
   for (int64_t t = t0; t < t0 + n; ++t) {
     c.timed(K_UPDATE_H, [&] {
-      if (c.classed)
+      if (c.dim != 3) {
+        if (c.classed)
+          xfd::Launch<T>::update_low(st, c.g, F, K, c.dim, true);
+        else
+          xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, true);
+      } else if (c.classed) {
         xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH, ZH);
-      else
+      } else {
         xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk, PH, ZH);
+      }
     });
     if (c.nseg_h)
       c.timed(K_REC_H, [&] {
@@ -1202,7 +1229,12 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
           xfd::Launch<T>::cpml(st, c.g, F, sh[q], false);
       });
     c.timed(K_UPDATE_E, [&] {
-      if (c.ade_scheme >= 0) {
+      if (c.dim != 3) {
+        if (c.classed)
+          xfd::Launch<T>::update_low(st, c.g, F, K, c.dim, false);
+        else
+          xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, false);
+      } else if (c.ade_scheme >= 0) {
         if (c.classed)
           xfd::Launch<T>::update_e_ade(st, c.g, F, K, A, c.ade_scheme, c.ichunk);
         else
@@ -1758,10 +1790,15 @@ int xfd_create(const xfd_desc* d, xfd_ctx** out) {
   return guard([&] {
     require(d != nullptr && out != nullptr, "null argument");
     *out = nullptr;
-    require(d->nx > 1 && d->ny > 1 && d->nz > 1, "grid too small");
+    const int dim = d->dim == 0 ? 3 : d->dim;
+    require(dim >= 1 && dim <= 3, "bad grid dimension");
+    if (dim == 3) require(d->nx > 1 && d->ny > 1 && d->nz > 1, "grid too small");
+    if (dim == 2) require(d->nx > 1 && d->ny > 1 && d->nz == 1, "a 2D grid has nx, ny > 1 and nz = 1");
+    if (dim == 1) require(d->nx == 1 && d->ny == 1 && d->nz > 1, "a 1D grid has nx = ny = 1 and nz > 1");
     require(d->dtype == XFD_F64 || d->dtype == XFD_F32, "bad dtype");
     require(d->n_cpml >= 0 && d->n_lumped >= 0 && d->n_tfsf >= 0 && d->n_gather >= 0, "bad counts");
     auto c = std::make_unique<xfd_ctx>();
+    c->dim = dim;
     c->dtype = d->dtype;
     c->elem = d->dtype == XFD_F64 ? 8 : 4;
     xfd::Grid& g = c->g;
@@ -1770,7 +1807,9 @@ int xfd_create(const xfd_desc* d, xfd_ctx** out) {
     g.nz = d->nz;
     g.NY = d->ny + 1;
     const int64_t line = 128 / static_cast<int64_t>(c->elem);
-    g.pz = ((d->nz + 1) + line - 1) / line * line;
+    // a 2D grid keeps k's two entries unpadded, so its rows along j are
+    // contiguous (pitch 2) for the TE kernels' lanes
+    g.pz = dim == 2 ? 2 : ((d->nz + 1) + line - 1) / line * line;
     g.plane = static_cast<int64_t>(g.NY) * g.pz;
     g.total = static_cast<int64_t>(d->nx + 1) * g.plane;
     const int64_t bxy = ((g.nz + xfd::kLanesK - 1) / xfd::kLanesK) * ((g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ);
@@ -1877,6 +1916,7 @@ int xfd_kernel_stats(xfd_ctx* c, int32_t k, const char** name, int64_t* launches
 int xfd_forward_checkpointed(xfd_ctx* c, int64_t nt, int64_t interval) {
   return guard([&] {
     require(c != nullptr, "null context");
+    require(c->dim == 3, "the adjoint form needs a 3D grid");
     if (c->dtype == XFD_F64)
       forwardCheckpointed<double>(*c, nt, interval);
     else
@@ -1895,6 +1935,7 @@ int xfd_design_edges(const int32_t origin[3], const int32_t size[3], int64_t cou
 int xfd_backward(xfd_ctx* c, const xfd_adjoint_desc* d, double* grad) {
   return guard([&] {
     require(c != nullptr && d != nullptr, "bad argument");
+    require(c->dim == 3, "the adjoint form needs a 3D grid");
     if (c->dtype == XFD_F64)
       backwardTyped<double>(*c, *d, grad);
     else
@@ -1959,6 +2000,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
     *out = nullptr;
     auto b = std::make_unique<xfd_batch>();
     b->dtype = descs[0].dtype;
+    for (int i = 0; i < n; ++i) require(descs[i].dim == 0 || descs[i].dim == 3, "batched cases must be 3D grids");
     for (int i = 0; i < n; ++i) {
       require(descs[i].dtype == b->dtype, "every case of a batch has the same dtype");
       xfd_ctx* c = nullptr;

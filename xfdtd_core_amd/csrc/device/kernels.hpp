@@ -351,6 +351,65 @@ __global__ __launch_bounds__(kTileThreads) void k_update_e(Grid g, FieldPtrs<T> 
   sweepE<T, Coef, Fused, ZF>(g, F, C, bx, by, bz, ichunk, P, Z);
 }
 
+// 2D (TE) grid, nz = 1, in a layout of pitch pz = 2 along k (k = 1 holds
+// the far face of Ex, Ey and Hz): lanes run along j, the contiguous axis
+// once k has two entries, and work-group rows along i.
+//
+// BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41) over
+// [0,nx) x [0,ny) x {0}: all three components with every term, as the
+// reference evaluates them (the Ex, Ey and Hz terms are zeros of a TE grid).
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_te_h(Grid g, FieldPtrs<T> F, Coef C) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (j >= g.ny || i >= g.nx) return;
+  const int64_t pz = g.pz, pl = g.plane;
+  const int64_t idx = (static_cast<int64_t>(i) * g.NY + j) * pz;
+  const T* __restrict__ ex = F.f[0];
+  const T* __restrict__ ey = F.f[1];
+  const T* __restrict__ ez = F.f[2];
+  T s, a, b;
+  C.get(3, idx, s, a, b);
+  const T vx = s * F.f[3][idx] + a * (ey[idx + 1] - ey[idx]) + b * (ez[idx + pz] - ez[idx]);
+  C.get(4, idx, s, a, b);
+  const T vy = s * F.f[4][idx] + a * (ez[idx + pl] - ez[idx]) + b * (ex[idx + 1] - ex[idx]);
+  C.get(5, idx, s, a, b);
+  const T vz = s * F.f[5][idx] + a * (ex[idx + pz] - ex[idx]) + b * (ey[idx + pl] - ey[idx]);
+  F.f[3][idx] = vx;
+  F.f[4][idx] = vy;
+  F.f[5][idx] = vz;
+}
+
+// BasicUpdatorTE::updateE (src/updator/basic_updator_te.cpp:23-38): Ez over
+// [1,nx) x [1,ny) x {0}; Ex and Ey (ranges [1,nz) along k) are not updated.
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_te_e(Grid g, FieldPtrs<T> F, Coef C) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (j < 1 || j >= g.ny || i < 1 || i >= g.nx) return;
+  const int64_t pz = g.pz, pl = g.plane;
+  const int64_t idx = (static_cast<int64_t>(i) * g.NY + j) * pz;
+  const T* __restrict__ hx = F.f[3];
+  const T* __restrict__ hy = F.f[4];
+  T s, a, b;
+  C.get(2, idx, s, a, b);
+  F.f[2][idx] = s * F.f[2][idx] + a * (hx[idx] - hx[idx - pz]) + b * (hy[idx] - hy[idx - pl]);
+}
+
+// BasicUpdatorTEM::updateE (src/updator/basic_updator.cpp:90-109) on a 1D
+// grid (nx = ny = 1): Ex(0,0,k) for k in [1,nz), its Hz term passed as
+// zeros, eNext(cexe, ex, cexhy, hy(k), hy(k-1), 0, 0, 0).
+template <class T, class Coef>
+__global__ __launch_bounds__(256) void k_tem_e(Grid g, FieldPtrs<T> F, Coef C) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < 1 || k >= g.nz) return;
+  const int64_t idx = k;
+  const T* __restrict__ hy = F.f[4];
+  T s, a, b;
+  C.get(0, idx, s, a, b);
+  F.f[0][idx] = s * F.f[0][idx] + a * (hy[idx] - hy[idx - 1]) + T(0) * (T(0) - T(0));
+}
+
 // Single-cell time series (FieldTimeMonitor sampling): one launch per step
 // for all of them; probe p writes out[p * stride + t].
 struct Probe {
@@ -1057,6 +1116,12 @@ struct Launch {
                        const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
                        const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
+  // the Yee half-steps of a 2D (TE, dim 2) or 1D (TEM, dim 1) grid; the 1D
+  // H half-step is the 3D sweep (update_h)
+  static void update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int dim,
+                         bool is_h);
+  static void update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int dim,
+                         bool is_h);
   static void cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
   static void cpml_zrest(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a,
                          const ZFrame& fr, bool is_e);

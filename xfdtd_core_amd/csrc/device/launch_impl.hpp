@@ -70,6 +70,30 @@ void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, co
                          const CpmlList<T>& P, const ZCpml<T>& Z) {
   launchUpdate<T>(false, s, g, F, C, ichunk, P, Z);
 }
+template <class T, class Coef>
+void launchLow(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int dim, bool is_h) {
+  if (dim == 2) {
+    const dim3 grid(blocks_for(g.ny, 64), blocks_for(g.nx, 4)), block(64, 4);
+    if (is_h)
+      hipLaunchKernelGGL((k_te_h<T, Coef>), grid, block, 0, s, g, F, C);
+    else
+      hipLaunchKernelGGL((k_te_e<T, Coef>), grid, block, 0, s, g, F, C);
+  } else if (!is_h) {
+    hipLaunchKernelGGL((k_tem_e<T, Coef>), dim3(blocks_for(g.nz, 256)), dim3(256), 0, s, g, F, C);
+  } else {
+    launchUpdate<T>(true, s, g, F, C, 1, CpmlList<T>{}, ZCpml<T>{});
+  }
+}
+template <class T>
+void Launch<T>::update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int dim,
+                           bool is_h) {
+  launchLow<T>(s, g, F, C, dim, is_h);
+}
+template <class T>
+void Launch<T>::update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int dim,
+                           bool is_h) {
+  launchLow<T>(s, g, F, C, dim, is_h);
+}
 template <class T>
 void Launch<T>::cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e) {
   const FacePieces f = facePieces(g, a);

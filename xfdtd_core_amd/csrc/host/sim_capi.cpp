@@ -189,6 +189,14 @@ std::unique_ptr<SimHandle> build(const std::string& spec) {
                   z = static_cast<std::size_t>(t.num());
       Real th = t.num(), ph = t.num(), ps = t.num();
       s.addWaveformSource(std::make_shared<TFSF3D>(x, y, z, th, ph, ps, waveform(t)));
+    } else if (d == "tfsf2d") {
+      std::size_t x = static_cast<std::size_t>(t.num()), y = static_cast<std::size_t>(t.num());
+      Real ph = t.num();
+      s.addWaveformSource(std::make_shared<TFSF2D>(x, y, ph, waveform(t)));
+    } else if (d == "tfsf1d") {
+      Index z = static_cast<Index>(t.num());
+      bool forward = t.num() != 0;
+      s.addWaveformSource(std::make_shared<TFSF1D>(z, forward, waveform(t)));
     } else if (d == "current_source" || d == "voltage_source") {
       std::string name = t.word();
       auto cb = cube(t);

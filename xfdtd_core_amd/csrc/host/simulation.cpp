@@ -230,7 +230,11 @@ PML::PML(int t, Axis::Direction d, int o, Real sr, Real amin, Real amax, Real km
     : _thickness(t), _direction(d), _order(o), _sigma_ratio(sr), _alpha_min(amin), _alpha_max(amax),
       _kappa_max(kmax) {}
 TFSF3D::TFSF3D(std::size_t x, std::size_t y, std::size_t z, Real th, Real ph, Real ps, std::unique_ptr<Waveform> w)
-    : WaveformSource(std::move(w)), _x(x), _y(y), _z(z), _theta(th), _phi(ph), _psi(ps) {}
+    : TFSF(x, y, z, th, ph, ps, std::move(w), Kind::THREE) {}
+TFSF2D::TFSF2D(std::size_t x, std::size_t y, Real ph, std::unique_ptr<Waveform> w)  // tfsf_2d.cpp:10-13
+    : TFSF(x, y, 0, constant::PI * 0.5, ph, 0, std::move(w), Kind::TWO) {}
+TFSF1D::TFSF1D(Index z, bool forward, std::unique_ptr<Waveform> w)  // tfsf_1d.cpp:14-23
+    : TFSF(0, 0, z, forward ? 0 : constant::PI, 0, 0, std::move(w), Kind::ONE), _forward(forward), _start(z) {}
 
 Array3D<Real>& EMF::field(Field f) {
   switch (f) {
@@ -518,6 +522,10 @@ struct Simulation::Impl {
   std::size_t N[3] = {0, 0, 0};
   std::vector<Real> e_node[3], h_node[3], e_size[3], h_size[3];
   Real min_d[3];
+  // GridSpace::Dimension: 3, 2 (z infinite, one cell along z) or 1 (x and
+  // y infinite, one cell along each); the based cell sizes of the grid
+  int dim = 3;
+  Real based[3] = {0, 0, 0};
   Real dt = 0;
   Index steps = 0;
   bool initialised = false;
@@ -617,9 +625,17 @@ struct Simulation::Impl {
   }
 
   // ---------------------------------------------------------------- grid --
-  Real tolerance() const { return std::min({min_d[0], min_d[1], min_d[2]}) / 1000; }  // grid_space.cpp:459-468
+  Real tolerance() const {  // grid_space.cpp:459-468, per dimension
+    if (dim == 1) return min_d[2] / 1000;
+    if (dim == 2) return std::min(min_d[0], min_d[1]) / 1000;
+    return std::min({min_d[0], min_d[1], min_d[2]}) / 1000;
+  }
 
   std::size_t nearest(int a, Real x) const {  // xt::argmin(abs(e_node - x)), first minimum
+    // the one-cell axes of a 1D/2D grid: GridSpace2D::handleTransformZ
+    // (grid_space_2d.cpp:116-122), GridSpace1D::handleTransformX/Y
+    // (grid_space_1d.cpp:117-135): 0 below +inf, else 1
+    if ((dim == 2 && a == 2) || (dim == 1 && a < 2)) return x < std::numeric_limits<Real>::infinity() ? 0 : 1;
     const auto& e = e_node[a];
     std::size_t best = 0;
     Real bv = std::abs(e[0] - x);
@@ -643,7 +659,9 @@ struct Simulation::Impl {
 
   void buildGrid() {
     // Shape::makeWrappedCube (shape.cpp:8-38) -> extendDomain
-    // (grid_space_generator.cpp:36-99) -> generateGridSpace3D (:183-217)
+    // (grid_space_generator.cpp:36-99) -> generateUniformGridSpace (:16-34):
+    // 3D (:183-217), 2D when only z is infinite (:152-181), 1D when x and y
+    // are (:126-150; decideDimension :101-124)
     if (objects.empty()) throw XFDTDSimulationException("No object is added");
     Real lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
@@ -658,15 +676,27 @@ struct Simulation::Impl {
       }
     }
     Cube dom(Vector(lo[0], lo[1], lo[2]), Vector(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]));
+    auto dimension = [](const Cube& c) {
+      const bool inf[3] = {std::isinf(c.size()[0]), std::isinf(c.size()[1]), std::isinf(c.size()[2])};
+      if (inf[0] && inf[1] && !inf[2]) return 1;
+      if (!inf[0] && !inf[1] && inf[2]) return 2;
+      if (!inf[0] && !inf[1] && !inf[2]) return 3;
+      return 0;
+    };
     bool any_pml = false;
     Real plo[3] = {dom.origin()[0], dom.origin()[1], dom.origin()[2]};
     Real phi[3] = {dom.end()[0], dom.end()[1], dom.end()[2]};
+    const int dim0 = dimension(dom);
     for (auto& b : boundaries) {
       auto* p = dynamic_cast<PML*>(b.get());
       if (!p) continue;
       any_pml = true;
-      if (p->thickness() < 0) continue;
       const int a = ax(dirAxis(p->direction()));
+      // grid_space_generator.cpp:52-65: a 1D grid takes PML along z only, a
+      // 2D grid along x and y only
+      if (dim0 == 1 && a != 2) throw XFDTDGridSpaceException("PML has to be in Z direction");
+      if (dim0 == 2 && a == 2) throw XFDTDGridSpaceException("PML has to be in X or Y direction");
+      if (p->thickness() < 0) continue;
       if (dirNeg(p->direction()))
         plo[a] -= p->thickness() * d[a];
       else
@@ -674,22 +704,41 @@ struct Simulation::Impl {
     }
     if (any_pml)
       dom = Cube(Vector(plo[0], plo[1], plo[2]), Vector(phi[0] - plo[0], phi[1] - plo[1], phi[2] - plo[2]));
-    Real mn[3], mx[3];
-    for (int a = 0; a < 3; ++a)
-      // an infinite extent makes the reference build a 1D/2D grid space
-      // (grid_space_generator.cpp), whose TE/TEM updators this library lacks
-      if (!std::isfinite(dom.origin()[a]) || !std::isfinite(dom.end()[a]))
-        throw XFDTDSimulationException("1D/2D grid spaces (objects of infinite extent) are not supported");
+    dim = dimension(dom);
+    if (dim == 0) throw XFDTDGridSpaceException("GridSpace dimension is undefined");
+    // based sizes: (dx, dy, dz) in 3D; GridSpace2D passes based_dx for both
+    // in-plane axes and 1 for z (grid_space_2d.cpp:14-23); GridSpace1D
+    // (1, 1, dz) (grid_space_1d.cpp:12-19)
+    const bool cells[3] = {dim != 1, dim != 1, dim != 2};
+    based[0] = dim == 1 ? 1 : d[0];
+    based[1] = dim == 1 ? 1 : (dim == 2 ? d[0] : d[1]);
+    based[2] = dim == 2 ? 1 : d[2];
     for (int a = 0; a < 3; ++a) {
-      mn[a] = dom.origin()[a];
-      mx[a] = dom.end()[a];
-      N[a] = static_cast<std::size_t>(std::round((mx[a] - mn[a]) / d[a]));
-      if (N[a] <= 1) throw XFDTDException("GridSpace is too small");
+      if (!cells[a]) continue;
+      N[a] = static_cast<std::size_t>(std::round((dom.end()[a] - dom.origin()[a]) / d[a]));
     }
-    for (int a = 0; a < 3; ++a) mx[a] = mn[a] + d[a] * N[a];
-    Cube region(Vector(mn[0], mn[1], mn[2]), Vector(mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]));
+    for (int a = 0; a < 3; ++a)
+      if (cells[a] && N[a] <= 1) throw XFDTDGridSpaceException("GridSpace is too small");
+    const Real inf = std::numeric_limits<Real>::infinity();
     for (int a = 0; a < 3; ++a) {
-      e_node[a] = linspaceAssign(region.origin()[a], region.end()[a], N[a] + 1);
+      if (!cells[a]) {
+        // the one-cell axis (grid_space_2d.cpp:96-106, grid_space_1d.cpp:
+        // 64-75): nodes {-inf, inf}, one cell of size 1 centred at 0
+        N[a] = 1;
+        e_node[a] = {-inf, inf};
+        h_node[a] = {0.0};
+        e_size[a] = {1.0};
+        h_size[a] = {1.0};
+        min_d[a] = 1;
+        continue;
+      }
+      // region end: min + (max - min) with max = min + d n in 3D
+      // (generateGridSpace3D), min + n d in 1D/2D (generateGridSpace1D/2D)
+      const Real mn = dom.origin()[a];
+      const Real end = dim == 3 ? mn + ((mn + d[a] * N[a]) - mn) : mn + static_cast<Real>(N[a]) * d[a];
+      e_node[a] = linspaceAssign(mn, end, N[a] + 1);
+      // GridSpace::correctGridSpaceForOne (grid_space.cpp:517-544) with the
+      // axis' based size for the half cells beyond the end nodes
       const auto& e = e_node[a];
       const std::size_t m = N[a];
       h_node[a].resize(m);
@@ -699,17 +748,23 @@ struct Simulation::Impl {
         h_node[a][i] = (e[i] + e[i + 1]) / 2;
         e_size[a][i] = e[i + 1] - e[i];
       }
-      // h sizes include the half cells beyond the end nodes
-      Real prev = e[0] - d[a] / 2;
+      Real prev = e[0] - based[a] / 2;
       for (std::size_t i = 0; i < m; ++i) {
         h_size[a][i] = h_node[a][i] - prev;
         prev = h_node[a][i];
       }
-      h_size[a][m] = (e[m] + d[a] / 2) - prev;
+      h_size[a][m] = (e[m] + based[a] / 2) - prev;
       min_d[a] = *std::min_element(e_size[a].begin(), e_size[a].end());
     }
-    dt = cfl / (kC0 * std::sqrt(1.0 / (min_d[0] * min_d[0]) + 1.0 / (min_d[1] * min_d[1]) +
-                                1.0 / (min_d[2] * min_d[2])));  // time_param.cpp:10-13
+    // TimeParam::calculateDt (time_param.cpp:10-21) for the grid's dimension
+    // (Simulation::makeTimeParam, simulation.cpp:654-676)
+    if (dim == 1)
+      dt = cfl / (kC0 * std::sqrt(1.0 / (min_d[2] * min_d[2])));
+    else if (dim == 2)
+      dt = cfl / (kC0 * std::sqrt(1.0 / (min_d[0] * min_d[0]) + 1.0 / (min_d[1] * min_d[1])));
+    else
+      dt = cfl / (kC0 * std::sqrt(1.0 / (min_d[0] * min_d[0]) + 1.0 / (min_d[1] * min_d[1]) +
+                                  1.0 / (min_d[2] * min_d[2])));
   }
 
   // ------------------------------------------------------------ material --
@@ -906,7 +961,10 @@ struct Simulation::Impl {
             for (std::size_t j = 0; j < ext[1]; ++j)
               for (std::size_t k = 0; k < ext[2]; ++k) {
                 const std::size_t id[3] = {i, j, k};
-                const Real da = (*sz[A])[id[A]], db = (*sz[B])[id[B]];
+                // xt::meshgrid broadcasts a one-element size array (the
+                // one-cell axis of a 1D/2D grid) over a component extent of 2
+                const Real da = (*sz[A])[sz[A]->size() == 1 ? 0 : id[A]];
+                const Real db = (*sz[B])[sz[B]->size() == 1 ? 0 : id[B]];
                 const std::size_t p = (i * ext[1] + j) * ext[2] + k;
                 const Real e = m.v[p], sg = s.v[p];
                 cs.v[p] = (2 * e - dtv * sg) / (2 * e + dtv * sg);
@@ -1318,7 +1376,7 @@ struct Simulation::Impl {
   }
 
   // ------------------------------------------------------------------ TFSF --
-  void buildTfsf(TFSF3D& t) {
+  void buildTfsf(TFSF& t) {
     // tfsf.cpp:14-30 (angles), :231-254 (transforms), :118-175 (box, aux
     // line, 1D coefficients), :256-339 (projections), :36-78 (incident
     // tables from the 1D auxiliary FDTD on eTime - dt).
@@ -1354,7 +1412,9 @@ struct Simulation::Impl {
       for (std::size_t i = 0; i < pi.size(); ++i) pi[i] = (i + org[a] - inc) * k[a] * rd;
       for (std::size_t i = 0; i < ph.size(); ++i) ph[i] = (i + org[a] - inc - 0.5) * k[a] * rd;
     }
-    const Real sdl = d[2] / rd;
+    // tfsf.cpp:153-161: the auxiliary line's cell is the based dz (1D and
+    // 3D) or based dx (2D) over ratio_delta
+    const Real sdl = (dim == 2 ? based[0] : based[2]) / rd;
     const Real ceie = 1, chih = 1;
     const Real ceih = -(dt / (kEps0 * sdl)), chie = -(dt / (kMu0 * sdl));
     const Real abc0 = (kC0 * dt - sdl) / (kC0 * dt + sdl), abc1 = 2 * sdl / (kC0 * dt + sdl);
@@ -1385,22 +1445,28 @@ struct Simulation::Impl {
       td.d.size[a] = static_cast<int32_t>(sz[a]);
       td.d.transform_e[a] = te[a];
       td.d.transform_h[a] = th[a];
-      td.d.ca[a] = dt / (kEps0 * d[a]);  // tfsf.cpp:177-205
-      td.d.cb[a] = dt / (kMu0 * d[a]);
+      td.d.ca[a] = dt / (kEps0 * based[a]);  // tfsf.cpp:177-205
+      td.d.cb[a] = dt / (kMu0 * based[a]);
     }
     td.d.nt = static_cast<int64_t>(nt);
     td.d.aux = static_cast<int64_t>(aux);
+    td.d.kind = static_cast<int32_t>(t.kind());
+    auto* t1 = dynamic_cast<TFSF1D*>(&t);
+    td.d.forward = t1 && t1->forward() ? 1 : 0;
     tfsf.push_back(std::move(td));
   }
 
   // -------------------------------------------------------------- monitors --
   void monitorBox(const Shape& s, int32_t o[3], int32_t e[3]) const {
     // Monitor::defaultInit (monitor.cpp:103-127) and the one-cell shift of a
-    // box starting at index 0 (field_monitor.cpp:24-67)
+    // box starting at index 0 (field_monitor.cpp:24-67): on x and y unless
+    // the grid is 1D, on z only in 3D (field_monitor.cpp:29-38,
+    // field_time_monitor.cpp:43-52)
     std::size_t so[3], se_[3];
     gridBox(s, so, se_);
     for (int a = 0; a < 3; ++a) {
-      o[a] = static_cast<int32_t>(so[a] == 0 ? 1 : so[a]);
+      const bool shift = a < 2 ? dim != 1 : dim == 3;
+      o[a] = static_cast<int32_t>(so[a] == 0 && shift ? 1 : so[a]);
       e[a] = static_cast<int32_t>(se_[a]);
     }
   }
@@ -1643,13 +1709,19 @@ struct Simulation::Impl {
       if (auto* p = dynamic_cast<PML*>(b.get())) buildPml(*p);
     stage("cpml");
     for (auto& s : sources) {
-      auto* t = dynamic_cast<TFSF3D*>(s.get());
+      auto* t = dynamic_cast<TFSF*>(s.get());
       if (!t) throw XFDTDException("unsupported waveform source");
       buildTfsf(*t);
     }
     stage("tfsf");
     buildDispersive();  // simulation.cpp:651, after every other coefficient patch
     stage("dispersive");
+    // Simulation::makeUpdator (simulation.cpp:688-743): the ADE updators
+    // are 3D only, any other dispersive case throws
+    if (has_ade && dim != 3) throw XFDTDSimulationException("don't support this type of updator yet.");
+    // the NF2FF surfaces (nffft.cpp:97-223) close a 3D box
+    if (dim != 3 && (!nf2ffs.empty() || !nf2ff_tds.empty()))
+      throw XFDTDSimulationException("NF2FF needs a 3D grid space");
     buildMonitors();
     buildNf2ff();
     coef_view.clear();
@@ -1673,6 +1745,7 @@ struct Simulation::Impl {
     dd.ny = static_cast<int32_t>(N[1]);
     dd.nz = static_cast<int32_t>(N[2]);
     dd.dtype = precision == Precision::FP64 ? XFD_F64 : XFD_F32;
+    dd.dim = dim;
     for (int s = 0; s < 18; ++s) dd.coeff[s] = coef[s].v.data();
     cp.clear();
     lp.clear();

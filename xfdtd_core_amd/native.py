@@ -175,7 +175,8 @@ class TfsfDesc(ctypes.Structure):
     _fields_ = [("origin", ctypes.c_int32 * 3), ("size", ctypes.c_int32 * 3), ("px_int", _dp), ("py_int", _dp),
                 ("pz_int", _dp), ("px_half", _dp), ("py_half", _dp), ("pz_half", _dp), ("nt", ctypes.c_int64),
                 ("aux", ctypes.c_int64), ("e_inc", _dp), ("h_inc", _dp), ("transform_e", ctypes.c_double * 3),
-                ("transform_h", ctypes.c_double * 3), ("ca", ctypes.c_double * 3), ("cb", ctypes.c_double * 3)]
+                ("transform_h", ctypes.c_double * 3), ("ca", ctypes.c_double * 3), ("cb", ctypes.c_double * 3),
+                ("kind", ctypes.c_int32), ("forward", ctypes.c_int32)]
 
 
 class LumpedDesc(ctypes.Structure):
@@ -207,7 +208,7 @@ class Desc(ctypes.Structure):
                 ("n_gather", ctypes.c_int32), ("gather", ctypes.POINTER(GatherDesc)),
                 ("ade", ctypes.c_void_p), ("n_nf2ff", ctypes.c_int32), ("nf2ff", ctypes.c_void_p),
                 ("n_line", ctypes.c_int32), ("line", ctypes.c_void_p),
-                ("n_tdnf", ctypes.c_int32), ("tdnf", ctypes.c_void_p)]
+                ("n_tdnf", ctypes.c_int32), ("tdnf", ctypes.c_void_p), ("dim", ctypes.c_int32)]
 
 
 def _ptr(a: np.ndarray):
@@ -219,13 +220,16 @@ class Context:
     with coefficient arrays supplied by the caller (no CPML, TFSF or lumped
     elements): the Yee loop of BasicUpdator3D alone."""
 
-    def __init__(self, coeffs, dtype: str = "f64", gathers=()):
+    def __init__(self, coeffs, dtype: str = "f64", gathers=(), dim: int = 3):
+        """`dim` 2 steps a 2D (TE, nz = 1) grid, 1 a 1D (TEM, nx = ny = 1)
+        one: BasicUpdatorTE / BasicUpdatorTEM instead of BasicUpdator3D."""
         self._L = load_library()
         self._keep = [np.ascontiguousarray(coeffs[n], dtype=np.float64) for n in COEFF_NAMES]
         d = Desc()
         shape_ex = self._keep[0].shape
         d.nx, d.ny, d.nz = shape_ex[0], shape_ex[1] - 1, shape_ex[2] - 1
         d.dtype = 0 if dtype == "f64" else 1
+        d.dim = dim
         for i, a in enumerate(self._keep):
             d.coeff[i] = _ptr(a)
         gl = (GatherDesc * max(1, len(gathers)))()

@@ -244,3 +244,67 @@ def adjoint_design(n_total: int = 160, pml: int = 10, dl: float = 1e-3, design: 
     po = (o + design + patch_gap, n_total // 2 - patch[1] // 2, n_total // 2 - patch[2] // 2)
     s.setDftObjective("EZ", po, patch, f0)
     return s, origin, eps
+
+
+def cylinder_scatter_2d(nx: int = 330, ny: int = 350, pml: int = 10, movie_interval: int = 20,
+                        probes: int = 8, eps_r: float = 0.0, tfsf: int = 50, radius: float = 0.03):
+    """2D (TE) workload after the reference's own example
+    (test/src/examples/cylinder_scatter_2d.cpp): dx = dy = c/(20 f_max) with
+    f_max = 20 GHz, an `nx` x `ny` air domain infinite along z, a cylinder of
+    radius 0.03 m at the centre (PEC, or a dielectric of `eps_r` when given),
+    TFSF2D at `tfsf` = 50 cells with phi = pi/4 and a cosine-modulated Gaussian
+    (f_c = 12 GHz), 10-cell PML on x and y, CFL 0.8, an Ez movie of the whole
+    plane every `movie_interval` steps and `probes` Ez time monitors on the
+    diagonal."""
+    inf = float("inf")
+    fc, fmax = 12e9, 20e9
+    dx = 3e8 / fmax / 20
+    tau = 1.7 / (fmax - fc)
+    t0 = 0.8 * tau
+    s = S.Simulation(dx, dx, 1.0, 0.8)
+    ox, oy = -(nx // 2) * dx, -(ny // 2) * dx
+    s.addObject(S.Object("domain", S.Cube(S.Vector(ox, oy, -inf), S.Vector(nx * dx, ny * dx, inf)),
+                         S.Material.createAir()))
+    mat = (S.Material("pec", S.ElectroMagneticProperty(1, 1, 1e10, 0)) if eps_r <= 0
+           else S.Material("dielectric", S.ElectroMagneticProperty(eps_r, 1, 0, 0)))
+    s.addObject(S.Object("cylinder", S.Cylinder(S.Vector(0.0, 0.0, -inf), radius, inf, "Z"), mat))
+    s.addWaveformSource(S.TFSF2D(tfsf, tfsf, math.pi * 0.25, S.Waveform.cosineModulatedGaussian(tau, t0, fc)))
+    for d in ("XN", "XP", "YN", "YP"):
+        s.addBoundary(S.PML(pml, d))
+    if movie_interval > 0:
+        s.addMonitor(S.MovieMonitor(S.FieldMonitor(S.Cube(S.Vector(ox, oy, -inf), S.Vector(nx * dx, ny * dx, inf)),
+                                                   "EZ"), movie_interval, "movie", ""))
+    for q in range(probes):
+        f = (q + 1) / (probes + 1)
+        x, y = ox + f * nx * dx, oy + f * ny * dx
+        s.addMonitor(S.FieldTimeMonitor(f"probe{q}", S.Cube(S.Vector(x, y, -inf), S.Vector(dx, dx, inf)), "", "EZ"))
+    return s
+
+
+def slab_line_1d(nz: int = 300, pml: int = 10, dl: float = 0.5e-3, slab=(50, 260), eps_r: float = 4.0,
+                 sigma: float = 0.0, movie_interval: int = 5):
+    """1D (TEM) workload after the reference's own example
+    (test/src/examples/slab_refection_transmission.cpp): a line of `nz`
+    cells at dl = 0.5 mm along z (infinite along x and y), a slab over cells
+    [50, 310) of eps_r = 4 (the example's modified-Lorentz blood model needs
+    the 3D ADE updator; the reference refuses it on a 1D grid), TFSF1D at 25
+    cells going forward, a Gaussian of tau = 40 dl / 6e8, 10-cell PML on z,
+    CFL 0.9, Ex time monitors at cells 10 and 50 and an Ex movie."""
+    inf = float("inf")
+    s = S.Simulation(dl, dl, dl, 0.9)
+    s.addObject(S.Object("domain", S.Cube(S.Vector(-inf, -inf, 0.0), S.Vector(inf, inf, nz * dl)),
+                         S.Material.createAir()))
+    s.addObject(S.Object("slab", S.Cube(S.Vector(-inf, -inf, slab[0] * dl), S.Vector(inf, inf, slab[1] * dl)),
+                         S.Material("slab", S.ElectroMagneticProperty(eps_r, 1, sigma, 0))))
+    tau = dl * 40 / 6e8
+    s.addWaveformSource(S.TFSF1D(25, True, S.Waveform.gaussian(tau, 4.5 * tau)))
+    s.addMonitor(S.FieldTimeMonitor("reflect_ex", S.Cube(S.Vector(-inf, -inf, 10 * dl), S.Vector(inf, inf, dl)),
+                                    "", "EX"))
+    s.addMonitor(S.FieldTimeMonitor("transmit_ex", S.Cube(S.Vector(-inf, -inf, 50 * dl), S.Vector(inf, inf, dl)),
+                                    "", "EX"))
+    if movie_interval > 0:
+        s.addMonitor(S.MovieMonitor(S.FieldMonitor(S.Cube(S.Vector(-inf, -inf, 0.0), S.Vector(inf, inf, 200 * dl)),
+                                                   "EX"), movie_interval, "movie", ""))
+    for d in ("ZP", "ZN"):
+        s.addBoundary(S.PML(pml, d))
+    return s
