@@ -54,9 +54,11 @@ def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0) 
     move their share to the sweep; their separate launches only finish the
     far grid planes (not counted), and the CPML figure is the mean over the
     six launches of a half-step."""
+    if kernel == "step":  # the single-pass step: the whole step's B_yee and CPML
+        return step_alg_bytes(n, pml, s)
     cells = n ** 3
     slab_cells = pml * (n + 1) * (n + 1)
-    inside = {0: 0, 1: 4, 2: 2}[int(fused)]
+    inside = {0: 0, 1: 4, 2: 2, 3: 6}[int(fused)]
     if kernel in ("update_h", "update_e"):
         return 15.0 * s * cells + 6.0 * s * inside * slab_cells
     if kernel in ("cpml_h", "cpml_e"):

@@ -287,8 +287,16 @@ int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
  *     points in no x or y slab, and the frame where slabs overlap follows
  *     the x/y slabs in one launch, so the reference's order holds (bitwise).
  *     XFD_CPML_ZFUSED=1 forces it, =0 turns it off.
- * Never with ADE materials or when a TFSF or lumped correction lands inside
- * a slab (the reference applies those first). */
+ *   *fused = 3: the single-pass step: one launch per step computes H, its
+ *     CPML slabs, E and its CPML slabs (every slab, in boundary order, so
+ *     FP64 stays bitwise), reading one set of fields and writing a second.
+ *     The default for 3D grids without TFSF (its H corrections would have to
+ *     reach recomputed halo points), ADE, records on a grid boundary plane
+ *     or more than 8 slabs; XFD_STEP_FUSED=0, XFD_CPML_FUSED=1 or
+ *     XFD_CPML_ZFUSED=1 keep the two sweeps.  XFD_STEP_ROWS (4 or 8) and
+ *     XFD_STEP_CHUNK (1..64) set its work-group tile.
+ * Modes 1 and 2 never with ADE materials or when a TFSF or lumped correction
+ * lands inside a slab (the reference applies those first). */
 int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);
 
 /* Zeroes a context's time-stepping state: fields, CPML auxiliaries, inductor

@@ -314,6 +314,7 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
     slab or ADE is present keep the separate launches."""
     sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
     say(f"[gpu] fused CPML vs separate {precision}")
+    monkeypatch.setenv("XFD_STEP_FUSED", "0")
     monkeypatch.setenv("XFD_CPML_FUSED", "1")
     fused = native.Simulation(sp, precision)
     fused.run(200)
@@ -361,6 +362,7 @@ def test_z_cpml_inside_sweeps(monkeypatch, precision):
     separate run.  Contexts with ADE keep separate slabs."""
     sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
     say(f"[gpu] z CPML inside the sweeps vs separate {precision}")
+    monkeypatch.setenv("XFD_STEP_FUSED", "0")
     monkeypatch.delenv("XFD_CPML_FUSED", raising=False)
     monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
     for nz, want in ((192, 0), (193, 2 if precision == "f32" else 0)):  # 3 and 4 tiles of 64 k-lanes
@@ -564,12 +566,30 @@ def test_time_domain_nf2ff(tmp_path, monkeypatch, precision, merge):
         np.testing.assert_array_equal(np.load(out / f"{name}.npy"), far[w])
 
 
-def test_profiling_selected_kernel_classes():
+@pytest.mark.parametrize("single_pass", [False, True])
+def test_profiling_selected_kernel_classes(monkeypatch, single_pass):
     """HIP-event timing can be limited to chosen kernel classes: only those
     classes report launches, and with every class the counts match the
-    launches of the steps."""
+    launches of the steps (two sweeps and six slab launches per half-step,
+    or one single-pass launch per step)."""
     sp = scenarios.inhomogeneous_box(n_total=40, pml=6, n_spheres=3, r_cells=(3, 5), seed=3).spec()
-    say("[gpu] profiling selected kernel classes")
+    say(f"[gpu] profiling selected kernel classes (single pass: {single_pass})")
+    if single_pass:
+        sim = native.Simulation(sp, "f64")
+        sim.init(40)
+        assert sim.cpml_fused() == 3
+        sim.set_profiling(True, classes=["step"])
+        sim.advance(0, 20)
+        st = sim.kernel_stats()
+        assert st["step"][0] == 20 and st["step"][1] > 0
+        assert all(v[0] == 0 for k, v in st.items() if k != "step"), st
+        sim.set_profiling(True)
+        sim.advance(20, 20)
+        st = sim.kernel_stats()
+        assert st["step"][0] == 20 and st["records_e"][0] == 20, st
+        assert st["update_e"][0] == st["update_h"][0] == st["cpml_e"][0] == st["cpml_h"][0] == 0, st
+        return
+    monkeypatch.setenv("XFD_STEP_FUSED", "0")
     sim = native.Simulation(sp, "f64")
     sim.init(40)
     sim.set_profiling(True, classes=["update_e"])
@@ -732,3 +752,84 @@ def test_full_size_fp64_matches_oracle():
         np.testing.assert_array_equal(a, b)
     assert worst <= TOL_F64, worst
     dev.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("tile", [(4, 32), (8, 7), (4, 1)])
+def test_single_pass_step(monkeypatch, precision, tile):
+    """The single-pass step (one launch per step: H, its CPML, E, its CPML;
+    two field sets) against the two Yee sweeps with separate slab launches,
+    on a 136^3 box (three k-tiles of 64 lanes, the last partial) with random
+    spheres, the current-source dipole, six-field snapshots and probes, and
+    several work-group tiles (rows per work-group, planes per chunk; a
+    chunk of 7 does not divide the grid, 1 recomputes every plane's i-1
+    neighbour).  FP64: every field, frame and probe equal to the separate
+    launches and to the oracle; FP32: within 1e-5 of the separate launches'
+    FP32 run."""
+    rows, chunk = tile
+    sp = scenarios.inhomogeneous_box(n_total=136, pml=9, n_spheres=12, r_cells=(4, 14), seed=11,
+                                     snapshot_interval=40, probes=(2, 2, 2)).spec()
+    steps = 120
+    say(f"[gpu] single-pass step {precision} rows {rows} chunk {chunk}")
+    monkeypatch.setenv("XFD_STEP_ROWS", str(rows))
+    monkeypatch.setenv("XFD_STEP_CHUNK", str(chunk))
+    one = native.Simulation(sp, precision)
+    one.run(steps)
+    assert one.cpml_fused() == 3
+    monkeypatch.setenv("XFD_STEP_FUSED", "0")
+    monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
+    sep = native.Simulation(sp, precision)
+    sep.run(steps)
+    assert sep.cpml_fused() == 0
+    for f in FIELDS:
+        a, b = one.field(f), sep.field(f)
+        if precision == "f64":
+            np.testing.assert_array_equal(a, b, err_msg=f)
+        else:
+            assert rel_l2(a, b) < 1e-5, (f, rel_l2(a, b))
+    for m in range(one.monitor_count()):
+        for a, b in zip(one.monitor_records(m), sep.monitor_records(m)):
+            if precision == "f64":
+                np.testing.assert_array_equal(a, b, err_msg=f"monitor {m}")
+            elif np.abs(b).max() > 0:
+                assert rel_l2(a, b) < 1e-5, (m, rel_l2(a, b))
+    if precision == "f64" and tile == (4, 32):
+        say("[gpu] single-pass step: oracle")
+        ref = RefSim(sp, threads=(8, 1, 1))
+        ref.run(steps)
+        for f in FIELDS:
+            np.testing.assert_array_equal(one.field(f), ref.field(f), err_msg=f)
+        for m in range(one.monitor_count()):
+            for a, b in zip(one.monitor_records(m), ref.monitor_records(m)):
+                np.testing.assert_array_equal(a, b, err_msg=f"monitor {m}")
+
+
+def test_single_pass_step_field_io():
+    """xfd_read_field / xfd_write_field around an odd number of single-pass
+    steps (the second field set current): a cavity mode stepped 31 times
+    equals the oracle; the state read back, Ez doubled and written, stepped
+    30 more times equals the oracle started from that state, bitwise."""
+    n, dl = (24, 20, 16), (1e-3, 1.2e-3, 0.9e-3)
+    ref = RefSim(em.cavity_spec(n, dl))
+    ref.init(31)
+    coeffs = {c: ref.coeff(c) for c in native.COEFF_NAMES}
+    e0, _, _ = em.mode("EZ", n)
+    ctx = native.Context(coeffs, "f64")
+    ctx.write("EZ", e0)
+    ref.write_field("EZ", e0)
+    ctx.step(0, 31)
+    ref.loop()
+    state = {}
+    for f in FIELDS:
+        state[f] = ctx.read(f)
+        np.testing.assert_array_equal(state[f], ref.field(f), err_msg=f)
+    state["EZ"] = 2 * state["EZ"]
+    ctx.write("EZ", state["EZ"])
+    ctx.step(31, 30)
+    ref2 = RefSim(em.cavity_spec(n, dl))
+    ref2.init(30)
+    for f in FIELDS:
+        ref2.write_field(f, state[f])
+    ref2.loop()
+    for f in FIELDS:
+        np.testing.assert_array_equal(ctx.read(f), ref2.field(f), err_msg=f)

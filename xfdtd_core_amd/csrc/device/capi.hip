@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -18,6 +19,7 @@
 #include "adjoint_kernels.hpp"
 #include "batch_kernels.hpp"
 #include "kernels.hpp"
+#include "step_kernels.hpp"
 
 namespace {
 
@@ -167,9 +169,11 @@ bool classify(const double* A, const double* B, const double* C, const int ext[3
   return true;
 }
 
-enum KernelClass { K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_NF2FF, K_COUNT };
-const char* kKernelNames[K_COUNT] = {"update_h", "update_e", "cpml_h",  "cpml_e",
-                                     "records_h", "records_e", "gather", "nf2ff"};
+enum KernelClass {
+  K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_NF2FF, K_STEP, K_COUNT
+};
+const char* kKernelNames[K_COUNT] = {"update_h",  "update_e", "cpml_h", "cpml_e", "records_h",
+                                     "records_e", "gather",   "nf2ff",  "step"};
 
 struct DevBuf {
   void* p = nullptr;
@@ -232,6 +236,7 @@ struct xfd_ctx {
     int lo_e[3], lo_h[3], ext[3] = {0, 0, 0}, ext_h[3] = {0, 0, 0};
     int extA_e[2], extB_e[2], extA_h[2], extB_h[2];
     DevBuf cpsi[4], psi[4];  // ea_hb, eb_ha, ha_eb, hb_ea (box layout)
+    DevBuf psi_h2[2];        // the single-pass step's second set of ha_eb, hb_ea
     DevBuf ae, be, ah, bh;
   };
   std::vector<std::unique_ptr<Slab>> slabs;
@@ -283,6 +288,14 @@ struct xfd_ctx {
   // of each half-step as device arrays
   // grid dimension: 3, 2 (TE, nz = 1) or 1 (TEM, nx = ny = 1)
   int dim = 3;
+  int device = 0;  // the HIP device the context lives on
+  // The single-pass step (k_step): a second set of the six fields and of
+  // the H-side CPML psi; `parity` says which set holds the current state
+  // (0: fields / psi, 1: fields_b / psi_h2).
+  bool fstep = false;
+  int fstep_rows = 4, fstep_chunk = 32;
+  int parity = 0;
+  DevBuf fields_b[6];
   bool fused = false;
   int n_fused = 0;  // slabs [0, n_fused) run inside the sweeps, the rest separately
   // the z slabs run inside the sweeps (ZCpml), the rest separately
@@ -375,11 +388,43 @@ struct xfd_ctx {
 
 namespace {
 
+// The set of field arrays holding the current state (set 1 only after an
+// odd number of single-pass steps).
+DevBuf* curFields(xfd_ctx& c) { return c.parity ? c.fields_b : c.fields; }
+
 template <class T>
 xfd::FieldPtrs<T> fieldPtrs(xfd_ctx& c) {
   xfd::FieldPtrs<T> F;
-  for (int i = 0; i < 6; ++i) F.f[i] = c.fields[i].as<T>();
+  for (int i = 0; i < 6; ++i) F.f[i] = curFields(c)[i].as<T>();
   return F;
+}
+
+template <class T>
+xfd::FieldPtrs<T> otherFieldPtrs(xfd_ctx& c) {
+  xfd::FieldPtrs<T> F;
+  for (int i = 0; i < 6; ++i) F.f[i] = (c.parity ? c.fields : c.fields_b)[i].as<T>();
+  return F;
+}
+
+// Brings the current state back to set 0 (fields, psi) and leaves the
+// single-pass step: for entry points that work on set 0 directly
+// (checkpoints, the adjoint, writes of whole fields, batches).
+void leaveStep(xfd_ctx& c) {
+  if (!c.fstep) return;
+  if (c.parity) {
+    for (int f = 0; f < 6; ++f)
+      XFD_HIP(hipMemcpyAsync(c.fields[f].p, c.fields_b[f].p, c.fields[f].bytes, hipMemcpyDeviceToDevice, c.stream));
+    for (auto& sl : c.slabs)
+      for (int q = 0; q < 2; ++q)
+        XFD_HIP(hipMemcpyAsync(sl->psi[2 + q].p, sl->psi_h2[q].p, sl->psi[2 + q].bytes, hipMemcpyDeviceToDevice,
+                               c.stream));
+    XFD_HIP(hipStreamSynchronize(c.stream));
+  }
+  c.parity = 0;
+  c.fstep = false;
+  for (auto& f : c.fields_b) f.reset();
+  for (auto& sl : c.slabs)
+    for (auto& b : sl->psi_h2) b.reset();
 }
 
 template <class T>
@@ -1061,7 +1106,79 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     if (c.zfused)
       for (int q : zq) c.zslab[q] = 1;
   }
+  {  // The single-pass step (k_step) where every correction between the H and
+    // E half-steps is a CPML slab: no TFSF H records, no ADE, no record on a
+    // point a slab corrects or on a grid boundary plane (the pass leaves the
+    // far planes alone), at most kMaxStepSlabs slabs.  XFD_STEP_FUSED=0 keeps
+    // the separate sweeps; XFD_STEP_ROWS (4 or 8) and XFD_STEP_CHUNK set the
+    // work-group tile.
+    const char* env = std::getenv("XFD_STEP_FUSED");
+    const char* fenv = std::getenv("XFD_CPML_FUSED");
+    const char* zenv1 = std::getenv("XFD_CPML_ZFUSED");
+    // an explicit request for slabs inside the two sweeps keeps the sweeps
+    const bool off = (env && env[0] == '0') || (fenv && fenv[0] == '1') || (zenv1 && zenv1[0] == '1');
+    bool edge = false;
+    for (const auto& r : re) {
+      const int64_t i = r.target / g.plane, rem = r.target % g.plane, j = rem / g.pz, k = rem % g.pz;
+      edge = edge || i == 0 || i >= g.nx || j == 0 || j >= g.ny || k == 0 || k >= g.nz;
+    }
+    c.fstep = !off && c.dim == 3 && c.ade_scheme < 0 && c.nseg_h == 0 && !records_meet_slabs && !edge &&
+              c.slabs.size() <= static_cast<size_t>(xfd::kMaxStepSlabs);
+    if (c.fstep) {
+      c.fused = false;
+      c.n_fused = 0;
+      c.zfused = false;
+      std::fill(c.zslab.begin(), c.zslab.end(), 0);
+      if (const char* e = std::getenv("XFD_STEP_ROWS")) c.fstep_rows = std::atoi(e) == 8 ? 8 : 4;
+      if (const char* e = std::getenv("XFD_STEP_CHUNK")) {
+        const int v = std::atoi(e);
+        require(v >= 1 && v <= xfd::kMaxStepChunk, "XFD_STEP_CHUNK out of range");
+        c.fstep_chunk = v;
+      }
+      for (int f = 0; f < 6; ++f) c.fields_b[f].alloc(c.fields[f].bytes);
+      for (auto& sl : c.slabs)
+        for (int q = 0; q < 2; ++q) sl->psi_h2[q].alloc(sl->psi[2 + q].bytes);
+      c.parity = 0;
+    }
+  }
   XFD_HIP(hipStreamSynchronize(c.stream));
+}
+
+// The slabs of one half-step as the single-pass step reads them: H side
+// with psi from the current set into the other, E side in place.
+template <class T>
+xfd::StepSlabs<T> stepSlabs(xfd_ctx& c, bool is_e) {
+  xfd::StepSlabs<T> L{};
+  for (auto& sp : c.slabs) {
+    const xfd::SlabArgs<T> a = slabArgs<T>(*sp, is_e);
+    xfd::StepSlab<T>& S = L.s[L.n++];
+    S.axis = a.axis;
+    S.start = a.start;
+    for (int q = 0; q < 3; ++q) {
+      S.lo[q] = a.lo[q];
+      S.ext[q] = a.ext[q];
+    }
+    for (int q = 0; q < 2; ++q) {
+      S.extA[q] = a.extA[q];
+      S.extB[q] = a.extB[q];
+    }
+    S.cpsiA = a.cpsiA;
+    S.cpsiB = a.cpsiB;
+    S.ca = a.ca;
+    S.cb = a.cb;
+    if (is_e) {
+      S.rdA = S.wrA = a.psiA;
+      S.rdB = S.wrB = a.psiB;
+    } else {
+      T* cur[2] = {sp->psi[2].as<T>(), sp->psi[3].as<T>()};
+      T* alt[2] = {sp->psi_h2[0].as<T>(), sp->psi_h2[1].as<T>()};
+      S.rdA = c.parity ? alt[0] : cur[0];
+      S.rdB = c.parity ? alt[1] : cur[1];
+      S.wrA = c.parity ? cur[0] : alt[0];
+      S.wrB = c.parity ? cur[1] : alt[1];
+    }
+  }
+  return L;
 }
 
 template <class T>
@@ -1164,7 +1281,7 @@ template <class T>
 void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   require(t0 >= 0 && n >= 0, "bad step range");
   require(t0 + n <= c.pool_rows, "step beyond the source tables (run length given at init)");
-  const auto F = fieldPtrs<T>(c);
+  auto F = fieldPtrs<T>(c);
   const auto C = coefPtrs<T>(c);
   xfd::ClassCoef<T> K{};
   for (int q = 0; q < 6; ++q) {
@@ -1202,6 +1319,20 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   if (c.zfused) zCpml<T>(c, ZE, ZH);
   auto inSweep = [&](size_t q) { return static_cast<int>(q) < c.n_fused; };
   for (int64_t t = t0; t < t0 + n; ++t) {
+    if (c.fstep) {
+      // H, its CPML, E and its CPML in one pass from the current set into
+      // the other, which then becomes current
+      const auto FB = otherFieldPtrs<T>(c);
+      const auto SH = stepSlabs<T>(c, false), SE = stepSlabs<T>(c, true);
+      c.timed(K_STEP, [&] {
+        if (c.classed)
+          xfd::StepLaunch<T>::step(st, c.g, F, FB, K, SH, SE, c.fstep_rows, c.fstep_chunk);
+        else
+          xfd::StepLaunch<T>::step(st, c.g, F, FB, C, SH, SE, c.fstep_rows, c.fstep_chunk);
+      });
+      c.parity ^= 1;
+      F = FB;
+    } else {
     c.timed(K_UPDATE_H, [&] {
       if (c.dim != 3) {
         if (c.classed)
@@ -1245,12 +1376,13 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk, PE, ZE);
       }
     });
+    }
     if (c.nseg_e)
       c.timed(K_REC_E, [&] {
         xfd::Launch<T>::records(st, F, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e,
                                 c.pool.as<double>(), c.state.as<T>(), t);
       });
-    for (size_t q = 0; q < se.size(); ++q)
+    for (size_t q = 0; q < se.size() && !c.fstep; ++q)
       c.timed(K_CPML_E, [&] {
         if (c.zfused && c.zslab[q] && sizeof(T) == 8)
           xfd::Launch<T>::cpml_zrest(st, c.g, F, se[q], ZE.fr, true);
@@ -1308,8 +1440,12 @@ void resetState(xfd_ctx& c) {
     if (b.p && b.bytes) XFD_HIP(hipMemsetAsync(b.p, 0, b.bytes, c.stream));
   };
   for (auto& f : c.fields) zero(f);
-  for (auto& sl : c.slabs)
+  for (auto& f : c.fields_b) zero(f);
+  for (auto& sl : c.slabs) {
     for (auto& p : sl->psi) zero(p);
+    for (auto& p : sl->psi_h2) zero(p);
+  }
+  c.parity = 0;
   zero(c.state);
   for (int q = 0; q < 3; ++q) {
     zero(c.ade_j[q]);
@@ -1799,6 +1935,7 @@ int xfd_create(const xfd_desc* d, xfd_ctx** out) {
     require(d->n_cpml >= 0 && d->n_lumped >= 0 && d->n_tfsf >= 0 && d->n_gather >= 0, "bad counts");
     auto c = std::make_unique<xfd_ctx>();
     c->dim = dim;
+    XFD_HIP(hipGetDevice(&c->device));
     c->dtype = d->dtype;
     c->elem = d->dtype == XFD_F64 ? 8 : 4;
     xfd::Grid& g = c->g;
@@ -1851,12 +1988,11 @@ int xfd_read_field(xfd_ctx* c, int32_t field, double* dst) {
     const size_t n = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
     DevBuf tmp;
     tmp.alloc(n * sizeof(double));
+    DevBuf& cur = curFields(*c)[field];
     if (c->dtype == XFD_F64)
-      xfd::Launch<double>::pack(c->stream, c->g, c->fields[field].as<double>(), tmp.as<double>(), ext[0], ext[1],
-                                ext[2]);
+      xfd::Launch<double>::pack(c->stream, c->g, cur.as<double>(), tmp.as<double>(), ext[0], ext[1], ext[2]);
     else
-      xfd::Launch<float>::pack(c->stream, c->g, c->fields[field].as<float>(), tmp.as<double>(), ext[0], ext[1],
-                               ext[2]);
+      xfd::Launch<float>::pack(c->stream, c->g, cur.as<float>(), tmp.as<double>(), ext[0], ext[1], ext[2]);
     XFD_HIP(hipMemcpyAsync(dst, tmp.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     XFD_HIP(hipStreamSynchronize(c->stream));
   });
@@ -1869,10 +2005,16 @@ int xfd_write_field(xfd_ctx* c, int32_t field, const double* src) {
     extentOf(field, c->g.nx, c->g.ny, c->g.nz, ext);
     DevBuf tmp;
     tmp.alloc(static_cast<size_t>(ext[0]) * ext[1] * ext[2] * sizeof(double));
+    DevBuf& cur = curFields(*c)[field];
     if (c->dtype == XFD_F64)
-      uploadPadded<double>(*c, src, ext, c->fields[field].as<double>(), tmp);
+      uploadPadded<double>(*c, src, ext, cur.as<double>(), tmp);
     else
-      uploadPadded<float>(*c, src, ext, c->fields[field].as<float>(), tmp);
+      uploadPadded<float>(*c, src, ext, cur.as<float>(), tmp);
+    if (c->fstep) {  // both sets hold the points the single-pass step does not visit
+      DevBuf& alt = (c->parity ? c->fields : c->fields_b)[field];
+      XFD_HIP(hipMemcpyAsync(alt.p, cur.p, cur.bytes, hipMemcpyDeviceToDevice, c->stream));
+      XFD_HIP(hipStreamSynchronize(c->stream));
+    }
   });
 }
 
@@ -1917,6 +2059,7 @@ int xfd_forward_checkpointed(xfd_ctx* c, int64_t nt, int64_t interval) {
   return guard([&] {
     require(c != nullptr, "null context");
     require(c->dim == 3, "the adjoint form needs a 3D grid");
+    leaveStep(*c);  // checkpoints save and restore set 0
     if (c->dtype == XFD_F64)
       forwardCheckpointed<double>(*c, nt, interval);
     else
@@ -1936,6 +2079,7 @@ int xfd_backward(xfd_ctx* c, const xfd_adjoint_desc* d, double* grad) {
   return guard([&] {
     require(c != nullptr && d != nullptr, "bad argument");
     require(c->dim == 3, "the adjoint form needs a 3D grid");
+    leaveStep(*c);
     if (c->dtype == XFD_F64)
       backwardTyped<double>(*c, *d, grad);
     else
@@ -1983,7 +2127,7 @@ int xfd_nf2ff_far_field(xfd_ctx* c, int32_t n, int32_t f, int32_t na, const doub
 int xfd_cpml_fused(xfd_ctx* c, int32_t* fused) {
   return guard([&] {
     require(c && fused, "bad argument");
-    *fused = c->fused ? 1 : (c->zfused ? 2 : 0);
+    *fused = c->fstep ? 3 : (c->fused ? 1 : (c->zfused ? 2 : 0));
   });
 }
 
@@ -2006,6 +2150,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       xfd_ctx* c = nullptr;
       if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
       b->cases.emplace_back(c);
+      leaveStep(*c);  // the batched launches step set 0 with their own tables
       // batched cases keep their z slabs as separate launches: with most
       // wavefronts of these small boxes carrying slab lanes, the sweeps were
       // measured slower with them inside

@@ -5,3 +5,4 @@
 template struct xfd::Launch<float>;
 template struct xfd::BatchLaunch<float>;
 template struct xfd::AdjLaunch<float>;
+template struct xfd::StepLaunch<float>;

@@ -5,3 +5,4 @@
 template struct xfd::Launch<double>;
 template struct xfd::BatchLaunch<double>;
 template struct xfd::AdjLaunch<double>;
+template struct xfd::StepLaunch<double>;

@@ -9,6 +9,7 @@
 #include "adjoint_kernels.hpp"
 #include "batch_kernels.hpp"
 #include "kernels.hpp"
+#include "step_kernels.hpp"
 
 namespace xfd {
 
@@ -346,6 +347,29 @@ void AdjLaunch<T>::source(hipStream_t s, const FieldPtrs<T>& L, const AdjPoint* 
                           int64_t nt, int64_t step) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_adj_source<T>, dim3(blocks_for(n, 64)), dim3(64), 0, s, L, pts, n, src, nt, step);
+}
+
+template <class T, class Coef>
+void launchStep(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B, const Coef& C,
+                const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows, int ichunk) {
+  const int nbj = static_cast<int>(blocks_for(g.ny, rows));
+  const dim3 grid(static_cast<unsigned>(nbj) * blocks_for(g.nx, ichunk));
+  if (rows == 8)
+    hipLaunchKernelGGL((k_step<T, Coef, 8>), grid, dim3(64, 9), 0, s, g, A, B, C, SH, SE, ichunk, nbj);
+  else
+    hipLaunchKernelGGL((k_step<T, Coef, 4>), grid, dim3(64, 5), 0, s, g, A, B, C, SH, SE, ichunk, nbj);
+}
+template <class T>
+void StepLaunch<T>::step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
+                         const CoefPtrs<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows,
+                         int ichunk) {
+  launchStep<T>(s, g, A, B, C, SH, SE, rows, ichunk);
+}
+template <class T>
+void StepLaunch<T>::step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
+                         const ClassCoef<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows,
+                         int ichunk) {
+  launchStep<T>(s, g, A, B, C, SH, SE, rows, ichunk);
 }
 
 }  // namespace xfd

@@ -1,0 +1,252 @@
+// One pass per time step: the H half-step, its CPML corrections, the E
+// half-step and its CPML corrections in a single launch.
+//
+// The reference runs updateH, the H correctors, updateE and the E correctors
+// as separate sweeps over the grid (src/domain/domain.cpp:82-135), so every
+// field is read twice and written once per step.  Here a work-group marches
+// along i over a chunk of planes for a tile of J rows (j) times the whole k
+// extent, in k-tiles of 64 lanes, and computes E(i) right after H(i) while
+// H(i), H(i-1), H(j-1) and H(k-1) are still on chip:
+//   * H(i-1) is the previous plane of the march (registers); the chunk's
+//     first plane recomputes H(i0-1) without storing it;
+//   * H(j-1) of the tile's first row comes from an extra wavefront that
+//     recomputes row j0-1 (the halo row) and hands it over through LDS with
+//     the other rows' values;
+//   * H(k-1) is the neighbouring lane's value (a lane shuffle), and for lane
+//     0 the previous k-tile's lane 63, kept in LDS per plane.
+// Halo and chunk-start recomputation read the neighbours' old state, which
+// their owners overwrite in the same launch; so E, H and the H-side CPML psi
+// are double-buffered: the step reads set A and writes set B, and the
+// buffers swap roles every step.  The E-side psi is read and written by its
+// owner only and stays in place.
+//
+// The arithmetic of every point is the separate sweeps' and k_cpml's, in
+// the same order (Yee terms left to right, then every slab containing the
+// point in boundary order), so FP64 results are bitwise those of the
+// reference's order.  Corrections that are not point-local -- TFSF H
+// records, which would have to reach the halo and chunk-start copies -- keep
+// the separate sweeps; E records (lumped sources, TFSF E) follow the pass as
+// before, which is the reference's order wherever no record shares a point
+// with a slab.
+//
+// Points on the far grid planes i = nx, j = ny, k = nz are not visited: no
+// Yee range reaches them and the only corrections that do (the CPML slabs'
+// transverse extension) couple them to each other alone, starting from and
+// keeping zero, so both buffer sets hold the same values there throughout.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace xfd {
+
+constexpr int kMaxStepSlabs = 8;
+constexpr int kMaxStepChunk = 64;
+
+// One CPML slab as the step pass sees it: its box (grid indices), the
+// components' extents along (a, b), the per-layer recursive-convolution
+// coefficients and c_psi, and psi to read (rd) and to write (wr).  On the
+// H side rd and wr are the two buffers of the ping-pong; on the E side they
+// are the same array.
+template <class T>
+struct StepSlab {
+  int axis, start;
+  int lo[3], ext[3];
+  int extA[2], extB[2];
+  const T* cpsiA;
+  const T* cpsiB;
+  const T* rdA;
+  const T* rdB;
+  T* wrA;
+  T* wrB;
+  const T* ca;
+  const T* cb;
+};
+
+template <class T>
+struct StepSlabs {
+  int n = 0;
+  StepSlab<T> s[kMaxStepSlabs];
+};
+
+// The CPML corrections of point (i, j, k) for every slab of L containing
+// it, in boundary order, as cpmlFused: component A = (c+1)%3 takes the dual
+// difference dA[c], component B = (c+2)%3 takes dB[c];
+//   psi = b psi + a d,  v += c_psi psi.
+// Write = false (recomputed halo and chunk-start points) leaves psi as it is.
+template <class T, bool Write>
+__device__ __forceinline__ void stepSlabs(const StepSlabs<T>& L, int i, int j, int k, T& v0, T& v1, T& v2,
+                                          const T dA0, const T dA1, const T dA2, const T dB0, const T dB1,
+                                          const T dB2) {
+  for (int n = 0; n < L.n; ++n) {
+    const StepSlab<T>& S = L.s[n];
+    const int u = i - S.lo[0], v = j - S.lo[1], w = k - S.lo[2];
+    if (static_cast<unsigned>(u) >= static_cast<unsigned>(S.ext[0]) ||
+        static_cast<unsigned>(v) >= static_cast<unsigned>(S.ext[1]) ||
+        static_cast<unsigned>(w) >= static_cast<unsigned>(S.ext[2]))
+      continue;
+    const int c = S.axis, a = c == 2 ? 0 : c + 1, b = c == 0 ? 2 : c - 1;
+    const int64_t t = (static_cast<int64_t>(u) * S.ext[1] + v) * S.ext[2] + w;
+    const int layer = pick3(c, i, j, k) - S.start;
+    const T ca = S.ca[layer], cb = S.cb[layer];
+    const int pa = pick3(a, i, j, k), pb = pick3(b, i, j, k);
+    if (pa < S.extA[0] && pb < S.extA[1]) {
+      const T d = c == 0 ? dA0 : (c == 1 ? dA1 : dA2);
+      const T psi = cb * S.rdA[t] + ca * d;
+      if (Write) S.wrA[t] = psi;
+      const T add = S.cpsiA[t] * psi;
+      const T n0 = a == 0 ? v0 + add : v0, n1 = a == 1 ? v1 + add : v1, n2 = a == 2 ? v2 + add : v2;
+      v0 = n0;
+      v1 = n1;
+      v2 = n2;
+    }
+    if (pa < S.extB[0] && pb < S.extB[1]) {
+      const T d = c == 0 ? dB0 : (c == 1 ? dB1 : dB2);
+      const T psi = cb * S.rdB[t] + ca * d;
+      if (Write) S.wrB[t] = psi;
+      const T add = S.cpsiB[t] * psi;
+      const T n0 = b == 0 ? v0 + add : v0, n1 = b == 1 ? v1 + add : v1, n2 = b == 2 ? v2 + add : v2;
+      v0 = n0;
+      v1 = n1;
+      v2 = n2;
+    }
+  }
+}
+
+// H at (i, j, k) (BasicUpdator::updateH, src/updator/basic_updator.cpp:
+// 23-41, then the H-side slabs) from set A; ex0, ey_i, ez_i are E at the
+// point, ey_n / ez_n return E at i + 1 (the next plane's ey_i, ez_i).
+template <class T, class Coef, bool Write>
+__device__ __forceinline__ void stepH(const Grid& g, const FieldPtrs<T>& A, const Coef& C, const StepSlabs<T>& S,
+                                      int i, int j, int k, int64_t idx, T ex0, T ey_i, T ez_i, T& ey_n, T& ez_n,
+                                      T& vx, T& vy, T& vz) {
+  const int64_t pz = g.pz, pl = g.plane;
+  const T ex_j = A.f[0][idx + pz], ex_k = A.f[0][idx + 1];
+  const T ey_k = A.f[1][idx + 1];
+  const T ez_j = A.f[2][idx + pz];
+  ey_n = A.f[1][idx + pl];
+  ez_n = A.f[2][idx + pl];
+  T s, a, b;
+  C.getOwn(3, idx, s, a, b);
+  vx = s * ldOwn(A.f[3] + idx) + a * (ey_k - ey_i) + b * (ez_j - ez_i);
+  C.getOwn(4, idx, s, a, b);
+  vy = s * ldOwn(A.f[4] + idx) + a * (ez_n - ez_i) + b * (ex_k - ex0);
+  C.getOwn(5, idx, s, a, b);
+  vz = s * ldOwn(A.f[5] + idx) + a * (ex_j - ex0) + b * (ey_n - ey_i);
+  // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
+  if (S.n)
+    stepSlabs<T, Write>(S, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
+                        ex_k - ex0);
+}
+
+// The step pass: work-group (by, bz) owns rows [by J, by J + J) and planes
+// [bz ichunk, bz ichunk + ichunk) over all k.  Wavefront ty (0..J) is row
+// j = by J + ty - 1: ty = 0 is the halo row, which only computes H.
+template <class T, class Coef, int J>
+__global__ __launch_bounds__(64 * (J + 1)) void k_step(Grid g, FieldPtrs<T> A, FieldPtrs<T> B, Coef C,
+                                                       StepSlabs<T> SH, StepSlabs<T> SE, int ichunk, int nbj) {
+  __shared__ T xj[2][J + 1][2][64];          // per row: Hz, Hx (the next row's j-1 values)
+  __shared__ T xk[J][2][kMaxStepChunk];      // per row and plane: Hy, Hx at the k-tile's last lane
+  const int lane = threadIdx.x;
+  const int ty = threadIdx.y;
+  const int by = blockIdx.x % nbj, bz = blockIdx.x / nbj;
+  const int j = __builtin_amdgcn_readfirstlane(by * J + ty - 1);
+  const int i0 = bz * ichunk;
+  const int i1 = min(i0 + ichunk, g.nx);
+  const bool halo = ty == 0;
+  const bool row = j >= 0 && j < g.ny;
+  const int nkt = (g.nz + 63) / 64;
+  const int64_t pz = g.pz, pl = g.plane;
+  int q = 0;
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int k = kt * 64 + lane;
+    const bool act = row && k < g.nz;
+    int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+    T hz_p = T(0), hy_p = T(0), ey_i = T(0), ez_i = T(0);
+    if (act) {
+      ey_i = A.f[1][idx];
+      ez_i = A.f[2][idx];
+      if (!halo && i0 > 0) {  // H(i0 - 1): the first plane's i-1 neighbour, recomputed, not stored
+        const int64_t pidx = idx - pl;
+        T en, zn, hx, hy, hz;
+        stepH<T, Coef, false>(g, A, C, SH, i0 - 1, j, k, pidx, A.f[0][pidx], A.f[1][pidx], A.f[2][pidx], en, zn,
+                              hx, hy, hz);
+        hz_p = hz;
+        hy_p = hy;
+      }
+    }
+    for (int i = i0; i < i1; ++i, idx += pl, ++q) {
+      T hx = T(0), hy = T(0), hz = T(0), ex0 = T(0), ey_n = T(0), ez_n = T(0);
+      if (act) {
+        ex0 = A.f[0][idx];
+        if (halo)
+          stepH<T, Coef, false>(g, A, C, SH, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy, hz);
+        else
+          stepH<T, Coef, true>(g, A, C, SH, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy, hz);
+      }
+      const int par = q & 1;
+      xj[par][ty][0][lane] = hz;
+      xj[par][ty][1][lane] = hx;
+      // k-1 neighbours: lane - 1, or for lane 0 the previous k-tile's lane 63
+      T hy_km = __shfl_up(hy, 1), hx_km = __shfl_up(hx, 1);
+      if (!halo) {
+        if (lane == 0 && kt > 0) {
+          hy_km = xk[ty - 1][0][i - i0];
+          hx_km = xk[ty - 1][1][i - i0];
+        }
+        if (lane == 63) {
+          xk[ty - 1][0][i - i0] = hy;
+          xk[ty - 1][1][i - i0] = hx;
+        }
+      }
+      __syncthreads();
+      if (act && !halo) {
+        stOwn(B.f[3] + idx, hx);
+        stOwn(B.f[4] + idx, hy);
+        stOwn(B.f[5] + idx, hz);
+        // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69)
+        // and the E-side slabs; every lane stores all three components
+        // (unchanged where the Yee ranges leave a face out)
+        const T hz_jm = xj[par][ty - 1][0][lane], hx_jm = xj[par][ty - 1][1][lane];
+        const T dzHy = k >= 1 ? hy - hy_km : T(0);
+        const T dzHx = k >= 1 ? hx - hx_km : T(0);
+        const T dyHz = j >= 1 ? hz - hz_jm : T(0);
+        const T dyHx = j >= 1 ? hx - hx_jm : T(0);
+        const T dxHz = hz - hz_p, dxHy = hy - hy_p;
+        T vx = ex0, vy = ey_i, vz = ez_i;
+        T s, a, b;
+        if (j >= 1 && k >= 1) {
+          C.getOwn(0, idx, s, a, b);
+          vx = s * vx + a * dzHy + b * dyHz;
+        }
+        if (i >= 1 && k >= 1) {
+          C.getOwn(1, idx, s, a, b);
+          vy = s * vy + a * dxHz + b * dzHx;
+        }
+        if (i >= 1 && j >= 1) {
+          C.getOwn(2, idx, s, a, b);
+          vz = s * vz + a * dyHx + b * dxHy;
+        }
+        // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
+        if (SE.n) stepSlabs<T, true>(SE, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+        stOwn(B.f[0] + idx, vx);
+        stOwn(B.f[1] + idx, vy);
+        stOwn(B.f[2] + idx, vz);
+      }
+      hz_p = hz;
+      hy_p = hy;
+      ey_i = ey_n;
+      ez_i = ez_n;
+    }
+  }
+}
+
+template <class T>
+struct StepLaunch {
+  // rows per work-group (4 or 8); ichunk <= kMaxStepChunk
+  static void step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
+                   const CoefPtrs<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows, int ichunk);
+  static void step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
+                   const ClassCoef<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows, int ichunk);
+};
+
+}  // namespace xfd

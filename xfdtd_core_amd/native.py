@@ -102,6 +102,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_reset": (ctypes.c_int, [vp]),
         "xfd_read_line": (ctypes.c_int, [vp, i32, vp]),
         "xfd_nf2ff_far_field": (ctypes.c_int, [vp, i32, i32, i32, vp, vp, vp, dbl, vp, vp, vp, vp]),
+        "xfd_sim_device_context": (vp, [vp]),
         "xfd_batch_create": (ctypes.c_int, [i32, vp, ctypes.POINTER(vp)]),
         "xfd_batch_destroy": (None, [vp]),
         "xfd_batch_case": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
@@ -195,7 +196,7 @@ class GatherDesc(ctypes.Structure):
 
 
 # kernel classes of xfd_kernel_stats, in bit order
-KERNEL_CLASSES = ("update_h", "update_e", "cpml_h", "cpml_e", "records_h", "records_e", "gather", "nf2ff")
+KERNEL_CLASSES = ("update_h", "update_e", "cpml_h", "cpml_e", "records_h", "records_e", "gather", "nf2ff", "step")
 
 
 class Desc(ctypes.Structure):
@@ -483,6 +484,10 @@ class Simulation:
 
     def nf2ff_output_power(self, n: int):
         self._check(self._L.xfd_sim_nf2ff_output_power(self._h, n))
+
+    def device_context(self):
+        """The xfd_ctx of the initialised scenario (for direct C-ABI calls)."""
+        return self._L.xfd_sim_device_context(self._h)
 
     def grid_array(self, which: int, axis: int) -> np.ndarray:
         n = self._L.xfd_sim_grid_array(self._h, which, axis, None)
