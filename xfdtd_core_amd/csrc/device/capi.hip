@@ -2124,6 +2124,109 @@ int xfd_nf2ff_far_field(xfd_ctx* c, int32_t n, int32_t f, int32_t na, const doub
   });
 }
 
+}  // extern "C"
+
+// Device memory of xfd_nf2ff_far_field_noalloc: the staged grid geometry and
+// directions and the kernel's output, in one file-scope array per device
+// (held until the process ends).  In host code `scratch` names a host-side
+// stand-in, not device memory: host code reaches each device's copy only
+// through the address hipGetSymbolAddress returns with that device current,
+// taken once per device on the entry point's first call there.
+constexpr size_t kScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratchPtr(int dev) {
+  require(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    XFD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+size_t alignUp(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int xfd_nf2ff_far_field_noalloc(xfd_ctx* c, int32_t n, int32_t f, int32_t na, const double* theta,
+                                const double* phi, const double* origin, double k, const double* const* e_node,
+                                const double* const* h_node, const double* const* e_size, double* dst) {
+  return guard([&] {
+    require(c != nullptr && dst != nullptr && theta != nullptr && phi != nullptr && origin != nullptr,
+            "bad argument");
+    require(n >= 0 && n < static_cast<int32_t>(c->nf2ffs.size()), "bad NF2FF index");
+    auto& N = *c->nf2ffs[n];
+    require(f >= 0 && f < N.a.nf, "bad frequency index");
+    require(na >= 0, "bad direction count");
+    if (na == 0) return;
+    int cur = 0;
+    XFD_HIP(hipGetDevice(&cur));
+    require(cur == c->device, "the context's device must be current");
+    const int dims[3] = {c->g.nx, c->g.ny, c->g.nz};
+    // layout: per axis e_node (N+1), h_node (N), e_size (N); theta, phi;
+    // the output (na x 8)
+    size_t need = 0;
+    size_t off_geo[9];
+    for (int a = 0; a < 3; ++a) {
+      const size_t cnt[3] = {static_cast<size_t>(dims[a]) + 1, static_cast<size_t>(dims[a]),
+                             static_cast<size_t>(dims[a])};
+      for (int w = 0; w < 3; ++w) {
+        off_geo[3 * w + a] = need;
+        need += alignUp(cnt[w] * sizeof(double));
+      }
+    }
+    const size_t off_th = need;
+    need += alignUp(static_cast<size_t>(na) * sizeof(double));
+    const size_t off_ph = need;
+    need += alignUp(static_cast<size_t>(na) * sizeof(double));
+    const size_t off_out = need;
+    need += alignUp(static_cast<size_t>(na) * 8 * sizeof(double));
+    if (need > kScratchBytes)
+      throw XfdError(XFD_EINVAL, "xfd_nf2ff_far_field_noalloc: grid and " + std::to_string(na) +
+                                     " directions need " + std::to_string(need) + " bytes of scratch, above its " +
+                                     std::to_string(kScratchBytes) + "-byte maximum");
+    unsigned char* base = static_cast<unsigned char*>(scratchPtr(c->device));
+    xfd::FarArgs fa{};
+    const double* const* src[3] = {e_node, h_node, e_size};
+    for (int a = 0; a < 3; ++a) {
+      require(e_node[a] && h_node[a] && e_size[a], "grid arrays missing");
+      const size_t cnt[3] = {static_cast<size_t>(dims[a]) + 1, static_cast<size_t>(dims[a]),
+                             static_cast<size_t>(dims[a])};
+      for (int w = 0; w < 3; ++w)
+        XFD_HIP(hipMemcpyAsync(base + off_geo[3 * w + a], src[w][a], cnt[w] * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+      fa.e_node[a] = reinterpret_cast<const double*>(base + off_geo[a]);
+      fa.h_node[a] = reinterpret_cast<const double*>(base + off_geo[3 + a]);
+      fa.e_size[a] = reinterpret_cast<const double*>(base + off_geo[6 + a]);
+      fa.origin[a] = origin[a];
+    }
+    XFD_HIP(hipMemcpyAsync(base + off_th, theta, static_cast<size_t>(na) * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    XFD_HIP(hipMemcpyAsync(base + off_ph, phi, static_cast<size_t>(na) * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    fa.theta = reinterpret_cast<const double*>(base + off_th);
+    fa.phi = reinterpret_cast<const double*>(base + off_ph);
+    fa.k = k;
+    double* out = reinterpret_cast<double*>(base + off_out);
+    xfd::Launch<double>::far_field(c->stream, N.a, f, fa, na, out);
+    XFD_HIP(hipGetLastError());
+    XFD_HIP(hipMemcpyAsync(dst, out, static_cast<size_t>(na) * 8 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    XFD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
 int xfd_cpml_fused(xfd_ctx* c, int32_t* fused) {
   return guard([&] {
     require(c && fused, "bad argument");

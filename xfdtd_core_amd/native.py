@@ -102,6 +102,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_reset": (ctypes.c_int, [vp]),
         "xfd_read_line": (ctypes.c_int, [vp, i32, vp]),
         "xfd_nf2ff_far_field": (ctypes.c_int, [vp, i32, i32, i32, vp, vp, vp, dbl, vp, vp, vp, vp]),
+        "xfd_nf2ff_far_field_noalloc": (ctypes.c_int, [vp, i32, i32, i32, vp, vp, vp, dbl, vp, vp, vp, vp]),
         "xfd_sim_device_context": (vp, [vp]),
         "xfd_batch_create": (ctypes.c_int, [i32, vp, ctypes.POINTER(vp)]),
         "xfd_batch_destroy": (None, [vp]),
