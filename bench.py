@@ -167,9 +167,18 @@ def run_device(sim, precision, warmup, steps, n, pml):
             if k in kernels:
                 kernels[k]["layout_bytes"] = ib
                 kernels[k]["layout_GBps"] = round(ib / (kernels[k]["avg_ms"] * 1e-3) / 1e9, 1)
+        if "step" in kernels:
+            # the single pass: 6 field reads and 6 writes, 6 class ids per
+            # cell, and psi read/write + c_psi read of 4 components per slab
+            # cell (six slabs)
+            sb = (12.0 * s + 12.0) * cells + 12.0 * s * 6 * pml * n * n
+            avg = kernels["step"].get("timed_region", {}).get("avg_ms", kernels["step"]["avg_ms"])
+            kernels["step"]["layout_bytes"] = sb
+            kernels["step"]["layout_GBps"] = round(sb / (avg * 1e-3) / 1e9, 1)
     res["coefficients"] = "class ids" if sim.coef_classed() else "dense"
     res["cpml"] = {0: "separate slab launches", 1: "x/y slabs inside the Yee sweeps",
-                   2: "z slabs inside the Yee sweeps, x/y slabs separate"}[int(fused)]
+                   2: "z slabs inside the Yee sweeps, x/y slabs separate",
+                   3: "every slab inside the single-pass step"}[int(fused)]
     sim.finish()
     return res
 

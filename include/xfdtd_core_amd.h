@@ -305,11 +305,11 @@ int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
  *   *fused = 3: the single-pass step: one launch per step computes H, its
  *     CPML slabs, E and its CPML slabs (every slab, in boundary order, so
  *     FP64 stays bitwise), reading one set of fields and writing a second.
- *     The default for 3D grids without TFSF (its H corrections would have to
- *     reach recomputed halo points), ADE, records on a grid boundary plane
- *     or more than 8 slabs; XFD_STEP_FUSED=0, XFD_CPML_FUSED=1 or
- *     XFD_CPML_ZFUSED=1 keep the two sweeps.  XFD_STEP_ROWS (4 or 8) and
- *     XFD_STEP_CHUNK (1..64) set its work-group tile.
+ *     Opt-in with XFD_STEP_FUSED=1, for 3D grids without TFSF (its H
+ *     corrections would have to reach recomputed points), ADE, records on a
+ *     grid boundary plane or more than 8 slabs; XFD_CPML_FUSED=1 or
+ *     XFD_CPML_ZFUSED=1 keep the two sweeps.  XFD_STEP_CHUNK (1..1024)
+ *     sets the i-planes each work-item marches.
  * Modes 1 and 2 never with ADE materials or when a TFSF or lumped correction
  * lands inside a slab (the reference applies those first). */
 int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);

@@ -575,6 +575,7 @@ def test_profiling_selected_kernel_classes(monkeypatch, single_pass):
     sp = scenarios.inhomogeneous_box(n_total=40, pml=6, n_spheres=3, r_cells=(3, 5), seed=3).spec()
     say(f"[gpu] profiling selected kernel classes (single pass: {single_pass})")
     if single_pass:
+        monkeypatch.setenv("XFD_STEP_FUSED", "1")
         sim = native.Simulation(sp, "f64")
         sim.init(40)
         assert sim.cpml_fused() == 3
@@ -755,23 +756,22 @@ def test_full_size_fp64_matches_oracle():
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-@pytest.mark.parametrize("tile", [(4, 32), (8, 7), (4, 1)])
-def test_single_pass_step(monkeypatch, precision, tile):
+@pytest.mark.parametrize("chunk", [4, 7, 1])
+def test_single_pass_step(monkeypatch, precision, chunk):
     """The single-pass step (one launch per step: H, its CPML, E, its CPML;
     two field sets) against the two Yee sweeps with separate slab launches,
     on a 136^3 box (three k-tiles of 64 lanes, the last partial) with random
     spheres, the current-source dipole, six-field snapshots and probes, and
-    several work-group tiles (rows per work-group, planes per chunk; a
-    chunk of 7 does not divide the grid, 1 recomputes every plane's i-1
-    neighbour).  FP64: every field, frame and probe equal to the separate
-    launches and to the oracle; FP32: within 1e-5 of the separate launches'
-    FP32 run."""
-    rows, chunk = tile
+    several i-chunks (planes per work-item: 7 does not divide the grid, 1
+    recomputes every plane's i-1 neighbour).  FP64: every field, frame and probe equal to the separate
+    launches and to the oracle; FP32: as close to the FP64 result as the
+    separate launches' FP32 run (contraction choices differ between the two
+    FP32 kernels, so the two FP32 runs agree to ~1e-5, not bitwise)."""
     sp = scenarios.inhomogeneous_box(n_total=136, pml=9, n_spheres=12, r_cells=(4, 14), seed=11,
                                      snapshot_interval=40, probes=(2, 2, 2)).spec()
     steps = 120
-    say(f"[gpu] single-pass step {precision} rows {rows} chunk {chunk}")
-    monkeypatch.setenv("XFD_STEP_ROWS", str(rows))
+    say(f"[gpu] single-pass step {precision} chunk {chunk}")
+    monkeypatch.setenv("XFD_STEP_FUSED", "1")
     monkeypatch.setenv("XFD_STEP_CHUNK", str(chunk))
     one = native.Simulation(sp, precision)
     one.run(steps)
@@ -781,19 +781,26 @@ def test_single_pass_step(monkeypatch, precision, tile):
     sep = native.Simulation(sp, precision)
     sep.run(steps)
     assert sep.cpml_fused() == 0
+    if precision == "f32":
+        r64 = native.Simulation(sp, "f64")
+        r64.run(steps)
     for f in FIELDS:
         a, b = one.field(f), sep.field(f)
         if precision == "f64":
             np.testing.assert_array_equal(a, b, err_msg=f)
         else:
-            assert rel_l2(a, b) < 1e-5, (f, rel_l2(a, b))
+            r = r64.field(f)
+            e_one, e_sep = rel_l2(a, r), rel_l2(b, r)
+            say(f"  {f}: single pass vs f64 {e_one:.2e}, separate vs f64 {e_sep:.2e}, between {rel_l2(a, b):.2e}")
+            assert rel_l2(a, b) < 1e-4 and e_one <= 2 * e_sep + 1e-6, (f, e_one, e_sep)
     for m in range(one.monitor_count()):
-        for a, b in zip(one.monitor_records(m), sep.monitor_records(m)):
+        for q, (a, b) in enumerate(zip(one.monitor_records(m), sep.monitor_records(m))):
             if precision == "f64":
                 np.testing.assert_array_equal(a, b, err_msg=f"monitor {m}")
             elif np.abs(b).max() > 0:
-                assert rel_l2(a, b) < 1e-5, (m, rel_l2(a, b))
-    if precision == "f64" and tile == (4, 32):
+                r = r64.monitor_records(m)[q]
+                assert rel_l2(a, r) <= 2 * rel_l2(b, r) + 1e-6, (m, rel_l2(a, r), rel_l2(b, r))
+    if precision == "f64" and chunk == 4:
         say("[gpu] single-pass step: oracle")
         ref = RefSim(sp, threads=(8, 1, 1))
         ref.run(steps)
@@ -804,11 +811,12 @@ def test_single_pass_step(monkeypatch, precision, tile):
                 np.testing.assert_array_equal(a, b, err_msg=f"monitor {m}")
 
 
-def test_single_pass_step_field_io():
+def test_single_pass_step_field_io(monkeypatch):
     """xfd_read_field / xfd_write_field around an odd number of single-pass
     steps (the second field set current): a cavity mode stepped 31 times
     equals the oracle; the state read back, Ez doubled and written, stepped
     30 more times equals the oracle started from that state, bitwise."""
+    monkeypatch.setenv("XFD_STEP_FUSED", "1")
     n, dl = (24, 20, 16), (1e-3, 1.2e-3, 0.9e-3)
     ref = RefSim(em.cavity_spec(n, dl))
     ref.init(31)

@@ -293,7 +293,7 @@ struct xfd_ctx {
   // the H-side CPML psi; `parity` says which set holds the current state
   // (0: fields / psi, 1: fields_b / psi_h2).
   bool fstep = false;
-  int fstep_rows = 4, fstep_chunk = 32;
+  int fstep_chunk = 4;
   int parity = 0;
   DevBuf fields_b[6];
   bool fused = false;
@@ -1109,14 +1109,14 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   {  // The single-pass step (k_step) where every correction between the H and
     // E half-steps is a CPML slab: no TFSF H records, no ADE, no record on a
     // point a slab corrects or on a grid boundary plane (the pass leaves the
-    // far planes alone), at most kMaxStepSlabs slabs.  XFD_STEP_FUSED=0 keeps
-    // the separate sweeps; XFD_STEP_ROWS (4 or 8) and XFD_STEP_CHUNK set the
-    // work-group tile.
+    // far planes alone), at most kMaxStepSlabs slabs.  Opt-in with
+    // XFD_STEP_FUSED=1 (measured slower than the two sweeps on the 384^3 box);
+    // XFD_STEP_CHUNK sets the i-planes per work-item.
     const char* env = std::getenv("XFD_STEP_FUSED");
     const char* fenv = std::getenv("XFD_CPML_FUSED");
     const char* zenv1 = std::getenv("XFD_CPML_ZFUSED");
     // an explicit request for slabs inside the two sweeps keeps the sweeps
-    const bool off = (env && env[0] == '0') || (fenv && fenv[0] == '1') || (zenv1 && zenv1[0] == '1');
+    const bool off = !(env && env[0] == '1') || (fenv && fenv[0] == '1') || (zenv1 && zenv1[0] == '1');
     bool edge = false;
     for (const auto& r : re) {
       const int64_t i = r.target / g.plane, rem = r.target % g.plane, j = rem / g.pz, k = rem % g.pz;
@@ -1129,7 +1129,6 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       c.n_fused = 0;
       c.zfused = false;
       std::fill(c.zslab.begin(), c.zslab.end(), 0);
-      if (const char* e = std::getenv("XFD_STEP_ROWS")) c.fstep_rows = std::atoi(e) == 8 ? 8 : 4;
       if (const char* e = std::getenv("XFD_STEP_CHUNK")) {
         const int v = std::atoi(e);
         require(v >= 1 && v <= xfd::kMaxStepChunk, "XFD_STEP_CHUNK out of range");
@@ -1326,9 +1325,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       const auto SH = stepSlabs<T>(c, false), SE = stepSlabs<T>(c, true);
       c.timed(K_STEP, [&] {
         if (c.classed)
-          xfd::StepLaunch<T>::step(st, c.g, F, FB, K, SH, SE, c.fstep_rows, c.fstep_chunk);
+          xfd::StepLaunch<T>::step(st, c.g, F, FB, K, SH, SE, c.fstep_chunk);
         else
-          xfd::StepLaunch<T>::step(st, c.g, F, FB, C, SH, SE, c.fstep_rows, c.fstep_chunk);
+          xfd::StepLaunch<T>::step(st, c.g, F, FB, C, SH, SE, c.fstep_chunk);
       });
       c.parity ^= 1;
       F = FB;

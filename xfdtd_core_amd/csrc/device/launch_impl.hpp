@@ -351,25 +351,20 @@ void AdjLaunch<T>::source(hipStream_t s, const FieldPtrs<T>& L, const AdjPoint* 
 
 template <class T, class Coef>
 void launchStep(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B, const Coef& C,
-                const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows, int ichunk) {
-  const int nbj = static_cast<int>(blocks_for(g.ny, rows));
-  const dim3 grid(static_cast<unsigned>(nbj) * blocks_for(g.nx, ichunk));
-  if (rows == 8)
-    hipLaunchKernelGGL((k_step<T, Coef, 8>), grid, dim3(64, 9), 0, s, g, A, B, C, SH, SE, ichunk, nbj);
-  else
-    hipLaunchKernelGGL((k_step<T, Coef, 4>), grid, dim3(64, 5), 0, s, g, A, B, C, SH, SE, ichunk, nbj);
+                const StepSlabs<T>& SH, const StepSlabs<T>& SE, int ichunk) {
+  const SweepMap m = sweepMap(g, ichunk);
+  hipLaunchKernelGGL((k_step<T, Coef>), dim3(static_cast<unsigned>(m.nbx * m.nby * m.nbz)),
+                     dim3(kLanesK, kRowsJ), 0, s, g, A, B, C, SH, SE, m, ichunk);
 }
 template <class T>
 void StepLaunch<T>::step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
-                         const CoefPtrs<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows,
-                         int ichunk) {
-  launchStep<T>(s, g, A, B, C, SH, SE, rows, ichunk);
+                         const CoefPtrs<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int ichunk) {
+  launchStep<T>(s, g, A, B, C, SH, SE, ichunk);
 }
 template <class T>
 void StepLaunch<T>::step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
-                         const ClassCoef<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows,
-                         int ichunk) {
-  launchStep<T>(s, g, A, B, C, SH, SE, rows, ichunk);
+                         const ClassCoef<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int ichunk) {
+  launchStep<T>(s, g, A, B, C, SH, SE, ichunk);
 }
 
 }  // namespace xfd

@@ -97,6 +97,19 @@ struct CoefPtrs {
     a = ldOwn(c[3 * q + 1] + idx);
     b = ldOwn(c[3 * q + 2] + idx);
   }
+  // split access (a load issued ahead of its use, then its values): the
+  // three coefficients themselves
+  struct Raw {
+    T s, a, b;
+  };
+  __device__ __forceinline__ Raw raw(int q, int64_t idx) const {
+    return Raw{ldOwn(c[3 * q] + idx), ldOwn(c[3 * q + 1] + idx), ldOwn(c[3 * q + 2] + idx)};
+  }
+  __device__ __forceinline__ void resolve(int, const Raw& r, T& s, T& a, T& b) const {
+    s = r.s;
+    a = r.a;
+    b = r.b;
+  }
 };
 
 template <class T>
@@ -112,6 +125,18 @@ struct ClassCoef {
   // the same for the cell a sweep owns (its id is read once per pass)
   __device__ __forceinline__ void getOwn(int q, int64_t idx, T& s, T& a, T& b) const {
     const T* e = tab[q] + 3 * static_cast<int>(ldOwn(id[q] + idx));
+    s = e[0];
+    a = e[1];
+    b = e[2];
+  }
+  // split access (a load issued ahead of its use, then its values): the
+  // class id, then its table entry
+  struct Raw {
+    int id;
+  };
+  __device__ __forceinline__ Raw raw(int q, int64_t idx) const { return Raw{static_cast<int>(ldOwn(id[q] + idx))}; }
+  __device__ __forceinline__ void resolve(int q, const Raw& r, T& s, T& a, T& b) const {
+    const T* e = tab[q] + 3 * r.id;
     s = e[0];
     a = e[1];
     b = e[2];

@@ -3,18 +3,11 @@
 //
 // The reference runs updateH, the H correctors, updateE and the E correctors
 // as separate sweeps over the grid (src/domain/domain.cpp:82-135), so every
-// field is read twice and written once per step.  Here a work-group marches
-// along i over a chunk of planes for a tile of J rows (j) times the whole k
-// extent, in k-tiles of 64 lanes, and computes E(i) right after H(i) while
-// H(i), H(i-1), H(j-1) and H(k-1) are still on chip:
-//   * H(i-1) is the previous plane of the march (registers); the chunk's
-//     first plane recomputes H(i0-1) without storing it;
-//   * H(j-1) of the tile's first row comes from an extra wavefront that
-//     recomputes row j0-1 (the halo row) and hands it over through LDS with
-//     the other rows' values;
-//   * H(k-1) is the neighbouring lane's value (a lane shuffle), and for lane
-//     0 the previous k-tile's lane 63, kept in LDS per plane.
-// Halo and chunk-start recomputation read the neighbours' old state, which
+// field is read twice and written once per step.  Here each lane marches
+// along i over a chunk of planes and computes E(i) right after H(i), with
+// the H values of its j-1 / k-1 neighbours recomputed or shuffled in and
+// H(i-1) carried in registers (k_step).
+// The recomputations read the neighbours' old state, which
 // their owners overwrite in the same launch; so E, H and the H-side CPML psi
 // are double-buffered: the step reads set A and writes set B, and the
 // buffers swap roles every step.  The E-side psi is read and written by its
@@ -24,7 +17,7 @@
 // the same order (Yee terms left to right, then every slab containing the
 // point in boundary order), so FP64 results are bitwise those of the
 // reference's order.  Corrections that are not point-local -- TFSF H
-// records, which would have to reach the halo and chunk-start copies -- keep
+// records, which would have to reach the recomputed copies -- keep
 // the separate sweeps; E records (lumped sources, TFSF E) follow the pass as
 // before, which is the reference's order wherever no record shares a point
 // with a slab.
@@ -40,7 +33,7 @@
 namespace xfd {
 
 constexpr int kMaxStepSlabs = 8;
-constexpr int kMaxStepChunk = 64;
+constexpr int kMaxStepChunk = 1024;
 
 // One CPML slab as the step pass sees it: its box (grid indices), the
 // components' extents along (a, b), the per-layer recursive-convolution
@@ -114,7 +107,9 @@ __device__ __forceinline__ void stepSlabs(const StepSlabs<T>& L, int i, int j, i
 
 // H at (i, j, k) (BasicUpdator::updateH, src/updator/basic_updator.cpp:
 // 23-41, then the H-side slabs) from set A; ex0, ey_i, ez_i are E at the
-// point, ey_n / ez_n return E at i + 1 (the next plane's ey_i, ez_i).
+// point, ey_n / ez_n return E at i + 1 (the next plane's ey_i, ez_i).  Plain
+// cached loads: rows and planes of set A are read by their owner and by the
+// neighbouring work-groups' halo and chunk-start recomputations.
 template <class T, class Coef, bool Write>
 __device__ __forceinline__ void stepH(const Grid& g, const FieldPtrs<T>& A, const Coef& C, const StepSlabs<T>& S,
                                       int i, int j, int k, int64_t idx, T ex0, T ey_i, T ez_i, T& ey_n, T& ez_n,
@@ -125,128 +120,121 @@ __device__ __forceinline__ void stepH(const Grid& g, const FieldPtrs<T>& A, cons
   const T ez_j = A.f[2][idx + pz];
   ey_n = A.f[1][idx + pl];
   ez_n = A.f[2][idx + pl];
+  const T hx0 = A.f[3][idx], hy0 = A.f[4][idx], hz0 = A.f[5][idx];
   T s, a, b;
-  C.getOwn(3, idx, s, a, b);
-  vx = s * ldOwn(A.f[3] + idx) + a * (ey_k - ey_i) + b * (ez_j - ez_i);
-  C.getOwn(4, idx, s, a, b);
-  vy = s * ldOwn(A.f[4] + idx) + a * (ez_n - ez_i) + b * (ex_k - ex0);
-  C.getOwn(5, idx, s, a, b);
-  vz = s * ldOwn(A.f[5] + idx) + a * (ex_j - ex0) + b * (ey_n - ey_i);
+  C.get(3, idx, s, a, b);
+  vx = s * hx0 + a * (ey_k - ey_i) + b * (ez_j - ez_i);
+  C.get(4, idx, s, a, b);
+  vy = s * hy0 + a * (ez_n - ez_i) + b * (ex_k - ex0);
+  C.get(5, idx, s, a, b);
+  vz = s * hz0 + a * (ex_j - ex0) + b * (ey_n - ey_i);
   // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
   if (S.n)
     stepSlabs<T, Write>(S, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
                         ex_k - ex0);
 }
 
-// The step pass: work-group (by, bz) owns rows [by J, by J + J) and planes
-// [bz ichunk, bz ichunk + ichunk) over all k.  Wavefront ty (0..J) is row
-// j = by J + ty - 1: ty = 0 is the halo row, which only computes H.
-template <class T, class Coef, int J>
-__global__ __launch_bounds__(64 * (J + 1)) void k_step(Grid g, FieldPtrs<T> A, FieldPtrs<T> B, Coef C,
-                                                       StepSlabs<T> SH, StepSlabs<T> SE, int ichunk, int nbj) {
-  __shared__ T xj[2][J + 1][2][64];          // per row: Hz, Hx (the next row's j-1 values)
-  __shared__ T xk[J][2][kMaxStepChunk];      // per row and plane: Hy, Hx at the k-tile's last lane
+// The step pass over the sweep tiles (64 k-lanes by kRowsJ rows, marching a
+// chunk of i-planes; work-group order as the Yee sweeps').  Each lane
+// computes, per plane, H at its point and the H values its E update needs
+// from the neighbours that the lanes and rows around it own:
+//   * H(i-1): the previous plane of the march (registers); the chunk's first
+//     plane recomputes H(i0-1) without storing it;
+//   * Hz, Hx at j-1: recomputed by every lane from set A (the row is another
+//     wavefront's, its loads hit the caches);
+//   * Hy, Hx at k-1: the neighbouring lane's values (a lane shuffle), lane 0
+//     recomputing them at k0-1.
+// No LDS and no barrier: every wavefront runs on its own.
+template <class T, class Coef>
+__global__ __launch_bounds__(kTileThreads) void k_step(Grid g, FieldPtrs<T> A, FieldPtrs<T> B, Coef C,
+                                                       StepSlabs<T> SH, StepSlabs<T> SE, SweepMap M, int ichunk) {
+  int bx, by, bz;
+  M.coords(bx, by, bz);
   const int lane = threadIdx.x;
-  const int ty = threadIdx.y;
-  const int by = blockIdx.x % nbj, bz = blockIdx.x / nbj;
-  const int j = __builtin_amdgcn_readfirstlane(by * J + ty - 1);
+  const int k = bx * kLanesK + lane;
+  const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
+  if (j >= g.ny || i0 >= g.nx) return;  // whole rows leave together: the shuffles below stay uniform
+  const bool act = k < g.nz;
   const int i1 = min(i0 + ichunk, g.nx);
-  const bool halo = ty == 0;
-  const bool row = j >= 0 && j < g.ny;
-  const int nkt = (g.nz + 63) / 64;
   const int64_t pz = g.pz, pl = g.plane;
-  int q = 0;
-  for (int kt = 0; kt < nkt; ++kt) {
-    const int k = kt * 64 + lane;
-    const bool act = row && k < g.nz;
-    int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
-    T hz_p = T(0), hy_p = T(0), ey_i = T(0), ez_i = T(0);
-    if (act) {
-      ey_i = A.f[1][idx];
-      ez_i = A.f[2][idx];
-      if (!halo && i0 > 0) {  // H(i0 - 1): the first plane's i-1 neighbour, recomputed, not stored
-        const int64_t pidx = idx - pl;
-        T en, zn, hx, hy, hz;
-        stepH<T, Coef, false>(g, A, C, SH, i0 - 1, j, k, pidx, A.f[0][pidx], A.f[1][pidx], A.f[2][pidx], en, zn,
-                              hx, hy, hz);
-        hz_p = hz;
-        hy_p = hy;
-      }
-    }
-    for (int i = i0; i < i1; ++i, idx += pl, ++q) {
-      T hx = T(0), hy = T(0), hz = T(0), ex0 = T(0), ey_n = T(0), ez_n = T(0);
-      if (act) {
-        ex0 = A.f[0][idx];
-        if (halo)
-          stepH<T, Coef, false>(g, A, C, SH, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy, hz);
-        else
-          stepH<T, Coef, true>(g, A, C, SH, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy, hz);
-      }
-      const int par = q & 1;
-      xj[par][ty][0][lane] = hz;
-      xj[par][ty][1][lane] = hx;
-      // k-1 neighbours: lane - 1, or for lane 0 the previous k-tile's lane 63
-      T hy_km = __shfl_up(hy, 1), hx_km = __shfl_up(hx, 1);
-      if (!halo) {
-        if (lane == 0 && kt > 0) {
-          hy_km = xk[ty - 1][0][i - i0];
-          hx_km = xk[ty - 1][1][i - i0];
-        }
-        if (lane == 63) {
-          xk[ty - 1][0][i - i0] = hy;
-          xk[ty - 1][1][i - i0] = hx;
-        }
-      }
-      __syncthreads();
-      if (act && !halo) {
-        stOwn(B.f[3] + idx, hx);
-        stOwn(B.f[4] + idx, hy);
-        stOwn(B.f[5] + idx, hz);
-        // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69)
-        // and the E-side slabs; every lane stores all three components
-        // (unchanged where the Yee ranges leave a face out)
-        const T hz_jm = xj[par][ty - 1][0][lane], hx_jm = xj[par][ty - 1][1][lane];
-        const T dzHy = k >= 1 ? hy - hy_km : T(0);
-        const T dzHx = k >= 1 ? hx - hx_km : T(0);
-        const T dyHz = j >= 1 ? hz - hz_jm : T(0);
-        const T dyHx = j >= 1 ? hx - hx_jm : T(0);
-        const T dxHz = hz - hz_p, dxHy = hy - hy_p;
-        T vx = ex0, vy = ey_i, vz = ez_i;
-        T s, a, b;
-        if (j >= 1 && k >= 1) {
-          C.getOwn(0, idx, s, a, b);
-          vx = s * vx + a * dzHy + b * dyHz;
-        }
-        if (i >= 1 && k >= 1) {
-          C.getOwn(1, idx, s, a, b);
-          vy = s * vy + a * dxHz + b * dzHx;
-        }
-        if (i >= 1 && j >= 1) {
-          C.getOwn(2, idx, s, a, b);
-          vz = s * vz + a * dyHx + b * dxHy;
-        }
-        // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
-        if (SE.n) stepSlabs<T, true>(SE, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
-        stOwn(B.f[0] + idx, vx);
-        stOwn(B.f[1] + idx, vy);
-        stOwn(B.f[2] + idx, vz);
-      }
+  int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  T hz_p = T(0), hy_p = T(0), ey_i = T(0), ez_i = T(0);
+  if (act) {
+    ey_i = A.f[1][idx];
+    ez_i = A.f[2][idx];
+    if (i0 > 0) {  // H(i0 - 1), recomputed, not stored
+      const int64_t p = idx - pl;
+      T en, zn, hx, hy, hz;
+      stepH<T, Coef, false>(g, A, C, SH, i0 - 1, j, k, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn, hx, hy, hz);
       hz_p = hz;
       hy_p = hy;
-      ey_i = ey_n;
-      ez_i = ez_n;
     }
+  }
+  for (int i = i0; i < i1; ++i, idx += pl) {
+    T hx = T(0), hy = T(0), hz = T(0), ex0 = T(0), ey_n = T(0), ez_n = T(0);
+    if (act) {
+      ex0 = A.f[0][idx];
+      stepH<T, Coef, true>(g, A, C, SH, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy, hz);
+    }
+    T hy_km = __shfl_up(hy, 1), hx_km = __shfl_up(hx, 1);
+    if (!act) continue;
+    if (lane == 0 && k >= 1) {  // H at k0 - 1
+      const int64_t p = idx - 1;
+      T en, zn, vx, vy, vz;
+      stepH<T, Coef, false>(g, A, C, SH, i, j, k - 1, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn, vx, vy, vz);
+      hy_km = vy;
+      hx_km = vx;
+    }
+    T hz_jm = T(0), hx_jm = T(0);
+    if (j >= 1) {  // H at j - 1
+      const int64_t p = idx - pz;
+      T en, zn, vy;
+      stepH<T, Coef, false>(g, A, C, SH, i, j - 1, k, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn, hx_jm, vy, hz_jm);
+    }
+    stOwn(B.f[3] + idx, hx);
+    stOwn(B.f[4] + idx, hy);
+    stOwn(B.f[5] + idx, hz);
+    // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69) and the
+    // E-side slabs; every lane stores all three components (unchanged where
+    // the Yee ranges leave a face out)
+    const T dzHy = k >= 1 ? hy - hy_km : T(0);
+    const T dzHx = k >= 1 ? hx - hx_km : T(0);
+    const T dyHz = j >= 1 ? hz - hz_jm : T(0);
+    const T dyHx = j >= 1 ? hx - hx_jm : T(0);
+    const T dxHz = hz - hz_p, dxHy = hy - hy_p;
+    T vx = ex0, vy = ey_i, vz = ez_i;
+    T s, a, b;
+    if (j >= 1 && k >= 1) {
+      C.getOwn(0, idx, s, a, b);
+      vx = s * vx + a * dzHy + b * dyHz;
+    }
+    if (i >= 1 && k >= 1) {
+      C.getOwn(1, idx, s, a, b);
+      vy = s * vy + a * dxHz + b * dzHx;
+    }
+    if (i >= 1 && j >= 1) {
+      C.getOwn(2, idx, s, a, b);
+      vz = s * vz + a * dyHx + b * dxHy;
+    }
+    // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
+    if (SE.n) stepSlabs<T, true>(SE, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+    stOwn(B.f[0] + idx, vx);
+    stOwn(B.f[1] + idx, vy);
+    stOwn(B.f[2] + idx, vz);
+    hz_p = hz;
+    hy_p = hy;
+    ey_i = ey_n;
+    ez_i = ez_n;
   }
 }
 
 template <class T>
 struct StepLaunch {
-  // rows per work-group (4 or 8); ichunk <= kMaxStepChunk
   static void step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
-                   const CoefPtrs<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows, int ichunk);
+                   const CoefPtrs<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int ichunk);
   static void step(hipStream_t s, const Grid& g, const FieldPtrs<T>& A, const FieldPtrs<T>& B,
-                   const ClassCoef<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int rows, int ichunk);
+                   const ClassCoef<T>& C, const StepSlabs<T>& SH, const StepSlabs<T>& SE, int ichunk);
 };
 
 }  // namespace xfd
