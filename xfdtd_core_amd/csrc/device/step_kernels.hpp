@@ -61,22 +61,45 @@ struct StepSlabs {
   StepSlab<T> s[kMaxStepSlabs];
 };
 
+// Bit n set: coordinate p lies in slab n's box along axis `ax`.  A point is
+// in slab n when its three coordinates are, so the membership mask of a
+// point is the AND of three per-axis masks; the i mask is uniform over a
+// plane, the j mask over a row, and the k mask fixed over a lane's march, so
+// each is taken once and points outside every slab cost one test.
+template <class T>
+__device__ __forceinline__ uint32_t axisMask(const StepSlabs<T>& L, int ax, int p) {
+  uint32_t m = 0;
+  for (int n = 0; n < L.n; ++n)
+    if (static_cast<unsigned>(p - L.s[n].lo[ax]) < static_cast<unsigned>(L.s[n].ext[ax])) m |= 1u << n;
+  return m;
+}
+
+// The union of a per-lane mask over the wavefront.
+__device__ __forceinline__ uint32_t waveUnion(uint32_t m) {
+  uint32_t u = 0;
+  for (int n = 0; n < 32; ++n)
+    if (__any((m >> n) & 1u)) u |= 1u << n;
+  return u;
+}
+
 // The CPML corrections of point (i, j, k) for every slab of L containing
-// it, in boundary order, as cpmlFused: component A = (c+1)%3 takes the dual
-// difference dA[c], component B = (c+2)%3 takes dB[c];
+// it (bits of m), in boundary order, as cpmlFused: component A = (c+1)%3
+// takes the dual difference dA[c], component B = (c+2)%3 takes dB[c];
 //   psi = b psi + a d,  v += c_psi psi.
-// Write = false (recomputed halo and chunk-start points) leaves psi as it is.
+// Write = false (recomputed neighbour points) leaves psi as it is.
+// mu is m's union over the wavefront (the slab loop stays uniform, so the
+// slab's parameters are scalar loads); lanes outside slab n skip it.
 template <class T, bool Write>
-__device__ __forceinline__ void stepSlabs(const StepSlabs<T>& L, int i, int j, int k, T& v0, T& v1, T& v2,
-                                          const T dA0, const T dA1, const T dA2, const T dB0, const T dB1,
-                                          const T dB2) {
-  for (int n = 0; n < L.n; ++n) {
+__device__ __forceinline__ void stepSlabs(const StepSlabs<T>& L, uint32_t mu, uint32_t m, int i, int j, int k,
+                                          T& v0, T& v1, T& v2, const T dA0, const T dA1, const T dA2, const T dB0,
+                                          const T dB1, const T dB2) {
+  mu = __builtin_amdgcn_readfirstlane(mu);
+  while (mu) {
+    const int n = __builtin_ctz(mu);
+    mu &= mu - 1;
+    if (!((m >> n) & 1u)) continue;
     const StepSlab<T>& S = L.s[n];
     const int u = i - S.lo[0], v = j - S.lo[1], w = k - S.lo[2];
-    if (static_cast<unsigned>(u) >= static_cast<unsigned>(S.ext[0]) ||
-        static_cast<unsigned>(v) >= static_cast<unsigned>(S.ext[1]) ||
-        static_cast<unsigned>(w) >= static_cast<unsigned>(S.ext[2]))
-      continue;
     const int c = S.axis, a = c == 2 ? 0 : c + 1, b = c == 0 ? 2 : c - 1;
     const int64_t t = (static_cast<int64_t>(u) * S.ext[1] + v) * S.ext[2] + w;
     const int layer = pick3(c, i, j, k) - S.start;
@@ -106,32 +129,34 @@ __device__ __forceinline__ void stepSlabs(const StepSlabs<T>& L, int i, int j, i
 }
 
 // H at (i, j, k) (BasicUpdator::updateH, src/updator/basic_updator.cpp:
-// 23-41, then the H-side slabs) from set A; ex0, ey_i, ez_i are E at the
-// point, ey_n / ez_n return E at i + 1 (the next plane's ey_i, ez_i).  Plain
-// cached loads: rows and planes of set A are read by their owner and by the
-// neighbouring work-groups' halo and chunk-start recomputations.
-template <class T, class Coef, bool Write>
+// 23-41, then the H-side slabs of mask m) from set A; ex0, ey_i, ez_i are E
+// at the point, ey_n / ez_n return E at i + 1 (the next plane's ey_i,
+// ez_i).  NeedY = false skips Hy and the loads only it uses (vy returns 0).
+template <class T, class Coef, bool Write, bool NeedY = true>
 __device__ __forceinline__ void stepH(const Grid& g, const FieldPtrs<T>& A, const Coef& C, const StepSlabs<T>& S,
-                                      int i, int j, int k, int64_t idx, T ex0, T ey_i, T ez_i, T& ey_n, T& ez_n,
-                                      T& vx, T& vy, T& vz) {
+                                      uint32_t mu, uint32_t m, int i, int j, int k, int64_t idx, T ex0, T ey_i,
+                                      T ez_i, T& ey_n, T& ez_n, T& vx, T& vy, T& vz) {
   const int64_t pz = g.pz, pl = g.plane;
-  const T ex_j = A.f[0][idx + pz], ex_k = A.f[0][idx + 1];
+  const T ex_j = A.f[0][idx + pz];
+  const T ex_k = NeedY ? A.f[0][idx + 1] : T(0);
   const T ey_k = A.f[1][idx + 1];
   const T ez_j = A.f[2][idx + pz];
   ey_n = A.f[1][idx + pl];
-  ez_n = A.f[2][idx + pl];
-  const T hx0 = A.f[3][idx], hy0 = A.f[4][idx], hz0 = A.f[5][idx];
+  ez_n = NeedY ? A.f[2][idx + pl] : T(0);
   T s, a, b;
   C.get(3, idx, s, a, b);
-  vx = s * hx0 + a * (ey_k - ey_i) + b * (ez_j - ez_i);
-  C.get(4, idx, s, a, b);
-  vy = s * hy0 + a * (ez_n - ez_i) + b * (ex_k - ex0);
+  vx = s * A.f[3][idx] + a * (ey_k - ey_i) + b * (ez_j - ez_i);
+  vy = T(0);
+  if (NeedY) {
+    C.get(4, idx, s, a, b);
+    vy = s * A.f[4][idx] + a * (ez_n - ez_i) + b * (ex_k - ex0);
+  }
   C.get(5, idx, s, a, b);
-  vz = s * hz0 + a * (ex_j - ex0) + b * (ey_n - ey_i);
+  vz = s * A.f[5][idx] + a * (ex_j - ex0) + b * (ey_n - ey_i);
   // A: Hy(x) <- Ez, Hz(y) <- Ex, Hx(z) <- Ey;  B: Hz(x) <- Ey, Hx(y) <- Ez, Hy(z) <- Ex
-  if (S.n)
-    stepSlabs<T, Write>(S, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i, ez_j - ez_i,
-                        ex_k - ex0);
+  if (mu)
+    stepSlabs<T, Write>(S, mu, m, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i, ey_n - ey_i,
+                        ez_j - ez_i, ex_k - ex0);
 }
 
 // The step pass over the sweep tiles (64 k-lanes by kRowsJ rows, marching a
@@ -159,6 +184,13 @@ __global__ __launch_bounds__(kTileThreads) void k_step(Grid g, FieldPtrs<T> A, F
   const int i1 = min(i0 + ichunk, g.nx);
   const int64_t pz = g.pz, pl = g.plane;
   int64_t idx = (static_cast<int64_t>(i0) * g.NY + j) * pz + k;
+  // slab membership per axis (H and E slabs), for the point and for its
+  // recomputed j-1 and k-1 neighbours
+  // (i and j are uniform over a wavefront, k is not: its masks also get
+  // their wavefront union)
+  const uint32_t hj = axisMask(SH, 1, j), hjm = axisMask(SH, 1, j - 1), ej = axisMask(SE, 1, j);
+  const uint32_t hk = axisMask(SH, 2, k), hkm = axisMask(SH, 2, k - 1), ek = axisMask(SE, 2, k);
+  const uint32_t hku = waveUnion(act ? hk : 0u), hkmu = waveUnion(act ? hkm : 0u), eku = waveUnion(act ? ek : 0u);
   T hz_p = T(0), hy_p = T(0), ey_i = T(0), ez_i = T(0);
   if (act) {
     ey_i = A.f[1][idx];
@@ -166,31 +198,37 @@ __global__ __launch_bounds__(kTileThreads) void k_step(Grid g, FieldPtrs<T> A, F
     if (i0 > 0) {  // H(i0 - 1), recomputed, not stored
       const int64_t p = idx - pl;
       T en, zn, hx, hy, hz;
-      stepH<T, Coef, false>(g, A, C, SH, i0 - 1, j, k, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn, hx, hy, hz);
+      const uint32_t hp = axisMask(SH, 0, i0 - 1) & hj;
+      stepH<T, Coef, false>(g, A, C, SH, hp & hku, hp & hk, i0 - 1, j, k, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn,
+                            hx, hy, hz);
       hz_p = hz;
       hy_p = hy;
     }
   }
   for (int i = i0; i < i1; ++i, idx += pl) {
+    const uint32_t hi = axisMask(SH, 0, i);
     T hx = T(0), hy = T(0), hz = T(0), ex0 = T(0), ey_n = T(0), ez_n = T(0);
     if (act) {
       ex0 = A.f[0][idx];
-      stepH<T, Coef, true>(g, A, C, SH, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy, hz);
+      stepH<T, Coef, true>(g, A, C, SH, hi & hj & hku, hi & hj & hk, i, j, k, idx, ex0, ey_i, ez_i, ey_n, ez_n, hx, hy,
+                           hz);
     }
     T hy_km = __shfl_up(hy, 1), hx_km = __shfl_up(hx, 1);
     if (!act) continue;
     if (lane == 0 && k >= 1) {  // H at k0 - 1
       const int64_t p = idx - 1;
       T en, zn, vx, vy, vz;
-      stepH<T, Coef, false>(g, A, C, SH, i, j, k - 1, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn, vx, vy, vz);
+      stepH<T, Coef, false>(g, A, C, SH, hi & hj & hkmu, hi & hj & hkm, i, j, k - 1, p, A.f[0][p], A.f[1][p],
+                            A.f[2][p], en, zn, vx, vy, vz);
       hy_km = vy;
       hx_km = vx;
     }
     T hz_jm = T(0), hx_jm = T(0);
-    if (j >= 1) {  // H at j - 1
+    if (j >= 1) {  // Hz and Hx at j - 1
       const int64_t p = idx - pz;
       T en, zn, vy;
-      stepH<T, Coef, false>(g, A, C, SH, i, j - 1, k, p, A.f[0][p], A.f[1][p], A.f[2][p], en, zn, hx_jm, vy, hz_jm);
+      stepH<T, Coef, false, false>(g, A, C, SH, hi & hjm & hku, hi & hjm & hk, i, j - 1, k, p, A.f[0][p], A.f[1][p],
+                                   A.f[2][p], en, zn, hx_jm, vy, hz_jm);
     }
     stOwn(B.f[3] + idx, hx);
     stOwn(B.f[4] + idx, hy);
@@ -218,7 +256,8 @@ __global__ __launch_bounds__(kTileThreads) void k_step(Grid g, FieldPtrs<T> A, F
       vz = s * vz + a * dyHx + b * dxHy;
     }
     // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
-    if (SE.n) stepSlabs<T, true>(SE, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+    const uint32_t ei = axisMask(SE, 0, i) & ej;
+    if (ei & eku) stepSlabs<T, true>(SE, ei & eku, ei & ek, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
     stOwn(B.f[0] + idx, vx);
     stOwn(B.f[1] + idx, vy);
     stOwn(B.f[2] + idx, vz);
