@@ -58,7 +58,7 @@ def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0) 
         return step_alg_bytes(n, pml, s)
     cells = n ** 3
     slab_cells = pml * (n + 1) * (n + 1)
-    inside = {0: 0, 1: 4, 2: 2, 3: 6}[int(fused)]
+    inside = {0: 0, 1: 4, 2: 2, 3: 6, 4: 6}[int(fused)]
     if kernel in ("update_h", "update_e"):
         return 15.0 * s * cells + 6.0 * s * inside * slab_cells
     if kernel in ("cpml_h", "cpml_e"):
@@ -178,7 +178,8 @@ def run_device(sim, precision, warmup, steps, n, pml):
     res["coefficients"] = "class ids" if sim.coef_classed() else "dense"
     res["cpml"] = {0: "separate slab launches", 1: "x/y slabs inside the Yee sweeps",
                    2: "z slabs inside the Yee sweeps, x/y slabs separate",
-                   3: "every slab inside the single-pass step"}[int(fused)]
+                   3: "every slab inside the single-pass step",
+                   4: "every slab inside the two Yee sweeps"}[int(fused)]
     sim.finish()
     return res
 

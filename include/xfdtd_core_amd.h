@@ -310,8 +310,12 @@ int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
  *     grid boundary plane or more than 8 slabs; XFD_CPML_FUSED=1 or
  *     XFD_CPML_ZFUSED=1 keep the two sweeps.  XFD_STEP_CHUNK (1..1024)
  *     sets the i-planes each work-item marches.
- * Modes 1 and 2 never with ADE materials or when a TFSF or lumped correction
- * lands inside a slab (the reference applies those first). */
+ *   *fused = 4: every slab inside the two sweeps, in boundary order per
+ *     point (FP64 bitwise), each slab's parameters as kernel arguments and a
+ *     point's membership from per-axis masks.  Opt-in with
+ *     XFD_CPML_INSWEEP=1 (slower than modes 0 and 2 on MI355X).
+ * Modes 1, 2 and 4 never with ADE materials or when a TFSF or lumped
+ * correction lands inside a slab (the reference applies those first). */
 int xfd_cpml_fused(xfd_ctx* ctx, int32_t* fused);
 
 /* Zeroes a context's time-stepping state: fields, CPML auxiliaries, inductor

@@ -315,6 +315,7 @@ def test_cpml_inside_sweeps_matches_separate_launches(monkeypatch, precision):
     sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
     say(f"[gpu] fused CPML vs separate {precision}")
     monkeypatch.setenv("XFD_STEP_FUSED", "0")
+    monkeypatch.setenv("XFD_CPML_INSWEEP", "0")
     monkeypatch.setenv("XFD_CPML_FUSED", "1")
     fused = native.Simulation(sp, precision)
     fused.run(200)
@@ -363,6 +364,7 @@ def test_z_cpml_inside_sweeps(monkeypatch, precision):
     sp = scenarios.inhomogeneous_box(n_total=48, pml=8, n_spheres=6, r_cells=(3, 7), seed=5).spec()
     say(f"[gpu] z CPML inside the sweeps vs separate {precision}")
     monkeypatch.setenv("XFD_STEP_FUSED", "0")
+    monkeypatch.setenv("XFD_CPML_INSWEEP", "0")
     monkeypatch.delenv("XFD_CPML_FUSED", raising=False)
     monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
     for nz, want in ((192, 0), (193, 2 if precision == "f32" else 0)):  # 3 and 4 tiles of 64 k-lanes
@@ -778,6 +780,7 @@ def test_single_pass_step(monkeypatch, precision, chunk):
     assert one.cpml_fused() == 3
     monkeypatch.setenv("XFD_STEP_FUSED", "0")
     monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
+    monkeypatch.setenv("XFD_CPML_INSWEEP", "0")
     sep = native.Simulation(sp, precision)
     sep.run(steps)
     assert sep.cpml_fused() == 0
@@ -841,3 +844,53 @@ def test_single_pass_step_field_io(monkeypatch):
     ref2.loop()
     for f in FIELDS:
         np.testing.assert_array_equal(ctx.read(f), ref2.field(f), err_msg=f)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_all_slabs_inside_sweeps(monkeypatch, precision):
+    """Every CPML slab applied inside the two Yee sweeps (XFD_CPML_INSWEEP=1,
+    where no record lands in a slab and no ADE is present; the far grid
+    planes in k_cpml_faces) against the separate slab launches, on a 136^3 box (three
+    k-tiles, the last partial, with lanes of both z slabs) and on the README
+    scenario (TFSF records outside the slabs).  FP64: every field and
+    recording equal to the separate launches, and the README case equal to
+    the oracle; FP32: as close to FP64 as the separate launches."""
+    cases = {
+        "box": (scenarios.inhomogeneous_box(n_total=136, pml=9, n_spheres=12, r_cells=(4, 14), seed=11,
+                                            snapshot_interval=40, probes=(2, 2, 2)).spec(), 120),
+        "readme": (scenarios.dielectric_sphere().spec(), 300),
+    }
+    for name, (sp, steps) in cases.items():
+        say(f"[gpu] all slabs inside the sweeps: {name} {precision}")
+        monkeypatch.setenv("XFD_CPML_INSWEEP", "1")
+        ins = native.Simulation(sp, precision)
+        ins.run(steps)
+        assert ins.cpml_fused() == 4
+        monkeypatch.setenv("XFD_CPML_INSWEEP", "0")
+        monkeypatch.setenv("XFD_CPML_ZFUSED", "0")
+        sep = native.Simulation(sp, precision)
+        sep.run(steps)
+        assert sep.cpml_fused() == 0
+        if precision == "f32":
+            r64 = native.Simulation(sp, "f64")
+            r64.run(steps)
+        for f in FIELDS:
+            a, b = ins.field(f), sep.field(f)
+            if precision == "f64":
+                np.testing.assert_array_equal(a, b, err_msg=f"{name} {f}")
+            else:
+                r = r64.field(f)
+                e_in, e_sep = rel_l2(a, r), rel_l2(b, r)
+                say(f"  {name} {f}: in-sweep vs f64 {e_in:.2e}, separate vs f64 {e_sep:.2e}")
+                assert e_in <= 2 * e_sep + 1e-6, (name, f, e_in, e_sep)
+        for m in range(ins.monitor_count()):
+            for q, (a, b) in enumerate(zip(ins.monitor_records(m), sep.monitor_records(m))):
+                if precision == "f64":
+                    np.testing.assert_array_equal(a, b, err_msg=f"{name} monitor {m}")
+        if precision == "f64" and name == "readme":
+            ref = RefSim(sp)
+            ref.run(steps)
+            for f in FIELDS:
+                np.testing.assert_array_equal(ins.field(f), ref.field(f), err_msg=f)
+        monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
+

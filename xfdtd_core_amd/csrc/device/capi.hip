@@ -294,6 +294,9 @@ struct xfd_ctx {
   // (0: fields / psi, 1: fields_b / psi_h2).
   bool fstep = false;
   int fstep_chunk = 4;
+  // every CPML slab applied inside the two Yee sweeps (kernel-argument slab
+  // list), the far grid planes by k_cpml_faces
+  bool allslabs = false;
   int parity = 0;
   DevBuf fields_b[6];
   bool fused = false;
@@ -1140,13 +1143,33 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       c.parity = 0;
     }
   }
+  {  // Every slab inside the two sweeps (boundary order per point, so FP64
+    // stays bitwise; the far planes follow in k_cpml_faces), for 3D grids
+    // without ADE, without records on points a slab corrects and with at
+    // most kMaxStepSlabs slabs.  Opt-in with XFD_CPML_INSWEEP=1: measured on
+    // the 384^3 box it is slower than the separate slab launches (FP32 1.41
+    // against 1.24 ms per step, FP64 2.27 against 2.24 ms).
+    const char* env = std::getenv("XFD_CPML_INSWEEP");
+    const char* fenv = std::getenv("XFD_CPML_FUSED");
+    const char* zenv1 = std::getenv("XFD_CPML_ZFUSED");
+    const bool other = (fenv && fenv[0] == '1') || (zenv1 && zenv1[0] == '1');
+    c.allslabs = (env && env[0] == '1') && !other && !c.fstep && c.dim == 3 && c.ade_scheme < 0 &&
+                 !records_meet_slabs && !c.slabs.empty() && c.slabs.size() <= static_cast<size_t>(xfd::kMaxStepSlabs);
+    if (c.allslabs) {
+      c.fused = false;
+      c.n_fused = 0;
+      c.zfused = false;
+      std::fill(c.zslab.begin(), c.zslab.end(), 0);
+    }
+  }
   XFD_HIP(hipStreamSynchronize(c.stream));
 }
 
-// The slabs of one half-step as the single-pass step reads them: H side
-// with psi from the current set into the other, E side in place.
+// The slabs of one half-step as the sweeps read them: in place, or for the
+// single-pass step's H side (pingpong) from the current psi set into the
+// other.
 template <class T>
-xfd::StepSlabs<T> stepSlabs(xfd_ctx& c, bool is_e) {
+xfd::StepSlabs<T> stepSlabs(xfd_ctx& c, bool is_e, bool pingpong = true) {
   xfd::StepSlabs<T> L{};
   for (auto& sp : c.slabs) {
     const xfd::SlabArgs<T> a = slabArgs<T>(*sp, is_e);
@@ -1165,7 +1188,7 @@ xfd::StepSlabs<T> stepSlabs(xfd_ctx& c, bool is_e) {
     S.cpsiB = a.cpsiB;
     S.ca = a.ca;
     S.cb = a.cb;
-    if (is_e) {
+    if (is_e || !pingpong) {
       S.rdA = S.wrA = a.psiA;
       S.rdB = S.wrB = a.psiB;
     } else {
@@ -1316,7 +1339,12 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
   }
   xfd::ZCpml<T> ZE{}, ZH{};
   if (c.zfused) zCpml<T>(c, ZE, ZH);
-  auto inSweep = [&](size_t q) { return static_cast<int>(q) < c.n_fused; };
+  xfd::StepSlabs<T> AH{}, AE{};
+  if (c.allslabs) {
+    AH = stepSlabs<T>(c, false, false);
+    AE = stepSlabs<T>(c, true, false);
+  }
+  auto inSweep = [&](size_t q) { return c.allslabs || static_cast<int>(q) < c.n_fused; };
   for (int64_t t = t0; t < t0 + n; ++t) {
     if (c.fstep) {
       // H, its CPML, E and its CPML in one pass from the current set into
@@ -1339,9 +1367,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         else
           xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, true);
       } else if (c.classed) {
-        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH, ZH);
+        xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH, ZH, AH);
       } else {
-        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk, PH, ZH);
+        xfd::Launch<T>::update_h(st, c.g, F, C, c.ichunk, PH, ZH, AH);
       }
     });
     if (c.nseg_h)
@@ -1370,9 +1398,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         else
           xfd::Launch<T>::update_e_ade(st, c.g, F, C, A, c.ade_scheme, c.ichunk);
       } else if (c.classed) {
-        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk, PE, ZE);
+        xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk, PE, ZE, AE);
       } else {
-        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk, PE, ZE);
+        xfd::Launch<T>::update_e(st, c.g, F, C, c.ichunk, PE, ZE, AE);
       }
     });
     }
@@ -2229,7 +2257,7 @@ int xfd_nf2ff_far_field_noalloc(xfd_ctx* c, int32_t n, int32_t f, int32_t na, co
 int xfd_cpml_fused(xfd_ctx* c, int32_t* fused) {
   return guard([&] {
     require(c && fused, "bad argument");
-    *fused = c->fstep ? 3 : (c->fused ? 1 : (c->zfused ? 2 : 0));
+    *fused = c->fstep ? 3 : (c->allslabs ? 4 : (c->fused ? 1 : (c->zfused ? 2 : 0)));
   });
 }
 
@@ -2253,6 +2281,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       if (xfd_create(&descs[i], &c) != XFD_OK) throw XfdError(XFD_EINVAL, std::string("case ") + std::to_string(i) + ": " + g_err);
       b->cases.emplace_back(c);
       leaveStep(*c);  // the batched launches step set 0 with their own tables
+      c->allslabs = false;
       // batched cases keep their z slabs as separate launches: with most
       // wavefronts of these small boxes carrying slab lanes, the sweeps were
       // measured slower with them inside

@@ -16,6 +16,7 @@
 #pragma once
 
 #include "layout.hpp"
+#include "slab_inline.hpp"
 
 namespace xfd {
 
@@ -40,9 +41,6 @@ static_assert(kLanesK % 64 == 0 && kTileThreads <= 1024, "sweep tile");
 // difference the sweep already holds: dA[c] for component A, dB[c] for B.
 // Same operations in the same order as k_cpml after the sweep, so results
 // are unchanged; the field and its dual are not read again.
-// Component c of a point, without a runtime-indexed local array (which the
-// compiler would place in scratch memory).
-__device__ __forceinline__ int pick3(int c, int x, int y, int z) { return c == 0 ? x : (c == 1 ? y : z); }
 
 // Bit n set: slab n of P contains (i, j, k) for some i of [i0, i1).  Taken
 // once per thread and i-chunk, so the sweep's inner loop only tests i.
@@ -195,10 +193,10 @@ __device__ __forceinline__ void zStore(const ZCpml<T>& Z, const ZLane<T>& L, con
 // One work-group tile: k-block bx, j-block by, i-chunk bz.  Fused: the CPML
 // H corrections of the visited points in the same pass (the H dual
 // differences along x, y, z are dual[idx + s_c] - dual[idx]).
-template <class T, class Coef, bool Fused = false, bool ZF = false>
+template <class T, class Coef, bool Fused = false, bool ZF = false, bool All = false>
 __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
                                        int ichunk, const CpmlList<T>& P = CpmlList<T>{},
-                                       const ZCpml<T>& Z = ZCpml<T>{}) {
+                                       const ZCpml<T>& Z = ZCpml<T>{}, const StepSlabs<T>& S = StepSlabs<T>{}) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -217,6 +215,9 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
   ZLane<T> zl;
   if (ZF) zl = zLane(Z, i0, j, k);
+  // All: every slab of S (its j and k masks, the k mask's wavefront union)
+  const uint32_t sj = All ? axisMask(S, 1, j) : 0u, sk = All ? axisMask(S, 2, k) : 0u;
+  const uint32_t sku = All ? waveUnion(sk) : 0u;
   for (int i = i0; i < i1; ++i, idx += pl) {
     ZVals<T> zv;
     if (ZF) zv = zLoad<T, false>(Z, zl, i, nullptr, nullptr, idx);
@@ -237,6 +238,12 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
       zStore(Z, zl, zv, vx, vy, ey_k - ey_i, ex_k - ex0);
       zl.t += zl.step;
     }
+    if (All) {
+      const uint32_t si = axisMask(S, 0, i) & sj;
+      if (si & sku)
+        stepSlabs<T, true>(S, si & sku, si & sk, i, j, k, vx, vy, vz, ez_n - ez_i, ex_j - ex0, ey_k - ey_i,
+                           ey_n - ey_i, ez_j - ez_i, ex_k - ex0);
+    }
     stOwn(hx + idx, vx);
     stOwn(hy + idx, vy);
     stOwn(hz + idx, vz);
@@ -245,12 +252,12 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   }
 }
 
-template <class T, class Coef, bool Fused, bool ZF>
+template <class T, class Coef, bool Fused, bool ZF, bool All>
 __global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
-                                                  CpmlList<T> P, ZCpml<T> Z) {
+                                                  CpmlList<T> P, ZCpml<T> Z, StepSlabs<T> S) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepH<T, Coef, Fused, ZF>(g, F, C, bx, by, bz, ichunk, P, Z);
+  sweepH<T, Coef, Fused, ZF, All>(g, F, C, bx, by, bz, ichunk, P, Z, S);
 }
 
 // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
@@ -259,10 +266,10 @@ __global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> 
 // visited points (i < nx, j < ny, k < nz) in the same pass, including the
 // face points a Yee range leaves out; the dual differences along x, y, z are
 // dual[idx] - dual[idx - s_c].
-template <class T, class Coef, bool Fused = false, bool ZF = false>
+template <class T, class Coef, bool Fused = false, bool ZF = false, bool All = false>
 __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
                                        int ichunk, const CpmlList<T>& P = CpmlList<T>{},
-                                       const ZCpml<T>& Z = ZCpml<T>{}) {
+                                       const ZCpml<T>& Z = ZCpml<T>{}, const StepSlabs<T>& S = StepSlabs<T>{}) {
   const int k = bx * kLanesK + threadIdx.x;
   const int j = by * kRowsJ + threadIdx.y;
   const int i0 = bz * ichunk;
@@ -282,12 +289,15 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
   ZLane<T> zl;
   if (ZF) zl = zLane(Z, i0, j, k);
+  const uint32_t sj = All ? axisMask(S, 1, j) : 0u, sk = All ? axisMask(S, 2, k) : 0u;
+  const uint32_t sku = All ? waveUnion(sk) : 0u;
   for (int i = i0; i < i1; ++i, idx += pl) {
     ZVals<T> zv;
     if (ZF) zv = zLoad<T, true>(Z, zl, i, hy, hx, idx);  // Ex <- Hy, Ey <- Hx (k >= 1 in a z slab)
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
     T s, a, b;
-    if (!Fused || !mask) {
+    const uint32_t si = All ? axisMask(S, 0, i) & sj : 0u;
+    if (All ? !(si & sku) : (!Fused || !mask)) {
       // every lane stores its three values (unchanged on the faces the Yee
       // ranges leave out), so each wavefront writes whole lines: skipping the
       // j = 0 / k = 0 lanes left partial lines and cost 6 % of the FP32 sweep
@@ -333,7 +343,10 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
         vz = s * vz + a * dyHx + b * dxHy;
       }
       // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
-      cpmlFused(P, mask, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+      if (All)
+        stepSlabs<T, true>(S, si & sku, si & sk, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
+      else
+        cpmlFused(P, mask, i, j, k, vx, vy, vz, dxHz, dyHx, dzHy, dxHy, dyHz, dzHx);
       ex[idx] = vx;
       ey[idx] = vy;
       ez[idx] = vz;
@@ -343,12 +356,12 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   }
 }
 
-template <class T, class Coef, bool Fused, bool ZF>
+template <class T, class Coef, bool Fused, bool ZF, bool All>
 __global__ __launch_bounds__(kTileThreads) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
-                                                  CpmlList<T> P, ZCpml<T> Z) {
+                                                  CpmlList<T> P, ZCpml<T> Z, StepSlabs<T> S) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepE<T, Coef, Fused, ZF>(g, F, C, bx, by, bz, ichunk, P, Z);
+  sweepE<T, Coef, Fused, ZF, All>(g, F, C, bx, by, bz, ichunk, P, Z, S);
 }
 
 // 2D (TE) grid, nz = 1, in a layout of pitch pz = 2 along k (k = 1 holds
@@ -1108,14 +1121,20 @@ struct Launch {
   // P.n > 0: the sweep applies the slabs of P in the same pass; Z.n > 0:
   // the sweep applies the z slabs of Z (then cpml_faces completes either on
   // the far grid planes)
+  // S.n > 0: the sweep applies every slab of S (in boundary order; P and Z
+  // unused)
   static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
+                       const StepSlabs<T>& S = StepSlabs<T>{});
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
+                       const StepSlabs<T>& S = StepSlabs<T>{});
   static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
+                       const StepSlabs<T>& S = StepSlabs<T>{});
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{});
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
+                       const StepSlabs<T>& S = StepSlabs<T>{});
   // the Yee half-steps of a 2D (TE, dim 2) or 1D (TEM, dim 1) grid; the 1D
   // H half-step is the 3D sweep (update_h)
   static void update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int dim,

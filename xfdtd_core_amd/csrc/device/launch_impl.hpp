@@ -29,47 +29,51 @@ inline SweepMap sweepMap(const Grid& g, int ichunk) {
 
 template <class T, class Coef>
 void launchUpdate(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk,
-                  const CpmlList<T>& P, const ZCpml<T>& Z) {
+                  const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S = StepSlabs<T>{}) {
   dim3 block(kLanesK, kRowsJ);
   const SweepMap m = sweepMap(g, ichunk);
   dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz));
-  const bool fused = P.n > 0, zf = !fused && Z.n > 0;
+  const bool all = S.n > 0, fused = !all && P.n > 0, zf = !all && !fused && Z.n > 0;
   if (is_h) {
-    if (fused)
-      hipLaunchKernelGGL((k_update_h<T, Coef, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+    if (all)
+      hipLaunchKernelGGL((k_update_h<T, Coef, false, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
+    else if (fused)
+      hipLaunchKernelGGL((k_update_h<T, Coef, true, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
     else if (zf)
-      hipLaunchKernelGGL((k_update_h<T, Coef, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+      hipLaunchKernelGGL((k_update_h<T, Coef, false, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
     else
-      hipLaunchKernelGGL((k_update_h<T, Coef, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+      hipLaunchKernelGGL((k_update_h<T, Coef, false, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
   } else {
-    if (fused)
-      hipLaunchKernelGGL((k_update_e<T, Coef, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+    if (all)
+      hipLaunchKernelGGL((k_update_e<T, Coef, false, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
+    else if (fused)
+      hipLaunchKernelGGL((k_update_e<T, Coef, true, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
     else if (zf)
-      hipLaunchKernelGGL((k_update_e<T, Coef, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+      hipLaunchKernelGGL((k_update_e<T, Coef, false, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
     else
-      hipLaunchKernelGGL((k_update_e<T, Coef, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z);
+      hipLaunchKernelGGL((k_update_e<T, Coef, false, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
   }
 }
 
 template <class T>
 void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                         const CpmlList<T>& P, const ZCpml<T>& Z) {
-  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z);
+                         const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z, S);
 }
 template <class T>
 void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int ichunk,
-                         const CpmlList<T>& P, const ZCpml<T>& Z) {
-  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z);
+                         const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z, S);
 }
 template <class T>
 void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                         const CpmlList<T>& P, const ZCpml<T>& Z) {
-  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z);
+                         const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z, S);
 }
 template <class T>
 void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
-                         const CpmlList<T>& P, const ZCpml<T>& Z) {
-  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z);
+                         const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z, S);
 }
 template <class T, class Coef>
 void launchLow(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int dim, bool is_h) {
