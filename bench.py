@@ -521,6 +521,74 @@ def bench_nf2ff_td(args):
     return res
 
 
+def bench_low_dim(args):
+    """The 2D (TE) and 1D (TEM) paths: the reference's cylinder-scatter
+    example (TFSF2D at phi = pi/4, PEC cylinder, 10-cell x/y PML, Ez probes)
+    on an n2d x n2d grid, and its slab reflection example (TFSF1D, z PML,
+    Ex probes) at its own size (300 cells, launch-bound), FP64 and FP32.
+    Algorithmic bytes of a TE cell-step: Ez, Hx, Hy read and written and the
+    seven coefficients their updates use (chxh, chxez, chyh, chyez, ceze,
+    cezhx, cezhy), 13 s.  The CPU baseline is the oracle on the 2D case."""
+    from xfdtd_core_amd import native, scenarios
+
+    n2d, steps = args.n2d, args.low_steps
+    sp2 = scenarios.cylinder_scatter_2d(nx=n2d - 20, ny=n2d - 20, movie_interval=0, probes=8).spec()
+    sp1 = scenarios.slab_line_1d(movie_interval=0).spec()
+    res = {"workload": f"2D TE cylinder scatter {n2d}x{n2d} incl. 10-cell PML (reference example "
+                       f"cylinder_scatter_2d at a larger size), 1D TEM slab line 320 cells", "steps": steps}
+    cells2 = n2d * n2d
+    for prec in ("f64", "f32"):
+        s = 8 if prec == "f64" else 4
+        row = {}
+        for label, sp, cells in (("te2d", sp2, cells2), ("tem1d", sp1, None)):
+            sim = native.Simulation(sp, prec)
+            sim.init(args.warmup + 20 + steps)
+            cells = cells or int(np.prod(sim.grid()))
+            sim.set_profiling(False)
+            sim.advance(0, args.warmup)
+            sim.set_profiling(True)
+            sim.advance(args.warmup, 20)
+            stats = sim.kernel_stats()
+            sim.set_profiling(False)
+            native.device_synchronize()
+            t0 = time.perf_counter()
+            sim.advance(args.warmup + 20, steps)
+            native.device_synchronize()
+            wall = time.perf_counter() - t0
+            sim.finish()
+            kern = {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 5)} for k, v in stats.items() if v[0]}
+            r = {"cells": cells, "grid": list(sim.grid()), "ms_per_step": round(wall / steps * 1e3, 4),
+                 "mcells_per_s": round(cells * steps / wall / 1e6, 1), "kernels": kern}
+            if label == "te2d":
+                yee_ms = sum(kern[k]["avg_ms"] for k in ("update_h", "update_e") if k in kern)
+                alg = 13.0 * s * cells
+                r["roofline"] = {"bound": "hbm", "kernel": "update_h + update_e (k_te_h, k_te_e)",
+                                 "achieved": round(alg / (yee_ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                                 "unit": "GB/s", "frac": round(alg / (yee_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                 "alg_bytes_per_step": alg}
+            row[label] = r
+            log(f"{label} {prec}: {r['mcells_per_s']:.0f} Mcells/s, {r['ms_per_step']:.4f} ms/step")
+            sim.close()
+        res[prec] = row
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle import RefSim  # CPU restatement of the reference path
+
+        cs = min(steps, 40)
+        ref = RefSim(sp2)
+        ref.init(cs + 1)
+        cores = max(1, min(len(os.sched_getaffinity(0)), physical_cores() or 16))
+        ref.set_threads(cores, 1, 1)
+        ref.loop()
+        t = ref.step_end_seconds()
+        sec = t[-1] - t[0]
+        res["cpu_baseline"] = {"value": round(cells2 * (len(t) - 1) / sec / 1e6, 2), "unit": "Mcells/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"the same 2D scenario, FP64 oracle, {len(t) - 1} steps after 1 warm-up "
+                                         f"({sec:.1f} s), ThreadConfig{{{cores},1,1}}"}
+    return res
+
+
 def bench_adjoint(args):
     """Adjoint inverse design at full size: FP32 forward with checkpoints
     every 100 steps and the reverse-time adjoint; optionally the FP64 adjoint
@@ -581,9 +649,13 @@ def main():
     ap.add_argument("--precision", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint", "dispersive", "nf2ff_td"], default="box",
+    ap.add_argument("--workload", choices=["box", "design_sweep", "adjoint", "dispersive", "nf2ff_td", "low_dim"],
+                    default="box",
                     help="box: the 384^3 headline; design_sweep: the batched form; adjoint: the gradient form; "
-                         "dispersive: the 256^3 ADE + lumped + NF2FF box; nf2ff_td: the time-domain NF2FF cost")
+                         "dispersive: the 256^3 ADE + lumped + NF2FF box; nf2ff_td: the time-domain NF2FF cost; "
+                         "low_dim: the 2D (TE) and 1D (TEM) paths")
+    ap.add_argument("--n2d", type=int, default=4096, help="low_dim: 2D grid cells per axis including the PML")
+    ap.add_argument("--low-steps", type=int, default=1000)
     ap.add_argument("--td-steps", type=int, default=1000)
     ap.add_argument("--dispersive-steps", type=int, default=3000)
     ap.add_argument("--dispersive-check", type=int, default=250)
@@ -610,6 +682,9 @@ def main():
         return
     if args.workload == "nf2ff_td":
         print(json.dumps(bench_nf2ff_td(args)), flush=True)
+        return
+    if args.workload == "low_dim":
+        print(json.dumps(bench_low_dim(args)), flush=True)
         return
     if args.workload == "adjoint":
         print(json.dumps({"workload": "adjoint inverse design 160^3", **bench_adjoint(args)}), flush=True)
