@@ -467,7 +467,7 @@ def _dispersive_error(spec, steps, check):
         sim.close()
     # bounded: the FP64 field's maximum stays within twice the largest value
     # it took during the first third of the run (the pulse and the sources)
-    early = max(r["fp64_max_abs_E"] for r in rows if r["step"] <= steps // 3) if rows else 0.0
+    early = max((r["fp64_max_abs_E"] for r in rows if r["step"] <= max(steps // 3, check)), default=0.0)
     bounded = [r for r in rows if r["fp64_max_abs_E"] <= 2 * early and r["field_state_rel_l2"] is not None]
     last = bounded[-1] if bounded else None
     return {"reference": "FP64 run of the same scenario on the GPU (equal to the oracle)",
