@@ -455,8 +455,11 @@ def _dispersive_error(spec, steps, check):
                 pa = [a.nf2ff_potential(0, f, ap, tc, theta, 0.0) for ap in (True, False) for tc in (True, False)]
                 pb = [b.nf2ff_potential(0, f, ap, tc, theta, 0.0) for ap in (True, False) for tc in (True, False)]
                 scale = max(float(np.abs(x).max()) for x in pb)
-                pot_err = max([pot_err] + [float(np.abs(x - y).max()) / scale for x, y in zip(pa, pb)])
-                pow_err = max(pow_err, abs(a.nf2ff_power(0, f) / b.nf2ff_power(0, f) - 1))
+                if scale > 0:  # nothing has reached the NF2FF surfaces yet: no figure
+                    pot_err = max([pot_err] + [float(np.abs(x - y).max()) / scale for x, y in zip(pa, pb)])
+                pw = b.nf2ff_power(0, f)
+                if pw > 0:
+                    pow_err = max(pow_err, abs(a.nf2ff_power(0, f) / pw - 1))
             row["far_field_potential_rel_max"] = pot_err
             row["radiated_power_rel_max"] = pow_err
         rows.append(row)
