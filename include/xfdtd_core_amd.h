@@ -224,7 +224,14 @@ int xfd_step(xfd_ctx* ctx, int64_t t0, int64_t n);
 int xfd_read_field(xfd_ctx* ctx, int32_t field, double* dst);
 int xfd_write_field(xfd_ctx* ctx, int32_t field, const double* src);
 
-/* Copies recording `g` (count x box volume doubles) to the host. */
+/* Copies recording `g` (count x box volume doubles) to the host.  Movies
+ * (four or more frames of 4 MiB or more) are staged while the steps run:
+ * each frame is gathered into one of two device slots and copied to pinned
+ * host memory on a second stream (MovieMonitor::update's cadence,
+ * movie_monitor.cpp:26-34), so device memory holds two frames whatever the
+ * run length; smaller recordings stay on the device until read.
+ * XFD_STAGE=0 keeps every recording on the device, =1 stages any recording
+ * of two or more frames. */
 int xfd_read_gather(xfd_ctx* ctx, int32_t g, double* dst);
 
 /* Copies weighted-sum recording `m` (count doubles) to the host. */

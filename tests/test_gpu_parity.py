@@ -894,3 +894,43 @@ def test_all_slabs_inside_sweeps(monkeypatch, precision):
                 np.testing.assert_array_equal(ins.field(f), ref.field(f), err_msg=f)
         monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
 
+
+
+def test_staged_recordings(monkeypatch):
+    """Recordings staged to pinned host memory at their cadence (two device
+    frame slots, copies on a second stream) equal the same recordings kept on
+    the device, bitwise: movies of six-field 136^3 frames every 10 steps
+    (staged by default: 20 MB frames), stepped in several calls, and the
+    README movie forced into staging (XFD_STAGE=1), whose frames also equal
+    the oracle's."""
+    big = scenarios.inhomogeneous_box(n_total=136, pml=9, n_spheres=8, r_cells=(4, 12), seed=3,
+                                      snapshot_interval=10, probes=(1, 1, 1)).spec()
+    readme = scenarios.dielectric_sphere().spec()
+    for name, sp, steps, force in (("box", big, 95, None), ("readme", readme, 400, "1")):
+        say(f"[gpu] staged recordings: {name}")
+        runs = {}
+        for mode in (force or "", "0"):
+            if mode:
+                monkeypatch.setenv("XFD_STAGE", mode)
+            else:
+                monkeypatch.delenv("XFD_STAGE", raising=False)
+            sim = native.Simulation(sp, "f64")
+            sim.init(steps)
+            for t0 in range(0, steps, 37):  # several calls, frames across their ends
+                sim.advance(t0, min(37, steps - t0))
+            sim.finish()
+            runs[mode or "default"] = [sim.monitor_records(m) for m in range(sim.monitor_count())]
+            sim.close()
+        a, b = list(runs.values())
+        nframes = 0
+        for ra, rb in zip(a, b):
+            assert len(ra) == len(rb)
+            for x, y in zip(ra, rb):
+                np.testing.assert_array_equal(x, y)
+                nframes += 1
+        say(f"  {name}: {nframes} recordings equal")
+        if name == "readme":
+            ref = RefSim(sp)
+            ref.run(steps)
+            for x, y in zip(a[0], ref.monitor_records(0)):
+                np.testing.assert_array_equal(x, y)

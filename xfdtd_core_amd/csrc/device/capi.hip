@@ -169,6 +169,9 @@ bool classify(const double* A, const double* B, const double* C, const int ext[3
   return true;
 }
 
+// Set while xfd_batch_create builds its cases.
+thread_local bool g_batch_create = false;
+
 enum KernelClass {
   K_UPDATE_H = 0, K_UPDATE_E, K_CPML_H, K_CPML_E, K_REC_H, K_REC_E, K_GATHER, K_NF2FF, K_STEP, K_COUNT
 };
@@ -273,7 +276,22 @@ struct xfd_ctx {
     xfd::GatherArgs a;
     DevBuf out;
     int probe = -1;  // row in the batched probe buffer, or -1
+    // Staged recording (movie-size frames): frames go to a ring of two
+    // device slots and each is copied to pinned host memory on the copy
+    // stream as soon as it is gathered; a slot is reused once its previous
+    // frame's copy is done.
+    bool staged = false;
+    double* host = nullptr;  // count x frame_elems, pinned
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    hipEvent_t gathered = nullptr;
+    ~Gather() {
+      for (auto e : copied)
+        if (e) (void)hipEventDestroy(e);
+      if (gathered) (void)hipEventDestroy(gathered);
+      if (host) (void)hipHostFree(host);
+    }
   };
+  hipStream_t copy_stream = nullptr;  // staged frames' device-to-host copies
   std::vector<std::unique_ptr<Gather>> gathers;
   // weighted-sum recordings (voltage / current monitors): one launch per step
   DevBuf line_terms, line_off, line_scale, line_out;
@@ -319,6 +337,9 @@ struct xfd_ctx {
 
   ~xfd_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    gathers.clear();  // their events and pinned buffers before the streams
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     for (auto& p : pending) {
       event_pool.push_back(p.second.first);
       event_pool.push_back(p.second.second);
@@ -995,9 +1016,27 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
                         (G.end[2] - G.origin[2]);
     const bool probe = gg->a.frame_elems == 1 && G.first == 0 && G.interval == 1 && G.count > 0 &&
                        (c.nprobe == 0 || G.count == c.probe_count);
+    // Movies (four or more frames of 4 MiB or more): staged to the host at
+    // their cadence, so device memory holds two frames whatever the run
+    // length, and the copies overlap the stepping (XFD_STAGE=0 keeps every
+    // frame on the device, =1 stages any recording of several frames);
+    // batched cases record on the device.
+    const char* senv = std::getenv("XFD_STAGE");
+    const size_t fbytes = static_cast<size_t>(gg->a.frame_elems) * sizeof(double);
+    const bool stage = !g_batch_create && !probe && G.count >= 2 && fbytes > 0 &&
+                       ((senv && senv[0] == '1') || (!(senv && senv[0] == '0') && G.count >= 4 &&
+                                                     fbytes >= (size_t(4) << 20)));
     if (probe) {
       gg->probe = c.nprobe++;
       c.probe_count = G.count;
+    } else if (stage) {
+      gg->staged = true;
+      gg->out.alloc(2 * fbytes);
+      XFD_HIP(hipHostMalloc(reinterpret_cast<void**>(&gg->host), fbytes * static_cast<size_t>(G.count)));
+      std::memset(gg->host, 0, fbytes * static_cast<size_t>(G.count));
+      for (auto& e : gg->copied) XFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      XFD_HIP(hipEventCreateWithFlags(&gg->gathered, hipEventDisableTiming));
+      if (!c.copy_stream) XFD_HIP(hipStreamCreate(&c.copy_stream));
     } else {
       gg->out.alloc(static_cast<size_t>(gg->a.frame_elems * G.count) * sizeof(double));
     }
@@ -1433,6 +1472,19 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       if (t < d.first || (t - d.first) % d.interval != 0) continue;
       const int64_t frame = (t - d.first) / d.interval;
       if (frame >= d.count) continue;
+      if (gp->staged) {
+        const int slot = static_cast<int>(frame & 1);
+        double* ring = gp->out.as<double>() + slot * gp->a.frame_elems;
+        const size_t fb = static_cast<size_t>(gp->a.frame_elems) * sizeof(double);
+        if (frame >= 2) XFD_HIP(hipStreamWaitEvent(st, gp->copied[slot], 0));  // the slot's last copy is done
+        c.timed(K_GATHER, [&] { xfd::Launch<T>::gather(st, c.g, F, gp->a, ring); });
+        XFD_HIP(hipEventRecord(gp->gathered, st));
+        XFD_HIP(hipStreamWaitEvent(c.copy_stream, gp->gathered, 0));
+        XFD_HIP(hipMemcpyAsync(gp->host + frame * gp->a.frame_elems, ring, fb, hipMemcpyDeviceToHost,
+                               c.copy_stream));
+        XFD_HIP(hipEventRecord(gp->copied[slot], c.copy_stream));
+        continue;
+      }
       c.timed(K_GATHER, [&] {
         xfd::Launch<T>::gather(st, c.g, F, gp->a, gp->out.as<double>() + frame * gp->a.frame_elems);
       });
@@ -1485,7 +1537,11 @@ void resetState(xfd_ctx& c) {
     zero(n->prev_b);
     zero(n->pot);
   }
-  for (auto& gg : c.gathers) zero(gg->out);
+  if (c.copy_stream) XFD_HIP(hipStreamSynchronize(c.copy_stream));
+  for (auto& gg : c.gathers) {
+    zero(gg->out);
+    if (gg->staged) std::memset(gg->host, 0, static_cast<size_t>(gg->a.frame_elems * gg->d.count) * sizeof(double));
+  }
   zero(c.probe_out);
   zero(c.line_out);
   XFD_HIP(hipStreamSynchronize(c.stream));
@@ -2049,6 +2105,12 @@ int xfd_read_gather(xfd_ctx* c, int32_t gi, double* dst) {
   return guard([&] {
     require(c && dst && gi >= 0 && gi < static_cast<int>(c->gathers.size()), "bad argument");
     auto& g = *c->gathers[gi];
+    if (g.staged) {  // the frames copied so far, from pinned host memory
+      XFD_HIP(hipStreamSynchronize(c->stream));
+      XFD_HIP(hipStreamSynchronize(c->copy_stream));
+      std::memcpy(dst, g.host, static_cast<size_t>(g.a.frame_elems * g.d.count) * sizeof(double));
+      return;
+    }
     if (g.probe >= 0)
       XFD_HIP(hipMemcpyAsync(dst, c->probe_out.as<double>() + g.probe * c->probe_count,
                              static_cast<size_t>(c->probe_count) * sizeof(double), hipMemcpyDeviceToHost,
@@ -2269,6 +2331,10 @@ int xfd_reset(xfd_ctx* c) {
 }
 
 int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
+  struct BatchFlag {  // batched cases record on the device (no staging)
+    BatchFlag() { g_batch_create = true; }
+    ~BatchFlag() { g_batch_create = false; }
+  } flag;
   return guard([&] {
     require(n > 0 && descs != nullptr && out != nullptr, "bad argument");
     *out = nullptr;
