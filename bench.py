@@ -310,7 +310,8 @@ def bench_design_sweep(args):
     t0 = time.perf_counter()
     batch.tune(3)
     info = batch.info()
-    log(f"design sweep: tuned in {time.perf_counter() - t0:.1f} s: ichunk {info['ichunk']}, grid {info['grid']}")
+    log(f"design sweep: tuned in {time.perf_counter() - t0:.1f} s: tile {info['tile'][0]}x{info['tile'][1]} "
+        f"(k-lanes x j-rows), ichunk {info['ichunk']}, grid {info['grid']}")
     batch.advance(0, args.warmup)
     native.device_synchronize()
     t0 = time.perf_counter()
@@ -331,7 +332,8 @@ def bench_design_sweep(args):
                                "{48,64,96,128,160} incl. 8-cell CPML, one dielectric cube each (eps_r U[1,8]), "
                                "Ez Gaussian dipole, 4-frequency FP64 DFT on the cube faces, HIP-graph steps",
                    "cells": cells, "parallelism": "one persistent launch per phase over all cases, 1 GPU"},
-        "batch": {"shared_launches": info["shared"], "ichunk": info["ichunk"], "grid": info["grid"],
+        "batch": {"shared_launches": info["shared"], "tile": list(info["tile"]), "ichunk": info["ichunk"],
+                  "grid": info["grid"],
                   "device_ms_per_step": round(info["device_ms"] / args.steps, 4)},
     }
     batch.close()

@@ -339,7 +339,8 @@ int xfd_reset(xfd_ctx* ctx);
  * at steps 0, K, 2K, ... in device memory.  xfd_backward then walks the
  * segments from last to first: it restores a checkpoint, recomputes the
  * segment while keeping the design edges' E, and runs the transposed step
- * (adjoint Yee sweeps and CPML) backwards over it, accumulating in FP64
+ * (adjoint Yee sweeps, CPML, inductor records and dispersive ADE updates)
+ * backwards over it, accumulating in FP64
  *   dJ/d eps_e = sum_n lambda_e^{n+1} (beta_e E_e^{n+1} + gamma_e E_e^n),
  * where, for E = s E + a dH_a + b dH_b with s = (2 eps - sigma dt) /
  * (2 eps + sigma dt) and a, b proportional to 1 / (2 eps + sigma dt),
@@ -347,9 +348,10 @@ int xfd_reset(xfd_ctx* ctx);
  *   - beta s.
  * The objective enters as `source[p * nt + n]` = dJ/dF_p after step n (F_p
  * the field value at point p once step n's corrections are applied).  The
- * design edges must be updated by the plain Yee sweep only (no CPML, TFSF or
- * lumped correction on them); ADE materials and inductors are not covered.
- * After xfd_backward the fields hold a recomputed intermediate state. */
+ * design edges must be updated by the plain Yee sweep only: a design box whose
+ * edges a CPML slab, an E record (TFSF, lumped element) or a dispersive cell
+ * touches is refused with XFD_EINVAL.  After xfd_backward the fields hold a
+ * recomputed intermediate state. */
 typedef struct {
   int32_t origin[3], size[3];  /* design cells [origin, origin + size) */
   const double* beta;          /* per design edge (xfd_design_edges order) */
@@ -388,12 +390,16 @@ int xfd_batch_case(xfd_batch* batch, int32_t i, xfd_ctx** ctx);
 /* shared: 1 when the batched launches are used; ichunk and grid: the current
  * i-planes per work-group and work-groups per persistent launch. */
 int xfd_batch_info(xfd_batch* batch, int32_t* shared, int32_t* ichunk, int32_t* grid);
+/* The current sweep tile of the batched launches: lanes along k by rows
+ * along j (64 x 4 until xfd_batch_tune picks another compiled shape). */
+int xfd_batch_tile(xfd_batch* batch, int32_t* lanes_k, int32_t* rows_j);
 /* on = 1: the first xfd_batch_step captures one step as a HIP graph and
  * every step replays it. */
 int xfd_batch_set_graph(xfd_batch* batch, int32_t on);
-/* Autotuner: times `reps` Yee sweep pairs for each i-chunk (4, 8, 16, 32) and
- * persistent grid size (4, 8, 16, 32 work-groups per CU), keeps the fastest,
- * then zeroes every case's state (xfd_reset). */
+/* Autotuner: times `reps` Yee sweep pairs for each compiled sweep tile
+ * (64x4, 64x8, 128x2, 128x4, 64x2 k-lanes by j-rows), i-chunk (4, 8, 16,
+ * 32) and persistent grid size (4, 8, 16, 32 work-groups per CU), keeps the
+ * fastest, then zeroes every case's state (xfd_reset). */
 int xfd_batch_tune(xfd_batch* batch, int32_t reps);
 /* Runs steps t0 .. t0+n-1 of every case and waits for them. */
 int xfd_batch_step(xfd_batch* batch, int64_t t0, int64_t n);

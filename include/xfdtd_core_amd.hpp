@@ -907,6 +907,7 @@ class SimulationBatch {
     bool shared;  // false: cases with dense coefficients or ADE, stepped one by one
     int ichunk;
     int grid;
+    int lanes_k, rows_j;  // the sweep tile
   };
   Layout layout() const;
 

@@ -129,6 +129,8 @@ int xfd_sim_batch_advance(void* batch, int64_t t0, int64_t n);
 int xfd_sim_batch_finish(void* batch);
 int xfd_sim_batch_info(void* batch, int32_t* shared, int32_t* ichunk, int32_t* grid, double* loop_s,
                        double* device_ms);
+/* The batched sweep tile (lanes along k, rows along j). */
+int xfd_sim_batch_tile(void* batch, int32_t* lanes_k, int32_t* rows_j);
 
 #ifdef __cplusplus
 }

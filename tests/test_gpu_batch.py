@@ -198,3 +198,30 @@ def test_batch_with_dispersive_case_steps_one_by_one():
     sims, b, got, info = _batched(specs, "f64", steps=60)
     assert not info["shared"]
     _compare(got, want, True, "fallback batch")
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+def test_batch_sweep_tiles_match_single_cases_fp64(monkeypatch, tile):
+    """Every compiled sweep tile of the batched launches (64x8, 128x2, 128x4,
+    64x2 k-lanes by j-rows; 64x4 is the default the other tests use) gives
+    the single cases' FP64 results bitwise."""
+    monkeypatch.setenv("XFD_BATCH_TILE", str(tile))
+    specs = _cases(n=4, seed=11, sizes=(24, 40, 72))
+    say(f"[gpu] batch tile {tile} vs single cases f64")
+    sims, b, got, info = _batched(specs, "f64", graph=True)
+    assert info["shared"]
+    assert info["tile"] == [(64, 4), (64, 8), (128, 2), (128, 4), (64, 2)][tile]
+    _compare(got, _alone(specs, "f64"), True, f"tile {tile}")
+
+
+def test_batch_tuner_picks_a_compiled_tile():
+    specs = _cases(n=4, seed=12, sizes=(32, 48))
+    sims = [native.Simulation(sp.spec(), "f32") for sp in specs]
+    b = native.SimulationBatch(sims)
+    b.init(20)
+    b.tune(2)
+    info = b.info()
+    say(f"[gpu] tuned: tile {info['tile']}, ichunk {info['ichunk']}, grid {info['grid']}")
+    assert info["tile"] in [(64, 4), (64, 8), (128, 2), (128, 4), (64, 2)]
+    assert info["ichunk"] in (4, 8, 16, 32)
+

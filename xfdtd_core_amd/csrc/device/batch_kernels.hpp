@@ -67,8 +67,14 @@ struct BatchNf2ff {
   Nf2ffArgs a;
 };
 
-template <class T, bool IsH, bool Fused>
-__global__ __launch_bounds__(kTileThreads) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
+// The sweep tiles the batched launches can use (TK k-lanes by TJ rows); the
+// autotuner times each (xfd_batch_tune) and keeps the fastest.
+constexpr int kBatchTiles = 5;
+constexpr int kBatchTileK[kBatchTiles] = {64, 64, 128, 128, 64};
+constexpr int kBatchTileJ[kBatchTiles] = {4, 8, 2, 4, 2};
+
+template <class T, bool IsH, bool Fused, int TK, int TJ>
+__global__ __launch_bounds__(TK * TJ) void k_batch_sweep(const BatchCase<T>* __restrict__ cases,
                                                      const int64_t* __restrict__ start, int n, int64_t total) {
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int e = findEntry(start, n, b);
@@ -79,9 +85,9 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_sweep(const BatchCase<T>
     const int by = static_cast<int>(r % C.nby);
     const int bz = static_cast<int>(r / C.nby);
     if (IsH)
-      sweepH<T, ClassCoef<T>, Fused>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
+      sweepH<T, ClassCoef<T>, Fused, false, false, TK, TJ>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
     else
-      sweepE<T, ClassCoef<T>, Fused>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
+      sweepE<T, ClassCoef<T>, Fused, false, false, TK, TJ>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
   }
 }
 
@@ -181,9 +187,10 @@ __global__ void k_tick(int64_t* t) {
 
 template <class T>
 struct BatchLaunch {
-  // fused: some case applies leading CPML slabs inside its sweeps (ph / pe)
+  // fused: some case applies leading CPML slabs inside its sweeps (ph / pe);
+  // tile: index into kBatchTileK / kBatchTileJ
   static void sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
-                    int grid, bool fused);
+                    int grid, bool fused, int tile = 0);
   static void cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n, int64_t total,
                    int grid);
   static void cpmlFaces(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,

@@ -1618,8 +1618,7 @@ template <class T>
 void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
   require(c.ck_nt > 0 && c.ckpt.p, "no checkpointed forward run (xfd_forward_checkpointed)");
   require(D.nt == c.ck_nt, "objective length differs from the forward run");
-  require(c.ade_scheme < 0, "the adjoint does not cover dispersive (ADE) materials");
-  require(c.state.bytes == 0, "the adjoint does not cover inductors");
+  require(c.dim == 3, "the adjoint needs a 3D grid");
   for (int a = 0; a < 3; ++a) {
     const int n = a == 0 ? c.g.nx : (a == 1 ? c.g.ny : c.g.nz);
     require(D.size[a] > 0 && D.origin[a] >= 1 && D.origin[a] + D.size[a] <= n - 1, "design box outside the grid");
@@ -1627,6 +1626,32 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
   require(D.n_points >= 0 && (D.n_points == 0 || (D.points && D.source)), "bad objective points");
   const xfd::DesignArgs d = designArgs(D.origin, D.size);
   const int64_t nd = d.off[3];
+  {  // The gradient reads each design edge's E after the Yee update: no E
+    // record (lumped element, TFSF) and no dispersive update may touch one.
+    std::vector<char> edge(static_cast<size_t>(c.g.total), 0);
+    for (int q = 0; q < 3; ++q)
+      for (int u = 0; u < d.e[q][0]; ++u)
+        for (int v = 0; v < d.e[q][1]; ++v)
+          for (int w = 0; w < d.e[q][2]; ++w)
+            edge[padIndex(c.g, d.o[q][0] + u, d.o[q][1] + v, d.o[q][2] + w)] |= static_cast<char>(1 << q);
+    if (c.nseg_e > 0) {
+      std::vector<xfd::Record> rec(c.rec_e.bytes / sizeof(xfd::Record));
+      XFD_HIP(hipMemcpy(rec.data(), c.rec_e.p, c.rec_e.bytes, hipMemcpyDeviceToHost));
+      for (const auto& r : rec)
+        require(r.field > 2 || !(edge[r.target] & (1 << r.field)), "an E record (lumped element or TFSF) touches a design edge");
+    }
+    for (const auto& sl : c.slabs) {  // CPML E corrections along the slab axis
+      const int a = sl->axis;
+      require(D.origin[a] + D.size[a] + 1 <= sl->e_start || D.origin[a] >= sl->e_start + sl->n,
+              "a CPML slab corrects a design edge");
+    }
+    if (c.ade_scheme >= 0) {
+      std::vector<int16_t> mat(static_cast<size_t>(c.g.total));
+      XFD_HIP(hipMemcpy(mat.data(), c.ade_mat.p, c.ade_mat.bytes, hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < mat.size(); ++q)
+        require(!(mat[q] >= 0 && edge[q]), "a dispersive (ADE) cell holds a design edge");
+    }
+  }
   require(D.beta && D.gamma && grad_out, "bad design coefficients");
   hipStream_t st = c.stream;
   const auto F = fieldPtrs<T>(c);
@@ -1658,6 +1683,36 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
     h.psiB = mu.back()[3].as<T>();
     me.push_back(e);
     mh.push_back(h);
+  }
+  // adjoint inductor currents; adjoint ADE state (the forward state's
+  // layout) and the deferred part of lambdaE on dispersive cells
+  DevBuf lam_state, lam_ade[9], ade_x[3];
+  xfd::AdeArgs<T> Ad{}, LA{};
+  xfd::FieldPtrs<T> X{};
+  if (c.state.bytes) {
+    lam_state.alloc(c.state.bytes);
+    XFD_HIP(hipMemset(lam_state.p, 0, c.state.bytes));
+  }
+  if (c.ade_scheme >= 0) {
+    Ad.poles = LA.poles = c.ade_poles;
+    Ad.stride = LA.stride = 2 + 7 * c.ade_poles;
+    Ad.total = LA.total = c.g.total;
+    Ad.mat = LA.mat = c.ade_mat.as<int16_t>();
+    Ad.tab = LA.tab = c.ade_tab.as<T>();
+    for (int q = 0; q < 3; ++q) {
+      DevBuf* src3[3] = {&c.ade_j[q], &c.ade_jp[q], &c.ade_eprev[q]};
+      T** dst3[3] = {&LA.j[q], &LA.jp[q], &LA.eprev[q]};
+      for (int w = 0; w < 3; ++w) {
+        DevBuf& b = lam_ade[3 * q + w];
+        if (src3[w]->bytes) {
+          b.alloc(src3[w]->bytes);
+          XFD_HIP(hipMemset(b.p, 0, b.bytes));
+        }
+        *dst3[w] = b.as<T>();
+      }
+      ade_x[q].alloc(c.fields[q].bytes);
+      X.f[q] = ade_x[q].as<T>();
+    }
   }
   const int64_t K_int = c.ck_interval;
   DevBuf hist, beta, gamma, grad, pts, src;
@@ -1692,16 +1747,20 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
       A::grad(st, c.g, L, d, H0 + (t - t0) * nd, H0 + (t - t0 + 1) * nd, beta.as<double>(), gamma.as<double>(),
               grad.as<double>());
       for (auto& a : me) A::cpmlMu(st, c.g, L, a);
+      if (c.state.bytes)
+        A::records(st, L, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e, lam_state.as<T>());
+      if (c.ade_scheme >= 0) A::adePre(st, c.g, L, Ad, LA, X, c.ade_scheme);
       if (c.classed)
-        A::adjH(st, c.g, L, K);
+        A::adjH(st, c.g, L, K, c.ichunk);
       else
-        A::adjH(st, c.g, L, C);
+        A::adjH(st, c.g, L, C, c.ichunk);
       for (auto& a : me) A::cpmlGather(st, true, c.g, L, a);
       for (auto& a : mh) A::cpmlMu(st, c.g, L, a);
       if (c.classed)
-        A::adjE(st, c.g, L, K);
+        A::adjE(st, c.g, L, K, c.ichunk);
       else
-        A::adjE(st, c.g, L, C);
+        A::adjE(st, c.g, L, C, c.ichunk);
+      if (c.ade_scheme >= 0) A::adePost(st, c.g, L, Ad.mat, X);
       for (auto& a : mh) A::cpmlGather(st, false, c.g, L, a);
       XFD_HIP(hipGetLastError());
     }
@@ -1723,6 +1782,7 @@ struct xfd_batch {
   hipStream_t stream = nullptr;
   DevBuf tdev;
   int ichunk = 16;
+  int tile = 0;     // sweep tile: index into kBatchTileK / kBatchTileJ
   int grid = 2048;  // work-groups of each persistent launch
   int cus = 256;
   // One table per launch: entries and their first work-group ids.  CPML
@@ -1812,8 +1872,9 @@ void batchTables(xfd_batch& b) {
       bc.pe = {c.cpml_list_e.as<xfd::SlabArgs<T>>(), c.n_fused};
     }
     require(!c.zfused, "batched cases keep their z slabs separate");
-    bc.nbx = (c.g.nz + xfd::kLanesK - 1) / xfd::kLanesK;
-    bc.nby = (c.g.ny + xfd::kRowsJ - 1) / xfd::kRowsJ;
+    const int tk = xfd::kBatchTileK[b.tile], tj = xfd::kBatchTileJ[b.tile];
+    bc.nbx = (c.g.nz + tk - 1) / tk;
+    bc.nby = (c.g.ny + tj - 1) / tj;
     bc.nbz = (c.g.nx + b.ichunk - 1) / b.ichunk;
     sw.add(bc, static_cast<int64_t>(bc.nbx) * bc.nby * bc.nbz);
     const int64_t stride[3] = {c.g.plane, c.g.pz, 1};
@@ -1870,7 +1931,7 @@ void batchEnqueueStep(xfd_batch& b) {
   const int64_t* t = b.tdev.as<int64_t>();
   auto sweep = [&](bool is_h) {
     L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n, b.sweep.total,
-             b.grid, b.any_fused);
+             b.grid, b.any_fused, b.tile);
   };
   auto records = [&](xfd_batch::Table& r) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
@@ -1934,15 +1995,17 @@ void batchStep(xfd_batch& b, int64_t t0, int64_t n) {
   XFD_HIP(hipEventElapsedTime(&b.last_ms, b.ev0, b.ev1));
 }
 
-// Times `reps` pairs of batched Yee sweeps for each candidate (i-chunk,
-// work-groups per launch) and keeps the fastest.
+// Times `reps` pairs of batched Yee sweeps for each candidate (sweep tile
+// shape, i-chunk, work-groups per launch) and keeps the fastest.
 template <class T>
 void batchTune(xfd_batch& b, int reps) {
   const int chunks[] = {4, 8, 16, 32};
   const int mults[] = {4, 8, 16, 32};
   float best = -1;
-  int best_chunk = b.ichunk, best_grid = b.grid;
+  int best_chunk = b.ichunk, best_grid = b.grid, best_tile = b.tile;
+  for (int tile = 0; tile < xfd::kBatchTiles; ++tile)
   for (int ch : chunks) {
+    b.tile = tile;
     b.ichunk = ch;
     batchTables<T>(b);
     for (int m : mults) {
@@ -1951,7 +2014,7 @@ void batchTune(xfd_batch& b, int reps) {
       auto pair = [&] {
         for (bool is_h : {true, false})
           L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
-                   b.sweep.total, b.grid, b.any_fused);
+                   b.sweep.total, b.grid, b.any_fused, b.tile);
       };
       pair();  // warm-up
       XFD_HIP(hipEventRecord(b.ev0, b.stream));
@@ -1964,9 +2027,11 @@ void batchTune(xfd_batch& b, int reps) {
         best = ms;
         best_chunk = ch;
         best_grid = b.grid;
+        best_tile = tile;
       }
     }
   }
+  b.tile = best_tile;
   b.ichunk = best_chunk;
   b.grid = best_grid;
   batchTables<T>(b);
@@ -2366,6 +2431,11 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
     XFD_HIP(hipGetDeviceProperties(&prop, dev));
     b->cus = std::max(1, prop.multiProcessorCount);
     b->grid = b->cus * 8;
+    if (const char* e = std::getenv("XFD_BATCH_TILE")) {  // a compiled sweep tile before any tuning
+      const int v = std::atoi(e);
+      require(v >= 0 && v < xfd::kBatchTiles, "XFD_BATCH_TILE out of range");
+      b->tile = v;
+    }
     if (b->shared) {
       if (b->dtype == XFD_F64)
         batchTables<double>(*b);
@@ -2382,6 +2452,14 @@ int xfd_batch_case(xfd_batch* b, int32_t i, xfd_ctx** ctx) {
   return guard([&] {
     require(b && ctx && i >= 0 && i < static_cast<int32_t>(b->cases.size()), "bad case index");
     *ctx = b->cases[i].get();
+  });
+}
+
+int xfd_batch_tile(xfd_batch* b, int32_t* lanes_k, int32_t* rows_j) {
+  return guard([&] {
+    require(b != nullptr, "null batch");
+    if (lanes_k) *lanes_k = xfd::kBatchTileK[b->tile];
+    if (rows_j) *rows_j = xfd::kBatchTileJ[b->tile];
   });
 }
 

@@ -193,12 +193,13 @@ __device__ __forceinline__ void zStore(const ZCpml<T>& Z, const ZLane<T>& L, con
 // One work-group tile: k-block bx, j-block by, i-chunk bz.  Fused: the CPML
 // H corrections of the visited points in the same pass (the H dual
 // differences along x, y, z are dual[idx + s_c] - dual[idx]).
-template <class T, class Coef, bool Fused = false, bool ZF = false, bool All = false>
+template <class T, class Coef, bool Fused = false, bool ZF = false, bool All = false, int TK = kLanesK,
+          int TJ = kRowsJ>
 __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
                                        int ichunk, const CpmlList<T>& P = CpmlList<T>{},
                                        const ZCpml<T>& Z = ZCpml<T>{}, const StepSlabs<T>& S = StepSlabs<T>{}) {
-  const int k = bx * kLanesK + threadIdx.x;
-  const int j = by * kRowsJ + threadIdx.y;
+  const int k = bx * TK + threadIdx.x;
+  const int j = by * TJ + threadIdx.y;
   const int i0 = bz * ichunk;
   if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
   const int i1 = min(i0 + ichunk, g.nx);
@@ -266,12 +267,13 @@ __global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> 
 // visited points (i < nx, j < ny, k < nz) in the same pass, including the
 // face points a Yee range leaves out; the dual differences along x, y, z are
 // dual[idx] - dual[idx - s_c].
-template <class T, class Coef, bool Fused = false, bool ZF = false, bool All = false>
+template <class T, class Coef, bool Fused = false, bool ZF = false, bool All = false, int TK = kLanesK,
+          int TJ = kRowsJ>
 __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, const Coef& C, int bx, int by, int bz,
                                        int ichunk, const CpmlList<T>& P = CpmlList<T>{},
                                        const ZCpml<T>& Z = ZCpml<T>{}, const StepSlabs<T>& S = StepSlabs<T>{}) {
-  const int k = bx * kLanesK + threadIdx.x;
-  const int j = by * kRowsJ + threadIdx.y;
+  const int k = bx * TK + threadIdx.x;
+  const int j = by * TJ + threadIdx.y;
   const int i0 = bz * ichunk;
   if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
   const int i1 = min(i0 + ichunk, g.nx);

@@ -212,20 +212,32 @@ void Launch<T>::pack(hipStream_t s, const Grid& g, const T* src, double* dst, in
 }
 
 // ------------------------------------------------------------- batched --
-template <class T>
-void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n,
-                           int64_t total, int grid, bool fused) {
-  if (total <= 0) return;
-  dim3 block(kLanesK, kRowsJ);
+template <class T, int TK, int TJ>
+void batchSweepTile(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
+                    int grid, bool fused) {
+  dim3 block(TK, TJ);
   dim3 g(static_cast<unsigned>(std::min<int64_t>(total, grid)));
   if (is_h && fused)
-    hipLaunchKernelGGL((k_batch_sweep<T, true, true>), g, block, 0, s, cases, start, n, total);
+    hipLaunchKernelGGL((k_batch_sweep<T, true, true, TK, TJ>), g, block, 0, s, cases, start, n, total);
   else if (is_h)
-    hipLaunchKernelGGL((k_batch_sweep<T, true, false>), g, block, 0, s, cases, start, n, total);
+    hipLaunchKernelGGL((k_batch_sweep<T, true, false, TK, TJ>), g, block, 0, s, cases, start, n, total);
   else if (fused)
-    hipLaunchKernelGGL((k_batch_sweep<T, false, true>), g, block, 0, s, cases, start, n, total);
+    hipLaunchKernelGGL((k_batch_sweep<T, false, true, TK, TJ>), g, block, 0, s, cases, start, n, total);
   else
-    hipLaunchKernelGGL((k_batch_sweep<T, false, false>), g, block, 0, s, cases, start, n, total);
+    hipLaunchKernelGGL((k_batch_sweep<T, false, false, TK, TJ>), g, block, 0, s, cases, start, n, total);
+}
+template <class T>
+void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n,
+                           int64_t total, int grid, bool fused, int tile) {
+  if (total <= 0) return;
+  static_assert(kBatchTiles == 5, "tile list");
+  switch (tile) {
+    case 1: batchSweepTile<T, kBatchTileK[1], kBatchTileJ[1]>(s, is_h, cases, start, n, total, grid, fused); break;
+    case 2: batchSweepTile<T, kBatchTileK[2], kBatchTileJ[2]>(s, is_h, cases, start, n, total, grid, fused); break;
+    case 3: batchSweepTile<T, kBatchTileK[3], kBatchTileJ[3]>(s, is_h, cases, start, n, total, grid, fused); break;
+    case 4: batchSweepTile<T, kBatchTileK[4], kBatchTileJ[4]>(s, is_h, cases, start, n, total, grid, fused); break;
+    default: batchSweepTile<T, kBatchTileK[0], kBatchTileJ[0]>(s, is_h, cases, start, n, total, grid, fused); break;
+  }
 }
 template <class T>
 void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,
@@ -302,21 +314,73 @@ void Launch<T>::far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArg
 inline dim3 adjGrid(const Grid& g) {
   return dim3(blocks_for(g.nz + 1, kLanesK), blocks_for(g.ny + 1, kRowsJ), static_cast<unsigned>(g.nx + 1));
 }
-template <class T>
-void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C) {
-  hipLaunchKernelGGL((k_adj_h<T, CoefPtrs<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+// The i-marching sweeps, or with XFD_ADJ_POINT=1 the one-point-per-thread
+// kernels (the transposes term by term, kept as their cross-check: in FP64
+// both give the same bits).  Read at every launch.
+inline bool adjPoint() {
+  const char* e = std::getenv("XFD_ADJ_POINT");
+  return e && e[0] == '1';
+}
+template <class T, class Coef>
+void launchAdj(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const Coef& C, int ichunk) {
+  const dim3 block(kLanesK, kRowsJ);
+  if (adjPoint()) {
+    if (is_h)
+      hipLaunchKernelGGL((k_adj_h_point<T, Coef>), adjGrid(g), block, 0, s, g, L, C);
+    else
+      hipLaunchKernelGGL((k_adj_e_point<T, Coef>), adjGrid(g), block, 0, s, g, L, C);
+    return;
+  }
+  ichunk = std::max(1, ichunk);
+  // H: the R_H tiles; E: every point up to the far faces
+  const int e = is_h ? 0 : 1;
+  const SweepMap m{static_cast<int>(blocks_for(g.nz + e, kLanesK)), static_cast<int>(blocks_for(g.ny + e, kRowsJ)),
+                   static_cast<int>(blocks_for(g.nx + e, ichunk)), 0};
+  const dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz));
+  if (is_h)
+    hipLaunchKernelGGL((k_adj_h<T, Coef>), grid, block, 0, s, g, L, C, m, ichunk);
+  else
+    hipLaunchKernelGGL((k_adj_e<T, Coef>), grid, block, 0, s, g, L, C, m, ichunk);
 }
 template <class T>
-void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C) {
-  hipLaunchKernelGGL((k_adj_h<T, ClassCoef<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk) {
+  launchAdj<T>(true, s, g, L, C, ichunk);
 }
 template <class T>
-void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C) {
-  hipLaunchKernelGGL((k_adj_e<T, CoefPtrs<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk) {
+  launchAdj<T>(true, s, g, L, C, ichunk);
 }
 template <class T>
-void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C) {
-  hipLaunchKernelGGL((k_adj_e<T, ClassCoef<T>>), adjGrid(g), dim3(kLanesK, kRowsJ), 0, s, g, L, C);
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk) {
+  launchAdj<T>(false, s, g, L, C, ichunk);
+}
+template <class T>
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk) {
+  launchAdj<T>(false, s, g, L, C, ichunk);
+}
+template <class T>
+void AdjLaunch<T>::records(hipStream_t s, const FieldPtrs<T>& L, const Record* rec, const int64_t* seg, int64_t nseg,
+                           T* lamJ) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_adj_records<T>, dim3(blocks_for(nseg, 256)), dim3(256), 0, s, L, rec, seg, nseg, lamJ);
+}
+template <class T>
+void AdjLaunch<T>::adePre(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const AdeArgs<T>& A,
+                          const AdeArgs<T>& LA, const FieldPtrs<T>& X, int scheme) {
+  const int64_t n = static_cast<int64_t>(g.nx) * g.ny * g.nz;
+  const dim3 grid(blocks_for(n, 256));
+  if (scheme == kAdeDrude)
+    hipLaunchKernelGGL((k_adj_ade_pre<T, kAdeDrude>), grid, dim3(256), 0, s, g, L, A, LA, X);
+  else if (scheme == kAdeDebye)
+    hipLaunchKernelGGL((k_adj_ade_pre<T, kAdeDebye>), grid, dim3(256), 0, s, g, L, A, LA, X);
+  else
+    hipLaunchKernelGGL((k_adj_ade_pre<T, kAdeMLorentz>), grid, dim3(256), 0, s, g, L, A, LA, X);
+}
+template <class T>
+void AdjLaunch<T>::adePost(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const int16_t* mat,
+                           const FieldPtrs<T>& X) {
+  const int64_t n = static_cast<int64_t>(g.nx) * g.ny * g.nz;
+  hipLaunchKernelGGL(k_adj_ade_post<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, L, mat, X);
 }
 template <class T>
 void AdjLaunch<T>::cpmlMu(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a) {

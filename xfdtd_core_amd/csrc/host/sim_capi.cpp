@@ -708,4 +708,12 @@ int xfd_sim_batch_info(void* p, int32_t* shared, int32_t* ichunk, int32_t* grid,
   });
 }
 
+int xfd_sim_batch_tile(void* p, int32_t* lanes_k, int32_t* rows_j) {
+  return guard([&] {
+    const auto l = B(p)->layout();
+    if (lanes_k) *lanes_k = l.lanes_k;
+    if (rows_j) *rows_j = l.rows_j;
+  });
+}
+
 }  // extern "C"

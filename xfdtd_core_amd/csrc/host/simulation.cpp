@@ -2325,9 +2325,10 @@ double SimulationBatch::loopSeconds() const { return _impl->loop_seconds; }
 double SimulationBatch::deviceMilliseconds() const { return _impl->device_ms; }
 
 SimulationBatch::Layout SimulationBatch::layout() const {
-  int32_t shared = 0, ichunk = 0, grid = 0;
+  int32_t shared = 0, ichunk = 0, grid = 0, lk = 0, rj = 0;
   Impl::check(xfd_batch_info(_impl->batch(), &shared, &ichunk, &grid), "xfd_batch_info");
-  return {shared != 0, ichunk, grid};
+  Impl::check(xfd_batch_tile(_impl->batch(), &lk, &rj), "xfd_batch_tile");
+  return {shared != 0, ichunk, grid, lk, rj};
 }
 
 }  // namespace xfdtd

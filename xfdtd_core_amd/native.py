@@ -108,6 +108,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_batch_destroy": (None, [vp]),
         "xfd_batch_case": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
         "xfd_batch_info": (ctypes.c_int, [vp, p32, p32, p32]),
+        "xfd_batch_tile": (ctypes.c_int, [vp, p32, p32]),
         "xfd_batch_set_graph": (ctypes.c_int, [vp, i32]),
         "xfd_batch_tune": (ctypes.c_int, [vp, i32]),
         "xfd_batch_step": (ctypes.c_int, [vp, i64, i64]),
@@ -128,6 +129,7 @@ def load_library() -> ctypes.CDLL:
         "xfd_sim_batch_finish": (ctypes.c_int, [vp]),
         "xfd_sim_batch_info": (ctypes.c_int, [vp, p32, p32, p32, ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double)]),
+        "xfd_sim_batch_tile": (ctypes.c_int, [vp, p32, p32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -585,5 +587,7 @@ class SimulationBatch:
         ls, ms = ctypes.c_double(), ctypes.c_double()
         self._check(self._L.xfd_sim_batch_info(self._h, ctypes.byref(sh), ctypes.byref(ic), ctypes.byref(gr),
                                                ctypes.byref(ls), ctypes.byref(ms)))
+        lk, rj = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._L.xfd_sim_batch_tile(self._h, ctypes.byref(lk), ctypes.byref(rj)))
         return {"shared": bool(sh.value), "ichunk": ic.value, "grid": gr.value, "loop_seconds": ls.value,
-                "device_ms": ms.value}
+                "device_ms": ms.value, "tile": (lk.value, rj.value)}
