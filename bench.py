@@ -41,7 +41,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0) -> float:
+def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0,
+                         cpml_launches: float = 6.0) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` on an n^3 grid, the
     per-cell-step figures of the leapfrog step split over its launches:
     B_yee = (6 field reads + 6 field writes + 18 coefficient reads) * s per
@@ -53,7 +54,8 @@ def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0) 
     inside the sweeps (`fused` 1: the four x/y slabs, 2: the two z slabs)
     move their share to the sweep; their separate launches only finish the
     far grid planes (not counted), and the CPML figure is the mean over the
-    six launches of a half-step."""
+    `cpml_launches` launches of a half-step (six, one per slab; fewer where
+    the two slabs of one axis share a launch)."""
     if kernel == "step":  # the single-pass step: the whole step's B_yee and CPML
         return step_alg_bytes(n, pml, s)
     cells = n ** 3
@@ -62,7 +64,7 @@ def alg_bytes_per_launch(kernel: str, n: int, pml: int, s: int, fused: int = 0) 
     if kernel in ("update_h", "update_e"):
         return 15.0 * s * cells + 6.0 * s * inside * slab_cells
     if kernel in ("cpml_h", "cpml_e"):
-        return 6.0 * s * slab_cells * (6 - inside) / 6.0
+        return 6.0 * s * slab_cells * (6 - inside) / cpml_launches
     return 0.0
 
 
@@ -114,7 +116,7 @@ def run_device(sim, precision, warmup, steps, n, pml):
         if launches == 0:
             continue
         avg_ms = ms / launches
-        b = alg_bytes_per_launch(name, n, pml, s, fused)
+        b = alg_bytes_per_launch(name, n, pml, s, fused, launches / BREAKDOWN_STEPS)
         kernels[name] = {"launches": launches, "avg_ms": round(avg_ms, 5), "total_ms": round(ms, 3),
                          "alg_GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
@@ -130,7 +132,8 @@ def run_device(sim, precision, warmup, steps, n, pml):
     dl, dms = sim.kernel_stats()[dom]
     dom_avg = dms / dl
     kernels[dom]["timed_region"] = {"launches": dl, "avg_ms": round(dom_avg, 5)}
-    ach = round(alg_bytes_per_launch(dom, n, pml, s, fused) / (dom_avg * 1e-3) / 1e9, 1)
+    ach = round(alg_bytes_per_launch(dom, n, pml, s, fused, kernels[dom]["launches"] / BREAKDOWN_STEPS)
+                / (dom_avg * 1e-3) / 1e9, 1)
     traffic = measured_traffic(precision, n, pml)
     for name, k in kernels.items():
         if traffic and name in traffic:

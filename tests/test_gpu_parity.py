@@ -572,8 +572,9 @@ def test_time_domain_nf2ff(tmp_path, monkeypatch, precision, merge):
 def test_profiling_selected_kernel_classes(monkeypatch, single_pass):
     """HIP-event timing can be limited to chosen kernel classes: only those
     classes report launches, and with every class the counts match the
-    launches of the steps (two sweeps and six slab launches per half-step,
-    or one single-pass launch per step)."""
+    launches of the steps (two sweeps and three slab launches per half-step,
+    the two slabs of each axis sharing one, or six with XFD_CPML_PAIR=0; or
+    one single-pass launch per step)."""
     sp = scenarios.inhomogeneous_box(n_total=40, pml=6, n_spheres=3, r_cells=(3, 5), seed=3).spec()
     say(f"[gpu] profiling selected kernel classes (single pass: {single_pass})")
     if single_pass:
@@ -603,7 +604,13 @@ def test_profiling_selected_kernel_classes(monkeypatch, single_pass):
     sim.set_profiling(True)
     sim.advance(20, 20)
     st = sim.kernel_stats()
-    assert st["update_e"][0] == 20 and st["update_h"][0] == 20 and st["cpml_e"][0] == 20 * 6, st
+    assert st["update_e"][0] == 20 and st["update_h"][0] == 20 and st["cpml_e"][0] == 20 * 3, st
+    monkeypatch.setenv("XFD_CPML_PAIR", "0")
+    sim.init(20)
+    sim.set_profiling(True)
+    sim.advance(0, 20)
+    st = sim.kernel_stats()
+    assert st["cpml_e"][0] == 20 * 6 and st["cpml_h"][0] == 20 * 6, st
 
 
 def test_full_size_box_is_repeatable_and_linear():

@@ -390,8 +390,7 @@ __global__ __launch_bounds__(kTileThreads) void k_adj_e(Grid g, FieldPtrs<T> L, 
 // mu = b mu + c_psi lambda_field (the slab's psi arrays in `s` hold mu, and
 // the field pointers are the adjoint fields).
 template <class T>
-__global__ __launch_bounds__(256) void k_adj_cpml_mu(Grid g, FieldPtrs<T> L, SlabArgs<T> s) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void adjMuCell(const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& s, int64_t t) {
   const int64_t vol = static_cast<int64_t>(s.ext[0]) * s.ext[1] * s.ext[2];
   if (t >= vol) return;
   const int w = static_cast<int>(t % s.ext[2]);
@@ -411,11 +410,11 @@ __global__ __launch_bounds__(256) void k_adj_cpml_mu(Grid g, FieldPtrs<T> L, Sla
 // dual(P), so P gathers + a(P - c) mu(P - c) - a(P) mu(P).  One thread per
 // dual point of the slab box widened by one plane (below for E, above for H).
 template <class T, bool IsE>
-__global__ __launch_bounds__(256) void k_adj_cpml_gather(Grid g, FieldPtrs<T> L, SlabArgs<T> s) {
+__device__ __forceinline__ void adjGatherCell(const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& s,
+                                              int64_t t) {
   const int c = s.axis, a = (c + 1) % 3, b = (c + 2) % 3;
   int wext[3] = {s.ext[0], s.ext[1], s.ext[2]};
   wext[c] += 1;
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t vol = static_cast<int64_t>(wext[0]) * wext[1] * wext[2];
   if (t >= vol) return;
   const int w = static_cast<int>(t % wext[2]);
@@ -563,6 +562,34 @@ __global__ __launch_bounds__(256) void k_adj_ade_post(Grid g, FieldPtrs<T> L, co
   if (i >= 1 && j >= 1) L.f[2][idx] += X.f[2][idx];
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_adj_cpml_mu(Grid g, FieldPtrs<T> L, SlabArgs<T> s) {
+  adjMuCell(g, L, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_adj_cpml_gather(Grid g, FieldPtrs<T> L, SlabArgs<T> s) {
+  adjGatherCell<T, IsE>(g, L, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+// Two slabs in one launch, work-groups [0, nb0) on s0 and the rest on s1:
+// any two for the mu update (each writes its own arrays and reads lambda
+// only); for the gather, two whose widened boxes share no dual point.
+template <class T>
+__global__ __launch_bounds__(256) void k_adj_cpml_mu2(Grid g, FieldPtrs<T> L, SlabArgs<T> s0, SlabArgs<T> s1,
+                                                      unsigned nb0) {
+  if (blockIdx.x < nb0)
+    adjMuCell(g, L, s0, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+  else
+    adjMuCell(g, L, s1, static_cast<int64_t>(blockIdx.x - nb0) * blockDim.x + threadIdx.x);
+}
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_adj_cpml_gather2(Grid g, FieldPtrs<T> L, SlabArgs<T> s0, SlabArgs<T> s1,
+                                                          unsigned nb0) {
+  if (blockIdx.x < nb0)
+    adjGatherCell<T, IsE>(g, L, s0, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+  else
+    adjGatherCell<T, IsE>(g, L, s1, static_cast<int64_t>(blockIdx.x - nb0) * blockDim.x + threadIdx.x);
+}
+
 // Design edges: the E edges of the design cells, as three boxes (Ex, Ey,
 // Ez), k fastest; off[q] is the first edge of component q, off[3] the total.
 struct DesignArgs {
@@ -627,6 +654,10 @@ struct AdjLaunch {
   static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk);
   static void cpmlMu(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a);
   static void cpmlGather(hipStream_t s, bool is_e, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a);
+  static void cpmlMu2(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a0,
+                      const SlabArgs<T>& a1);
+  static void cpmlGather2(hipStream_t s, bool is_e, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a0,
+                          const SlabArgs<T>& a1);
   static void hist(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const DesignArgs& d, T* hist);
   static void grad(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const DesignArgs& d, const T* h0, const T* h1,
                    const double* beta, const double* gamma, double* grad);

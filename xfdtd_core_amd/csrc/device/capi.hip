@@ -1384,6 +1384,18 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
     AE = stepSlabs<T>(c, true, false);
   }
   auto inSweep = [&](size_t q) { return c.allslabs || static_cast<int>(q) < c.n_fused; };
+  // Slabs q and q + 1 share one launch when both run separately, lie on one
+  // axis and share no point (XFD_CPML_PAIR=0: one launch per slab).
+  const char* pair_env = std::getenv("XFD_CPML_PAIR");
+  const bool pair_on = !(pair_env && pair_env[0] == '0');
+  auto separate = [&](size_t q) { return !inSweep(q) && !(c.zfused && c.zslab[q]); };
+  auto pairs = [&](const std::vector<xfd::SlabArgs<T>>& v, size_t q) {
+    if (!pair_on || q + 1 >= v.size() || !separate(q) || !separate(q + 1)) return false;
+    const auto &a = v[q], &b = v[q + 1];
+    if (a.axis != b.axis) return false;
+    const int x = a.axis;
+    return a.lo[x] + a.ext[x] <= b.lo[x] || b.lo[x] + b.ext[x] <= a.lo[x];
+  };
   for (int64_t t = t0; t < t0 + n; ++t) {
     if (c.fstep) {
       // H, its CPML, E and its CPML in one pass from the current set into
@@ -1418,7 +1430,10 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (size_t q = 0; q < sh.size(); ++q)
       c.timed(K_CPML_H, [&] {
-        if (c.zfused && c.zslab[q] && sizeof(T) == 8)
+        if (pairs(sh, q)) {
+          xfd::Launch<T>::cpml2(st, c.g, F, sh[q], sh[q + 1], false);
+          ++q;
+        } else if (c.zfused && c.zslab[q] && sizeof(T) == 8)
           xfd::Launch<T>::cpml_zrest(st, c.g, F, sh[q], ZH.fr, false);
         else if (inSweep(q) || (c.zfused && c.zslab[q]))
           xfd::Launch<T>::cpml_faces(st, c.g, F, sh[q], false);
@@ -1450,7 +1465,10 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
       });
     for (size_t q = 0; q < se.size() && !c.fstep; ++q)
       c.timed(K_CPML_E, [&] {
-        if (c.zfused && c.zslab[q] && sizeof(T) == 8)
+        if (pairs(se, q)) {
+          xfd::Launch<T>::cpml2(st, c.g, F, se[q], se[q + 1], true);
+          ++q;
+        } else if (c.zfused && c.zslab[q] && sizeof(T) == 8)
           xfd::Launch<T>::cpml_zrest(st, c.g, F, se[q], ZE.fr, true);
         else if (inSweep(q) || (c.zfused && c.zslab[q]))
           xfd::Launch<T>::cpml_faces(st, c.g, F, se[q], true);
@@ -1732,6 +1750,30 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
   if (D.n_points)
     uploadVector(std::vector<double>(D.source, D.source + static_cast<int64_t>(D.n_points) * D.nt), src);
   using A = xfd::AdjLaunch<T>;
+  // The transposed slabs two per launch (XFD_CPML_PAIR=0: one each): the mu
+  // updates of any two slabs; the gathers of two slabs of one axis whose
+  // boxes, widened by the gathered plane, share no point.
+  const char* pe = std::getenv("XFD_CPML_PAIR");
+  const bool pair_on = !(pe && pe[0] == '0');
+  auto muAll = [&](const std::vector<xfd::SlabArgs<T>>& v) {
+    size_t q = 0;
+    for (; pair_on && q + 1 < v.size(); q += 2) A::cpmlMu2(st, c.g, L, v[q], v[q + 1]);
+    for (; q < v.size(); ++q) A::cpmlMu(st, c.g, L, v[q]);
+  };
+  auto gatherAll = [&](const std::vector<xfd::SlabArgs<T>>& v, bool is_e) {
+    for (size_t q = 0; q < v.size(); ++q) {
+      if (pair_on && q + 1 < v.size() && v[q].axis == v[q + 1].axis) {
+        const auto &a = v[q], &b = v[q + 1];
+        const int x = a.axis;
+        if (a.lo[x] + a.ext[x] + 1 <= b.lo[x] || b.lo[x] + b.ext[x] + 1 <= a.lo[x]) {
+          A::cpmlGather2(st, is_e, c.g, L, a, b);
+          ++q;
+          continue;
+        }
+      }
+      A::cpmlGather(st, is_e, c.g, L, v[q]);
+    }
+  };
   const int64_t n_ck = (c.ck_nt + K_int - 1) / K_int;
   T* H0 = hist.as<T>();
   for (int64_t seg = n_ck - 1; seg >= 0; --seg) {
@@ -1746,7 +1788,7 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
       A::source(st, L, pts.as<xfd::AdjPoint>(), D.n_points, src.as<double>(), D.nt, t);
       A::grad(st, c.g, L, d, H0 + (t - t0) * nd, H0 + (t - t0 + 1) * nd, beta.as<double>(), gamma.as<double>(),
               grad.as<double>());
-      for (auto& a : me) A::cpmlMu(st, c.g, L, a);
+      muAll(me);
       if (c.state.bytes)
         A::records(st, L, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e, lam_state.as<T>());
       if (c.ade_scheme >= 0) A::adePre(st, c.g, L, Ad, LA, X, c.ade_scheme);
@@ -1754,14 +1796,14 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
         A::adjH(st, c.g, L, K, c.ichunk);
       else
         A::adjH(st, c.g, L, C, c.ichunk);
-      for (auto& a : me) A::cpmlGather(st, true, c.g, L, a);
-      for (auto& a : mh) A::cpmlMu(st, c.g, L, a);
+      gatherAll(me, true);
+      muAll(mh);
       if (c.classed)
         A::adjE(st, c.g, L, K, c.ichunk);
       else
         A::adjE(st, c.g, L, C, c.ichunk);
       if (c.ade_scheme >= 0) A::adePost(st, c.g, L, Ad.mat, X);
-      for (auto& a : mh) A::cpmlGather(st, false, c.g, L, a);
+      gatherAll(mh, false);
       XFD_HIP(hipGetLastError());
     }
   }
@@ -1842,6 +1884,10 @@ struct Collect {
     total += groups;
   }
   void to(xfd_batch::Table& t) const { fillTable(t, v, start, total); }
+  void append(const Collect& o) {
+    for (size_t i = 0; i < o.v.size(); ++i)
+      add(o.v[i], (i + 1 < o.v.size() ? o.start[i + 1] : o.total) - o.start[i]);
+  }
 };
 
 // Global work-group ids an entry of `items` point-wise work-items owns.
@@ -1855,6 +1901,20 @@ void batchTables(xfd_batch& b) {
   std::vector<Collect<xfd::BatchSlab<T>>> ce, ch, fe, fh;
   Collect<xfd::BatchRec<T>> re, rh;
   Collect<xfd::BatchNf2ff<T>> nf;
+  // ordinal o and o + 1 share a launch when, in every case holding both, the
+  // two slabs run separately, lie on one axis and share no point
+  std::vector<char> pairable;
+  for (auto& up : b.cases) {
+    const xfd_ctx& c = *up;
+    if (pairable.size() < c.slabs.size()) pairable.resize(c.slabs.size(), 1);
+    for (size_t o = 0; o + 1 < c.slabs.size(); ++o) {
+      const auto &u = *c.slabs[o], &w = *c.slabs[o + 1];
+      const bool ok = static_cast<int>(o) >= c.n_fused && u.axis == w.axis &&
+                      (u.e_start + u.n <= w.e_start || w.e_start + w.n <= u.e_start) &&
+                      (u.h_start + u.n <= w.h_start || w.h_start + w.n <= u.h_start);
+      if (!ok) pairable[o] = 0;
+    }
+  }
   for (auto& up : b.cases) {
     xfd_ctx& c = *up;
     const auto F = fieldPtrs<T>(c);
@@ -1910,15 +1970,31 @@ void batchTables(xfd_batch& b) {
   re.to(b.rec_e);
   rh.to(b.rec_h);
   nf.to(b.nf);
+  const char* pair_env = std::getenv("XFD_CPML_PAIR");
+  const bool pair_on = !(pair_env && pair_env[0] == '0');
+  std::vector<size_t> launch_of(ce.size());
+  size_t nl = 0;
+  for (size_t o = 0; o < ce.size(); ++o) {
+    launch_of[o] = nl++;
+    if (pair_on && o + 1 < ce.size() && pairable[o] && fe[o].v.empty() && fh[o].v.empty() && fe[o + 1].v.empty() &&
+        fh[o + 1].v.empty()) {
+      launch_of[o + 1] = launch_of[o];
+      ce[o].append(ce[o + 1]);
+      ch[o].append(ch[o + 1]);
+      ++o;
+    }
+  }
   for (auto* v : {&b.cpml_e, &b.cpml_h, &b.faces_e, &b.faces_h}) {
     v->clear();
-    v->resize(ce.size());
+    v->resize(nl);
   }
   for (size_t o = 0; o < ce.size(); ++o) {
-    ce[o].to(b.cpml_e[o]);
-    ch[o].to(b.cpml_h[o]);
-    fe[o].to(b.faces_e[o]);
-    fh[o].to(b.faces_h[o]);
+    if (o > 0 && launch_of[o] == launch_of[o - 1]) continue;  // appended to the previous ordinal's launch
+    const size_t l = launch_of[o];
+    ce[o].to(b.cpml_e[l]);
+    ch[o].to(b.cpml_h[l]);
+    fe[o].to(b.faces_e[l]);
+    fh[o].to(b.faces_h[l]);
   }
 }
 

@@ -524,6 +524,18 @@ __global__ __launch_bounds__(256) void k_cpml(Grid g, FieldPtrs<T> F, SlabArgs<T
   cpmlCell<T, IsE>(g, F, s, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
+// Two slabs that share no point (the two faces of one axis) in one launch:
+// work-groups [0, nb0) take s0, the rest s1.  Each point still receives its
+// slabs' corrections in boundary order, since no point belongs to both.
+template <class T, bool IsE>
+__global__ __launch_bounds__(256) void k_cpml2(Grid g, FieldPtrs<T> F, SlabArgs<T> s0, SlabArgs<T> s1,
+                                               unsigned nb0) {
+  if (blockIdx.x < nb0)
+    cpmlCell<T, IsE>(g, F, s0, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+  else
+    cpmlCell<T, IsE>(g, F, s1, static_cast<int64_t>(blockIdx.x - nb0) * blockDim.x + threadIdx.x);
+}
+
 // The points of a slab box that the fused sweeps do not visit: the far grid
 // planes i = nx, j = ny, k = nz.  Point q of the union of the three plane
 // pieces (X: i = nx; Y: j = ny, i < nx; Z: k = nz, i < nx, j < ny), or false.
@@ -1153,6 +1165,8 @@ struct Launch {
   static void probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                      int64_t t);
   static void cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
+  static void cpml2(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a0, const SlabArgs<T>& a1,
+                    bool is_e);
   static void records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
                       const double* pool, T* state, int64_t t);
   static void gather(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const GatherArgs& a, double* out);

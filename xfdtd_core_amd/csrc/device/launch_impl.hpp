@@ -184,6 +184,18 @@ void Launch<T>::cpml(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const 
 }
 
 template <class T>
+void Launch<T>::cpml2(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a0,
+                      const SlabArgs<T>& a1, bool is_e) {
+  const unsigned nb0 = blocks_for(static_cast<int64_t>(a0.ext[0]) * a0.ext[1] * a0.ext[2], 256);
+  const unsigned nb1 = blocks_for(static_cast<int64_t>(a1.ext[0]) * a1.ext[1] * a1.ext[2], 256);
+  if (nb0 + nb1 == 0) return;
+  if (is_e)
+    hipLaunchKernelGGL((k_cpml2<T, true>), dim3(nb0 + nb1), dim3(256), 0, s, g, F, a0, a1, nb0);
+  else
+    hipLaunchKernelGGL((k_cpml2<T, false>), dim3(nb0 + nb1), dim3(256), 0, s, g, F, a0, a1, nb0);
+}
+
+template <class T>
 void Launch<T>::records(hipStream_t s, const FieldPtrs<T>& F, const Record* rec, const int64_t* seg, int64_t nseg,
                         const double* pool, T* state, int64_t t) {
   if (nseg <= 0) return;
@@ -397,6 +409,28 @@ void AdjLaunch<T>::cpmlGather(hipStream_t s, bool is_e, const Grid& g, const Fie
     hipLaunchKernelGGL((k_adj_cpml_gather<T, true>), dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, L, a);
   else
     hipLaunchKernelGGL((k_adj_cpml_gather<T, false>), dim3(blocks_for(vol, 256)), dim3(256), 0, s, g, L, a);
+}
+template <class T>
+void AdjLaunch<T>::cpmlMu2(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a0,
+                           const SlabArgs<T>& a1) {
+  const unsigned nb0 = blocks_for(static_cast<int64_t>(a0.ext[0]) * a0.ext[1] * a0.ext[2], 256);
+  const unsigned nb1 = blocks_for(static_cast<int64_t>(a1.ext[0]) * a1.ext[1] * a1.ext[2], 256);
+  if (nb0 + nb1 == 0) return;
+  hipLaunchKernelGGL(k_adj_cpml_mu2<T>, dim3(nb0 + nb1), dim3(256), 0, s, g, L, a0, a1, nb0);
+}
+template <class T>
+void AdjLaunch<T>::cpmlGather2(hipStream_t s, bool is_e, const Grid& g, const FieldPtrs<T>& L,
+                               const SlabArgs<T>& a0, const SlabArgs<T>& a1) {
+  auto widened = [](const SlabArgs<T>& a) {
+    const int64_t vol = static_cast<int64_t>(a.ext[0]) * a.ext[1] * a.ext[2];
+    return vol <= 0 ? int64_t(0) : vol / a.ext[a.axis] * (a.ext[a.axis] + 1);
+  };
+  const unsigned nb0 = blocks_for(widened(a0), 256), nb1 = blocks_for(widened(a1), 256);
+  if (nb0 + nb1 == 0) return;
+  if (is_e)
+    hipLaunchKernelGGL((k_adj_cpml_gather2<T, true>), dim3(nb0 + nb1), dim3(256), 0, s, g, L, a0, a1, nb0);
+  else
+    hipLaunchKernelGGL((k_adj_cpml_gather2<T, false>), dim3(nb0 + nb1), dim3(256), 0, s, g, L, a0, a1, nb0);
 }
 template <class T>
 void AdjLaunch<T>::hist(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const DesignArgs& d, T* h) {
