@@ -288,6 +288,7 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   T hz_p = i0 > 0 ? hz[idx - pl] : T(0);
   T hy_p = i0 > 0 ? hy[idx - pl] : T(0);
   const bool jk = j >= 1 && k >= 1;
+  const int64_t dk = k >= 1 ? 1 : 0, dj = j >= 1 ? pz : 0;
   const uint32_t mask = Fused ? cpmlChunkMask(P, i0, i1, j, k) : 0u;
   ZLane<T> zl;
   if (ZF) zl = zLane(Z, i0, j, k);
@@ -302,20 +303,23 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
     if (All ? !(si & sku) : (!Fused || !mask)) {
       // every lane stores its three values (unchanged on the faces the Yee
       // ranges leave out), so each wavefront writes whole lines: skipping the
-      // j = 0 / k = 0 lanes left partial lines and cost 6 % of the FP32 sweep
+      // j = 0 / k = 0 lanes left partial lines and cost 6 % of the FP32 sweep.
+      // The loads are unconditional (the k-1 / j-1 neighbours read at the
+      // point itself on the k = 0 / j = 0 faces) and the Yee ranges select,
+      // so all of a plane's loads are in flight together: loads under the
+      // per-component range branches waited one round trip per component.
       T vx = ldOwn(ex + idx), vy = ldOwn(ey + idx), vz = ldOwn(ez + idx);
-      if (jk) {
-        C.getOwn(0, idx, s, a, b);
-        vx = s * vx + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
-      }
-      if (i >= 1 && k >= 1) {
-        C.getOwn(1, idx, s, a, b);
-        vy = s * vy + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
-      }
-      if (i >= 1 && j >= 1) {
-        C.getOwn(2, idx, s, a, b);
-        vz = s * vz + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
-      }
+      const T hy_k = hy[idx - dk], hx_k = hx[idx - dk], hz_j = hz[idx - dj], hx_j = hx[idx - dj];
+      T s1, a1, b1, s2, a2, b2;
+      C.getOwn(0, idx, s, a, b);
+      C.getOwn(1, idx, s1, a1, b1);
+      C.getOwn(2, idx, s2, a2, b2);
+      const T ux = s * vx + a * (hy0 - hy_k) + b * (hz0 - hz_j);
+      const T uy = s1 * vy + a1 * (hz0 - hz_p) + b1 * (hx0 - hx_k);
+      const T uz = s2 * vz + a2 * (hx0 - hx_j) + b2 * (hy0 - hy_p);
+      vx = jk ? ux : vx;
+      vy = (i >= 1 && k >= 1) ? uy : vy;
+      vz = (i >= 1 && j >= 1) ? uz : vz;
       if (ZF) {
         zStore(Z, zl, zv, vx, vy, hy0 - zv.dualA, hx0 - zv.dualB);
         zl.t += zl.step;
