@@ -646,10 +646,77 @@ __device__ __forceinline__ T adeUpdate(const AdeArgs<T>& A, int m, int q, int64_
   return enew;
 }
 
+// adeUpdate for the three components of one cell with every load issued
+// first: the cell's pole currents (and, modified Lorentz, previous currents
+// and previous E) of all components, then the arithmetic of adeUpdate in its
+// order, component by component.  `r[q]`: component q is inside its Yee range
+// (outside it the forward update leaves E and the currents alone).  Up to
+// kAdeFastPoles poles (a uniform bound: the unused slots are never loaded).
+constexpr int kAdeFastPoles = 4;
+template <class T, int S>
+__device__ __forceinline__ void adeCell(const AdeArgs<T>& A, int m, int64_t idx, const bool r[3], const T cur[3],
+                                        const T en[3], T out[3]) {
+  constexpr int P = kAdeFastPoles;
+  const int np = A.poles;
+  const T* c = A.tab + static_cast<int64_t>(m) * A.stride;
+  T jv[3][P], jpv[3][P], ep[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    ep[q] = S == kAdeMLorentz ? A.eprev[q][idx] : T(0);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      jv[q][p] = T(0);
+      jpv[q][p] = T(0);
+      if (p < np) {
+        jv[q][p] = A.j[q][p * A.total + idx];
+        if (S == kAdeMLorentz) jpv[q][p] = A.jp[q][p * A.total + idx];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    out[q] = cur[q];
+    if (!r[q]) continue;
+    T jsum = T(0);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (p >= np) break;
+      const T* cp = c + 2 + 7 * p;
+      if (S == kAdeMLorentz)
+        jsum += cp[5] * jv[q][p] + cp[6] * jpv[q][p];
+      else
+        jsum += cp[5] * jv[q][p];
+    }
+    const T eprev = S == kAdeMLorentz ? ep[q] : T(0);
+    const T ceep = S == kAdeMLorentz ? c[1] : T(0);
+    const T enew = ceep * eprev + en[q] + c[0] * jsum;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (p >= np) break;
+      const T* cp = c + 2 + 7 * p;
+      const int64_t o = p * A.total + idx;
+      const T j0 = jv[q][p];
+      if (S == kAdeDebye) {
+        A.j[q][o] = cp[3] * j0 + cp[1] * (enew - cur[q]);
+      } else if (S == kAdeDrude) {
+        A.j[q][o] = cp[3] * j0 + cp[1] * (enew + cur[q]);
+      } else {
+        const T jn = cp[0] * enew + cp[1] * cur[q] + cp[2] * eprev + cp[3] * j0 + cp[4] * jpv[q][p];
+        A.jp[q][o] = j0;
+        A.j[q][o] = jn;
+      }
+    }
+    if (S == kAdeMLorentz) A.eprev[q][idx] = cur[q];
+    out[q] = enew;
+  }
+}
+
 // The E sweep of k_update_e with the dispersive update on cells whose
 // material id is set (the reference runs its ADE updator on every cell; on
 // a cell without dispersion its coefficients are zero and the result is the
-// plain update).
+// plain update).  As in sweepE the curl terms' loads are unconditional (the
+// k-1 / j-1 neighbours read at the point itself on the k = 0 / j = 0 faces)
+// and the Yee ranges select, so a plane's loads are in flight together.
 template <class T, class Coef, int S>
 __device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
                                           int bx, int by, int bz, int ichunk) {
@@ -669,30 +736,33 @@ __device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, 
   T hz_p = i0 > 0 ? hz[idx - pl] : T(0);
   T hy_p = i0 > 0 ? hy[idx - pl] : T(0);
   const bool jk = j >= 1 && k >= 1;
+  const int64_t dk = k >= 1 ? 1 : 0, dj = j >= 1 ? pz : 0;
+  const bool fast = A.poles <= kAdeFastPoles;
   for (int i = i0; i < i1; ++i, idx += pl) {
     const T hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
     const int m = ldOwn(A.mat + idx);
-    T s, a, b;
     // whole-line stores as in sweepE (face values rewritten unchanged)
-    T vx = ldOwn(ex + idx), vy = ldOwn(ey + idx), vz = ldOwn(ez + idx);
-    if (jk) {
-      C.getOwn(0, idx, s, a, b);
-      const T en = s * vx + a * (hy0 - hy[idx - 1]) + b * (hz0 - hz[idx - pz]);
-      vx = m < 0 ? en : adeUpdate<T, S>(A, m, 0, idx, vx, en);
+    const T cur[3] = {ldOwn(ex + idx), ldOwn(ey + idx), ldOwn(ez + idx)};
+    const T hy_k = hy[idx - dk], hx_k = hx[idx - dk], hz_j = hz[idx - dj], hx_j = hx[idx - dj];
+    T s0, a0, b0, s1, a1, b1, s2, a2, b2;
+    C.getOwn(0, idx, s0, a0, b0);
+    C.getOwn(1, idx, s1, a1, b1);
+    C.getOwn(2, idx, s2, a2, b2);
+    const T en[3] = {s0 * cur[0] + a0 * (hy0 - hy_k) + b0 * (hz0 - hz_j),
+                     s1 * cur[1] + a1 * (hz0 - hz_p) + b1 * (hx0 - hx_k),
+                     s2 * cur[2] + a2 * (hx0 - hx_j) + b2 * (hy0 - hy_p)};
+    const bool r[3] = {jk, i >= 1 && k >= 1, i >= 1 && j >= 1};
+    T v[3];
+    if (m < 0) {
+      for (int q = 0; q < 3; ++q) v[q] = r[q] ? en[q] : cur[q];
+    } else if (fast) {
+      adeCell<T, S>(A, m, idx, r, cur, en, v);
+    } else {
+      for (int q = 0; q < 3; ++q) v[q] = r[q] ? adeUpdate<T, S>(A, m, q, idx, cur[q], en[q]) : cur[q];
     }
-    if (i >= 1 && k >= 1) {
-      C.getOwn(1, idx, s, a, b);
-      const T en = s * vy + a * (hz0 - hz_p) + b * (hx0 - hx[idx - 1]);
-      vy = m < 0 ? en : adeUpdate<T, S>(A, m, 1, idx, vy, en);
-    }
-    if (i >= 1 && j >= 1) {
-      C.getOwn(2, idx, s, a, b);
-      const T en = s * vz + a * (hx0 - hx[idx - pz]) + b * (hy0 - hy_p);
-      vz = m < 0 ? en : adeUpdate<T, S>(A, m, 2, idx, vz, en);
-    }
-    stOwn(ex + idx, vx);
-    stOwn(ey + idx, vy);
-    stOwn(ez + idx, vz);
+    stOwn(ex + idx, v[0]);
+    stOwn(ey + idx, v[1]);
+    stOwn(ez + idx, v[2]);
     hz_p = hz0;
     hy_p = hy0;
   }
