@@ -2347,7 +2347,9 @@ int xfd_nf2ff_far_field(xfd_ctx* c, int32_t n, int32_t f, int32_t na, const doub
     fa.phi = ang[1].as<double>();
     fa.k = k;
     out.alloc(static_cast<size_t>(na) * 8 * sizeof(double));
-    xfd::Launch<double>::far_field(c->stream, N.a, f, fa, na, out.as<double>());
+    DevBuf part;
+    part.alloc(static_cast<size_t>(na) * xfd::Launch<double>::farChunks(na) * 8 * sizeof(double));
+    xfd::Launch<double>::far_field(c->stream, N.a, f, fa, na, part.as<double>(), out.as<double>());
     XFD_HIP(hipGetLastError());
     XFD_HIP(hipMemcpyAsync(dst, out.p, out.bytes, hipMemcpyDeviceToHost, c->stream));
     XFD_HIP(hipStreamSynchronize(c->stream));
@@ -2405,7 +2407,7 @@ int xfd_nf2ff_far_field_noalloc(xfd_ctx* c, int32_t n, int32_t f, int32_t na, co
     require(cur == c->device, "the context's device must be current");
     const int dims[3] = {c->g.nx, c->g.ny, c->g.nz};
     // layout: per axis e_node (N+1), h_node (N), e_size (N); theta, phi;
-    // the output (na x 8)
+    // the output (na x 8); the per-chunk partial sums (na x chunks x 8)
     size_t need = 0;
     size_t off_geo[9];
     for (int a = 0; a < 3; ++a) {
@@ -2422,6 +2424,8 @@ int xfd_nf2ff_far_field_noalloc(xfd_ctx* c, int32_t n, int32_t f, int32_t na, co
     need += alignUp(static_cast<size_t>(na) * sizeof(double));
     const size_t off_out = need;
     need += alignUp(static_cast<size_t>(na) * 8 * sizeof(double));
+    const size_t off_part = need;  // per-chunk partial sums
+    need += alignUp(static_cast<size_t>(na) * xfd::Launch<double>::farChunks(na) * 8 * sizeof(double));
     if (need > kScratchBytes)
       throw XfdError(XFD_EINVAL, "xfd_nf2ff_far_field_noalloc: grid and " + std::to_string(na) +
                                      " directions need " + std::to_string(need) + " bytes of scratch, above its " +
@@ -2449,7 +2453,7 @@ int xfd_nf2ff_far_field_noalloc(xfd_ctx* c, int32_t n, int32_t f, int32_t na, co
     fa.phi = reinterpret_cast<const double*>(base + off_ph);
     fa.k = k;
     double* out = reinterpret_cast<double*>(base + off_out);
-    xfd::Launch<double>::far_field(c->stream, N.a, f, fa, na, out);
+    xfd::Launch<double>::far_field(c->stream, N.a, f, fa, na, reinterpret_cast<double*>(base + off_part), out);
     XFD_HIP(hipGetLastError());
     XFD_HIP(hipMemcpyAsync(dst, out, static_cast<size_t>(na) * 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));

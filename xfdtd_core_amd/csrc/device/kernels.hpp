@@ -871,11 +871,17 @@ struct FarArgs {
   double k;                 // 2 pi f / c
 };
 
+// Work-groups (direction, chunk): chunk c of `chunks` sums its share of the
+// surface cells for direction n into part[(n * chunks + c) * 8 + w];
+// k_far_reduce then adds the chunks in order (a fixed summation order, so
+// repeated calls give the same bits).  Several chunks per direction keep the
+// GPU filled when there are few directions.
 // Templated on the field type only so that each precision's unit holds its own copy.
 template <class T>
-__global__ __launch_bounds__(256) void k_far_field(Nf2ffArgs a, int f, FarArgs fa, double* out) {
+__global__ __launch_bounds__(256) void k_far_field(Nf2ffArgs a, int f, FarArgs fa, int chunks, double* part) {
   __shared__ double red[8][256];
   const int n = blockIdx.x;
+  const int chunk = blockIdx.y;
   const double th = fa.theta[n], ph = fa.phi[n];
   const double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
   const double stcp = st * cp, stsp = st * sp;
@@ -884,7 +890,9 @@ __global__ __launch_bounds__(256) void k_far_field(Nf2ffArgs a, int f, FarArgs f
   const int64_t cells = a.off[6];
   const double2* acc = a.acc + static_cast<int64_t>(f) * 4 * cells;
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // A_theta, A_phi, F_theta, F_phi (re, im)
-  for (int64_t q = threadIdx.x; q < cells; q += blockDim.x) {
+  const int64_t per = (cells + chunks - 1) / chunks;
+  const int64_t q0 = static_cast<int64_t>(chunk) * per, q1 = min(cells, q0 + per);
+  for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     int d = 0;
     while (q >= a.off[d + 1]) ++d;
     const int64_t l = q - a.off[d];
@@ -936,7 +944,18 @@ __global__ __launch_bounds__(256) void k_far_field(Nf2ffArgs a, int f, FarArgs f
       for (int w = 0; w < 8; ++w) red[w][threadIdx.x] += red[w][threadIdx.x + half];
     __syncthreads();
   }
-  if (threadIdx.x < 8) out[static_cast<int64_t>(n) * 8 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 8) part[(static_cast<int64_t>(n) * chunks + chunk) * 8 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_far_reduce(int na, int chunks, const double* __restrict__ part,
+                                                    double* __restrict__ out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<int64_t>(na) * 8) return;
+  const int64_t n = t / 8, w = t % 8;
+  double v = 0;
+  for (int c = 0; c < chunks; ++c) v += part[(n * chunks + c) * 8 + w];
+  out[t] = v;
 }
 
 // Lumped-element and TFSF corrections: one thread per corrected cell applies
@@ -1250,7 +1269,11 @@ struct Launch {
   // time-domain NF2FF: both phases of step t
   static void tdNf2ff(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const TdNf2ffArgs& a, int64_t t);
   // na directions, out: na x 8 doubles (A_theta, A_phi, F_theta, F_phi as re, im)
-  static void far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* out);
+  // chunks per direction for na directions, and the partial sums it needs
+  // (doubles) beside the na x 8 output
+  static int farChunks(int na);
+  static void far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* part,
+                        double* out);
   static void unpack(hipStream_t s, const Grid& g, const double* src, T* dst, int n0, int n1, int n2);
   static void pack(hipStream_t s, const Grid& g, const T* src, double* dst, int n0, int n1, int n2);
 };

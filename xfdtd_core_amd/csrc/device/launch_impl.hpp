@@ -317,9 +317,18 @@ void BatchLaunch<T>::tick(hipStream_t s, int64_t* t) {
 }
 
 template <class T>
-void Launch<T>::far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* out) {
+int Launch<T>::farChunks(int na) {
+  // about 2048 work-groups in all, at most 32 per direction
+  return static_cast<int>(std::max(1, std::min(32, (2048 + std::max(na, 1) - 1) / std::max(na, 1))));
+}
+template <class T>
+void Launch<T>::far_field(hipStream_t s, const Nf2ffArgs& a, int f, const FarArgs& fa, int na, double* part,
+                          double* out) {
   if (na <= 0) return;
-  hipLaunchKernelGGL(k_far_field<T>, dim3(na), dim3(256), 0, s, a, f, fa, out);
+  const int chunks = farChunks(na);
+  hipLaunchKernelGGL(k_far_field<T>, dim3(na, chunks), dim3(256), 0, s, a, f, fa, chunks, part);
+  hipLaunchKernelGGL(k_far_reduce<T>, dim3(blocks_for(static_cast<int64_t>(na) * 8, 256)), dim3(256), 0, s, na,
+                     chunks, part, out);
 }
 
 // ---------------------------------------------------------------- adjoint --
