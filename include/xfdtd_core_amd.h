@@ -396,10 +396,12 @@ int xfd_batch_tile(xfd_batch* batch, int32_t* lanes_k, int32_t* rows_j);
 /* on = 1: the first xfd_batch_step captures one step as a HIP graph and
  * every step replays it. */
 int xfd_batch_set_graph(xfd_batch* batch, int32_t on);
-/* Autotuner: times `reps` Yee sweep pairs for each compiled sweep tile
+/* Autotuner: times `reps` steps' Yee sweeps and CPML launches (no records,
+ * no DFTs) for each compiled sweep tile
  * (64x4, 64x8, 128x2, 128x4, 64x2 k-lanes by j-rows), i-chunk (4, 8, 16,
- * 32) and persistent grid size (4, 8, 16, 32 work-groups per CU), keeps the
- * fastest, then zeroes every case's state (xfd_reset). */
+ * 32) and persistent grid size (4, 8, 16, 32 work-groups per CU), times the
+ * four fastest again (three trials of 2 x reps steps, each one's best) and
+ * keeps the fastest of those, then zeroes every case's state (xfd_reset). */
 int xfd_batch_tune(xfd_batch* batch, int32_t reps);
 /* Runs steps t0 .. t0+n-1 of every case and waits for them. */
 int xfd_batch_step(xfd_batch* batch, int64_t t0, int64_t n);
