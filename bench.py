@@ -178,7 +178,7 @@ def run_device(sim, precision, warmup, steps, n, pml):
             avg = kernels["step"].get("timed_region", {}).get("avg_ms", kernels["step"]["avg_ms"])
             kernels["step"]["layout_bytes"] = sb
             kernels["step"]["layout_GBps"] = round(sb / (avg * 1e-3) / 1e9, 1)
-    res["coefficients"] = "class ids" if sim.coef_classed() else "dense"
+    res["coefficients"] = {0: "dense", 1: "class ids", 2: "32-bit class ids"}[sim.coef_layout()]
     res["cpml"] = {0: "separate slab launches", 1: "x/y slabs inside the Yee sweeps",
                    2: "z slabs inside the Yee sweeps, x/y slabs separate",
                    3: "every slab inside the single-pass step",
@@ -617,7 +617,8 @@ def bench_adjoint(args):
         g = sim.backward(eps.shape)
         tb = time.perf_counter() - t0
         grads[prec] = g
-        out[prec] = {"J": J, "forward_s": round(tf, 3), "backward_s": round(tb, 3)}
+        out[prec] = {"J": J, "forward_s": round(tf, 3), "backward_s": round(tb, 3),
+                     "coefficients": {0: "dense", 1: "class ids", 2: "32-bit class ids"}[sim.coef_layout()]}
         log(f"adjoint {prec}: J {J:.6e}, forward {tf:.2f} s (incl. init), backward {tb:.2f} s")
         sim.close()
     res = {"steps": steps, "checkpoint_interval": K, "design_cells": int(eps.size), "runs": out}

@@ -292,8 +292,11 @@ int xfd_device_bytes(xfd_ctx* ctx, int64_t* bytes);
 
 /* Coefficient storage chosen by xfd_create: *classed = 1 when every
  * component's distinct (self, dual-A, dual-B) triples fit a 16-bit class id
- * (classes[q] = table size of Ex..Hz), 0 for the 18 dense arrays.  The
- * environment variable XFD_COEF_MODE=dense forces the dense layout. */
+ * (classes[q] = table size of Ex..Hz); 2 (3D grids) when some component has
+ * more distinct triples than that but none more than a quarter of its
+ * cells: 32-bit class ids; 0 for the 18 dense arrays.  The environment
+ * variable XFD_COEF_MODE=dense forces the dense layout, XFD_COEF_MODE=classed
+ * rules out the 32-bit ids. */
 int xfd_coef_layout(xfd_ctx* ctx, int32_t* classed, int64_t* classes);
 
 /* Which CPML slabs the Yee sweeps apply in the same pass (the far grid

@@ -841,6 +841,8 @@ class Simulation {
   // True when the device holds per-cell coefficient class ids instead of the
   // 18 dense coefficient arrays (same values; chosen at init).
   bool classedCoefficients() const;
+  // 0: dense coefficient arrays, 1: class ids, 2: 32-bit class ids
+  int coefficientLayout() const;
   // Which CPML slabs the Yee sweeps apply in the same pass: 0 none, 1 the
   // leading x/y slabs (XFD_CPML_FUSED=1), 2 the z slabs (FP32 when nz > 192,
   // either precision with XFD_CPML_ZFUSED=1).

@@ -42,7 +42,8 @@ double xfd_sim_dt(void* sim);
 double xfd_sim_loop_seconds(void* sim);
 int xfd_sim_grid(void* sim, int64_t* n3);
 int64_t xfd_sim_device_bytes(void* sim);
-/* 1 when the device holds coefficient class ids (xfd_coef_layout), else 0. */
+/* Coefficient storage as xfd_coef_layout: 1 16-bit class ids, 2 32-bit
+ * class ids, 0 dense arrays. */
 int32_t xfd_sim_coef_classed(void* sim);
 /* Which CPML slabs run inside the Yee sweeps: 0, 1 or 2 as xfd_cpml_fused. */
 int32_t xfd_sim_cpml_fused(void* sim);

@@ -237,3 +237,28 @@ def test_design_edge_under_a_record_is_refused():
     d, J = _device(s, "f64", 50)
     with pytest.raises(RuntimeError, match="design edge"):
         d.backward(eps.shape)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_wide_class_ids_equal_dense(monkeypatch, precision):
+    """A 40^3 design block of random permittivities gives each component
+    ~67k distinct coefficient triples, more than 16-bit class ids hold: the
+    context takes 32-bit class ids.  Forward objective and gradient equal
+    the dense layout's bitwise (the same coefficient values, read either
+    way)."""
+    s, origin, eps = scenarios.adjoint_design(n_total=64, pml=6, design=40, patch_gap=2, tfsf_margin=3, seed=6)
+    runs = {}
+    for mode in ("wide", "dense"):
+        if mode == "dense":
+            monkeypatch.setenv("XFD_COEF_MODE", "dense")
+        d = native.Simulation(s.spec(), precision)
+        J = d.forward(200, 50)
+        layout = d.coef_layout()
+        g = d.backward(eps.shape)
+        runs[mode] = (J, g, layout)
+        d.close()
+    say(f"[gpu] 32-bit class ids vs dense coefficients ({precision}): J {runs['wide'][0]:.6e}, layouts "
+        f"{runs['wide'][2]} / {runs['dense'][2]}")
+    assert runs["wide"][2] == 2 and runs["dense"][2] == 0
+    assert runs["wide"][0] == runs["dense"][0] and runs["wide"][0] > 0
+    np.testing.assert_array_equal(runs["wide"][1], runs["dense"][1])

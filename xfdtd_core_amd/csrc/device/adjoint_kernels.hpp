@@ -652,6 +652,8 @@ struct AdjLaunch {
   static void adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk);
   static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk);
   static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk);
+  static void adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const WideCoef<T>& C, int ichunk);
+  static void adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const WideCoef<T>& C, int ichunk);
   static void cpmlMu(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a);
   static void cpmlGather(hipStream_t s, bool is_e, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a);
   static void cpmlMu2(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const SlabArgs<T>& a0,

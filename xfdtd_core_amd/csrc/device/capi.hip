@@ -86,8 +86,10 @@ struct TripleHash {
   }
 };
 
+// Id: uint16_t (at most 65535 classes) or uint32_t (at most max_classes).
+template <class Id>
 bool classify(const double* A, const double* B, const double* C, const int ext[3], const xfd::Grid& g,
-              std::vector<uint16_t>& ids, std::vector<double>& table) {
+              std::vector<Id>& ids, std::vector<double>& table, size_t max_classes = 65535) {
   const int64_t plane = static_cast<int64_t>(ext[1]) * ext[2];
   const unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   const int parts = static_cast<int>(std::min<int64_t>(hw, ext[0]));
@@ -121,7 +123,7 @@ bool classify(const double* A, const double* B, const double* C, const int ext[3
           last = t;
           have = true;
           if (seen.emplace(t, 0).second) found[p].push_back(t);
-          if (seen.size() > 65535) {
+          if (seen.size() > max_classes) {
             overflow = true;
             return;
           }
@@ -130,13 +132,13 @@ bool classify(const double* A, const double* B, const double* C, const int ext[3
     for (auto& t : th) t.join();
   }
   if (overflow) return false;
-  std::unordered_map<Triple, uint16_t, TripleHash> id;
+  std::unordered_map<Triple, Id, TripleHash> id;
   table.clear();
   for (auto& f : found)
     for (auto& t : f)
       if (id.find(t) == id.end()) {
-        if (id.size() >= 65535) return false;
-        id.emplace(t, static_cast<uint16_t>(id.size()));
+        if (id.size() >= max_classes) return false;
+        id.emplace(t, static_cast<Id>(id.size()));
         for (int q = 0; q < 3; ++q) {
           double d;
           std::memcpy(&d, &t.v[q], 8);
@@ -151,7 +153,7 @@ bool classify(const double* A, const double* B, const double* C, const int ext[3
       int lo, hi;
       rangeOf(p, lo, hi);
       Triple last{};
-      uint16_t last_id = 0;
+      Id last_id = 0;
       bool have = false;
       for (int64_t i = lo; i < hi; ++i)
         for (int64_t j = 0; j < ext[1]; ++j)
@@ -230,7 +232,10 @@ struct xfd_ctx {
   DevBuf fields[6];
   DevBuf coefs[18];   // dense coefficient layout
   bool classed = false;
-  DevBuf ids[6];      // classed layout: uint16 class per cell and component
+  // wide class ids (classed too): uint32 ids, when a component has more
+  // than 65535 distinct triples
+  bool wide = false;
+  DevBuf ids[6];      // classed layout: uint16 (uint32 when wide) class per cell and component
   DevBuf tabs[6];     // and the triple table of each component
   int64_t classes[6] = {0, 0, 0, 0, 0, 0};
 
@@ -452,6 +457,16 @@ void leaveStep(xfd_ctx& c) {
 }
 
 template <class T>
+xfd::WideCoef<T> wideCoef(xfd_ctx& c) {
+  xfd::WideCoef<T> M{};
+  for (int q = 0; q < 6; ++q) {
+    M.id[q] = c.ids[q].as<uint32_t>();
+    M.tab[q] = c.tabs[q].as<T>();
+  }
+  return M;
+}
+
+template <class T>
 xfd::CoefPtrs<T> coefPtrs(xfd_ctx& c) {
   xfd::CoefPtrs<T> C;
   for (int i = 0; i < 18; ++i) C.c[i] = c.coefs[i].as<T>();
@@ -629,23 +644,47 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     max_ext = std::max(max_ext, static_cast<size_t>(ext[0]) * ext[1] * ext[2]);
   }
   for (int s = 0; s < 18; ++s) require(d.coeff[s] != nullptr, "coefficient array missing");
-  // Classed layout unless a component has too many distinct triples, or
-  // XFD_COEF_MODE=dense asks for the 18-array layout.
+  // Classed layout with 16-bit ids; when a component has more than 65535
+  // distinct triples, 32-bit ids for every component (3D grids) while each
+  // component's distinct triples are at most a quarter of its cells;
+  // otherwise, or with XFD_COEF_MODE=dense, the 18-array dense layout.
+  // XFD_COEF_MODE=classed rules out the 32-bit ids.
   const char* mode = std::getenv("XFD_COEF_MODE");
-  c.classed = !(mode && std::string(mode) == "dense");
-  for (int q = 0; q < 6 && c.classed; ++q) {
-    int ext[3];
-    extentOf(q, g.nx, g.ny, g.nz, ext);
-    std::vector<uint16_t> ids;
-    std::vector<double> tab;
-    if (!classify(d.coeff[3 * q], d.coeff[3 * q + 1], d.coeff[3 * q + 2], ext, g, ids, tab)) {
+  const bool force_dense = mode && std::string(mode) == "dense";
+  const bool no_wide = mode && std::string(mode) == "classed";
+  c.classed = !force_dense;
+  c.wide = false;
+  for (int pass = 0; pass < 2 && c.classed; ++pass) {
+    const bool wide = pass == 1;
+    if (wide && (no_wide || c.dim != 3)) {
       c.classed = false;
       break;
     }
-    uploadVector(ids, c.ids[q]);
-    std::vector<T> tt(tab.begin(), tab.end());
-    uploadVector(tt, c.tabs[q]);
-    c.classes[q] = static_cast<int64_t>(tab.size() / 3);
+    bool ok = true;
+    for (int q = 0; q < 6 && ok; ++q) {
+      int ext[3];
+      extentOf(q, g.nx, g.ny, g.nz, ext);
+      std::vector<double> tab;
+      if (wide) {
+        std::vector<uint32_t> ids;
+        const size_t cells = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+        ok = classify(d.coeff[3 * q], d.coeff[3 * q + 1], d.coeff[3 * q + 2], ext, g, ids, tab, cells / 4);
+        if (ok) uploadVector(ids, c.ids[q]);
+      } else {
+        std::vector<uint16_t> ids;
+        ok = classify(d.coeff[3 * q], d.coeff[3 * q + 1], d.coeff[3 * q + 2], ext, g, ids, tab);
+        if (ok) uploadVector(ids, c.ids[q]);
+      }
+      if (!ok) break;
+      std::vector<T> tt(tab.begin(), tab.end());
+      uploadVector(tt, c.tabs[q]);
+      c.classes[q] = static_cast<int64_t>(tab.size() / 3);
+    }
+    if (ok) {
+      c.wide = wide;
+      break;
+    }
+    if (wide) c.classed = false;
   }
   if (!c.classed) {
     for (int q = 0; q < 6; ++q) {
@@ -655,11 +694,11 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     }
     DevBuf staging;
     staging.alloc(max_ext * sizeof(double));
-    for (int s = 0; s < 18; ++s) {
+    for (int s3 = 0; s3 < 18; ++s3) {
       int ext[3];
-      extentOf(coefField(s), g.nx, g.ny, g.nz, ext);
-      c.coefs[s].alloc(g.total * sizeof(T));
-      uploadPadded<T>(c, d.coeff[s], ext, c.coefs[s].as<T>(), staging);
+      extentOf(coefField(s3), g.nx, g.ny, g.nz, ext);
+      c.coefs[s3].alloc(g.total * sizeof(T));
+      uploadPadded<T>(c, d.coeff[s3], ext, c.coefs[s3].as<T>(), staging);
     }
   }
 
@@ -1349,6 +1388,7 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
     K.id[q] = c.ids[q].as<uint16_t>();
     K.tab[q] = c.tabs[q].as<T>();
   }
+  const auto M = wideCoef<T>(c);
   xfd::AdeArgs<T> A{};
   if (c.ade_scheme >= 0) {
     A.poles = c.ade_poles;
@@ -1417,6 +1457,8 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
           xfd::Launch<T>::update_low(st, c.g, F, K, c.dim, true);
         else
           xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, true);
+      } else if (c.wide) {
+        xfd::Launch<T>::update_h(st, c.g, F, M, c.ichunk, PH, ZH, AH);
       } else if (c.classed) {
         xfd::Launch<T>::update_h(st, c.g, F, K, c.ichunk, PH, ZH, AH);
       } else {
@@ -1447,10 +1489,14 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
         else
           xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, false);
       } else if (c.ade_scheme >= 0) {
-        if (c.classed)
+        if (c.wide)
+          xfd::Launch<T>::update_e_ade(st, c.g, F, M, A, c.ade_scheme, c.ichunk);
+        else if (c.classed)
           xfd::Launch<T>::update_e_ade(st, c.g, F, K, A, c.ade_scheme, c.ichunk);
         else
           xfd::Launch<T>::update_e_ade(st, c.g, F, C, A, c.ade_scheme, c.ichunk);
+      } else if (c.wide) {
+        xfd::Launch<T>::update_e(st, c.g, F, M, c.ichunk, PE, ZE, AE);
       } else if (c.classed) {
         xfd::Launch<T>::update_e(st, c.g, F, K, c.ichunk, PE, ZE, AE);
       } else {
@@ -1679,6 +1725,7 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
     K.id[q] = c.ids[q].as<uint16_t>();
     K.tab[q] = c.tabs[q].as<T>();
   }
+  const auto Mx = wideCoef<T>(c);
   // adjoint fields, adjoint psi, history, design coefficients, objective
   DevBuf lam[6];
   xfd::FieldPtrs<T> L;
@@ -1792,13 +1839,17 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
       if (c.state.bytes)
         A::records(st, L, c.rec_e.as<xfd::Record>(), c.seg_e.as<int64_t>(), c.nseg_e, lam_state.as<T>());
       if (c.ade_scheme >= 0) A::adePre(st, c.g, L, Ad, LA, X, c.ade_scheme);
-      if (c.classed)
+      if (c.wide)
+        A::adjH(st, c.g, L, Mx, c.ichunk);
+      else if (c.classed)
         A::adjH(st, c.g, L, K, c.ichunk);
       else
         A::adjH(st, c.g, L, C, c.ichunk);
       gatherAll(me, true);
       muAll(mh);
-      if (c.classed)
+      if (c.wide)
+        A::adjE(st, c.g, L, Mx, c.ichunk);
+      else if (c.classed)
         A::adjE(st, c.g, L, K, c.ichunk);
       else
         A::adjE(st, c.g, L, C, c.ichunk);
@@ -2653,7 +2704,7 @@ int xfd_read_td_nf2ff(xfd_ctx* c, int32_t n, int32_t d, int32_t which, double* d
 int xfd_coef_layout(xfd_ctx* c, int32_t* classed, int64_t* classes) {
   return guard([&] {
     require(c && classed, "bad argument");
-    *classed = c->classed ? 1 : 0;
+    *classed = c->wide ? 2 : (c->classed ? 1 : 0);
     if (classes)
       for (int q = 0; q < 6; ++q) classes[q] = c->classes[q];
   });

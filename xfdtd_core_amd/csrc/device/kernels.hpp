@@ -1242,6 +1242,12 @@ struct Launch {
   static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int ichunk,
                        const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
                        const StepSlabs<T>& S = StepSlabs<T>{});
+  static void update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const WideCoef<T>& C, int ichunk,
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
+                       const StepSlabs<T>& S = StepSlabs<T>{});
+  static void update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const WideCoef<T>& C, int ichunk,
+                       const CpmlList<T>& P = CpmlList<T>{}, const ZCpml<T>& Z = ZCpml<T>{},
+                       const StepSlabs<T>& S = StepSlabs<T>{});
   // the Yee half-steps of a 2D (TE, dim 2) or 1D (TEM, dim 1) grid; the 1D
   // H half-step is the 3D sweep (update_h)
   static void update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int dim,
@@ -1254,6 +1260,8 @@ struct Launch {
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
                            const AdeArgs<T>& A, int scheme, int ichunk);
   static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C,
+                           const AdeArgs<T>& A, int scheme, int ichunk);
+  static void update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const WideCoef<T>& C,
                            const AdeArgs<T>& A, int scheme, int ichunk);
   static void probes(hipStream_t s, const FieldPtrs<T>& F, const Probe* p, int n, double* out, int64_t stride,
                      int64_t t);

@@ -75,6 +75,16 @@ void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, co
                          const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
   launchUpdate<T>(false, s, g, F, C, ichunk, P, Z, S);
 }
+template <class T>
+void Launch<T>::update_h(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const WideCoef<T>& C, int ichunk,
+                         const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  launchUpdate<T>(true, s, g, F, C, ichunk, P, Z, S);
+}
+template <class T>
+void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const WideCoef<T>& C, int ichunk,
+                         const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  launchUpdate<T>(false, s, g, F, C, ichunk, P, Z, S);
+}
 template <class T, class Coef>
 void launchLow(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int dim, bool is_h) {
   if (dim == 2) {
@@ -146,6 +156,11 @@ void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F
 }
 template <class T>
 void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C,
+                             const AdeArgs<T>& A, int scheme, int ichunk) {
+  launchUpdateAde<T>(s, g, F, C, A, scheme, ichunk);
+}
+template <class T>
+void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const WideCoef<T>& C,
                              const AdeArgs<T>& A, int scheme, int ichunk) {
   launchUpdateAde<T>(s, g, F, C, A, scheme, ichunk);
 }
@@ -372,11 +387,19 @@ void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, con
   launchAdj<T>(true, s, g, L, C, ichunk);
 }
 template <class T>
+void AdjLaunch<T>::adjH(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const WideCoef<T>& C, int ichunk) {
+  launchAdj<T>(true, s, g, L, C, ichunk);
+}
+template <class T>
 void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const CoefPtrs<T>& C, int ichunk) {
   launchAdj<T>(false, s, g, L, C, ichunk);
 }
 template <class T>
 void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const ClassCoef<T>& C, int ichunk) {
+  launchAdj<T>(false, s, g, L, C, ichunk);
+}
+template <class T>
+void AdjLaunch<T>::adjE(hipStream_t s, const Grid& g, const FieldPtrs<T>& L, const WideCoef<T>& C, int ichunk) {
   launchAdj<T>(false, s, g, L, C, ichunk);
 }
 template <class T>

@@ -112,9 +112,9 @@ struct CoefPtrs {
   }
 };
 
-template <class T>
+template <class T, class Id = uint16_t>
 struct ClassCoef {
-  const uint16_t* id[6];
+  const Id* id[6];
   const T* tab[6];  // 3 values per class
   __device__ __forceinline__ void get(int q, int64_t idx, T& s, T& a, T& b) const {
     const T* e = tab[q] + 3 * static_cast<int>(id[q][idx]);
@@ -142,6 +142,13 @@ struct ClassCoef {
     b = e[2];
   }
 };
+
+// 32-bit class ids, for grids with more distinct triples per component than
+// 16-bit ids hold (per-cell permittivity maps, design regions): 4 bytes per
+// cell and component instead of 2, still far below the dense layout's three
+// coefficients, and the same branch-free reads.
+template <class T>
+using WideCoef = ClassCoef<T, uint32_t>;
 
 // ADE state and per-material coefficients for the dispersive E update.
 // `mat` is a per-cell material id in the padded layout (-1: plain update);

@@ -415,7 +415,7 @@ int xfd_sim_grid(void* p, int64_t* n3) {
 }
 
 int64_t xfd_sim_device_bytes(void* p) { return H(p)->sim->deviceBytes(); }
-int32_t xfd_sim_coef_classed(void* p) { return H(p)->sim->classedCoefficients() ? 1 : 0; }
+int32_t xfd_sim_coef_classed(void* p) { return H(p)->sim->coefficientLayout(); }
 int32_t xfd_sim_cpml_fused(void* p) { return H(p)->sim->fusedCpml(); }
 
 int32_t xfd_sim_nf2ff_count(void* p) { return static_cast<int32_t>(H(p)->nf2ffs.size()); }

@@ -612,7 +612,7 @@ struct Simulation::Impl {
   uint32_t profiling_classes = ~0u;
   std::vector<KernelStat> stats;
   long long dev_bytes = 0;
-  bool coef_classed = false;
+  int coef_layout = 0;  // 0 dense, 1 16-bit class ids, 2 32-bit class ids
   int cpml_fused = 0;
 
   ~Impl() { releaseDevice(); }
@@ -1860,7 +1860,7 @@ struct Simulation::Impl {
     dev_bytes = b;
     int32_t cl = 0;
     xfd_coef_layout(ctx, &cl, nullptr);
-    coef_classed = cl != 0;
+    coef_layout = cl;
     int32_t fu = 0;
     xfd_cpml_fused(ctx, &fu);
     cpml_fused = fu;
@@ -2198,7 +2198,8 @@ void Simulation::setProfilingClasses(unsigned mask) {
 std::vector<Simulation::KernelStat> Simulation::kernelStats() const { return _impl->stats; }
 long long Simulation::deviceBytes() const { return _impl->dev_bytes; }
 void* Simulation::deviceContext() const { return _impl->ctx; }
-bool Simulation::classedCoefficients() const { return _impl->coef_classed; }
+bool Simulation::classedCoefficients() const { return _impl->coef_layout != 0; }
+int Simulation::coefficientLayout() const { return _impl->coef_layout; }
 int Simulation::fusedCpml() const { return _impl->cpml_fused; }
 
 void Simulation::setDesignRegion(const std::array<Index, 3>& origin, const std::array<Index, 3>& size,

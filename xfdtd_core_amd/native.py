@@ -397,7 +397,12 @@ class Simulation:
 
     def coef_classed(self) -> bool:
         """True when the device stores coefficient class ids, not 18 arrays."""
-        return bool(self._L.xfd_sim_coef_classed(self._h))
+        return self.coef_layout() != 0
+
+    def coef_layout(self) -> int:
+        """0: the 18 dense arrays; 1: class ids; 2: 32-bit class ids, when a
+        component has more than 65535 distinct triples (xfd_coef_layout)."""
+        return int(self._L.xfd_sim_coef_classed(self._h))
 
     # ------------------------------------------------------- time-domain NF2FF --
     def nf2ff_td_potential(self, n: int, d: int, which: int) -> np.ndarray:
