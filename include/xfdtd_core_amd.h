@@ -263,10 +263,10 @@ int xfd_nf2ff_far_field(xfd_ctx* ctx, int32_t n, int32_t f, int32_t na, const do
 /* xfd_nf2ff_far_field without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
- * The staged grid arrays, directions and the kernel's output live in the
- * library's file-scope device array of 4 MiB (one per GPU, held until the
- * process ends); a call whose grid and `na` need more (about 48 000
- * directions on a 384^3 grid) fails with a message and does nothing.  Same
+ * The staged grid arrays, directions, the kernels' partial sums and output
+ * live in the library's file-scope device array of 4 MiB (one per GPU, held
+ * until the process ends); a call whose grid and `na` need more (about
+ * 29 000 directions on a 384^3 grid) fails with a message and does nothing.  Same
  * arguments, kernel and results (bitwise) as xfd_nf2ff_far_field.  Every
  * call on one GPU shares that array, so calls that overlap on one GPU (from
  * several host threads or streams) race; the context's device must be
