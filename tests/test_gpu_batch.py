@@ -186,18 +186,49 @@ def test_batch_tune_resets_state():
     _compare(got, want, True, "tuned batch")
 
 
-def test_batch_with_dispersive_case_steps_one_by_one():
-    """A case with an ADE material is outside the batched launches: the batch
-    falls back to stepping its contexts in turn, with the same results."""
+@pytest.mark.parametrize("tile", [0, 2])
+def test_batch_with_dispersive_cases_fp64(monkeypatch, tile):
+    """Cases with ADE materials share the batched launches: their E sweep is
+    one launch over the dispersive cases (k_batch_sweep_ade, each case's own
+    scheme: Drude only, Debye only, and the modified-Lorentz scheme a Drude +
+    Lorentz mix takes), beside the plain cases' E sweep.  Bitwise the single
+    contexts' FP64 results, with a HIP graph, for two sweep tiles."""
+    monkeypatch.setenv("XFD_BATCH_TILE", str(tile))
     specs = _cases(n=2, seed=9)
-    disp = scenarios.dispersive_slabs(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9, lumped=False,
-                                      kinds=("drude", "lorentz"), nf2ff_distance=8, nf2ff_freqs=[1e10, 2e10])
-    specs = specs + [disp]
-    say("[gpu] batch with dispersive case")
+    kw = dict(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9, lumped=False, nf2ff_distance=8,
+              nf2ff_freqs=[1e10, 2e10])
+    disp = [scenarios.dispersive_slabs(kinds=("drude", "lorentz"), **kw),
+            scenarios.dispersive_slabs(kinds=("drude",), drude_wp=2e11, **kw),
+            scenarios.dispersive_slabs(kinds=("debye",), **kw)]
+    specs = specs + disp
+    say(f"[gpu] batch with dispersive cases, tile {tile}")
     want = _alone(specs, "f64", steps=60)
-    sims, b, got, info = _batched(specs, "f64", steps=60)
-    assert not info["shared"]
-    _compare(got, want, True, "fallback batch")
+    sims, b, got, info = _batched(specs, "f64", steps=60, graph=True)
+    assert info["shared"]
+    _compare(got, want, True, "batch with dispersive cases")
+
+
+def test_batch_with_32bit_class_ids_steps_one_by_one():
+    """A case whose coefficients need 32-bit class ids is outside the batched
+    launches (they read 16-bit ids): the batch steps its contexts in turn,
+    with the same results."""
+    s, origin, eps = scenarios.adjoint_design(n_total=64, pml=6, design=40, patch_gap=2, tfsf_margin=3, seed=6)
+    specs = _cases(n=1, seed=10) + [s]
+    say("[gpu] batch with a 32-bit class-id case")
+    want = []
+    for sp in specs:
+        one = native.Simulation(sp.spec(), "f64")
+        one.run(40)
+        want.append({f: one.field(f) for f in FIELDS})
+        one.close()
+    sims = [native.Simulation(sp.spec(), "f64") for sp in specs]
+    b = native.SimulationBatch(sims)
+    b.init(40)
+    b.advance(0, 40)
+    b.finish()
+    assert sims[1].coef_layout() == 2
+    assert not b.info()["shared"]
+    _compare([{f: x.field(f) for f in FIELDS} for x in sims], want, True, "fallback batch")
 
 
 @pytest.mark.parametrize("tile", [1, 2, 3, 4])

@@ -41,6 +41,8 @@ struct BatchCase {
   ClassCoef<T> K;
   CpmlList<T> ph, pe;  // slabs applied inside the sweeps (empty: separate CPML launches)
   int nbx, nby, nbz, ichunk;
+  AdeArgs<T> A;        // dispersive cases (k_batch_sweep_ade): state and material tables
+  int scheme;          // AdeScheme, or -1
 };
 
 template <class T>
@@ -88,6 +90,28 @@ __global__ __launch_bounds__(TK * TJ) void k_batch_sweep(const BatchCase<T>* __r
       sweepH<T, ClassCoef<T>, Fused, false, false, TK, TJ>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.ph);
     else
       sweepE<T, ClassCoef<T>, Fused, false, false, TK, TJ>(C.g, C.F, C.K, bx, by, bz, C.ichunk, C.pe);
+  }
+}
+
+// The E sweep of the dispersive cases (their H sweep is k_batch_sweep's):
+// sweepEAde with the case's scheme.
+template <class T, int TK, int TJ>
+__global__ __launch_bounds__(TK * TJ) void k_batch_sweep_ade(const BatchCase<T>* __restrict__ cases,
+                                                         const int64_t* __restrict__ start, int n, int64_t total) {
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int e = findEntry(start, n, b);
+    const BatchCase<T>& C = cases[e];
+    const int64_t local = b - start[e];
+    const int bx = static_cast<int>(local % C.nbx);
+    const int64_t r = local / C.nbx;
+    const int by = static_cast<int>(r % C.nby);
+    const int bz = static_cast<int>(r / C.nby);
+    if (C.scheme == kAdeDrude)
+      sweepEAde<T, ClassCoef<T>, kAdeDrude, TK, TJ>(C.g, C.F, C.K, C.A, bx, by, bz, C.ichunk);
+    else if (C.scheme == kAdeDebye)
+      sweepEAde<T, ClassCoef<T>, kAdeDebye, TK, TJ>(C.g, C.F, C.K, C.A, bx, by, bz, C.ichunk);
+    else
+      sweepEAde<T, ClassCoef<T>, kAdeMLorentz, TK, TJ>(C.g, C.F, C.K, C.A, bx, by, bz, C.ichunk);
   }
 }
 
@@ -191,6 +215,9 @@ struct BatchLaunch {
   // tile: index into kBatchTileK / kBatchTileJ
   static void sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
                     int grid, bool fused, int tile = 0);
+  // the dispersive cases' E sweep (k_batch_sweep_ade)
+  static void sweepAde(hipStream_t s, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total, int grid,
+                       int tile = 0);
   static void cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n, int64_t total,
                    int grid);
   static void cpmlFaces(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,

@@ -456,6 +456,24 @@ void leaveStep(xfd_ctx& c) {
     for (auto& b : sl->psi_h2) b.reset();
 }
 
+// The dispersive update's arguments (zero when the context has no ADE).
+template <class T>
+xfd::AdeArgs<T> adeArgs(xfd_ctx& c) {
+  xfd::AdeArgs<T> A{};
+  if (c.ade_scheme < 0) return A;
+  A.poles = c.ade_poles;
+  A.stride = 2 + 7 * c.ade_poles;
+  A.total = c.g.total;
+  A.mat = c.ade_mat.as<int16_t>();
+  A.tab = c.ade_tab.as<T>();
+  for (int q = 0; q < 3; ++q) {
+    A.j[q] = c.ade_j[q].as<T>();
+    A.jp[q] = c.ade_jp[q].as<T>();
+    A.eprev[q] = c.ade_eprev[q].as<T>();
+  }
+  return A;
+}
+
 template <class T>
 xfd::WideCoef<T> wideCoef(xfd_ctx& c) {
   xfd::WideCoef<T> M{};
@@ -1204,7 +1222,7 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
       edge = edge || i == 0 || i >= g.nx || j == 0 || j >= g.ny || k == 0 || k >= g.nz;
     }
     c.fstep = !off && c.dim == 3 && c.ade_scheme < 0 && c.nseg_h == 0 && !records_meet_slabs && !edge &&
-              c.slabs.size() <= static_cast<size_t>(xfd::kMaxStepSlabs);
+              !c.wide && c.slabs.size() <= static_cast<size_t>(xfd::kMaxStepSlabs);
     if (c.fstep) {
       c.fused = false;
       c.n_fused = 0;
@@ -1389,19 +1407,7 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
     K.tab[q] = c.tabs[q].as<T>();
   }
   const auto M = wideCoef<T>(c);
-  xfd::AdeArgs<T> A{};
-  if (c.ade_scheme >= 0) {
-    A.poles = c.ade_poles;
-    A.stride = 2 + 7 * c.ade_poles;
-    A.total = c.g.total;
-    A.mat = c.ade_mat.as<int16_t>();
-    A.tab = c.ade_tab.as<T>();
-    for (int q = 0; q < 3; ++q) {
-      A.j[q] = c.ade_j[q].as<T>();
-      A.jp[q] = c.ade_jp[q].as<T>();
-      A.eprev[q] = c.ade_eprev[q].as<T>();
-    }
-  }
+  const xfd::AdeArgs<T> A = adeArgs<T>(c);
   std::vector<xfd::SlabArgs<T>> se, sh;
   for (auto& s : c.slabs) {
     se.push_back(slabArgs<T>(*s, true));
@@ -1870,7 +1876,7 @@ void backwardTyped(xfd_ctx& c, const xfd_adjoint_desc& D, double* grad_out) {
 struct xfd_batch {
   int dtype = XFD_F64;
   std::vector<std::unique_ptr<xfd_ctx>> cases;
-  bool shared = true;  // every case class-indexed and free of ADE: batched launches
+  bool shared = true;  // every case class-indexed (16-bit ids): batched launches
 
   hipStream_t stream = nullptr;
   DevBuf tdev;
@@ -1887,7 +1893,11 @@ struct xfd_batch {
     int n = 0;
     int64_t total = 0;
   };
-  Table sweep, rec_e, rec_h, nf;
+  // sweep: every case's sweep (the H sweeps; the E sweeps too when no case
+  // is dispersive); sweep_e / sweep_ade: the E sweeps of the other and of the
+  // dispersive cases when some case is
+  Table sweep, sweep_e, sweep_ade, rec_e, rec_h, nf;
+  bool any_ade = false;
   std::vector<Table> cpml_e, cpml_h;            // slabs stepped by separate launches
   std::vector<Table> faces_e, faces_h;          // far planes of slabs fused into the sweeps
   bool any_fused = false;  // some case applies leading slabs inside its sweeps
@@ -1948,7 +1958,8 @@ int64_t groupsOf(int64_t items) { return (items + xfd::kBatchGroupItems - 1) / x
 template <class T>
 void batchTables(xfd_batch& b) {
   b.any_fused = false;
-  Collect<xfd::BatchCase<T>> sw;
+  Collect<xfd::BatchCase<T>> sw, swe, swa;
+  b.any_ade = false;
   std::vector<Collect<xfd::BatchSlab<T>>> ce, ch, fe, fh;
   Collect<xfd::BatchRec<T>> re, rh;
   Collect<xfd::BatchNf2ff<T>> nf;
@@ -1987,7 +1998,14 @@ void batchTables(xfd_batch& b) {
     bc.nbx = (c.g.nz + tk - 1) / tk;
     bc.nby = (c.g.ny + tj - 1) / tj;
     bc.nbz = (c.g.nx + b.ichunk - 1) / b.ichunk;
-    sw.add(bc, static_cast<int64_t>(bc.nbx) * bc.nby * bc.nbz);
+    bc.scheme = c.ade_scheme;
+    if (c.ade_scheme >= 0) {
+      b.any_ade = true;
+      bc.A = adeArgs<T>(c);
+    }
+    const int64_t tiles = static_cast<int64_t>(bc.nbx) * bc.nby * bc.nbz;
+    sw.add(bc, tiles);
+    (c.ade_scheme >= 0 ? swa : swe).add(bc, tiles);
     const int64_t stride[3] = {c.g.plane, c.g.pz, 1};
     for (size_t o = 0; o < c.slabs.size(); ++o) {
       if (ce.size() <= o) {
@@ -2018,6 +2036,8 @@ void batchTables(xfd_batch& b) {
     b.pool_rows = std::min(b.pool_rows, c.pool_rows);
   }
   sw.to(b.sweep);
+  swe.to(b.sweep_e);
+  swa.to(b.sweep_ade);
   re.to(b.rec_e);
   rh.to(b.rec_h);
   nf.to(b.nf);
@@ -2057,8 +2077,15 @@ void batchEnqueueStep(xfd_batch& b) {
   hipStream_t st = b.stream;
   const int64_t* t = b.tdev.as<int64_t>();
   auto sweep = [&](bool is_h) {
-    L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n, b.sweep.total,
-             b.grid, b.any_fused, b.tile);
+    if (is_h || !b.any_ade) {
+      L::sweep(st, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
+               b.sweep.total, b.grid, b.any_fused, b.tile);
+      return;
+    }
+    L::sweep(st, false, b.sweep_e.tab.as<xfd::BatchCase<T>>(), b.sweep_e.start.as<int64_t>(), b.sweep_e.n,
+             b.sweep_e.total, b.grid, b.any_fused, b.tile);
+    L::sweepAde(st, b.sweep_ade.tab.as<xfd::BatchCase<T>>(), b.sweep_ade.start.as<int64_t>(), b.sweep_ade.n,
+                b.sweep_ade.total, b.grid, b.tile);
   };
   auto records = [&](xfd_batch::Table& r) {
     if (r.n) L::records(st, r.tab.as<xfd::BatchRec<T>>(), r.start.as<int64_t>(), r.n, r.total, b.grid, t);
@@ -2135,8 +2162,15 @@ void batchTune(xfd_batch& b, int reps) {
   auto timePair = [&](int n) {
     auto pair = [&] {
       for (bool is_h : {true, false}) {
-        L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
-                 b.sweep.total, b.grid, b.any_fused, b.tile);
+        if (is_h || !b.any_ade) {
+          L::sweep(b.stream, is_h, b.sweep.tab.as<xfd::BatchCase<T>>(), b.sweep.start.as<int64_t>(), b.sweep.n,
+                   b.sweep.total, b.grid, b.any_fused, b.tile);
+        } else {
+          L::sweep(b.stream, false, b.sweep_e.tab.as<xfd::BatchCase<T>>(), b.sweep_e.start.as<int64_t>(),
+                   b.sweep_e.n, b.sweep_e.total, b.grid, b.any_fused, b.tile);
+          L::sweepAde(b.stream, b.sweep_ade.tab.as<xfd::BatchCase<T>>(), b.sweep_ade.start.as<int64_t>(),
+                      b.sweep_ade.n, b.sweep_ade.total, b.grid, b.tile);
+        }
         auto& full = is_h ? b.cpml_h : b.cpml_e;
         auto& faces = is_h ? b.faces_h : b.faces_e;
         for (size_t o = 0; o < full.size(); ++o) {
@@ -2580,7 +2614,7 @@ int xfd_batch_create(int32_t n, const xfd_desc* descs, xfd_batch** out) {
       // measured slower with them inside
       c->zfused = false;
       std::fill(c->zslab.begin(), c->zslab.end(), 0);
-      b->shared = b->shared && c->classed && c->ade_scheme < 0 && c->tdnfs.empty();
+      b->shared = b->shared && c->classed && !c->wide && c->tdnfs.empty();
 
     }
     XFD_HIP(hipStreamCreate(&b->stream));

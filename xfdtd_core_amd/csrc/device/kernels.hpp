@@ -717,11 +717,11 @@ __device__ __forceinline__ void adeCell(const AdeArgs<T>& A, int m, int64_t idx,
 // plain update).  As in sweepE the curl terms' loads are unconditional (the
 // k-1 / j-1 neighbours read at the point itself on the k = 0 / j = 0 faces)
 // and the Yee ranges select, so a plane's loads are in flight together.
-template <class T, class Coef, int S>
+template <class T, class Coef, int S, int TK = kLanesK, int TJ = kRowsJ>
 __device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
                                           int bx, int by, int bz, int ichunk) {
-  const int k = bx * kLanesK + threadIdx.x;
-  const int j = by * kRowsJ + threadIdx.y;
+  const int k = bx * TK + threadIdx.x;
+  const int j = by * TJ + threadIdx.y;
   const int i0 = bz * ichunk;
   if (k >= g.nz || j >= g.ny || i0 >= g.nx) return;
   const int i1 = min(i0 + ichunk, g.nx);

@@ -266,6 +266,24 @@ void BatchLaunch<T>::sweep(hipStream_t s, bool is_h, const BatchCase<T>* cases, 
     default: batchSweepTile<T, kBatchTileK[0], kBatchTileJ[0]>(s, is_h, cases, start, n, total, grid, fused); break;
   }
 }
+template <class T, int TK, int TJ>
+void batchSweepAdeTile(hipStream_t s, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
+                       int grid) {
+  hipLaunchKernelGGL((k_batch_sweep_ade<T, TK, TJ>), dim3(static_cast<unsigned>(std::min<int64_t>(total, grid))),
+                     dim3(TK, TJ), 0, s, cases, start, n, total);
+}
+template <class T>
+void BatchLaunch<T>::sweepAde(hipStream_t s, const BatchCase<T>* cases, const int64_t* start, int n, int64_t total,
+                              int grid, int tile) {
+  if (total <= 0) return;
+  switch (tile) {
+    case 1: batchSweepAdeTile<T, kBatchTileK[1], kBatchTileJ[1]>(s, cases, start, n, total, grid); break;
+    case 2: batchSweepAdeTile<T, kBatchTileK[2], kBatchTileJ[2]>(s, cases, start, n, total, grid); break;
+    case 3: batchSweepAdeTile<T, kBatchTileK[3], kBatchTileJ[3]>(s, cases, start, n, total, grid); break;
+    case 4: batchSweepAdeTile<T, kBatchTileK[4], kBatchTileJ[4]>(s, cases, start, n, total, grid); break;
+    default: batchSweepAdeTile<T, kBatchTileK[0], kBatchTileJ[0]>(s, cases, start, n, total, grid); break;
+  }
+}
 template <class T>
 void BatchLaunch<T>::cpml(hipStream_t s, bool is_e, const BatchSlab<T>* slabs, const int64_t* start, int n,
                           int64_t total, int grid) {
