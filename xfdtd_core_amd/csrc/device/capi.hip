@@ -311,6 +311,7 @@ struct xfd_ctx {
   // of each half-step as device arrays
   // grid dimension: 3, 2 (TE, nz = 1) or 1 (TEM, nx = ny = 1)
   int dim = 3;
+  bool te_zero = false;  // 2D: Ex, Ey and Hz stay zero (k_te_h<..., true>)
   int device = 0;  // the HIP device the context lives on
   // The single-pass step (k_step): a second set of the six fields and of
   // the H-side CPML psi; `parity` says which set holds the current state
@@ -865,6 +866,12 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
   c.nseg_e = re.empty() ? 0 : static_cast<int64_t>(se.size()) - 1;
   c.nseg_h = rh.empty() ? 0 : static_cast<int64_t>(sh.size()) - 1;
   const bool records_meet_slabs = recordsMeetSlabs(c, re, true) || recordsMeetSlabs(c, rh, false);
+  // A 2D grid whose Ex, Ey and Hz no correction drives keeps them at zero
+  // (their CPML psi are driven by each other only), so its H half-step can
+  // leave out Hz and the Ex / Ey terms (adding a zero changes no value).
+  c.te_zero = c.dim == 2;
+  for (const auto& r : re) c.te_zero = c.te_zero && r.field != 0 && r.field != 1;
+  for (const auto& r : rh) c.te_zero = c.te_zero && r.field != 5;
 
   require(!d.ade || c.dim == 3, "ADE materials need a 3D grid");
   require((d.n_nf2ff == 0 && d.n_tdnf == 0) || c.dim == 3, "NF2FF needs a 3D grid");
@@ -1460,9 +1467,9 @@ void stepTyped(xfd_ctx& c, int64_t t0, int64_t n, bool record = true) {
     c.timed(K_UPDATE_H, [&] {
       if (c.dim != 3) {
         if (c.classed)
-          xfd::Launch<T>::update_low(st, c.g, F, K, c.dim, true);
+          xfd::Launch<T>::update_low(st, c.g, F, K, c.dim, true, c.te_zero);
         else
-          xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, true);
+          xfd::Launch<T>::update_low(st, c.g, F, C, c.dim, true, c.te_zero);
       } else if (c.wide) {
         xfd::Launch<T>::update_h(st, c.g, F, M, c.ichunk, PH, ZH, AH);
       } else if (c.classed) {
@@ -2341,6 +2348,7 @@ int xfd_read_field(xfd_ctx* c, int32_t field, double* dst) {
 int xfd_write_field(xfd_ctx* c, int32_t field, const double* src) {
   return guard([&] {
     require(c && src && field >= 0 && field < 6, "bad argument");
+    if (field == 0 || field == 1 || field == 5) c->te_zero = false;  // no longer known to stay zero
     int ext[3];
     extentOf(field, c->g.nx, c->g.ny, c->g.nz, ext);
     DevBuf tmp;

@@ -377,7 +377,10 @@ __global__ __launch_bounds__(kTileThreads) void k_update_e(Grid g, FieldPtrs<T> 
 // BasicUpdator::updateH (src/updator/basic_updator.cpp:23-41) over
 // [0,nx) x [0,ny) x {0}: all three components with every term, as the
 // reference evaluates them (the Ex, Ey and Hz terms are zeros of a TE grid).
-template <class T, class Coef>
+// TeZero: Ex, Ey and Hz are known to stay zero (no correction drives them),
+// so the Ex / Ey terms are zeros whose addition changes no value and Hz keeps
+// its zero: the half-step reads and writes Hx, Hy and Ez only.
+template <class T, class Coef, bool TeZero = false>
 __global__ __launch_bounds__(256) void k_te_h(Grid g, FieldPtrs<T> F, Coef C) {
   const int j = blockIdx.x * 64 + threadIdx.x;
   const int i = blockIdx.y * 4 + threadIdx.y;
@@ -388,6 +391,16 @@ __global__ __launch_bounds__(256) void k_te_h(Grid g, FieldPtrs<T> F, Coef C) {
   const T* __restrict__ ey = F.f[1];
   const T* __restrict__ ez = F.f[2];
   T s, a, b;
+  if (TeZero) {
+    C.get(3, idx, s, a, b);
+    const T vx = s * F.f[3][idx] + b * (ez[idx + pz] - ez[idx]);
+    T s4, a4, b4;
+    C.get(4, idx, s4, a4, b4);
+    const T vy = s4 * F.f[4][idx] + a4 * (ez[idx + pl] - ez[idx]);
+    F.f[3][idx] = vx;
+    F.f[4][idx] = vy;
+    return;
+  }
   C.get(3, idx, s, a, b);
   const T vx = s * F.f[3][idx] + a * (ey[idx + 1] - ey[idx]) + b * (ez[idx + pz] - ez[idx]);
   C.get(4, idx, s, a, b);
@@ -1250,10 +1263,12 @@ struct Launch {
                        const StepSlabs<T>& S = StepSlabs<T>{});
   // the Yee half-steps of a 2D (TE, dim 2) or 1D (TEM, dim 1) grid; the 1D
   // H half-step is the 3D sweep (update_h)
+  // te_zero (2D): Ex, Ey and Hz are known to stay zero, so the H half-step
+  // leaves out Hz and the Ex / Ey terms
   static void update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int dim,
-                         bool is_h);
+                         bool is_h, bool te_zero = false);
   static void update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int dim,
-                         bool is_h);
+                         bool is_h, bool te_zero = false);
   static void cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e);
   static void cpml_zrest(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a,
                          const ZFrame& fr, bool is_e);

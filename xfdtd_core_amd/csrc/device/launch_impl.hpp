@@ -86,10 +86,13 @@ void Launch<T>::update_e(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, co
   launchUpdate<T>(false, s, g, F, C, ichunk, P, Z, S);
 }
 template <class T, class Coef>
-void launchLow(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int dim, bool is_h) {
+void launchLow(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int dim, bool is_h,
+               bool te_zero) {
   if (dim == 2) {
     const dim3 grid(blocks_for(g.ny, 64), blocks_for(g.nx, 4)), block(64, 4);
-    if (is_h)
+    if (is_h && te_zero)
+      hipLaunchKernelGGL((k_te_h<T, Coef, true>), grid, block, 0, s, g, F, C);
+    else if (is_h)
       hipLaunchKernelGGL((k_te_h<T, Coef>), grid, block, 0, s, g, F, C);
     else
       hipLaunchKernelGGL((k_te_e<T, Coef>), grid, block, 0, s, g, F, C);
@@ -101,13 +104,13 @@ void launchLow(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& 
 }
 template <class T>
 void Launch<T>::update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C, int dim,
-                           bool is_h) {
-  launchLow<T>(s, g, F, C, dim, is_h);
+                           bool is_h, bool te_zero) {
+  launchLow<T>(s, g, F, C, dim, is_h, te_zero);
 }
 template <class T>
 void Launch<T>::update_low(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const ClassCoef<T>& C, int dim,
-                           bool is_h) {
-  launchLow<T>(s, g, F, C, dim, is_h);
+                           bool is_h, bool te_zero) {
+  launchLow<T>(s, g, F, C, dim, is_h, te_zero);
 }
 template <class T>
 void Launch<T>::cpml_faces(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const SlabArgs<T>& a, bool is_e) {
