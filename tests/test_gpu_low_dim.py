@@ -122,6 +122,31 @@ def test_te_cavity_eigenmode_through_capi():
     np.testing.assert_allclose(ctx.read("EZ"), g * e0, rtol=0, atol=2e-12)
 
 
+def test_te_grid_with_nonzero_ex_ey_hz_through_capi():
+    """A 2D grid whose Ex, Ey and Hz are written nonzero: the H half-step
+    must then evaluate every term (the reference's updateH on all three
+    components) rather than the TE shortcut that assumes them zero; bitwise
+    equal to the oracle."""
+    n, dl, steps = (30, 24), (1e-3, 1.2e-3), 120
+    s = S.Simulation(dl[0], dl[1], 1.0, 0.8)
+    s.addObject(S.Object("cavity", S.Cube(S.Vector(0, 0, -INF), S.Vector(n[0] * dl[0], n[1] * dl[1], INF)),
+                         S.Material.createAir()))
+    ref = RefSim(s.spec())
+    ref.init(steps)
+    coeffs = {c: ref.coeff(c) for c in native.COEFF_NAMES}
+    ctx = native.Context(coeffs, "f64", dim=2)
+    rng = np.random.default_rng(3)
+    for f in ("EZ", "EX", "EY", "HZ"):
+        v = rng.standard_normal(ctx.read(f).shape)
+        ctx.write(f, v)
+        ref.write_field(f, v)
+    ctx.step(0, steps)
+    ref.loop()
+    for f in FIELDS:
+        np.testing.assert_array_equal(ctx.read(f), ref.field(f), err_msg=f)
+    assert np.abs(ctx.read("HZ")).max() > 0
+
+
 def test_tem_standing_wave_through_capi():
     """xfd_create with dim = 1: an Ex standing wave on the line evolves by
     sin(theta/2) = c dt sin(p/2) / dz, bitwise equal to the oracle."""
