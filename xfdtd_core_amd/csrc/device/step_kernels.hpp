@@ -127,12 +127,16 @@ __global__ __launch_bounds__(kTileThreads) void k_step(Grid g, FieldPtrs<T> A, F
       hy_km = vy;
       hx_km = vx;
     }
+    // Hz and Hx at j - 1, computed unconditionally (at the point itself on
+    // the j = 0 face, where the result is not used) so that their loads are
+    // not held behind a branch
     T hz_jm = T(0), hx_jm = T(0);
-    if (j >= 1) {  // Hz and Hx at j - 1
-      const int64_t p = idx - pz;
+    {
+      const int64_t p = j >= 1 ? idx - pz : idx;
+      const uint32_t mj = j >= 1 ? hi & hjm : 0u;
       T en, zn, vy;
-      stepH<T, Coef, false, false>(g, A, C, SH, hi & hjm & hku, hi & hjm & hk, i, j - 1, k, p, A.f[0][p], A.f[1][p],
-                                   A.f[2][p], en, zn, hx_jm, vy, hz_jm);
+      stepH<T, Coef, false, false>(g, A, C, SH, mj & hku, mj & hk, i, j - 1, k, p, A.f[0][p], A.f[1][p], A.f[2][p],
+                                   en, zn, hx_jm, vy, hz_jm);
     }
     stOwn(B.f[3] + idx, hx);
     stOwn(B.f[4] + idx, hy);
@@ -146,18 +150,17 @@ __global__ __launch_bounds__(kTileThreads) void k_step(Grid g, FieldPtrs<T> A, F
     const T dyHx = j >= 1 ? hx - hx_jm : T(0);
     const T dxHz = hz - hz_p, dxHy = hy - hy_p;
     T vx = ex0, vy = ey_i, vz = ez_i;
-    T s, a, b;
-    if (j >= 1 && k >= 1) {
-      C.getOwn(0, idx, s, a, b);
-      vx = s * vx + a * dzHy + b * dyHz;
-    }
-    if (i >= 1 && k >= 1) {
-      C.getOwn(1, idx, s, a, b);
-      vy = s * vy + a * dxHz + b * dzHx;
-    }
-    if (i >= 1 && j >= 1) {
-      C.getOwn(2, idx, s, a, b);
-      vz = s * vz + a * dyHx + b * dxHy;
+    {  // the Yee ranges select (the coefficient loads unconditional)
+      T s0, a0, b0, s1, a1, b1, s2, a2, b2;
+      C.getOwn(0, idx, s0, a0, b0);
+      C.getOwn(1, idx, s1, a1, b1);
+      C.getOwn(2, idx, s2, a2, b2);
+      const T ux = s0 * vx + a0 * dzHy + b0 * dyHz;
+      const T uy = s1 * vy + a1 * dxHz + b1 * dzHx;
+      const T uz = s2 * vz + a2 * dyHx + b2 * dxHy;
+      vx = (j >= 1 && k >= 1) ? ux : vx;
+      vy = (i >= 1 && k >= 1) ? uy : vy;
+      vz = (i >= 1 && j >= 1) ? uz : vz;
     }
     // A: Ey(x) <- Hz, Ez(y) <- Hx, Ex(z) <- Hy;  B: Ez(x) <- Hy, Ex(y) <- Hz, Ey(z) <- Hx
     const uint32_t ei = axisMask(SE, 0, i) & ej;
