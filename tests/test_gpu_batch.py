@@ -208,6 +208,19 @@ def test_batch_with_dispersive_cases_fp64(monkeypatch, tile):
     _compare(got, want, True, "batch with dispersive cases")
 
 
+def test_batch_tuner_with_dispersive_cases_fp64():
+    """The autotuner times the dispersive cases' E sweep launch too, then
+    zeroes every case: the tuned batch still equals the single contexts."""
+    kw = dict(n_total=40, pml=6, slab=6, gap=2, tfsf_offset=9, lumped=False, nf2ff_distance=8,
+              nf2ff_freqs=[1e10, 2e10])
+    specs = _cases(n=2, seed=13) + [scenarios.dispersive_slabs(kinds=("drude", "lorentz"), **kw)]
+    say("[gpu] tuned batch with a dispersive case")
+    want = _alone(specs, "f64", steps=40)
+    sims, b, got, info = _batched(specs, "f64", steps=40, graph=True, tune=1)
+    assert info["shared"]
+    _compare(got, want, True, "tuned batch with a dispersive case")
+
+
 def test_batch_with_32bit_class_ids_steps_one_by_one():
     """A case whose coefficients need 32-bit class ids is outside the batched
     launches (they read 16-bit ids): the batch steps its contexts in turn,
