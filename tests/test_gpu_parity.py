@@ -355,7 +355,7 @@ def _tall_box(nz_total, pml=8, n_xy=24):
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_z_cpml_inside_sweeps(monkeypatch, precision):
     """The sweeps apply the z CPML slabs: FP32 by default when a k-row spans
-    at least 4 sweep tiles, the whole slab; FP64 only when forced, and only
+    at least 3 sweep tiles, the whole slab; FP64 only when forced, and only
     where no x/y slab corrects the point (the frame follows the x/y slabs in
     a launch of its own, keeping the reference's order).  Forced on a 48^3
     box, where one wavefront holds lanes of both z slabs: FP64 equals the
@@ -367,7 +367,7 @@ def test_z_cpml_inside_sweeps(monkeypatch, precision):
     monkeypatch.setenv("XFD_CPML_INSWEEP", "0")
     monkeypatch.delenv("XFD_CPML_FUSED", raising=False)
     monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
-    for nz, want in ((192, 0), (193, 2 if precision == "f32" else 0)):  # 3 and 4 tiles of 64 k-lanes
+    for nz, want in ((128, 0), (129, 2 if precision == "f32" else 0)):  # 2 and 3 tiles of 64 k-lanes
         tall = native.Simulation(_tall_box(nz), precision)
         tall.init(2)
         assert tall.cpml_fused() == want, (nz, tall.cpml_fused())

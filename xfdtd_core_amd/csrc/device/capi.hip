@@ -1181,8 +1181,9 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     // 1.244 ms without, while the batched design sweep (nz 48-160, 1-3
     // k-tiles) steps 3.66 ms with it and 3.41 ms without.  FP32 applies the
     // whole slab in the sweep (its corrections then precede the x/y slabs'
-    // where they overlap: FP32 rounding only); default on rows of at least 4
-    // k-tiles.  FP64 keeps the reference's order of corrections: the sweeps
+    // where they overlap: FP32 rounding only); default on rows of at least 3
+    // k-tiles (FP32 ms per step, separate / inside: n = 128 0.109 / 0.125,
+    // 160 0.163 / 0.149, 192 0.201 / 0.198, 384 ~1.18 / 1.12).  FP64 keeps the reference's order of corrections: the sweeps
     // apply a z slab only where no x/y slab corrects the point, and the frame
     // where they overlap follows the x/y slabs at the z slab's place
     // (k_cpml_zrest), bitwise; the frame's launch costs more than the fusion
@@ -1192,7 +1193,7 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     const char* zenv = std::getenv("XFD_CPML_ZFUSED");
     const bool zoff = zenv && zenv[0] == '0';
     const bool zforce = zenv && zenv[0] == '1';
-    const bool zwide = (g.nz + xfd::kLanesK - 1) / xfd::kLanesK >= 4;
+    const bool zwide = (g.nz + xfd::kLanesK - 1) / xfd::kLanesK >= 3;
     std::vector<int> zq;
     for (size_t q = 0; q < c.slabs.size(); ++q)
       if (c.slabs[q]->axis == 2) zq.push_back(static_cast<int>(q));
