@@ -1183,7 +1183,10 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     // whole slab in the sweep (its corrections then precede the x/y slabs'
     // where they overlap: FP32 rounding only); default on rows of at least 3
     // k-tiles (FP32 ms per step, separate / inside: n = 128 0.109 / 0.125,
-    // 160 0.163 / 0.149, 192 0.201 / 0.198, 384 ~1.18 / 1.12).  FP64 keeps the reference's order of corrections: the sweeps
+    // 160 0.163 / 0.149, 192 0.201 / 0.198, 384 ~1.18 / 1.12) and 16-bit
+    // class ids or dense coefficients; with 32-bit class ids the sweeps lose
+    // more than the z launches cost (160^3 adjoint design run, 400 steps:
+    // sweeps 37.5 -> 43.5 us, 206.7 ms separate against 214.0 inside).  FP64 keeps the reference's order of corrections: the sweeps
     // apply a z slab only where no x/y slab corrects the point, and the frame
     // where they overlap follows the x/y slabs at the z slab's place
     // (k_cpml_zrest), bitwise; the frame's launch costs more than the fusion
@@ -1207,7 +1210,7 @@ void createTyped(xfd_ctx& c, const xfd_desc& d) {
     int per_axis[2] = {0, 0};
     for (auto& sl : c.slabs)
       if (sl->axis < 2) ++per_axis[sl->axis];
-    c.zfused = !zoff && (zforce || (zwide && c.elem == 4)) && off && !zq.empty() && zq.size() <= 2 && disjoint &&
+    c.zfused = !zoff && (zforce || (zwide && c.elem == 4 && !c.wide)) && off && !zq.empty() && zq.size() <= 2 && disjoint &&
                per_axis[0] <= 4 && per_axis[1] <= 4 && c.ade_scheme < 0 && !records_meet_slabs && c.dim == 3;
     c.zslab.assign(c.slabs.size(), 0);
     if (c.zfused)
