@@ -414,6 +414,46 @@ def test_z_cpml_inside_sweeps(monkeypatch, precision):
     assert disp.cpml_fused() == 0
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_sweep_tile_128_by_2(monkeypatch, precision):
+    """The plain and z-in-sweep Yee sweeps take 128 k-lanes x 2 j-rows where
+    nz is a multiple of 128 and 64 x 4 elsewhere (XFD_SWEEP_TILE forces
+    either).  Every point's arithmetic is the same under both tiles, so they
+    give the same bits: a 128^3 box (128 x 2 by default; FP64 also equal to
+    the oracle) and a 96^3 box forced to 128 x 2 (every row one full and one
+    half-idle wavefront), each against 64 x 4, with the z CPML slabs as
+    separate launches and inside the sweeps."""
+    steps = 60
+    monkeypatch.setenv("XFD_STEP_FUSED", "0")
+    monkeypatch.setenv("XFD_CPML_INSWEEP", "0")
+    monkeypatch.delenv("XFD_CPML_FUSED", raising=False)
+    for n, forced in ((128, None), (96, "128")):
+        sp = scenarios.inhomogeneous_box(n_total=n, pml=8, n_spheres=6, r_cells=(3, 9), seed=11).spec()
+        for zf in ("0", "1"):
+            monkeypatch.setenv("XFD_CPML_ZFUSED", zf)
+            say(f"[gpu] sweep tile 128 x 2 vs 64 x 4, {n}^3 {precision}, z slabs {'inside' if zf == '1' else 'apart'}")
+            if forced:
+                monkeypatch.setenv("XFD_SWEEP_TILE", forced)
+            else:
+                monkeypatch.delenv("XFD_SWEEP_TILE", raising=False)
+            a = native.Simulation(sp, precision)
+            a.run(steps)
+            assert a.cpml_fused() == (2 if zf == "1" else 0)
+            monkeypatch.setenv("XFD_SWEEP_TILE", "64")
+            b = native.Simulation(sp, precision)
+            b.run(steps)
+            for f in FIELDS:
+                np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f"{n} {zf} {f}")
+            assert np.abs(a.field("EZ")).max() > 0
+            if precision == "f64" and n == 128 and zf == "0":
+                ref = RefSim(sp)
+                ref.run(steps)
+                for f in FIELDS:
+                    np.testing.assert_array_equal(a.field(f), ref.field(f), err_msg=f"oracle {f}")
+            a.close()
+            b.close()
+
+
 def test_far_field_on_device_matches_host(monkeypatch):
     """The far-field potentials summed on the GPU (xfd_nf2ff_far_field) equal
     the host sums over the same currents, for a theta cut and a phi cut

@@ -253,12 +253,12 @@ __device__ __forceinline__ void sweepH(const Grid& g, const FieldPtrs<T>& F, con
   }
 }
 
-template <class T, class Coef, bool Fused, bool ZF, bool All>
-__global__ __launch_bounds__(kTileThreads) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
-                                                  CpmlList<T> P, ZCpml<T> Z, StepSlabs<T> S) {
+template <class T, class Coef, bool Fused, bool ZF, bool All, int TK = kLanesK, int TJ = kRowsJ>
+__global__ __launch_bounds__(TK * TJ) void k_update_h(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
+                                             CpmlList<T> P, ZCpml<T> Z, StepSlabs<T> S) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepH<T, Coef, Fused, ZF, All>(g, F, C, bx, by, bz, ichunk, P, Z, S);
+  sweepH<T, Coef, Fused, ZF, All, TK, TJ>(g, F, C, bx, by, bz, ichunk, P, Z, S);
 }
 
 // BasicUpdator3D::updateE (src/updator/basic_updator_3d.cpp:38-69):
@@ -362,12 +362,12 @@ __device__ __forceinline__ void sweepE(const Grid& g, const FieldPtrs<T>& F, con
   }
 }
 
-template <class T, class Coef, bool Fused, bool ZF, bool All>
-__global__ __launch_bounds__(kTileThreads) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
-                                                  CpmlList<T> P, ZCpml<T> Z, StepSlabs<T> S) {
+template <class T, class Coef, bool Fused, bool ZF, bool All, int TK = kLanesK, int TJ = kRowsJ>
+__global__ __launch_bounds__(TK * TJ) void k_update_e(Grid g, FieldPtrs<T> F, Coef C, SweepMap M, int ichunk,
+                                             CpmlList<T> P, ZCpml<T> Z, StepSlabs<T> S) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepE<T, Coef, Fused, ZF, All>(g, F, C, bx, by, bz, ichunk, P, Z, S);
+  sweepE<T, Coef, Fused, ZF, All, TK, TJ>(g, F, C, bx, by, bz, ichunk, P, Z, S);
 }
 
 // 2D (TE) grid, nz = 1, in a layout of pitch pz = 2 along k (k = 1 holds

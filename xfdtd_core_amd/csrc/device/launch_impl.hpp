@@ -18,41 +18,54 @@ inline unsigned blocks_for(int64_t n, int per) { return static_cast<unsigned>((n
 // The sweep's work-group map.  Measured on the 384^3 box the dispatcher's
 // own order is ~4 % faster than the per-XCD contiguous runs, so the remap is
 // off unless XFD_XCD_REMAP=1.
-inline SweepMap sweepMap(const Grid& g, int ichunk) {
+inline SweepMap sweepMap(const Grid& g, int ichunk, int tk = kLanesK, int tj = kRowsJ) {
   static const int xcd = [] {
     const char* e = std::getenv("XFD_XCD_REMAP");
     return (e && e[0] == '1') ? 8 : 0;
   }();
-  return SweepMap{static_cast<int>(blocks_for(g.nz, kLanesK)), static_cast<int>(blocks_for(g.ny, kRowsJ)),
+  return SweepMap{static_cast<int>(blocks_for(g.nz, tk)), static_cast<int>(blocks_for(g.ny, tj)),
                   static_cast<int>(blocks_for(g.nx, ichunk)), xcd};
+}
+
+// The plain and z-in-sweep Yee sweeps take 128 k-lanes x 2 j-rows where nz
+// is a multiple of 128 (no idle lanes; half the k-halo loads per point of a
+// 64-lane row): measured on the 384^3 box, ms per step FP32 / FP64, 64 x 4
+// 1.115 / 2.216, 128 x 2 1.088 / 2.183.  Elsewhere 64 x 4, whose rows waste
+// fewer lanes on ragged nz.  XFD_SWEEP_TILE=64 / 128 forces one of them.
+inline bool sweepTile128(const Grid& g) {
+  const char* e = std::getenv("XFD_SWEEP_TILE");
+  if (e && e[0]) return std::atoi(e) == 128;
+  return g.nz % 128 == 0;
+}
+
+template <class T, class Coef, bool Fused, bool ZF, bool All, int TK, int TJ>
+void launchSweep(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk,
+                 const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S) {
+  const SweepMap m = sweepMap(g, ichunk, TK, TJ);
+  const dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz)), block(TK, TJ);
+  if (is_h)
+    hipLaunchKernelGGL((k_update_h<T, Coef, Fused, ZF, All, TK, TJ>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
+  else
+    hipLaunchKernelGGL((k_update_e<T, Coef, Fused, ZF, All, TK, TJ>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
 }
 
 template <class T, class Coef>
 void launchUpdate(bool is_h, hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, int ichunk,
                   const CpmlList<T>& P, const ZCpml<T>& Z, const StepSlabs<T>& S = StepSlabs<T>{}) {
-  dim3 block(kLanesK, kRowsJ);
-  const SweepMap m = sweepMap(g, ichunk);
-  dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz));
   const bool all = S.n > 0, fused = !all && P.n > 0, zf = !all && !fused && Z.n > 0;
-  if (is_h) {
-    if (all)
-      hipLaunchKernelGGL((k_update_h<T, Coef, false, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-    else if (fused)
-      hipLaunchKernelGGL((k_update_h<T, Coef, true, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-    else if (zf)
-      hipLaunchKernelGGL((k_update_h<T, Coef, false, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-    else
-      hipLaunchKernelGGL((k_update_h<T, Coef, false, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-  } else {
-    if (all)
-      hipLaunchKernelGGL((k_update_e<T, Coef, false, false, true>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-    else if (fused)
-      hipLaunchKernelGGL((k_update_e<T, Coef, true, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-    else if (zf)
-      hipLaunchKernelGGL((k_update_e<T, Coef, false, true, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-    else
-      hipLaunchKernelGGL((k_update_e<T, Coef, false, false, false>), grid, block, 0, s, g, F, C, m, ichunk, P, Z, S);
-  }
+  const bool t128 = !all && !fused && sweepTile128(g);
+  if (all)
+    launchSweep<T, Coef, false, false, true, kLanesK, kRowsJ>(is_h, s, g, F, C, ichunk, P, Z, S);
+  else if (fused)
+    launchSweep<T, Coef, true, false, false, kLanesK, kRowsJ>(is_h, s, g, F, C, ichunk, P, Z, S);
+  else if (zf && t128)
+    launchSweep<T, Coef, false, true, false, 128, 2>(is_h, s, g, F, C, ichunk, P, Z, S);
+  else if (zf)
+    launchSweep<T, Coef, false, true, false, kLanesK, kRowsJ>(is_h, s, g, F, C, ichunk, P, Z, S);
+  else if (t128)
+    launchSweep<T, Coef, false, false, false, 128, 2>(is_h, s, g, F, C, ichunk, P, Z, S);
+  else
+    launchSweep<T, Coef, false, false, false, kLanesK, kRowsJ>(is_h, s, g, F, C, ichunk, P, Z, S);
 }
 
 template <class T>
