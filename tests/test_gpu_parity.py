@@ -454,6 +454,31 @@ def test_sweep_tile_128_by_2(monkeypatch, precision):
             b.close()
 
 
+@pytest.mark.parametrize("kinds", [("drude", "debye", "lorentz", "mlorentz"), ("drude",), ("debye",)],
+                         ids=["mlorentz", "drude", "debye"])
+def test_dispersive_sweep_tile_128_by_2(monkeypatch, kinds):
+    """The dispersive E sweep (k_update_e_ade, each of its three schemes)
+    takes the plain sweeps' tile: 128 x 2 on a 128^3 box by default, the same
+    bits as 64 x 4 in FP64 and FP32."""
+    steps = 60
+    monkeypatch.delenv("XFD_CPML_ZFUSED", raising=False)
+    sp = scenarios.dispersive_slabs(n_total=128, pml=8, slab=16, gap=4, tfsf_offset=12, kinds=kinds,
+                                    lumped=False).spec()
+    for precision in ("f64", "f32"):
+        say(f"[gpu] dispersive sweep tile 128 x 2 vs 64 x 4, {'+'.join(kinds)} {precision}")
+        monkeypatch.delenv("XFD_SWEEP_TILE", raising=False)
+        a = native.Simulation(sp, precision)
+        a.run(steps)
+        monkeypatch.setenv("XFD_SWEEP_TILE", "64")
+        b = native.Simulation(sp, precision)
+        b.run(steps)
+        for f in FIELDS:
+            np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f"{precision} {f}")
+        assert np.abs(a.field("EZ")).max() > 0 or np.abs(a.field("EX")).max() > 0
+        a.close()
+        b.close()
+
+
 def test_far_field_on_device_matches_host(monkeypatch):
     """The far-field potentials summed on the GPU (xfd_nf2ff_far_field) equal
     the host sums over the same currents, for a theta cut and a phi cut

@@ -781,12 +781,12 @@ __device__ __forceinline__ void sweepEAde(const Grid& g, const FieldPtrs<T>& F, 
   }
 }
 
-template <class T, class Coef, int S>
-__global__ __launch_bounds__(kTileThreads) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
-                                                      int ichunk) {
+template <class T, class Coef, int S, int TK = kLanesK, int TJ = kRowsJ>
+__global__ __launch_bounds__(TK * TJ) void k_update_e_ade(Grid g, FieldPtrs<T> F, Coef C, AdeArgs<T> A, SweepMap M,
+                                                         int ichunk) {
   int bx, by, bz;
   M.coords(bx, by, bz);
-  sweepEAde<T, Coef, S>(g, F, C, A, bx, by, bz, ichunk);
+  sweepEAde<T, Coef, S, TK, TJ>(g, F, C, A, bx, by, bz, ichunk);
 }
 
 // interpolate::interpolateSurfaceCenter (interpolate_scheme.h:18-138): field

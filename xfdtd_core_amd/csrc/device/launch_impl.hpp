@@ -147,23 +147,31 @@ void Launch<T>::cpml_zrest(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, 
     hipLaunchKernelGGL((k_cpml_zrest<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, g, F, a, fr);
 }
 
+template <class T, class Coef, int TK, int TJ>
+void launchUpdateAdeTile(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
+                         int scheme, int ichunk) {
+  const SweepMap m = sweepMap(g, ichunk, TK, TJ);
+  const dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz)), block(TK, TJ);
+  switch (scheme) {
+    case kAdeDrude:
+      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeDrude, TK, TJ>), grid, block, 0, s, g, F, C, A, m, ichunk);
+      break;
+    case kAdeDebye:
+      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeDebye, TK, TJ>), grid, block, 0, s, g, F, C, A, m, ichunk);
+      break;
+    default:
+      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeMLorentz, TK, TJ>), grid, block, 0, s, g, F, C, A, m, ichunk);
+      break;
+  }
+}
+// the dispersive E sweep takes the plain sweeps' tile (sweepTile128)
 template <class T, class Coef>
 void launchUpdateAde(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const Coef& C, const AdeArgs<T>& A,
                      int scheme, int ichunk) {
-  dim3 block(kLanesK, kRowsJ);
-  const SweepMap m = sweepMap(g, ichunk);
-  dim3 grid(static_cast<unsigned>(m.nbx * m.nby * m.nbz));
-  switch (scheme) {
-    case kAdeDrude:
-      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeDrude>), grid, block, 0, s, g, F, C, A, m, ichunk);
-      break;
-    case kAdeDebye:
-      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeDebye>), grid, block, 0, s, g, F, C, A, m, ichunk);
-      break;
-    default:
-      hipLaunchKernelGGL((k_update_e_ade<T, Coef, kAdeMLorentz>), grid, block, 0, s, g, F, C, A, m, ichunk);
-      break;
-  }
+  if (sweepTile128(g))
+    launchUpdateAdeTile<T, Coef, 128, 2>(s, g, F, C, A, scheme, ichunk);
+  else
+    launchUpdateAdeTile<T, Coef, kLanesK, kRowsJ>(s, g, F, C, A, scheme, ichunk);
 }
 template <class T>
 void Launch<T>::update_e_ade(hipStream_t s, const Grid& g, const FieldPtrs<T>& F, const CoefPtrs<T>& C,
